@@ -1,0 +1,207 @@
+"""ctypes wrapper around the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg use
+this module, as the checker.  It mirrors the reference ``surface`` calls:
+``loadLattice`` -> ``generatePlan`` -> ``setg0`` -> ``energy`` / ``grad``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+MODE_LIPID = 0   # FIXED_A + LOCAL_LIPID_ENERGY (reference default)
+MODE_HELFRICH = 1
+
+
+class OParams(C.Structure):
+    _fields_ = [("kc", C.c_double), ("kg", C.c_double), ("KA", C.c_double),
+                ("k_reg", C.c_double), ("mode", C.c_int), ("lipid_c0", C.c_double),
+                ("lipid_apl", C.c_double)]
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return os.path.join(_HERE, "liboracle.so")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        vp = C.c_void_p
+        L.orc_load.argtypes = [C.c_char_p, C.POINTER(vp)]
+        L.orc_free.argtypes = [vp]
+        L.orc_nv.argtypes = [vp]
+        L.orc_nfaces.argtypes = [vp, ip, ip]
+        L.orc_get_positions.argtypes = [vp, dp]
+        L.orc_plan.argtypes = [vp]
+        L.orc_set_params.argtypes = [vp, C.POINTER(OParams)]
+        L.orc_set_face_c0.argtypes = [vp, dp]
+        L.orc_setg0.argtypes = [vp, dp]
+        L.orc_energy.argtypes = [vp, dp]
+        L.orc_energy.restype = C.c_double
+        L.orc_face_energy.argtypes = [vp, C.c_int, dp]
+        L.orc_face_energy.restype = C.c_double
+        L.orc_grad.argtypes = [vp, dp, dp]
+        L.orc_area.argtypes = [vp, dp, dp, dp]
+        L.orc_bd_steps.argtypes = [vp, dp, dp, C.c_int, C.c_double, C.c_double, dp]
+        L.orc_face_info.argtypes = [vp, C.c_int, ip, ip, ip, ip]
+        L.orc_face_table.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp, dp]
+        L.orc_tri_lipids.argtypes = [vp, C.c_int, dp, dp]
+        L.orc_ntri.argtypes = [vp]
+        L.orc_tri_order_face.argtypes = [vp, C.c_int]
+        L.orc_neighbors.argtypes = [vp, C.c_int, ip, ip]
+        L.orc_get_v_pts.argtypes = [dp, dp, C.c_int, C.c_double]
+        L.orc_get_u_pts.argtypes = [dp, dp, C.c_int, C.c_double, C.c_double]
+        L.orc_basis.argtypes = [C.c_double, C.c_double, dp]
+        L.orc_irregular_table.argtypes = [C.c_int, C.c_double, C.c_double, dp]
+        _LIB = L
+    return _LIB
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class OracleSurface:
+    """The reference surface on the CPU: load, plan, setg0, energy, grad."""
+
+    def __init__(self, mesh_path: str):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.orc_load(mesh_path.encode(), C.byref(h))
+        if rc != 0:
+            raise ValueError("oracle: cannot load %s (code %d)" % (mesh_path, rc))
+        self._h = h
+        self.nv = L.orc_nv(h)
+        rc = L.orc_plan(h)
+        if rc != 0:
+            raise ValueError("oracle: illegal mesh for plan (code %d)" % rc)
+        nr, ni = C.c_int(), C.c_int()
+        L.orc_nfaces(h, C.byref(nr), C.byref(ni))
+        self.nreg, self.nirr = nr.value, ni.value
+        self.nfaces = self.nreg + self.nirr
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _LIB is not None:
+            _LIB.orc_free(h)
+            self._h = None
+
+    def positions(self) -> np.ndarray:
+        r = np.zeros(3 * self.nv + 3)
+        lib().orc_get_positions(self._h, _p(r))
+        return r
+
+    def set_params(self, kc=14.0, kg=0.0, KA=0.0, k_reg=0.0, mode=MODE_LIPID,
+                   lipid_c0=0.0, lipid_apl=68.3):
+        p = OParams(kc, kg, KA, k_reg, mode, lipid_c0, lipid_apl)
+        lib().orc_set_params(self._h, C.byref(p))
+
+    def set_face_c0(self, c0: np.ndarray):
+        c0 = np.ascontiguousarray(c0, dtype=np.float64)
+        assert c0.size == self.nfaces
+        lib().orc_set_face_c0(self._h, _p(c0))
+
+    def setg0(self, r: np.ndarray):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        lib().orc_setg0(self._h, _p(r))
+
+    def energy(self, r: np.ndarray) -> float:
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        return lib().orc_energy(self._h, _p(r))
+
+    def face_energy(self, f: int, r: np.ndarray) -> float:
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        return lib().orc_face_energy(self._h, f, _p(r))
+
+    def grad(self, r: np.ndarray, g: np.ndarray | None = None) -> np.ndarray:
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        if g is None:
+            g = np.zeros_like(r)
+        lib().orc_grad(self._h, _p(r), _p(g))
+        return g
+
+    def area(self, r: np.ndarray):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        a, a0 = C.c_double(), C.c_double()
+        lib().orc_area(self._h, _p(r), C.byref(a), C.byref(a0))
+        return a.value, a0.value
+
+    def bd_steps(self, r: np.ndarray, xi: np.ndarray, mobility_dt: float,
+                 noise_amp: float, want_energies: bool = False):
+        r = np.ascontiguousarray(r, dtype=np.float64).copy()
+        xi = np.ascontiguousarray(xi, dtype=np.float64)
+        n = xi.size // (3 * self.nv)
+        en = np.zeros(n) if want_energies else None
+        lib().orc_bd_steps(self._h, _p(r), _p(xi), n, mobility_dt, noise_amp,
+                           _p(en) if en is not None else None)
+        return (r, en) if want_energies else r
+
+    def face_info(self, f: int):
+        owner, ncp, tri = C.c_int(), C.c_int(), C.c_int()
+        cp = (C.c_int * 16)()
+        npt = lib().orc_face_info(self._h, f, C.byref(owner), C.byref(ncp), cp, C.byref(tri))
+        return dict(owner=owner.value, ncp=ncp.value, cp=list(cp)[:ncp.value], tri=tri.value,
+                    npt=npt)
+
+    def face_table(self, f: int, p: int):
+        w6 = np.zeros(6 * 16)
+        pbc = np.zeros(48)
+        wt, g0 = C.c_double(), C.c_double()
+        ncp = lib().orc_face_table(self._h, f, p, _p(w6), C.byref(wt), _p(pbc), C.byref(g0))
+        return (w6[:6 * ncp].reshape(6, ncp), wt.value, pbc[:3 * ncp].reshape(ncp, 3), g0.value)
+
+    def tri_lipids(self, t: int):
+        lo, li = C.c_double(), C.c_double()
+        lib().orc_tri_lipids(self._h, t, C.byref(lo), C.byref(li))
+        return lo.value, li.value
+
+    def ntri(self) -> int:
+        return lib().orc_ntri(self._h)
+
+    def tri_face(self, t: int) -> int:
+        return lib().orc_tri_order_face(self._h, t)
+
+    def neighbors(self, v: int):
+        val = C.c_int()
+        nb = (C.c_int * 16)()
+        lib().orc_neighbors(self._h, v, C.byref(val), nb)
+        return list(nb)[:val.value]
+
+
+def v_points(k: float, n: int = 4):
+    x, w = np.zeros(n), np.zeros(n)
+    lib().orc_get_v_pts(_p(x), _p(w), n, k)
+    return x, w
+
+
+def u_points(k: float, v: float, n: int = 4):
+    x, w = np.zeros(n), np.zeros(n)
+    lib().orc_get_u_pts(_p(x), _p(w), n, k, v)
+    return x, w
+
+
+def basis(u: float, v: float) -> np.ndarray:
+    out = np.zeros(72)
+    lib().orc_basis(u, v, _p(out))
+    return out.reshape(6, 12)
+
+
+def irregular_table(val: int, u: float, v: float) -> np.ndarray:
+    out = np.zeros(6 * (val + 6))
+    rc = lib().orc_irregular_table(val, u, v, _p(out))
+    if rc:
+        raise ValueError("bad valence")
+    return out.reshape(6, val + 6)

@@ -1,0 +1,72 @@
+/*
+ * umcp_oracle.h -- CPU oracle for the membrane-surface hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C, FP64, serial restatement of
+ * the reference (alexsodt/UMCP) algorithm: mesh load + orientation
+ * (src/interpLib.C:1144-2606), plan / formula tables (src/interpLib.C:
+ * 4998-6670, src/gauss.C:27-299), reference state (src/interpLib.C:2969-3388),
+ * energy (src/interpLib.C:3405-3881, 4180-4352), gradient (src/grad.C:65-942,
+ * 6445-7457) and the per-vertex Brownian step (docs/timescales.tex:59-67,
+ * src/bd.C:1035-1036).  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it, and only as the checker.  The product library
+ * (umcp_amd/libumcp_core.so) never links or calls it.
+ */
+#ifndef UMCP_ORACLE_H
+#define UMCP_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct omesh omesh;
+
+/* energy modes: 0 = reference default (FIXED_A + LOCAL_LIPID_ENERGY,
+ * src/local_config.h:21-22); 1 = per-face c0 Helfrich with g0 area element
+ * (the #else branches of src/interpLib.C:3766-3844). */
+typedef struct {
+    double kc, kg, KA, k_reg;
+    int mode;
+    double lipid_c0, lipid_apl;   /* single lipid type, default "ZERO": 0, 68.3 */
+} oparams;
+
+int    orc_load(const char *path, omesh **out);        /* loadLattice + orientSurface */
+void   orc_free(omesh *m);
+int    orc_nv(const omesh *m);
+int    orc_nfaces(const omesh *m, int *nreg, int *nirr);
+void   orc_get_positions(const omesh *m, double *r);   /* r[3nv+3], alpha = 1 */
+int    orc_plan(omesh *m);                              /* generatePlan */
+void   orc_set_params(omesh *m, const oparams *p);
+void   orc_set_face_c0(omesh *m, const double *c0);    /* per face, face order */
+void   orc_setg0(omesh *m, const double *r);            /* setg0 */
+double orc_energy(omesh *m, const double *r);           /* surface::energy */
+double orc_face_energy(omesh *m, int f, const double *r);
+void   orc_grad(omesh *m, const double *r, double *g);  /* grad + igrad, accumulates */
+void   orc_area(omesh *m, const double *r, double *A, double *A0);
+void   orc_bd_steps(omesh *m, double *r, const double *xi, int nsteps,
+                    double mobility_dt, double noise_amp, double *energies);
+
+/* introspection for tests */
+int    orc_face_info(const omesh *m, int f, int *owner, int *ncp, int *cp, int *tri);
+int    orc_face_table(const omesh *m, int f, int p, double *w6, double *weight,
+                      double *pbc, double *g0);
+void   orc_tri_lipids(const omesh *m, int t, double *Lo, double *Li);
+int    orc_ntri(const omesh *m);
+int    orc_tri_order_face(const omesh *m, int t);
+void   orc_neighbors(const omesh *m, int v, int *val, int *nb);
+
+/* quadrature restatement (src/gauss.C:96-299 over QUADPACK QAGS) */
+void   orc_get_v_pts(double *v_pts, double *w_pts, int npts, double k);
+void   orc_get_u_pts(double *u_pts, double *w_pts, int npts, double k, double v);
+double orc_qags(double (*f)(double, void *), void *ctx, double a, double b,
+                double epsabs, double epsrel, int limit, double *abserr, int *ier, int *last);
+
+/* basis restatement: the 12 quartic box splines and derivatives at (u,v),
+ * in point_list order (src/interpLib.C:5836-5918, 6183-6189) */
+void   orc_basis(double u, double v, double out[6][12]);
+/* irregular (Stam) table for valence val at parameter (u,v): out[6][val+6] */
+int    orc_irregular_table(int val, double u, double v, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
