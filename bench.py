@@ -1,0 +1,171 @@
+"""Benchmark: Brownian dynamics of the 40962-vertex vesicle on one MI355X.
+
+One step = one pass of the hot path over the resident mesh: the fused
+limit-surface energy + gradient of all 81920 faces (81860 regular, 60
+irregular) and the per-vertex Brownian update, driven by a device-resident
+standard-normal array (seeded).  Timed with HIP events on the stream the work
+is launched on; K steps between two barriers + device synchronisations.
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line (metric, value = face-evals/s, roofline of the dominant
+kernel, the CPU oracle timed on a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "energy+grad face-evals/s & BD steps/s, FP64 vs FP32-mixed; p99 request latency"
+
+# Algorithmic work per face-eval (energy + gradient of one face over all its
+# Gauss points), in the reference algorithm's terms:
+#   F = P * (2*6*ncp*3 forward + 2*5*ncp*3 scatter + 2*5*ncp*3 alpha + C_pt)
+# with C_pt = 700 flop, the static operation count of the pointwise curvature
+# and chain rule of src/grad.C:158-466, 470-850 as restated in the oracle.
+C_PT = 700
+
+
+def face_flops(ncp: int, npt: int) -> int:
+    return npt * (2 * 6 * ncp * 3 + 2 * 5 * ncp * 3 + 2 * 5 * ncp * 3 + C_PT)
+
+
+def face_bytes(ncp: int, npt: int, nv: int, nf: int) -> float:
+    """Minimal unique FP64 traffic of one face-eval: ids, g0 per point, the
+    two lipid areas, and its share of reading r and writing g once."""
+    return 4 * ncp + 8 * npt + 16 + 48.0 * nv / nf
+
+
+PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
+PEAK_HBM_GBS = 8000.0
+
+
+def cpu_baseline(w, c0, budget_s: float = 12.0):
+    """The oracle (serial C, one core) on energy+gradient evaluations of the
+    same vesicle until ~budget_s of CPU work."""
+    import numpy as np
+    from oracle import oracle as O
+    o = O.OracleSurface(w.mesh_path)
+    o.set_params(**w.params)
+    o.set_face_c0(c0)
+    o.setg0(w.r0)
+    t0 = time.perf_counter()
+    n = 0
+    g = np.zeros_like(w.r)
+    while True:
+        o.energy(w.r)
+        g[:] = 0.0
+        o.grad(w.r, g)
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": n * o.nfaces / dt, "unit": "face-evals/s", "cores": 1, "kind": "port",
+            "sample": "%d energy+gradient evaluations of the 40962-vertex vesicle (%d faces), "
+                      "serial oracle, %.1f s" % (n, o.nfaces, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build runs one mesh per GPU and has no "
+              "multi-GPU path (--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch  # device memory and streams; its HIP runtime is shared with the library
+
+    from umcp_amd import surface as S
+    from umcp_amd import workloads
+
+    dev = torch.device("cuda:0")
+    w = workloads.vesicle_40962()
+    c0 = workloads.face_c0_uniform(81920, w.extra["c0_seed"])
+    s = S.Surface(w.mesh_path)
+    s.set_params(**w.params)
+    s.generatePlan()
+    s.set_face_c0(c0)
+    s.setg0(w.r0)
+    classes = s.classes()
+    nv, nf = s.nv, s.nfaces
+    mob, amp = S.bd_coefficients(1e-12, 1e-9, w.T)
+
+    K, W = args.steps, args.warmup
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(12345)
+    d_xi = torch.randn((W + K, 3 * nv), generator=gen, device=dev, dtype=torch.float64)
+    d_r = torch.tensor(w.r, device=dev, dtype=torch.float64)
+    d_en = torch.zeros(W + K, device=dev, dtype=torch.float64)
+    stream = torch.cuda.current_stream(dev)
+    sptr = stream.cuda_stream
+
+    s.bd_steps_dev(d_r, d_xi[:W], W, mob, amp, d_en[:W], stream=sptr)
+    torch.cuda.synchronize()
+
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    s.profile_begin()
+    start.record(stream)
+    s.bd_steps_dev(d_r, d_xi[W:], K, mob, amp, d_en[W:], stream=sptr)
+    end.record(stream)
+    torch.cuda.synchronize()
+    prof = s.profile_end()
+    ms = start.elapsed_time(end)
+    energies = d_en.cpu().numpy()
+    if not np.all(np.isfinite(energies)):
+        raise SystemExit("non-finite energy during the timed steps")
+
+    secs = ms / 1e3
+    value = nf * K / secs
+    # dominant kernel: the regular-face class
+    reg = classes[0]
+    reg_ms, reg_n = prof[0]
+    reg_avg_s = reg_ms / reg_n / 1e3
+    flops = reg["nf"] * face_flops(reg["ncp"], reg["npt"])
+    achieved = flops / reg_avg_s / 1e12
+    total_face_ms = sum(p[0] for p in prof) / K
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "face-evals/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": ms / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded icosphere, seeded device-resident N(0,1) noise)",
+        "config": {"workload": "vesicle_40962_bd", "nv": nv, "faces": nf,
+                   "faces_regular": reg["nf"], "faces_irregular": nf - reg["nf"],
+                   "energy_form": "helfrich per-face c0", "precision": "fp64",
+                   "D_m2_per_s": 1e-12, "dt_s": 1e-9, "lanes_per_face": s.get_launch(0)},
+        "bd_steps_per_s": K / secs,
+        "face_kernel_ms_per_step": total_face_ms,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "face_energy_grad_f64<12,15,%d>" % s.get_launch(0),
+                     "avg_launch_us": reg_avg_s * 1e6,
+                     "flop_per_launch": flops,
+                     "algorithmic_bytes_per_launch": reg["nf"] * face_bytes(12, 15, nv, nf)},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(w, c0, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,151 @@
+/*
+ * umcp.h -- C ABI of libumcp_core.so, the MI355X (gfx950) implementation of
+ * the UMCP membrane-surface hot path: the Loop-subdivision limit-surface
+ * Helfrich energy, its vertex gradient and the Brownian step that consumes it.
+ *
+ * Each entry point replaces one `surface` method or driver step of the
+ * reference (alexsodt/UMCP); the comment above each gives the interface it
+ * replaces.  Host arrays use the reference layout: r and g are double[3nv+3],
+ * x y z per vertex followed by the box-scale triple alpha (src/hd.C:232-247).
+ * Device state (mesh, tables, work buffers) lives in HBM behind the handle.
+ * All calls return UMCP_OK or a negative status; umcp_last_error() gives text.
+ * Calls on one handle are not thread safe; distinct handles are independent.
+ */
+#ifndef UMCP_H
+#define UMCP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UMCP_OK 0
+#define UMCP_ERR_BAD_MESH (-1)      /* src/interpLib.C:1287-1291, 5500-5509 exit(1) */
+#define UMCP_ERR_NAN_ENERGY (-2)    /* src/interpLib.C:3546-3550 "nan en." exit(1) */
+#define UMCP_ERR_OOM (-3)
+#define UMCP_ERR_DEVICE (-4)
+#define UMCP_ERR_ARG (-5)
+#define UMCP_ERR_STATE (-6)         /* e.g. grad before plan, src/grad.C:67-72 */
+
+/* energy forms */
+#define UMCP_MODE_LIPID 0     /* reference default: FIXED_A + LOCAL_LIPID_ENERGY (src/local_config.h:21-22) */
+#define UMCP_MODE_HELFRICH 1  /* per-face c0, g0 area element (#else branches of src/interpLib.C:3766-3844) */
+
+/* arithmetic of the per-face kernels */
+#define UMCP_PREC_FP64 0
+#define UMCP_PREC_MIXED 1     /* FP32 per-face arithmetic, FP64 reductions and positions */
+
+typedef struct umcp_surface umcp_surface;
+
+/* Physical constants of one surface; replaces the process globals kc, kg,
+ * KA, k_reg (src/interpLib.C:85-95, set by src/hd.C:125-128) and the lipid
+ * library entry "ZERO" (src/lipid_composition.C:18-34). Energies kcal/mol,
+ * lengths Angstrom. */
+typedef struct {
+    double kc;
+    double kg;
+    double KA;
+    double k_reg;
+    int mode;
+    int precision;
+    double lipid_c0;
+    double lipid_apl;
+} umcp_params;
+
+/* Per-vertex overdamped Brownian step, r <- r - mobility_dt * dE/dr + noise_amp * xi
+ * with mobility_dt = D dt / kT and noise_amp = sqrt(2 D dt) (docs/timescales.tex:59-67). */
+typedef struct {
+    double mobility_dt;
+    double noise_amp;
+} umcp_bd_params;
+
+/* link probe; replaces hd_umcp_link_check (src/interpLib.C:9577-9582) */
+int umcp_link_check(void);
+
+/* selects the HIP device for handles created afterwards on this thread */
+int umcp_set_device(int device);
+
+/* surface::loadLattice(fileName, noise=0) (src/interp.h:291): parse the .mesh
+ * file and orient it as the reference does */
+int umcp_load_mesh(const char *path, umcp_surface **out);
+void umcp_destroy(umcp_surface *s);
+const char *umcp_last_error(const umcp_surface *s);
+
+int umcp_nv(const umcp_surface *s);
+/* number of faces (regular + irregular); nreg, nirr may be NULL */
+int umcp_nfaces(const umcp_surface *s, int *nreg, int *nirr);
+
+/* surface::get (src/interp.h:427): positions of the loaded mesh, alpha = 1 */
+int umcp_get(const umcp_surface *s, double *r);
+
+/* parameters; see umcp_params */
+int umcp_set_params(umcp_surface *s, const umcp_params *p);
+
+/* host half of surface::generatePlan (src/interp.h:337): face ownership,
+ * control-point lists, periodic image codes and the shared Gauss-point
+ * tables; needs no GPU (introspection works after it) */
+int umcp_prepare(umcp_surface *s);
+
+/* surface::generatePlan (src/interp.h:337): umcp_prepare, then the plan,
+ * per-face state and work buffers resident on the device */
+int umcp_plan(umcp_surface *s);
+
+/* per-face spontaneous curvature (UMCP_MODE_HELFRICH), reference face order
+ * (regular faces, then irregular); replaces theVertices[i].c0 of the owner */
+int umcp_set_face_c0(umcp_surface *s, const double *c0);
+
+/* surface::setg0(r) (src/interp.h:293): reference metric, RuRv0 and the
+ * single-lipid packing per triangle */
+int umcp_setg0(umcp_surface *s, const double *r);
+
+/* surface::energy(r, NULL) (src/interp.h:294) */
+int umcp_energy(umcp_surface *s, const double *r, double *E);
+
+/* surface::grad(r, g) (src/interp.h:304): ADDS dE/dr (and dE/dalpha) into g */
+int umcp_grad(umcp_surface *s, const double *r, double *g);
+
+/* energy and gradient in one pass; g is overwritten (not accumulated) */
+int umcp_energy_grad(umcp_surface *s, const double *r, double *E, double *g);
+
+/* surface::area (src/interp.h:301): geometric area sum(dudv g w) and
+ * reference area sum(dudv g0 w) */
+int umcp_area(umcp_surface *s, const double *r, double *A, double *A0);
+
+/* nsteps Brownian steps from host arrays; xi is [nsteps][3nv] standard
+ * normals; r (3nv+3) is updated in place; energies (nsteps, may be NULL)
+ * receives the energy at the start of each step (src/hd.C:1128, 1543-1720) */
+int umcp_bd_steps(umcp_surface *s, double *r, const double *xi, int nsteps,
+                  const umcp_bd_params *bd, double *energies);
+
+/* Device-resident variants: all pointers are device pointers (HBM); stream
+ * is a hipStream_t (NULL = the surface's own stream).  They enqueue work and
+ * return without synchronising. */
+int umcp_energy_grad_dev(umcp_surface *s, const double *d_r, double *d_E, double *d_g,
+                         void *stream);
+int umcp_bd_steps_dev(umcp_surface *s, double *d_r, const double *d_xi, int nsteps,
+                      const umcp_bd_params *bd, double *d_energies, void *stream);
+
+/* launch shape of a face class: lanes_per_face in {1, 2, 4, 8} share the
+ * Gauss points of one face (no reference counterpart; used by the tuner) */
+int umcp_set_launch(umcp_surface *s, int cls, int lanes_per_face);
+int umcp_get_launch(const umcp_surface *s, int cls);
+
+/* per-launch timing of the face kernels with HIP events recorded on the
+ * stream each launch uses (no reference counterpart; benchmark support):
+ * begin, run work, then end returns per class the summed milliseconds and
+ * the number of launches */
+int umcp_profile_begin(umcp_surface *s);
+int umcp_profile_end(umcp_surface *s, int max_cls, double *ms, int *launches);
+
+/* introspection (tests): the shared table of a face class and its points */
+int umcp_class_count(const umcp_surface *s);
+int umcp_class_info(const umcp_surface *s, int cls, int *valence, int *ncp, int *npt, int *nf);
+int umcp_class_table(const umcp_surface *s, int cls, double *table, double *weights);
+int umcp_face_points(const umcp_surface *s, int face, int *ncp, int *cp);
+int umcp_face_lipids(const umcp_surface *s, int face, double *Lo, double *Li);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,70 @@
+// Host-side model of one membrane surface: mesh topology in the reference's
+// orientation, the face plan and the shared Gauss-point tables.  Everything
+// here runs once per mesh (setup); the per-step work is on the GPU.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace umcp {
+
+constexpr int kMaxValence = 12;
+constexpr int kMaxCp = kMaxValence + 6;
+constexpr int kRegPts = 15;  // regular-face Gauss points (src/interpLib.C:5336-5375)
+constexpr int kIrrPts = 16;  // 4 x 4 irregular-face points (src/interpLib.C:5608-5645)
+
+// Vertex ring structure as loadLattice leaves it (src/interpLib.C:1144-2606).
+struct Topology {
+    int nv = 0;
+    double pbc[3][3] = {{1e10, 0, 0}, {0, 1e10, 0}, {0, 0, 1e10}};
+    int fix_sense = 1;
+    std::vector<double> r;      // 3 nv, positions from the file
+    std::vector<int> valence;   // nv
+    std::vector<int> ring;      // nv * kMaxValence, oriented neighbour rings
+    std::vector<int> rev;       // position of i in the ring of ring[i][e]
+    int nb(int v, int e) const {
+        int n = valence[v];
+        return ring[v * kMaxValence + ((e % n) + n) % n];
+    }
+};
+
+// All faces with the same valence at their owning vertex share one table set.
+struct FaceClass {
+    int valence = 6;
+    int ncp = 12;
+    int npt = kRegPts;
+    double dudv = 0.5;                 // regular faces integrate over half the unit square
+    std::vector<int> face_id;          // global face index (reference face order)
+    std::vector<int> cp;               // [ncp][nf]   control-point vertex ids (SoA)
+    std::vector<uint8_t> img;          // [ncp][nf]   periodic image code, 13 = none
+    std::vector<uint8_t> has_img;      // [nf]        any image shift on this face
+    std::vector<double> table;         // [npt][6][ncp] r_w, r_u, r_v, r_uu, r_uv, r_vv
+    std::vector<double> weight;        // [npt]
+    int nf() const { return (int)face_id.size(); }
+};
+
+struct Plan {
+    int nfaces = 0;
+    int nreg = 0;
+    std::vector<FaceClass> classes;    // classes[0] is the regular class
+    // vertex -> gradient slot incidence, CSR (slot = class base + c * nf + f)
+    std::vector<int> vstart;           // nv + 1
+    std::vector<int> vslot;            // nnz
+    std::vector<int> class_slot_base;  // per class
+    int nslots = 0;
+};
+
+int parse_mesh(const char *path, Topology &T, std::string &err);
+int orient_mesh(Topology &T, std::string &err);
+int build_plan(const Topology &T, Plan &P, std::string &err);
+
+// 12 quartic box-spline basis functions and their first/second derivatives at
+// (u, v), in the control-point slot order of the face plan: out[d * 12 + y].
+void box_spline_basis(double u, double v, double *out);
+// Stam evaluation of an irregular patch of valence val at (u, v): out[d*(val+6)+x].
+int irregular_patch_table(int val, double u, double v, double *out);
+// The 16 (u, v, weight) points of an irregular face of valence val.
+int irregular_points(int val, double pts[kIrrPts][3]);
+
+}  // namespace umcp

@@ -1,0 +1,680 @@
+// The device-resident surface object and the extern "C" boundary (umcp.h).
+//
+// Lifecycle mirrors the reference driver (src/min.C:68-99, src/hd.C:162-273):
+// loadLattice -> readLipidComposition (default single "ZERO" lipid) ->
+// generatePlan -> setg0 -> energy / grad per step.  The mesh, the shared
+// Gauss-point tables, the per-face reference state and all work buffers stay
+// in HBM between calls; host arrays cross the boundary only in the host-array
+// entry points.
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/umcp.h"
+#include "surface_host.hpp"
+#include "umcp_kernels.hpp"
+
+using namespace umcp;
+
+namespace {
+
+thread_local int g_device = 0;
+
+struct ClassDev {
+    int *cp = nullptr;
+    uint8_t *img = nullptr, *has_img = nullptr;
+    double *g0 = nullptr, *rurv0 = nullptr, *Lo = nullptr, *Li = nullptr, *c0 = nullptr;
+    double *table = nullptr, *weight = nullptr;
+    int grp = 1;
+    int partial_base = 0;
+};
+
+}  // namespace
+
+struct umcp_surface {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    Topology T;
+    Plan P;
+    umcp_params prm{};
+    bool prepared = false, planned = false, g0_set = false;
+    std::vector<double> face_c0;   // global face order
+    std::vector<double> face_Lo, face_Li;
+    std::vector<ClassDev> cls;
+    std::vector<void *> allocations;
+    // work buffers
+    double *d_r = nullptr, *d_g = nullptr, *d_gslot = nullptr, *d_partial = nullptr;
+    double *d_scalar = nullptr;    // [8]: E, sums[4], ...
+    int *d_vstart = nullptr, *d_vslot = nullptr;
+    int npartial = 0;
+    double *d_xi = nullptr;        // host-array BD staging, chunk of steps
+    int xi_chunk = 0;
+    double *d_energies = nullptr;
+    int energies_cap = 0;
+    // optional per-launch timing of the face kernels (HIP events on the
+    // stream each kernel is launched on)
+    bool prof_on = false;
+    std::vector<hipEvent_t> prof_events;  // pairs: start, end
+    std::vector<int> prof_class;          // class of each pair
+    size_t prof_used = 0;
+};
+
+namespace {
+
+int fail(umcp_surface *s, int code, const std::string &msg)
+{
+    if (s) s->err = msg;
+    return code;
+}
+
+int hip_fail(umcp_surface *s, hipError_t e, const char *where)
+{
+    return fail(s, UMCP_ERR_DEVICE, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+int dev_alloc(umcp_surface *s, T **p, size_t n)
+{
+    if (n == 0) n = 1;
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, n * sizeof(T));
+    if (e != hipSuccess) return fail(s, UMCP_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    s->allocations.push_back(q);
+    *p = (T *)q;
+    return UMCP_OK;
+}
+
+template <typename T>
+int dev_upload(umcp_surface *s, T *dst, const T *src, size_t n)
+{
+    if (n == 0) return UMCP_OK;
+    hipError_t e = hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s->stream);
+    if (e != hipSuccess) return hip_fail(s, e, "upload");
+    return UMCP_OK;
+}
+
+int sync(umcp_surface *s)
+{
+    hipError_t e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return hip_fail(s, e, "hipStreamSynchronize");
+    return UMCP_OK;
+}
+
+PhysParams phys(const umcp_surface *s)
+{
+    PhysParams ph;
+    ph.kc = s->prm.kc;
+    ph.kg = s->prm.kg;
+    ph.KA = s->prm.KA;
+    ph.k_reg = s->prm.k_reg;
+    ph.lipid_c0 = s->prm.lipid_c0;
+    ph.mode = s->prm.mode;
+    return ph;
+}
+
+FaceArgs face_args(umcp_surface *s, size_t c, const double *d_r)
+{
+    const FaceClass &C = s->P.classes[c];
+    const ClassDev &D = s->cls[c];
+    FaceArgs a{};
+    a.r = d_r;
+    a.nv = s->T.nv;
+    a.nf = C.nf();
+    a.cp = D.cp;
+    a.img = D.img;
+    a.has_img = D.has_img;
+    a.g0 = D.g0;
+    a.rurv0 = D.rurv0;
+    a.Lo = D.Lo;
+    a.Li = D.Li;
+    a.c0 = D.c0;
+    a.table = D.table;
+    a.weight = D.weight;
+    a.dudv = C.dudv;
+    std::memcpy(a.pbc, s->T.pbc, sizeof a.pbc);
+    a.gslot = s->d_gslot;
+    a.slot_base = s->P.class_slot_base[c];
+    a.nslots = s->P.nslots;
+    a.partial = s->d_partial;
+    a.partial_base = D.partial_base;
+    a.ph = phys(s);
+    return a;
+}
+
+void layout_partials(umcp_surface *s)
+{
+    int base = 0;
+    for (size_t c = 0; c < s->cls.size(); c++) {
+        s->cls[c].partial_base = base;
+        base += face_blocks(s->P.classes[c].nf(), s->cls[c].grp);
+    }
+    s->npartial = base;
+}
+
+// enqueue one energy (+ gradient) evaluation on the surface stream
+int prof_pair(umcp_surface *s, int cls, hipEvent_t *ev)
+{
+    if (s->prof_used * 2 + 2 > s->prof_events.size()) {
+        for (int k = 0; k < 2; k++) {
+            hipEvent_t e;
+            hipError_t err = hipEventCreate(&e);
+            if (err != hipSuccess) return hip_fail(s, err, "hipEventCreate");
+            s->prof_events.push_back(e);
+        }
+        s->prof_class.push_back(cls);
+    }
+    s->prof_class[s->prof_used] = cls;
+    ev[0] = s->prof_events[2 * s->prof_used];
+    ev[1] = s->prof_events[2 * s->prof_used + 1];
+    s->prof_used++;
+    return UMCP_OK;
+}
+
+int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs ga)
+{
+    for (size_t c = 0; c < s->cls.size(); c++) {
+        const FaceClass &C = s->P.classes[c];
+        FaceArgs a = face_args(s, c, d_r);
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        if (s->prof_on) {
+            int rc = prof_pair(s, (int)c, ev);
+            if (rc) return rc;
+            (void)hipEventRecord(ev[0], st);
+        }
+        hipError_t e = launch_face_energy_grad(a, C.ncp, C.npt, s->cls[c].grp, st);
+        if (e != hipSuccess) return hip_fail(s, e, "face kernel launch");
+        if (s->prof_on) (void)hipEventRecord(ev[1], st);
+    }
+    ga.nv = s->T.nv;
+    ga.vstart = s->d_vstart;
+    ga.vslot = s->d_vslot;
+    ga.gslot = s->d_gslot;
+    ga.nslots = s->P.nslots;
+    ga.partial = s->d_partial;
+    ga.npartial = s->npartial;
+    hipError_t e = launch_vertex_gather(ga, st);
+    if (e != hipSuccess) return hip_fail(s, e, "gather kernel launch");
+    return UMCP_OK;
+}
+
+int check_ready(umcp_surface *s, bool need_g0 = true)
+{
+    if (!s) return UMCP_ERR_ARG;
+    if (!s->planned) return fail(s, UMCP_ERR_STATE, "plan has not been generated (call umcp_plan)");
+    if (need_g0 && !s->g0_set) return fail(s, UMCP_ERR_STATE, "reference state not set (call umcp_setg0)");
+    hipError_t e = hipSetDevice(s->device);
+    if (e != hipSuccess) return hip_fail(s, e, "hipSetDevice");
+    return UMCP_OK;
+}
+
+int upload_r(umcp_surface *s, const double *r)
+{
+    return dev_upload(s, s->d_r, r, 3 * (size_t)s->T.nv + 3);
+}
+
+}  // namespace
+
+extern "C" {
+
+int umcp_link_check(void) { return 1; }
+
+int umcp_set_device(int device)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return UMCP_ERR_DEVICE;
+    g_device = device;
+    return UMCP_OK;
+}
+
+int umcp_load_mesh(const char *path, umcp_surface **out)
+{
+    if (!path || !out) return UMCP_ERR_ARG;
+    *out = nullptr;
+    auto *s = new umcp_surface();
+    s->device = g_device;
+    std::string err;
+    int rc = parse_mesh(path, s->T, err);
+    if (rc == 0) rc = orient_mesh(s->T, err);
+    if (rc != 0) {
+        s->err = err;
+        *out = s;  // caller may read umcp_last_error, then destroy
+        return UMCP_ERR_BAD_MESH;
+    }
+    // default parameters: reference globals (src/interpLib.C:85-95) and the
+    // "ZERO" lipid (src/lipid_composition.C:18-22)
+    s->prm.kc = 14.0;
+    s->prm.kg = 0.0;
+    s->prm.KA = 0.0;
+    s->prm.k_reg = 0.0;
+    s->prm.mode = UMCP_MODE_LIPID;
+    s->prm.precision = UMCP_PREC_FP64;
+    s->prm.lipid_c0 = 0.0;
+    s->prm.lipid_apl = 68.3;
+    *out = s;
+    return UMCP_OK;
+}
+
+void umcp_destroy(umcp_surface *s)
+{
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (void *p : s->allocations) (void)hipFree(p);
+    for (hipEvent_t e : s->prof_events) (void)hipEventDestroy(e);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+const char *umcp_last_error(const umcp_surface *s) { return s ? s->err.c_str() : "null handle"; }
+
+int umcp_nv(const umcp_surface *s) { return s ? s->T.nv : UMCP_ERR_ARG; }
+
+int umcp_nfaces(const umcp_surface *s, int *nreg, int *nirr)
+{
+    if (!s) return UMCP_ERR_ARG;
+    if (!s->prepared) return 0;
+    if (nreg) *nreg = s->P.nreg;
+    if (nirr) *nirr = s->P.nfaces - s->P.nreg;
+    return s->P.nfaces;
+}
+
+int umcp_get(const umcp_surface *s, double *r)
+{
+    if (!s || !r) return UMCP_ERR_ARG;
+    std::memcpy(r, s->T.r.data(), sizeof(double) * 3 * (size_t)s->T.nv);
+    r[3 * s->T.nv] = r[3 * s->T.nv + 1] = r[3 * s->T.nv + 2] = 1.0;
+    return UMCP_OK;
+}
+
+int umcp_set_params(umcp_surface *s, const umcp_params *p)
+{
+    if (!s || !p) return UMCP_ERR_ARG;
+    if (p->mode != UMCP_MODE_LIPID && p->mode != UMCP_MODE_HELFRICH)
+        return fail(s, UMCP_ERR_ARG, "unknown energy mode");
+    if (p->precision != UMCP_PREC_FP64 && p->precision != UMCP_PREC_MIXED)
+        return fail(s, UMCP_ERR_ARG, "unknown precision");
+    if (p->precision == UMCP_PREC_MIXED)
+        return fail(s, UMCP_ERR_ARG, "mixed precision is not available in this build");
+    if (!(p->lipid_apl > 0)) return fail(s, UMCP_ERR_ARG, "lipid area must be positive");
+    bool repack = s->g0_set && p->lipid_apl != s->prm.lipid_apl;
+    s->prm = *p;
+    if (repack) s->g0_set = false;  // lipid packing depends on the lipid area
+    return UMCP_OK;
+}
+
+int umcp_prepare(umcp_surface *s)
+{
+    if (!s) return UMCP_ERR_ARG;
+    if (s->prepared) return UMCP_OK;
+    std::string err;
+    if (build_plan(s->T, s->P, err) != 0) return fail(s, UMCP_ERR_BAD_MESH, err);
+    s->prepared = true;
+    return UMCP_OK;
+}
+
+int umcp_plan(umcp_surface *s)
+{
+    if (!s) return UMCP_ERR_ARG;
+    if (s->planned) return UMCP_OK;
+    int rc0 = umcp_prepare(s);
+    if (rc0) return rc0;
+    hipError_t e = hipSetDevice(s->device);
+    if (e != hipSuccess) return hip_fail(s, e, "hipSetDevice");
+    if (!s->stream) {
+        e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) return hip_fail(s, e, "hipStreamCreate");
+    }
+    const int nv = s->T.nv;
+    int rc;
+    s->face_c0.assign(s->P.nfaces, 0.0);
+    s->face_Lo.assign(s->P.nfaces, 0.0);
+    s->face_Li.assign(s->P.nfaces, 0.0);
+    s->cls.assign(s->P.classes.size(), ClassDev());
+    for (size_t c = 0; c < s->P.classes.size(); c++) {
+        const FaceClass &C = s->P.classes[c];
+        ClassDev &D = s->cls[c];
+        const size_t nf = C.nf();
+        // four lanes per face spread the Gauss points of a face over a quad
+        D.grp = 4;
+        if ((rc = dev_alloc(s, &D.cp, C.cp.size())) || (rc = dev_alloc(s, &D.img, C.img.size())) ||
+            (rc = dev_alloc(s, &D.has_img, nf)) || (rc = dev_alloc(s, &D.g0, nf * C.npt)) ||
+            (rc = dev_alloc(s, &D.rurv0, nf * C.npt)) || (rc = dev_alloc(s, &D.Lo, nf)) ||
+            (rc = dev_alloc(s, &D.Li, nf)) || (rc = dev_alloc(s, &D.c0, nf)) ||
+            (rc = dev_alloc(s, &D.table, C.table.size())) || (rc = dev_alloc(s, &D.weight, C.weight.size())))
+            return rc;
+        if ((rc = dev_upload(s, D.cp, C.cp.data(), C.cp.size())) ||
+            (rc = dev_upload(s, D.img, C.img.data(), C.img.size())) ||
+            (rc = dev_upload(s, D.has_img, C.has_img.data(), nf)) ||
+            (rc = dev_upload(s, D.table, C.table.data(), C.table.size())) ||
+            (rc = dev_upload(s, D.weight, C.weight.data(), C.weight.size())))
+            return rc;
+        std::vector<double> zero(nf * C.npt, 0.0);
+        if ((rc = dev_upload(s, D.c0, zero.data(), nf)) || (rc = dev_upload(s, D.Lo, zero.data(), nf)) ||
+            (rc = dev_upload(s, D.Li, zero.data(), nf)) || (rc = dev_upload(s, D.g0, zero.data(), nf * C.npt)) ||
+            (rc = dev_upload(s, D.rurv0, zero.data(), nf * C.npt)))
+            return rc;
+    }
+    int max_partial = 0;
+    for (size_t c = 0; c < s->P.classes.size(); c++) max_partial += face_blocks(s->P.classes[c].nf(), 8);
+    if ((rc = dev_alloc(s, &s->d_r, 3 * (size_t)nv + 3)) || (rc = dev_alloc(s, &s->d_g, 3 * (size_t)nv + 3)) ||
+        (rc = dev_alloc(s, &s->d_gslot, 3 * (size_t)s->P.nslots)) ||
+        (rc = dev_alloc(s, &s->d_partial, 4 * (size_t)max_partial)) || (rc = dev_alloc(s, &s->d_scalar, 8)) ||
+        (rc = dev_alloc(s, &s->d_vstart, s->P.vstart.size())) || (rc = dev_alloc(s, &s->d_vslot, s->P.vslot.size())))
+        return rc;
+    if ((rc = dev_upload(s, s->d_vstart, s->P.vstart.data(), s->P.vstart.size())) ||
+        (rc = dev_upload(s, s->d_vslot, s->P.vslot.data(), s->P.vslot.size())))
+        return rc;
+    layout_partials(s);
+    if ((rc = sync(s))) return rc;
+    s->planned = true;
+    return UMCP_OK;
+}
+
+int umcp_set_face_c0(umcp_surface *s, const double *c0)
+{
+    int rc = check_ready(s, false);
+    if (rc) return rc;
+    if (!c0) return fail(s, UMCP_ERR_ARG, "null c0");
+    std::memcpy(s->face_c0.data(), c0, sizeof(double) * s->P.nfaces);
+    for (size_t c = 0; c < s->P.classes.size(); c++) {
+        const FaceClass &C = s->P.classes[c];
+        std::vector<double> v(C.nf());
+        for (int f = 0; f < C.nf(); f++) v[f] = c0[C.face_id[f]];
+        if ((rc = dev_upload(s, s->cls[c].c0, v.data(), v.size()))) return rc;
+    }
+    return sync(s);
+}
+
+int umcp_setg0(umcp_surface *s, const double *r)
+{
+    int rc = check_ready(s, false);
+    if (rc) return rc;
+    if (!r) return fail(s, UMCP_ERR_ARG, "null r");
+    if ((rc = upload_r(s, r))) return rc;
+    for (size_t c = 0; c < s->P.classes.size(); c++) {
+        const FaceClass &C = s->P.classes[c];
+        FaceArgs a = face_args(s, c, s->d_r);
+        hipError_t e = launch_face_metric(a, C.ncp, C.npt, 0, s->cls[c].g0, s->cls[c].rurv0, s->stream);
+        if (e != hipSuccess) return hip_fail(s, e, "metric kernel launch");
+    }
+    // single-lipid packing per triangle from the reference area of each face
+    // (src/interpLib.C:3113-3196): add APL per leaflet until it exceeds A - 65/2
+    const double apl = s->prm.lipid_apl;
+    for (size_t c = 0; c < s->P.classes.size(); c++) {
+        const FaceClass &C = s->P.classes[c];
+        const int nf = C.nf();
+        std::vector<double> g0((size_t)nf * C.npt), Lo(nf), Li(nf);
+        hipError_t e = hipMemcpyAsync(g0.data(), s->cls[c].g0, g0.size() * sizeof(double),
+                                      hipMemcpyDeviceToHost, s->stream);
+        if (e != hipSuccess) return hip_fail(s, e, "download g0");
+        if ((rc = sync(s))) return rc;
+        for (int f = 0; f < nf; f++) {
+            double A = 0.0;
+            for (int p = 0; p < C.npt; p++) A += g0[(size_t)p * nf + f] * C.dudv * C.weight[p];
+            double packed = 0.0, L = 0.0;
+            while (!(packed > 0 && packed > A - 0.5 * 65.0)) {
+                L += apl;
+                packed += apl;
+            }
+            Lo[f] = Li[f] = L;
+            s->face_Lo[C.face_id[f]] = L;
+            s->face_Li[C.face_id[f]] = L;
+        }
+        if ((rc = dev_upload(s, s->cls[c].Lo, Lo.data(), nf)) || (rc = dev_upload(s, s->cls[c].Li, Li.data(), nf)))
+            return rc;
+    }
+    if ((rc = sync(s))) return rc;
+    s->g0_set = true;
+    return UMCP_OK;
+}
+
+static int eval_host(umcp_surface *s, const double *r, double *E, double *g, bool accumulate)
+{
+    int rc = check_ready(s);
+    if (rc) return rc;
+    if (!r) return fail(s, UMCP_ERR_ARG, "null r");
+    if ((rc = upload_r(s, r))) return rc;
+    GatherArgs ga{};
+    ga.out_mode = g ? 0 : 2;
+    ga.g = s->d_g;
+    ga.E_out = s->d_scalar;
+    if ((rc = enqueue_eval(s, s->d_r, s->stream, ga))) return rc;
+    const size_t n = 3 * (size_t)s->T.nv + 3;
+    std::vector<double> gh;
+    double Eh = 0.0;
+    hipError_t e = hipMemcpyAsync(&Eh, s->d_scalar, sizeof(double), hipMemcpyDeviceToHost, s->stream);
+    if (e != hipSuccess) return hip_fail(s, e, "download E");
+    if (g) {
+        gh.resize(n);
+        e = hipMemcpyAsync(gh.data(), s->d_g, n * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+        if (e != hipSuccess) return hip_fail(s, e, "download g");
+    }
+    if ((rc = sync(s))) return rc;
+    if (std::isnan(Eh)) return fail(s, UMCP_ERR_NAN_ENERGY, "nan energy");
+    if (E) *E = Eh;
+    if (g) {
+        if (accumulate)
+            for (size_t i = 0; i < n; i++) g[i] += gh[i];
+        else
+            std::memcpy(g, gh.data(), n * sizeof(double));
+    }
+    return UMCP_OK;
+}
+
+int umcp_energy(umcp_surface *s, const double *r, double *E)
+{
+    if (!E) return fail(s, UMCP_ERR_ARG, "null E");
+    return eval_host(s, r, E, nullptr, false);
+}
+
+int umcp_grad(umcp_surface *s, const double *r, double *g)
+{
+    if (!g) return fail(s, UMCP_ERR_ARG, "null g");
+    return eval_host(s, r, nullptr, g, true);
+}
+
+int umcp_energy_grad(umcp_surface *s, const double *r, double *E, double *g)
+{
+    if (!g || !E) return fail(s, UMCP_ERR_ARG, "null output");
+    return eval_host(s, r, E, g, false);
+}
+
+int umcp_area(umcp_surface *s, const double *r, double *A, double *A0)
+{
+    int rc = check_ready(s);
+    if (rc) return rc;
+    if (!r || !A || !A0) return fail(s, UMCP_ERR_ARG, "null argument");
+    if ((rc = upload_r(s, r))) return rc;
+    int base = 0;
+    for (size_t c = 0; c < s->P.classes.size(); c++) {
+        const FaceClass &C = s->P.classes[c];
+        FaceArgs a = face_args(s, c, s->d_r);
+        a.partial_base = base;
+        base += face_blocks(C.nf(), 1);
+        hipError_t e = launch_face_metric(a, C.ncp, C.npt, 1, nullptr, nullptr, s->stream);
+        if (e != hipSuccess) return hip_fail(s, e, "metric kernel launch");
+    }
+    GatherArgs ga{};
+    ga.nv = s->T.nv;
+    ga.partial = s->d_partial;
+    ga.npartial = base;
+    ga.out_mode = 2;
+    ga.sums = s->d_scalar;
+    hipError_t e = launch_vertex_gather(ga, s->stream);
+    if (e != hipSuccess) return hip_fail(s, e, "gather kernel launch");
+    double h[4];
+    e = hipMemcpyAsync(h, s->d_scalar, 4 * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+    if (e != hipSuccess) return hip_fail(s, e, "download area");
+    if ((rc = sync(s))) return rc;
+    *A = h[0];
+    *A0 = h[1];
+    return UMCP_OK;
+}
+
+int umcp_energy_grad_dev(umcp_surface *s, const double *d_r, double *d_E, double *d_g, void *stream)
+{
+    int rc = check_ready(s);
+    if (rc) return rc;
+    GatherArgs ga{};
+    ga.out_mode = d_g ? 0 : 2;
+    ga.g = d_g;
+    ga.E_out = d_E;
+    return enqueue_eval(s, d_r, stream ? (hipStream_t)stream : s->stream, ga);
+}
+
+int umcp_bd_steps_dev(umcp_surface *s, double *d_r, const double *d_xi, int nsteps, const umcp_bd_params *bd,
+                      double *d_energies, void *stream)
+{
+    int rc = check_ready(s);
+    if (rc) return rc;
+    if (!bd || nsteps < 0) return fail(s, UMCP_ERR_ARG, "bad Brownian parameters");
+    hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+    const size_t n3 = 3 * (size_t)s->T.nv;
+    for (int k = 0; k < nsteps; k++) {
+        GatherArgs ga{};
+        ga.out_mode = 1;
+        ga.r = d_r;
+        ga.xi = d_xi + (size_t)k * n3;
+        ga.mobility_dt = bd->mobility_dt;
+        ga.noise_amp = bd->noise_amp;
+        ga.energies = d_energies;
+        ga.step = k;
+        if ((rc = enqueue_eval(s, d_r, st, ga))) return rc;
+    }
+    return UMCP_OK;
+}
+
+int umcp_bd_steps(umcp_surface *s, double *r, const double *xi, int nsteps, const umcp_bd_params *bd,
+                  double *energies)
+{
+    int rc = check_ready(s);
+    if (rc) return rc;
+    if (!r || !xi || !bd || nsteps < 0) return fail(s, UMCP_ERR_ARG, "null argument");
+    const size_t n3 = 3 * (size_t)s->T.nv;
+    const int chunk = nsteps < 64 ? (nsteps > 0 ? nsteps : 1) : 64;
+    if (s->xi_chunk < chunk) {
+        if ((rc = dev_alloc(s, &s->d_xi, n3 * chunk))) return rc;
+        s->xi_chunk = chunk;
+    }
+    if (s->energies_cap < chunk) {
+        if ((rc = dev_alloc(s, &s->d_energies, chunk))) return rc;
+        s->energies_cap = chunk;
+    }
+    if ((rc = upload_r(s, r))) return rc;
+    for (int k0 = 0; k0 < nsteps; k0 += chunk) {
+        const int nk = (nsteps - k0 < chunk) ? nsteps - k0 : chunk;
+        if ((rc = dev_upload(s, s->d_xi, xi + (size_t)k0 * n3, n3 * nk))) return rc;
+        if ((rc = umcp_bd_steps_dev(s, s->d_r, s->d_xi, nk, bd, s->d_energies, s->stream))) return rc;
+        if (energies) {
+            hipError_t e = hipMemcpyAsync(energies + k0, s->d_energies, nk * sizeof(double),
+                                          hipMemcpyDeviceToHost, s->stream);
+            if (e != hipSuccess) return hip_fail(s, e, "download energies");
+        }
+        if ((rc = sync(s))) return rc;  // the staging buffer is reused by the next chunk
+        if (energies)
+            for (int k = 0; k < nk; k++)
+                if (std::isnan(energies[k0 + k])) return fail(s, UMCP_ERR_NAN_ENERGY, "nan energy");
+    }
+    hipError_t e = hipMemcpyAsync(r, s->d_r, n3 * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+    if (e != hipSuccess) return hip_fail(s, e, "download r");
+    return sync(s);
+}
+
+int umcp_set_launch(umcp_surface *s, int cls, int lanes_per_face)
+{
+    if (!s || !s->planned) return UMCP_ERR_STATE;
+    if (cls < 0 || cls >= (int)s->cls.size()) return fail(s, UMCP_ERR_ARG, "bad face class");
+    if (lanes_per_face != 1 && lanes_per_face != 2 && lanes_per_face != 4 && lanes_per_face != 8)
+        return fail(s, UMCP_ERR_ARG, "lanes per face must be 1, 2, 4 or 8");
+    s->cls[cls].grp = lanes_per_face;
+    layout_partials(s);
+    return UMCP_OK;
+}
+
+int umcp_get_launch(const umcp_surface *s, int cls)
+{
+    if (!s || !s->planned || cls < 0 || cls >= (int)s->cls.size()) return UMCP_ERR_ARG;
+    return s->cls[cls].grp;
+}
+
+int umcp_profile_begin(umcp_surface *s)
+{
+    if (!s || !s->planned) return UMCP_ERR_STATE;
+    s->prof_on = true;
+    s->prof_used = 0;
+    return UMCP_OK;
+}
+
+int umcp_profile_end(umcp_surface *s, int max_cls, double *ms, int *launches)
+{
+    if (!s || !s->planned) return UMCP_ERR_STATE;
+    s->prof_on = false;
+    for (int c = 0; c < max_cls; c++) {
+        if (ms) ms[c] = 0.0;
+        if (launches) launches[c] = 0;
+    }
+    for (size_t i = 0; i < s->prof_used; i++) {
+        hipError_t e = hipEventSynchronize(s->prof_events[2 * i + 1]);
+        if (e != hipSuccess) return hip_fail(s, e, "hipEventSynchronize");
+        float t = 0.0f;
+        e = hipEventElapsedTime(&t, s->prof_events[2 * i], s->prof_events[2 * i + 1]);
+        if (e != hipSuccess) return hip_fail(s, e, "hipEventElapsedTime");
+        int c = s->prof_class[i];
+        if (c < max_cls) {
+            if (ms) ms[c] += t;
+            if (launches) launches[c] += 1;
+        }
+    }
+    s->prof_used = 0;
+    return UMCP_OK;
+}
+
+int umcp_class_count(const umcp_surface *s) { return (s && s->prepared) ? (int)s->P.classes.size() : 0; }
+
+int umcp_class_info(const umcp_surface *s, int cls, int *valence, int *ncp, int *npt, int *nf)
+{
+    if (!s || !s->prepared || cls < 0 || cls >= (int)s->P.classes.size()) return UMCP_ERR_ARG;
+    const FaceClass &C = s->P.classes[cls];
+    if (valence) *valence = C.valence;
+    if (ncp) *ncp = C.ncp;
+    if (npt) *npt = C.npt;
+    if (nf) *nf = C.nf();
+    return UMCP_OK;
+}
+
+int umcp_class_table(const umcp_surface *s, int cls, double *table, double *weights)
+{
+    if (!s || !s->prepared || cls < 0 || cls >= (int)s->P.classes.size()) return UMCP_ERR_ARG;
+    const FaceClass &C = s->P.classes[cls];
+    if (table) std::memcpy(table, C.table.data(), C.table.size() * sizeof(double));
+    if (weights) std::memcpy(weights, C.weight.data(), C.weight.size() * sizeof(double));
+    return UMCP_OK;
+}
+
+int umcp_face_points(const umcp_surface *s, int face, int *ncp, int *cp)
+{
+    if (!s || !s->prepared || face < 0 || face >= s->P.nfaces) return UMCP_ERR_ARG;
+    for (const FaceClass &C : s->P.classes)
+        for (int f = 0; f < C.nf(); f++)
+            if (C.face_id[f] == face) {
+                if (ncp) *ncp = C.ncp;
+                if (cp)
+                    for (int q = 0; q < C.ncp; q++) cp[q] = C.cp[(size_t)q * C.nf() + f];
+                return UMCP_OK;
+            }
+    return UMCP_ERR_ARG;
+}
+
+int umcp_face_lipids(const umcp_surface *s, int face, double *Lo, double *Li)
+{
+    if (!s || !s->g0_set || face < 0 || face >= s->P.nfaces) return UMCP_ERR_ARG;
+    if (Lo) *Lo = s->face_Lo[face];
+    if (Li) *Li = s->face_Li[face];
+    return UMCP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,414 @@
+// HIP kernels for gfx950 (MI355X): fused limit-surface energy + gradient per
+// face class, the deterministic vertex gather (which also finishes the global
+// reductions and can apply the Brownian update), and the metric kernel used
+// by setg0 and area.
+//
+// Reference computations, alexsodt/UMCP:
+//   per-point energy      src/interpLib.C:3641-3881 (fenergy), 3426-3639 (ifenergy)
+//   per-point gradient    src/grad.C:246-930 (grad), 6605-7450 (igrad)
+//   reference metric      src/interpLib.C:2969-3111 (setg0)
+//   area                  src/interpLib.C:4355-4611
+//   Brownian step         docs/timescales.tex:59-67, src/bd.C:1035-1036
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "umcp_kernels.hpp"
+
+namespace umcp {
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give every XCD a
+// contiguous run of logical blocks so neighbouring faces (which share
+// vertices) hit the same L2.
+__device__ __forceinline__ int xcd_logical_block(int bid, int nb)
+{
+    constexpr int kXcd = 8;
+    int q = nb / kXcd, rem = nb % kXcd;
+    int x = bid % kXcd, i = bid / kXcd;
+    int start = (x < rem) ? x * (q + 1) : rem * (q + 1) + (x - rem) * q;
+    return start + i;
+}
+
+// Deterministic reduction of four doubles over a block of kBlock threads;
+// thread 0 receives the sums.
+template <int kBlock>
+__device__ __forceinline__ void block_sum4(double v[4])
+{
+    __shared__ double red[kBlock / 64][4];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int i = 0; i < 4; i++) v[i] += __shfl_down(v[i], off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < 4; i++) red[wave][i] = v[i];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < kBlock / 64; w++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) v[i] += red[w][i];
+}
+
+__device__ __forceinline__ void image_offset(int code, const double pbc[3][3], double out[3])
+{
+    const int ix = code / 9 - 1, iy = (code / 3) % 3 - 1, iz = code % 3 - 1;
+#pragma unroll
+    for (int k = 0; k < 3; k++) out[k] = ix * pbc[0][k] + iy * pbc[1][k] + iz * pbc[2][k];
+}
+
+// One Gauss point: energy and dE/d(Ru, Rv, Ruu, Ruv, Rvv).
+//
+// The reference builds the principal curvatures c1,2 = -(a+d -/+ sqrt(SAFETY
+// + (a-d)^2 + 4bc))/2 of the shape operator S = I^-1 II and differentiates
+// through both; every energy term depends on c1 + c2 = -(a + d) = -tr S and
+// c1 c2 = det S - SAFETY/4 only, so this evaluates tr S and det S directly:
+//   tr S  = (L E22 - 2 M E12 + N E11) / det I,   det S = (L N - M^2) / det I
+// with E = first fundamental form, (L, M, N) = n . (Ruu, Ruv, Rvv) and
+// n = (Ru x Rv)/|Ru x Rv|; the gradient is the exact reverse of this chain.
+template <typename R>
+__device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R rurv0, R Lo, R Li,
+                                               R c0f, R dudv, const PhysParams &ph, R D[5][3])
+{
+    const R *Ru = X[0], *Rv = X[1], *Suu = X[2], *Suv = X[3], *Svv = X[4];
+    const R E11 = Ru[0] * Ru[0] + Ru[1] * Ru[1] + Ru[2] * Ru[2];
+    const R E12 = Ru[0] * Rv[0] + Ru[1] * Rv[1] + Ru[2] * Rv[2];
+    const R E22 = Rv[0] * Rv[0] + Rv[1] * Rv[1] + Rv[2] * Rv[2];
+    const R det = E11 * E22 - E12 * E12;
+    const R g = sqrt(det);
+    const R idet = R(1) / det;
+    R Nv[3];
+    Nv[0] = Ru[1] * Rv[2] - Ru[2] * Rv[1];
+    Nv[1] = Ru[2] * Rv[0] - Ru[0] * Rv[2];
+    Nv[2] = Ru[0] * Rv[1] - Ru[1] * Rv[0];
+    const R inn = R(1) / sqrt(Nv[0] * Nv[0] + Nv[1] * Nv[1] + Nv[2] * Nv[2]);
+    const R n[3] = {Nv[0] * inn, Nv[1] * inn, Nv[2] * inn};
+    const R L = n[0] * Suu[0] + n[1] * Suu[1] + n[2] * Suu[2];
+    const R M = n[0] * Suv[0] + n[1] * Suv[1] + n[2] * Suv[2];
+    const R N = n[0] * Svv[0] + n[1] * Svv[1] + n[2] * Svv[2];
+    const R T = L * E22 - R(2) * M * E12 + N * E11;
+    const R H = -T * idet;                       // c1 + c2
+    const R K = (L * N - M * M) * idet;          // c1 c2 + SAFETY/4
+
+    const R kc = (R)ph.kc, kg = (R)ph.kg, KA = (R)ph.KA, kreg = (R)ph.k_reg;
+    const R g0w = g0 * w;
+    R e, dH;
+    if (ph.mode == 0) {
+        // lipid-area weighted leaflets, no dudv (src/interpLib.C:3832-3834)
+        const R lc0 = (R)ph.lipid_c0;
+        const R ho = H - lc0, hi = H + lc0;
+        const R q = R(0.25) * kc * w;
+        e = q * (ho * ho * Lo + hi * hi * Li);
+        dH = R(2) * q * (ho * Lo + hi * Li);
+    } else {
+        const R hd = H - c0f;
+        e = R(0.5) * kc * dudv * g0w * hd * hd;
+        dH = kc * dudv * g0w * hd;
+    }
+    const R dK = kg * dudv * g0w;
+    e += dK * (K - R(0.25e-12));
+    // per-point area strain: A = dudv g w, A0 = dudv g0 w
+    const R A = dudv * g * w, A0 = dudv * g0w;
+    const R strain = (A - A0) / A0;
+    e += KA * A0 * strain * strain;
+    const R dg = R(2) * KA * strain * dudv * w;
+    // shear regulariser on cos(angle(Ru, Rv)) (src/interpLib.C:3797-3801)
+    const R s1122 = sqrt(E11 * E22);
+    const R val = E12 / s1122;
+    const R dv_ = val - rurv0;
+    e += kreg * dv_ * dv_;
+    const R dval = R(2) * kreg * dv_;
+
+    // reverse pass
+    const R dL = (-dH * E22 + dK * N) * idet;
+    const R dM = (R(2) * dH * E12 - R(2) * dK * M) * idet;
+    const R dN = (-dH * E11 + dK * L) * idet;
+    const R ddet = dH * T * idet * idet - dK * K * idet + dg * R(0.5) / g;
+    const R dE11 = -dH * N * idet + ddet * E22 - dval * val * R(0.5) / E11;
+    const R dE22 = -dH * L * idet + ddet * E11 - dval * val * R(0.5) / E22;
+    const R dE12 = R(2) * dH * M * idet - R(2) * ddet * E12 + dval / s1122;
+    R Q[3], dNv[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) Q[k] = dL * Suu[k] + dM * Suv[k] + dN * Svv[k];
+    const R qn = Q[0] * n[0] + Q[1] * n[1] + Q[2] * n[2];
+#pragma unroll
+    for (int k = 0; k < 3; k++) dNv[k] = (Q[k] - qn * n[k]) * inn;
+    // dE/dRu += Rv x dN ; dE/dRv += dN x Ru
+    D[0][0] = R(2) * dE11 * Ru[0] + dE12 * Rv[0] + (Rv[1] * dNv[2] - Rv[2] * dNv[1]);
+    D[0][1] = R(2) * dE11 * Ru[1] + dE12 * Rv[1] + (Rv[2] * dNv[0] - Rv[0] * dNv[2]);
+    D[0][2] = R(2) * dE11 * Ru[2] + dE12 * Rv[2] + (Rv[0] * dNv[1] - Rv[1] * dNv[0]);
+    D[1][0] = R(2) * dE22 * Rv[0] + dE12 * Ru[0] + (dNv[1] * Ru[2] - dNv[2] * Ru[1]);
+    D[1][1] = R(2) * dE22 * Rv[1] + dE12 * Ru[1] + (dNv[2] * Ru[0] - dNv[0] * Ru[2]);
+    D[1][2] = R(2) * dE22 * Rv[2] + dE12 * Ru[2] + (dNv[0] * Ru[1] - dNv[1] * Ru[0]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        D[2][k] = dL * n[k];
+        D[3][k] = dM * n[k];
+        D[4][k] = dN * n[k];
+    }
+    return e;
+}
+
+// Fused energy + gradient, FP64.  GRP lanes share one face; lane j takes the
+// Gauss points p = j, j + GRP, ...  Outputs: dE/dr per (face, control point)
+// into gslot (component-major SoA), block partials of E and dE/dalpha.
+template <int NCP, int NPT, int GRP>
+__global__ __launch_bounds__(kFaceBlock) void face_energy_grad_f64(FaceArgs a)
+{
+    const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+    const int t = lb * kFaceBlock + threadIdx.x;
+    const int f = t / GRP, j = t % GRP;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const bool live = f < a.nf;
+    const int fc = live ? f : a.nf - 1;  // idle lanes shadow the last face (shuffles stay uniform)
+    const int nf = a.nf;
+    const double al[3] = {a.r[3 * a.nv], a.r[3 * a.nv + 1], a.r[3 * a.nv + 2]};
+    double P[NCP][3];
+    const bool shifted = a.has_img[fc] != 0;
+#pragma unroll
+    for (int c = 0; c < NCP; c++) {
+        const int v = a.cp[c * nf + fc];
+        P[c][0] = a.r[3 * v];
+        P[c][1] = a.r[3 * v + 1];
+        P[c][2] = a.r[3 * v + 2];
+    }
+    if (shifted) {
+#pragma unroll
+        for (int c = 0; c < NCP; c++) {
+            double o[3];
+            image_offset(a.img[c * nf + fc], a.pbc, o);
+#pragma unroll
+            for (int k = 0; k < 3; k++) P[c][k] += o[k];
+        }
+    }
+    double G[NCP][3];
+#pragma unroll
+    for (int c = 0; c < NCP; c++) G[c][0] = G[c][1] = G[c][2] = 0.0;
+    const double Lo = a.Lo[fc], Li = a.Li[fc], c0f = a.c0[fc];
+    double e = 0.0;
+    for (int p = j; p < NPT; p += GRP) {
+        const double *T = a.table + p * 6 * NCP;  // wave-uniform only when GRP == 1
+        double X[5][3];
+#pragma unroll
+        for (int d = 0; d < 5; d++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                double s = 0.0;
+#pragma unroll
+                for (int c = 0; c < NCP; c++) s += T[(d + 1) * NCP + c] * P[c][k];
+                X[d][k] = s * al[k];
+            }
+        double D[5][3];
+        e += point_energy_grad<double>(X, a.weight[p], a.g0[p * nf + fc], a.rurv0[p * nf + fc],
+                                       Lo, Li, c0f, a.dudv, a.ph, D);
+#pragma unroll
+        for (int c = 0; c < NCP; c++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                double s = G[c][k];
+#pragma unroll
+                for (int d = 0; d < 5; d++) s += T[(d + 1) * NCP + c] * D[d][k];
+                G[c][k] = s;
+            }
+    }
+    if (GRP > 1) {
+#pragma unroll
+        for (int m = 1; m < GRP; m <<= 1)
+#pragma unroll
+            for (int c = 0; c < NCP; c++)
+#pragma unroll
+                for (int k = 0; k < 3; k++) G[c][k] += __shfl_xor(G[c][k], m, GRP);
+    }
+    if (live) {
+        acc[0] = e;
+        double *out = a.gslot + a.slot_base;
+#pragma unroll
+        for (int c = 0; c < NCP; c++) {
+            if (c % GRP != j) continue;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                out[(size_t)k * a.nslots + c * nf + f] = G[c][k] * al[k];
+                acc[1 + k] += G[c][k] * P[c][k];
+            }
+        }
+    }
+    block_sum4<kFaceBlock>(acc);
+    if (threadIdx.x == 0) {
+        double *dst = a.partial + 4 * (size_t)(a.partial_base + lb);
+#pragma unroll
+        for (int i = 0; i < 4; i++) dst[i] = acc[i];
+    }
+}
+
+// Per-point metric: mode 0 writes g0 and RuRv0 (no alpha, as setg0 does);
+// mode 1 accumulates sum(dudv g w) and sum(dudv g0 w) into the block partials.
+template <int NCP, int NPT>
+__global__ __launch_bounds__(kFaceBlock) void face_metric(FaceArgs a, int mode, double *g0_out,
+                                                          double *rurv0_out)
+{
+    const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+    const int f = lb * kFaceBlock + threadIdx.x;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (f < a.nf) {
+        const int nf = a.nf;
+        const double al[3] = {mode ? a.r[3 * a.nv] : 1.0, mode ? a.r[3 * a.nv + 1] : 1.0,
+                              mode ? a.r[3 * a.nv + 2] : 1.0};
+        double P[NCP][3];
+#pragma unroll
+        for (int c = 0; c < NCP; c++) {
+            const int v = a.cp[c * nf + f];
+            double o[3] = {0.0, 0.0, 0.0};
+            if (a.has_img[f]) image_offset(a.img[c * nf + f], a.pbc, o);
+#pragma unroll
+            for (int k = 0; k < 3; k++) P[c][k] = a.r[3 * v + k] + o[k];
+        }
+        for (int p = 0; p < NPT; p++) {
+            const double *T = a.table + p * 6 * NCP;
+            double Ru[3], Rv[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                double su = 0.0, sv = 0.0;
+#pragma unroll
+                for (int c = 0; c < NCP; c++) {
+                    su += T[1 * NCP + c] * P[c][k];
+                    sv += T[2 * NCP + c] * P[c][k];
+                }
+                Ru[k] = su * al[k];
+                Rv[k] = sv * al[k];
+            }
+            const double E11 = Ru[0] * Ru[0] + Ru[1] * Ru[1] + Ru[2] * Ru[2];
+            const double E12 = Ru[0] * Rv[0] + Ru[1] * Rv[1] + Ru[2] * Rv[2];
+            const double E22 = Rv[0] * Rv[0] + Rv[1] * Rv[1] + Rv[2] * Rv[2];
+            const double g = sqrt(E11 * E22 - E12 * E12);
+            if (mode == 0) {
+                g0_out[p * nf + f] = g;
+                rurv0_out[p * nf + f] = E12 / sqrt(E11 * E22);
+            } else {
+                acc[0] += a.dudv * g * a.weight[p];
+                acc[1] += a.dudv * a.g0[p * nf + f] * a.weight[p];
+            }
+        }
+    }
+    if (mode == 1) {
+        block_sum4<kFaceBlock>(acc);
+        if (threadIdx.x == 0) {
+            double *dst = a.partial + 4 * (size_t)(a.partial_base + lb);
+#pragma unroll
+            for (int i = 0; i < 4; i++) dst[i] = acc[i];
+        }
+    }
+}
+
+// Vertex-owned gather over the face->vertex CSR (no atomics, fixed order), and
+// in block 0 the fixed-order reduction of the face-kernel partials.
+//   out_mode 0: write g (3 nv) and dE/dalpha into g[3nv..3nv+2]
+//   out_mode 1: Brownian update of r, no g
+//   out_mode 2: reductions only
+__global__ __launch_bounds__(kGatherBlock) void vertex_gather(GatherArgs a)
+{
+    if (blockIdx.x == 0) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int i = threadIdx.x; i < a.npartial; i += kGatherBlock)
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc[q] += a.partial[4 * (size_t)i + q];
+        block_sum4<kGatherBlock>(acc);
+        if (threadIdx.x == 0) {
+            if (a.E_out) a.E_out[0] = acc[0];
+            if (a.energies) a.energies[a.step] = acc[0];
+            if (a.out_mode == 0 && a.g) {
+                a.g[3 * a.nv + 0] = acc[1];
+                a.g[3 * a.nv + 1] = acc[2];
+                a.g[3 * a.nv + 2] = acc[3];
+            }
+            if (a.sums) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) a.sums[q] = acc[q];
+            }
+        }
+    }
+    if (a.out_mode == 2) return;
+    const int v = blockIdx.x * kGatherBlock + threadIdx.x;
+    if (v >= a.nv) return;
+    const int s0 = a.vstart[v], s1 = a.vstart[v + 1];
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    for (int s = s0; s < s1; s++) {
+        const size_t slot = (size_t)a.vslot[s];
+        g0 += a.gslot[slot];
+        g1 += a.gslot[(size_t)a.nslots + slot];
+        g2 += a.gslot[2 * (size_t)a.nslots + slot];
+    }
+    if (a.out_mode == 0) {
+        a.g[3 * v] = g0;
+        a.g[3 * v + 1] = g1;
+        a.g[3 * v + 2] = g2;
+    } else {
+        const double *xi = a.xi + 3 * (size_t)v;
+        double *r = a.r + 3 * (size_t)v;
+        r[0] += -a.mobility_dt * g0 + a.noise_amp * xi[0];
+        r[1] += -a.mobility_dt * g1 + a.noise_amp * xi[1];
+        r[2] += -a.mobility_dt * g2 + a.noise_amp * xi[2];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+template <int NCP, int NPT, int GRP>
+static hipError_t launch_eg(const FaceArgs &a, int nblocks, hipStream_t st)
+{
+    hipLaunchKernelGGL((face_energy_grad_f64<NCP, NPT, GRP>), dim3(nblocks), dim3(kFaceBlock), 0,
+                       st, a);
+    return hipGetLastError();
+}
+
+int face_blocks(int nf, int grp) { return (nf * grp + kFaceBlock - 1) / kFaceBlock; }
+
+hipError_t launch_face_energy_grad(const FaceArgs &a, int ncp, int npt, int grp, hipStream_t st)
+{
+    const int nb = face_blocks(a.nf, grp);
+    if (a.nf == 0) return hipSuccess;
+#define UMCP_EG(NCPv, NPTv)                                            \
+    if (ncp == NCPv && npt == NPTv) {                                  \
+        switch (grp) {                                                 \
+        case 1: return launch_eg<NCPv, NPTv, 1>(a, nb, st);            \
+        case 2: return launch_eg<NCPv, NPTv, 2>(a, nb, st);            \
+        case 4: return launch_eg<NCPv, NPTv, 4>(a, nb, st);            \
+        case 8: return launch_eg<NCPv, NPTv, 8>(a, nb, st);            \
+        default: return hipErrorInvalidValue;                          \
+        }                                                              \
+    }
+    UMCP_EG(12, 15)
+    UMCP_EG(11, 16)
+    UMCP_EG(13, 16)
+#undef UMCP_EG
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_face_metric(const FaceArgs &a, int ncp, int npt, int mode, double *g0_out,
+                              double *rurv0_out, hipStream_t st)
+{
+    if (a.nf == 0) return hipSuccess;
+    const int nb = face_blocks(a.nf, 1);
+#define UMCP_MT(NCPv, NPTv)                                                                      \
+    if (ncp == NCPv && npt == NPTv) {                                                            \
+        hipLaunchKernelGGL((face_metric<NCPv, NPTv>), dim3(nb), dim3(kFaceBlock), 0, st, a, mode, \
+                           g0_out, rurv0_out);                                                   \
+        return hipGetLastError();                                                                \
+    }
+    UMCP_MT(12, 15)
+    UMCP_MT(11, 16)
+    UMCP_MT(13, 16)
+#undef UMCP_MT
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_vertex_gather(const GatherArgs &a, hipStream_t st)
+{
+    int nb = (a.out_mode == 2) ? 1 : (a.nv + kGatherBlock - 1) / kGatherBlock;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(vertex_gather, dim3(nb), dim3(kGatherBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace umcp

@@ -1,0 +1,280 @@
+"""Python mirror of the reference ``surface`` interface over libumcp_core.so.
+
+The reference drives its hot path through C++ methods of ``struct surface``
+(src/interp.h:210-552): ``loadLattice`` (291), ``generatePlan`` (337),
+``setg0`` (293), ``energy`` (294), ``grad`` (304, accumulating), ``area``
+(301).  ``Surface`` keeps those names, argument meanings and the
+``r``/``g`` layout (3 nv coordinates + the alpha triple), and calls the HIP
+library through its C ABI (include/umcp.h).  There is no CPU fallback: if the
+library or a GPU is missing, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import units
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libumcp_core.so")
+
+MODE_LIPID = 0
+MODE_HELFRICH = 1
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+# every symbol include/umcp.h declares
+EXPORTS = [
+    "umcp_link_check", "umcp_set_device", "umcp_load_mesh", "umcp_destroy", "umcp_last_error",
+    "umcp_nv", "umcp_nfaces", "umcp_get", "umcp_set_params", "umcp_prepare", "umcp_plan", "umcp_set_face_c0",
+    "umcp_setg0", "umcp_energy", "umcp_grad", "umcp_energy_grad", "umcp_area", "umcp_bd_steps",
+    "umcp_energy_grad_dev", "umcp_bd_steps_dev", "umcp_class_count", "umcp_class_info",
+    "umcp_class_table", "umcp_face_points", "umcp_face_lipids", "umcp_set_launch",
+    "umcp_get_launch", "umcp_profile_begin", "umcp_profile_end",
+]
+
+
+class UmcpError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [("kc", C.c_double), ("kg", C.c_double), ("KA", C.c_double), ("k_reg", C.c_double),
+                ("mode", C.c_int), ("precision", C.c_int), ("lipid_c0", C.c_double),
+                ("lipid_apl", C.c_double)]
+
+
+class BDParams(C.Structure):
+    _fields_ = [("mobility_dt", C.c_double), ("noise_amp", C.c_double)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libumcp_core.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise UmcpError("libumcp_core.so is not built (run python -m umcp_amd.build)")
+    L = C.CDLL(path)
+    dp, ip, vp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_void_p
+    sig = {
+        "umcp_link_check": ([], C.c_int),
+        "umcp_set_device": ([C.c_int], C.c_int),
+        "umcp_load_mesh": ([C.c_char_p, C.POINTER(vp)], C.c_int),
+        "umcp_destroy": ([vp], None),
+        "umcp_last_error": ([vp], C.c_char_p),
+        "umcp_nv": ([vp], C.c_int),
+        "umcp_nfaces": ([vp, ip, ip], C.c_int),
+        "umcp_get": ([vp, dp], C.c_int),
+        "umcp_set_params": ([vp, C.POINTER(Params)], C.c_int),
+        "umcp_prepare": ([vp], C.c_int),
+        "umcp_plan": ([vp], C.c_int),
+        "umcp_set_face_c0": ([vp, dp], C.c_int),
+        "umcp_setg0": ([vp, dp], C.c_int),
+        "umcp_energy": ([vp, dp, dp], C.c_int),
+        "umcp_grad": ([vp, dp, dp], C.c_int),
+        "umcp_energy_grad": ([vp, dp, dp, dp], C.c_int),
+        "umcp_area": ([vp, dp, dp, dp], C.c_int),
+        "umcp_bd_steps": ([vp, dp, dp, C.c_int, C.POINTER(BDParams), dp], C.c_int),
+        "umcp_energy_grad_dev": ([vp, vp, vp, vp, vp], C.c_int),
+        "umcp_bd_steps_dev": ([vp, vp, vp, C.c_int, C.POINTER(BDParams), vp, vp], C.c_int),
+        "umcp_class_count": ([vp], C.c_int),
+        "umcp_class_info": ([vp, C.c_int, ip, ip, ip, ip], C.c_int),
+        "umcp_class_table": ([vp, C.c_int, dp, dp], C.c_int),
+        "umcp_face_points": ([vp, C.c_int, ip, ip], C.c_int),
+        "umcp_face_lipids": ([vp, C.c_int, dp, dp], C.c_int),
+        "umcp_set_launch": ([vp, C.c_int, C.c_int], C.c_int),
+        "umcp_get_launch": ([vp, C.c_int], C.c_int),
+        "umcp_profile_begin": ([vp], C.c_int),
+        "umcp_profile_end": ([vp, C.c_int, dp, ip], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Surface:
+    """One membrane surface resident on a GPU.
+
+    Mirrors the reference call sequence of src/min.C:68-99:
+    ``loadLattice(path)`` -> ``generatePlan()`` -> ``get()`` -> ``setg0(r)``
+    -> ``energy(r)`` / ``grad(r, g)``.
+    """
+
+    def __init__(self, mesh_path: str | None = None, device: int = 0):
+        self._L = load_library()
+        self._h = None
+        self.device = device
+        if mesh_path is not None:
+            self.loadLattice(mesh_path)
+
+    # -- lifecycle -------------------------------------------------------
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            msg = self._L.umcp_last_error(self._h).decode() if self._h else ""
+            raise UmcpError("%s failed (%d): %s" % (what, rc, msg))
+
+    def loadLattice(self, path: str, noise: float = 0.0):
+        if noise != 0.0:
+            raise UmcpError("loadLattice noise is not supported; perturb r instead")
+        self._check(self._L.umcp_set_device(self.device), "umcp_set_device")
+        h = C.c_void_p()
+        rc = self._L.umcp_load_mesh(path.encode(), C.byref(h))
+        self._h = h if h.value else None
+        self._check(rc, "loadLattice")
+        self.nv = self._L.umcp_nv(self._h)
+        return 0
+
+    def close(self):
+        if self._h is not None:
+            self._L.umcp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- setup -----------------------------------------------------------
+    def set_params(self, kc=14.0, kg=0.0, KA=0.0, k_reg=0.0, mode=MODE_LIPID,
+                   precision=PREC_FP64, lipid_c0=0.0, lipid_apl=68.3):
+        p = Params(kc, kg, KA, k_reg, mode, precision, lipid_c0, lipid_apl)
+        self._check(self._L.umcp_set_params(self._h, C.byref(p)), "set_params")
+
+    def generatePlan(self):
+        self._check(self._L.umcp_plan(self._h), "generatePlan")
+        nr, ni = C.c_int(), C.c_int()
+        self.nfaces = self._L.umcp_nfaces(self._h, C.byref(nr), C.byref(ni))
+        self.nreg, self.nirr = nr.value, ni.value
+
+    def get(self) -> np.ndarray:
+        r = np.zeros(3 * self.nv + 3)
+        self._check(self._L.umcp_get(self._h, _dp(r)), "get")
+        return r
+
+    def setg0(self, r: np.ndarray):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        self._check(self._L.umcp_setg0(self._h, _dp(r)), "setg0")
+
+    def set_face_c0(self, c0: np.ndarray):
+        c0 = np.ascontiguousarray(c0, dtype=np.float64)
+        if c0.size != self.nfaces:
+            raise UmcpError("c0 needs one value per face")
+        self._check(self._L.umcp_set_face_c0(self._h, _dp(c0)), "set_face_c0")
+
+    # -- hot path --------------------------------------------------------
+    def energy(self, r: np.ndarray) -> float:
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        E = C.c_double()
+        self._check(self._L.umcp_energy(self._h, _dp(r), C.byref(E)), "energy")
+        return E.value
+
+    def grad(self, r: np.ndarray, g: np.ndarray) -> np.ndarray:
+        """Adds dE/dr (and dE/dalpha) into g, as surface::grad does."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        if g.dtype != np.float64 or not g.flags.c_contiguous:
+            raise UmcpError("g must be a contiguous float64 array")
+        self._check(self._L.umcp_grad(self._h, _dp(r), _dp(g)), "grad")
+        return g
+
+    def energy_grad(self, r: np.ndarray):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        g = np.zeros_like(r)
+        E = C.c_double()
+        self._check(self._L.umcp_energy_grad(self._h, _dp(r), C.byref(E), _dp(g)), "energy_grad")
+        return E.value, g
+
+    def area(self, r: np.ndarray):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        A, A0 = C.c_double(), C.c_double()
+        self._check(self._L.umcp_area(self._h, _dp(r), C.byref(A), C.byref(A0)), "area")
+        return A.value, A0.value
+
+    def bd_steps(self, r: np.ndarray, xi: np.ndarray, mobility_dt: float, noise_amp: float,
+                 want_energies: bool = False):
+        """Brownian steps from host arrays; returns the new r (and energies)."""
+        r = np.ascontiguousarray(r, dtype=np.float64).copy()
+        xi = np.ascontiguousarray(xi, dtype=np.float64)
+        n = xi.size // (3 * self.nv)
+        en = np.zeros(max(n, 1)) if want_energies else None
+        bd = BDParams(mobility_dt, noise_amp)
+        rc = self._L.umcp_bd_steps(self._h, _dp(r), _dp(xi), n, C.byref(bd),
+                                   _dp(en) if en is not None else None)
+        self._check(rc, "bd_steps")
+        return (r, en[:n]) if want_energies else r
+
+    # -- device-resident (torch tensors on the GPU) ----------------------
+    def energy_grad_dev(self, d_r, d_E, d_g, stream=None):
+        rc = self._L.umcp_energy_grad_dev(self._h, C.c_void_p(d_r.data_ptr()),
+                                          C.c_void_p(d_E.data_ptr()),
+                                          C.c_void_p(d_g.data_ptr()) if d_g is not None else None,
+                                          C.c_void_p(stream) if stream else None)
+        self._check(rc, "energy_grad_dev")
+
+    def bd_steps_dev(self, d_r, d_xi, nsteps: int, mobility_dt: float, noise_amp: float,
+                     d_energies=None, stream=None):
+        bd = BDParams(mobility_dt, noise_amp)
+        rc = self._L.umcp_bd_steps_dev(self._h, C.c_void_p(d_r.data_ptr()),
+                                       C.c_void_p(d_xi.data_ptr()), nsteps, C.byref(bd),
+                                       C.c_void_p(d_energies.data_ptr()) if d_energies is not None
+                                       else None, C.c_void_p(stream) if stream else None)
+        self._check(rc, "bd_steps_dev")
+
+    # -- introspection ---------------------------------------------------
+    def classes(self):
+        out = []
+        for c in range(self._L.umcp_class_count(self._h)):
+            val, ncp, npt, nf = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+            self._L.umcp_class_info(self._h, c, C.byref(val), C.byref(ncp), C.byref(npt), C.byref(nf))
+            tab = np.zeros(npt.value * 6 * ncp.value)
+            w = np.zeros(npt.value)
+            self._L.umcp_class_table(self._h, c, _dp(tab), _dp(w))
+            out.append(dict(valence=val.value, ncp=ncp.value, npt=npt.value, nf=nf.value,
+                            table=tab.reshape(npt.value, 6, ncp.value), weights=w))
+        return out
+
+    def set_launch(self, cls: int, lanes_per_face: int):
+        self._check(self._L.umcp_set_launch(self._h, cls, lanes_per_face), "set_launch")
+
+    def get_launch(self, cls: int) -> int:
+        return self._L.umcp_get_launch(self._h, cls)
+
+    def profile_begin(self):
+        self._check(self._L.umcp_profile_begin(self._h), "profile_begin")
+
+    def profile_end(self):
+        n = self._L.umcp_class_count(self._h)
+        ms = np.zeros(max(n, 1))
+        cnt = (C.c_int * max(n, 1))()
+        self._check(self._L.umcp_profile_end(self._h, n, _dp(ms), cnt), "profile_end")
+        return [(float(ms[c]), int(cnt[c])) for c in range(n)]
+
+    def face_points(self, f: int):
+        ncp = C.c_int()
+        cp = (C.c_int * 32)()
+        self._check(self._L.umcp_face_points(self._h, f, C.byref(ncp), cp), "face_points")
+        return list(cp)[:ncp.value]
+
+    def face_lipids(self, f: int):
+        lo, li = C.c_double(), C.c_double()
+        self._check(self._L.umcp_face_lipids(self._h, f, C.byref(lo), C.byref(li)), "face_lipids")
+        return lo.value, li.value
+
+
+def bd_coefficients(D_m2_per_s: float, dt_s: float, T_K: float = 298.0):
+    """(mobility_dt, noise_amp) of r <- r - (D dt/kT) dE/dr + sqrt(2 D dt) xi."""
+    D = units.diffusion_A2_per_s(D_m2_per_s)
+    return D * dt_s / units.kT(T_K), float(np.sqrt(2.0 * D * dt_s))
