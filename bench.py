@@ -78,6 +78,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--split", type=int, default=0, help="waves sharing a face group's Gauss points (0: library default)")
+    ap.add_argument("--flags", type=int, default=-1, help="launch flags (1 LDS staging, 2 two-wave register bound)")
+    ap.add_argument("--sweep", action="store_true", help="report face-kernel time per launch shape and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs one mesh per GPU and has no "
@@ -108,9 +111,25 @@ def main():
     d_xi = torch.randn((W + K, 3 * nv), generator=gen, device=dev, dtype=torch.float64)
     d_r = torch.tensor(w.r, device=dev, dtype=torch.float64)
     d_en = torch.zeros(W + K, device=dev, dtype=torch.float64)
-    stream = torch.cuda.current_stream(dev)
+    # a stream of our own at default priority; the library enqueues on it
+    stream = torch.cuda.Stream(device=dev)
     sptr = stream.cuda_stream
-
+    if args.split or args.flags >= 0:
+        sp0, fl0 = s.get_launch()
+        s.set_launch(args.split or sp0, args.flags if args.flags >= 0 else fl0)
+    if args.sweep:
+        for split in (1, 2, 3, 4, 5):
+            for flags in (0, 1, 2, 3):
+                s.set_launch(split, flags)
+                s.bd_steps_dev(d_r, d_xi[:W], W, mob, amp, d_en[:W], stream=sptr)
+                torch.cuda.synchronize()
+                s.profile_begin()
+                s.bd_steps_dev(d_r, d_xi[:W], W, mob, amp, d_en[:W], stream=sptr)
+                torch.cuda.synchronize()
+                tot, n = s.profile_end()
+                print(json.dumps({"split": split, "flags": flags, "face_us": 1e3 * tot / max(n, 1)}),
+                      flush=True)
+        return
     s.bd_steps_dev(d_r, d_xi[:W], W, mob, amp, d_en[:W], stream=sptr)
     torch.cuda.synchronize()
 
@@ -129,13 +148,13 @@ def main():
 
     secs = ms / 1e3
     value = nf * K / secs
-    # dominant kernel: the regular-face class
+    # dominant kernel: the fused face kernel (all face classes in one launch)
     reg = classes[0]
-    reg_ms, reg_n = prof[0]
-    reg_avg_s = reg_ms / reg_n / 1e3
-    flops = reg["nf"] * face_flops(reg["ncp"], reg["npt"])
-    achieved = flops / reg_avg_s / 1e12
-    total_face_ms = sum(p[0] for p in prof) / K
+    face_ms, face_n = prof
+    face_avg_s = face_ms / face_n / 1e3
+    flops = sum(c["nf"] * face_flops(c["ncp"], c["npt"]) for c in classes)
+    achieved = flops / face_avg_s / 1e12
+    split, flags = s.get_launch()
     out = {
         "metric": METRIC,
         "value": value,
@@ -152,15 +171,16 @@ def main():
         "config": {"workload": "vesicle_40962_bd", "nv": nv, "faces": nf,
                    "faces_regular": reg["nf"], "faces_irregular": nf - reg["nf"],
                    "energy_form": "helfrich per-face c0", "precision": "fp64",
-                   "D_m2_per_s": 1e-12, "dt_s": 1e-9, "lanes_per_face": s.get_launch(0)},
+                   "D_m2_per_s": 1e-12, "dt_s": 1e-9, "point_split": split, "launch_flags": flags},
         "bd_steps_per_s": K / secs,
-        "face_kernel_ms_per_step": total_face_ms,
+        "face_kernel_ms_per_step": face_ms / face_n,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": "face_energy_grad_f64<12,15,%d>" % s.get_launch(0),
-                     "avg_launch_us": reg_avg_s * 1e6,
+                     "kernel": "faces_f64<%d,%d,%d>" % (split, flags & 1, 2 if flags & 2 else 1),
+                     "avg_launch_us": face_avg_s * 1e6,
                      "flop_per_launch": flops,
-                     "algorithmic_bytes_per_launch": reg["nf"] * face_bytes(12, 15, nv, nf)},
+                     "algorithmic_bytes_per_launch": sum(
+                         c["nf"] * face_bytes(c["ncp"], c["npt"], nv, nf) for c in classes)},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, c0, args.cpu_budget)

@@ -126,15 +126,20 @@ int umcp_energy_grad_dev(umcp_surface *s, const double *d_r, double *d_E, double
 int umcp_bd_steps_dev(umcp_surface *s, double *d_r, const double *d_xi, int nsteps,
                       const umcp_bd_params *bd, double *d_energies, void *stream);
 
-/* launch shape of a face class: lanes_per_face in {1, 2, 4, 8} share the
- * Gauss points of one face (no reference counterpart; used by the tuner) */
-int umcp_set_launch(umcp_surface *s, int cls, int lanes_per_face);
-int umcp_get_launch(const umcp_surface *s, int cls);
+/* launch shape of the face kernel (no reference counterpart; set by the
+ * tuner): point_split in 1..5 waves share the Gauss points of 64 faces;
+ * flags: UMCP_LAUNCH_STAGE_LDS keeps the faces' control points in LDS
+ * instead of registers, UMCP_LAUNCH_OCC2 bounds registers for two waves per
+ * SIMD */
+#define UMCP_LAUNCH_STAGE_LDS 1
+#define UMCP_LAUNCH_OCC2 2
+int umcp_set_launch(umcp_surface *s, int point_split, int flags);
+int umcp_get_launch(const umcp_surface *s, int *point_split, int *flags);
 
-/* per-launch timing of the face kernels with HIP events recorded on the
+/* per-launch timing of the face kernel with HIP events recorded on the
  * stream each launch uses (no reference counterpart; benchmark support):
- * begin, run work, then end returns per class the summed milliseconds and
- * the number of launches */
+ * begin, run work, then end returns the summed milliseconds and the number
+ * of launches (all face classes share one launch, reported as entry 0) */
 int umcp_profile_begin(umcp_surface *s);
 int umcp_profile_end(umcp_surface *s, int max_cls, double *ms, int *launches);
 

@@ -77,14 +77,14 @@ def test_vesicle_642_lipid_mode(ves642):
         assert g.face_lipids(f) == o.tri_lipids(t)
 
 
-def test_lanes_per_face_variants(ves642):
+def test_launch_shapes(ves642):
+    """Every launch shape (point split across 1-5 waves, control points in
+    registers or LDS, register bound for 1 or 2 waves/SIMD) meets the bar."""
     g, o = make_pair(ves642)
-    E_ref, g_ref = g.energy_grad(ves642.r)
-    for c in range(len(g.classes())):
-        for lanes in (1, 2, 8, 4):
-            g.set_launch(c, lanes)
-            say("[parity] lanes", c, lanes)
-            check_eval(g, o, ves642.r, "vesicle_642 class %d lanes %d" % (c, lanes))
+    for split in (1, 2, 3, 4, 5):
+        for flags in (0, 1, 2, 3):
+            g.set_launch(split, flags)
+            check_eval(g, o, ves642.r, "vesicle_642 split %d flags %d" % (split, flags))
 
 
 def test_all_terms_helfrich():

@@ -549,14 +549,8 @@ int build_plan(const Topology &T, Plan &P, std::string &err)
             }
         }
     }
-    // vertex -> slot CSR
-    P.class_slot_base.resize(P.classes.size());
-    int base = 0;
-    for (size_t c = 0; c < P.classes.size(); c++) {
-        P.class_slot_base[c] = base;
-        base += P.classes[c].ncp * P.classes[c].nf();
-    }
-    P.nslots = base;
+    // vertex -> (face, slot) CSR; within a vertex, entries ordered by class,
+    // then slot, then face
     std::vector<int> cnt(T.nv + 1, 0);
     for (size_t c = 0; c < P.classes.size(); c++) {
         const FaceClass &C = P.classes[c];
@@ -564,11 +558,12 @@ int build_plan(const Topology &T, Plan &P, std::string &err)
     }
     for (int v = 0; v < T.nv; v++) cnt[v + 1] += cnt[v];
     P.vstart = cnt;
-    P.vslot.assign(cnt[T.nv], 0);
-    std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+    P.nnz = cnt[T.nv];
+    std::vector<int> next(cnt.begin(), cnt.end() - 1);
     for (size_t c = 0; c < P.classes.size(); c++) {
-        const FaceClass &C = P.classes[c];
-        for (size_t s = 0; s < C.cp.size(); s++) P.vslot[pos[C.cp[s]]++] = P.class_slot_base[c] + (int)s;
+        FaceClass &C = P.classes[c];
+        C.pos.resize(C.cp.size());
+        for (size_t s = 0; s < C.cp.size(); s++) C.pos[s] = next[C.cp[s]]++;
     }
     return 0;
 }

@@ -41,6 +41,7 @@ struct FaceClass {
     std::vector<uint8_t> has_img;      // [nf]        any image shift on this face
     std::vector<double> table;         // [npt][6][ncp] r_w, r_u, r_v, r_uu, r_uv, r_vv
     std::vector<double> weight;        // [npt]
+    std::vector<int> pos;              // [ncp][nf]   CSR position of (face, slot)
     int nf() const { return (int)face_id.size(); }
 };
 
@@ -48,11 +49,10 @@ struct Plan {
     int nfaces = 0;
     int nreg = 0;
     std::vector<FaceClass> classes;    // classes[0] is the regular class
-    // vertex -> gradient slot incidence, CSR (slot = class base + c * nf + f)
+    // vertex -> (face, slot) incidence as CSR: the gradient contribution of
+    // slot c of face f lands at position pos[c][f] inside its vertex's range
     std::vector<int> vstart;           // nv + 1
-    std::vector<int> vslot;            // nnz
-    std::vector<int> class_slot_base;  // per class
-    int nslots = 0;
+    int nnz = 0;
 };
 
 int parse_mesh(const char *path, Topology &T, std::string &err);

@@ -8,6 +8,7 @@
 // entry points.
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -24,12 +25,9 @@ namespace {
 thread_local int g_device = 0;
 
 struct ClassDev {
-    int *cp = nullptr;
+    int *cp = nullptr, *pos = nullptr;
     uint8_t *img = nullptr, *has_img = nullptr;
     double *g0 = nullptr, *rurv0 = nullptr, *Lo = nullptr, *Li = nullptr, *c0 = nullptr;
-    double *table = nullptr, *weight = nullptr;
-    int grp = 1;
-    int partial_base = 0;
 };
 
 }  // namespace
@@ -47,10 +45,16 @@ struct umcp_surface {
     std::vector<ClassDev> cls;
     std::vector<void *> allocations;
     // work buffers
-    double *d_r = nullptr, *d_g = nullptr, *d_gslot = nullptr, *d_partial = nullptr;
+    double *d_r = nullptr, *d_g = nullptr, *d_gbuf = nullptr, *d_partial = nullptr;
     double *d_scalar = nullptr;    // [8]: E, sums[4], ...
-    int *d_vstart = nullptr, *d_vslot = nullptr;
+    int *d_vstart = nullptr;
     int npartial = 0;
+    // launch shape of the face kernel: waves sharing 64 faces' Gauss points,
+    // and whether the faces' control points are staged in LDS
+    int point_split = 3;
+    int launch_flags = 0;
+    std::vector<int> launch_order;  // classes in launch order (irregular first)
+    std::vector<int> block_start;
     double *d_xi = nullptr;        // host-array BD staging, chunk of steps
     int xi_chunk = 0;
     double *d_energies = nullptr;
@@ -132,27 +136,25 @@ FaceArgs face_args(umcp_surface *s, size_t c, const double *d_r)
     a.Lo = D.Lo;
     a.Li = D.Li;
     a.c0 = D.c0;
-    a.table = D.table;
-    a.weight = D.weight;
+    a.pos = D.pos;
     a.dudv = C.dudv;
     std::memcpy(a.pbc, s->T.pbc, sizeof a.pbc);
-    a.gslot = s->d_gslot;
-    a.slot_base = s->P.class_slot_base[c];
-    a.nslots = s->P.nslots;
-    a.partial = s->d_partial;
-    a.partial_base = D.partial_base;
+    a.gbuf = s->d_gbuf;
     a.ph = phys(s);
     return a;
 }
 
-void layout_partials(umcp_surface *s)
+// irregular classes first so their few blocks start early and overlap the
+// regular faces
+void layout_launch(umcp_surface *s)
 {
-    int base = 0;
-    for (size_t c = 0; c < s->cls.size(); c++) {
-        s->cls[c].partial_base = base;
-        base += face_blocks(s->P.classes[c].nf(), s->cls[c].grp);
-    }
-    s->npartial = base;
+    s->launch_order.clear();
+    for (size_t c = 1; c < s->cls.size(); c++)
+        if (s->P.classes[c].nf() > 0) s->launch_order.push_back((int)c);
+    if (!s->P.classes.empty() && s->P.classes[0].nf() > 0) s->launch_order.push_back(0);
+    s->block_start.assign(1, 0);
+    for (int c : s->launch_order) s->block_start.push_back(s->block_start.back() + face_groups(s->P.classes[c].nf()));
+    s->npartial = s->block_start.back();
 }
 
 // enqueue one energy (+ gradient) evaluation on the surface stream
@@ -176,27 +178,31 @@ int prof_pair(umcp_surface *s, int cls, hipEvent_t *ev)
 
 int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs ga)
 {
-    for (size_t c = 0; c < s->cls.size(); c++) {
-        const FaceClass &C = s->P.classes[c];
-        FaceArgs a = face_args(s, c, d_r);
-        hipEvent_t ev[2] = {nullptr, nullptr};
-        if (s->prof_on) {
-            int rc = prof_pair(s, (int)c, ev);
-            if (rc) return rc;
-            (void)hipEventRecord(ev[0], st);
-        }
-        hipError_t e = launch_face_energy_grad(a, C.ncp, C.npt, s->cls[c].grp, st);
-        if (e != hipSuccess) return hip_fail(s, e, "face kernel launch");
-        if (s->prof_on) (void)hipEventRecord(ev[1], st);
+    MultiFaceArgs m{};
+    m.ncls = (int)s->launch_order.size();
+    for (int i = 0; i < m.ncls; i++) {
+        const int c = s->launch_order[i];
+        m.cls[i] = face_args(s, c, d_r);
+        m.kind[i] = kind_of(s->P.classes[c].ncp, s->P.classes[c].npt);
+        m.block_start[i] = s->block_start[i];
     }
+    m.block_start[m.ncls] = s->block_start[m.ncls];
+    m.partial = s->d_partial;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (s->prof_on) {
+        int rc = prof_pair(s, 0, ev);
+        if (rc) return rc;
+        (void)hipEventRecord(ev[0], st);
+    }
+    hipError_t e = launch_faces(m, s->npartial, s->point_split, s->launch_flags, st);
+    if (e != hipSuccess) return hip_fail(s, e, "face kernel launch");
+    if (s->prof_on) (void)hipEventRecord(ev[1], st);
     ga.nv = s->T.nv;
     ga.vstart = s->d_vstart;
-    ga.vslot = s->d_vslot;
-    ga.gslot = s->d_gslot;
-    ga.nslots = s->P.nslots;
+    ga.gbuf = s->d_gbuf;
     ga.partial = s->d_partial;
     ga.npartial = s->npartial;
-    hipError_t e = launch_vertex_gather(ga, st);
+    e = launch_vertex_gather(ga, st);
     if (e != hipSuccess) return hip_fail(s, e, "gather kernel launch");
     return UMCP_OK;
 }
@@ -338,19 +344,20 @@ int umcp_plan(umcp_surface *s)
         const FaceClass &C = s->P.classes[c];
         ClassDev &D = s->cls[c];
         const size_t nf = C.nf();
-        // four lanes per face spread the Gauss points of a face over a quad
-        D.grp = 4;
-        if ((rc = dev_alloc(s, &D.cp, C.cp.size())) || (rc = dev_alloc(s, &D.img, C.img.size())) ||
+        if (kind_of(C.ncp, C.npt) < 0) return fail(s, UMCP_ERR_BAD_MESH, "no kernel for this face valence");
+        if ((rc = dev_alloc(s, &D.cp, C.cp.size())) || (rc = dev_alloc(s, &D.pos, C.pos.size())) || (rc = dev_alloc(s, &D.img, C.img.size())) ||
             (rc = dev_alloc(s, &D.has_img, nf)) || (rc = dev_alloc(s, &D.g0, nf * C.npt)) ||
             (rc = dev_alloc(s, &D.rurv0, nf * C.npt)) || (rc = dev_alloc(s, &D.Lo, nf)) ||
             (rc = dev_alloc(s, &D.Li, nf)) || (rc = dev_alloc(s, &D.c0, nf)) ||
-            (rc = dev_alloc(s, &D.table, C.table.size())) || (rc = dev_alloc(s, &D.weight, C.weight.size())))
+            0)
             return rc;
+        e = upload_tables(kind_of(C.ncp, C.npt), C.table.data(), C.weight.data(), C.npt, C.ncp);
+        if (e != hipSuccess) return hip_fail(s, e, "upload Gauss-point tables");
         if ((rc = dev_upload(s, D.cp, C.cp.data(), C.cp.size())) ||
+            (rc = dev_upload(s, D.pos, C.pos.data(), C.pos.size())) ||
             (rc = dev_upload(s, D.img, C.img.data(), C.img.size())) ||
             (rc = dev_upload(s, D.has_img, C.has_img.data(), nf)) ||
-            (rc = dev_upload(s, D.table, C.table.data(), C.table.size())) ||
-            (rc = dev_upload(s, D.weight, C.weight.data(), C.weight.size())))
+            0)
             return rc;
         std::vector<double> zero(nf * C.npt, 0.0);
         if ((rc = dev_upload(s, D.c0, zero.data(), nf)) || (rc = dev_upload(s, D.Lo, zero.data(), nf)) ||
@@ -358,17 +365,16 @@ int umcp_plan(umcp_surface *s)
             (rc = dev_upload(s, D.rurv0, zero.data(), nf * C.npt)))
             return rc;
     }
-    int max_partial = 0;
-    for (size_t c = 0; c < s->P.classes.size(); c++) max_partial += face_blocks(s->P.classes[c].nf(), 8);
+    layout_launch(s);
+    int metric_total = 0;
+    for (size_t c = 0; c < s->P.classes.size(); c++) metric_total += metric_blocks(s->P.classes[c].nf());
+    const int max_partial = std::max(s->npartial, metric_total);
     if ((rc = dev_alloc(s, &s->d_r, 3 * (size_t)nv + 3)) || (rc = dev_alloc(s, &s->d_g, 3 * (size_t)nv + 3)) ||
-        (rc = dev_alloc(s, &s->d_gslot, 3 * (size_t)s->P.nslots)) ||
+        (rc = dev_alloc(s, &s->d_gbuf, 3 * (size_t)s->P.nnz)) ||
         (rc = dev_alloc(s, &s->d_partial, 4 * (size_t)max_partial)) || (rc = dev_alloc(s, &s->d_scalar, 8)) ||
-        (rc = dev_alloc(s, &s->d_vstart, s->P.vstart.size())) || (rc = dev_alloc(s, &s->d_vslot, s->P.vslot.size())))
+        (rc = dev_alloc(s, &s->d_vstart, s->P.vstart.size())))
         return rc;
-    if ((rc = dev_upload(s, s->d_vstart, s->P.vstart.data(), s->P.vstart.size())) ||
-        (rc = dev_upload(s, s->d_vslot, s->P.vslot.data(), s->P.vslot.size())))
-        return rc;
-    layout_partials(s);
+    if ((rc = dev_upload(s, s->d_vstart, s->P.vstart.data(), s->P.vstart.size()))) return rc;
     if ((rc = sync(s))) return rc;
     s->planned = true;
     return UMCP_OK;
@@ -398,7 +404,8 @@ int umcp_setg0(umcp_surface *s, const double *r)
     for (size_t c = 0; c < s->P.classes.size(); c++) {
         const FaceClass &C = s->P.classes[c];
         FaceArgs a = face_args(s, c, s->d_r);
-        hipError_t e = launch_face_metric(a, C.ncp, C.npt, 0, s->cls[c].g0, s->cls[c].rurv0, s->stream);
+        hipError_t e = launch_face_metric(a, kind_of(C.ncp, C.npt), 0, s->cls[c].g0, s->cls[c].rurv0,
+                                          nullptr, 0, s->stream);
         if (e != hipSuccess) return hip_fail(s, e, "metric kernel launch");
     }
     // single-lipid packing per triangle from the reference area of each face
@@ -493,9 +500,9 @@ int umcp_area(umcp_surface *s, const double *r, double *A, double *A0)
     for (size_t c = 0; c < s->P.classes.size(); c++) {
         const FaceClass &C = s->P.classes[c];
         FaceArgs a = face_args(s, c, s->d_r);
-        a.partial_base = base;
-        base += face_blocks(C.nf(), 1);
-        hipError_t e = launch_face_metric(a, C.ncp, C.npt, 1, nullptr, nullptr, s->stream);
+        hipError_t e = launch_face_metric(a, kind_of(C.ncp, C.npt), 1, nullptr, nullptr, s->d_partial, base,
+                                          s->stream);
+        base += metric_blocks(C.nf());
         if (e != hipSuccess) return hip_fail(s, e, "metric kernel launch");
     }
     GatherArgs ga{};
@@ -584,21 +591,23 @@ int umcp_bd_steps(umcp_surface *s, double *r, const double *xi, int nsteps, cons
     return sync(s);
 }
 
-int umcp_set_launch(umcp_surface *s, int cls, int lanes_per_face)
+int umcp_set_launch(umcp_surface *s, int point_split, int flags)
 {
     if (!s || !s->planned) return UMCP_ERR_STATE;
-    if (cls < 0 || cls >= (int)s->cls.size()) return fail(s, UMCP_ERR_ARG, "bad face class");
-    if (lanes_per_face != 1 && lanes_per_face != 2 && lanes_per_face != 4 && lanes_per_face != 8)
-        return fail(s, UMCP_ERR_ARG, "lanes per face must be 1, 2, 4 or 8");
-    s->cls[cls].grp = lanes_per_face;
-    layout_partials(s);
+    if (point_split < 1 || point_split > kMaxSplit)
+        return fail(s, UMCP_ERR_ARG, "point split must be between 1 and 5 waves");
+    if (flags & ~3) return fail(s, UMCP_ERR_ARG, "unknown launch flags");
+    s->point_split = point_split;
+    s->launch_flags = flags;
     return UMCP_OK;
 }
 
-int umcp_get_launch(const umcp_surface *s, int cls)
+int umcp_get_launch(const umcp_surface *s, int *point_split, int *flags)
 {
-    if (!s || !s->planned || cls < 0 || cls >= (int)s->cls.size()) return UMCP_ERR_ARG;
-    return s->cls[cls].grp;
+    if (!s || !s->planned) return UMCP_ERR_STATE;
+    if (point_split) *point_split = s->point_split;
+    if (flags) *flags = s->launch_flags;
+    return UMCP_OK;
 }
 
 int umcp_profile_begin(umcp_surface *s)

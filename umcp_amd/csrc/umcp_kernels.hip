@@ -17,6 +17,34 @@
 
 namespace umcp {
 
+// The Gauss-point tables depend only on the valence of the face's owner, so
+// every surface on the device shares one copy in constant memory; wave-
+// uniform reads from it become scalar loads whose values feed the FMAs as
+// SGPR operands.  Layout per kind: [npt][ncp][5] (the five derivative rows
+// Ru, Rv, Ruu, Ruv, Rvv of each control point contiguous, so one scalar load
+// brings a control point's coefficients) then [npt] weights.  The gradient
+// contraction reads a second copy, so the compiler cannot keep the forward
+// pass's coefficients alive in SGPRs across the pointwise math.
+constexpr int kTabReg = 15 * 5 * 12, kTabV5 = 16 * 5 * 11, kTabV7 = 16 * 5 * 13;
+__constant__ double c_tab_reg[kTabReg + 15];
+__constant__ double c_tab_v5[kTabV5 + 16];
+__constant__ double c_tab_v7[kTabV7 + 16];
+// second copies read by the gradient (backward) contraction only
+__constant__ double c_tab_reg_b[kTabReg];
+__constant__ double c_tab_v5_b[kTabV5];
+__constant__ double c_tab_v7_b[kTabV7];
+
+template <int KIND>
+__device__ __forceinline__ const double *kind_table()
+{
+    return KIND == 0 ? c_tab_reg : (KIND == 1 ? c_tab_v5 : c_tab_v7);
+}
+template <int KIND>
+__device__ __forceinline__ const double *kind_table_b()
+{
+    return KIND == 0 ? c_tab_reg_b : (KIND == 1 ? c_tab_v5_b : c_tab_v7_b);
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs; give every XCD a
 // contiguous run of logical blocks so neighbouring faces (which share
 // vertices) hit the same L2.
@@ -149,105 +177,166 @@ __device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R rur
     return e;
 }
 
-// Fused energy + gradient, FP64.  GRP lanes share one face; lane j takes the
-// Gauss points p = j, j + GRP, ...  Outputs: dE/dr per (face, control point)
-// into gslot (component-major SoA), block partials of E and dE/dalpha.
-template <int NCP, int NPT, int GRP>
-__global__ __launch_bounds__(kFaceBlock) void face_energy_grad_f64(FaceArgs a)
+// Fused energy + gradient of 64 faces by one workgroup of WS waves, FP64.
+// Lane l of every wave owns face group*64 + l; wave w evaluates the Gauss
+// points p = w, w + WS, ... so the table row of each point is wave-uniform
+// (scalar loads feed the FMAs as SGPR operands).  The waves' partial
+// gradients are summed in LDS in a fixed order, then wave 0 writes each
+// (face, control point) gradient into that vertex's CSR slot.
+template <int NCP, int NPT, int KIND, int WS, bool PLDS>
+__device__ __forceinline__ void face_group_f64(const FaceArgs &a, int group, double *lds, double acc[4])
 {
-    const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-    const int t = lb * kFaceBlock + threadIdx.x;
-    const int f = t / GRP, j = t % GRP;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    const bool live = f < a.nf;
-    const int fc = live ? f : a.nf - 1;  // idle lanes shadow the last face (shuffles stay uniform)
+    const double *tab = kind_table<KIND>();
+    const double *wts = tab + NPT * 5 * NCP;
+    const int lane = threadIdx.x & (kWave - 1);
+    // wave index made provably wave-uniform so every table address below is
+    // too (the compiler then reads the tables with scalar loads)
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int nf = a.nf;
+    const int f = group * kWave + lane;
+    const bool live = f < nf;
+    const int fc = live ? f : nf - 1;
     const double al[3] = {a.r[3 * a.nv], a.r[3 * a.nv + 1], a.r[3 * a.nv + 2]};
-    double P[NCP][3];
     const bool shifted = a.has_img[fc] != 0;
-#pragma unroll
-    for (int c = 0; c < NCP; c++) {
+
+    double P[PLDS ? 1 : NCP][3];
+    auto load_cp = [&](int c, double out[3]) {
         const int v = a.cp[c * nf + fc];
-        P[c][0] = a.r[3 * v];
-        P[c][1] = a.r[3 * v + 1];
-        P[c][2] = a.r[3 * v + 2];
-    }
-    if (shifted) {
-#pragma unroll
-        for (int c = 0; c < NCP; c++) {
+        out[0] = a.r[3 * v];
+        out[1] = a.r[3 * v + 1];
+        out[2] = a.r[3 * v + 2];
+        if (shifted) {
             double o[3];
             image_offset(a.img[c * nf + fc], a.pbc, o);
-#pragma unroll
-            for (int k = 0; k < 3; k++) P[c][k] += o[k];
+            out[0] += o[0];
+            out[1] += o[1];
+            out[2] += o[2];
         }
+    };
+    if (PLDS) {
+        for (int c = w; c < NCP; c += WS) {
+            double t[3];
+            load_cp(c, t);
+#pragma unroll
+            for (int k = 0; k < 3; k++) lds[(c * 3 + k) * kWave + lane] = t[k];
+        }
+        __syncthreads();
+    } else {
+#pragma unroll
+        for (int c = 0; c < NCP; c++) load_cp(c, P[c]);
     }
+#define PV(c, k) (PLDS ? lds[((c) * 3 + (k)) * kWave + lane] : P[PLDS ? 0 : (c)][k])
+
     double G[NCP][3];
 #pragma unroll
     for (int c = 0; c < NCP; c++) G[c][0] = G[c][1] = G[c][2] = 0.0;
     const double Lo = a.Lo[fc], Li = a.Li[fc], c0f = a.c0[fc];
     double e = 0.0;
-    for (int p = j; p < NPT; p += GRP) {
-        const double *T = a.table + p * 6 * NCP;  // wave-uniform only when GRP == 1
+    for (int p = w; p < NPT; p += WS) {
+        const double *T = tab + p * 5 * NCP;
+        const double *TB = kind_table_b<KIND>() + p * 5 * NCP;
         double X[5][3];
 #pragma unroll
         for (int d = 0; d < 5; d++)
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                double s = 0.0;
+            for (int k = 0; k < 3; k++) X[d][k] = 0.0;
 #pragma unroll
-                for (int c = 0; c < NCP; c++) s += T[(d + 1) * NCP + c] * P[c][k];
-                X[d][k] = s * al[k];
+        for (int c = 0; c < NCP; c++) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const double x = PV(c, k);
+#pragma unroll
+                    for (int d = 0; d < 5; d++) X[d][k] += T[c * 5 + d] * x;
+                }
             }
-        double D[5][3];
-        e += point_energy_grad<double>(X, a.weight[p], a.g0[p * nf + fc], a.rurv0[p * nf + fc],
-                                       Lo, Li, c0f, a.dudv, a.ph, D);
 #pragma unroll
-        for (int c = 0; c < NCP; c++)
+        for (int d = 0; d < 5; d++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) X[d][k] *= al[k];
+        double D[5][3];
+        e += point_energy_grad<double>(X, wts[p], a.g0[p * nf + fc], a.rurv0[p * nf + fc], Lo, Li,
+                                       c0f, a.dudv, a.ph, D);
+#pragma unroll
+        for (int c = 0; c < NCP; c++) {
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 double s = G[c][k];
 #pragma unroll
-                for (int d = 0; d < 5; d++) s += T[(d + 1) * NCP + c] * D[d][k];
+                for (int d = 0; d < 5; d++) s += TB[c * 5 + d] * D[d][k];
                 G[c][k] = s;
-            }
-    }
-    if (GRP > 1) {
-#pragma unroll
-        for (int m = 1; m < GRP; m <<= 1)
-#pragma unroll
-            for (int c = 0; c < NCP; c++)
-#pragma unroll
-                for (int k = 0; k < 3; k++) G[c][k] += __shfl_xor(G[c][k], m, GRP);
-    }
-    if (live) {
-        acc[0] = e;
-        double *out = a.gslot + a.slot_base;
-#pragma unroll
-        for (int c = 0; c < NCP; c++) {
-            if (c % GRP != j) continue;
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                out[(size_t)k * a.nslots + c * nf + f] = G[c][k] * al[k];
-                acc[1 + k] += G[c][k] * P[c][k];
             }
         }
     }
-    block_sum4<kFaceBlock>(acc);
+    if (live) {
+        acc[0] += e;
+        // dE/dalpha_k = sum_c dE/dR_ck (r + shift)_ck is linear in G: each
+        // wave adds its own points' share
+#pragma unroll
+        for (int c = 0; c < NCP; c++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) acc[1 + k] += G[c][k] * PV(c, k);
+    }
+#undef PV
+    if (WS > 1) {
+        // fixed-order cross-wave sum of G through LDS
+        if (PLDS) __syncthreads();
+        for (int src = 1; src < WS; src++) {
+            if (w == src)
+#pragma unroll
+                for (int c = 0; c < NCP; c++)
+#pragma unroll
+                    for (int k = 0; k < 3; k++) lds[(c * 3 + k) * kWave + lane] = G[c][k];
+            __syncthreads();
+            if (w == 0)
+#pragma unroll
+                for (int c = 0; c < NCP; c++)
+#pragma unroll
+                    for (int k = 0; k < 3; k++) G[c][k] += lds[(c * 3 + k) * kWave + lane];
+            __syncthreads();
+        }
+    }
+    if (w == 0 && live) {
+#pragma unroll
+        for (int c = 0; c < NCP; c++) {
+            double *dst = a.gbuf + 3 * (size_t)a.pos[c * nf + f];
+            dst[0] = G[c][0] * al[0];
+            dst[1] = G[c][1] * al[1];
+            dst[2] = G[c][2] * al[2];
+        }
+    }
+}
+
+template <int WS, bool PLDS, int OCC>
+__global__ __launch_bounds__(kWave *WS, OCC) void faces_f64(MultiFaceArgs m)
+{
+    __shared__ double lds[13 * 3 * kWave];
+    const int b = xcd_logical_block(blockIdx.x, gridDim.x);
+    int c = 0;
+    while (c + 1 < m.ncls && b >= m.block_start[c + 1]) c++;
+    const int group = b - m.block_start[c];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    switch (m.kind[c]) {
+    case 0: face_group_f64<12, 15, 0, WS, PLDS>(m.cls[c], group, lds, acc); break;
+    case 1: face_group_f64<11, 16, 1, WS, PLDS>(m.cls[c], group, lds, acc); break;
+    default: face_group_f64<13, 16, 2, WS, PLDS>(m.cls[c], group, lds, acc); break;
+    }
+    block_sum4<kWave * WS>(acc);
     if (threadIdx.x == 0) {
-        double *dst = a.partial + 4 * (size_t)(a.partial_base + lb);
+        double *dst = m.partial + 4 * (size_t)b;
 #pragma unroll
         for (int i = 0; i < 4; i++) dst[i] = acc[i];
     }
 }
 
 // Per-point metric: mode 0 writes g0 and RuRv0 (no alpha, as setg0 does);
-// mode 1 accumulates sum(dudv g w) and sum(dudv g0 w) into the block partials.
-template <int NCP, int NPT>
-__global__ __launch_bounds__(kFaceBlock) void face_metric(FaceArgs a, int mode, double *g0_out,
-                                                          double *rurv0_out)
+// mode 1 accumulates sum(dudv g w) and sum(dudv g0 w) into block partials.
+template <int NCP, int NPT, int KIND>
+__global__ __launch_bounds__(kMetricBlock) void face_metric(FaceArgs a, int mode, double *g0_out,
+                                                            double *rurv0_out, double *partial,
+                                                            int partial_base)
 {
     const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-    const int f = lb * kFaceBlock + threadIdx.x;
+    const int f = lb * kMetricBlock + threadIdx.x;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     if (f < a.nf) {
         const int nf = a.nf;
@@ -263,15 +352,16 @@ __global__ __launch_bounds__(kFaceBlock) void face_metric(FaceArgs a, int mode, 
             for (int k = 0; k < 3; k++) P[c][k] = a.r[3 * v + k] + o[k];
         }
         for (int p = 0; p < NPT; p++) {
-            const double *T = a.table + p * 6 * NCP;
+            const double *T = kind_table<KIND>() + p * 5 * NCP;
+            const double wp = kind_table<KIND>()[NPT * 5 * NCP + p];
             double Ru[3], Rv[3];
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 double su = 0.0, sv = 0.0;
 #pragma unroll
                 for (int c = 0; c < NCP; c++) {
-                    su += T[1 * NCP + c] * P[c][k];
-                    sv += T[2 * NCP + c] * P[c][k];
+                    su += T[c * 5 + 0] * P[c][k];
+                    sv += T[c * 5 + 1] * P[c][k];
                 }
                 Ru[k] = su * al[k];
                 Rv[k] = sv * al[k];
@@ -284,15 +374,15 @@ __global__ __launch_bounds__(kFaceBlock) void face_metric(FaceArgs a, int mode, 
                 g0_out[p * nf + f] = g;
                 rurv0_out[p * nf + f] = E12 / sqrt(E11 * E22);
             } else {
-                acc[0] += a.dudv * g * a.weight[p];
-                acc[1] += a.dudv * a.g0[p * nf + f] * a.weight[p];
+                acc[0] += a.dudv * g * wp;
+                acc[1] += a.dudv * a.g0[p * nf + f] * wp;
             }
         }
     }
     if (mode == 1) {
-        block_sum4<kFaceBlock>(acc);
+        block_sum4<kMetricBlock>(acc);
         if (threadIdx.x == 0) {
-            double *dst = a.partial + 4 * (size_t)(a.partial_base + lb);
+            double *dst = partial + 4 * (size_t)(partial_base + lb);
 #pragma unroll
             for (int i = 0; i < 4; i++) dst[i] = acc[i];
         }
@@ -331,11 +421,11 @@ __global__ __launch_bounds__(kGatherBlock) void vertex_gather(GatherArgs a)
     if (v >= a.nv) return;
     const int s0 = a.vstart[v], s1 = a.vstart[v + 1];
     double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-    for (int s = s0; s < s1; s++) {
-        const size_t slot = (size_t)a.vslot[s];
-        g0 += a.gslot[slot];
-        g1 += a.gslot[(size_t)a.nslots + slot];
-        g2 += a.gslot[2 * (size_t)a.nslots + slot];
+    const double *src = a.gbuf + 3 * (size_t)s0;
+    for (int s = s0; s < s1; s++, src += 3) {
+        g0 += src[0];
+        g1 += src[1];
+        g2 += src[2];
     }
     if (a.out_mode == 0) {
         a.g[3 * v] = g0;
@@ -354,53 +444,91 @@ __global__ __launch_bounds__(kGatherBlock) void vertex_gather(GatherArgs a)
 // launchers
 // ---------------------------------------------------------------------------
 
-template <int NCP, int NPT, int GRP>
-static hipError_t launch_eg(const FaceArgs &a, int nblocks, hipStream_t st)
+int kind_of(int ncp, int npt)
 {
-    hipLaunchKernelGGL((face_energy_grad_f64<NCP, NPT, GRP>), dim3(nblocks), dim3(kFaceBlock), 0,
-                       st, a);
+    if (ncp == 12 && npt == 15) return 0;
+    if (ncp == 11 && npt == 16) return 1;
+    if (ncp == 13 && npt == 16) return 2;
+    return -1;
+}
+
+hipError_t upload_tables(int kind, const double *table, const double *weights, int npt, int ncp)
+{
+    // host layout [npt][6][ncp] (r_w, r_u, r_v, r_uu, r_uv, r_vv) -> device
+    // [npt][ncp][5] without the position row, then the weights
+    const size_t n = (size_t)npt * 5 * ncp;
+    const size_t cap = kind == 0 ? kTabReg : (kind == 1 ? kTabV5 : kTabV7);
+    if (n != cap) return hipErrorInvalidValue;
+    double *buf = new double[n + npt];
+    for (int p = 0; p < npt; p++)
+        for (int c = 0; c < ncp; c++)
+            for (int d = 0; d < 5; d++) buf[(size_t)(p * ncp + c) * 5 + d] = table[(size_t)(p * 6 + d + 1) * ncp + c];
+    for (int p = 0; p < npt; p++) buf[n + p] = weights[p];
+    const void *fwd = kind == 0 ? (const void *)HIP_SYMBOL(c_tab_reg)
+                                : (kind == 1 ? (const void *)HIP_SYMBOL(c_tab_v5) : (const void *)HIP_SYMBOL(c_tab_v7));
+    const void *bwd = kind == 0 ? (const void *)HIP_SYMBOL(c_tab_reg_b)
+                                : (kind == 1 ? (const void *)HIP_SYMBOL(c_tab_v5_b) : (const void *)HIP_SYMBOL(c_tab_v7_b));
+    hipError_t e = hipMemcpyToSymbol(fwd, buf, (n + npt) * sizeof(double), 0, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpyToSymbol(bwd, buf, n * sizeof(double), 0, hipMemcpyHostToDevice);
+    delete[] buf;
+    return e;
+}
+
+int face_groups(int nf) { return (nf + kWave - 1) / kWave; }
+int metric_blocks(int nf) { return (nf + kMetricBlock - 1) / kMetricBlock; }
+
+template <int WS, bool PLDS, int OCC>
+static hipError_t launch_ws(const MultiFaceArgs &a, int nb, hipStream_t st)
+{
+    hipLaunchKernelGGL((faces_f64<WS, PLDS, OCC>), dim3(nb), dim3(kWave * WS), 0, st, a);
     return hipGetLastError();
 }
 
-int face_blocks(int nf, int grp) { return (nf * grp + kFaceBlock - 1) / kFaceBlock; }
-
-hipError_t launch_face_energy_grad(const FaceArgs &a, int ncp, int npt, int grp, hipStream_t st)
+template <int WS>
+static hipError_t launch_ws_flags(const MultiFaceArgs &a, int nb, int flags, hipStream_t st)
 {
-    const int nb = face_blocks(a.nf, grp);
-    if (a.nf == 0) return hipSuccess;
-#define UMCP_EG(NCPv, NPTv)                                            \
-    if (ncp == NCPv && npt == NPTv) {                                  \
-        switch (grp) {                                                 \
-        case 1: return launch_eg<NCPv, NPTv, 1>(a, nb, st);            \
-        case 2: return launch_eg<NCPv, NPTv, 2>(a, nb, st);            \
-        case 4: return launch_eg<NCPv, NPTv, 4>(a, nb, st);            \
-        case 8: return launch_eg<NCPv, NPTv, 8>(a, nb, st);            \
-        default: return hipErrorInvalidValue;                          \
-        }                                                              \
+    switch (flags & 3) {
+    case 0: return launch_ws<WS, false, 1>(a, nb, st);
+    case 1: return launch_ws<WS, true, 1>(a, nb, st);
+    case 2: return launch_ws<WS, false, 2>(a, nb, st);
+    default: return launch_ws<WS, true, 2>(a, nb, st);
     }
-    UMCP_EG(12, 15)
-    UMCP_EG(11, 16)
-    UMCP_EG(13, 16)
-#undef UMCP_EG
-    return hipErrorInvalidValue;
 }
 
-hipError_t launch_face_metric(const FaceArgs &a, int ncp, int npt, int mode, double *g0_out,
-                              double *rurv0_out, hipStream_t st)
+hipError_t launch_faces(const MultiFaceArgs &a, int nb, int ws, int flags, hipStream_t st)
+{
+    if (nb <= 0) return hipSuccess;
+    switch (ws) {
+    case 1: return launch_ws_flags<1>(a, nb, flags, st);
+    case 2: return launch_ws_flags<2>(a, nb, flags, st);
+    case 3: return launch_ws_flags<3>(a, nb, flags, st);
+    case 4: return launch_ws_flags<4>(a, nb, flags, st);
+    case 5: return launch_ws_flags<5>(a, nb, flags, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_face_metric(const FaceArgs &a, int kind, int mode, double *g0_out, double *rurv0_out,
+                              double *partial, int partial_base, hipStream_t st)
 {
     if (a.nf == 0) return hipSuccess;
-    const int nb = face_blocks(a.nf, 1);
-#define UMCP_MT(NCPv, NPTv)                                                                      \
-    if (ncp == NCPv && npt == NPTv) {                                                            \
-        hipLaunchKernelGGL((face_metric<NCPv, NPTv>), dim3(nb), dim3(kFaceBlock), 0, st, a, mode, \
-                           g0_out, rurv0_out);                                                   \
-        return hipGetLastError();                                                                \
+    const int nb = metric_blocks(a.nf);
+    switch (kind) {
+    case 0:
+        hipLaunchKernelGGL((face_metric<12, 15, 0>), dim3(nb), dim3(kMetricBlock), 0, st, a, mode, g0_out,
+                           rurv0_out, partial, partial_base);
+        break;
+    case 1:
+        hipLaunchKernelGGL((face_metric<11, 16, 1>), dim3(nb), dim3(kMetricBlock), 0, st, a, mode, g0_out,
+                           rurv0_out, partial, partial_base);
+        break;
+    case 2:
+        hipLaunchKernelGGL((face_metric<13, 16, 2>), dim3(nb), dim3(kMetricBlock), 0, st, a, mode, g0_out,
+                           rurv0_out, partial, partial_base);
+        break;
+    default: return hipErrorInvalidValue;
     }
-    UMCP_MT(12, 15)
-    UMCP_MT(11, 16)
-    UMCP_MT(13, 16)
-#undef UMCP_MT
-    return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 hipError_t launch_vertex_gather(const GatherArgs &a, hipStream_t st)

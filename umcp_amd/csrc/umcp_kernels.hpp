@@ -8,8 +8,11 @@
 
 namespace umcp {
 
-constexpr int kFaceBlock = 256;
+constexpr int kWave = 64;
 constexpr int kGatherBlock = 256;
+constexpr int kMetricBlock = 256;
+constexpr int kMaxClasses = 4;
+constexpr int kMaxSplit = 5;
 
 struct PhysParams {
     double kc, kg, KA, k_reg, lipid_c0;
@@ -21,31 +24,35 @@ struct FaceArgs {
     const double *r;        // 3 nv + 3 (alpha last)
     int nv;
     int nf;
-    const int *cp;          // [ncp][nf]
-    const uint8_t *img;     // [ncp][nf]
+    const int *cp;          // [ncp][nf]   control-point vertex ids
+    const int *pos;         // [ncp][nf]   position of (face, slot) in the vertex CSR
+    const uint8_t *img;     // [ncp][nf]   periodic image codes
     const uint8_t *has_img; // [nf]
     const double *g0;       // [npt][nf]
     const double *rurv0;    // [npt][nf]
     const double *Lo, *Li;  // [nf] lipid areas of the face's triangle
     const double *c0;       // [nf]
-    const double *table;    // [npt][6][ncp]
-    const double *weight;   // [npt]
     double dudv;
     double pbc[3][3];
-    double *gslot;          // [3][nslots]
-    int slot_base;
-    int nslots;
-    double *partial;        // [npartial][4]
-    int partial_base;
+    double *gbuf;           // [nnz][3] per-(face, slot) gradient, CSR order
     PhysParams ph;
+};
+
+// All face classes of a surface in one launch: blocks [block_start[c],
+// block_start[c+1]) work on class c (kind: 0 = 12 control points / 15 Gauss
+// points, 1 = 11 / 16, 2 = 13 / 16).
+struct MultiFaceArgs {
+    FaceArgs cls[kMaxClasses];
+    int kind[kMaxClasses];
+    int block_start[kMaxClasses + 1];
+    int ncls;
+    double *partial;        // [nblocks][4]: energy, dE/dalpha
 };
 
 struct GatherArgs {
     int nv;
-    const int *vstart;
-    const int *vslot;
-    const double *gslot;
-    int nslots;
+    const int *vstart;      // nv + 1
+    const double *gbuf;
     const double *partial;
     int npartial;
     int out_mode;           // 0 gradient, 1 Brownian update, 2 reductions only
@@ -59,10 +66,20 @@ struct GatherArgs {
     double *sums;           // may be null: the four reduced partial sums
 };
 
-int face_blocks(int nf, int grp);
-hipError_t launch_face_energy_grad(const FaceArgs &a, int ncp, int npt, int grp, hipStream_t st);
-hipError_t launch_face_metric(const FaceArgs &a, int ncp, int npt, int mode, double *g0_out,
-                              double *rurv0_out, hipStream_t st);
+int kind_of(int ncp, int npt);
+// copies one kind's shared Gauss-point table ([npt][6][ncp]) and weights into
+// the device's constant memory (current device)
+hipError_t upload_tables(int kind, const double *table, const double *weights, int npt, int ncp);
+int face_groups(int nf);
+// launch shape: point_split waves share the Gauss points of 64 faces;
+// flags bit 0 keeps the faces' control points in LDS instead of registers,
+// bit 1 bounds registers for two waves per SIMD
+constexpr int kLaunchStageLds = 1;
+constexpr int kLaunchOcc2 = 2;
+hipError_t launch_faces(const MultiFaceArgs &a, int nblocks, int point_split, int flags, hipStream_t st);
+hipError_t launch_face_metric(const FaceArgs &a, int kind, int mode, double *g0_out,
+                              double *rurv0_out, double *partial, int partial_base, hipStream_t st);
+int metric_blocks(int nf);
 hipError_t launch_vertex_gather(const GatherArgs &a, hipStream_t st);
 
 }  // namespace umcp

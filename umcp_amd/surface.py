@@ -24,6 +24,8 @@ MODE_LIPID = 0
 MODE_HELFRICH = 1
 PREC_FP64 = 0
 PREC_MIXED = 1
+LAUNCH_STAGE_LDS = 1
+LAUNCH_OCC2 = 2
 
 # every symbol include/umcp.h declares
 EXPORTS = [
@@ -89,7 +91,7 @@ def load_library(path: str = LIB_PATH):
         "umcp_face_points": ([vp, C.c_int, ip, ip], C.c_int),
         "umcp_face_lipids": ([vp, C.c_int, dp, dp], C.c_int),
         "umcp_set_launch": ([vp, C.c_int, C.c_int], C.c_int),
-        "umcp_get_launch": ([vp, C.c_int], C.c_int),
+        "umcp_get_launch": ([vp, ip, ip], C.c_int),
         "umcp_profile_begin": ([vp], C.c_int),
         "umcp_profile_end": ([vp, C.c_int, dp, ip], C.c_int),
     }
@@ -246,21 +248,24 @@ class Surface:
                             table=tab.reshape(npt.value, 6, ncp.value), weights=w))
         return out
 
-    def set_launch(self, cls: int, lanes_per_face: int):
-        self._check(self._L.umcp_set_launch(self._h, cls, lanes_per_face), "set_launch")
+    def set_launch(self, point_split: int, flags: int = 0):
+        """point_split waves share 64 faces' Gauss points; flags: LAUNCH_*."""
+        self._check(self._L.umcp_set_launch(self._h, point_split, int(flags)), "set_launch")
 
-    def get_launch(self, cls: int) -> int:
-        return self._L.umcp_get_launch(self._h, cls)
+    def get_launch(self):
+        sp, fl = C.c_int(), C.c_int()
+        self._check(self._L.umcp_get_launch(self._h, C.byref(sp), C.byref(fl)), "get_launch")
+        return sp.value, fl.value
 
     def profile_begin(self):
         self._check(self._L.umcp_profile_begin(self._h), "profile_begin")
 
     def profile_end(self):
-        n = self._L.umcp_class_count(self._h)
-        ms = np.zeros(max(n, 1))
-        cnt = (C.c_int * max(n, 1))()
-        self._check(self._L.umcp_profile_end(self._h, n, _dp(ms), cnt), "profile_end")
-        return [(float(ms[c]), int(cnt[c])) for c in range(n)]
+        """(summed milliseconds, launches) of the face kernel since profile_begin."""
+        ms = np.zeros(1)
+        cnt = (C.c_int * 1)()
+        self._check(self._L.umcp_profile_end(self._h, 1, _dp(ms), cnt), "profile_end")
+        return float(ms[0]), int(cnt[0])
 
     def face_points(self, f: int):
         ncp = C.c_int()
