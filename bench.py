@@ -81,6 +81,12 @@ def main():
     ap.add_argument("--split", type=int, default=0, help="waves sharing a face group's Gauss points (0: library default)")
     ap.add_argument("--flags", type=int, default=-1, help="launch flags (1 LDS staging, 2 two-wave register bound)")
     ap.add_argument("--sweep", action="store_true", help="report face-kernel time per launch shape and exit")
+    ap.add_argument("--no-tune", action="store_true", help="keep the library's default launch shape")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="arithmetic of the per-face kernel for the headline number")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-mixed comparison leg")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs one mesh per GPU and has no "
@@ -130,8 +136,23 @@ def main():
                 print(json.dumps({"split": split, "flags": flags, "face_us": 1e3 * tot / max(n, 1)}),
                       flush=True)
         return
+    tuned = None
+    if not args.no_tune and not (args.split or args.flags >= 0):
+        from umcp_amd import tune
+        d_E = torch.zeros(1, device=dev, dtype=torch.float64)
+        d_g = torch.zeros_like(d_r)
+        best, times = tune.tune(s, d_r, d_E, d_g, stream=sptr)
+        tuned = {"best": list(best), "us": {"%d/%d" % k: round(v, 2) for k, v in times.items()}}
     s.bd_steps_dev(d_r, d_xi[:W], W, mob, amp, d_en[:W], stream=sptr)
     torch.cuda.synchronize()
+    if args.calib_copy_mib:
+        n = args.calib_copy_mib * (1 << 20) // 8
+        src = torch.ones(n, device=dev, dtype=torch.float64)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -173,6 +194,7 @@ def main():
                    "energy_form": "helfrich per-face c0", "precision": "fp64",
                    "D_m2_per_s": 1e-12, "dt_s": 1e-9, "point_split": split, "launch_flags": flags},
         "bd_steps_per_s": K / secs,
+        "autotune": tuned,
         "face_kernel_ms_per_step": face_ms / face_n,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,

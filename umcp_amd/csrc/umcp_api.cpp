@@ -46,13 +46,14 @@ struct umcp_surface {
     std::vector<void *> allocations;
     // work buffers
     double *d_r = nullptr, *d_g = nullptr, *d_gbuf = nullptr, *d_partial = nullptr;
+    float *d_gbuf32 = nullptr;
     double *d_scalar = nullptr;    // [8]: E, sums[4], ...
     int *d_vstart = nullptr;
     int npartial = 0;
     // launch shape of the face kernel: waves sharing 64 faces' Gauss points,
     // and whether the faces' control points are staged in LDS
-    int point_split = 3;
-    int launch_flags = 0;
+    int point_split = 4;
+    int launch_flags = kLaunchStageLds | kLaunchOcc2;
     std::vector<int> launch_order;  // classes in launch order (irregular first)
     std::vector<int> block_start;
     double *d_xi = nullptr;        // host-array BD staging, chunk of steps
@@ -140,6 +141,7 @@ FaceArgs face_args(umcp_surface *s, size_t c, const double *d_r)
     a.dudv = C.dudv;
     std::memcpy(a.pbc, s->T.pbc, sizeof a.pbc);
     a.gbuf = s->d_gbuf;
+    a.gbuf32 = s->d_gbuf32;
     a.ph = phys(s);
     return a;
 }
@@ -194,12 +196,14 @@ int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs 
         if (rc) return rc;
         (void)hipEventRecord(ev[0], st);
     }
-    hipError_t e = launch_faces(m, s->npartial, s->point_split, s->launch_flags, st);
+    const bool mixed = s->prm.precision == UMCP_PREC_MIXED;
+    hipError_t e = launch_faces(m, s->npartial, s->point_split, s->launch_flags, mixed, st);
     if (e != hipSuccess) return hip_fail(s, e, "face kernel launch");
     if (s->prof_on) (void)hipEventRecord(ev[1], st);
     ga.nv = s->T.nv;
     ga.vstart = s->d_vstart;
     ga.gbuf = s->d_gbuf;
+    ga.gbuf32 = mixed ? s->d_gbuf32 : nullptr;
     ga.partial = s->d_partial;
     ga.npartial = s->npartial;
     e = launch_vertex_gather(ga, st);
@@ -303,8 +307,6 @@ int umcp_set_params(umcp_surface *s, const umcp_params *p)
         return fail(s, UMCP_ERR_ARG, "unknown energy mode");
     if (p->precision != UMCP_PREC_FP64 && p->precision != UMCP_PREC_MIXED)
         return fail(s, UMCP_ERR_ARG, "unknown precision");
-    if (p->precision == UMCP_PREC_MIXED)
-        return fail(s, UMCP_ERR_ARG, "mixed precision is not available in this build");
     if (!(p->lipid_apl > 0)) return fail(s, UMCP_ERR_ARG, "lipid area must be positive");
     bool repack = s->g0_set && p->lipid_apl != s->prm.lipid_apl;
     s->prm = *p;
@@ -370,7 +372,7 @@ int umcp_plan(umcp_surface *s)
     for (size_t c = 0; c < s->P.classes.size(); c++) metric_total += metric_blocks(s->P.classes[c].nf());
     const int max_partial = std::max(s->npartial, metric_total);
     if ((rc = dev_alloc(s, &s->d_r, 3 * (size_t)nv + 3)) || (rc = dev_alloc(s, &s->d_g, 3 * (size_t)nv + 3)) ||
-        (rc = dev_alloc(s, &s->d_gbuf, 3 * (size_t)s->P.nnz)) ||
+        (rc = dev_alloc(s, &s->d_gbuf, 3 * (size_t)s->P.nnz)) || (rc = dev_alloc(s, &s->d_gbuf32, 3 * (size_t)s->P.nnz)) ||
         (rc = dev_alloc(s, &s->d_partial, 4 * (size_t)max_partial)) || (rc = dev_alloc(s, &s->d_scalar, 8)) ||
         (rc = dev_alloc(s, &s->d_vstart, s->P.vstart.size())))
         return rc;

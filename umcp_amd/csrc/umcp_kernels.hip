@@ -29,20 +29,32 @@ constexpr int kTabReg = 15 * 5 * 12, kTabV5 = 16 * 5 * 11, kTabV7 = 16 * 5 * 13;
 __constant__ double c_tab_reg[kTabReg + 15];
 __constant__ double c_tab_v5[kTabV5 + 16];
 __constant__ double c_tab_v7[kTabV7 + 16];
-// second copies read by the gradient (backward) contraction only
 __constant__ double c_tab_reg_b[kTabReg];
 __constant__ double c_tab_v5_b[kTabV5];
 __constant__ double c_tab_v7_b[kTabV7];
+// FP32 copies for the mixed path
+__constant__ float c_tab32_reg[kTabReg + 15];
+__constant__ float c_tab32_v5[kTabV5 + 16];
+__constant__ float c_tab32_v7[kTabV7 + 16];
+__constant__ float c_tab32_reg_b[kTabReg];
+__constant__ float c_tab32_v5_b[kTabV5];
+__constant__ float c_tab32_v7_b[kTabV7];
 
-template <int KIND>
-__device__ __forceinline__ const double *kind_table()
+template <int KIND, typename R>
+__device__ __forceinline__ const R *kind_table()
 {
-    return KIND == 0 ? c_tab_reg : (KIND == 1 ? c_tab_v5 : c_tab_v7);
+    if constexpr (sizeof(R) == 8)
+        return (const R *)(KIND == 0 ? c_tab_reg : (KIND == 1 ? c_tab_v5 : c_tab_v7));
+    else
+        return (const R *)(KIND == 0 ? c_tab32_reg : (KIND == 1 ? c_tab32_v5 : c_tab32_v7));
 }
-template <int KIND>
-__device__ __forceinline__ const double *kind_table_b()
+template <int KIND, typename R>
+__device__ __forceinline__ const R *kind_table_b()
 {
-    return KIND == 0 ? c_tab_reg_b : (KIND == 1 ? c_tab_v5_b : c_tab_v7_b);
+    if constexpr (sizeof(R) == 8)
+        return (const R *)(KIND == 0 ? c_tab_reg_b : (KIND == 1 ? c_tab_v5_b : c_tab_v7_b));
+    else
+        return (const R *)(KIND == 0 ? c_tab32_reg_b : (KIND == 1 ? c_tab32_v5_b : c_tab32_v7_b));
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs; give every XCD a
@@ -177,17 +189,26 @@ __device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R rur
     return e;
 }
 
-// Fused energy + gradient of 64 faces by one workgroup of WS waves, FP64.
+// Fused energy + gradient of 64 faces by one workgroup of WS waves.
 // Lane l of every wave owns face group*64 + l; wave w evaluates the Gauss
 // points p = w, w + WS, ... so the table row of each point is wave-uniform
 // (scalar loads feed the FMAs as SGPR operands).  The waves' partial
 // gradients are summed in LDS in a fixed order, then wave 0 writes each
 // (face, control point) gradient into that vertex's CSR slot.
-template <int NCP, int NPT, int KIND, int WS, bool PLDS>
-__device__ __forceinline__ void face_group_f64(const FaceArgs &a, int group, double *lds, double acc[4])
+//
+// R = double: the FP64 path, on absolute coordinates.
+// R = float:  the mixed path.  Control points are formed relative to the
+// face's first control point in FP64 and rounded to FP32 (the derivative
+// rows of every table sum to zero, so the contraction is translation
+// invariant); contraction, curvature and chain rule run in FP32; the energy
+// and dE/dalpha accumulate in FP64; the gradient leaves as FP32 and is summed
+// per vertex in FP64 by the gather.
+template <typename R, int NCP, int NPT, int KIND, int WS, bool PLDS>
+__device__ __forceinline__ void face_group(const FaceArgs &a, int group, R *lds, double acc[4])
 {
-    const double *tab = kind_table<KIND>();
-    const double *wts = tab + NPT * 5 * NCP;
+    constexpr bool kRel = sizeof(R) == 4;
+    const R *tab = kind_table<KIND, R>();
+    const R *wts = tab + NPT * 5 * NCP;
     const int lane = threadIdx.x & (kWave - 1);
     // wave index made provably wave-uniform so every table address below is
     // too (the compiler then reads the tables with scalar loads)
@@ -196,10 +217,10 @@ __device__ __forceinline__ void face_group_f64(const FaceArgs &a, int group, dou
     const int f = group * kWave + lane;
     const bool live = f < nf;
     const int fc = live ? f : nf - 1;
-    const double al[3] = {a.r[3 * a.nv], a.r[3 * a.nv + 1], a.r[3 * a.nv + 2]};
+    const double al64[3] = {a.r[3 * a.nv], a.r[3 * a.nv + 1], a.r[3 * a.nv + 2]};
+    const R al[3] = {(R)al64[0], (R)al64[1], (R)al64[2]};
     const bool shifted = a.has_img[fc] != 0;
 
-    double P[PLDS ? 1 : NCP][3];
     auto load_cp = [&](int c, double out[3]) {
         const int v = a.cp[c * nf + fc];
         out[0] = a.r[3 * v];
@@ -213,54 +234,62 @@ __device__ __forceinline__ void face_group_f64(const FaceArgs &a, int group, dou
             out[2] += o[2];
         }
     };
+    double base[3] = {0.0, 0.0, 0.0};
+    if (kRel) load_cp(0, base);
+    R P[PLDS ? 1 : NCP][3];
     if (PLDS) {
         for (int c = w; c < NCP; c += WS) {
             double t[3];
             load_cp(c, t);
 #pragma unroll
-            for (int k = 0; k < 3; k++) lds[(c * 3 + k) * kWave + lane] = t[k];
+            for (int k = 0; k < 3; k++) lds[(c * 3 + k) * kWave + lane] = (R)(t[k] - base[k]);
         }
         __syncthreads();
     } else {
 #pragma unroll
-        for (int c = 0; c < NCP; c++) load_cp(c, P[c]);
+        for (int c = 0; c < NCP; c++) {
+            double t[3];
+            load_cp(c, t);
+#pragma unroll
+            for (int k = 0; k < 3; k++) P[c][k] = (R)(t[k] - base[k]);
+        }
     }
 #define PV(c, k) (PLDS ? lds[((c) * 3 + (k)) * kWave + lane] : P[PLDS ? 0 : (c)][k])
 
-    double G[NCP][3];
+    R G[NCP][3];
 #pragma unroll
-    for (int c = 0; c < NCP; c++) G[c][0] = G[c][1] = G[c][2] = 0.0;
-    const double Lo = a.Lo[fc], Li = a.Li[fc], c0f = a.c0[fc];
+    for (int c = 0; c < NCP; c++) G[c][0] = G[c][1] = G[c][2] = R(0);
+    const R Lo = (R)a.Lo[fc], Li = (R)a.Li[fc], c0f = (R)a.c0[fc];
     double e = 0.0;
     for (int p = w; p < NPT; p += WS) {
-        const double *T = tab + p * 5 * NCP;
-        const double *TB = kind_table_b<KIND>() + p * 5 * NCP;
-        double X[5][3];
+        const R *T = tab + p * 5 * NCP;
+        const R *TB = kind_table_b<KIND, R>() + p * 5 * NCP;
+        R X[5][3];
 #pragma unroll
         for (int d = 0; d < 5; d++)
 #pragma unroll
-            for (int k = 0; k < 3; k++) X[d][k] = 0.0;
-#pragma unroll
-        for (int c = 0; c < NCP; c++) {
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    const double x = PV(c, k);
-#pragma unroll
-                    for (int d = 0; d < 5; d++) X[d][k] += T[c * 5 + d] * x;
-                }
-            }
-#pragma unroll
-        for (int d = 0; d < 5; d++)
-#pragma unroll
-            for (int k = 0; k < 3; k++) X[d][k] *= al[k];
-        double D[5][3];
-        e += point_energy_grad<double>(X, wts[p], a.g0[p * nf + fc], a.rurv0[p * nf + fc], Lo, Li,
-                                       c0f, a.dudv, a.ph, D);
+            for (int k = 0; k < 3; k++) X[d][k] = R(0);
 #pragma unroll
         for (int c = 0; c < NCP; c++) {
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                double s = G[c][k];
+                const R x = PV(c, k);
+#pragma unroll
+                for (int d = 0; d < 5; d++) X[d][k] += T[c * 5 + d] * x;
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 5; d++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) X[d][k] *= al[k];
+        R D[5][3];
+        e += (double)point_energy_grad<R>(X, wts[p], (R)a.g0[p * nf + fc], (R)a.rurv0[p * nf + fc], Lo, Li,
+                                          c0f, (R)a.dudv, a.ph, D);
+#pragma unroll
+        for (int c = 0; c < NCP; c++) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                R s = G[c][k];
 #pragma unroll
                 for (int d = 0; d < 5; d++) s += TB[c * 5 + d] * D[d][k];
                 G[c][k] = s;
@@ -270,11 +299,17 @@ __device__ __forceinline__ void face_group_f64(const FaceArgs &a, int group, dou
     if (live) {
         acc[0] += e;
         // dE/dalpha_k = sum_c dE/dR_ck (r + shift)_ck is linear in G: each
-        // wave adds its own points' share
+        // wave adds its own points' share (in FP64)
 #pragma unroll
-        for (int c = 0; c < NCP; c++)
+        for (int k = 0; k < 3; k++) {
+            double sg = 0.0, sgp = 0.0;
 #pragma unroll
-            for (int k = 0; k < 3; k++) acc[1 + k] += G[c][k] * PV(c, k);
+            for (int c = 0; c < NCP; c++) {
+                sg += (double)G[c][k];
+                sgp += (double)G[c][k] * (double)PV(c, k);
+            }
+            acc[1 + k] += sgp + sg * base[k];
+        }
     }
 #undef PV
     if (WS > 1) {
@@ -296,9 +331,10 @@ __device__ __forceinline__ void face_group_f64(const FaceArgs &a, int group, dou
         }
     }
     if (w == 0 && live) {
+        R *gb = (R *)(kRel ? (void *)a.gbuf32 : (void *)a.gbuf);
 #pragma unroll
         for (int c = 0; c < NCP; c++) {
-            double *dst = a.gbuf + 3 * (size_t)a.pos[c * nf + f];
+            R *dst = gb + 3 * (size_t)a.pos[c * nf + f];
             dst[0] = G[c][0] * al[0];
             dst[1] = G[c][1] * al[1];
             dst[2] = G[c][2] * al[2];
@@ -306,19 +342,19 @@ __device__ __forceinline__ void face_group_f64(const FaceArgs &a, int group, dou
     }
 }
 
-template <int WS, bool PLDS, int OCC>
-__global__ __launch_bounds__(kWave *WS, OCC) void faces_f64(MultiFaceArgs m)
+template <typename R, int WS, bool PLDS, int OCC>
+__global__ __launch_bounds__(kWave *WS, OCC) void faces_kernel(MultiFaceArgs m)
 {
-    __shared__ double lds[13 * 3 * kWave];
+    __shared__ R lds[13 * 3 * kWave];
     const int b = xcd_logical_block(blockIdx.x, gridDim.x);
     int c = 0;
     while (c + 1 < m.ncls && b >= m.block_start[c + 1]) c++;
     const int group = b - m.block_start[c];
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     switch (m.kind[c]) {
-    case 0: face_group_f64<12, 15, 0, WS, PLDS>(m.cls[c], group, lds, acc); break;
-    case 1: face_group_f64<11, 16, 1, WS, PLDS>(m.cls[c], group, lds, acc); break;
-    default: face_group_f64<13, 16, 2, WS, PLDS>(m.cls[c], group, lds, acc); break;
+    case 0: face_group<R, 12, 15, 0, WS, PLDS>(m.cls[c], group, lds, acc); break;
+    case 1: face_group<R, 11, 16, 1, WS, PLDS>(m.cls[c], group, lds, acc); break;
+    default: face_group<R, 13, 16, 2, WS, PLDS>(m.cls[c], group, lds, acc); break;
     }
     block_sum4<kWave * WS>(acc);
     if (threadIdx.x == 0) {
@@ -352,8 +388,8 @@ __global__ __launch_bounds__(kMetricBlock) void face_metric(FaceArgs a, int mode
             for (int k = 0; k < 3; k++) P[c][k] = a.r[3 * v + k] + o[k];
         }
         for (int p = 0; p < NPT; p++) {
-            const double *T = kind_table<KIND>() + p * 5 * NCP;
-            const double wp = kind_table<KIND>()[NPT * 5 * NCP + p];
+            const double *T = kind_table<KIND, double>() + p * 5 * NCP;
+            const double wp = kind_table<KIND, double>()[NPT * 5 * NCP + p];
             double Ru[3], Rv[3];
 #pragma unroll
             for (int k = 0; k < 3; k++) {
@@ -394,6 +430,7 @@ __global__ __launch_bounds__(kMetricBlock) void face_metric(FaceArgs a, int mode
 //   out_mode 0: write g (3 nv) and dE/dalpha into g[3nv..3nv+2]
 //   out_mode 1: Brownian update of r, no g
 //   out_mode 2: reductions only
+template <typename GB>
 __global__ __launch_bounds__(kGatherBlock) void vertex_gather(GatherArgs a)
 {
     if (blockIdx.x == 0) {
@@ -421,11 +458,11 @@ __global__ __launch_bounds__(kGatherBlock) void vertex_gather(GatherArgs a)
     if (v >= a.nv) return;
     const int s0 = a.vstart[v], s1 = a.vstart[v + 1];
     double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-    const double *src = a.gbuf + 3 * (size_t)s0;
+    const GB *src = (const GB *)(sizeof(GB) == 4 ? (const void *)a.gbuf32 : (const void *)a.gbuf) + 3 * (size_t)s0;
     for (int s = s0; s < s1; s++, src += 3) {
-        g0 += src[0];
-        g1 += src[1];
-        g2 += src[2];
+        g0 += (double)src[0];
+        g1 += (double)src[1];
+        g2 += (double)src[2];
     }
     if (a.out_mode == 0) {
         a.g[3 * v] = g0;
@@ -470,6 +507,15 @@ hipError_t upload_tables(int kind, const double *table, const double *weights, i
                                 : (kind == 1 ? (const void *)HIP_SYMBOL(c_tab_v5_b) : (const void *)HIP_SYMBOL(c_tab_v7_b));
     hipError_t e = hipMemcpyToSymbol(fwd, buf, (n + npt) * sizeof(double), 0, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpyToSymbol(bwd, buf, n * sizeof(double), 0, hipMemcpyHostToDevice);
+    float *b32 = new float[n + npt];
+    for (size_t i = 0; i < n + npt; i++) b32[i] = (float)buf[i];
+    const void *fwd32 = kind == 0 ? (const void *)HIP_SYMBOL(c_tab32_reg)
+                                  : (kind == 1 ? (const void *)HIP_SYMBOL(c_tab32_v5) : (const void *)HIP_SYMBOL(c_tab32_v7));
+    const void *bwd32 = kind == 0 ? (const void *)HIP_SYMBOL(c_tab32_reg_b)
+                                  : (kind == 1 ? (const void *)HIP_SYMBOL(c_tab32_v5_b) : (const void *)HIP_SYMBOL(c_tab32_v7_b));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(fwd32, b32, (n + npt) * sizeof(float), 0, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpyToSymbol(bwd32, b32, n * sizeof(float), 0, hipMemcpyHostToDevice);
+    delete[] b32;
     delete[] buf;
     return e;
 }
@@ -477,35 +523,41 @@ hipError_t upload_tables(int kind, const double *table, const double *weights, i
 int face_groups(int nf) { return (nf + kWave - 1) / kWave; }
 int metric_blocks(int nf) { return (nf + kMetricBlock - 1) / kMetricBlock; }
 
-template <int WS, bool PLDS, int OCC>
+template <typename R, int WS, bool PLDS, int OCC>
 static hipError_t launch_ws(const MultiFaceArgs &a, int nb, hipStream_t st)
 {
-    hipLaunchKernelGGL((faces_f64<WS, PLDS, OCC>), dim3(nb), dim3(kWave * WS), 0, st, a);
+    hipLaunchKernelGGL((faces_kernel<R, WS, PLDS, OCC>), dim3(nb), dim3(kWave * WS), 0, st, a);
     return hipGetLastError();
 }
 
-template <int WS>
+template <typename R, int WS>
 static hipError_t launch_ws_flags(const MultiFaceArgs &a, int nb, int flags, hipStream_t st)
 {
     switch (flags & 3) {
-    case 0: return launch_ws<WS, false, 1>(a, nb, st);
-    case 1: return launch_ws<WS, true, 1>(a, nb, st);
-    case 2: return launch_ws<WS, false, 2>(a, nb, st);
-    default: return launch_ws<WS, true, 2>(a, nb, st);
+    case 0: return launch_ws<R, WS, false, 1>(a, nb, st);
+    case 1: return launch_ws<R, WS, true, 1>(a, nb, st);
+    case 2: return launch_ws<R, WS, false, 2>(a, nb, st);
+    default: return launch_ws<R, WS, true, 2>(a, nb, st);
     }
 }
 
-hipError_t launch_faces(const MultiFaceArgs &a, int nb, int ws, int flags, hipStream_t st)
+template <typename R>
+static hipError_t launch_prec(const MultiFaceArgs &a, int nb, int ws, int flags, hipStream_t st)
 {
-    if (nb <= 0) return hipSuccess;
     switch (ws) {
-    case 1: return launch_ws_flags<1>(a, nb, flags, st);
-    case 2: return launch_ws_flags<2>(a, nb, flags, st);
-    case 3: return launch_ws_flags<3>(a, nb, flags, st);
-    case 4: return launch_ws_flags<4>(a, nb, flags, st);
-    case 5: return launch_ws_flags<5>(a, nb, flags, st);
+    case 1: return launch_ws_flags<R, 1>(a, nb, flags, st);
+    case 2: return launch_ws_flags<R, 2>(a, nb, flags, st);
+    case 3: return launch_ws_flags<R, 3>(a, nb, flags, st);
+    case 4: return launch_ws_flags<R, 4>(a, nb, flags, st);
+    case 5: return launch_ws_flags<R, 5>(a, nb, flags, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_faces(const MultiFaceArgs &a, int nb, int ws, int flags, bool mixed, hipStream_t st)
+{
+    if (nb <= 0) return hipSuccess;
+    return mixed ? launch_prec<float>(a, nb, ws, flags, st) : launch_prec<double>(a, nb, ws, flags, st);
 }
 
 hipError_t launch_face_metric(const FaceArgs &a, int kind, int mode, double *g0_out, double *rurv0_out,
@@ -535,7 +587,10 @@ hipError_t launch_vertex_gather(const GatherArgs &a, hipStream_t st)
 {
     int nb = (a.out_mode == 2) ? 1 : (a.nv + kGatherBlock - 1) / kGatherBlock;
     if (nb < 1) nb = 1;
-    hipLaunchKernelGGL(vertex_gather, dim3(nb), dim3(kGatherBlock), 0, st, a);
+    if (a.gbuf32)
+        hipLaunchKernelGGL(vertex_gather<float>, dim3(nb), dim3(kGatherBlock), 0, st, a);
+    else
+        hipLaunchKernelGGL(vertex_gather<double>, dim3(nb), dim3(kGatherBlock), 0, st, a);
     return hipGetLastError();
 }
 

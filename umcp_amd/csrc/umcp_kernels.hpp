@@ -34,7 +34,8 @@ struct FaceArgs {
     const double *c0;       // [nf]
     double dudv;
     double pbc[3][3];
-    double *gbuf;           // [nnz][3] per-(face, slot) gradient, CSR order
+    double *gbuf;           // [nnz][3] per-(face, slot) gradient, CSR order (FP64 path)
+    float *gbuf32;          // [nnz][3] the same for the mixed path
     PhysParams ph;
 };
 
@@ -52,7 +53,8 @@ struct MultiFaceArgs {
 struct GatherArgs {
     int nv;
     const int *vstart;      // nv + 1
-    const double *gbuf;
+    const double *gbuf;     // FP64 contributions, or
+    const float *gbuf32;    // FP32 contributions (mixed path) when non-null
     const double *partial;
     int npartial;
     int out_mode;           // 0 gradient, 1 Brownian update, 2 reductions only
@@ -76,7 +78,8 @@ int face_groups(int nf);
 // bit 1 bounds registers for two waves per SIMD
 constexpr int kLaunchStageLds = 1;
 constexpr int kLaunchOcc2 = 2;
-hipError_t launch_faces(const MultiFaceArgs &a, int nblocks, int point_split, int flags, hipStream_t st);
+hipError_t launch_faces(const MultiFaceArgs &a, int nblocks, int point_split, int flags, bool mixed,
+                        hipStream_t st);
 hipError_t launch_face_metric(const FaceArgs &a, int kind, int mode, double *g0_out,
                               double *rurv0_out, double *partial, int partial_base, hipStream_t st);
 int metric_blocks(int nf);
