@@ -4,12 +4,14 @@ One step = one pass of the hot path over the resident mesh: the fused
 limit-surface energy + gradient of all 81920 faces (81860 regular, 60
 irregular) and the per-vertex Brownian update, driven by a device-resident
 standard-normal array (seeded).  Timed with HIP events on the stream the work
-is launched on; K steps between two barriers + device synchronisations.
+is launched on; K steps between two device synchronisations.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1]
 
-Prints one JSON line (metric, value = face-evals/s, roofline of the dominant
-kernel, the CPU oracle timed on a bounded sample of the same workload).
+Prints one JSON line: value = FP64 face-evals/s (the plain path), the
+roofline of the dominant kernel, the FP32-mixed leg (throughput, speedup and
+its energy / gradient error against the FP64 oracle) and the CPU oracle timed
+on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -43,12 +45,12 @@ def face_bytes(ncp: int, npt: int, nv: int, nf: int) -> float:
 
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
-PEAK_HBM_GBS = 8000.0
 
 
-def cpu_baseline(w, c0, budget_s: float = 12.0):
+def cpu_baseline(w, c0, budget_s: float):
     """The oracle (serial C, one core) on energy+gradient evaluations of the
-    same vesicle until ~budget_s of CPU work."""
+    same vesicle until ~budget_s of CPU work; also returns its E and gradient
+    at the start configuration for the error report."""
     import numpy as np
     from oracle import oracle as O
     o = O.OracleSurface(w.mesh_path)
@@ -58,17 +60,84 @@ def cpu_baseline(w, c0, budget_s: float = 12.0):
     t0 = time.perf_counter()
     n = 0
     g = np.zeros_like(w.r)
+    E = g_ref = None
     while True:
-        o.energy(w.r)
+        E = o.energy(w.r)
         g[:] = 0.0
         o.grad(w.r, g)
+        if g_ref is None:
+            g_ref = g.copy()
         n += 1
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
-    return {"value": n * o.nfaces / dt, "unit": "face-evals/s", "cores": 1, "kind": "port",
+    base = {"value": n * o.nfaces / dt, "unit": "face-evals/s", "cores": 1, "kind": "port",
             "sample": "%d energy+gradient evaluations of the 40962-vertex vesicle (%d faces), "
                       "serial oracle, %.1f s" % (n, o.nfaces, dt)}
+    return base, E, g_ref
+
+
+def run_leg(args, w, c0, precision, d_xi, dev, torch):
+    """Sets up one surface, tunes the launch shape, runs W + K Brownian steps."""
+    import numpy as np
+    from umcp_amd import surface as S
+    from umcp_amd import tune
+
+    s = S.Surface(w.mesh_path)
+    s.set_params(precision=S.PREC_MIXED if precision == "mixed" else S.PREC_FP64, **w.params)
+    s.generatePlan()
+    s.set_face_c0(c0)
+    s.setg0(w.r0)
+    classes = s.classes()
+    nv, nf = s.nv, s.nfaces
+    mob, amp = S.bd_coefficients(1e-12, 1e-9, w.T)
+    K, W = args.steps, args.warmup
+    stream = torch.cuda.Stream(device=dev)          # default priority
+    sptr = stream.cuda_stream
+    d_r = torch.tensor(w.r, device=dev, dtype=torch.float64)
+    d_E = torch.zeros(1, device=dev, dtype=torch.float64)
+    d_g = torch.zeros_like(d_r)
+    d_en = torch.zeros(W + K, device=dev, dtype=torch.float64)
+    tuned = None
+    if args.split or args.flags >= 0:
+        sp0, fl0 = s.get_launch()
+        s.set_launch(args.split or sp0, args.flags if args.flags >= 0 else fl0)
+    elif not args.no_tune:
+        best, times = tune.tune(s, d_r, d_E, d_g, stream=sptr)
+        tuned = {"best": list(best), "us": {"%d/%d" % k: round(v, 2) for k, v in times.items()}}
+    # energy and gradient at the start configuration (for the error report)
+    torch.cuda.synchronize()
+    s.energy_grad_dev(d_r, d_E, d_g, stream=sptr)
+    torch.cuda.synchronize()
+    E0, g0 = float(d_E.cpu()[0]), d_g.cpu().numpy()
+
+    s.bd_steps_dev(d_r, d_xi[:W], W, mob, amp, d_en[:W], stream=sptr)
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.profile_begin()
+    start.record(stream)
+    s.bd_steps_dev(d_r, d_xi[W:W + K], K, mob, amp, d_en[W:], stream=sptr)
+    end.record(stream)
+    torch.cuda.synchronize()
+    face_ms, face_n = s.profile_end()
+    ms = start.elapsed_time(end)
+    energies = d_en.cpu().numpy()
+    if not np.all(np.isfinite(energies)):
+        raise SystemExit("non-finite energy during the timed steps (%s)" % precision)
+    split, flags = s.get_launch()
+    leg = {
+        "face_evals_per_s": nf * K / (ms / 1e3),
+        "bd_steps_per_s": K / (ms / 1e3),
+        "ms_per_step": ms / K,
+        "face_kernel_us": 1e3 * face_ms / face_n,
+        "launch": {"point_split": split, "flags": flags},
+        "autotune": tuned,
+        "flop_per_launch": sum(c["nf"] * face_flops(c["ncp"], c["npt"]) for c in classes),
+        "bytes_per_launch": sum(c["nf"] * face_bytes(c["ncp"], c["npt"], nv, nf) for c in classes),
+        "energy_first_last": [float(energies[W]), float(energies[-1])],
+    }
+    s.close()
+    return leg, E0, g0
 
 
 def main():
@@ -78,13 +147,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--split", type=int, default=0, help="waves sharing a face group's Gauss points (0: library default)")
-    ap.add_argument("--flags", type=int, default=-1, help="launch flags (1 LDS staging, 2 two-wave register bound)")
-    ap.add_argument("--sweep", action="store_true", help="report face-kernel time per launch shape and exit")
+    ap.add_argument("--split", type=int, default=0, help="fix the point split (skip tuning)")
+    ap.add_argument("--flags", type=int, default=-1, help="fix the launch flags (skip tuning)")
     ap.add_argument("--no-tune", action="store_true", help="keep the library's default launch shape")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
-                    help="arithmetic of the per-face kernel for the headline number")
-    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-mixed comparison leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-mixed leg")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -96,55 +162,16 @@ def main():
     import numpy as np
     import torch  # device memory and streams; its HIP runtime is shared with the library
 
-    from umcp_amd import surface as S
     from umcp_amd import workloads
 
     dev = torch.device("cuda:0")
     w = workloads.vesicle_40962()
     c0 = workloads.face_c0_uniform(81920, w.extra["c0_seed"])
-    s = S.Surface(w.mesh_path)
-    s.set_params(**w.params)
-    s.generatePlan()
-    s.set_face_c0(c0)
-    s.setg0(w.r0)
-    classes = s.classes()
-    nv, nf = s.nv, s.nfaces
-    mob, amp = S.bd_coefficients(1e-12, 1e-9, w.T)
-
     K, W = args.steps, args.warmup
     gen = torch.Generator(device=dev)
     gen.manual_seed(12345)
-    d_xi = torch.randn((W + K, 3 * nv), generator=gen, device=dev, dtype=torch.float64)
-    d_r = torch.tensor(w.r, device=dev, dtype=torch.float64)
-    d_en = torch.zeros(W + K, device=dev, dtype=torch.float64)
-    # a stream of our own at default priority; the library enqueues on it
-    stream = torch.cuda.Stream(device=dev)
-    sptr = stream.cuda_stream
-    if args.split or args.flags >= 0:
-        sp0, fl0 = s.get_launch()
-        s.set_launch(args.split or sp0, args.flags if args.flags >= 0 else fl0)
-    if args.sweep:
-        for split in (1, 2, 3, 4, 5):
-            for flags in (0, 1, 2, 3):
-                s.set_launch(split, flags)
-                s.bd_steps_dev(d_r, d_xi[:W], W, mob, amp, d_en[:W], stream=sptr)
-                torch.cuda.synchronize()
-                s.profile_begin()
-                s.bd_steps_dev(d_r, d_xi[:W], W, mob, amp, d_en[:W], stream=sptr)
-                torch.cuda.synchronize()
-                tot, n = s.profile_end()
-                print(json.dumps({"split": split, "flags": flags, "face_us": 1e3 * tot / max(n, 1)}),
-                      flush=True)
-        return
-    tuned = None
-    if not args.no_tune and not (args.split or args.flags >= 0):
-        from umcp_amd import tune
-        d_E = torch.zeros(1, device=dev, dtype=torch.float64)
-        d_g = torch.zeros_like(d_r)
-        best, times = tune.tune(s, d_r, d_E, d_g, stream=sptr)
-        tuned = {"best": list(best), "us": {"%d/%d" % k: round(v, 2) for k, v in times.items()}}
-    s.bd_steps_dev(d_r, d_xi[:W], W, mob, amp, d_en[:W], stream=sptr)
-    torch.cuda.synchronize()
+    d_xi = torch.randn((W + K, 3 * w.nv), generator=gen, device=dev, dtype=torch.float64)
+
     if args.calib_copy_mib:
         n = args.calib_copy_mib * (1 << 20) // 8
         src = torch.ones(n, device=dev, dtype=torch.float64)
@@ -154,58 +181,53 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    s.profile_begin()
-    start.record(stream)
-    s.bd_steps_dev(d_r, d_xi[W:], K, mob, amp, d_en[W:], stream=sptr)
-    end.record(stream)
-    torch.cuda.synchronize()
-    prof = s.profile_end()
-    ms = start.elapsed_time(end)
-    energies = d_en.cpu().numpy()
-    if not np.all(np.isfinite(energies)):
-        raise SystemExit("non-finite energy during the timed steps")
-
-    secs = ms / 1e3
-    value = nf * K / secs
-    # dominant kernel: the fused face kernel (all face classes in one launch)
-    reg = classes[0]
-    face_ms, face_n = prof
-    face_avg_s = face_ms / face_n / 1e3
-    flops = sum(c["nf"] * face_flops(c["ncp"], c["npt"]) for c in classes)
-    achieved = flops / face_avg_s / 1e12
-    split, flags = s.get_launch()
+    leg64, E64, g64 = run_leg(args, w, c0, "fp64", d_xi, dev, torch)
+    face_s = leg64["face_kernel_us"] / 1e6
+    achieved = leg64["flop_per_launch"] / face_s / 1e12
+    fl = leg64["launch"]["flags"]
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": leg64["face_evals_per_s"],
         "unit": "face-evals/s",
         "n_gpus": 1,
         "steps": K,
         "warmup": W,
-        "ms_per_step": ms / K,
+        "ms_per_step": leg64["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded icosphere, seeded device-resident N(0,1) noise)",
-        "config": {"workload": "vesicle_40962_bd", "nv": nv, "faces": nf,
-                   "faces_regular": reg["nf"], "faces_irregular": nf - reg["nf"],
+        "config": {"workload": "vesicle_40962_bd", "nv": w.nv, "faces": 81920,
+                   "faces_regular": 81860, "faces_irregular": 60,
                    "energy_form": "helfrich per-face c0", "precision": "fp64",
-                   "D_m2_per_s": 1e-12, "dt_s": 1e-9, "point_split": split, "launch_flags": flags},
-        "bd_steps_per_s": K / secs,
-        "autotune": tuned,
-        "face_kernel_ms_per_step": face_ms / face_n,
+                   "D_m2_per_s": 1e-12, "dt_s": 1e-9},
+        "bd_steps_per_s": leg64["bd_steps_per_s"],
+        "fp64": leg64,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": "faces_f64<%d,%d,%d>" % (split, flags & 1, 2 if flags & 2 else 1),
-                     "avg_launch_us": face_avg_s * 1e6,
-                     "flop_per_launch": flops,
-                     "algorithmic_bytes_per_launch": sum(
-                         c["nf"] * face_bytes(c["ncp"], c["npt"], nv, nf) for c in classes)},
+                     "kernel": "faces_kernel<double,%d,%d,%d>" % (
+                         leg64["launch"]["point_split"], fl & 1, 2 if fl & 2 else 1),
+                     "avg_launch_us": leg64["face_kernel_us"],
+                     "flop_per_launch": leg64["flop_per_launch"],
+                     "algorithmic_bytes_per_launch": leg64["bytes_per_launch"]},
     }
+    if not args.no_mixed:
+        legm, Em, gm = run_leg(args, w, c0, "mixed", d_xi, dev, torch)
+        legm["speedup_vs_fp64"] = legm["face_evals_per_s"] / leg64["face_evals_per_s"]
+        legm["face_kernel_speedup_vs_fp64"] = leg64["face_kernel_us"] / legm["face_kernel_us"]
+        out["mixed"] = legm
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(w, c0, args.cpu_budget)
+        base, E_o, g_o = cpu_baseline(w, c0, args.cpu_budget)
+        out["cpu_baseline"] = base
+
+        def rms(a):
+            return float(np.sqrt(np.mean(a ** 2)))
+        out["fp64"]["vs_oracle"] = {"energy_rel": abs(E64 - E_o) / abs(E_o),
+                                    "grad_rms_rel": rms(g64[:-3] - g_o[:-3]) / rms(g_o[:-3])}
+        if "mixed" in out:
+            out["mixed"]["vs_oracle"] = {"energy_rel": abs(Em - E_o) / abs(E_o),
+                                         "grad_rms_rel": rms(gm[:-3] - g_o[:-3]) / rms(g_o[:-3])}
     print(json.dumps(out), flush=True)
 
 
