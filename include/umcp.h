@@ -127,11 +127,9 @@ int umcp_bd_steps_dev(umcp_surface *s, double *d_r, const double *d_xi, int nste
                       const umcp_bd_params *bd, double *d_energies, void *stream);
 
 /* launch shape of the face kernel (no reference counterpart; set by the
- * tuner): point_split in 1..5 waves share the Gauss points of 64 faces;
- * flags: UMCP_LAUNCH_STAGE_LDS keeps the faces' control points in LDS
- * instead of registers, UMCP_LAUNCH_OCC2 bounds registers for two waves per
- * SIMD */
-#define UMCP_LAUNCH_STAGE_LDS 1
+ * tuner): point_split in 1..4 waves share the Gauss points of 64 faces
+ * (their control points staged in LDS); flags: UMCP_LAUNCH_OCC2 bounds
+ * registers for two waves per SIMD */
 #define UMCP_LAUNCH_OCC2 2
 int umcp_set_launch(umcp_surface *s, int point_split, int flags);
 int umcp_get_launch(const umcp_surface *s, int *point_split, int *flags);

@@ -66,8 +66,8 @@ def test_mixed_vesicle_40962():
 def test_mixed_small_meshes_all_shapes():
     w = workloads.vesicle_642()
     g, o = pair(w)
-    for split in (1, 2, 3, 4, 5):
-        for flags in (0, 1, 2, 3):
+    for split in (1, 2, 3, 4):
+        for flags in (0, 2):
             g.set_launch(split, flags)
             rel_e, rms = errors(g, o, w.r, "vesicle_642 split %d flags %d" % (split, flags))
             assert rel_e <= E_TOL and rms <= G_RMS_TOL

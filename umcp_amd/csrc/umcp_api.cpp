@@ -27,7 +27,7 @@ thread_local int g_device = 0;
 struct ClassDev {
     int *cp = nullptr, *pos = nullptr;
     uint8_t *img = nullptr, *has_img = nullptr;
-    double *g0 = nullptr, *rurv0 = nullptr, *Lo = nullptr, *Li = nullptr, *c0 = nullptr;
+    double *g0 = nullptr, *rurv0 = nullptr, *ig0 = nullptr, *Lo = nullptr, *Li = nullptr, *c0 = nullptr;
 };
 
 }  // namespace
@@ -53,7 +53,7 @@ struct umcp_surface {
     // launch shape of the face kernel: waves sharing 64 faces' Gauss points,
     // and whether the faces' control points are staged in LDS
     int point_split = 4;
-    int launch_flags = kLaunchStageLds | kLaunchOcc2;
+    int launch_flags = kLaunchOcc2;
     std::vector<int> launch_order;  // classes in launch order (irregular first)
     std::vector<int> block_start;
     double *d_xi = nullptr;        // host-array BD staging, chunk of steps
@@ -134,6 +134,7 @@ FaceArgs face_args(umcp_surface *s, size_t c, const double *d_r)
     a.has_img = D.has_img;
     a.g0 = D.g0;
     a.rurv0 = D.rurv0;
+    a.ig0 = D.ig0;
     a.Lo = D.Lo;
     a.Li = D.Li;
     a.c0 = D.c0;
@@ -197,7 +198,7 @@ int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs 
         (void)hipEventRecord(ev[0], st);
     }
     const bool mixed = s->prm.precision == UMCP_PREC_MIXED;
-    hipError_t e = launch_faces(m, s->npartial, s->point_split, s->launch_flags, mixed, st);
+    hipError_t e = launch_faces(m, s->npartial, s->point_split, s->launch_flags, mixed, features_of(m.cls[0].ph), st);
     if (e != hipSuccess) return hip_fail(s, e, "face kernel launch");
     if (s->prof_on) (void)hipEventRecord(ev[1], st);
     ga.nv = s->T.nv;
@@ -349,7 +350,7 @@ int umcp_plan(umcp_surface *s)
         if (kind_of(C.ncp, C.npt) < 0) return fail(s, UMCP_ERR_BAD_MESH, "no kernel for this face valence");
         if ((rc = dev_alloc(s, &D.cp, C.cp.size())) || (rc = dev_alloc(s, &D.pos, C.pos.size())) || (rc = dev_alloc(s, &D.img, C.img.size())) ||
             (rc = dev_alloc(s, &D.has_img, nf)) || (rc = dev_alloc(s, &D.g0, nf * C.npt)) ||
-            (rc = dev_alloc(s, &D.rurv0, nf * C.npt)) || (rc = dev_alloc(s, &D.Lo, nf)) ||
+            (rc = dev_alloc(s, &D.rurv0, nf * C.npt)) || (rc = dev_alloc(s, &D.ig0, nf * C.npt)) || (rc = dev_alloc(s, &D.Lo, nf)) ||
             (rc = dev_alloc(s, &D.Li, nf)) || (rc = dev_alloc(s, &D.c0, nf)) ||
             0)
             return rc;
@@ -364,7 +365,8 @@ int umcp_plan(umcp_surface *s)
         std::vector<double> zero(nf * C.npt, 0.0);
         if ((rc = dev_upload(s, D.c0, zero.data(), nf)) || (rc = dev_upload(s, D.Lo, zero.data(), nf)) ||
             (rc = dev_upload(s, D.Li, zero.data(), nf)) || (rc = dev_upload(s, D.g0, zero.data(), nf * C.npt)) ||
-            (rc = dev_upload(s, D.rurv0, zero.data(), nf * C.npt)))
+            (rc = dev_upload(s, D.rurv0, zero.data(), nf * C.npt)) ||
+            (rc = dev_upload(s, D.ig0, zero.data(), nf * C.npt)))
             return rc;
     }
     layout_launch(s);
@@ -407,7 +409,7 @@ int umcp_setg0(umcp_surface *s, const double *r)
         const FaceClass &C = s->P.classes[c];
         FaceArgs a = face_args(s, c, s->d_r);
         hipError_t e = launch_face_metric(a, kind_of(C.ncp, C.npt), 0, s->cls[c].g0, s->cls[c].rurv0,
-                                          nullptr, 0, s->stream);
+                                          s->cls[c].ig0, nullptr, 0, s->stream);
         if (e != hipSuccess) return hip_fail(s, e, "metric kernel launch");
     }
     // single-lipid packing per triangle from the reference area of each face
@@ -502,8 +504,8 @@ int umcp_area(umcp_surface *s, const double *r, double *A, double *A0)
     for (size_t c = 0; c < s->P.classes.size(); c++) {
         const FaceClass &C = s->P.classes[c];
         FaceArgs a = face_args(s, c, s->d_r);
-        hipError_t e = launch_face_metric(a, kind_of(C.ncp, C.npt), 1, nullptr, nullptr, s->d_partial, base,
-                                          s->stream);
+        hipError_t e = launch_face_metric(a, kind_of(C.ncp, C.npt), 1, nullptr, nullptr, nullptr, s->d_partial,
+                                          base, s->stream);
         base += metric_blocks(C.nf());
         if (e != hipSuccess) return hip_fail(s, e, "metric kernel launch");
     }
@@ -597,8 +599,8 @@ int umcp_set_launch(umcp_surface *s, int point_split, int flags)
 {
     if (!s || !s->planned) return UMCP_ERR_STATE;
     if (point_split < 1 || point_split > kMaxSplit)
-        return fail(s, UMCP_ERR_ARG, "point split must be between 1 and 5 waves");
-    if (flags & ~3) return fail(s, UMCP_ERR_ARG, "unknown launch flags");
+        return fail(s, UMCP_ERR_ARG, "point split must be between 1 and 4 waves");
+    if (flags & ~kLaunchOcc2) return fail(s, UMCP_ERR_ARG, "unknown launch flags");
     s->point_split = point_split;
     s->launch_flags = flags;
     return UMCP_OK;

@@ -97,6 +97,27 @@ __device__ __forceinline__ void image_offset(int code, const double pbc[3][3], d
     for (int k = 0; k < 3; k++) out[k] = ix * pbc[0][k] + iy * pbc[1][k] + iz * pbc[2][k];
 }
 
+// Reciprocal and square root: IEEE-exact in FP64; the hardware
+// single-instruction forms (about 1 ulp) in the FP32 path, whose error budget
+// is set by FP32 rounding anyway.
+template <typename R>
+__device__ __forceinline__ R recip(R x)
+{
+    if constexpr (sizeof(R) == 4) return __builtin_amdgcn_rcpf(x);
+    else return R(1) / x;
+}
+template <typename R>
+__device__ __forceinline__ R root(R x)
+{
+    if constexpr (sizeof(R) == 4) return __builtin_amdgcn_sqrtf(x);
+    else return sqrt(x);
+}
+
+// Energy-form feature sets, fixed at compile time so the common cases skip
+// unused terms: general (runtime mode, kg and k_reg), lipid mode with kg =
+// k_reg = 0 (the reference default), Helfrich mode with kg = k_reg = 0.
+constexpr int kFeatGeneral = 0, kFeatLipid = 1, kFeatHelfrich = 2;
+
 // One Gauss point: energy and dE/d(Ru, Rv, Ruu, Ruv, Rvv).
 //
 // The reference builds the principal curvatures c1,2 = -(a+d -/+ sqrt(SAFETY
@@ -105,35 +126,38 @@ __device__ __forceinline__ void image_offset(int code, const double pbc[3][3], d
 // c1 c2 = det S - SAFETY/4 only, so this evaluates tr S and det S directly:
 //   tr S  = (L E22 - 2 M E12 + N E11) / det I,   det S = (L N - M^2) / det I
 // with E = first fundamental form, (L, M, N) = n . (Ruu, Ruv, Rvv) and
-// n = (Ru x Rv)/|Ru x Rv|; the gradient is the exact reverse of this chain.
-template <typename R>
-__device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R rurv0, R Lo, R Li,
+// n = (Ru x Rv)/g, g = sqrt(det I) = |Ru x Rv|; the gradient is the exact
+// reverse of this chain.  ig0 = 1/g0 of this point.
+template <typename R, int FEAT>
+__device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R ig0, R rurv0, R Lo, R Li,
                                                R c0f, R dudv, const PhysParams &ph, R D[5][3])
 {
+    constexpr bool kAnyMode = FEAT == kFeatGeneral;
+    constexpr bool kExtras = FEAT == kFeatGeneral;   // kg and k_reg terms
     const R *Ru = X[0], *Rv = X[1], *Suu = X[2], *Suv = X[3], *Svv = X[4];
     const R E11 = Ru[0] * Ru[0] + Ru[1] * Ru[1] + Ru[2] * Ru[2];
     const R E12 = Ru[0] * Rv[0] + Ru[1] * Rv[1] + Ru[2] * Rv[2];
     const R E22 = Rv[0] * Rv[0] + Rv[1] * Rv[1] + Rv[2] * Rv[2];
     const R det = E11 * E22 - E12 * E12;
-    const R g = sqrt(det);
-    const R idet = R(1) / det;
+    const R g = root<R>(det);
+    const R idet = recip<R>(det);
+    const R ig = g * idet;                       // 1/g
     R Nv[3];
     Nv[0] = Ru[1] * Rv[2] - Ru[2] * Rv[1];
     Nv[1] = Ru[2] * Rv[0] - Ru[0] * Rv[2];
     Nv[2] = Ru[0] * Rv[1] - Ru[1] * Rv[0];
-    const R inn = R(1) / sqrt(Nv[0] * Nv[0] + Nv[1] * Nv[1] + Nv[2] * Nv[2]);
-    const R n[3] = {Nv[0] * inn, Nv[1] * inn, Nv[2] * inn};
+    const R n[3] = {Nv[0] * ig, Nv[1] * ig, Nv[2] * ig};
     const R L = n[0] * Suu[0] + n[1] * Suu[1] + n[2] * Suu[2];
     const R M = n[0] * Suv[0] + n[1] * Suv[1] + n[2] * Suv[2];
     const R N = n[0] * Svv[0] + n[1] * Svv[1] + n[2] * Svv[2];
     const R T = L * E22 - R(2) * M * E12 + N * E11;
     const R H = -T * idet;                       // c1 + c2
-    const R K = (L * N - M * M) * idet;          // c1 c2 + SAFETY/4
 
-    const R kc = (R)ph.kc, kg = (R)ph.kg, KA = (R)ph.KA, kreg = (R)ph.k_reg;
+    const R kc = (R)ph.kc, KA = (R)ph.KA;
     const R g0w = g0 * w;
     R e, dH;
-    if (ph.mode == 0) {
+    const bool lipid = kAnyMode ? (ph.mode == 0) : (FEAT == kFeatLipid);
+    if (lipid) {
         // lipid-area weighted leaflets, no dudv (src/interpLib.C:3832-3834)
         const R lc0 = (R)ph.lipid_c0;
         const R ho = H - lc0, hi = H + lc0;
@@ -145,34 +169,45 @@ __device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R rur
         e = R(0.5) * kc * dudv * g0w * hd * hd;
         dH = kc * dudv * g0w * hd;
     }
-    const R dK = kg * dudv * g0w;
-    e += dK * (K - R(0.25e-12));
-    // per-point area strain: A = dudv g w, A0 = dudv g0 w
-    const R A = dudv * g * w, A0 = dudv * g0w;
-    const R strain = (A - A0) / A0;
-    e += KA * A0 * strain * strain;
+    // per-point area strain (A - A0)/A0 with A = dudv g w, A0 = dudv g0 w
+    const R strain = g * ig0 - R(1);
+    e += KA * dudv * g0w * strain * strain;
     const R dg = R(2) * KA * strain * dudv * w;
-    // shear regulariser on cos(angle(Ru, Rv)) (src/interpLib.C:3797-3801)
-    const R s1122 = sqrt(E11 * E22);
-    const R val = E12 / s1122;
-    const R dv_ = val - rurv0;
-    e += kreg * dv_ * dv_;
-    const R dval = R(2) * kreg * dv_;
 
-    // reverse pass
-    const R dL = (-dH * E22 + dK * N) * idet;
-    const R dM = (R(2) * dH * E12 - R(2) * dK * M) * idet;
-    const R dN = (-dH * E11 + dK * L) * idet;
-    const R ddet = dH * T * idet * idet - dK * K * idet + dg * R(0.5) / g;
-    const R dE11 = -dH * N * idet + ddet * E22 - dval * val * R(0.5) / E11;
-    const R dE22 = -dH * L * idet + ddet * E11 - dval * val * R(0.5) / E22;
-    const R dE12 = R(2) * dH * M * idet - R(2) * ddet * E12 + dval / s1122;
+    R dL = -dH * E22 * idet, dM = R(2) * dH * E12 * idet, dN = -dH * E11 * idet;
+    R ddet = dH * T * idet * idet + dg * R(0.5) * ig;
+    R dE11 = -dH * N * idet + ddet * E22;
+    R dE22 = -dH * L * idet + ddet * E11;
+    R dE12 = R(2) * dH * M * idet - R(2) * ddet * E12;
+    if constexpr (kExtras) {
+        // Gaussian curvature term kg c1 c2 = kg (det S - SAFETY/4)
+        const R K = (L * N - M * M) * idet;
+        const R dK = (R)ph.kg * dudv * g0w;
+        e += dK * (K - R(0.25e-12));
+        dL += dK * N * idet;
+        dM -= R(2) * dK * M * idet;
+        dN += dK * L * idet;
+        const R dd = -dK * K * idet;
+        dE11 += dd * E22;
+        dE22 += dd * E11;
+        dE12 -= R(2) * dd * E12;
+        // shear regulariser on cos(angle(Ru, Rv)) (src/interpLib.C:3797-3801)
+        const R kreg = (R)ph.k_reg;
+        const R s1122 = sqrt(E11 * E22);
+        const R val = E12 / s1122;
+        const R dv_ = val - rurv0;
+        e += kreg * dv_ * dv_;
+        const R dval = R(2) * kreg * dv_;
+        dE11 -= dval * val * R(0.5) / E11;
+        dE22 -= dval * val * R(0.5) / E22;
+        dE12 += dval / s1122;
+    }
     R Q[3], dNv[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) Q[k] = dL * Suu[k] + dM * Suv[k] + dN * Svv[k];
     const R qn = Q[0] * n[0] + Q[1] * n[1] + Q[2] * n[2];
 #pragma unroll
-    for (int k = 0; k < 3; k++) dNv[k] = (Q[k] - qn * n[k]) * inn;
+    for (int k = 0; k < 3; k++) dNv[k] = (Q[k] - qn * n[k]) * ig;
     // dE/dRu += Rv x dN ; dE/dRv += dN x Ru
     D[0][0] = R(2) * dE11 * Ru[0] + dE12 * Rv[0] + (Rv[1] * dNv[2] - Rv[2] * dNv[1]);
     D[0][1] = R(2) * dE11 * Ru[1] + dE12 * Rv[1] + (Rv[2] * dNv[0] - Rv[0] * dNv[2]);
@@ -203,7 +238,7 @@ __device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R rur
 // invariant); contraction, curvature and chain rule run in FP32; the energy
 // and dE/dalpha accumulate in FP64; the gradient leaves as FP32 and is summed
 // per vertex in FP64 by the gather.
-template <typename R, int NCP, int NPT, int KIND, int WS, bool PLDS>
+template <typename R, int NCP, int NPT, int KIND, int WS, bool PLDS, int FEAT>
 __device__ __forceinline__ void face_group(const FaceArgs &a, int group, R *lds, double acc[4])
 {
     constexpr bool kRel = sizeof(R) == 4;
@@ -283,8 +318,9 @@ __device__ __forceinline__ void face_group(const FaceArgs &a, int group, R *lds,
 #pragma unroll
             for (int k = 0; k < 3; k++) X[d][k] *= al[k];
         R D[5][3];
-        e += (double)point_energy_grad<R>(X, wts[p], (R)a.g0[p * nf + fc], (R)a.rurv0[p * nf + fc], Lo, Li,
-                                          c0f, (R)a.dudv, a.ph, D);
+        const R rr0 = FEAT == kFeatGeneral ? (R)a.rurv0[p * nf + fc] : R(0);
+        e += (double)point_energy_grad<R, FEAT>(X, wts[p], (R)a.g0[p * nf + fc], (R)a.ig0[p * nf + fc], rr0,
+                                                Lo, Li, c0f, (R)a.dudv, a.ph, D);
 #pragma unroll
         for (int c = 0; c < NCP; c++) {
 #pragma unroll
@@ -342,7 +378,7 @@ __device__ __forceinline__ void face_group(const FaceArgs &a, int group, R *lds,
     }
 }
 
-template <typename R, int WS, bool PLDS, int OCC>
+template <typename R, int WS, bool PLDS, int OCC, int FEAT>
 __global__ __launch_bounds__(kWave *WS, OCC) void faces_kernel(MultiFaceArgs m)
 {
     __shared__ R lds[13 * 3 * kWave];
@@ -352,9 +388,9 @@ __global__ __launch_bounds__(kWave *WS, OCC) void faces_kernel(MultiFaceArgs m)
     const int group = b - m.block_start[c];
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     switch (m.kind[c]) {
-    case 0: face_group<R, 12, 15, 0, WS, PLDS>(m.cls[c], group, lds, acc); break;
-    case 1: face_group<R, 11, 16, 1, WS, PLDS>(m.cls[c], group, lds, acc); break;
-    default: face_group<R, 13, 16, 2, WS, PLDS>(m.cls[c], group, lds, acc); break;
+    case 0: face_group<R, 12, 15, 0, WS, PLDS, FEAT>(m.cls[c], group, lds, acc); break;
+    case 1: face_group<R, 11, 16, 1, WS, PLDS, FEAT>(m.cls[c], group, lds, acc); break;
+    default: face_group<R, 13, 16, 2, WS, PLDS, FEAT>(m.cls[c], group, lds, acc); break;
     }
     block_sum4<kWave * WS>(acc);
     if (threadIdx.x == 0) {
@@ -368,8 +404,8 @@ __global__ __launch_bounds__(kWave *WS, OCC) void faces_kernel(MultiFaceArgs m)
 // mode 1 accumulates sum(dudv g w) and sum(dudv g0 w) into block partials.
 template <int NCP, int NPT, int KIND>
 __global__ __launch_bounds__(kMetricBlock) void face_metric(FaceArgs a, int mode, double *g0_out,
-                                                            double *rurv0_out, double *partial,
-                                                            int partial_base)
+                                                            double *rurv0_out, double *ig0_out,
+                                                            double *partial, int partial_base)
 {
     const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
     const int f = lb * kMetricBlock + threadIdx.x;
@@ -409,6 +445,7 @@ __global__ __launch_bounds__(kMetricBlock) void face_metric(FaceArgs a, int mode
             if (mode == 0) {
                 g0_out[p * nf + f] = g;
                 rurv0_out[p * nf + f] = E12 / sqrt(E11 * E22);
+                ig0_out[p * nf + f] = 1.0 / g;
             } else {
                 acc[0] += a.dudv * g * wp;
                 acc[1] += a.dudv * a.g0[p * nf + f] * wp;
@@ -523,60 +560,71 @@ hipError_t upload_tables(int kind, const double *table, const double *weights, i
 int face_groups(int nf) { return (nf + kWave - 1) / kWave; }
 int metric_blocks(int nf) { return (nf + kMetricBlock - 1) / kMetricBlock; }
 
-template <typename R, int WS, bool PLDS, int OCC>
+template <typename R, int WS, int OCC, int FEAT>
 static hipError_t launch_ws(const MultiFaceArgs &a, int nb, hipStream_t st)
 {
-    hipLaunchKernelGGL((faces_kernel<R, WS, PLDS, OCC>), dim3(nb), dim3(kWave * WS), 0, st, a);
+    hipLaunchKernelGGL((faces_kernel<R, WS, true, OCC, FEAT>), dim3(nb), dim3(kWave * WS), 0, st, a);
     return hipGetLastError();
 }
 
-template <typename R, int WS>
-static hipError_t launch_ws_flags(const MultiFaceArgs &a, int nb, int flags, hipStream_t st)
+// the register-bound flag asks for 2 waves per SIMD in FP64 and 4 in FP32
+template <typename R, int FEAT>
+static hipError_t launch_shape(const MultiFaceArgs &a, int nb, int ws, int occ2, hipStream_t st)
 {
-    switch (flags & 3) {
-    case 0: return launch_ws<R, WS, false, 1>(a, nb, st);
-    case 1: return launch_ws<R, WS, true, 1>(a, nb, st);
-    case 2: return launch_ws<R, WS, false, 2>(a, nb, st);
-    default: return launch_ws<R, WS, true, 2>(a, nb, st);
-    }
-}
-
-template <typename R>
-static hipError_t launch_prec(const MultiFaceArgs &a, int nb, int ws, int flags, hipStream_t st)
-{
-    switch (ws) {
-    case 1: return launch_ws_flags<R, 1>(a, nb, flags, st);
-    case 2: return launch_ws_flags<R, 2>(a, nb, flags, st);
-    case 3: return launch_ws_flags<R, 3>(a, nb, flags, st);
-    case 4: return launch_ws_flags<R, 4>(a, nb, flags, st);
-    case 5: return launch_ws_flags<R, 5>(a, nb, flags, st);
+    constexpr int kB = sizeof(R) == 4 ? 4 : 2;
+    switch (ws * 2 + (occ2 ? 1 : 0)) {
+    case 2: return launch_ws<R, 1, 1, FEAT>(a, nb, st);
+    case 3: return launch_ws<R, 1, kB, FEAT>(a, nb, st);
+    case 4: return launch_ws<R, 2, 1, FEAT>(a, nb, st);
+    case 5: return launch_ws<R, 2, kB, FEAT>(a, nb, st);
+    case 6: return launch_ws<R, 3, 1, FEAT>(a, nb, st);
+    case 7: return launch_ws<R, 3, kB, FEAT>(a, nb, st);
+    case 8: return launch_ws<R, 4, 1, FEAT>(a, nb, st);
+    case 9: return launch_ws<R, 4, kB, FEAT>(a, nb, st);
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_faces(const MultiFaceArgs &a, int nb, int ws, int flags, bool mixed, hipStream_t st)
+template <typename R>
+static hipError_t launch_prec(const MultiFaceArgs &a, int nb, int ws, int occ2, int feat, hipStream_t st)
+{
+    switch (feat) {
+    case kFeatLipid: return launch_shape<R, kFeatLipid>(a, nb, ws, occ2, st);
+    case kFeatHelfrich: return launch_shape<R, kFeatHelfrich>(a, nb, ws, occ2, st);
+    default: return launch_shape<R, kFeatGeneral>(a, nb, ws, occ2, st);
+    }
+}
+
+int features_of(const PhysParams &ph)
+{
+    if (ph.kg != 0.0 || ph.k_reg != 0.0) return kFeatGeneral;
+    return ph.mode == 0 ? kFeatLipid : kFeatHelfrich;
+}
+
+hipError_t launch_faces(const MultiFaceArgs &a, int nb, int ws, int flags, bool mixed, int feat, hipStream_t st)
 {
     if (nb <= 0) return hipSuccess;
-    return mixed ? launch_prec<float>(a, nb, ws, flags, st) : launch_prec<double>(a, nb, ws, flags, st);
+    const int occ2 = (flags & kLaunchOcc2) ? 1 : 0;
+    return mixed ? launch_prec<float>(a, nb, ws, occ2, feat, st) : launch_prec<double>(a, nb, ws, occ2, feat, st);
 }
 
 hipError_t launch_face_metric(const FaceArgs &a, int kind, int mode, double *g0_out, double *rurv0_out,
-                              double *partial, int partial_base, hipStream_t st)
+                              double *ig0_out, double *partial, int partial_base, hipStream_t st)
 {
     if (a.nf == 0) return hipSuccess;
     const int nb = metric_blocks(a.nf);
     switch (kind) {
     case 0:
         hipLaunchKernelGGL((face_metric<12, 15, 0>), dim3(nb), dim3(kMetricBlock), 0, st, a, mode, g0_out,
-                           rurv0_out, partial, partial_base);
+                           rurv0_out, ig0_out, partial, partial_base);
         break;
     case 1:
         hipLaunchKernelGGL((face_metric<11, 16, 1>), dim3(nb), dim3(kMetricBlock), 0, st, a, mode, g0_out,
-                           rurv0_out, partial, partial_base);
+                           rurv0_out, ig0_out, partial, partial_base);
         break;
     case 2:
         hipLaunchKernelGGL((face_metric<13, 16, 2>), dim3(nb), dim3(kMetricBlock), 0, st, a, mode, g0_out,
-                           rurv0_out, partial, partial_base);
+                           rurv0_out, ig0_out, partial, partial_base);
         break;
     default: return hipErrorInvalidValue;
     }

@@ -12,7 +12,7 @@ constexpr int kWave = 64;
 constexpr int kGatherBlock = 256;
 constexpr int kMetricBlock = 256;
 constexpr int kMaxClasses = 4;
-constexpr int kMaxSplit = 5;
+constexpr int kMaxSplit = 4;
 
 struct PhysParams {
     double kc, kg, KA, k_reg, lipid_c0;
@@ -30,6 +30,7 @@ struct FaceArgs {
     const uint8_t *has_img; // [nf]
     const double *g0;       // [npt][nf]
     const double *rurv0;    // [npt][nf]
+    const double *ig0;      // [npt][nf]   1 / g0
     const double *Lo, *Li;  // [nf] lipid areas of the face's triangle
     const double *c0;       // [nf]
     double dudv;
@@ -73,15 +74,16 @@ int kind_of(int ncp, int npt);
 // the device's constant memory (current device)
 hipError_t upload_tables(int kind, const double *table, const double *weights, int npt, int ncp);
 int face_groups(int nf);
-// launch shape: point_split waves share the Gauss points of 64 faces;
-// flags bit 0 keeps the faces' control points in LDS instead of registers,
-// bit 1 bounds registers for two waves per SIMD
-constexpr int kLaunchStageLds = 1;
+// launch shape: point_split (1..4) waves share the Gauss points of 64 faces
+// (whose control points are staged in LDS); flag kLaunchOcc2 bounds registers
+// for two waves per SIMD.  feat: energy-form specialisation (features_of).
 constexpr int kLaunchOcc2 = 2;
-hipError_t launch_faces(const MultiFaceArgs &a, int nblocks, int point_split, int flags, bool mixed,
+int features_of(const PhysParams &ph);
+hipError_t launch_faces(const MultiFaceArgs &a, int nblocks, int point_split, int flags, bool mixed, int feat,
                         hipStream_t st);
 hipError_t launch_face_metric(const FaceArgs &a, int kind, int mode, double *g0_out,
-                              double *rurv0_out, double *partial, int partial_base, hipStream_t st);
+                              double *rurv0_out, double *ig0_out, double *partial, int partial_base,
+                              hipStream_t st);
 int metric_blocks(int nf);
 hipError_t launch_vertex_gather(const GatherArgs &a, hipStream_t st);
 

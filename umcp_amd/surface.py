@@ -24,7 +24,6 @@ MODE_LIPID = 0
 MODE_HELFRICH = 1
 PREC_FP64 = 0
 PREC_MIXED = 1
-LAUNCH_STAGE_LDS = 1
 LAUNCH_OCC2 = 2
 
 # every symbol include/umcp.h declares

@@ -1,8 +1,8 @@
 """Launch-shape autotuner for the face kernel.
 
 The fused face kernel has two tunable choices: how many waves share the
-Gauss points of a 64-face group (1..5) and two flags (control points staged
-in LDS; registers bounded for two waves per SIMD).  ``tune`` times every
+Gauss points of a 64-face group (1..4) and whether registers are bounded for
+two waves per SIMD.  ``tune`` times every
 shape on the caller's own surface with the library's HIP-event profiling and
 keeps the fastest; it works on device buffers and never changes ``r`` (it
 evaluates energy + gradient only).
@@ -11,8 +11,8 @@ from __future__ import annotations
 
 import itertools
 
-SPLITS = (1, 2, 3, 4, 5)
-FLAGS = (0, 1, 2, 3)
+SPLITS = (1, 2, 3, 4)
+FLAGS = (0, 2)
 
 
 def tune(surface, d_r, d_E, d_g, stream=None, reps: int = 10, sync=None):
