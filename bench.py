@@ -45,6 +45,31 @@ def face_bytes(ncp: int, npt: int, nv: int, nf: int) -> float:
 
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
+FEAT_HELFRICH = 2         # energy-form template argument of the face kernel (mode B, kg = k_reg = 0)
+
+
+def kernel_name(precision: str, split: int, flags: int, feat: int = FEAT_HELFRICH) -> str:
+    """The face kernel's symbol as rocprofv3 prints it."""
+    occ = (4 if precision == "mixed" else 2) if flags & 2 else 1
+    return "faces_kernel<%s, %d, true, %d, %d>" % (
+        "float" if precision == "mixed" else "double", split, occ, feat)
+
+
+def measured_traffic(name: str):
+    """HBM bytes per launch of kernel `name` from the committed PMC summaries
+    (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each scaled by the factor a
+    plain device copy of known size reads on the same counter); None when no
+    summary covers this exact kernel."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for line in open(path):
+            line = line.strip()
+            if not line.startswith("traffic " + name + " "):
+                continue
+            kv = dict(t.split("=") for t in line[len("traffic " + name) + 1:].split())
+            return 1024.0 * (float(kv["fetch_kib"]) * float(kv["fetch_factor"])
+                             + float(kv["write_kib"]) * float(kv["write_factor"]))
+    return None
 
 
 def cpu_baseline(w, c0, budget_s: float):
@@ -185,6 +210,7 @@ def main():
     face_s = leg64["face_kernel_us"] / 1e6
     achieved = leg64["flop_per_launch"] / face_s / 1e12
     fl = leg64["launch"]["flags"]
+    kname = kernel_name("fp64", leg64["launch"]["point_split"], fl)
     out = {
         "metric": METRIC,
         "value": leg64["face_evals_per_s"],
@@ -205,9 +231,10 @@ def main():
         "bd_steps_per_s": leg64["bd_steps_per_s"],
         "fp64": leg64,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": "faces_kernel<double,%d,%d,%d>" % (
-                         leg64["launch"]["point_split"], fl & 1, 2 if fl & 2 else 1),
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": measured_traffic(kname),
+                     "traffic_unit": "bytes/launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                     "kernel": kname,
                      "avg_launch_us": leg64["face_kernel_us"],
                      "flop_per_launch": leg64["flop_per_launch"],
                      "algorithmic_bytes_per_launch": leg64["bytes_per_launch"]},
@@ -216,6 +243,8 @@ def main():
         legm, Em, gm = run_leg(args, w, c0, "mixed", d_xi, dev, torch)
         legm["speedup_vs_fp64"] = legm["face_evals_per_s"] / leg64["face_evals_per_s"]
         legm["face_kernel_speedup_vs_fp64"] = leg64["face_kernel_us"] / legm["face_kernel_us"]
+        legm["kernel"] = kernel_name("mixed", legm["launch"]["point_split"], legm["launch"]["flags"])
+        legm["traffic"] = measured_traffic(legm["kernel"])
         out["mixed"] = legm
     if not args.no_cpu_baseline:
         base, E_o, g_o = cpu_baseline(w, c0, args.cpu_budget)
