@@ -126,6 +126,76 @@ int umcp_energy_grad_dev(umcp_surface *s, const double *d_r, double *d_E, double
 int umcp_bd_steps_dev(umcp_surface *s, double *d_r, const double *d_xi, int nsteps,
                       const umcp_bd_params *bd, double *d_energies, void *stream);
 
+/* Global restraints: area KA_global*A0*((A-A0)/A0)^2 with A = sum dudv g w
+ * (the GLOBAL_AREA form, src/interpLib.C:4322-4326) and volume
+ * KV/2*V0*((V-V0)/V0)^2 (src/minimize.C:90-114) of the limit-point
+ * polyhedron (src/interpLib.C:997-1098).  A0 <= 0 takes the reference area
+ * sum(dudv g0 w) of setg0, V0 == 0 the volume at setg0 (src/hd.C:273-279).
+ * Once set they are part of every energy, gradient and Brownian step.  KV
+ * needs a closed mesh (no periodic images).  KA_global in kcal/mol/A^2, KV in
+ * kcal/mol/A^3. */
+typedef struct {
+    double KA_global;
+    double KV;
+    double A0;
+    double V0;
+} umcp_restraints;
+int umcp_set_restraints(umcp_surface *s, const umcp_restraints *rs);
+
+/* surface::volume (src/interpLib.C:997-1098): volume enclosed by the
+ * polyhedron of the limit positions of the vertices (closed meshes) */
+int umcp_volume(umcp_surface *s, const double *r, double *V);
+
+/* Brownian steps drawing their noise on the device: xi of step k is the
+ * counter-based stream of (seed, k) that umcp_noise writes (Philox4x64-10
+ * keyed by seed, Box-Muller), so a trajectory depends only on (seed, first
+ * step), not on the launch or on other surfaces.  The step counter starts at
+ * the umcp_set_step value (0 after umcp_plan) and advances by one per step;
+ * d_energies[k] (device, capacity energies_cap, may be NULL) receives the
+ * energy at the start of step k.  Asynchronous like the other _dev calls.
+ * (hd.C draws with gsl_ran_gaussian, src/hd.C:1658; no bitwise counterpart.) */
+int umcp_set_step(umcp_surface *s, int step);
+int umcp_get_step(umcp_surface *s, int *step);
+int umcp_bd_steps_rng_dev(umcp_surface *s, double *d_r, int nsteps, const umcp_bd_params *bd,
+                          unsigned long long seed, double *d_energies, int energies_cap,
+                          void *stream);
+/* the deviates of (seed, step) for vertices [0, n): d_out[3v + k] (device) */
+int umcp_noise(unsigned long long seed, int step, int n, double *d_out, void *stream);
+
+/* Ensembles (no reference counterpart: the reference runs one surface per
+ * process, src/hd.C): many independent resident surfaces stepped together.
+ * Each member keeps its own tables, parameters and restraints; one step of
+ * all members is one batched launch each of the restraint, face and vertex
+ * kernels over device work tables (optionally captured into a hipGraph of
+ * graph_steps steps and replayed).  A member's trajectory is the same alone
+ * (umcp_bd_steps_rng_dev) and in any mix.  Members must be planned and
+ * setg0'd, share one precision, and must not be changed while members
+ * (umcp_ensemble_refresh after a change); d_r (3nv+3) and d_energies are
+ * device arrays owned by the caller.  Before removing a member or freeing its
+ * arrays, the caller synchronises the stream the steps ran on. */
+typedef struct umcp_ensemble umcp_ensemble;
+int umcp_ensemble_create(umcp_ensemble **out);
+void umcp_ensemble_destroy(umcp_ensemble *e);
+const char *umcp_ensemble_last_error(const umcp_ensemble *e);
+int umcp_ensemble_add(umcp_ensemble *e, umcp_surface *s, double *d_r, const umcp_bd_params *bd,
+                      unsigned long long seed, double *d_energies, int energies_cap);
+int umcp_ensemble_remove(umcp_ensemble *e, umcp_surface *s);
+int umcp_ensemble_size(const umcp_ensemble *e);
+int umcp_ensemble_refresh(umcp_ensemble *e);
+/* batched launch shape: point_split and flags as umcp_set_launch; face_grid
+ * = 0 launches one workgroup per 64-face item, > 0 a persistent grid of that
+ * many workgroups striding over the items; order 0 = member after member,
+ * 1 = round-robin over members */
+int umcp_ensemble_set_launch(umcp_ensemble *e, int point_split, int flags, int face_grid, int order);
+int umcp_ensemble_get_launch(const umcp_ensemble *e, int *point_split, int *flags, int *face_grid,
+                             int *order);
+/* nsteps Brownian steps of every member with device noise (as
+ * umcp_bd_steps_rng_dev); graph_steps > 0 replays a captured graph of that
+ * many steps; stream NULL = the ensemble's own stream; asynchronous */
+int umcp_ensemble_bd_steps(umcp_ensemble *e, int nsteps, int graph_steps, void *stream);
+/* work items of the current tables (after the first umcp_ensemble_bd_steps) */
+int umcp_ensemble_work(const umcp_ensemble *e, int *face_items, int *gather_items, int *restraint_items);
+
 /* launch shape of the face kernel (no reference counterpart; set by the
  * tuner): point_split in 1..4 waves share the Gauss points of 64 faces
  * (their control points staged in LDS); flags: UMCP_LAUNCH_OCC2 bounds

@@ -66,6 +66,13 @@ def lib():
         L.orc_get_u_pts.argtypes = [dp, dp, C.c_int, C.c_double, C.c_double]
         L.orc_basis.argtypes = [C.c_double, C.c_double, dp]
         L.orc_irregular_table.argtypes = [C.c_int, C.c_double, C.c_double, dp]
+        L.orc_set_restraints.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_limit_points.argtypes = [vp, dp, dp]
+        L.orc_volume.argtypes = [vp, dp]
+        L.orc_volume.restype = C.c_double
+        L.orc_volume_grad.argtypes = [vp, dp, dp]
+        L.orc_restraint_energy.argtypes = [vp, dp, dp, dp]
+        L.orc_restraint_energy.restype = C.c_double
         _LIB = L
     return _LIB
 
@@ -138,6 +145,35 @@ class OracleSurface:
         a, a0 = C.c_double(), C.c_double()
         lib().orc_area(self._h, _p(r), C.byref(a), C.byref(a0))
         return a.value, a0.value
+
+    def set_restraints(self, KA_global: float = 0.0, KV: float = 0.0, A0: float = 0.0,
+                       V0: float = 0.0):
+        """Global area / volume restraints; A0 <= 0 and V0 == 0 take the
+        setg0 reference values."""
+        lib().orc_set_restraints(self._h, KA_global, KV, A0, V0)
+
+    def limit_points(self, r: np.ndarray) -> np.ndarray:
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        L = np.zeros(3 * self.nv)
+        lib().orc_limit_points(self._h, _p(r), _p(L))
+        return L
+
+    def volume(self, r: np.ndarray) -> float:
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        return lib().orc_volume(self._h, _p(r))
+
+    def volume_grad(self, r: np.ndarray) -> np.ndarray:
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        d = np.zeros(3 * self.nv)
+        lib().orc_volume_grad(self._h, _p(r), _p(d))
+        return d
+
+    def restraint_energy(self, r: np.ndarray):
+        """(E_restraint, dE/dA_tot, dE/dV) at r."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        la, lv = C.c_double(), C.c_double()
+        e = lib().orc_restraint_energy(self._h, _p(r), C.byref(la), C.byref(lv))
+        return e, la.value, lv.value
 
     def bd_steps(self, r: np.ndarray, xi: np.ndarray, mobility_dt: float,
                  noise_amp: float, want_energies: bool = False):

@@ -790,6 +790,9 @@ struct omesh {
     oform *F;         /* regular: nfr*15 ; irregular after: nfi*16 */
     double *face_c0;
     oparams prm;
+    /* global restraints: area (GLOBAL_AREA form) and enclosed volume */
+    double KA_global, KV, A0_target, V0_target;
+    double A0_setg0, V_setg0;   /* sum(dudv g0 w) and the volume at setg0 */
 };
 
 static int omesh_adjacent(const omesh *m, int a, int b)
@@ -1342,6 +1345,17 @@ void orc_setg0(omesh *m, const double *r)
         m->Li[t] = Li;
         m->Lo[t] = Lo;
     }
+    double A0 = 0.0;
+    for (int f = 0; f < nf; f++) {
+        int npt;
+        double dudv;
+        oform *F = face_forms(m, f, &npt, &dudv);
+        for (int p = 0; p < npt; p++) A0 += dudv * F[p].g0 * F[p].weight;
+    }
+    m->A0_setg0 = A0;
+    /* the reference records V0 right after setg0 on the same positions
+     * (src/hd.C:273-279) */
+    m->V_setg0 = orc_volume(m, r);
 }
 
 /* per-face energy, src/interpLib.C:3426-3881 (mode 0 = FIXED_A +
@@ -1404,13 +1418,14 @@ double orc_energy(omesh *m, const double *r)
         int f = m->tri_face[t];
         if (f >= 0) e += orc_face_energy(m, f, r);
     }
-    return e;
+    return e + orc_restraint_energy(m, r, NULL, NULL);
 }
 
 /* one Gauss point of grad/igrad (src/grad.C:246-930, 6605-7450), through the
  * reference's graph: E(c1, c2, g) -> (a,b,c,d,g) -> (nsuu,nsuv,nsvv,RuRu,RuRv,
  * RvRv) -> (Ru, Rv, Suu, Suv, Svv) -> control points */
-static void point_grad(omesh *m, int f, const oform *F, double dudv, const double *r, double *gr)
+static void point_grad(omesh *m, int f, const oform *F, double dudv, const double *r, double *gr,
+                       double lamA)
 {
     const oparams *P = &m->prm;
     int nv = m->nv;
@@ -1456,6 +1471,8 @@ static void point_grad(omesh *m, int f, const oform *F, double dudv, const doubl
     de_c1 += P->kg * e2 * dudv * dAf;
     de_c2 += P->kg * e1 * dudv * dAf;
     de_g += 2 * P->KA * ((A - A0) / A0) * w * dudv;
+    /* global area restraint: dE/dA_tot * dA_tot/dg, A_tot = sum dudv g w */
+    de_g += lamA * w * dudv;
 
     /* c1, c2 as functions of the shape operator entries (src/grad.C:381-399) */
     double dc1_a = -0.5 * (1 - (2 * a - 2 * d) / (2. * S));
@@ -1546,6 +1563,8 @@ static void point_grad(omesh *m, int f, const oform *F, double dudv, const doubl
 
 void orc_grad(omesh *m, const double *r, double *g)
 {
+    double lamA = 0.0, lamV = 0.0;
+    orc_restraint_energy(m, r, &lamA, &lamV);
     /* regular faces in triangle order, then irregular (grad then igrad) */
     for (int pass = 0; pass < 2; pass++)
         for (int t = 0; t < m->nt; t++) {
@@ -1555,8 +1574,16 @@ void orc_grad(omesh *m, const double *r, double *g)
             int npt;
             double dudv;
             oform *F = face_forms(m, f, &npt, &dudv);
-            for (int p = 0; p < npt; p++) point_grad(m, f, F + p, dudv, r, g);
+            for (int p = 0; p < npt; p++) point_grad(m, f, F + p, dudv, r, g, lamA);
         }
+    if (lamV != 0.0) {
+        /* volume restraint: KV strain dV/dr (src/minimize.C:351-376) */
+        int n3 = 3 * m->nv;
+        double *dV = malloc(sizeof(double) * (size_t)n3);
+        orc_volume_grad(m, r, dV);
+        for (int i = 0; i < n3; i++) g[i] += lamV * dV[i];
+        free(dV);
+    }
 }
 
 /* geometric area sum(dudv g w) and reference area sum(dudv g0 w) */
@@ -1582,6 +1609,140 @@ void orc_area(omesh *m, const double *r, double *A, double *A0)
             *A0 += dudv * F[p].g0 * F[p].weight;
         }
     }
+}
+
+/* ---- global restraints ------------------------------------------------ */
+
+void orc_set_restraints(omesh *m, double KA_global, double KV, double A0, double V0)
+{
+    m->KA_global = KA_global;
+    m->KV = KV;
+    m->A0_target = A0;
+    m->V0_target = V0;
+}
+
+/* Limit position of every vertex.  The reference's volume plan evaluates the
+ * limit surface at the corners of each face (ni = 2, src/interpLib.C:307,
+ * 426-461): the top eigenvector of the valence-n subdivision stencil at the
+ * owner corner, the box spline (or the Stam route) at the other two.  With the
+ * stencil weight beta = 3/(8n) (src/interpLib.C:134) every one of these is
+ * Loop's limit mask: L_v = (1 - n chi) r_v + chi sum(ring), chi = 1/(n +
+ * 3/(8 beta)) = 1/(2n).  Periodic meshes take the minimum image of each ring
+ * vertex relative to v. */
+void orc_limit_points(const omesh *m, const double *r, double *L)
+{
+    for (int v = 0; v < m->nv; v++) {
+        int n = m->val[v];
+        double s[3] = {0.0, 0.0, 0.0};
+        for (int e = 0; e < n; e++) {
+            int u = m->nb[v * OMAXV + e];
+            double dr[3], add[3];
+            for (int c = 0; c < 3; c++) dr[c] = r[3 * u + c] - r[3 * v + c];
+            min_image(dr, (double (*)[3])m->PBC, add);
+            for (int c = 0; c < 3; c++)
+                s[c] += r[3 * u + c] + add[0] * m->PBC[0][c] + add[1] * m->PBC[1][c] + add[2] * m->PBC[2][c];
+        }
+        double chi = 1.0 / (2.0 * n);
+        for (int c = 0; c < 3; c++) L[3 * v + c] = (1.0 - n * chi) * r[3 * v + c] + chi * s[c];
+    }
+}
+
+/* Enclosed volume of the polyhedron of limit points, one flat triangle per
+ * face (surface::volume at ni = 2, src/interpLib.C:997-1098; the in-box branch
+ * of new_volume, src/volume.C:350-430): r1 = owner corner (u,v) = (0,0), r2 =
+ * the u = 1 corner (ring[e]), r3 = the v = 1 corner (ring[e+1]); vol +=
+ * (r1 . n) A / 3 with n, A the unit normal and area of (r2 - r1) x (r3 - r1).
+ * Faces in plan order (owner ascending, ring position ascending).  The
+ * reference's box-centre shift cancels for a closed surface and is omitted. */
+double orc_volume(const omesh *m, const double *r)
+{
+    double *L = malloc(sizeof(double) * 3 * (size_t)m->nv);
+    orc_limit_points(m, r, L);
+    double vol = 0.0;
+    for (int i = 0; i < m->nv; i++) {
+        int val = m->val[i];
+        for (int e = 0; e < val; e++) {
+            int j = m->nb[i * OMAXV + e], k = m->nb[i * OMAXV + (e + 1) % val];
+            if (j < i || k < i) continue;
+            const double *r1 = L + 3 * i, *r2 = L + 3 * j, *r3 = L + 3 * k;
+            double d1[3] = {r2[0] - r1[0], r2[1] - r1[1], r2[2] - r1[2]};
+            double d2[3] = {r3[0] - r1[0], r3[1] - r1[1], r3[2] - r1[2]};
+            double nrm[3];
+            cross3(d1, d2, nrm);
+            double len = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+            double A = len / 2;
+            for (int c = 0; c < 3; c++) nrm[c] /= len;
+            double kk = r1[0] * nrm[0] + r1[1] * nrm[1] + r1[2] * nrm[2];
+            vol += kk * A / 3.0;
+        }
+    }
+    free(L);
+    return vol;
+}
+
+/* dV/dr (3 nv, overwritten): per face dV/dL_i = (L_j x L_k)/6 and cyclic
+ * (the closed form of the derivative src/volume.C:258-430 builds term by
+ * term), then through the limit mask: dV/dr_u = sum_v M_vu dV/dL_v */
+void orc_volume_grad(const omesh *m, const double *r, double *dV)
+{
+    int nv = m->nv;
+    double *L = malloc(sizeof(double) * 3 * (size_t)nv);
+    double *D = calloc(3 * (size_t)nv, sizeof(double));
+    orc_limit_points(m, r, L);
+    for (int i = 0; i < nv; i++) {
+        int val = m->val[i];
+        for (int e = 0; e < val; e++) {
+            int j = m->nb[i * OMAXV + e], k = m->nb[i * OMAXV + (e + 1) % val];
+            if (j < i || k < i) continue;
+            double t[3];
+            cross3(L + 3 * j, L + 3 * k, t);
+            for (int c = 0; c < 3; c++) D[3 * i + c] += t[c] / 6.0;
+            cross3(L + 3 * k, L + 3 * i, t);
+            for (int c = 0; c < 3; c++) D[3 * j + c] += t[c] / 6.0;
+            cross3(L + 3 * i, L + 3 * j, t);
+            for (int c = 0; c < 3; c++) D[3 * k + c] += t[c] / 6.0;
+        }
+    }
+    memset(dV, 0, sizeof(double) * 3 * (size_t)nv);
+    for (int v = 0; v < nv; v++) {
+        int n = m->val[v];
+        double chi = 1.0 / (2.0 * n);
+        for (int c = 0; c < 3; c++) dV[3 * v + c] += (1.0 - n * chi) * D[3 * v + c];
+        for (int e = 0; e < n; e++) {
+            int u = m->nb[v * OMAXV + e];
+            for (int c = 0; c < 3; c++) dV[3 * u + c] += chi * D[3 * v + c];
+        }
+    }
+    free(L);
+    free(D);
+}
+
+/* Restraint energy at r and the multipliers dE/dA_tot, dE/dV:
+ *   area   KA_g A0 ((A - A0)/A0)^2, A = sum dudv g w (src/interpLib.C:
+ *          4322-4326, the GLOBAL_AREA branch; A0 = sum dudv g0 w unless set)
+ *   volume KV/2 V0 ((V - V0)/V0)^2 (src/minimize.C:90-114; V0 = V at setg0
+ *          unless set, src/hd.C:273-279) */
+double orc_restraint_energy(omesh *m, const double *r, double *lamA, double *lamV)
+{
+    double e = 0.0;
+    if (lamA) *lamA = 0.0;
+    if (lamV) *lamV = 0.0;
+    if (m->KA_global != 0.0) {
+        double A, A0g;
+        orc_area(m, r, &A, &A0g);
+        double A0 = m->A0_target > 0.0 ? m->A0_target : m->A0_setg0;
+        double s = (A - A0) / A0;
+        e += m->KA_global * A0 * s * s;
+        if (lamA) *lamA = 2.0 * m->KA_global * s;
+    }
+    if (m->KV != 0.0) {
+        double V = orc_volume(m, r);
+        double V0 = m->V0_target != 0.0 ? m->V0_target : m->V_setg0;
+        double s = (V - V0) / V0;
+        e += 0.5 * m->KV * V0 * s * s;
+        if (lamV) *lamV = m->KV * s;
+    }
+    return e;
 }
 
 /* nsteps of r <- r - mobility_dt * dE/dr + noise_amp * xi_s; energies[s] is

@@ -45,6 +45,15 @@ void   orc_area(omesh *m, const double *r, double *A, double *A0);
 void   orc_bd_steps(omesh *m, double *r, const double *xi, int nsteps,
                     double mobility_dt, double noise_amp, double *energies);
 
+/* global restraints (area: src/interpLib.C:4322-4326; volume: src/minimize.C:
+ * 90-114 over src/interpLib.C:997-1098).  A0 <= 0: sum(dudv g0 w) of setg0;
+ * V0 == 0: the volume at setg0.  orc_energy and orc_grad include them. */
+void   orc_set_restraints(omesh *m, double KA_global, double KV, double A0, double V0);
+void   orc_limit_points(const omesh *m, const double *r, double *L);   /* 3 nv */
+double orc_volume(const omesh *m, const double *r);
+void   orc_volume_grad(const omesh *m, const double *r, double *dV);  /* 3 nv */
+double orc_restraint_energy(omesh *m, const double *r, double *lamA, double *lamV);
+
 /* introspection for tests */
 int    orc_face_info(const omesh *m, int f, int *owner, int *ncp, int *cp, int *tri);
 int    orc_face_table(const omesh *m, int f, int p, double *w6, double *weight,

@@ -1,0 +1,180 @@
+"""Pins the oracle's global restraints and the noise restatement (CPU only).
+
+* Limit points: the Loop limit mask (1/2 on the vertex, 1/(2n) on its ring)
+  is checked against the reference's own corner evaluation — the box-spline
+  basis at the three corners of every regular face and the Stam table at the
+  corners of every irregular face — so a wrong weight or a swapped corner
+  (u = 1 is ring[e], v = 1 is ring[e+1]) fails.
+* Volume: an independent triple-product sum over the faces, exact scaling
+  V(s r) = s^3 V(r), translation invariance, convergence to 4/3 pi R^3 with
+  subdivision, positive sign for the outward-oriented vesicles.
+* Gradients: Richardson finite differences of the volume, and of the full
+  energy with both restraints away from their targets (every term at once).
+* The restraints vanish at the setg0 state.
+* Noise: the Philox4x64-10 restatement against numpy's own Philox bit
+  generator, and the Box-Muller deviates' moments.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from umcp_amd import workloads
+
+import noise_ref
+
+
+@pytest.fixture(scope="module")
+def ves(mesh_dir):
+    w = workloads.vesicle_642()
+    s = O.OracleSurface(w.mesh_path)
+    s.set_params(**w.params)
+    s.setg0(w.r0)
+    return w, s
+
+
+def test_limit_points_match_reference_corner_evaluation(ves):
+    w, s = ves
+    L = s.limit_points(w.r).reshape(-1, 3)
+    P = w.r[:-3].reshape(-1, 3)
+    corners = {(0.0, 0.0): 0, (1.0, 0.0): 2, (0.0, 1.0): 1}   # slot of i, j = ring[e], k = ring[e+1]
+    B = {uv: O.basis(*uv)[0] for uv in corners}
+    nf_total = s.nfaces
+    checked_irr = 0
+    for f in range(nf_total):
+        info = s.face_info(f)
+        cp = np.array(info["cp"])
+        if info["ncp"] == 12:
+            for uv, slot in corners.items():
+                pos = B[uv] @ P[cp]
+                assert np.allclose(pos, L[cp[slot]], rtol=0, atol=1e-11), (f, uv)
+        else:
+            val = info["ncp"] - 6
+            for uv, slot in ((1.0, 0.0), 2), ((0.0, 1.0), 1):
+                pos = O.irregular_table(val, *uv)[0] @ P[cp]
+                assert np.allclose(pos, L[cp[slot]], rtol=0, atol=1e-10), (f, uv)
+            # the owner corner is the limit of the Stam evaluation as (u, v) -> 0
+            pos = O.irregular_table(val, 1e-9, 1e-9)[0] @ P[cp]
+            assert np.allclose(pos, L[cp[0]], rtol=0, atol=1e-6), f
+            checked_irr += 1
+    assert checked_irr == 60
+
+
+def _volume_triple(s, r):
+    """Independent: sum over faces of L_i . (L_j x L_k) / 6."""
+    L = s.limit_points(r).reshape(-1, 3)
+    V = 0.0
+    for f in range(s.nfaces):
+        cp = s.face_info(f)["cp"]
+        i, k, j = cp[0], cp[1], cp[2]
+        V += np.dot(L[i], np.cross(L[j], L[k])) / 6.0
+    return V
+
+
+def test_volume_invariants(ves):
+    w, s = ves
+    V = s.volume(w.r)
+    assert V > 0
+    assert abs(_volume_triple(s, w.r) - V) < 1e-12 * V
+    r2 = w.r.copy()
+    r2[:-3] *= 1.37
+    assert abs(s.volume(r2) - 1.37 ** 3 * V) < 1e-12 * V
+    r3 = w.r.copy()
+    r3[:-3].reshape(-1, 3)[:] += np.array([13.0, -7.0, 2.5])
+    assert abs(s.volume(r3) - V) < 1e-12 * V
+
+
+def test_volume_converges_to_sphere(mesh_dir):
+    R = 100.0
+    ratios = []
+    for level in (2, 3, 4):
+        w = workloads.vesicle(level, R, 0.0, seed=1, name="sph_L%d" % level)
+        s = O.OracleSurface(w.mesh_path)
+        ratios.append(s.volume(w.r0) / (4.0 / 3.0 * math.pi * R ** 3))
+    assert ratios[0] < ratios[1] < ratios[2] < 1.0
+    assert 1.0 - ratios[2] < 4.0 * (1.0 - ratios[1]) / 3.0
+    assert ratios[2] > 0.99
+
+
+def test_volume_gradient_fd(ves):
+    w, s = ves
+    d = s.volume_grad(w.r)
+    rng = np.random.default_rng(3)
+    scale = np.abs(d).max()
+    for i in list(range(36)) + list(rng.integers(0, 3 * s.nv, 40)):
+        # V is a cubic polynomial of each coordinate: Richardson's central
+        # difference is exact, so a large step only reduces rounding
+        h = 0.5
+
+        def fd(hh):
+            rp, rm = w.r.copy(), w.r.copy()
+            rp[i] += hh
+            rm[i] -= hh
+            return (s.volume(rp) - s.volume(rm)) / (2 * hh)
+        rich = (4 * fd(h / 2) - fd(h)) / 3
+        assert abs(rich - d[i]) < 1e-9 * scale, i
+
+
+def test_restraint_zero_at_setg0(ves):
+    w, s = ves
+    s.set_restraints(KA_global=0.3, KV=0.01)
+    e, la, lv = s.restraint_energy(w.r0)
+    assert abs(e) < 1e-20 and abs(la) < 1e-12 and abs(lv) < 1e-12
+    s.set_restraints()
+
+
+def test_restraint_energy_closed_form(ves):
+    w, s = ves
+    A, A0 = s.area(w.r)
+    V = s.volume(w.r)
+    s.set_restraints(KA_global=0.3, KV=0.01, A0=0.98 * A, V0=1.03 * V)
+    e, la, lv = s.restraint_energy(w.r)
+    sa, sv = (A - 0.98 * A) / (0.98 * A), (V - 1.03 * V) / (1.03 * V)
+    assert abs(e - (0.3 * 0.98 * A * sa * sa + 0.5 * 0.01 * 1.03 * V * sv * sv)) < 1e-10 * abs(e)
+    assert abs(la - 2 * 0.3 * sa) < 1e-14 and abs(lv - 0.01 * sv) < 1e-14
+    s.set_restraints()
+
+
+def test_fd_gradient_with_restraints(mesh_dir):
+    w = workloads.vesicle(2, 60.0, 0.8, seed=5, name="ves162_rs")
+    s = O.OracleSurface(w.mesh_path)
+    s.set_params(kc=5.0, kg=0.0, KA=0.2, k_reg=0.0, mode=O.MODE_HELFRICH)
+    s.set_face_c0(np.linspace(-0.02, 0.02, s.nfaces))
+    s.setg0(w.r0)
+    A, _ = s.area(w.r)
+    V = s.volume(w.r)
+    s.set_restraints(KA_global=0.5, KV=0.02, A0=0.97 * A, V0=1.05 * V)
+    g = s.grad(w.r)
+    scale = np.abs(g).max()
+    rng = np.random.default_rng(9)
+    for i in list(range(30)) + list(rng.integers(0, 3 * s.nv, 30)) + [3 * s.nv, 3 * s.nv + 1]:
+        h = 2e-3 if i < 3 * s.nv else 2e-6
+
+        def fd(hh):
+            rp, rm = w.r.copy(), w.r.copy()
+            rp[i] += hh
+            rm[i] -= hh
+            return (s.energy(rp) - s.energy(rm)) / (2 * hh)
+        rich = (4 * fd(h / 2) - fd(h)) / 3
+        assert abs(rich - g[i]) < 2e-7 * scale + 1e-8 * abs(g[i]), i
+
+
+def test_philox_matches_numpy():
+    for seed, v, step in ((12345, 7, 3), (2 ** 63 + 5, 0, 0), (1, 40961, 4999), (8, 0, 12)):
+        mine = noise_ref.philox((v, step, 0, 0), (seed, noise_ref.KEY1))
+        # numpy increments its (256-bit) counter before the first block
+        c = (v + (step << 64) - 1) % (1 << 256)
+        words = np.array([(c >> (64 * q)) & noise_ref.MASK for q in range(4)], dtype=np.uint64)
+        bg = np.random.Philox(key=np.array([seed, noise_ref.KEY1], dtype=np.uint64), counter=words)
+        assert [int(x) for x in bg.random_raw(4)] == mine
+
+
+def test_noise_moments():
+    z = noise_ref.normals(99, 5, 20000).reshape(-1, 3)
+    assert abs(z.mean()) < 0.02
+    assert abs(z.var() - 1.0) < 0.02
+    c = np.corrcoef(z.T)
+    assert np.all(np.abs(c - np.eye(3)) < 0.03)
+    z2 = noise_ref.normals(99, 6, 20000)
+    assert abs(np.corrcoef(z.reshape(-1), z2)[0, 1]) < 0.02

@@ -17,6 +17,7 @@
 #include "../../include/umcp.h"
 #include "surface_host.hpp"
 #include "umcp_kernels.hpp"
+#include "umcp_state.hpp"
 
 using namespace umcp;
 
@@ -24,49 +25,7 @@ namespace {
 
 thread_local int g_device = 0;
 
-struct ClassDev {
-    int *cp = nullptr, *pos = nullptr;
-    uint8_t *img = nullptr, *has_img = nullptr;
-    double *g0 = nullptr, *rurv0 = nullptr, *ig0 = nullptr, *Lo = nullptr, *Li = nullptr, *c0 = nullptr;
-};
-
 }  // namespace
-
-struct umcp_surface {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    Topology T;
-    Plan P;
-    umcp_params prm{};
-    bool prepared = false, planned = false, g0_set = false;
-    std::vector<double> face_c0;   // global face order
-    std::vector<double> face_Lo, face_Li;
-    std::vector<ClassDev> cls;
-    std::vector<void *> allocations;
-    // work buffers
-    double *d_r = nullptr, *d_g = nullptr, *d_gbuf = nullptr, *d_partial = nullptr;
-    float *d_gbuf32 = nullptr;
-    double *d_scalar = nullptr;    // [8]: E, sums[4], ...
-    int *d_vstart = nullptr;
-    int npartial = 0;
-    // launch shape of the face kernel: waves sharing 64 faces' Gauss points,
-    // and whether the faces' control points are staged in LDS
-    int point_split = 4;
-    int launch_flags = kLaunchOcc2;
-    std::vector<int> launch_order;  // classes in launch order (irregular first)
-    std::vector<int> block_start;
-    double *d_xi = nullptr;        // host-array BD staging, chunk of steps
-    int xi_chunk = 0;
-    double *d_energies = nullptr;
-    int energies_cap = 0;
-    // optional per-launch timing of the face kernels (HIP events on the
-    // stream each kernel is launched on)
-    bool prof_on = false;
-    std::vector<hipEvent_t> prof_events;  // pairs: start, end
-    std::vector<int> prof_class;          // class of each pair
-    size_t prof_used = 0;
-};
 
 namespace {
 
@@ -121,6 +80,12 @@ PhysParams phys(const umcp_surface *s)
     return ph;
 }
 
+}  // namespace
+
+namespace umcp {
+
+bool restraints_on(const umcp_surface *s) { return s->rsp.KA_global != 0.0 || s->rsp.KV != 0.0; }
+
 FaceArgs face_args(umcp_surface *s, size_t c, const double *d_r)
 {
     const FaceClass &C = s->P.classes[c];
@@ -144,7 +109,54 @@ FaceArgs face_args(umcp_surface *s, size_t c, const double *d_r)
     a.gbuf = s->d_gbuf;
     a.gbuf32 = s->d_gbuf32;
     a.ph = phys(s);
+    a.rs = restraints_on(s) ? s->d_rs : nullptr;
+    a.step_ctr = nullptr;
+    a.partial = s->d_partial;
     return a;
+}
+
+// restraint arguments on positions d_r; volume/area switched by the caller
+RestraintArgs restraint_args(umcp_surface *s, const double *d_r, bool area, bool volume)
+{
+    RestraintArgs ra{};
+    ra.r = d_r;
+    ra.nv = s->T.nv;
+    ra.nb_start = s->d_nb_start;
+    ra.nb = s->d_nb;
+    ra.Dh = s->d_Dh;
+    ra.Apart = s->d_Apart;
+    ra.Vpart = s->d_Vpart;
+    ra.nApart = area ? s->nApart : 0;
+    ra.nVpart = volume ? s->nVpart : 0;
+    ra.rs = s->d_rs;
+    ra.KA_g = s->rsp.KA_global;
+    ra.KV = s->rsp.KV;
+    ra.A0 = s->rsp.A0 > 0.0 ? s->rsp.A0 : s->A0_setg0;
+    ra.V0 = s->rsp.V0 != 0.0 ? s->rsp.V0 : s->V_setg0;
+    ra.do_area = area ? 1 : 0;
+    ra.do_volume = volume ? 1 : 0;
+    return ra;
+}
+
+}  // namespace umcp
+
+namespace {
+
+// area and/or volume partials of d_r, then the finalize into d_rs
+hipError_t enqueue_restraints(umcp_surface *s, const double *d_r, bool area, bool volume, hipStream_t st)
+{
+    RestraintLaunch L{};
+    L.ncls = (int)s->P.classes.size();
+    L.area_start[0] = 0;
+    for (int c = 0; c < L.ncls; c++) {
+        L.cls[c] = face_args(s, c, d_r);
+        L.kind[c] = kind_of(s->P.classes[c].ncp, s->P.classes[c].npt);
+        L.area_start[c + 1] = L.area_start[c] + (area ? s->area_start[c + 1] - s->area_start[c] : 0);
+    }
+    L.ra = restraint_args(s, d_r, area, volume);
+    hipError_t e = launch_restraints(L, st);
+    if (e == hipSuccess) e = launch_restraint_finalize(nullptr, L.ra, 1, st);
+    return e;
 }
 
 // irregular classes first so their few blocks start early and overlap the
@@ -179,13 +191,19 @@ int prof_pair(umcp_surface *s, int cls, hipEvent_t *ev)
     return UMCP_OK;
 }
 
-int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs ga)
+int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs ga, int *step_ctr = nullptr)
 {
+    const bool rson = restraints_on(s);
+    if (rson) {
+        hipError_t e = enqueue_restraints(s, d_r, s->rsp.KA_global != 0.0, s->rsp.KV != 0.0, st);
+        if (e != hipSuccess) return hip_fail(s, e, "restraint kernel launch");
+    }
     MultiFaceArgs m{};
     m.ncls = (int)s->launch_order.size();
     for (int i = 0; i < m.ncls; i++) {
         const int c = s->launch_order[i];
         m.cls[i] = face_args(s, c, d_r);
+        m.cls[i].step_ctr = step_ctr;
         m.kind[i] = kind_of(s->P.classes[c].ncp, s->P.classes[c].npt);
         m.block_start[i] = s->block_start[i];
     }
@@ -207,6 +225,14 @@ int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs 
     ga.gbuf32 = mixed ? s->d_gbuf32 : nullptr;
     ga.partial = s->d_partial;
     ga.npartial = s->npartial;
+    if (rson) {
+        ga.rs = s->d_rs;
+        if (s->rsp.KV != 0.0) {
+            ga.Dh = s->d_Dh;
+            ga.nb_start = s->d_nb_start;
+            ga.nb = s->d_nb;
+        }
+    }
     e = launch_vertex_gather(ga, st);
     if (e != hipSuccess) return hip_fail(s, e, "gather kernel launch");
     return UMCP_OK;
@@ -379,6 +405,31 @@ int umcp_plan(umcp_surface *s)
         (rc = dev_alloc(s, &s->d_vstart, s->P.vstart.size())))
         return rc;
     if ((rc = dev_upload(s, s->d_vstart, s->P.vstart.data(), s->P.vstart.size()))) return rc;
+    // restraint work: oriented rings as CSR, area blocks per class, partials
+    std::vector<int> nbs(nv + 1, 0), nbl;
+    for (int v = 0; v < nv; v++) {
+        for (int e2 = 0; e2 < s->T.valence[v]; e2++) nbl.push_back(s->T.nb(v, e2));
+        nbs[v + 1] = (int)nbl.size();
+    }
+    s->area_start.assign(1, 0);
+    s->has_images = false;
+    for (size_t c = 0; c < s->P.classes.size(); c++) {
+        const FaceClass &C = s->P.classes[c];
+        s->area_start.push_back(s->area_start.back() + (C.nf() + kRestraintBlock - 1) / kRestraintBlock);
+        for (uint8_t h : C.has_img) s->has_images = s->has_images || h;
+    }
+    s->nApart = s->area_start.back();
+    s->nVpart = (nv + kRestraintBlock - 1) / kRestraintBlock;
+    if ((rc = dev_alloc(s, &s->d_rs, 8)) || (rc = dev_alloc(s, &s->d_Dh, 3 * (size_t)nv)) ||
+        (rc = dev_alloc(s, &s->d_Apart, s->nApart)) || (rc = dev_alloc(s, &s->d_Vpart, s->nVpart)) ||
+        (rc = dev_alloc(s, &s->d_nb_start, nbs.size())) || (rc = dev_alloc(s, &s->d_nb, nbl.size())) ||
+        (rc = dev_alloc(s, &s->d_step, 1)))
+        return rc;
+    const int zero_step = 0;
+    const double zeros8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if ((rc = dev_upload(s, s->d_nb_start, nbs.data(), nbs.size())) || (rc = dev_upload(s, s->d_nb, nbl.data(), nbl.size())) ||
+        (rc = dev_upload(s, s->d_step, &zero_step, 1)) || (rc = dev_upload(s, s->d_rs, zeros8, 8)))
+        return rc;
     if ((rc = sync(s))) return rc;
     s->planned = true;
     return UMCP_OK;
@@ -415,6 +466,7 @@ int umcp_setg0(umcp_surface *s, const double *r)
     // single-lipid packing per triangle from the reference area of each face
     // (src/interpLib.C:3113-3196): add APL per leaflet until it exceeds A - 65/2
     const double apl = s->prm.lipid_apl;
+    double A0 = 0.0;
     for (size_t c = 0; c < s->P.classes.size(); c++) {
         const FaceClass &C = s->P.classes[c];
         const int nf = C.nf();
@@ -437,6 +489,21 @@ int umcp_setg0(umcp_surface *s, const double *r)
         }
         if ((rc = dev_upload(s, s->cls[c].Lo, Lo.data(), nf)) || (rc = dev_upload(s, s->cls[c].Li, Li.data(), nf)))
             return rc;
+        for (int p = 0; p < C.npt; p++)
+            for (int f = 0; f < nf; f++) A0 += C.dudv * g0[(size_t)p * nf + f] * C.weight[p];
+    }
+    s->A0_setg0 = A0;
+    // the reference records V0 right after setg0 on the same positions
+    // (src/hd.C:273-279)
+    s->V_setg0 = 0.0;
+    if (!s->has_images) {
+        hipError_t e = enqueue_restraints(s, s->d_r, false, true, s->stream);
+        if (e != hipSuccess) return hip_fail(s, e, "volume kernel launch");
+        double h[8];
+        e = hipMemcpyAsync(h, s->d_rs, sizeof h, hipMemcpyDeviceToHost, s->stream);
+        if (e != hipSuccess) return hip_fail(s, e, "download volume");
+        if ((rc = sync(s))) return rc;
+        s->V_setg0 = h[4];
     }
     if ((rc = sync(s))) return rc;
     s->g0_set = true;
@@ -523,6 +590,84 @@ int umcp_area(umcp_surface *s, const double *r, double *A, double *A0)
     if ((rc = sync(s))) return rc;
     *A = h[0];
     *A0 = h[1];
+    return UMCP_OK;
+}
+
+int umcp_set_restraints(umcp_surface *s, const umcp_restraints *rs)
+{
+    if (!s || !rs) return UMCP_ERR_ARG;
+    if (!(rs->KA_global >= 0.0) || !(rs->KV >= 0.0)) return fail(s, UMCP_ERR_ARG, "restraint constants must be >= 0");
+    if (!s->planned) return fail(s, UMCP_ERR_STATE, "plan has not been generated (call umcp_plan)");
+    if (rs->KV != 0.0 && s->has_images)
+        return fail(s, UMCP_ERR_ARG, "volume restraint needs a closed mesh (this one has periodic images)");
+    s->rsp = *rs;
+    return UMCP_OK;
+}
+
+int umcp_volume(umcp_surface *s, const double *r, double *V)
+{
+    int rc = check_ready(s, false);
+    if (rc) return rc;
+    if (!r || !V) return fail(s, UMCP_ERR_ARG, "null argument");
+    if (s->has_images) return fail(s, UMCP_ERR_ARG, "no enclosed volume: the mesh has periodic images");
+    if ((rc = upload_r(s, r))) return rc;
+    hipError_t e = enqueue_restraints(s, s->d_r, false, true, s->stream);
+    if (e != hipSuccess) return hip_fail(s, e, "volume kernel launch");
+    double h[8];
+    e = hipMemcpyAsync(h, s->d_rs, sizeof h, hipMemcpyDeviceToHost, s->stream);
+    if (e != hipSuccess) return hip_fail(s, e, "download volume");
+    if ((rc = sync(s))) return rc;
+    *V = h[4];
+    return UMCP_OK;
+}
+
+int umcp_set_step(umcp_surface *s, int step)
+{
+    int rc = check_ready(s, false);
+    if (rc) return rc;
+    if (step < 0) return fail(s, UMCP_ERR_ARG, "negative step");
+    if ((rc = dev_upload(s, s->d_step, &step, 1))) return rc;
+    return sync(s);
+}
+
+int umcp_get_step(umcp_surface *s, int *step)
+{
+    int rc = check_ready(s, false);
+    if (rc) return rc;
+    if (!step) return fail(s, UMCP_ERR_ARG, "null step");
+    hipError_t e = hipMemcpyAsync(step, s->d_step, sizeof(int), hipMemcpyDeviceToHost, s->stream);
+    if (e != hipSuccess) return hip_fail(s, e, "download step");
+    return sync(s);
+}
+
+int umcp_bd_steps_rng_dev(umcp_surface *s, double *d_r, int nsteps, const umcp_bd_params *bd,
+                          unsigned long long seed, double *d_energies, int energies_cap, void *stream)
+{
+    int rc = check_ready(s);
+    if (rc) return rc;
+    if (!d_r || !bd || nsteps < 0 || energies_cap < 0) return fail(s, UMCP_ERR_ARG, "bad Brownian arguments");
+    hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+    for (int k = 0; k < nsteps; k++) {
+        GatherArgs ga{};
+        ga.out_mode = 1;
+        ga.r = d_r;
+        ga.mobility_dt = bd->mobility_dt;
+        ga.noise_amp = bd->noise_amp;
+        ga.energies = d_energies;
+        ga.energies_cap = d_energies ? energies_cap : 0;
+        ga.noise = kNoisePhilox;
+        ga.seed = seed;
+        ga.step_ctr = s->d_step;
+        if ((rc = enqueue_eval(s, d_r, st, ga, s->d_step))) return rc;
+    }
+    return UMCP_OK;
+}
+
+int umcp_noise(unsigned long long seed, int step, int n, double *d_out, void *stream)
+{
+    if (!d_out || n < 0 || step < 0) return UMCP_ERR_ARG;
+    if (hipSetDevice(g_device) != hipSuccess) return UMCP_ERR_DEVICE;
+    if (launch_noise(seed, step, n, d_out, (hipStream_t)stream) != hipSuccess) return UMCP_ERR_DEVICE;
     return UMCP_OK;
 }
 

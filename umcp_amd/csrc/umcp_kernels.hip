@@ -130,7 +130,7 @@ constexpr int kFeatGeneral = 0, kFeatLipid = 1, kFeatHelfrich = 2;
 // reverse of this chain.  ig0 = 1/g0 of this point.
 template <typename R, int FEAT>
 __device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R ig0, R rurv0, R Lo, R Li,
-                                               R c0f, R dudv, const PhysParams &ph, R D[5][3])
+                                               R c0f, R dudv, R lamA, const PhysParams &ph, R D[5][3])
 {
     constexpr bool kAnyMode = FEAT == kFeatGeneral;
     constexpr bool kExtras = FEAT == kFeatGeneral;   // kg and k_reg terms
@@ -169,10 +169,11 @@ __device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R ig0
         e = R(0.5) * kc * dudv * g0w * hd * hd;
         dH = kc * dudv * g0w * hd;
     }
-    // per-point area strain (A - A0)/A0 with A = dudv g w, A0 = dudv g0 w
+    // per-point area strain (A - A0)/A0 with A = dudv g w, A0 = dudv g0 w;
+    // lamA = dE/dA_tot of the global area restraint (A_tot = sum dudv g w)
     const R strain = g * ig0 - R(1);
     e += KA * dudv * g0w * strain * strain;
-    const R dg = R(2) * KA * strain * dudv * w;
+    const R dg = (R(2) * KA * strain + lamA) * dudv * w;
 
     R dL = -dH * E22 * idet, dM = R(2) * dH * E12 * idet, dN = -dH * E11 * idet;
     R ddet = dH * T * idet * idet + dg * R(0.5) * ig;
@@ -295,6 +296,7 @@ __device__ __forceinline__ void face_group(const FaceArgs &a, int group, R *lds,
 #pragma unroll
     for (int c = 0; c < NCP; c++) G[c][0] = G[c][1] = G[c][2] = R(0);
     const R Lo = (R)a.Lo[fc], Li = (R)a.Li[fc], c0f = (R)a.c0[fc];
+    const R lamA = a.rs ? (R)a.rs[0] : R(0);
     double e = 0.0;
     for (int p = w; p < NPT; p += WS) {
         const R *T = tab + p * 5 * NCP;
@@ -320,7 +322,7 @@ __device__ __forceinline__ void face_group(const FaceArgs &a, int group, R *lds,
         R D[5][3];
         const R rr0 = FEAT == kFeatGeneral ? (R)a.rurv0[p * nf + fc] : R(0);
         e += (double)point_energy_grad<R, FEAT>(X, wts[p], (R)a.g0[p * nf + fc], (R)a.ig0[p * nf + fc], rr0,
-                                                Lo, Li, c0f, (R)a.dudv, a.ph, D);
+                                                Lo, Li, c0f, (R)a.dudv, lamA, a.ph, D);
 #pragma unroll
         for (int c = 0; c < NCP; c++) {
 #pragma unroll
@@ -397,6 +399,35 @@ __global__ __launch_bounds__(kWave *WS, OCC) void faces_kernel(MultiFaceArgs m)
         double *dst = m.partial + 4 * (size_t)b;
 #pragma unroll
         for (int i = 0; i < 4; i++) dst[i] = acc[i];
+        if (b == 0 && m.cls[0].step_ctr) m.cls[0].step_ctr[0] += 1;
+    }
+}
+
+// The same face groups for many surfaces: one item per 64-face group, read
+// from a device table.  grid < nitems makes the launch persistent (each block
+// strides over the items); every item runs exactly the code a launch of its
+// surface alone runs, and writes its partials to the same slot.
+template <typename R, int WS, bool PLDS, int OCC, int FEAT>
+__global__ __launch_bounds__(kWave *WS, OCC) void faces_batch_kernel(BatchFaceArgs B)
+{
+    __shared__ R lds[13 * 3 * kWave];
+    for (int it = xcd_logical_block(blockIdx.x, gridDim.x); it < B.nitems; it += gridDim.x) {
+        const FaceItem item = B.items[it];
+        const FaceArgs &a = B.fa[item.fa];
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        switch (item.kind) {
+        case 0: face_group<R, 12, 15, 0, WS, PLDS, FEAT>(a, item.group, lds, acc); break;
+        case 1: face_group<R, 11, 16, 1, WS, PLDS, FEAT>(a, item.group, lds, acc); break;
+        default: face_group<R, 13, 16, 2, WS, PLDS, FEAT>(a, item.group, lds, acc); break;
+        }
+        block_sum4<kWave * WS>(acc);
+        if (threadIdx.x == 0) {
+            double *dst = a.partial + 4 * (size_t)item.slot;
+#pragma unroll
+            for (int i = 0; i < 4; i++) dst[i] = acc[i];
+            if (item.slot == 0 && a.step_ctr) a.step_ctr[0] += 1;
+        }
+        __syncthreads();   // LDS and the reduction scratch are reused by the next item
     }
 }
 
@@ -462,36 +493,104 @@ __global__ __launch_bounds__(kMetricBlock) void face_metric(FaceArgs a, int mode
     }
 }
 
-// Vertex-owned gather over the face->vertex CSR (no atomics, fixed order), and
-// in block 0 the fixed-order reduction of the face-kernel partials.
+// ---------------------------------------------------------------------------
+// Counter-based normal deviates (Philox4x64-10, Salmon, Moraes, Dror & Shaw,
+// SC'11 "Parallel random numbers: as easy as 1, 2, 3"; Box-Muller transform).
+// The deviates of (seed, step, vertex) do not depend on how many surfaces
+// share a launch or on the launch shape.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void philox4x64_10(unsigned long long x[4], unsigned long long k0,
+                                              unsigned long long k1)
+{
+    constexpr unsigned long long kM0 = 0xD2E7470EE14C6C93ULL, kM1 = 0xCA5A826395121157ULL;
+    constexpr unsigned long long kW0 = 0x9E3779B97F4A7C15ULL, kW1 = 0xBB67AE8584CAA73BULL;
+#pragma unroll
+    for (int round = 0; round < 10; round++) {
+        const unsigned long long hi0 = __umul64hi(kM0, x[0]), lo0 = kM0 * x[0];
+        const unsigned long long hi1 = __umul64hi(kM1, x[2]), lo1 = kM1 * x[2];
+        const unsigned long long y0 = hi1 ^ x[1] ^ k0, y2 = hi0 ^ x[3] ^ k1;
+        x[0] = y0;
+        x[1] = lo1;
+        x[2] = y2;
+        x[3] = lo0;
+        k0 += kW0;
+        k1 += kW1;
+    }
+}
+
+constexpr unsigned long long kNoiseKey1 = 0x554D4350ULL;   // "UMCP"
+
+// three N(0,1) deviates for (seed, step, vertex v)
+__device__ __forceinline__ void normal3(unsigned long long seed, int step, int v, double z[3])
+{
+    unsigned long long x[4] = {(unsigned long long)v, (unsigned long long)(unsigned)step, 0ULL, 0ULL};
+    philox4x64_10(x, seed, kNoiseKey1);
+    constexpr double k53 = 1.0 / 9007199254740992.0;   // 2^-53
+    const double u0 = ((double)(x[0] >> 11) + 0.5) * k53, u1 = ((double)(x[1] >> 11) + 0.5) * k53;
+    const double u2 = ((double)(x[2] >> 11) + 0.5) * k53, u3 = ((double)(x[3] >> 11) + 0.5) * k53;
+    constexpr double kTwoPi = 6.283185307179586476925286766559;
+    const double rho0 = sqrt(-2.0 * log(u0)), rho1 = sqrt(-2.0 * log(u2));
+    z[0] = rho0 * cos(kTwoPi * u1);
+    z[1] = rho0 * sin(kTwoPi * u1);
+    z[2] = rho1 * cos(kTwoPi * u3);
+}
+
+__global__ __launch_bounds__(kGatherBlock) void noise_kernel(unsigned long long seed, int step, int n, double *out)
+{
+    const int v = blockIdx.x * kGatherBlock + threadIdx.x;
+    if (v >= n) return;
+    double z[3];
+    normal3(seed, step, v, z);
+    out[3 * (size_t)v] = z[0];
+    out[3 * (size_t)v + 1] = z[1];
+    out[3 * (size_t)v + 2] = z[2];
+}
+
+// ---------------------------------------------------------------------------
+// Vertex-owned gather over the face->vertex CSR (no atomics, fixed order).
 //   out_mode 0: write g (3 nv) and dE/dalpha into g[3nv..3nv+2]
 //   out_mode 1: Brownian update of r, no g
 //   out_mode 2: reductions only
-template <typename GB>
-__global__ __launch_bounds__(kGatherBlock) void vertex_gather(GatherArgs a)
+// The block that owns vertex block 0 of a surface also reduces that surface's
+// face-kernel partials in fixed order (energy, dE/dalpha) and adds the
+// restraint energy.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int philox_step(const GatherArgs &a)
 {
-    if (blockIdx.x == 0) {
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int i = threadIdx.x; i < a.npartial; i += kGatherBlock)
+    return a.step_ctr ? a.step_ctr[0] - 1 : a.step;
+}
+
+__device__ __forceinline__ void gather_reduce(const GatherArgs &a)
+{
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < a.npartial; i += kGatherBlock)
 #pragma unroll
-            for (int q = 0; q < 4; q++) acc[q] += a.partial[4 * (size_t)i + q];
-        block_sum4<kGatherBlock>(acc);
-        if (threadIdx.x == 0) {
-            if (a.E_out) a.E_out[0] = acc[0];
-            if (a.energies) a.energies[a.step] = acc[0];
-            if (a.out_mode == 0 && a.g) {
-                a.g[3 * a.nv + 0] = acc[1];
-                a.g[3 * a.nv + 1] = acc[2];
-                a.g[3 * a.nv + 2] = acc[3];
-            }
-            if (a.sums) {
+        for (int q = 0; q < 4; q++) acc[q] += a.partial[4 * (size_t)i + q];
+    block_sum4<kGatherBlock>(acc);
+    if (threadIdx.x == 0) {
+        if (a.rs) acc[0] += a.rs[2];
+        if (a.E_out) a.E_out[0] = acc[0];
+        if (a.energies) {
+            const int step = a.noise == kNoisePhilox ? philox_step(a) : a.step;
+            if (a.noise != kNoisePhilox || (step >= 0 && step < a.energies_cap)) a.energies[step] = acc[0];
+        }
+        if (a.out_mode == 0 && a.g) {
+            a.g[3 * a.nv + 0] = acc[1];
+            a.g[3 * a.nv + 1] = acc[2];
+            a.g[3 * a.nv + 2] = acc[3];
+        }
+        if (a.sums) {
 #pragma unroll
-                for (int q = 0; q < 4; q++) a.sums[q] = acc[q];
-            }
+            for (int q = 0; q < 4; q++) a.sums[q] = acc[q];
         }
     }
+}
+
+template <typename GB>
+__device__ __forceinline__ void gather_vertices(const GatherArgs &a, int vblock)
+{
     if (a.out_mode == 2) return;
-    const int v = blockIdx.x * kGatherBlock + threadIdx.x;
+    const int v = vblock * kGatherBlock + threadIdx.x;
     if (v >= a.nv) return;
     const int s0 = a.vstart[v], s1 = a.vstart[v + 1];
     double g0 = 0.0, g1 = 0.0, g2 = 0.0;
@@ -501,17 +600,253 @@ __global__ __launch_bounds__(kGatherBlock) void vertex_gather(GatherArgs a)
         g1 += (double)src[1];
         g2 += (double)src[2];
     }
+    if (a.Dh) {
+        // volume restraint: dE/dV * dV/dr_v = lamV (n_v Dh_v + sum_ring Dh_u),
+        // Dh = (dV/dL)/(2n) (the limit mask has weight 1/2 on v, 1/(2n_u) on
+        // the ring of u)
+        const double lamV = a.rs[1];
+        const int b0 = a.nb_start[v], b1 = a.nb_start[v + 1];
+        const double nvv = (double)(b1 - b0);
+        double d0 = nvv * a.Dh[3 * (size_t)v], d1 = nvv * a.Dh[3 * (size_t)v + 1], d2 = nvv * a.Dh[3 * (size_t)v + 2];
+        for (int q = b0; q < b1; q++) {
+            const int u = a.nb[q];
+            d0 += a.Dh[3 * (size_t)u];
+            d1 += a.Dh[3 * (size_t)u + 1];
+            d2 += a.Dh[3 * (size_t)u + 2];
+        }
+        g0 += lamV * d0;
+        g1 += lamV * d1;
+        g2 += lamV * d2;
+    }
     if (a.out_mode == 0) {
         a.g[3 * v] = g0;
         a.g[3 * v + 1] = g1;
         a.g[3 * v + 2] = g2;
     } else {
-        const double *xi = a.xi + 3 * (size_t)v;
+        double z[3];
+        if (a.noise == kNoisePhilox) {
+            normal3(a.seed, philox_step(a), v, z);
+        } else {
+            const double *xi = a.xi + 3 * (size_t)v;
+            z[0] = xi[0];
+            z[1] = xi[1];
+            z[2] = xi[2];
+        }
         double *r = a.r + 3 * (size_t)v;
-        r[0] += -a.mobility_dt * g0 + a.noise_amp * xi[0];
-        r[1] += -a.mobility_dt * g1 + a.noise_amp * xi[1];
-        r[2] += -a.mobility_dt * g2 + a.noise_amp * xi[2];
+        r[0] += -a.mobility_dt * g0 + a.noise_amp * z[0];
+        r[1] += -a.mobility_dt * g1 + a.noise_amp * z[1];
+        r[2] += -a.mobility_dt * g2 + a.noise_amp * z[2];
     }
+}
+
+template <typename GB>
+__global__ __launch_bounds__(kGatherBlock) void vertex_gather(GatherArgs a)
+{
+    if (blockIdx.x == 0) gather_reduce(a);
+    gather_vertices<GB>(a, blockIdx.x);
+}
+
+template <typename GB>
+__global__ __launch_bounds__(kGatherBlock) void vertex_gather_batch(BatchGatherArgs B)
+{
+    for (int it = blockIdx.x; it < B.nitems; it += gridDim.x) {
+        const GatherItem item = B.items[it];
+        const GatherArgs &a = B.ga[item.surf];
+        if (item.vblock == 0) gather_reduce(a);
+        gather_vertices<GB>(a, item.vblock);
+        __syncthreads();   // the reduction scratch is reused by the next item
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Global restraints (area: the GLOBAL_AREA form of src/interpLib.C:4322-4326;
+// volume: src/minimize.C:90-114 over the limit-point polyhedron of
+// src/interpLib.C:997-1098 / src/volume.C:350-430).
+// ---------------------------------------------------------------------------
+
+// sum over the Gauss points of face f of dudv g w (alpha-scaled, as the
+// energy's g)
+template <int NCP, int NPT, int KIND>
+__device__ __forceinline__ double face_area(const FaceArgs &a, int f)
+{
+    const int nf = a.nf;
+    const double al[3] = {a.r[3 * a.nv], a.r[3 * a.nv + 1], a.r[3 * a.nv + 2]};
+    double P[NCP][3];
+#pragma unroll
+    for (int c = 0; c < NCP; c++) {
+        const int v = a.cp[c * nf + f];
+        double o[3] = {0.0, 0.0, 0.0};
+        if (a.has_img[f]) image_offset(a.img[c * nf + f], a.pbc, o);
+#pragma unroll
+        for (int k = 0; k < 3; k++) P[c][k] = a.r[3 * v + k] + o[k];
+    }
+    double A = 0.0;
+    for (int p = 0; p < NPT; p++) {
+        const double *T = kind_table<KIND, double>() + p * 5 * NCP;
+        const double wp = kind_table<KIND, double>()[NPT * 5 * NCP + p];
+        double Ru[3], Rv[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            double su = 0.0, sv = 0.0;
+#pragma unroll
+            for (int c = 0; c < NCP; c++) {
+                su += T[c * 5 + 0] * P[c][k];
+                sv += T[c * 5 + 1] * P[c][k];
+            }
+            Ru[k] = su * al[k];
+            Rv[k] = sv * al[k];
+        }
+        const double E11 = Ru[0] * Ru[0] + Ru[1] * Ru[1] + Ru[2] * Ru[2];
+        const double E12 = Ru[0] * Rv[0] + Ru[1] * Rv[1] + Ru[2] * Rv[2];
+        const double E22 = Rv[0] * Rv[0] + Rv[1] * Rv[1] + Rv[2] * Rv[2];
+        A += a.dudv * sqrt(E11 * E22 - E12 * E12) * wp;
+    }
+    return A;
+}
+
+__device__ __forceinline__ double area_block(const FaceArgs &a, int kind, int blk)
+{
+    const int f = blk * kRestraintBlock + threadIdx.x;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (f < a.nf) {
+        switch (kind) {
+        case 0: acc[0] = face_area<12, 15, 0>(a, f); break;
+        case 1: acc[0] = face_area<11, 16, 1>(a, f); break;
+        default: acc[0] = face_area<13, 16, 2>(a, f); break;
+        }
+    }
+    block_sum4<kRestraintBlock>(acc);
+    return acc[0];
+}
+
+// Loop limit position of vertex u: r_u / 2 + sum(ring) / (2 n)
+__device__ __forceinline__ void limit_point(const RestraintArgs &ra, int u, double L[3])
+{
+    const int b0 = ra.nb_start[u], b1 = ra.nb_start[u + 1];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int q = b0; q < b1; q++) {
+        const int w = ra.nb[q];
+        s0 += ra.r[3 * (size_t)w];
+        s1 += ra.r[3 * (size_t)w + 1];
+        s2 += ra.r[3 * (size_t)w + 2];
+    }
+    const double chi = 0.5 / (double)(b1 - b0);
+    L[0] = 0.5 * ra.r[3 * (size_t)u] + chi * s0;
+    L[1] = 0.5 * ra.r[3 * (size_t)u + 1] + chi * s1;
+    L[2] = 0.5 * ra.r[3 * (size_t)u + 2] + chi * s2;
+}
+
+// Per vertex v: D_v = dV/dL_v = (1/6) sum_e L_ring[e] x L_ring[e+1] (every
+// face around v, oriented), Dh_v = D_v / (2 n_v); block sum of L_v . D_v / 3
+// (each face's triple product is counted once per corner).
+__device__ __forceinline__ double volume_block(const RestraintArgs &ra, int blk)
+{
+    const int v = blk * kRestraintBlock + threadIdx.x;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (v < ra.nv) {
+        const int b0 = ra.nb_start[v], b1 = ra.nb_start[v + 1];
+        double Lv[3], L0[3], Lp[3], D[3] = {0.0, 0.0, 0.0};
+        limit_point(ra, v, Lv);
+        limit_point(ra, ra.nb[b0], L0);
+        Lp[0] = L0[0];
+        Lp[1] = L0[1];
+        Lp[2] = L0[2];
+        for (int q = b0 + 1; q <= b1; q++) {
+            double Lq[3];
+            if (q < b1) limit_point(ra, ra.nb[q], Lq);
+            else {
+                Lq[0] = L0[0];
+                Lq[1] = L0[1];
+                Lq[2] = L0[2];
+            }
+            D[0] += Lp[1] * Lq[2] - Lp[2] * Lq[1];
+            D[1] += Lp[2] * Lq[0] - Lp[0] * Lq[2];
+            D[2] += Lp[0] * Lq[1] - Lp[1] * Lq[0];
+            Lp[0] = Lq[0];
+            Lp[1] = Lq[1];
+            Lp[2] = Lq[2];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) D[k] *= (1.0 / 6.0);
+        const double h = 0.5 / (double)(b1 - b0);
+        ra.Dh[3 * (size_t)v] = D[0] * h;
+        ra.Dh[3 * (size_t)v + 1] = D[1] * h;
+        ra.Dh[3 * (size_t)v + 2] = D[2] * h;
+        acc[0] = (Lv[0] * D[0] + Lv[1] * D[1] + Lv[2] * D[2]) * (1.0 / 3.0);
+    }
+    block_sum4<kRestraintBlock>(acc);
+    return acc[0];
+}
+
+__global__ __launch_bounds__(kRestraintBlock) void restraint_kernel(RestraintLaunch L)
+{
+    const int b = blockIdx.x;
+    const int na = L.area_start[L.ncls];
+    if (b < na) {
+        int c = 0;
+        while (c + 1 < L.ncls && b >= L.area_start[c + 1]) c++;
+        const double A = area_block(L.cls[c], L.kind[c], b - L.area_start[c]);
+        if (threadIdx.x == 0) L.ra.Apart[b] = A;
+    } else {
+        const double V = volume_block(L.ra, b - na);
+        if (threadIdx.x == 0) L.ra.Vpart[b - na] = V;
+    }
+}
+
+// batched restraint items: type 0..2 = area block of a class of that kind
+// (index = FaceArgs entry), type 3 = volume block (index = surface)
+__global__ __launch_bounds__(kRestraintBlock) void restraint_batch_kernel(BatchRestraintArgs B)
+{
+    for (int it = blockIdx.x; it < B.nitems; it += gridDim.x) {
+        const RestraintItem item = B.items[it];
+        if (item.type < 3) {
+            const FaceArgs &a = B.fa[item.index];
+            const RestraintArgs &ra = B.ra[B.fa_surface[item.index]];
+            const double A = area_block(a, item.type, item.block);
+            if (threadIdx.x == 0) ra.Apart[item.slot] = A;
+        } else {
+            const RestraintArgs &ra = B.ra[item.index];
+            const double V = volume_block(ra, item.block);
+            if (threadIdx.x == 0) ra.Vpart[item.block] = V;
+        }
+        __syncthreads();   // the reduction scratch is reused by the next item
+    }
+}
+
+// one block per surface: fixed-order sums of the area and volume partials,
+// then the restraint energy and its multipliers
+__device__ __forceinline__ void restraint_finish(const RestraintArgs &ra)
+{
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (ra.do_area)
+        for (int i = threadIdx.x; i < ra.nApart; i += kRestraintBlock) acc[0] += ra.Apart[i];
+    if (ra.do_volume)
+        for (int i = threadIdx.x; i < ra.nVpart; i += kRestraintBlock) acc[1] += ra.Vpart[i];
+    block_sum4<kRestraintBlock>(acc);
+    if (threadIdx.x == 0) {
+        double lamA = 0.0, lamV = 0.0, E = 0.0;
+        if (ra.do_area) {
+            const double s = (acc[0] - ra.A0) / ra.A0;
+            E += ra.KA_g * ra.A0 * s * s;
+            lamA = 2.0 * ra.KA_g * s;
+        }
+        if (ra.do_volume) {
+            const double s = (acc[1] - ra.V0) / ra.V0;
+            E += 0.5 * ra.KV * ra.V0 * s * s;
+            lamV = ra.KV * s;
+        }
+        ra.rs[0] = lamA;
+        ra.rs[1] = lamV;
+        ra.rs[2] = E;
+        ra.rs[3] = acc[0];
+        ra.rs[4] = acc[1];
+    }
+}
+
+__global__ __launch_bounds__(kRestraintBlock) void restraint_finalize(const RestraintArgs *table, RestraintArgs one)
+{
+    if (table) restraint_finish(table[blockIdx.x]);
+    else restraint_finish(one);
 }
 
 // ---------------------------------------------------------------------------
@@ -567,9 +902,16 @@ static hipError_t launch_ws(const MultiFaceArgs &a, int nb, hipStream_t st)
     return hipGetLastError();
 }
 
+template <typename R, int WS, int OCC, int FEAT>
+static hipError_t launch_ws(const BatchFaceArgs &a, int nb, hipStream_t st)
+{
+    hipLaunchKernelGGL((faces_batch_kernel<R, WS, true, OCC, FEAT>), dim3(nb), dim3(kWave * WS), 0, st, a);
+    return hipGetLastError();
+}
+
 // the register-bound flag asks for 2 waves per SIMD in FP64 and 4 in FP32
-template <typename R, int FEAT>
-static hipError_t launch_shape(const MultiFaceArgs &a, int nb, int ws, int occ2, hipStream_t st)
+template <typename R, int FEAT, typename Args>
+static hipError_t launch_shape(const Args &a, int nb, int ws, int occ2, hipStream_t st)
 {
     constexpr int kB = sizeof(R) == 4 ? 4 : 2;
     switch (ws * 2 + (occ2 ? 1 : 0)) {
@@ -585,8 +927,8 @@ static hipError_t launch_shape(const MultiFaceArgs &a, int nb, int ws, int occ2,
     }
 }
 
-template <typename R>
-static hipError_t launch_prec(const MultiFaceArgs &a, int nb, int ws, int occ2, int feat, hipStream_t st)
+template <typename R, typename Args>
+static hipError_t launch_prec(const Args &a, int nb, int ws, int occ2, int feat, hipStream_t st)
 {
     switch (feat) {
     case kFeatLipid: return launch_shape<R, kFeatLipid>(a, nb, ws, occ2, st);
@@ -631,14 +973,64 @@ hipError_t launch_face_metric(const FaceArgs &a, int kind, int mode, double *g0_
     return hipGetLastError();
 }
 
+int gather_blocks(int nv) { return nv > 0 ? (nv + kGatherBlock - 1) / kGatherBlock : 1; }
+
 hipError_t launch_vertex_gather(const GatherArgs &a, hipStream_t st)
 {
-    int nb = (a.out_mode == 2) ? 1 : (a.nv + kGatherBlock - 1) / kGatherBlock;
-    if (nb < 1) nb = 1;
+    const int nb = (a.out_mode == 2) ? 1 : gather_blocks(a.nv);
     if (a.gbuf32)
         hipLaunchKernelGGL(vertex_gather<float>, dim3(nb), dim3(kGatherBlock), 0, st, a);
     else
         hipLaunchKernelGGL(vertex_gather<double>, dim3(nb), dim3(kGatherBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_faces_batch(const BatchFaceArgs &a, int grid, int ws, int flags, bool mixed, int feat,
+                              hipStream_t st)
+{
+    if (a.nitems <= 0) return hipSuccess;
+    const int nb = (grid > 0 && grid < a.nitems) ? grid : a.nitems;
+    const int occ2 = (flags & kLaunchOcc2) ? 1 : 0;
+    return mixed ? launch_prec<float>(a, nb, ws, occ2, feat, st) : launch_prec<double>(a, nb, ws, occ2, feat, st);
+}
+
+hipError_t launch_gather_batch(const BatchGatherArgs &a, int grid, bool mixed, hipStream_t st)
+{
+    if (a.nitems <= 0) return hipSuccess;
+    const int nb = (grid > 0 && grid < a.nitems) ? grid : a.nitems;
+    if (mixed)
+        hipLaunchKernelGGL(vertex_gather_batch<float>, dim3(nb), dim3(kGatherBlock), 0, st, a);
+    else
+        hipLaunchKernelGGL(vertex_gather_batch<double>, dim3(nb), dim3(kGatherBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_restraints(const RestraintLaunch &a, hipStream_t st)
+{
+    const int nb = a.area_start[a.ncls] + a.ra.nVpart;
+    if (nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(restraint_kernel, dim3(nb), dim3(kRestraintBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_restraints_batch(const BatchRestraintArgs &a, hipStream_t st)
+{
+    if (a.nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(restraint_batch_kernel, dim3(a.nitems), dim3(kRestraintBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_restraint_finalize(const RestraintArgs *table, const RestraintArgs &one, int nsurf, hipStream_t st)
+{
+    if (nsurf <= 0) return hipSuccess;
+    hipLaunchKernelGGL(restraint_finalize, dim3(nsurf), dim3(kRestraintBlock), 0, st, table, one);
+    return hipGetLastError();
+}
+
+hipError_t launch_noise(unsigned long long seed, int step, int n, double *out, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(noise_kernel, dim3(gather_blocks(n)), dim3(kGatherBlock), 0, st, seed, step, n, out);
     return hipGetLastError();
 }
 

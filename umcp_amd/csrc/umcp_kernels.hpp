@@ -38,6 +38,9 @@ struct FaceArgs {
     double *gbuf;           // [nnz][3] per-(face, slot) gradient, CSR order (FP64 path)
     float *gbuf32;          // [nnz][3] the same for the mixed path
     PhysParams ph;
+    const double *rs;       // restraint scalars (rs[0] = dE/dA_tot) or null
+    int *step_ctr;          // Brownian step counter, advanced once per evaluation, or null
+    double *partial;        // the surface's block partials (batched launches)
 };
 
 // All face classes of a surface in one launch: blocks [block_start[c],
@@ -50,6 +53,24 @@ struct MultiFaceArgs {
     int ncls;
     double *partial;        // [nblocks][4]: energy, dE/dalpha
 };
+
+// One work item of a batched face launch: 64 faces of one class of one
+// surface.  slot = the block index the same group has in the surface's own
+// launch, so its partial sums land where a launch of that surface alone puts
+// them (identical reduction order alone and in a batch).
+struct FaceItem {
+    int fa, group, slot, kind;
+};
+
+struct BatchFaceArgs {
+    const FaceArgs *fa;     // device array, one per (surface, class)
+    const FaceItem *items;  // device array
+    int nitems;
+};
+
+// Counter-based normal deviates: Philox4x64-10 keyed by (seed, 'UMCP') at
+// counter (vertex, step, 0, 0), Box-Muller on the first four outputs.
+constexpr int kNoiseArray = 0, kNoisePhilox = 1;
 
 struct GatherArgs {
     int nv;
@@ -67,6 +88,65 @@ struct GatherArgs {
     double *energies;       // may be null: energies[step] = E
     int step;
     double *sums;           // may be null: the four reduced partial sums
+    // global restraints: rs[0..4] = dE/dA, dE/dV, E_restraint, A, V (null: none);
+    // Dh[3nv] = (dV/dL_v) / (2 n_v); the oriented rings in CSR form
+    const double *rs;
+    const double *Dh;
+    const int *nb_start, *nb;
+    // noise source of out_mode 1
+    int noise;              // kNoiseArray (xi) or kNoisePhilox
+    unsigned long long seed;
+    const int *step_ctr;    // Philox: the step is *step_ctr - 1 (advanced by the face kernel)
+    int energies_cap;       // energies[step] is written while step < energies_cap (Philox)
+};
+
+struct GatherItem {
+    int surf, vblock;
+};
+
+struct BatchGatherArgs {
+    const GatherArgs *ga;   // device array, one per surface
+    const GatherItem *items;
+    int nitems;
+};
+
+// Global area / volume restraints of one surface.
+struct RestraintArgs {
+    const double *r;
+    int nv;
+    const int *nb_start, *nb;
+    double *Dh;             // [3nv] out
+    double *Apart;          // [nApart] sum(dudv g w) per area block
+    double *Vpart;          // [nVpart] volume per vertex block
+    int nApart, nVpart;
+    double *rs;             // [5] out: dE/dA, dE/dV, E, A, V
+    double KA_g, KV, A0, V0;
+    int do_area, do_volume;
+};
+
+// restraint work: blocks [0, area_start[ncls]) evaluate the area of 256
+// faces each (class c from area_start), then nVpart volume blocks
+struct RestraintLaunch {
+    FaceArgs cls[kMaxClasses];
+    int kind[kMaxClasses];
+    int area_start[kMaxClasses + 1];
+    int ncls;
+    RestraintArgs ra;
+};
+
+// batched restraint work item: type 0..2 = area block of a class of that
+// kind (index = FaceArgs entry, slot = area partial of its surface), type 3 =
+// volume block (index = RestraintArgs entry)
+struct RestraintItem {
+    int type, index, block, slot;
+};
+
+struct BatchRestraintArgs {
+    const FaceArgs *fa;
+    const RestraintArgs *ra;   // one per surface
+    const int *fa_surface;     // FaceArgs entry -> RestraintArgs entry
+    const RestraintItem *items;
+    int nitems;
 };
 
 int kind_of(int ncp, int npt);
@@ -86,5 +166,23 @@ hipError_t launch_face_metric(const FaceArgs &a, int kind, int mode, double *g0_
                               hipStream_t st);
 int metric_blocks(int nf);
 hipError_t launch_vertex_gather(const GatherArgs &a, hipStream_t st);
+int gather_blocks(int nv);
+
+// restraints: area + volume partials, then the finalize (one block per surface)
+constexpr int kRestraintBlock = 256;
+hipError_t launch_restraints(const RestraintLaunch &a, hipStream_t st);
+hipError_t launch_restraint_finalize(const RestraintArgs *ra_dev_or_null, const RestraintArgs &ra, int nsurf,
+                                     hipStream_t st);
+
+// batched launches over many surfaces (grid = 0: one block per item, else a
+// persistent grid of that many blocks striding over the items)
+hipError_t launch_faces_batch(const BatchFaceArgs &a, int grid, int point_split, int flags, bool mixed, int feat,
+                              hipStream_t st);
+hipError_t launch_gather_batch(const BatchGatherArgs &a, int grid, bool mixed, hipStream_t st);
+hipError_t launch_restraints_batch(const BatchRestraintArgs &a, hipStream_t st);
+
+// 3 n normal deviates of (seed, step) for vertices [0, n) (the stream the
+// Brownian update draws from)
+hipError_t launch_noise(unsigned long long seed, int step, int n, double *out, hipStream_t st);
 
 }  // namespace umcp

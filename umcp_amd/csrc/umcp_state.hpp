@@ -1,0 +1,79 @@
+// Internal state of a surface handle, shared by the C-ABI implementation
+// (umcp_api.cpp) and the batched ensemble (umcp_ensemble.cpp).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/umcp.h"
+#include "surface_host.hpp"
+#include "umcp_kernels.hpp"
+
+namespace umcp {
+
+struct ClassDev {
+    int *cp = nullptr, *pos = nullptr;
+    uint8_t *img = nullptr, *has_img = nullptr;
+    double *g0 = nullptr, *rurv0 = nullptr, *ig0 = nullptr, *Lo = nullptr, *Li = nullptr, *c0 = nullptr;
+};
+
+}  // namespace umcp
+
+struct umcp_surface {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    umcp::Topology T;
+    umcp::Plan P;
+    umcp_params prm{};
+    bool prepared = false, planned = false, g0_set = false;
+    std::vector<double> face_c0;   // global face order
+    std::vector<double> face_Lo, face_Li;
+    std::vector<umcp::ClassDev> cls;
+    std::vector<void *> allocations;
+    // work buffers
+    double *d_r = nullptr, *d_g = nullptr, *d_gbuf = nullptr, *d_partial = nullptr;
+    float *d_gbuf32 = nullptr;
+    double *d_scalar = nullptr;    // [8]: E, sums[4], ...
+    int *d_vstart = nullptr;
+    int npartial = 0;
+    // launch shape of the face kernel: waves sharing 64 faces' Gauss points,
+    // and whether the faces' control points are staged in LDS
+    int point_split = 4;
+    int launch_flags = umcp::kLaunchOcc2;
+    std::vector<int> launch_order;  // classes in launch order (irregular first)
+    std::vector<int> block_start;
+    double *d_xi = nullptr;        // host-array BD staging, chunk of steps
+    int xi_chunk = 0;
+    double *d_energies = nullptr;
+    int energies_cap = 0;
+    // optional per-launch timing of the face kernels (HIP events on the
+    // stream each kernel is launched on)
+    bool prof_on = false;
+    std::vector<hipEvent_t> prof_events;  // pairs: start, end
+    std::vector<int> prof_class;          // class of each pair
+    size_t prof_used = 0;
+    // global area / volume restraints
+    umcp_restraints rsp{};
+    bool has_images = false;              // periodic mesh: no enclosed volume
+    double A0_setg0 = 0.0, V_setg0 = 0.0;
+    double *d_rs = nullptr;               // [8] dE/dA, dE/dV, E, A, V
+    double *d_Dh = nullptr;               // [3nv]
+    double *d_Apart = nullptr, *d_Vpart = nullptr;
+    int *d_nb_start = nullptr, *d_nb = nullptr;
+    int nApart = 0, nVpart = 0;
+    std::vector<int> area_start;          // area blocks per class (class order)
+    // Brownian step counter of the device-noise path
+    int *d_step = nullptr;
+};
+
+namespace umcp {
+
+// helpers of the surface object used by both translation units
+bool restraints_on(const umcp_surface *s);
+FaceArgs face_args(umcp_surface *s, size_t c, const double *d_r);
+RestraintArgs restraint_args(umcp_surface *s, const double *d_r, bool area, bool volume);
+
+}  // namespace umcp

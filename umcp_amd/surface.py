@@ -34,6 +34,10 @@ EXPORTS = [
     "umcp_energy_grad_dev", "umcp_bd_steps_dev", "umcp_class_count", "umcp_class_info",
     "umcp_class_table", "umcp_face_points", "umcp_face_lipids", "umcp_set_launch",
     "umcp_get_launch", "umcp_profile_begin", "umcp_profile_end",
+    "umcp_set_restraints", "umcp_volume", "umcp_set_step", "umcp_get_step", "umcp_bd_steps_rng_dev",
+    "umcp_noise", "umcp_ensemble_create", "umcp_ensemble_destroy", "umcp_ensemble_last_error",
+    "umcp_ensemble_add", "umcp_ensemble_remove", "umcp_ensemble_size", "umcp_ensemble_refresh",
+    "umcp_ensemble_set_launch", "umcp_ensemble_get_launch", "umcp_ensemble_bd_steps", "umcp_ensemble_work",
 ]
 
 
@@ -49,6 +53,10 @@ class Params(C.Structure):
 
 class BDParams(C.Structure):
     _fields_ = [("mobility_dt", C.c_double), ("noise_amp", C.c_double)]
+
+
+class Restraints(C.Structure):
+    _fields_ = [("KA_global", C.c_double), ("KV", C.c_double), ("A0", C.c_double), ("V0", C.c_double)]
 
 
 _lib = None
@@ -93,6 +101,24 @@ def load_library(path: str = LIB_PATH):
         "umcp_get_launch": ([vp, ip, ip], C.c_int),
         "umcp_profile_begin": ([vp], C.c_int),
         "umcp_profile_end": ([vp, C.c_int, dp, ip], C.c_int),
+        "umcp_set_restraints": ([vp, C.POINTER(Restraints)], C.c_int),
+        "umcp_volume": ([vp, dp, dp], C.c_int),
+        "umcp_set_step": ([vp, C.c_int], C.c_int),
+        "umcp_get_step": ([vp, ip], C.c_int),
+        "umcp_bd_steps_rng_dev": ([vp, vp, C.c_int, C.POINTER(BDParams), C.c_ulonglong, vp, C.c_int, vp],
+                                  C.c_int),
+        "umcp_noise": ([C.c_ulonglong, C.c_int, C.c_int, vp, vp], C.c_int),
+        "umcp_ensemble_create": ([C.POINTER(vp)], C.c_int),
+        "umcp_ensemble_destroy": ([vp], None),
+        "umcp_ensemble_last_error": ([vp], C.c_char_p),
+        "umcp_ensemble_add": ([vp, vp, vp, C.POINTER(BDParams), C.c_ulonglong, vp, C.c_int], C.c_int),
+        "umcp_ensemble_remove": ([vp, vp], C.c_int),
+        "umcp_ensemble_size": ([vp], C.c_int),
+        "umcp_ensemble_refresh": ([vp], C.c_int),
+        "umcp_ensemble_set_launch": ([vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
+        "umcp_ensemble_get_launch": ([vp, ip, ip, ip, ip], C.c_int),
+        "umcp_ensemble_bd_steps": ([vp, C.c_int, C.c_int, vp], C.c_int),
+        "umcp_ensemble_work": ([vp, ip, ip, ip], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -234,6 +260,38 @@ class Surface:
                                        else None, C.c_void_p(stream) if stream else None)
         self._check(rc, "bd_steps_dev")
 
+    def bd_steps_rng_dev(self, d_r, nsteps: int, mobility_dt: float, noise_amp: float, seed: int,
+                         d_energies=None, stream=None):
+        """Brownian steps with the device noise stream of (seed, step)."""
+        bd = BDParams(mobility_dt, noise_amp)
+        cap = int(d_energies.numel()) if d_energies is not None else 0
+        rc = self._L.umcp_bd_steps_rng_dev(self._h, C.c_void_p(d_r.data_ptr()), nsteps, C.byref(bd),
+                                           C.c_ulonglong(seed),
+                                           C.c_void_p(d_energies.data_ptr()) if d_energies is not None
+                                           else None, cap, C.c_void_p(stream) if stream else None)
+        self._check(rc, "bd_steps_rng_dev")
+
+    def set_step(self, step: int):
+        self._check(self._L.umcp_set_step(self._h, int(step)), "set_step")
+
+    def get_step(self) -> int:
+        k = C.c_int()
+        self._check(self._L.umcp_get_step(self._h, C.byref(k)), "get_step")
+        return k.value
+
+    # -- global restraints -------------------------------------------------
+    def set_restraints(self, KA_global: float = 0.0, KV: float = 0.0, A0: float = 0.0, V0: float = 0.0):
+        """Global area (KA_global A0 s^2) and volume (KV/2 V0 s^2) restraints;
+        A0 <= 0 / V0 == 0 take the setg0 reference values."""
+        rs = Restraints(KA_global, KV, A0, V0)
+        self._check(self._L.umcp_set_restraints(self._h, C.byref(rs)), "set_restraints")
+
+    def volume(self, r: np.ndarray) -> float:
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        V = C.c_double()
+        self._check(self._L.umcp_volume(self._h, _dp(r), C.byref(V)), "volume")
+        return V.value
+
     # -- introspection ---------------------------------------------------
     def classes(self):
         out = []
@@ -282,3 +340,78 @@ def bd_coefficients(D_m2_per_s: float, dt_s: float, T_K: float = 298.0):
     """(mobility_dt, noise_amp) of r <- r - (D dt/kT) dE/dr + sqrt(2 D dt) xi."""
     D = units.diffusion_A2_per_s(D_m2_per_s)
     return D * dt_s / units.kT(T_K), float(np.sqrt(2.0 * D * dt_s))
+
+
+def noise(seed: int, step: int, n: int, d_out, stream=None):
+    """The device noise of (seed, step) for vertices [0, n) into d_out (3n)."""
+    L = load_library()
+    rc = L.umcp_noise(C.c_ulonglong(seed), int(step), int(n), C.c_void_p(d_out.data_ptr()),
+                      C.c_void_p(stream) if stream else None)
+    if rc != 0:
+        raise UmcpError("noise failed (%d)" % rc)
+
+
+class Ensemble:
+    """Many resident surfaces stepped together in batched launches."""
+
+    def __init__(self):
+        self._L = load_library()
+        h = C.c_void_p()
+        rc = self._L.umcp_ensemble_create(C.byref(h))
+        if rc != 0:
+            raise UmcpError("ensemble_create failed (%d)" % rc)
+        self._h = h
+        self._members = {}
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            raise UmcpError("%s failed (%d): %s" % (what, rc, self._L.umcp_ensemble_last_error(self._h).decode()))
+
+    def close(self):
+        if self._h:
+            self._L.umcp_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, surface: Surface, d_r, mobility_dt: float, noise_amp: float, seed: int, d_energies=None):
+        bd = BDParams(mobility_dt, noise_amp)
+        cap = int(d_energies.numel()) if d_energies is not None else 0
+        rc = self._L.umcp_ensemble_add(self._h, surface._h, C.c_void_p(d_r.data_ptr()), C.byref(bd),
+                                       C.c_ulonglong(seed),
+                                       C.c_void_p(d_energies.data_ptr()) if d_energies is not None else None,
+                                       cap)
+        self._check(rc, "ensemble_add")
+        self._members[id(surface)] = (surface, d_r, d_energies)   # keep the arrays alive
+
+    def remove(self, surface: Surface):
+        self._check(self._L.umcp_ensemble_remove(self._h, surface._h), "ensemble_remove")
+        self._members.pop(id(surface), None)
+
+    def __len__(self):
+        return self._L.umcp_ensemble_size(self._h)
+
+    def refresh(self):
+        self._check(self._L.umcp_ensemble_refresh(self._h), "ensemble_refresh")
+
+    def set_launch(self, point_split: int = 4, flags: int = LAUNCH_OCC2, face_grid: int = 0, order: int = 0):
+        self._check(self._L.umcp_ensemble_set_launch(self._h, point_split, flags, face_grid, order),
+                    "ensemble_set_launch")
+
+    def get_launch(self):
+        v = [C.c_int() for _ in range(4)]
+        self._check(self._L.umcp_ensemble_get_launch(self._h, *[C.byref(x) for x in v]), "ensemble_get_launch")
+        return tuple(x.value for x in v)
+
+    def bd_steps(self, nsteps: int, graph_steps: int = 0, stream=None):
+        self._check(self._L.umcp_ensemble_bd_steps(self._h, int(nsteps), int(graph_steps),
+                                                   C.c_void_p(stream) if stream else None), "ensemble_bd_steps")
+
+    def work(self):
+        v = [C.c_int() for _ in range(3)]
+        self._check(self._L.umcp_ensemble_work(self._h, *[C.byref(x) for x in v]), "ensemble_work")
+        return tuple(x.value for x in v)
