@@ -198,17 +198,20 @@ int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs 
         hipError_t e = enqueue_restraints(s, d_r, s->rsp.KA_global != 0.0, s->rsp.KV != 0.0, st);
         if (e != hipSuccess) return hip_fail(s, e, "restraint kernel launch");
     }
-    MultiFaceArgs m{};
-    m.ncls = (int)s->launch_order.size();
-    for (int i = 0; i < m.ncls; i++) {
-        const int c = s->launch_order[i];
-        m.cls[i] = face_args(s, c, d_r);
-        m.cls[i].step_ctr = step_ctr;
-        m.kind[i] = kind_of(s->P.classes[c].ncp, s->P.classes[c].npt);
-        m.block_start[i] = s->block_start[i];
+    if (s->fa_dirty) {
+        s->h_fa.resize(s->P.classes.size());
+        for (size_t c = 0; c < s->P.classes.size(); c++) s->h_fa[c] = face_args(s, c, s->d_r);
+        hipError_t e = hipMemcpyAsync(s->d_fa_tab, s->h_fa.data(), s->h_fa.size() * sizeof(FaceArgs),
+                                      hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return hip_fail(s, e, "upload face table");
+        s->fa_dirty = false;
     }
-    m.block_start[m.ncls] = s->block_start[m.ncls];
-    m.partial = s->d_partial;
+    BatchFaceArgs fb{};
+    fb.fa = s->d_fa_tab;
+    fb.items = s->d_items;
+    fb.nitems = s->npartial;
+    fb.r = d_r;
+    fb.step_ctr = step_ctr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     if (s->prof_on) {
         int rc = prof_pair(s, 0, ev);
@@ -216,7 +219,7 @@ int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs 
         (void)hipEventRecord(ev[0], st);
     }
     const bool mixed = s->prm.precision == UMCP_PREC_MIXED;
-    hipError_t e = launch_faces(m, s->npartial, s->point_split, s->launch_flags, mixed, features_of(m.cls[0].ph), st);
+    hipError_t e = launch_faces_batch(fb, 0, s->point_split, s->launch_flags, mixed, features_of(phys(s)), st);
     if (e != hipSuccess) return hip_fail(s, e, "face kernel launch");
     if (s->prof_on) (void)hipEventRecord(ev[1], st);
     ga.nv = s->T.nv;
@@ -337,6 +340,7 @@ int umcp_set_params(umcp_surface *s, const umcp_params *p)
     if (!(p->lipid_apl > 0)) return fail(s, UMCP_ERR_ARG, "lipid area must be positive");
     bool repack = s->g0_set && p->lipid_apl != s->prm.lipid_apl;
     s->prm = *p;
+    s->fa_dirty = true;
     if (repack) s->g0_set = false;  // lipid packing depends on the lipid area
     return UMCP_OK;
 }
@@ -425,6 +429,18 @@ int umcp_plan(umcp_surface *s)
         (rc = dev_alloc(s, &s->d_nb_start, nbs.size())) || (rc = dev_alloc(s, &s->d_nb, nbl.size())) ||
         (rc = dev_alloc(s, &s->d_step, 1)))
         return rc;
+    // face work tables
+    std::vector<FaceItem> items;
+    for (size_t i = 0; i < s->launch_order.size(); i++) {
+        const int c = s->launch_order[i];
+        const FaceClass &C = s->P.classes[c];
+        for (int g = 0; g < face_groups(C.nf()); g++)
+            items.push_back(FaceItem{c, g, s->block_start[i] + g, kind_of(C.ncp, C.npt)});
+    }
+    if ((rc = dev_alloc(s, &s->d_fa_tab, s->P.classes.size())) || (rc = dev_alloc(s, &s->d_items, items.size())) ||
+        (rc = dev_upload(s, s->d_items, items.data(), items.size())))
+        return rc;
+    s->fa_dirty = true;
     const int zero_step = 0;
     const double zeros8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if ((rc = dev_upload(s, s->d_nb_start, nbs.data(), nbs.size())) || (rc = dev_upload(s, s->d_nb, nbl.data(), nbl.size())) ||
@@ -601,6 +617,7 @@ int umcp_set_restraints(umcp_surface *s, const umcp_restraints *rs)
     if (rs->KV != 0.0 && s->has_images)
         return fail(s, UMCP_ERR_ARG, "volume restraint needs a closed mesh (this one has periodic images)");
     s->rsp = *rs;
+    s->fa_dirty = true;
     return UMCP_OK;
 }
 

@@ -43,9 +43,10 @@ struct umcp_ensemble {
     bool dirty = true;
     // launch shape
     int point_split = 4, flags = kLaunchOcc2, face_grid = 0, gather_grid = 0, order = 0;
-    // built tables
+    // built tables; face items are grouped by energy form (the kernel a member
+    // runs alone), one batched launch per non-empty group
     bool mixed = false;
-    int feat = 0;
+    int feat_start[4] = {0, 0, 0, 0};
     FaceArgs *d_fa = nullptr;
     FaceItem *d_items = nullptr;
     GatherArgs *d_ga = nullptr;
@@ -124,15 +125,14 @@ int build(umcp_ensemble *e)
     e->dirty = false;
     if (e->members.empty()) return UMCP_OK;
     const int prec = e->members[0].s->prm.precision;
-    int feat = -1;
-    for (const auto &m : e->members) {
+    std::vector<int> feat_of(e->members.size());
+    for (size_t mi = 0; mi < e->members.size(); mi++) {
+        const auto &m = e->members[mi];
         if (m.s->prm.precision != prec)
             return efail(e, UMCP_ERR_ARG, "all members of an ensemble must use the same precision");
-        const int f = features_of(face_args(m.s, 0, m.d_r).ph);
-        feat = (feat < 0 || feat == f) ? f : 0;   // mixed energy forms: the general kernel
+        feat_of[mi] = features_of(face_args(m.s, 0, m.d_r).ph);
     }
     e->mixed = prec == UMCP_PREC_MIXED;
-    e->feat = feat;
 
     std::vector<FaceArgs> fa;
     std::vector<int> fa_surface;
@@ -199,17 +199,25 @@ int build(umcp_ensemble *e)
         ga.push_back(g);
         for (int b = 0; b < gather_blocks(s->T.nv); b++) gitems.push_back(GatherItem{gi, b});
     }
-    // item order: 0 = member after member, 1 = round-robin over members
+    // items grouped by energy form; within a group, order 0 = member after
+    // member, 1 = round-robin over members
     std::vector<FaceItem> items;
-    if (e->order == 1) {
-        size_t longest = 0;
-        for (const auto &v : per_member) longest = std::max(longest, v.size());
-        for (size_t k = 0; k < longest; k++)
-            for (const auto &v : per_member)
-                if (k < v.size()) items.push_back(v[k]);
-    } else {
-        for (const auto &v : per_member) items.insert(items.end(), v.begin(), v.end());
+    for (int f = 0; f < 3; f++) {
+        e->feat_start[f] = (int)items.size();
+        std::vector<const std::vector<FaceItem> *> grp;
+        for (size_t mi = 0; mi < per_member.size(); mi++)
+            if (feat_of[mi] == f) grp.push_back(&per_member[mi]);
+        if (e->order == 1) {
+            size_t longest = 0;
+            for (const auto *v : grp) longest = std::max(longest, v->size());
+            for (size_t k = 0; k < longest; k++)
+                for (const auto *v : grp)
+                    if (k < v->size()) items.push_back((*v)[k]);
+        } else {
+            for (const auto *v : grp) items.insert(items.end(), v->begin(), v->end());
+        }
     }
+    e->feat_start[3] = (int)items.size();
     int rc;
     if ((rc = upload_table(e, &e->d_fa, fa)) || (rc = upload_table(e, &e->d_items, items)) ||
         (rc = upload_table(e, &e->d_ga, ga)) || (rc = upload_table(e, &e->d_gitems, gitems)) ||
@@ -234,9 +242,13 @@ hipError_t enqueue_step(umcp_ensemble *e, hipStream_t st)
         if (err == hipSuccess) err = launch_restraint_finalize(e->d_ra, RestraintArgs{}, e->nra, st);
         if (err != hipSuccess) return err;
     }
-    BatchFaceArgs fb{e->d_fa, e->d_items, e->nitems};
-    err = launch_faces_batch(fb, e->face_grid, e->point_split, e->flags, e->mixed, e->feat, st);
-    if (err != hipSuccess) return err;
+    for (int f = 0; f < 3; f++) {
+        const int n = e->feat_start[f + 1] - e->feat_start[f];
+        if (n == 0) continue;
+        BatchFaceArgs fb{e->d_fa, e->d_items + e->feat_start[f], n, nullptr, nullptr};
+        err = launch_faces_batch(fb, e->face_grid, e->point_split, e->flags, e->mixed, f, st);
+        if (err != hipSuccess) return err;
+    }
     BatchGatherArgs gb{e->d_ga, e->d_gitems, e->ngitems};
     return launch_gather_batch(gb, e->gather_grid, e->mixed, st);
 }

@@ -240,7 +240,8 @@ __device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R ig0
 // and dE/dalpha accumulate in FP64; the gradient leaves as FP32 and is summed
 // per vertex in FP64 by the gather.
 template <typename R, int NCP, int NPT, int KIND, int WS, bool PLDS, int FEAT>
-__device__ __forceinline__ void face_group(const FaceArgs &a, int group, R *lds, double acc[4])
+__device__ __forceinline__ void face_group(const FaceArgs &a, const double *__restrict__ r, int group, R *lds,
+                                           double acc[4])
 {
     constexpr bool kRel = sizeof(R) == 4;
     const R *tab = kind_table<KIND, R>();
@@ -253,15 +254,15 @@ __device__ __forceinline__ void face_group(const FaceArgs &a, int group, R *lds,
     const int f = group * kWave + lane;
     const bool live = f < nf;
     const int fc = live ? f : nf - 1;
-    const double al64[3] = {a.r[3 * a.nv], a.r[3 * a.nv + 1], a.r[3 * a.nv + 2]};
+    const double al64[3] = {r[3 * a.nv], r[3 * a.nv + 1], r[3 * a.nv + 2]};
     const R al[3] = {(R)al64[0], (R)al64[1], (R)al64[2]};
     const bool shifted = a.has_img[fc] != 0;
 
     auto load_cp = [&](int c, double out[3]) {
         const int v = a.cp[c * nf + fc];
-        out[0] = a.r[3 * v];
-        out[1] = a.r[3 * v + 1];
-        out[2] = a.r[3 * v + 2];
+        out[0] = r[3 * v];
+        out[1] = r[3 * v + 1];
+        out[2] = r[3 * v + 2];
         if (shifted) {
             double o[3];
             image_offset(a.img[c * nf + fc], a.pbc, o);
@@ -380,53 +381,37 @@ __device__ __forceinline__ void face_group(const FaceArgs &a, int group, R *lds,
     }
 }
 
-template <typename R, int WS, bool PLDS, int OCC, int FEAT>
-__global__ __launch_bounds__(kWave *WS, OCC) void faces_kernel(MultiFaceArgs m)
-{
-    __shared__ R lds[13 * 3 * kWave];
-    const int b = xcd_logical_block(blockIdx.x, gridDim.x);
-    int c = 0;
-    while (c + 1 < m.ncls && b >= m.block_start[c + 1]) c++;
-    const int group = b - m.block_start[c];
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    switch (m.kind[c]) {
-    case 0: face_group<R, 12, 15, 0, WS, PLDS, FEAT>(m.cls[c], group, lds, acc); break;
-    case 1: face_group<R, 11, 16, 1, WS, PLDS, FEAT>(m.cls[c], group, lds, acc); break;
-    default: face_group<R, 13, 16, 2, WS, PLDS, FEAT>(m.cls[c], group, lds, acc); break;
-    }
-    block_sum4<kWave * WS>(acc);
-    if (threadIdx.x == 0) {
-        double *dst = m.partial + 4 * (size_t)b;
-#pragma unroll
-        for (int i = 0; i < 4; i++) dst[i] = acc[i];
-        if (b == 0 && m.cls[0].step_ctr) m.cls[0].step_ctr[0] += 1;
-    }
-}
-
-// The same face groups for many surfaces: one item per 64-face group, read
-// from a device table.  grid < nitems makes the launch persistent (each block
-// strides over the items); every item runs exactly the code a launch of its
-// surface alone runs, and writes its partials to the same slot.
-template <typename R, int WS, bool PLDS, int OCC, int FEAT>
+// Fused face kernel over a table of work items, one item per 64-face group of
+// one face class of one surface.  A surface evaluated alone and every member
+// of a batched ensemble run this same kernel; an item writes its partial sums
+// to the slot its group has in its surface's own launch, so a surface's
+// results do not depend on what else shares the launch.  PERSIST = false: one
+// workgroup per item; PERSIST = true: a smaller grid strides over the items.
+// B.r / B.step_ctr (non-null for a single surface's own launch) override the
+// table's positions and step counter.
+template <typename R, int WS, bool PLDS, int OCC, int FEAT, bool PERSIST>
 __global__ __launch_bounds__(kWave *WS, OCC) void faces_batch_kernel(BatchFaceArgs B)
 {
     __shared__ R lds[13 * 3 * kWave];
     for (int it = xcd_logical_block(blockIdx.x, gridDim.x); it < B.nitems; it += gridDim.x) {
         const FaceItem item = B.items[it];
         const FaceArgs &a = B.fa[item.fa];
+        const double *r = B.r ? B.r : a.r;
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         switch (item.kind) {
-        case 0: face_group<R, 12, 15, 0, WS, PLDS, FEAT>(a, item.group, lds, acc); break;
-        case 1: face_group<R, 11, 16, 1, WS, PLDS, FEAT>(a, item.group, lds, acc); break;
-        default: face_group<R, 13, 16, 2, WS, PLDS, FEAT>(a, item.group, lds, acc); break;
+        case 0: face_group<R, 12, 15, 0, WS, PLDS, FEAT>(a, r, item.group, lds, acc); break;
+        case 1: face_group<R, 11, 16, 1, WS, PLDS, FEAT>(a, r, item.group, lds, acc); break;
+        default: face_group<R, 13, 16, 2, WS, PLDS, FEAT>(a, r, item.group, lds, acc); break;
         }
         block_sum4<kWave * WS>(acc);
         if (threadIdx.x == 0) {
             double *dst = a.partial + 4 * (size_t)item.slot;
 #pragma unroll
             for (int i = 0; i < 4; i++) dst[i] = acc[i];
-            if (item.slot == 0 && a.step_ctr) a.step_ctr[0] += 1;
+            int *ctr = B.step_ctr ? B.step_ctr : a.step_ctr;
+            if (item.slot == 0 && ctr) ctr[0] += 1;
         }
+        if (!PERSIST) break;
         __syncthreads();   // LDS and the reduction scratch are reused by the next item
     }
 }
@@ -523,6 +508,9 @@ constexpr unsigned long long kNoiseKey1 = 0x554D4350ULL;   // "UMCP"
 // three N(0,1) deviates for (seed, step, vertex v)
 __device__ __forceinline__ void normal3(unsigned long long seed, int step, int v, double z[3])
 {
+    // every operation as written (no FMA contraction): the single-surface and
+    // batched kernels that inline this must round identically
+    #pragma clang fp contract(off)
     unsigned long long x[4] = {(unsigned long long)v, (unsigned long long)(unsigned)step, 0ULL, 0ULL};
     philox4x64_10(x, seed, kNoiseKey1);
     constexpr double k53 = 1.0 / 9007199254740992.0;   // 2^-53
@@ -562,6 +550,9 @@ __device__ __forceinline__ int philox_step(const GatherArgs &a)
 
 __device__ __forceinline__ void gather_reduce(const GatherArgs &a)
 {
+    // every operation as written (no FMA contraction): the single-surface and
+    // batched kernels that inline this must round identically
+    #pragma clang fp contract(off)
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int i = threadIdx.x; i < a.npartial; i += kGatherBlock)
 #pragma unroll
@@ -589,6 +580,9 @@ __device__ __forceinline__ void gather_reduce(const GatherArgs &a)
 template <typename GB>
 __device__ __forceinline__ void gather_vertices(const GatherArgs &a, int vblock)
 {
+    // every operation as written (no FMA contraction): the single-surface and
+    // batched kernels that inline this must round identically
+    #pragma clang fp contract(off)
     if (a.out_mode == 2) return;
     const int v = vblock * kGatherBlock + threadIdx.x;
     if (v >= a.nv) return;
@@ -669,6 +663,9 @@ __global__ __launch_bounds__(kGatherBlock) void vertex_gather_batch(BatchGatherA
 template <int NCP, int NPT, int KIND>
 __device__ __forceinline__ double face_area(const FaceArgs &a, int f)
 {
+    // every operation as written (no FMA contraction): the single-surface and
+    // batched kernels that inline this must round identically
+    #pragma clang fp contract(off)
     const int nf = a.nf;
     const double al[3] = {a.r[3 * a.nv], a.r[3 * a.nv + 1], a.r[3 * a.nv + 2]};
     double P[NCP][3];
@@ -722,6 +719,9 @@ __device__ __forceinline__ double area_block(const FaceArgs &a, int kind, int bl
 // Loop limit position of vertex u: r_u / 2 + sum(ring) / (2 n)
 __device__ __forceinline__ void limit_point(const RestraintArgs &ra, int u, double L[3])
 {
+    // every operation as written (no FMA contraction): the single-surface and
+    // batched kernels that inline this must round identically
+    #pragma clang fp contract(off)
     const int b0 = ra.nb_start[u], b1 = ra.nb_start[u + 1];
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     for (int q = b0; q < b1; q++) {
@@ -741,6 +741,9 @@ __device__ __forceinline__ void limit_point(const RestraintArgs &ra, int u, doub
 // (each face's triple product is counted once per corner).
 __device__ __forceinline__ double volume_block(const RestraintArgs &ra, int blk)
 {
+    // every operation as written (no FMA contraction): the single-surface and
+    // batched kernels that inline this must round identically
+    #pragma clang fp contract(off)
     const int v = blk * kRestraintBlock + threadIdx.x;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     if (v < ra.nv) {
@@ -817,6 +820,9 @@ __global__ __launch_bounds__(kRestraintBlock) void restraint_batch_kernel(BatchR
 // then the restraint energy and its multipliers
 __device__ __forceinline__ void restraint_finish(const RestraintArgs &ra)
 {
+    // every operation as written (no FMA contraction): the single-surface and
+    // batched kernels that inline this must round identically
+    #pragma clang fp contract(off)
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     if (ra.do_area)
         for (int i = threadIdx.x; i < ra.nApart; i += kRestraintBlock) acc[0] += ra.Apart[i];
@@ -896,44 +902,41 @@ int face_groups(int nf) { return (nf + kWave - 1) / kWave; }
 int metric_blocks(int nf) { return (nf + kMetricBlock - 1) / kMetricBlock; }
 
 template <typename R, int WS, int OCC, int FEAT>
-static hipError_t launch_ws(const MultiFaceArgs &a, int nb, hipStream_t st)
+static hipError_t launch_ws(const BatchFaceArgs &a, int nb, bool persist, hipStream_t st)
 {
-    hipLaunchKernelGGL((faces_kernel<R, WS, true, OCC, FEAT>), dim3(nb), dim3(kWave * WS), 0, st, a);
-    return hipGetLastError();
-}
-
-template <typename R, int WS, int OCC, int FEAT>
-static hipError_t launch_ws(const BatchFaceArgs &a, int nb, hipStream_t st)
-{
-    hipLaunchKernelGGL((faces_batch_kernel<R, WS, true, OCC, FEAT>), dim3(nb), dim3(kWave * WS), 0, st, a);
+    if (persist)
+        hipLaunchKernelGGL((faces_batch_kernel<R, WS, true, OCC, FEAT, true>), dim3(nb), dim3(kWave * WS), 0, st, a);
+    else
+        hipLaunchKernelGGL((faces_batch_kernel<R, WS, true, OCC, FEAT, false>), dim3(nb), dim3(kWave * WS), 0, st, a);
     return hipGetLastError();
 }
 
 // the register-bound flag asks for 2 waves per SIMD in FP64 and 4 in FP32
-template <typename R, int FEAT, typename Args>
-static hipError_t launch_shape(const Args &a, int nb, int ws, int occ2, hipStream_t st)
+template <typename R, int FEAT>
+static hipError_t launch_shape(const BatchFaceArgs &a, int nb, bool persist, int ws, int occ2, hipStream_t st)
 {
     constexpr int kB = sizeof(R) == 4 ? 4 : 2;
     switch (ws * 2 + (occ2 ? 1 : 0)) {
-    case 2: return launch_ws<R, 1, 1, FEAT>(a, nb, st);
-    case 3: return launch_ws<R, 1, kB, FEAT>(a, nb, st);
-    case 4: return launch_ws<R, 2, 1, FEAT>(a, nb, st);
-    case 5: return launch_ws<R, 2, kB, FEAT>(a, nb, st);
-    case 6: return launch_ws<R, 3, 1, FEAT>(a, nb, st);
-    case 7: return launch_ws<R, 3, kB, FEAT>(a, nb, st);
-    case 8: return launch_ws<R, 4, 1, FEAT>(a, nb, st);
-    case 9: return launch_ws<R, 4, kB, FEAT>(a, nb, st);
+    case 2: return launch_ws<R, 1, 1, FEAT>(a, nb, persist, st);
+    case 3: return launch_ws<R, 1, kB, FEAT>(a, nb, persist, st);
+    case 4: return launch_ws<R, 2, 1, FEAT>(a, nb, persist, st);
+    case 5: return launch_ws<R, 2, kB, FEAT>(a, nb, persist, st);
+    case 6: return launch_ws<R, 3, 1, FEAT>(a, nb, persist, st);
+    case 7: return launch_ws<R, 3, kB, FEAT>(a, nb, persist, st);
+    case 8: return launch_ws<R, 4, 1, FEAT>(a, nb, persist, st);
+    case 9: return launch_ws<R, 4, kB, FEAT>(a, nb, persist, st);
     default: return hipErrorInvalidValue;
     }
 }
 
-template <typename R, typename Args>
-static hipError_t launch_prec(const Args &a, int nb, int ws, int occ2, int feat, hipStream_t st)
+template <typename R>
+static hipError_t launch_prec(const BatchFaceArgs &a, int nb, bool persist, int ws, int occ2, int feat,
+                              hipStream_t st)
 {
     switch (feat) {
-    case kFeatLipid: return launch_shape<R, kFeatLipid>(a, nb, ws, occ2, st);
-    case kFeatHelfrich: return launch_shape<R, kFeatHelfrich>(a, nb, ws, occ2, st);
-    default: return launch_shape<R, kFeatGeneral>(a, nb, ws, occ2, st);
+    case kFeatLipid: return launch_shape<R, kFeatLipid>(a, nb, persist, ws, occ2, st);
+    case kFeatHelfrich: return launch_shape<R, kFeatHelfrich>(a, nb, persist, ws, occ2, st);
+    default: return launch_shape<R, kFeatGeneral>(a, nb, persist, ws, occ2, st);
     }
 }
 
@@ -941,13 +944,6 @@ int features_of(const PhysParams &ph)
 {
     if (ph.kg != 0.0 || ph.k_reg != 0.0) return kFeatGeneral;
     return ph.mode == 0 ? kFeatLipid : kFeatHelfrich;
-}
-
-hipError_t launch_faces(const MultiFaceArgs &a, int nb, int ws, int flags, bool mixed, int feat, hipStream_t st)
-{
-    if (nb <= 0) return hipSuccess;
-    const int occ2 = (flags & kLaunchOcc2) ? 1 : 0;
-    return mixed ? launch_prec<float>(a, nb, ws, occ2, feat, st) : launch_prec<double>(a, nb, ws, occ2, feat, st);
 }
 
 hipError_t launch_face_metric(const FaceArgs &a, int kind, int mode, double *g0_out, double *rurv0_out,
@@ -989,9 +985,11 @@ hipError_t launch_faces_batch(const BatchFaceArgs &a, int grid, int ws, int flag
                               hipStream_t st)
 {
     if (a.nitems <= 0) return hipSuccess;
-    const int nb = (grid > 0 && grid < a.nitems) ? grid : a.nitems;
+    const bool persist = grid > 0 && grid < a.nitems;
+    const int nb = persist ? grid : a.nitems;
     const int occ2 = (flags & kLaunchOcc2) ? 1 : 0;
-    return mixed ? launch_prec<float>(a, nb, ws, occ2, feat, st) : launch_prec<double>(a, nb, ws, occ2, feat, st);
+    return mixed ? launch_prec<float>(a, nb, persist, ws, occ2, feat, st)
+                 : launch_prec<double>(a, nb, persist, ws, occ2, feat, st);
 }
 
 hipError_t launch_gather_batch(const BatchGatherArgs &a, int grid, bool mixed, hipStream_t st)

@@ -43,17 +43,6 @@ struct FaceArgs {
     double *partial;        // the surface's block partials (batched launches)
 };
 
-// All face classes of a surface in one launch: blocks [block_start[c],
-// block_start[c+1]) work on class c (kind: 0 = 12 control points / 15 Gauss
-// points, 1 = 11 / 16, 2 = 13 / 16).
-struct MultiFaceArgs {
-    FaceArgs cls[kMaxClasses];
-    int kind[kMaxClasses];
-    int block_start[kMaxClasses + 1];
-    int ncls;
-    double *partial;        // [nblocks][4]: energy, dE/dalpha
-};
-
 // One work item of a batched face launch: 64 faces of one class of one
 // surface.  slot = the block index the same group has in the surface's own
 // launch, so its partial sums land where a launch of that surface alone puts
@@ -66,6 +55,8 @@ struct BatchFaceArgs {
     const FaceArgs *fa;     // device array, one per (surface, class)
     const FaceItem *items;  // device array
     int nitems;
+    const double *r;        // non-null: positions for every item (a surface's own launch)
+    int *step_ctr;          // non-null: the step counter the slot-0 item advances
 };
 
 // Counter-based normal deviates: Philox4x64-10 keyed by (seed, 'UMCP') at
@@ -159,8 +150,6 @@ int face_groups(int nf);
 // for two waves per SIMD.  feat: energy-form specialisation (features_of).
 constexpr int kLaunchOcc2 = 2;
 int features_of(const PhysParams &ph);
-hipError_t launch_faces(const MultiFaceArgs &a, int nblocks, int point_split, int flags, bool mixed, int feat,
-                        hipStream_t st);
 hipError_t launch_face_metric(const FaceArgs &a, int kind, int mode, double *g0_out,
                               double *rurv0_out, double *ig0_out, double *partial, int partial_base,
                               hipStream_t st);
@@ -174,8 +163,9 @@ hipError_t launch_restraints(const RestraintLaunch &a, hipStream_t st);
 hipError_t launch_restraint_finalize(const RestraintArgs *ra_dev_or_null, const RestraintArgs &ra, int nsurf,
                                      hipStream_t st);
 
-// batched launches over many surfaces (grid = 0: one block per item, else a
-// persistent grid of that many blocks striding over the items)
+// face kernel over a work-item table (one surface or many); grid = 0: one
+// workgroup per item, else a persistent grid of that many workgroups striding
+// over the items.  Batched gather and restraint launches likewise.
 hipError_t launch_faces_batch(const BatchFaceArgs &a, int grid, int point_split, int flags, bool mixed, int feat,
                               hipStream_t st);
 hipError_t launch_gather_batch(const BatchGatherArgs &a, int grid, bool mixed, hipStream_t st);

@@ -67,6 +67,12 @@ struct umcp_surface {
     std::vector<int> area_start;          // area blocks per class (class order)
     // Brownian step counter of the device-noise path
     int *d_step = nullptr;
+    // the face kernel's work tables: per-class arguments and one item per
+    // 64-face group in launch order (re-uploaded when parameters change)
+    umcp::FaceArgs *d_fa_tab = nullptr;
+    umcp::FaceItem *d_items = nullptr;
+    std::vector<umcp::FaceArgs> h_fa;
+    bool fa_dirty = true;
 };
 
 namespace umcp {
