@@ -165,6 +165,201 @@ def run_leg(args, w, c0, precision, d_xi, dev, torch):
     return leg, E0, g0
 
 
+def ensemble_surfaces(members, precision, S):
+    out = []
+    for m in members:
+        s = S.Surface(m.mesh_path)
+        s.set_params(precision=precision, **m.params)
+        s.generatePlan()
+        s.setg0(m.r0)
+        s.set_step(0)
+        out.append(s)
+    return out
+
+
+def run_ensemble_leg(args, dev, torch):
+    """Batched ensemble: 256 vesicles of levels 2-5 stepped together (one
+    batched launch each of the face and vertex kernels per step, captured in a
+    hipGraph), FP32-mixed; the launch shape picked by the ensemble tuner.
+    Reports member-steps/s and face-evals/s, the same steps without the graph,
+    the one-at-a-time rate measured on a sample of members, and per-member
+    energy errors of the mixed run against the FP64 ensemble."""
+    import numpy as np
+    from umcp_amd import surface as S
+    from umcp_amd import tune, workloads
+
+    members = workloads.ensemble(n=args.ensemble_size)
+    K, W, G = args.ensemble_steps, 10, args.graph_steps
+    t0 = time.perf_counter()
+    surfs = ensemble_surfaces(members, S.PREC_MIXED, S)
+    setup_s = time.perf_counter() - t0
+    stream = torch.cuda.Stream(device=dev)
+    sptr = stream.cuda_stream
+    d_rs = [torch.tensor(m.r, device=dev, dtype=torch.float64) for m in members]
+    d_es = [torch.zeros(W + K, device=dev, dtype=torch.float64) for _ in members]
+    ens = S.Ensemble()
+    for m, s, d_r, d_e in zip(members, surfs, d_rs, d_es):
+        mob, amp = S.bd_coefficients(1e-12, m.dt, m.T)
+        ens.add(s, d_r, mob, amp, m.seed, d_e)
+    torch.cuda.synchronize()
+    tuned = None
+    if not args.no_tune:
+        best, times = tune.tune_ensemble(ens, surfs, d_rs, torch, stream)
+        tuned = {"best": list(best), "ms_per_step": {"%d/%d/%d/%d" % k: round(v, 4) for k, v in times.items()}}
+    nf = sum(m.nfaces for m in members)
+    # first W steps (graph capture included), then K timed steps
+    ens.bd_steps(W, G, sptr)
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    ens.bd_steps(K, G, sptr)
+    end.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end)
+    energies = torch.stack(d_es).cpu().numpy()
+    if not np.all(np.isfinite(energies)):
+        raise SystemExit("non-finite energy in the ensemble")
+    # the same launches without the graph
+    Ke = min(K, 200)
+    start.record(stream)
+    ens.bd_steps(Ke, 0, sptr)
+    end.record(stream)
+    torch.cuda.synchronize()
+    ms_eager = start.elapsed_time(end) / Ke
+    items = ens.work()
+    ens.close()
+    del surfs, d_rs, d_es
+
+    # one-at-a-time on a sample of members (device noise, no graph): the rate
+    # each level reaches alone, weighted by the ensemble's composition
+    rate_alone = {}
+    Ks = 100
+    for L in sorted({m.level for m in members}):
+        m = next(x for x in members if x.level == L)
+        s = ensemble_surfaces([m], S.PREC_MIXED, S)[0]
+        d_r = torch.tensor(m.r, device=dev, dtype=torch.float64)
+        mob, amp = S.bd_coefficients(1e-12, m.dt, m.T)
+        s.bd_steps_rng_dev(d_r, 10, mob, amp, m.seed, None, sptr)
+        torch.cuda.synchronize()
+        start.record(stream)
+        s.bd_steps_rng_dev(d_r, Ks, mob, amp, m.seed, None, sptr)
+        end.record(stream)
+        torch.cuda.synchronize()
+        rate_alone[L] = start.elapsed_time(end) / Ks      # ms per step alone
+        s.close()
+    seq_ms_per_step = sum(rate_alone[m.level] for m in members)
+
+    # mixed against FP64, per member, over the first Kc steps from the start
+    Kc = 20
+    res = {}
+    for prec in (S.PREC_FP64, S.PREC_MIXED):
+        ss = ensemble_surfaces(members, prec, S)
+        rr = [torch.tensor(m.r, device=dev, dtype=torch.float64) for m in members]
+        ee = [torch.zeros(Kc, device=dev, dtype=torch.float64) for _ in members]
+        en = S.Ensemble()
+        for m, s, d_r, d_e in zip(members, ss, rr, ee):
+            mob, amp = S.bd_coefficients(1e-12, m.dt, m.T)
+            en.add(s, d_r, mob, amp, m.seed, d_e)
+        en.bd_steps(Kc, Kc, sptr)
+        torch.cuda.synchronize()
+        res[prec] = (torch.stack(rr).cpu().numpy(), torch.stack(ee).cpu().numpy())
+        en.close()
+        del ss
+    r64, e64 = res[S.PREC_FP64]
+    r32, e32 = res[S.PREC_MIXED]
+    e_rel = np.abs(e32 / e64 - 1).max(axis=1)
+    dr = np.abs(r32 - r64).max(axis=1)
+
+    levels = {str(L): sum(1 for m in members if m.level == L) for L in sorted({m.level for m in members})}
+    return {
+        "workload": "ensemble_%d_levels2-5" % len(members),
+        "members": len(members), "levels": levels, "faces": nf, "vertices": sum(m.nv for m in members),
+        "precision": "mixed", "steps": K, "graph_steps": G,
+        "member_steps_per_s": len(members) * K / (ms / 1e3),
+        "face_evals_per_s": nf * K / (ms / 1e3),
+        "ms_per_step": ms / K,
+        "ms_per_step_no_graph": ms_eager,
+        "launch": tuned["best"] if tuned else [4, 2, 0, 0],
+        "autotune": tuned,
+        "work_items": {"face": items[0], "gather": items[1], "restraint": items[2]},
+        "one_at_a_time_ms_per_step_by_level": {str(k): round(v, 4) for k, v in rate_alone.items()},
+        "speedup_vs_one_at_a_time": seq_ms_per_step / (ms / K),
+        "vs_fp64": {"steps": Kc, "energy_rel_max": float(e_rel.max()), "energy_rel_median": float(np.median(e_rel)),
+                    "max_dr_A": float(dr.max())},
+        "setup_s": setup_s,
+    }
+
+
+def run_serving_leg(args, dev, torch):
+    """Serving stream: Poisson arrivals of Brownian-dynamics requests
+    (icospheres of level 2-6 or periodic sheets of 32^2..256^2 vertices,
+    log-uniform steps in [100, 10000], per-request kc and c0), FP32-mixed,
+    through the serving loop with continuous batching, then through the same
+    loop one request at a time.  Reports latency percentiles (submit to done),
+    aggregate request and face-eval throughput over the makespan, and that
+    every request's final positions are the same in both runs."""
+    import numpy as np
+    from umcp_amd import surface as S
+    from umcp_amd import workloads
+
+    reqs = workloads.serving(n=args.serve_requests, rate=args.serve_rate)
+
+    def run(slots):
+        sv = S.Server(max_active=slots, chunk_steps=args.serve_chunk, precision=S.PREC_MIXED)
+        # the mesh library is loaded before the stream starts: one zero-step
+        # request per distinct mesh file parses and plans its template
+        first = {}
+        for q in reqs:
+            first.setdefault(q.mesh_path, q)
+        for q in first.values():
+            sv.wait(sv.submit(q.mesh_path, q.nv, q.params, q.c0, 0, q.seed, 0.0, 0.0, q.r0, q.r))
+        ids = []
+        t0 = time.perf_counter()
+        for q in reqs:
+            delay = t0 + q.arrival - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            mob, amp = S.bd_coefficients(1e-12, q.dt, q.T)
+            ids.append(sv.submit(q.mesh_path, q.nv, q.params, q.c0, q.steps, q.seed, mob, amp, q.r0, q.r))
+        res = [sv.wait(i) for i in ids]
+        st = sv.stats()
+        sv.close()
+        t_sub = np.array([t[0] for _, _, t in res])
+        t_done = np.array([t[3] for _, _, t in res])
+        lat = t_done - t_sub
+        makespan = t_done.max() - t_sub.min()
+        face_steps = float(sum(q.nfaces * q.steps for q in reqs))
+        return {
+            "slots": slots,
+            "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                          "mean": float(lat.mean()), "max": float(lat.max())},
+            "queue_wait_s_p50": float(np.percentile(np.array([t[1] - t[0] for _, _, t in res]), 50)),
+            "makespan_s": float(makespan),
+            "requests_per_s": len(reqs) / makespan,
+            "face_evals_per_s": face_steps / makespan,
+            "device_busy_fraction": st["busy_s"] / max(st["clock_s"], 1e-9),
+            "chunks": int(st["chunks"]), "peak_active": int(st["peak_active"]),
+            "peak_reserved_GB": st["peak_bytes"] / 1e9,
+        }, res
+
+    batched, res_b = run(args.serve_slots)
+    single, res_1 = run(1)
+    dr = max(float(np.abs(a[0] - b[0]).max()) for a, b in zip(res_b, res_1))
+    same_e = all(np.array_equal(a[1], b[1]) for a, b in zip(res_b, res_1))
+    return {
+        "workload": "serving_%d_requests_poisson_%g_per_s" % (len(reqs), args.serve_rate),
+        "precision": "mixed",
+        "requests": len(reqs),
+        "icospheres": sum(1 for q in reqs if q.kind == "icosphere"),
+        "sheets": sum(1 for q in reqs if q.kind == "sheet"),
+        "face_steps": float(sum(q.nfaces * q.steps for q in reqs)),
+        "continuous_batching": batched,
+        "one_at_a_time": single,
+        "throughput_ratio": batched["requests_per_s"] / single["requests_per_s"],
+        "same_result_alone_and_in_mix": {"max_abs_dr_A": dr, "energies_bitwise_equal": bool(same_e)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +373,15 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-mixed leg")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--ensemble-size", type=int, default=256)
+    ap.add_argument("--ensemble-steps", type=int, default=2000)
+    ap.add_argument("--graph-steps", type=int, default=50)
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream leg")
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-slots", type=int, default=64)
+    ap.add_argument("--serve-chunk", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs one mesh per GPU and has no "
@@ -246,6 +450,10 @@ def main():
         legm["kernel"] = kernel_name("mixed", legm["launch"]["point_split"], legm["launch"]["flags"])
         legm["traffic"] = measured_traffic(legm["kernel"])
         out["mixed"] = legm
+    if not args.no_ensemble:
+        out["ensemble"] = run_ensemble_leg(args, dev, torch)
+    if not args.no_serving:
+        out["serving"] = run_serving_leg(args, dev, torch)
     if not args.no_cpu_baseline:
         base, E_o, g_o = cpu_baseline(w, c0, args.cpu_budget)
         out["cpu_baseline"] = base

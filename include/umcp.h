@@ -91,6 +91,16 @@ int umcp_prepare(umcp_surface *s);
  * per-face state and work buffers resident on the device */
 int umcp_plan(umcp_surface *s);
 
+/* a new handle with the prepared host plan of tmpl (the reference's
+ * copyFrom argument of loadLattice, src/interp.h:291): no parsing,
+ * orientation or table construction; the copy is then planned, parametrised
+ * and setg0'd on its own (on the current device) */
+int umcp_clone(const umcp_surface *tmpl, umcp_surface **out);
+
+/* device bytes of a prepared surface: what umcp_plan allocates (planned, also
+ * before the plan) and what this handle holds now (allocated) */
+int umcp_device_bytes(const umcp_surface *s, size_t *planned, size_t *allocated);
+
 /* per-face spontaneous curvature (UMCP_MODE_HELFRICH), reference face order
  * (regular faces, then irregular); replaces theVertices[i].c0 of the owner */
 int umcp_set_face_c0(umcp_surface *s, const double *c0);
@@ -195,6 +205,56 @@ int umcp_ensemble_get_launch(const umcp_ensemble *e, int *point_split, int *flag
 int umcp_ensemble_bd_steps(umcp_ensemble *e, int nsteps, int graph_steps, void *stream);
 /* work items of the current tables (after the first umcp_ensemble_bd_steps) */
 int umcp_ensemble_work(const umcp_ensemble *e, int *face_items, int *gather_items, int *restraint_items);
+
+/* Serving loop (no reference counterpart: the reference runs one simulation
+ * per process).  Requests (mesh file, parameters, spontaneous curvature, N
+ * Brownian steps with device noise) are queued as they arrive; a preparation
+ * thread admits the queue head once a slot is free and its device footprint
+ * fits the budget (template parsed once per mesh file, then clone, plan, setg0,
+ * start positions); the worker thread steps every admitted request together
+ * (batched launches of an ensemble) in chunks of at most chunk_steps, ending
+ * each chunk at the fewest remaining steps, and retires finished requests.
+ * Each request's result equals running it alone (umcp_bd_steps_rng_dev). */
+typedef struct umcp_server umcp_server;
+typedef struct {
+    double budget_bytes;   /* device bytes admitted requests may hold (<= 0: 200e9) */
+    int max_active;        /* requests stepped together */
+    int chunk_steps;       /* most steps between two scheduling points */
+    int precision;         /* UMCP_PREC_* of every request */
+    int point_split;       /* ensemble launch shape, as umcp_ensemble_set_launch */
+    int flags;
+    int face_grid;
+    int order;
+    int graph_steps;       /* > 0: replay captured graphs of up to that many steps */
+} umcp_server_config;
+typedef struct {
+    const char *mesh_path; /* .mesh file of the request's surface */
+    int nv;                /* its vertex count (r0, r hold 3nv+3 doubles) */
+    umcp_params params;    /* precision is the server's */
+    double c0;             /* spontaneous curvature of every face (UMCP_MODE_HELFRICH) */
+    int steps;
+    unsigned long long seed;
+    umcp_bd_params bd;
+    const double *r0;      /* setg0 state (copied at submit) */
+    const double *r;       /* start positions (copied at submit) */
+} umcp_request;
+int umcp_server_create(const umcp_server_config *cfg, umcp_server **out);
+/* thread safe; *id identifies the request */
+int umcp_server_submit(umcp_server *sv, const umcp_request *rq, int *id);
+/* blocks until request id is done, then releases it: r_out (3nv+3), energies
+ * (steps: the energy at the start of each step) and times[4] = submitted,
+ * admitted, first step, done (seconds on the server's clock); each may be
+ * NULL.  Returns the request's status. */
+int umcp_server_wait(umcp_server *sv, int id, double *r_out, double *energies, double *times);
+/* blocks until every submitted request is done */
+int umcp_server_drain(umcp_server *sv);
+/* stats[8]: requests done, chunks, request-steps, face-steps, seconds the
+ * device was stepping, peak reserved bytes, peak active requests, seconds
+ * since creation */
+int umcp_server_stats(umcp_server *sv, double *stats);
+const char *umcp_server_last_error(const umcp_server *sv);
+/* finishes admitted requests, drops queued ones, joins the threads */
+void umcp_server_destroy(umcp_server *sv);
 
 /* launch shape of the face kernel (no reference counterpart; set by the
  * tuner): point_split in 1..4 waves share the Gauss points of 64 faces

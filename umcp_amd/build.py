@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libumcp_core.so")
-SOURCES = ["umcp_kernels.hip", "surface_host.cpp", "umcp_api.cpp", "umcp_ensemble.cpp"]
+SOURCES = ["umcp_kernels.hip", "surface_host.cpp", "umcp_api.cpp", "umcp_ensemble.cpp", "umcp_server.cpp"]
 ARCH = os.environ.get("UMCP_OFFLOAD_ARCH", "gfx950")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
@@ -44,13 +44,13 @@ def build(verbose: bool = False) -> str:
                 "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
         else:  # host-only C++: no device pass
             cmd = [os.environ.get("CXX", "g++"), "-c", path, "-o", obj] + common + [
-                "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include")]
+                "-pthread", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include")]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    cmd = [hipcc(), "-shared", "-fPIC", "-pthread", "--offload-arch=" + ARCH, "-o", tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

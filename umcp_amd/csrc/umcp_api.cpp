@@ -48,6 +48,7 @@ int dev_alloc(umcp_surface *s, T **p, size_t n)
     hipError_t e = hipMalloc(&q, n * sizeof(T));
     if (e != hipSuccess) return fail(s, UMCP_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
     s->allocations.push_back(q);
+    s->device_bytes += n * sizeof(T);
     *p = (T *)q;
     return UMCP_OK;
 }
@@ -136,6 +137,31 @@ RestraintArgs restraint_args(umcp_surface *s, const double *d_r, bool area, bool
     ra.do_area = area ? 1 : 0;
     ra.do_volume = volume ? 1 : 0;
     return ra;
+}
+
+// mirrors every dev_alloc of umcp_plan (zero-sized requests allocate one element)
+size_t plan_bytes(const umcp_surface *s)
+{
+    auto sz = [](size_t n, size_t elem) { return (n ? n : 1) * elem; };
+    size_t b = 0;
+    int metric_total = 0, ngroups = 0, nApart = 0;
+    for (const FaceClass &C : s->P.classes) {
+        const size_t nf = C.nf();
+        b += sz(C.cp.size(), 4) + sz(C.pos.size(), 4) + sz(C.img.size(), 1) + sz(nf, 1);
+        b += 3 * sz(nf * C.npt, 8) + 3 * sz(nf, 8);
+        metric_total += metric_blocks(C.nf());
+        ngroups += face_groups(C.nf());
+        nApart += (C.nf() + kRestraintBlock - 1) / kRestraintBlock;
+    }
+    const size_t nv = s->T.nv;
+    size_t nbl = 0;
+    for (size_t v = 0; v < nv; v++) nbl += s->T.valence[v];
+    b += 2 * sz(3 * nv + 3, 8) + sz(3 * (size_t)s->P.nnz, 8) + sz(3 * (size_t)s->P.nnz, 4);
+    b += sz(4 * (size_t)std::max(ngroups, metric_total), 8) + sz(8, 8) + sz(s->P.vstart.size(), 4);
+    b += sz(8, 8) + sz(3 * nv, 8) + sz(nApart, 8) + sz((nv + kRestraintBlock - 1) / kRestraintBlock, 8);
+    b += sz(nv + 1, 4) + sz(nbl, 4) + sz(1, 4);
+    b += sz(s->P.classes.size(), sizeof(FaceArgs)) + sz(ngroups, sizeof(FaceItem));
+    return b;
 }
 
 }  // namespace umcp
@@ -352,6 +378,30 @@ int umcp_prepare(umcp_surface *s)
     std::string err;
     if (build_plan(s->T, s->P, err) != 0) return fail(s, UMCP_ERR_BAD_MESH, err);
     s->prepared = true;
+    return UMCP_OK;
+}
+
+int umcp_clone(const umcp_surface *tmpl, umcp_surface **out)
+{
+    if (!tmpl || !out) return UMCP_ERR_ARG;
+    *out = nullptr;
+    if (!tmpl->prepared) return UMCP_ERR_STATE;
+    auto *s = new umcp_surface();
+    s->device = g_device;
+    s->T = tmpl->T;
+    s->P = tmpl->P;
+    s->prm = tmpl->prm;
+    s->prepared = true;
+    *out = s;
+    return UMCP_OK;
+}
+
+int umcp_device_bytes(const umcp_surface *s, size_t *planned, size_t *allocated)
+{
+    if (!s) return UMCP_ERR_ARG;
+    if (!s->prepared) return UMCP_ERR_STATE;
+    if (planned) *planned = plan_bytes(s);
+    if (allocated) *allocated = s->device_bytes;
     return UMCP_OK;
 }
 

@@ -33,6 +33,7 @@ struct umcp_surface {
     std::vector<double> face_Lo, face_Li;
     std::vector<umcp::ClassDev> cls;
     std::vector<void *> allocations;
+    size_t device_bytes = 0;              // sum of the allocation sizes
     // work buffers
     double *d_r = nullptr, *d_g = nullptr, *d_gbuf = nullptr, *d_partial = nullptr;
     float *d_gbuf32 = nullptr;
@@ -79,6 +80,9 @@ namespace umcp {
 
 // helpers of the surface object used by both translation units
 bool restraints_on(const umcp_surface *s);
+// device bytes umcp_plan allocates for a prepared surface (the serving
+// loop's admission footprint); equals device_bytes after the plan
+size_t plan_bytes(const umcp_surface *s);
 FaceArgs face_args(umcp_surface *s, size_t c, const double *d_r);
 RestraintArgs restraint_args(umcp_surface *s, const double *d_r, bool area, bool volume);
 

@@ -38,6 +38,8 @@ EXPORTS = [
     "umcp_noise", "umcp_ensemble_create", "umcp_ensemble_destroy", "umcp_ensemble_last_error",
     "umcp_ensemble_add", "umcp_ensemble_remove", "umcp_ensemble_size", "umcp_ensemble_refresh",
     "umcp_ensemble_set_launch", "umcp_ensemble_get_launch", "umcp_ensemble_bd_steps", "umcp_ensemble_work",
+    "umcp_clone", "umcp_device_bytes", "umcp_server_create", "umcp_server_submit", "umcp_server_wait",
+    "umcp_server_drain", "umcp_server_stats", "umcp_server_last_error", "umcp_server_destroy",
 ]
 
 
@@ -57,6 +59,18 @@ class BDParams(C.Structure):
 
 class Restraints(C.Structure):
     _fields_ = [("KA_global", C.c_double), ("KV", C.c_double), ("A0", C.c_double), ("V0", C.c_double)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("budget_bytes", C.c_double), ("max_active", C.c_int), ("chunk_steps", C.c_int),
+                ("precision", C.c_int), ("point_split", C.c_int), ("flags", C.c_int), ("face_grid", C.c_int),
+                ("order", C.c_int), ("graph_steps", C.c_int)]
+
+
+class Request(C.Structure):
+    _fields_ = [("mesh_path", C.c_char_p), ("nv", C.c_int), ("params", Params), ("c0", C.c_double),
+                ("steps", C.c_int), ("seed", C.c_ulonglong), ("bd", BDParams),
+                ("r0", C.POINTER(C.c_double)), ("r", C.POINTER(C.c_double))]
 
 
 _lib = None
@@ -119,6 +133,15 @@ def load_library(path: str = LIB_PATH):
         "umcp_ensemble_get_launch": ([vp, ip, ip, ip, ip], C.c_int),
         "umcp_ensemble_bd_steps": ([vp, C.c_int, C.c_int, vp], C.c_int),
         "umcp_ensemble_work": ([vp, ip, ip, ip], C.c_int),
+        "umcp_clone": ([vp, C.POINTER(vp)], C.c_int),
+        "umcp_device_bytes": ([vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], C.c_int),
+        "umcp_server_create": ([C.POINTER(ServerConfig), C.POINTER(vp)], C.c_int),
+        "umcp_server_submit": ([vp, C.POINTER(Request), ip], C.c_int),
+        "umcp_server_wait": ([vp, C.c_int, dp, dp, dp], C.c_int),
+        "umcp_server_drain": ([vp], C.c_int),
+        "umcp_server_stats": ([vp, dp], C.c_int),
+        "umcp_server_last_error": ([vp], C.c_char_p),
+        "umcp_server_destroy": ([vp], None),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -415,3 +438,72 @@ class Ensemble:
         v = [C.c_int() for _ in range(3)]
         self._check(self._L.umcp_ensemble_work(self._h, *[C.byref(x) for x in v]), "ensemble_work")
         return tuple(x.value for x in v)
+
+
+class Server:
+    """The serving loop: Brownian-dynamics requests admitted as they arrive
+    and stepped together on the device (umcp_server_* in include/umcp.h)."""
+
+    def __init__(self, budget_bytes: float = 200e9, max_active: int = 64, chunk_steps: int = 100,
+                 precision: int = PREC_MIXED, point_split: int = 4, flags: int = LAUNCH_OCC2,
+                 face_grid: int = 0, order: int = 0, graph_steps: int = 0):
+        self._L = load_library()
+        cfg = ServerConfig(budget_bytes, max_active, chunk_steps, precision, point_split, flags, face_grid,
+                           order, graph_steps)
+        h = C.c_void_p()
+        rc = self._L.umcp_server_create(C.byref(cfg), C.byref(h))
+        if rc != 0:
+            raise UmcpError("server_create failed (%d)" % rc)
+        self._h = h
+        self._pending = {}
+
+    def submit(self, mesh_path: str, nv: int, params: dict, c0: float, steps: int, seed: int,
+               mobility_dt: float, noise_amp: float, r0: np.ndarray, r: np.ndarray) -> int:
+        p = dict(kc=14.0, kg=0.0, KA=0.0, k_reg=0.0, mode=MODE_LIPID, precision=PREC_FP64, lipid_c0=0.0,
+                 lipid_apl=68.3)
+        p.update(params)
+        r0 = np.ascontiguousarray(r0, dtype=np.float64)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        if r0.size != 3 * nv + 3 or r.size != 3 * nv + 3:
+            raise UmcpError("positions must hold 3nv+3 doubles")
+        rq = Request(mesh_path.encode(), nv,
+                     Params(p["kc"], p["kg"], p["KA"], p["k_reg"], p["mode"], p["precision"], p["lipid_c0"],
+                            p["lipid_apl"]), c0, int(steps), C.c_ulonglong(seed), BDParams(mobility_dt, noise_amp),
+                     _dp(r0), _dp(r))
+        rid = C.c_int()
+        rc = self._L.umcp_server_submit(self._h, C.byref(rq), C.byref(rid))
+        if rc != 0:
+            raise UmcpError("server_submit failed (%d)" % rc)
+        self._pending[rid.value] = (nv, int(steps))
+        return rid.value
+
+    def wait(self, rid: int):
+        """(positions, energies per step, times [submitted, admitted, first step, done])."""
+        nv, steps = self._pending.pop(rid)
+        r = np.zeros(3 * nv + 3)
+        e = np.zeros(max(steps, 1))
+        t = np.zeros(4)
+        rc = self._L.umcp_server_wait(self._h, rid, _dp(r), _dp(e), _dp(t))
+        if rc != 0:
+            raise UmcpError("request %d failed (%d): %s" % (rid, rc, self._L.umcp_server_last_error(self._h).decode()))
+        return r, e[:steps], t
+
+    def drain(self):
+        self._L.umcp_server_drain(self._h)
+
+    def stats(self) -> dict:
+        v = np.zeros(8)
+        self._L.umcp_server_stats(self._h, _dp(v))
+        keys = ("done", "chunks", "request_steps", "face_steps", "busy_s", "peak_bytes", "peak_active", "clock_s")
+        return dict(zip(keys, v.tolist()))
+
+    def close(self):
+        if self._h:
+            self._L.umcp_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

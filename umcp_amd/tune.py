@@ -34,3 +34,37 @@ def tune(surface, d_r, d_E, d_g, stream=None, reps: int = 10, sync=None):
     best = min(times, key=times.get)
     surface.set_launch(*best)
     return best, times
+
+
+# batched ensembles: waves per face group, register bound, persistent grid
+# (0 = one workgroup per item) and item order
+ENSEMBLE_SHAPES = tuple((sp, fl, grid, order)
+                        for sp in (2, 4) for fl in (0, 2) for grid in (0, 1024, 2048) for order in (0, 1))
+
+
+def tune_ensemble(ens, surfaces, d_rs, torch, stream, steps: int = 8, shapes=ENSEMBLE_SHAPES):
+    """Times `steps` captured Brownian steps of the whole ensemble per launch
+    shape and keeps the fastest.  Positions and step counters are restored
+    afterwards, so tuning leaves the trajectory untouched.
+    Returns (best shape, {shape: ms per step})."""
+    saved = [d.clone() for d in d_rs]
+    torch.cuda.synchronize()
+    times = {}
+    sptr = stream.cuda_stream
+    for shape in shapes:
+        ens.set_launch(*shape)
+        ens.bd_steps(steps, steps, sptr)          # build, capture, first replay
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        ens.bd_steps(steps, steps, sptr)
+        end.record(stream)
+        torch.cuda.synchronize()
+        times[shape] = start.elapsed_time(end) / steps
+    best = min(times, key=times.get)
+    ens.set_launch(*best)
+    for d, s in zip(d_rs, saved):
+        d.copy_(s)
+    for surf in surfaces:
+        surf.set_step(0)
+    torch.cuda.synchronize()
+    return best, times
