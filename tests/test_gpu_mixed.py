@@ -59,6 +59,8 @@ def test_mixed_vesicle_40962():
     w = workloads.vesicle_40962()
     c0 = workloads.face_c0_uniform(81920, w.extra["c0_seed"])
     g, o = pair(w, c0=c0)
+    g.set_restraints(**w.extra["restraints"])
+    o.set_restraints(**w.extra["restraints"])
     rel_e, rms = errors(g, o, w.r, "vesicle_40962")
     assert rel_e <= E_TOL and rms <= G_RMS_TOL
 

@@ -114,7 +114,9 @@ def test_vesicle_40962_full_size():
     c0 = workloads.face_c0_uniform(g.nfaces, w.extra["c0_seed"])
     g.set_face_c0(c0)
     o.set_face_c0(c0)
-    check_eval(g, o, w.r, "vesicle_40962 helfrich c0")
+    g.set_restraints(**w.extra["restraints"])
+    o.set_restraints(**w.extra["restraints"])
+    check_eval(g, o, w.r, "vesicle_40962 helfrich c0, area and volume restraints")
 
 
 def test_grad_accumulates_and_is_deterministic(ves642):

@@ -11,7 +11,8 @@ parameters in reference units (kcal/mol, Angstrom).
 * ``sheet_128``      periodic 128 x 128 hex lattice in a 1000 x 866 A box,
   heights = sum of Fourier modes with amplitude ~ q^-2 (kc = 20 kT, T = 310 K).
 * ``vesicle_40962``  icosahedron subdivided 6 times, radius 1000 A, radial
-  N(0, 2^2) noise, kc = 20 kT, per-face c0 ~ U[-0.01, 0.01] 1/A (Helfrich form).
+  N(0, 2^2) noise, kc = 20 kT, per-face c0 ~ U[-0.01, 0.01] 1/A (Helfrich form),
+  global area and volume restraints.
 
 Reference state: the unperturbed mesh (r0) is the setg0 state; the energy is
 evaluated at the perturbed r.
@@ -79,11 +80,19 @@ def vesicle_642(seed: int = 1) -> Workload:
     return vesicle(3, 100.0, 0.5, seed, name="vesicle_642")
 
 
-def vesicle_40962(seed: int = 3, with_c0: bool = True) -> Workload:
+def vesicle_40962(seed: int = 3, with_c0: bool = True, with_restraints: bool = True) -> Workload:
+    """The large vesicle; with_restraints: global area and volume restraints
+    (``extra["restraints"]``: KA_global = 0.215 kT/A^2 on the total area, as
+    the reference's GLOBAL_AREA build, which drops the per-point KA term, and
+    KV = 0.0056 kcal/mol/A^3, the reference's water_KV default,
+    src/interpLib.C:91), targets = the setg0 state."""
     w = vesicle(6, 1000.0, 2.0, seed, radial=True, name="vesicle_40962")
     if with_c0:
         w.params["mode"] = 1
         w.extra["c0_seed"] = seed + 100
+    if with_restraints:
+        w.extra["restraints"] = dict(KA_global=w.params["KA"], KV=0.0056)
+        w.params["KA"] = 0.0
     return w
 
 
