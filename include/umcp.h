@@ -203,8 +203,10 @@ int umcp_ensemble_get_launch(const umcp_ensemble *e, int *point_split, int *flag
  * umcp_bd_steps_rng_dev); graph_steps > 0 replays a captured graph of that
  * many steps; stream NULL = the ensemble's own stream; asynchronous */
 int umcp_ensemble_bd_steps(umcp_ensemble *e, int nsteps, int graph_steps, void *stream);
-/* work items of the current tables (after the first umcp_ensemble_bd_steps) */
-int umcp_ensemble_work(const umcp_ensemble *e, int *face_items, int *gather_items, int *restraint_items);
+/* work of the current tables (after the first umcp_ensemble_bd_steps): face
+ * items (64-face groups), gather items (256-vertex blocks), members with
+ * restraints */
+int umcp_ensemble_work(const umcp_ensemble *e, int *face_items, int *gather_items, int *restrained);
 
 /* Serving loop (no reference counterpart: the reference runs one simulation
  * per process).  Requests (mesh file, parameters, spontaneous curvature, N

@@ -49,7 +49,7 @@ def make_pair(w, c0=None, params=None):
     return g, o
 
 
-def check_eval(g, o, r, tag):
+def check_eval(g, o, r, tag, alpha_tol=1e-11):
     say("[parity]", tag, "nv", g.nv, "faces", g.nfaces)
     E_o = o.energy(r)
     g_o = o.grad(r)
@@ -60,7 +60,7 @@ def check_eval(g, o, r, tag):
     say("   E", E_g, E_o, "rel", rel_e, "grad rel", rel_g, "alpha rel", rel_a)
     assert rel_e <= E_TOL
     assert rel_g <= G_TOL
-    assert rel_a <= 1e-11
+    assert rel_a <= alpha_tol
     return E_g, g_g
 
 
@@ -116,7 +116,11 @@ def test_vesicle_40962_full_size():
     o.set_face_c0(c0)
     g.set_restraints(**w.extra["restraints"])
     o.set_restraints(**w.extra["restraints"])
-    check_eval(g, o, w.r, "vesicle_40962 helfrich c0, area and volume restraints")
+    # With the area restraint, dE/dalpha is dominated by lamA * 2A, lamA =
+    # 2 KA_g (A - A0)/A0.  A - A0 is ~4e-6 of A here, so the ~1e-13 relative
+    # rounding of the two 1.2M-point area sums (GPU tree order vs the serial
+    # oracle) becomes ~1e-8 in lamA: the alpha bound is that conditioning.
+    check_eval(g, o, w.r, "vesicle_40962 helfrich c0, area and volume restraints", alpha_tol=2e-8)
 
 
 def test_grad_accumulates_and_is_deterministic(ves642):

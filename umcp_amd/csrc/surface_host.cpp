@@ -452,6 +452,15 @@ int build_plan(const Topology &T, Plan &P, std::string &err)
         R.weight.assign(kRegPts, 1.0 / 15);
         for (int p = 0; p < kRegPts; p++)
             box_spline_basis(kRegGauss[p][0], kRegGauss[p][1], &R.table[(size_t)p * 72]);
+        // corners of the volume polyhedron: the box spline at the three corners
+        // (src/interpLib.C:426-461 evaluates the volume plan the same way)
+        const double uv[3][2] = {{0.0, 0.0}, {1.0, 0.0}, {0.0, 1.0}};
+        R.corner.resize(3 * 12);
+        for (int c = 0; c < 3; c++) {
+            double B[6 * 12];
+            box_spline_basis(uv[c][0], uv[c][1], B);
+            for (int y = 0; y < 12; y++) R.corner[c * 12 + y] = B[y];
+        }
     }
     struct Face { int cls; int cp[kMaxCp]; uint8_t img[kMaxCp]; bool any; };
     std::vector<Face> faces;
@@ -510,6 +519,19 @@ int build_plan(const Topology &T, Plan &P, std::string &err)
                         irregular_patch_table(val, pts[p][0], pts[p][1], &C.table[(size_t)p * 6 * ncp]);
                         C.weight[p] = pts[p][2];
                     }
+                    // corners: the owner's limit point is the top eigenvector of
+                    // the stencil (src/interpLib.C:455-459), Loop's limit mask
+                    // 1/2 on the vertex and 1/(2 val) on its ring (slots 1..val)
+                    // for the stencil weight 3/(8 val); the other two corners by
+                    // the Stam evaluation
+                    C.corner.assign(3 * (size_t)ncp, 0.0);
+                    C.corner[0] = 0.5;
+                    for (int x = 1; x <= val; x++) C.corner[x] = 0.5 / val;
+                    std::vector<double> tb(6 * (size_t)ncp);
+                    irregular_patch_table(val, 1.0, 0.0, tb.data());
+                    for (int x = 0; x < ncp; x++) C.corner[ncp + x] = tb[x];
+                    irregular_patch_table(val, 0.0, 1.0, tb.data());
+                    for (int x = 0; x < ncp; x++) C.corner[2 * ncp + x] = tb[x];
                 }
                 F.cls = cls_of_val[val];
                 irregular_order.push_back((int)faces.size());

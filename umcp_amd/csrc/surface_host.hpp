@@ -41,6 +41,9 @@ struct FaceClass {
     std::vector<uint8_t> has_img;      // [nf]        any image shift on this face
     std::vector<double> table;         // [npt][6][ncp] r_w, r_u, r_v, r_uu, r_uv, r_vv
     std::vector<double> weight;        // [npt]
+    // position weights of the three corners (u, v) = (0,0), (1,0), (0,1): the
+    // limit points of the owner i, of ring[e] and of ring[e+1]  [3][ncp]
+    std::vector<double> corner;
     std::vector<int> pos;              // [ncp][nf]   CSR position of (face, slot)
     int nf() const { return (int)face_id.size(); }
 };

@@ -110,33 +110,18 @@ FaceArgs face_args(umcp_surface *s, size_t c, const double *d_r)
     a.gbuf = s->d_gbuf;
     a.gbuf32 = s->d_gbuf32;
     a.ph = phys(s);
-    a.rs = restraints_on(s) ? s->d_rs : nullptr;
     a.step_ctr = nullptr;
     a.partial = s->d_partial;
+    a.rs = restraints_on(s) ? s->d_rs : nullptr;
+    a.rpart = s->d_rpart;
+    a.ngroups = s->npartial;
+    a.KA_g = s->rsp.KA_global;
+    a.KV = s->rsp.KV;
+    a.A0 = s->rsp.A0 > 0.0 ? s->rsp.A0 : s->A0_setg0;
+    a.V0 = s->rsp.V0 != 0.0 ? s->rsp.V0 : s->V_setg0;
+    a.do_area = s->rsp.KA_global != 0.0 ? 1 : 0;
+    a.do_volume = s->rsp.KV != 0.0 ? 1 : 0;
     return a;
-}
-
-// restraint arguments on positions d_r; volume/area switched by the caller
-RestraintArgs restraint_args(umcp_surface *s, const double *d_r, bool area, bool volume)
-{
-    RestraintArgs ra{};
-    ra.r = d_r;
-    ra.nv = s->T.nv;
-    ra.nb_start = s->d_nb_start;
-    ra.nb = s->d_nb;
-    ra.Dh = s->d_Dh;
-    ra.Apart = s->d_Apart;
-    ra.Vpart = s->d_Vpart;
-    ra.nApart = area ? s->nApart : 0;
-    ra.nVpart = volume ? s->nVpart : 0;
-    ra.rs = s->d_rs;
-    ra.KA_g = s->rsp.KA_global;
-    ra.KV = s->rsp.KV;
-    ra.A0 = s->rsp.A0 > 0.0 ? s->rsp.A0 : s->A0_setg0;
-    ra.V0 = s->rsp.V0 != 0.0 ? s->rsp.V0 : s->V_setg0;
-    ra.do_area = area ? 1 : 0;
-    ra.do_volume = volume ? 1 : 0;
-    return ra;
 }
 
 // mirrors every dev_alloc of umcp_plan (zero-sized requests allocate one element)
@@ -144,23 +129,19 @@ size_t plan_bytes(const umcp_surface *s)
 {
     auto sz = [](size_t n, size_t elem) { return (n ? n : 1) * elem; };
     size_t b = 0;
-    int metric_total = 0, ngroups = 0, nApart = 0;
+    int metric_total = 0, ngroups = 0;
     for (const FaceClass &C : s->P.classes) {
         const size_t nf = C.nf();
         b += sz(C.cp.size(), 4) + sz(C.pos.size(), 4) + sz(C.img.size(), 1) + sz(nf, 1);
         b += 3 * sz(nf * C.npt, 8) + 3 * sz(nf, 8);
         metric_total += metric_blocks(C.nf());
         ngroups += face_groups(C.nf());
-        nApart += (C.nf() + kRestraintBlock - 1) / kRestraintBlock;
     }
     const size_t nv = s->T.nv;
-    size_t nbl = 0;
-    for (size_t v = 0; v < nv; v++) nbl += s->T.valence[v];
     b += 2 * sz(3 * nv + 3, 8) + sz(3 * (size_t)s->P.nnz, 8) + sz(3 * (size_t)s->P.nnz, 4);
     b += sz(4 * (size_t)std::max(ngroups, metric_total), 8) + sz(8, 8) + sz(s->P.vstart.size(), 4);
-    b += sz(8, 8) + sz(3 * nv, 8) + sz(nApart, 8) + sz((nv + kRestraintBlock - 1) / kRestraintBlock, 8);
-    b += sz(nv + 1, 4) + sz(nbl, 4) + sz(1, 4);
-    b += sz(s->P.classes.size(), sizeof(FaceArgs)) + sz(ngroups, sizeof(FaceItem));
+    b += sz(8, 8) + sz(2 * (size_t)ngroups, 8) + sz(1, 4) + sz(1, 4);   // rs, rpart, zero, step
+    b += 2 * sz(s->P.classes.size(), sizeof(FaceArgs)) + sz(ngroups, sizeof(FaceItem));
     return b;
 }
 
@@ -168,21 +149,25 @@ size_t plan_bytes(const umcp_surface *s)
 
 namespace {
 
-// area and/or volume partials of d_r, then the finalize into d_rs
-hipError_t enqueue_restraints(umcp_surface *s, const double *d_r, bool area, bool volume, hipStream_t st)
+// volume of the limit-point polyhedron of d_r through the restraint pre-pass
+// (a volume-only face table); the result lands in d_rs[4]
+int enqueue_volume(umcp_surface *s, const double *d_r, hipStream_t st)
 {
-    RestraintLaunch L{};
-    L.ncls = (int)s->P.classes.size();
-    L.area_start[0] = 0;
-    for (int c = 0; c < L.ncls; c++) {
-        L.cls[c] = face_args(s, c, d_r);
-        L.kind[c] = kind_of(s->P.classes[c].ncp, s->P.classes[c].npt);
-        L.area_start[c + 1] = L.area_start[c] + (area ? s->area_start[c + 1] - s->area_start[c] : 0);
+    std::vector<FaceArgs> fa(s->P.classes.size());
+    for (size_t c = 0; c < fa.size(); c++) {
+        fa[c] = face_args(s, c, d_r);
+        fa[c].rs = s->d_rs;
+        fa[c].do_area = 0;
+        fa[c].do_volume = 1;
+        fa[c].KA_g = 0.0;
+        fa[c].KV = 0.0;
     }
-    L.ra = restraint_args(s, d_r, area, volume);
-    hipError_t e = launch_restraints(L, st);
-    if (e == hipSuccess) e = launch_restraint_finalize(nullptr, L.ra, 1, st);
-    return e;
+    hipError_t e = hipMemcpyAsync(s->d_fa_aux, fa.data(), fa.size() * sizeof(FaceArgs), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(s, e, "upload face table");
+    BatchFaceArgs fb{s->d_fa_aux, s->d_items, s->npartial, d_r, nullptr};
+    e = launch_restraint_prepass(fb, s->d_zero, 1, st);
+    if (e != hipSuccess) return hip_fail(s, e, "restraint kernel launch");
+    return UMCP_OK;
 }
 
 // irregular classes first so their few blocks start early and overlap the
@@ -220,10 +205,6 @@ int prof_pair(umcp_surface *s, int cls, hipEvent_t *ev)
 int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs ga, int *step_ctr = nullptr)
 {
     const bool rson = restraints_on(s);
-    if (rson) {
-        hipError_t e = enqueue_restraints(s, d_r, s->rsp.KA_global != 0.0, s->rsp.KV != 0.0, st);
-        if (e != hipSuccess) return hip_fail(s, e, "restraint kernel launch");
-    }
     if (s->fa_dirty) {
         s->h_fa.resize(s->P.classes.size());
         for (size_t c = 0; c < s->P.classes.size(); c++) s->h_fa[c] = face_args(s, c, s->d_r);
@@ -238,6 +219,10 @@ int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs 
     fb.nitems = s->npartial;
     fb.r = d_r;
     fb.step_ctr = step_ctr;
+    if (rson) {
+        hipError_t e = launch_restraint_prepass(fb, s->d_zero, 1, st);
+        if (e != hipSuccess) return hip_fail(s, e, "restraint kernel launch");
+    }
     hipEvent_t ev[2] = {nullptr, nullptr};
     if (s->prof_on) {
         int rc = prof_pair(s, 0, ev);
@@ -254,14 +239,7 @@ int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs 
     ga.gbuf32 = mixed ? s->d_gbuf32 : nullptr;
     ga.partial = s->d_partial;
     ga.npartial = s->npartial;
-    if (rson) {
-        ga.rs = s->d_rs;
-        if (s->rsp.KV != 0.0) {
-            ga.Dh = s->d_Dh;
-            ga.nb_start = s->d_nb_start;
-            ga.nb = s->d_nb;
-        }
-    }
+    if (rson) ga.rs = s->d_rs;
     e = launch_vertex_gather(ga, st);
     if (e != hipSuccess) return hip_fail(s, e, "gather kernel launch");
     return UMCP_OK;
@@ -434,7 +412,7 @@ int umcp_plan(umcp_surface *s)
             (rc = dev_alloc(s, &D.Li, nf)) || (rc = dev_alloc(s, &D.c0, nf)) ||
             0)
             return rc;
-        e = upload_tables(kind_of(C.ncp, C.npt), C.table.data(), C.weight.data(), C.npt, C.ncp);
+        e = upload_tables(kind_of(C.ncp, C.npt), C.table.data(), C.weight.data(), C.corner.data(), C.npt, C.ncp);
         if (e != hipSuccess) return hip_fail(s, e, "upload Gauss-point tables");
         if ((rc = dev_upload(s, D.cp, C.cp.data(), C.cp.size())) ||
             (rc = dev_upload(s, D.pos, C.pos.data(), C.pos.size())) ||
@@ -459,25 +437,12 @@ int umcp_plan(umcp_surface *s)
         (rc = dev_alloc(s, &s->d_vstart, s->P.vstart.size())))
         return rc;
     if ((rc = dev_upload(s, s->d_vstart, s->P.vstart.data(), s->P.vstart.size()))) return rc;
-    // restraint work: oriented rings as CSR, area blocks per class, partials
-    std::vector<int> nbs(nv + 1, 0), nbl;
-    for (int v = 0; v < nv; v++) {
-        for (int e2 = 0; e2 < s->T.valence[v]; e2++) nbl.push_back(s->T.nb(v, e2));
-        nbs[v + 1] = (int)nbl.size();
-    }
-    s->area_start.assign(1, 0);
+    // restraint work: per-group partials, completion count, scalars
     s->has_images = false;
-    for (size_t c = 0; c < s->P.classes.size(); c++) {
-        const FaceClass &C = s->P.classes[c];
-        s->area_start.push_back(s->area_start.back() + (C.nf() + kRestraintBlock - 1) / kRestraintBlock);
+    for (const FaceClass &C : s->P.classes)
         for (uint8_t h : C.has_img) s->has_images = s->has_images || h;
-    }
-    s->nApart = s->area_start.back();
-    s->nVpart = (nv + kRestraintBlock - 1) / kRestraintBlock;
-    if ((rc = dev_alloc(s, &s->d_rs, 8)) || (rc = dev_alloc(s, &s->d_Dh, 3 * (size_t)nv)) ||
-        (rc = dev_alloc(s, &s->d_Apart, s->nApart)) || (rc = dev_alloc(s, &s->d_Vpart, s->nVpart)) ||
-        (rc = dev_alloc(s, &s->d_nb_start, nbs.size())) || (rc = dev_alloc(s, &s->d_nb, nbl.size())) ||
-        (rc = dev_alloc(s, &s->d_step, 1)))
+    if ((rc = dev_alloc(s, &s->d_rs, 8)) || (rc = dev_alloc(s, &s->d_rpart, 2 * (size_t)s->npartial)) ||
+        (rc = dev_alloc(s, &s->d_zero, 1)) || (rc = dev_alloc(s, &s->d_step, 1)))
         return rc;
     // face work tables
     std::vector<FaceItem> items;
@@ -487,14 +452,15 @@ int umcp_plan(umcp_surface *s)
         for (int g = 0; g < face_groups(C.nf()); g++)
             items.push_back(FaceItem{c, g, s->block_start[i] + g, kind_of(C.ncp, C.npt)});
     }
-    if ((rc = dev_alloc(s, &s->d_fa_tab, s->P.classes.size())) || (rc = dev_alloc(s, &s->d_items, items.size())) ||
+    if ((rc = dev_alloc(s, &s->d_fa_tab, s->P.classes.size())) || (rc = dev_alloc(s, &s->d_fa_aux, s->P.classes.size())) ||
+        (rc = dev_alloc(s, &s->d_items, items.size())) ||
         (rc = dev_upload(s, s->d_items, items.data(), items.size())))
         return rc;
     s->fa_dirty = true;
     const int zero_step = 0;
     const double zeros8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if ((rc = dev_upload(s, s->d_nb_start, nbs.data(), nbs.size())) || (rc = dev_upload(s, s->d_nb, nbl.data(), nbl.size())) ||
-        (rc = dev_upload(s, s->d_step, &zero_step, 1)) || (rc = dev_upload(s, s->d_rs, zeros8, 8)))
+    if ((rc = dev_upload(s, s->d_step, &zero_step, 1)) || (rc = dev_upload(s, s->d_zero, &zero_step, 1)) ||
+        (rc = dev_upload(s, s->d_rs, zeros8, 8)))
         return rc;
     if ((rc = sync(s))) return rc;
     s->planned = true;
@@ -563,16 +529,16 @@ int umcp_setg0(umcp_surface *s, const double *r)
     // (src/hd.C:273-279)
     s->V_setg0 = 0.0;
     if (!s->has_images) {
-        hipError_t e = enqueue_restraints(s, s->d_r, false, true, s->stream);
-        if (e != hipSuccess) return hip_fail(s, e, "volume kernel launch");
+        if ((rc = enqueue_volume(s, s->d_r, s->stream))) return rc;
         double h[8];
-        e = hipMemcpyAsync(h, s->d_rs, sizeof h, hipMemcpyDeviceToHost, s->stream);
+        hipError_t e = hipMemcpyAsync(h, s->d_rs, sizeof h, hipMemcpyDeviceToHost, s->stream);
         if (e != hipSuccess) return hip_fail(s, e, "download volume");
         if ((rc = sync(s))) return rc;
         s->V_setg0 = h[4];
     }
     if ((rc = sync(s))) return rc;
     s->g0_set = true;
+    s->fa_dirty = true;   // restraint targets may default to this state
     return UMCP_OK;
 }
 
@@ -678,10 +644,9 @@ int umcp_volume(umcp_surface *s, const double *r, double *V)
     if (!r || !V) return fail(s, UMCP_ERR_ARG, "null argument");
     if (s->has_images) return fail(s, UMCP_ERR_ARG, "no enclosed volume: the mesh has periodic images");
     if ((rc = upload_r(s, r))) return rc;
-    hipError_t e = enqueue_restraints(s, s->d_r, false, true, s->stream);
-    if (e != hipSuccess) return hip_fail(s, e, "volume kernel launch");
+    if ((rc = enqueue_volume(s, s->d_r, s->stream))) return rc;
     double h[8];
-    e = hipMemcpyAsync(h, s->d_rs, sizeof h, hipMemcpyDeviceToHost, s->stream);
+    hipError_t e = hipMemcpyAsync(h, s->d_rs, sizeof h, hipMemcpyDeviceToHost, s->stream);
     if (e != hipSuccess) return hip_fail(s, e, "download volume");
     if ((rc = sync(s))) return rc;
     *V = h[4];

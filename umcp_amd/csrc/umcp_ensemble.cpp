@@ -4,8 +4,8 @@
 // HBM with its own tables and work buffers, and one step of all members is
 // three launches over device-resident work tables:
 //
-//   [restraints + finalize]  area / volume partials of the members that have
-//                            restraints, one finalize block per such member
+//   restraint pre-pass       area / volume partials and multipliers of the
+//                            members that have restraints (same items)
 //   faces_batch_kernel       one item per 64-face group of every member
 //   vertex_gather_batch      one item per 256-vertex block of every member
 //
@@ -51,10 +51,8 @@ struct umcp_ensemble {
     FaceItem *d_items = nullptr;
     GatherArgs *d_ga = nullptr;
     GatherItem *d_gitems = nullptr;
-    RestraintArgs *d_ra = nullptr;
-    int *d_fa_surface = nullptr;
-    RestraintItem *d_ritems = nullptr;
-    int nitems = 0, ngitems = 0, nritems = 0, nra = 0;
+    int *d_first = nullptr;       // FaceArgs entry (class 0) of each member with restraints
+    int nitems = 0, ngitems = 0, nrestrained = 0;
     std::vector<void *> tables;   // allocations of the current build
     // captured step sequence
     hipGraph_t graph = nullptr;
@@ -94,10 +92,8 @@ void drop_tables(umcp_ensemble *e)
     e->d_items = nullptr;
     e->d_ga = nullptr;
     e->d_gitems = nullptr;
-    e->d_ra = nullptr;
-    e->d_fa_surface = nullptr;
-    e->d_ritems = nullptr;
-    e->nitems = e->ngitems = e->nritems = e->nra = 0;
+    e->d_first = nullptr;
+    e->nitems = e->ngitems = e->nrestrained = 0;
 }
 
 template <typename T>
@@ -135,34 +131,22 @@ int build(umcp_ensemble *e)
     e->mixed = prec == UMCP_PREC_MIXED;
 
     std::vector<FaceArgs> fa;
-    std::vector<int> fa_surface;
     std::vector<std::vector<FaceItem>> per_member(e->members.size());
     std::vector<GatherArgs> ga;
     std::vector<GatherItem> gitems;
-    std::vector<RestraintArgs> ra;
-    std::vector<RestraintItem> ritems;
+    std::vector<int> first;
     for (size_t mi = 0; mi < e->members.size(); mi++) {
         const auto &m = e->members[mi];
         umcp_surface *s = m.s;
         const bool rson = restraints_on(s);
-        const bool area = s->rsp.KA_global != 0.0, volume = s->rsp.KV != 0.0;
-        const int ri = rson ? (int)ra.size() : -1;
-        if (rson) ra.push_back(restraint_args(s, m.d_r, area, volume));
+        if (rson) first.push_back((int)fa.size());
         std::vector<int> fa_of_class(s->P.classes.size());
         for (size_t c = 0; c < s->P.classes.size(); c++) {
             FaceArgs a = face_args(s, c, m.d_r);
             a.step_ctr = s->d_step;
             fa_of_class[c] = (int)fa.size();
             fa.push_back(a);
-            fa_surface.push_back(ri);
-            if (rson && area) {
-                const int kind = kind_of(s->P.classes[c].ncp, s->P.classes[c].npt);
-                for (int b = 0; b < s->area_start[c + 1] - s->area_start[c]; b++)
-                    ritems.push_back(RestraintItem{kind, fa_of_class[c], b, s->area_start[c] + b});
-            }
         }
-        if (rson && volume)
-            for (int b = 0; b < s->nVpart; b++) ritems.push_back(RestraintItem{3, ri, b, b});
         // face items in the member's own launch order and partial slots
         for (size_t i = 0; i < s->launch_order.size(); i++) {
             const int c = s->launch_order[i];
@@ -187,14 +171,7 @@ int build(umcp_ensemble *e)
         g.noise = kNoisePhilox;
         g.seed = m.seed;
         g.step_ctr = s->d_step;
-        if (rson) {
-            g.rs = s->d_rs;
-            if (volume) {
-                g.Dh = s->d_Dh;
-                g.nb_start = s->d_nb_start;
-                g.nb = s->d_nb;
-            }
-        }
+        if (rson) g.rs = s->d_rs;
         const int gi = (int)ga.size();
         ga.push_back(g);
         for (int b = 0; b < gather_blocks(s->T.nv); b++) gitems.push_back(GatherItem{gi, b});
@@ -221,13 +198,11 @@ int build(umcp_ensemble *e)
     int rc;
     if ((rc = upload_table(e, &e->d_fa, fa)) || (rc = upload_table(e, &e->d_items, items)) ||
         (rc = upload_table(e, &e->d_ga, ga)) || (rc = upload_table(e, &e->d_gitems, gitems)) ||
-        (rc = upload_table(e, &e->d_ra, ra)) || (rc = upload_table(e, &e->d_fa_surface, fa_surface)) ||
-        (rc = upload_table(e, &e->d_ritems, ritems)))
+        (rc = upload_table(e, &e->d_first, first)))
         return rc;
     e->nitems = (int)items.size();
     e->ngitems = (int)gitems.size();
-    e->nritems = (int)ritems.size();
-    e->nra = (int)ra.size();
+    e->nrestrained = (int)first.size();
     err = hipStreamSynchronize(e->stream);
     if (err != hipSuccess) return ehip(e, err, "hipStreamSynchronize");
     return UMCP_OK;
@@ -236,10 +211,10 @@ int build(umcp_ensemble *e)
 hipError_t enqueue_step(umcp_ensemble *e, hipStream_t st)
 {
     hipError_t err = hipSuccess;
-    if (e->nra > 0) {
-        BatchRestraintArgs rb{e->d_fa, e->d_ra, e->d_fa_surface, e->d_ritems, e->nritems};
-        err = launch_restraints_batch(rb, st);
-        if (err == hipSuccess) err = launch_restraint_finalize(e->d_ra, RestraintArgs{}, e->nra, st);
+    if (e->nrestrained > 0) {
+        // over every item: groups of members without restraints return at once
+        BatchFaceArgs rb{e->d_fa, e->d_items, e->nitems, nullptr, nullptr};
+        err = launch_restraint_prepass(rb, e->d_first, e->nrestrained, st);
         if (err != hipSuccess) return err;
     }
     for (int f = 0; f < 3; f++) {
@@ -388,12 +363,12 @@ int umcp_ensemble_bd_steps(umcp_ensemble *e, int nsteps, int graph_steps, void *
     return UMCP_OK;
 }
 
-int umcp_ensemble_work(const umcp_ensemble *e, int *face_items, int *gather_items, int *restraint_items)
+int umcp_ensemble_work(const umcp_ensemble *e, int *face_items, int *gather_items, int *restrained)
 {
     if (!e) return UMCP_ERR_ARG;
     if (face_items) *face_items = e->nitems;
     if (gather_items) *gather_items = e->ngitems;
-    if (restraint_items) *restraint_items = e->nritems;
+    if (restrained) *restrained = e->nrestrained;
     return UMCP_OK;
 }
 

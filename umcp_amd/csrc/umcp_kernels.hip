@@ -40,6 +40,18 @@ __constant__ float c_tab32_reg_b[kTabReg];
 __constant__ float c_tab32_v5_b[kTabV5];
 __constant__ float c_tab32_v7_b[kTabV7];
 
+// position weights of the three corners of a face (the limit points of its
+// vertices i, ring[e], ring[e+1]): [3][ncp] per kind
+__constant__ double c_corner_reg[3 * 12];
+__constant__ double c_corner_v5[3 * 11];
+__constant__ double c_corner_v7[3 * 13];
+
+template <int KIND>
+__device__ __forceinline__ const double *kind_corner()
+{
+    return KIND == 0 ? c_corner_reg : (KIND == 1 ? c_corner_v5 : c_corner_v7);
+}
+
 template <int KIND, typename R>
 __device__ __forceinline__ const R *kind_table()
 {
@@ -296,6 +308,42 @@ __device__ __forceinline__ void face_group(const FaceArgs &a, const double *__re
     R G[NCP][3];
 #pragma unroll
     for (int c = 0; c < NCP; c++) G[c][0] = G[c][1] = G[c][2] = R(0);
+    if (a.rs && a.do_volume && w == 0) {
+        // Volume restraint, seeded into wave 0's accumulators before the
+        // points (few registers are live here): dE/dV dV/dP of this face's
+        // flat triangle of corner limit points L0, L1, L2 (V_face = L0 . (L1 x
+        // L2) / 6), on absolute positions in FP64 (the faces' terms cancel only
+        // about a common origin).  The write-out multiplies G by alpha, so the
+        // seed is divided by it; the volume does not depend on alpha, so the
+        // seed's share of the alpha gradient is taken off acc here.
+        const double *W = kind_corner<KIND>();
+        double L[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+#pragma unroll
+        for (int c = 0; c < NCP; c++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double x = base[k] + (double)PV(c, k);
+#pragma unroll
+                for (int q = 0; q < 3; q++) L[q][k] += W[q * NCP + c] * x;
+            }
+        const double lamV = a.rs[1] * (1.0 / 6.0);
+        double D[3][3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const double *A = L[(q + 1) % 3], *B = L[(q + 2) % 3];
+            D[q][0] = lamV * (A[1] * B[2] - A[2] * B[1]);
+            D[q][1] = lamV * (A[2] * B[0] - A[0] * B[2]);
+            D[q][2] = lamV * (A[0] * B[1] - A[1] * B[0]);
+        }
+#pragma unroll
+        for (int c = 0; c < NCP; c++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double v = W[c] * D[0][k] + W[NCP + c] * D[1][k] + W[2 * NCP + c] * D[2][k];
+                G[c][k] = (R)(v / al64[k]);
+                if (live) acc[1 + k] -= (double)G[c][k] * (base[k] + (double)PV(c, k));
+            }
+    }
     const R Lo = (R)a.Lo[fc], Li = (R)a.Li[fc], c0f = (R)a.c0[fc];
     const R lamA = a.rs ? (R)a.rs[0] : R(0);
     double e = 0.0;
@@ -594,24 +642,6 @@ __device__ __forceinline__ void gather_vertices(const GatherArgs &a, int vblock)
         g1 += (double)src[1];
         g2 += (double)src[2];
     }
-    if (a.Dh) {
-        // volume restraint: dE/dV * dV/dr_v = lamV (n_v Dh_v + sum_ring Dh_u),
-        // Dh = (dV/dL)/(2n) (the limit mask has weight 1/2 on v, 1/(2n_u) on
-        // the ring of u)
-        const double lamV = a.rs[1];
-        const int b0 = a.nb_start[v], b1 = a.nb_start[v + 1];
-        const double nvv = (double)(b1 - b0);
-        double d0 = nvv * a.Dh[3 * (size_t)v], d1 = nvv * a.Dh[3 * (size_t)v + 1], d2 = nvv * a.Dh[3 * (size_t)v + 2];
-        for (int q = b0; q < b1; q++) {
-            const int u = a.nb[q];
-            d0 += a.Dh[3 * (size_t)u];
-            d1 += a.Dh[3 * (size_t)u + 1];
-            d2 += a.Dh[3 * (size_t)u + 2];
-        }
-        g0 += lamV * d0;
-        g1 += lamV * d1;
-        g2 += lamV * d2;
-    }
     if (a.out_mode == 0) {
         a.g[3 * v] = g0;
         a.g[3 * v + 1] = g1;
@@ -655,204 +685,143 @@ __global__ __launch_bounds__(kGatherBlock) void vertex_gather_batch(BatchGatherA
 // ---------------------------------------------------------------------------
 // Global restraints (area: the GLOBAL_AREA form of src/interpLib.C:4322-4326;
 // volume: src/minimize.C:90-114 over the limit-point polyhedron of
-// src/interpLib.C:997-1098 / src/volume.C:350-430).
+// src/interpLib.C:997-1098 / src/volume.C:350-430), as a pre-pass over the
+// face kernel's items: A_tot = sum dudv g w over every Gauss point (alpha-
+// scaled, as the energy's g) and V = sum over faces of L0 . (L1 x L2) / 6 of
+// the face's corner limit points, per 64-face group; a one-block-per-surface
+// finalize sums the groups' partials in fixed order and writes the
+// multipliers the face kernel reads.
 // ---------------------------------------------------------------------------
-
-// sum over the Gauss points of face f of dudv g w (alpha-scaled, as the
-// energy's g)
-template <int NCP, int NPT, int KIND>
-__device__ __forceinline__ double face_area(const FaceArgs &a, int f)
+// One 64-face group: the waves stage the group's control points in LDS and
+// take the Gauss points p = w, w + WS, ...; the last wave also forms the
+// faces' corner limit points and volume terms.  (Single-surface and batched
+// evaluations run this same kernel, so contraction choices cannot differ
+// between them.)
+template <int NCP, int NPT, int KIND, int WS>
+__device__ __forceinline__ void restraint_group(const FaceArgs &a, const double *__restrict__ r, int group,
+                                                double *lds, double acc[4])
 {
-    // every operation as written (no FMA contraction): the single-surface and
-    // batched kernels that inline this must round identically
-    #pragma clang fp contract(off)
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int nf = a.nf;
-    const double al[3] = {a.r[3 * a.nv], a.r[3 * a.nv + 1], a.r[3 * a.nv + 2]};
-    double P[NCP][3];
-#pragma unroll
-    for (int c = 0; c < NCP; c++) {
-        const int v = a.cp[c * nf + f];
+    const int f = group * kWave + lane;
+    const bool live = f < nf;
+    const int fc = live ? f : nf - 1;
+    const double al[3] = {r[3 * a.nv], r[3 * a.nv + 1], r[3 * a.nv + 2]};
+    const bool shifted = a.has_img[fc] != 0;
+    for (int c = w; c < NCP; c += WS) {
+        const int v = a.cp[c * nf + fc];
         double o[3] = {0.0, 0.0, 0.0};
-        if (a.has_img[f]) image_offset(a.img[c * nf + f], a.pbc, o);
+        if (shifted) image_offset(a.img[c * nf + fc], a.pbc, o);
 #pragma unroll
-        for (int k = 0; k < 3; k++) P[c][k] = a.r[3 * v + k] + o[k];
+        for (int k = 0; k < 3; k++) lds[(c * 3 + k) * kWave + lane] = r[3 * v + k] + o[k];
     }
-    double A = 0.0;
-    for (int p = 0; p < NPT; p++) {
-        const double *T = kind_table<KIND, double>() + p * 5 * NCP;
-        const double wp = kind_table<KIND, double>()[NPT * 5 * NCP + p];
-        double Ru[3], Rv[3];
+    __syncthreads();
+#define PX(c, k) lds[((c) * 3 + (k)) * kWave + lane]
+    double A = 0.0, V = 0.0;
+    if (a.do_area) {
+        const double *tab = kind_table<KIND, double>();
+        for (int p = w; p < NPT; p += WS) {
+            const double *T = tab + p * 5 * NCP;
+            double Ru[3] = {0.0, 0.0, 0.0}, Rv[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            double su = 0.0, sv = 0.0;
+            for (int c = 0; c < NCP; c++)
 #pragma unroll
-            for (int c = 0; c < NCP; c++) {
-                su += T[c * 5 + 0] * P[c][k];
-                sv += T[c * 5 + 1] * P[c][k];
+                for (int k = 0; k < 3; k++) {
+                    const double x = PX(c, k);
+                    Ru[k] += T[c * 5 + 0] * x;
+                    Rv[k] += T[c * 5 + 1] * x;
+                }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                Ru[k] *= al[k];
+                Rv[k] *= al[k];
             }
-            Ru[k] = su * al[k];
-            Rv[k] = sv * al[k];
-        }
-        const double E11 = Ru[0] * Ru[0] + Ru[1] * Ru[1] + Ru[2] * Ru[2];
-        const double E12 = Ru[0] * Rv[0] + Ru[1] * Rv[1] + Ru[2] * Rv[2];
-        const double E22 = Rv[0] * Rv[0] + Rv[1] * Rv[1] + Rv[2] * Rv[2];
-        A += a.dudv * sqrt(E11 * E22 - E12 * E12) * wp;
-    }
-    return A;
-}
-
-__device__ __forceinline__ double area_block(const FaceArgs &a, int kind, int blk)
-{
-    const int f = blk * kRestraintBlock + threadIdx.x;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    if (f < a.nf) {
-        switch (kind) {
-        case 0: acc[0] = face_area<12, 15, 0>(a, f); break;
-        case 1: acc[0] = face_area<11, 16, 1>(a, f); break;
-        default: acc[0] = face_area<13, 16, 2>(a, f); break;
+            const double E11 = Ru[0] * Ru[0] + Ru[1] * Ru[1] + Ru[2] * Ru[2];
+            const double E12 = Ru[0] * Rv[0] + Ru[1] * Rv[1] + Ru[2] * Rv[2];
+            const double E22 = Rv[0] * Rv[0] + Rv[1] * Rv[1] + Rv[2] * Rv[2];
+            A += a.dudv * sqrt(E11 * E22 - E12 * E12) * tab[NPT * 5 * NCP + p];
         }
     }
-    block_sum4<kRestraintBlock>(acc);
-    return acc[0];
-}
-
-// Loop limit position of vertex u: r_u / 2 + sum(ring) / (2 n)
-__device__ __forceinline__ void limit_point(const RestraintArgs &ra, int u, double L[3])
-{
-    // every operation as written (no FMA contraction): the single-surface and
-    // batched kernels that inline this must round identically
-    #pragma clang fp contract(off)
-    const int b0 = ra.nb_start[u], b1 = ra.nb_start[u + 1];
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int q = b0; q < b1; q++) {
-        const int w = ra.nb[q];
-        s0 += ra.r[3 * (size_t)w];
-        s1 += ra.r[3 * (size_t)w + 1];
-        s2 += ra.r[3 * (size_t)w + 2];
-    }
-    const double chi = 0.5 / (double)(b1 - b0);
-    L[0] = 0.5 * ra.r[3 * (size_t)u] + chi * s0;
-    L[1] = 0.5 * ra.r[3 * (size_t)u + 1] + chi * s1;
-    L[2] = 0.5 * ra.r[3 * (size_t)u + 2] + chi * s2;
-}
-
-// Per vertex v: D_v = dV/dL_v = (1/6) sum_e L_ring[e] x L_ring[e+1] (every
-// face around v, oriented), Dh_v = D_v / (2 n_v); block sum of L_v . D_v / 3
-// (each face's triple product is counted once per corner).
-__device__ __forceinline__ double volume_block(const RestraintArgs &ra, int blk)
-{
-    // every operation as written (no FMA contraction): the single-surface and
-    // batched kernels that inline this must round identically
-    #pragma clang fp contract(off)
-    const int v = blk * kRestraintBlock + threadIdx.x;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    if (v < ra.nv) {
-        const int b0 = ra.nb_start[v], b1 = ra.nb_start[v + 1];
-        double Lv[3], L0[3], Lp[3], D[3] = {0.0, 0.0, 0.0};
-        limit_point(ra, v, Lv);
-        limit_point(ra, ra.nb[b0], L0);
-        Lp[0] = L0[0];
-        Lp[1] = L0[1];
-        Lp[2] = L0[2];
-        for (int q = b0 + 1; q <= b1; q++) {
-            double Lq[3];
-            if (q < b1) limit_point(ra, ra.nb[q], Lq);
-            else {
-                Lq[0] = L0[0];
-                Lq[1] = L0[1];
-                Lq[2] = L0[2];
-            }
-            D[0] += Lp[1] * Lq[2] - Lp[2] * Lq[1];
-            D[1] += Lp[2] * Lq[0] - Lp[0] * Lq[2];
-            D[2] += Lp[0] * Lq[1] - Lp[1] * Lq[0];
-            Lp[0] = Lq[0];
-            Lp[1] = Lq[1];
-            Lp[2] = Lq[2];
-        }
+    if (a.do_volume && w == WS - 1) {
+        const double *W = kind_corner<KIND>();
+        double L[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
 #pragma unroll
-        for (int k = 0; k < 3; k++) D[k] *= (1.0 / 6.0);
-        const double h = 0.5 / (double)(b1 - b0);
-        ra.Dh[3 * (size_t)v] = D[0] * h;
-        ra.Dh[3 * (size_t)v + 1] = D[1] * h;
-        ra.Dh[3 * (size_t)v + 2] = D[2] * h;
-        acc[0] = (Lv[0] * D[0] + Lv[1] * D[1] + Lv[2] * D[2]) * (1.0 / 3.0);
+        for (int c = 0; c < NCP; c++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double x = PX(c, k);
+#pragma unroll
+                for (int q = 0; q < 3; q++) L[q][k] += W[q * NCP + c] * x;
+            }
+        const double cx = L[1][1] * L[2][2] - L[1][2] * L[2][1];
+        const double cy = L[1][2] * L[2][0] - L[1][0] * L[2][2];
+        const double cz = L[1][0] * L[2][1] - L[1][1] * L[2][0];
+        V = (L[0][0] * cx + L[0][1] * cy + L[0][2] * cz) * (1.0 / 6.0);
     }
-    block_sum4<kRestraintBlock>(acc);
-    return acc[0];
-}
-
-__global__ __launch_bounds__(kRestraintBlock) void restraint_kernel(RestraintLaunch L)
-{
-    const int b = blockIdx.x;
-    const int na = L.area_start[L.ncls];
-    if (b < na) {
-        int c = 0;
-        while (c + 1 < L.ncls && b >= L.area_start[c + 1]) c++;
-        const double A = area_block(L.cls[c], L.kind[c], b - L.area_start[c]);
-        if (threadIdx.x == 0) L.ra.Apart[b] = A;
-    } else {
-        const double V = volume_block(L.ra, b - na);
-        if (threadIdx.x == 0) L.ra.Vpart[b - na] = V;
+#undef PX
+    if (live) {
+        acc[0] += A;
+        acc[1] += V;
     }
 }
 
-// batched restraint items: type 0..2 = area block of a class of that kind
-// (index = FaceArgs entry), type 3 = volume block (index = surface)
-__global__ __launch_bounds__(kRestraintBlock) void restraint_batch_kernel(BatchRestraintArgs B)
-{
-    for (int it = blockIdx.x; it < B.nitems; it += gridDim.x) {
-        const RestraintItem item = B.items[it];
-        if (item.type < 3) {
-            const FaceArgs &a = B.fa[item.index];
-            const RestraintArgs &ra = B.ra[B.fa_surface[item.index]];
-            const double A = area_block(a, item.type, item.block);
-            if (threadIdx.x == 0) ra.Apart[item.slot] = A;
-        } else {
-            const RestraintArgs &ra = B.ra[item.index];
-            const double V = volume_block(ra, item.block);
-            if (threadIdx.x == 0) ra.Vpart[item.block] = V;
-        }
-        __syncthreads();   // the reduction scratch is reused by the next item
-    }
-}
+constexpr int kPrepassWaves = 4;   // waves per 64-face group (about four Gauss points each)
 
-// one block per surface: fixed-order sums of the area and volume partials,
-// then the restraint energy and its multipliers
-__device__ __forceinline__ void restraint_finish(const RestraintArgs &ra)
+template <int WS>
+__global__ __launch_bounds__(kWave *WS) void restraint_prepass_kernel(BatchFaceArgs B)
 {
-    // every operation as written (no FMA contraction): the single-surface and
-    // batched kernels that inline this must round identically
-    #pragma clang fp contract(off)
+    __shared__ double lds[13 * 3 * kWave];
+    const int it = xcd_logical_block(blockIdx.x, gridDim.x);
+    if (it >= B.nitems) return;
+    const FaceItem item = B.items[it];
+    const FaceArgs &a = B.fa[item.fa];
+    if (!a.rs) return;   // a surface without restraints (uniform over the block)
+    const double *r = B.r ? B.r : a.r;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    if (ra.do_area)
-        for (int i = threadIdx.x; i < ra.nApart; i += kRestraintBlock) acc[0] += ra.Apart[i];
-    if (ra.do_volume)
-        for (int i = threadIdx.x; i < ra.nVpart; i += kRestraintBlock) acc[1] += ra.Vpart[i];
-    block_sum4<kRestraintBlock>(acc);
+    switch (item.kind) {
+    case 0: restraint_group<12, 15, 0, WS>(a, r, item.group, lds, acc); break;
+    case 1: restraint_group<11, 16, 1, WS>(a, r, item.group, lds, acc); break;
+    default: restraint_group<13, 16, 2, WS>(a, r, item.group, lds, acc); break;
+    }
+    block_sum4<kWave * WS>(acc);
+    if (threadIdx.x == 0) {
+        a.rpart[2 * (size_t)item.slot] = acc[0];
+        a.rpart[2 * (size_t)item.slot + 1] = acc[1];
+    }
+}
+
+// one block per surface (the surface's FaceArgs entry of class 0): the
+// groups' partials summed in fixed order, then the restraint energy and its
+// multipliers
+__global__ __launch_bounds__(kGatherBlock) void restraint_finalize_kernel(const FaceArgs *fa, const int *first)
+{
+    #pragma clang fp contract(off)
+    const FaceArgs &a = fa[first[blockIdx.x]];
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < a.ngroups; i += kGatherBlock) {
+        s[0] += a.rpart[2 * (size_t)i];
+        s[1] += a.rpart[2 * (size_t)i + 1];
+    }
+    block_sum4<kGatherBlock>(s);
     if (threadIdx.x == 0) {
         double lamA = 0.0, lamV = 0.0, E = 0.0;
-        if (ra.do_area) {
-            const double s = (acc[0] - ra.A0) / ra.A0;
-            E += ra.KA_g * ra.A0 * s * s;
-            lamA = 2.0 * ra.KA_g * s;
+        if (a.do_area && a.KA_g != 0.0) {
+            const double sa = (s[0] - a.A0) / a.A0;
+            E += a.KA_g * a.A0 * sa * sa;
+            lamA = 2.0 * a.KA_g * sa;
         }
-        if (ra.do_volume) {
-            const double s = (acc[1] - ra.V0) / ra.V0;
-            E += 0.5 * ra.KV * ra.V0 * s * s;
-            lamV = ra.KV * s;
+        if (a.do_volume && a.KV != 0.0) {
+            const double sv = (s[1] - a.V0) / a.V0;
+            E += 0.5 * a.KV * a.V0 * sv * sv;
+            lamV = a.KV * sv;
         }
-        ra.rs[0] = lamA;
-        ra.rs[1] = lamV;
-        ra.rs[2] = E;
-        ra.rs[3] = acc[0];
-        ra.rs[4] = acc[1];
+        a.rs[0] = lamA;
+        a.rs[1] = lamV;
+        a.rs[2] = E;
+        a.rs[3] = s[0];
+        a.rs[4] = s[1];
     }
-}
-
-__global__ __launch_bounds__(kRestraintBlock) void restraint_finalize(const RestraintArgs *table, RestraintArgs one)
-{
-    if (table) restraint_finish(table[blockIdx.x]);
-    else restraint_finish(one);
 }
 
 // ---------------------------------------------------------------------------
@@ -867,7 +836,8 @@ int kind_of(int ncp, int npt)
     return -1;
 }
 
-hipError_t upload_tables(int kind, const double *table, const double *weights, int npt, int ncp)
+hipError_t upload_tables(int kind, const double *table, const double *weights, const double *corner, int npt,
+                         int ncp)
 {
     // host layout [npt][6][ncp] (r_w, r_u, r_v, r_uu, r_uv, r_vv) -> device
     // [npt][ncp][5] without the position row, then the weights
@@ -893,6 +863,9 @@ hipError_t upload_tables(int kind, const double *table, const double *weights, i
                                   : (kind == 1 ? (const void *)HIP_SYMBOL(c_tab32_v5_b) : (const void *)HIP_SYMBOL(c_tab32_v7_b));
     if (e == hipSuccess) e = hipMemcpyToSymbol(fwd32, b32, (n + npt) * sizeof(float), 0, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpyToSymbol(bwd32, b32, n * sizeof(float), 0, hipMemcpyHostToDevice);
+    const void *cor = kind == 0 ? (const void *)HIP_SYMBOL(c_corner_reg)
+                                : (kind == 1 ? (const void *)HIP_SYMBOL(c_corner_v5) : (const void *)HIP_SYMBOL(c_corner_v7));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(cor, corner, 3 * (size_t)ncp * sizeof(double), 0, hipMemcpyHostToDevice);
     delete[] b32;
     delete[] buf;
     return e;
@@ -1003,25 +976,13 @@ hipError_t launch_gather_batch(const BatchGatherArgs &a, int grid, bool mixed, h
     return hipGetLastError();
 }
 
-hipError_t launch_restraints(const RestraintLaunch &a, hipStream_t st)
+hipError_t launch_restraint_prepass(const BatchFaceArgs &a, const int *first, int nsurf, hipStream_t st)
 {
-    const int nb = a.area_start[a.ncls] + a.ra.nVpart;
-    if (nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL(restraint_kernel, dim3(nb), dim3(kRestraintBlock), 0, st, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_restraints_batch(const BatchRestraintArgs &a, hipStream_t st)
-{
-    if (a.nitems <= 0) return hipSuccess;
-    hipLaunchKernelGGL(restraint_batch_kernel, dim3(a.nitems), dim3(kRestraintBlock), 0, st, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_restraint_finalize(const RestraintArgs *table, const RestraintArgs &one, int nsurf, hipStream_t st)
-{
-    if (nsurf <= 0) return hipSuccess;
-    hipLaunchKernelGGL(restraint_finalize, dim3(nsurf), dim3(kRestraintBlock), 0, st, table, one);
+    if (a.nitems <= 0 || nsurf <= 0) return hipSuccess;
+    hipLaunchKernelGGL(restraint_prepass_kernel<kPrepassWaves>, dim3(a.nitems), dim3(kWave * kPrepassWaves), 0, st, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(restraint_finalize_kernel, dim3(nsurf), dim3(kGatherBlock), 0, st, a.fa, first);
     return hipGetLastError();
 }
 

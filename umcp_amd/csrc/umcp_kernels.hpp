@@ -38,9 +38,16 @@ struct FaceArgs {
     double *gbuf;           // [nnz][3] per-(face, slot) gradient, CSR order (FP64 path)
     float *gbuf32;          // [nnz][3] the same for the mixed path
     PhysParams ph;
-    const double *rs;       // restraint scalars (rs[0] = dE/dA_tot) or null
     int *step_ctr;          // Brownian step counter, advanced once per evaluation, or null
-    double *partial;        // the surface's block partials (batched launches)
+    double *partial;        // the surface's block partials [ngroups][4]
+    // global restraints (rs == null: none).  rs[0..4] = dE/dA_tot, dE/dV,
+    // restraint energy, A_tot, V, written by the restraint finalize; rpart
+    // [ngroups][2] the pre-pass's per-group area and volume
+    double *rs;
+    double *rpart;
+    int ngroups;
+    double KA_g, KV, A0, V0;
+    int do_area, do_volume;
 };
 
 // One work item of a batched face launch: 64 faces of one class of one
@@ -79,11 +86,8 @@ struct GatherArgs {
     double *energies;       // may be null: energies[step] = E
     int step;
     double *sums;           // may be null: the four reduced partial sums
-    // global restraints: rs[0..4] = dE/dA, dE/dV, E_restraint, A, V (null: none);
-    // Dh[3nv] = (dV/dL_v) / (2 n_v); the oriented rings in CSR form
+    // restraint scalars of the surface (rs[2] = restraint energy) or null
     const double *rs;
-    const double *Dh;
-    const int *nb_start, *nb;
     // noise source of out_mode 1
     int noise;              // kNoiseArray (xi) or kNoisePhilox
     unsigned long long seed;
@@ -101,49 +105,11 @@ struct BatchGatherArgs {
     int nitems;
 };
 
-// Global area / volume restraints of one surface.
-struct RestraintArgs {
-    const double *r;
-    int nv;
-    const int *nb_start, *nb;
-    double *Dh;             // [3nv] out
-    double *Apart;          // [nApart] sum(dudv g w) per area block
-    double *Vpart;          // [nVpart] volume per vertex block
-    int nApart, nVpart;
-    double *rs;             // [5] out: dE/dA, dE/dV, E, A, V
-    double KA_g, KV, A0, V0;
-    int do_area, do_volume;
-};
-
-// restraint work: blocks [0, area_start[ncls]) evaluate the area of 256
-// faces each (class c from area_start), then nVpart volume blocks
-struct RestraintLaunch {
-    FaceArgs cls[kMaxClasses];
-    int kind[kMaxClasses];
-    int area_start[kMaxClasses + 1];
-    int ncls;
-    RestraintArgs ra;
-};
-
-// batched restraint work item: type 0..2 = area block of a class of that
-// kind (index = FaceArgs entry, slot = area partial of its surface), type 3 =
-// volume block (index = RestraintArgs entry)
-struct RestraintItem {
-    int type, index, block, slot;
-};
-
-struct BatchRestraintArgs {
-    const FaceArgs *fa;
-    const RestraintArgs *ra;   // one per surface
-    const int *fa_surface;     // FaceArgs entry -> RestraintArgs entry
-    const RestraintItem *items;
-    int nitems;
-};
-
 int kind_of(int ncp, int npt);
-// copies one kind's shared Gauss-point table ([npt][6][ncp]) and weights into
-// the device's constant memory (current device)
-hipError_t upload_tables(int kind, const double *table, const double *weights, int npt, int ncp);
+// copies one kind's shared Gauss-point table ([npt][6][ncp]), weights and
+// corner weights ([3][ncp]) into the device's constant memory (current device)
+hipError_t upload_tables(int kind, const double *table, const double *weights, const double *corner, int npt,
+                         int ncp);
 int face_groups(int nf);
 // launch shape: point_split (1..4) waves share the Gauss points of 64 faces
 // (whose control points are staged in LDS); flag kLaunchOcc2 bounds registers
@@ -157,11 +123,10 @@ int metric_blocks(int nf);
 hipError_t launch_vertex_gather(const GatherArgs &a, hipStream_t st);
 int gather_blocks(int nv);
 
-// restraints: area + volume partials, then the finalize (one block per surface)
-constexpr int kRestraintBlock = 256;
-hipError_t launch_restraints(const RestraintLaunch &a, hipStream_t st);
-hipError_t launch_restraint_finalize(const RestraintArgs *ra_dev_or_null, const RestraintArgs &ra, int nsurf,
-                                     hipStream_t st);
+// restraint pre-pass over the same items as the face kernel (per-group area
+// and volume), then one finalize block per surface: first[i] is the FaceArgs
+// entry (in a.fa) of the i-th surface with restraints
+hipError_t launch_restraint_prepass(const BatchFaceArgs &a, const int *first, int nsurf, hipStream_t st);
 
 // face kernel over a work-item table (one surface or many); grid = 0: one
 // workgroup per item, else a persistent grid of that many workgroups striding
@@ -169,7 +134,6 @@ hipError_t launch_restraint_finalize(const RestraintArgs *ra_dev_or_null, const 
 hipError_t launch_faces_batch(const BatchFaceArgs &a, int grid, int point_split, int flags, bool mixed, int feat,
                               hipStream_t st);
 hipError_t launch_gather_batch(const BatchGatherArgs &a, int grid, bool mixed, hipStream_t st);
-hipError_t launch_restraints_batch(const BatchRestraintArgs &a, hipStream_t st);
 
 // 3 n normal deviates of (seed, step) for vertices [0, n) (the stream the
 // Brownian update draws from)

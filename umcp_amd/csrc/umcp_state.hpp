@@ -61,11 +61,9 @@ struct umcp_surface {
     bool has_images = false;              // periodic mesh: no enclosed volume
     double A0_setg0 = 0.0, V_setg0 = 0.0;
     double *d_rs = nullptr;               // [8] dE/dA, dE/dV, E, A, V
-    double *d_Dh = nullptr;               // [3nv]
-    double *d_Apart = nullptr, *d_Vpart = nullptr;
-    int *d_nb_start = nullptr, *d_nb = nullptr;
-    int nApart = 0, nVpart = 0;
-    std::vector<int> area_start;          // area blocks per class (class order)
+    double *d_rpart = nullptr;            // [npartial][2] per-group area and volume
+    int *d_zero = nullptr;                // [1] = 0: the finalize's table entry (class 0)
+    umcp::FaceArgs *d_fa_aux = nullptr;   // a second face table (volume-only passes)
     // Brownian step counter of the device-noise path
     int *d_step = nullptr;
     // the face kernel's work tables: per-class arguments and one item per
@@ -84,6 +82,5 @@ bool restraints_on(const umcp_surface *s);
 // loop's admission footprint); equals device_bytes after the plan
 size_t plan_bytes(const umcp_surface *s);
 FaceArgs face_args(umcp_surface *s, size_t c, const double *d_r);
-RestraintArgs restraint_args(umcp_surface *s, const double *d_r, bool area, bool volume);
 
 }  // namespace umcp
