@@ -625,6 +625,10 @@ __device__ __forceinline__ void gather_reduce(const GatherArgs &a)
     }
 }
 
+// kGatherLanes lanes per vertex: lane q sums entries s0 + q, s0 + q + 8, ...
+// of the vertex's CSR range (neighbouring lanes read neighbouring entries),
+// then a fixed xor butterfly combines the lanes (IEEE addition commutes, so
+// every lane of the group ends with the same sum).
 template <typename GB>
 __device__ __forceinline__ void gather_vertices(const GatherArgs &a, int vblock)
 {
@@ -632,16 +636,27 @@ __device__ __forceinline__ void gather_vertices(const GatherArgs &a, int vblock)
     // batched kernels that inline this must round identically
     #pragma clang fp contract(off)
     if (a.out_mode == 2) return;
-    const int v = vblock * kGatherBlock + threadIdx.x;
-    if (v >= a.nv) return;
-    const int s0 = a.vstart[v], s1 = a.vstart[v + 1];
+    const int q = threadIdx.x & (kGatherLanes - 1);
+    const int v = vblock * kGatherVerts + (int)threadIdx.x / kGatherLanes;
+    const bool mine = v < a.nv;
     double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-    const GB *src = (const GB *)(sizeof(GB) == 4 ? (const void *)a.gbuf32 : (const void *)a.gbuf) + 3 * (size_t)s0;
-    for (int s = s0; s < s1; s++, src += 3) {
-        g0 += (double)src[0];
-        g1 += (double)src[1];
-        g2 += (double)src[2];
+    if (mine) {
+        const int s0 = a.vstart[v], s1 = a.vstart[v + 1];
+        const GB *src = (const GB *)(sizeof(GB) == 4 ? (const void *)a.gbuf32 : (const void *)a.gbuf);
+        for (int e = s0 + q; e < s1; e += kGatherLanes) {
+            g0 += (double)src[3 * (size_t)e];
+            g1 += (double)src[3 * (size_t)e + 1];
+            g2 += (double)src[3 * (size_t)e + 2];
+        }
     }
+#pragma unroll
+    for (int off = 1; off < kGatherLanes; off <<= 1) {
+        g0 += __shfl_xor(g0, off, kWave);
+        g1 += __shfl_xor(g1, off, kWave);
+        g2 += __shfl_xor(g2, off, kWave);
+    }
+    if (q != 0) return;
+    if (!mine) return;
     if (a.out_mode == 0) {
         a.g[3 * v] = g0;
         a.g[3 * v + 1] = g1;
@@ -942,7 +957,7 @@ hipError_t launch_face_metric(const FaceArgs &a, int kind, int mode, double *g0_
     return hipGetLastError();
 }
 
-int gather_blocks(int nv) { return nv > 0 ? (nv + kGatherBlock - 1) / kGatherBlock : 1; }
+int gather_blocks(int nv) { return nv > 0 ? (nv + kGatherVerts - 1) / kGatherVerts : 1; }
 
 hipError_t launch_vertex_gather(const GatherArgs &a, hipStream_t st)
 {
@@ -989,7 +1004,7 @@ hipError_t launch_restraint_prepass(const BatchFaceArgs &a, const int *first, in
 hipError_t launch_noise(unsigned long long seed, int step, int n, double *out, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(noise_kernel, dim3(gather_blocks(n)), dim3(kGatherBlock), 0, st, seed, step, n, out);
+    hipLaunchKernelGGL(noise_kernel, dim3((n + kGatherBlock - 1) / kGatherBlock), dim3(kGatherBlock), 0, st, seed, step, n, out);
     return hipGetLastError();
 }
 

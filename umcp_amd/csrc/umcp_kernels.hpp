@@ -10,6 +10,8 @@ namespace umcp {
 
 constexpr int kWave = 64;
 constexpr int kGatherBlock = 256;
+constexpr int kGatherLanes = 8;                          // lanes summing one vertex's CSR range
+constexpr int kGatherVerts = kGatherBlock / kGatherLanes;  // vertices per gather block
 constexpr int kMetricBlock = 256;
 constexpr int kMaxClasses = 4;
 constexpr int kMaxSplit = 4;
