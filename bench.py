@@ -320,6 +320,7 @@ def run_serving_leg(args, dev, torch):
             first.setdefault(q.mesh_path, q)
         for q in first.values():
             sv.wait(sv.submit(q.mesh_path, q.nv, q.params, q.c0, 0, q.seed, 0.0, 0.0, q.r0, q.r))
+        st0 = sv.stats()
         ids = []
         t0 = time.perf_counter()
         for q in reqs:
@@ -344,7 +345,9 @@ def run_serving_leg(args, dev, torch):
             "makespan_s": float(makespan),
             "requests_per_s": len(reqs) / makespan,
             "face_evals_per_s": face_steps / makespan,
-            "device_busy_fraction": st["busy_s"] / max(st["clock_s"], 1e-9),
+            "device_busy_fraction": (st["busy_s"] - st0["busy_s"]) / makespan,
+            "worker_s": {k: st[k + "_s"] - st0[k + "_s"] for k in ("busy", "build", "retire", "wait")},
+            "setup_s_total": st["setup_s"] - st0["setup_s"],
             "chunks": int(st["chunks"]), "peak_active": int(st["peak_active"]),
             "peak_reserved_GB": st["peak_bytes"] / 1e9,
         }, res

@@ -210,8 +210,8 @@ int umcp_ensemble_work(const umcp_ensemble *e, int *face_items, int *gather_item
 
 /* Serving loop (no reference counterpart: the reference runs one simulation
  * per process).  Requests (mesh file, parameters, spontaneous curvature, N
- * Brownian steps with device noise) are queued as they arrive; a preparation
- * thread admits the queue head once a slot is free and its device footprint
+ * Brownian steps with device noise) are queued as they arrive; preparation
+ * threads admit the queue head once a slot is free and its device footprint
  * fits the budget (template parsed once per mesh file, then clone, plan, setg0,
  * start positions); the worker thread steps every admitted request together
  * (batched launches of an ensemble) in chunks of at most chunk_steps, ending
@@ -228,6 +228,7 @@ typedef struct {
     int face_grid;
     int order;
     int graph_steps;       /* > 0: replay captured graphs of up to that many steps */
+    int prep_threads;      /* requests prepared concurrently (<= 0: 1) */
 } umcp_server_config;
 typedef struct {
     const char *mesh_path; /* .mesh file of the request's surface */
@@ -250,10 +251,13 @@ int umcp_server_submit(umcp_server *sv, const umcp_request *rq, int *id);
 int umcp_server_wait(umcp_server *sv, int id, double *r_out, double *energies, double *times);
 /* blocks until every submitted request is done */
 int umcp_server_drain(umcp_server *sv);
-/* stats[8]: requests done, chunks, request-steps, face-steps, seconds the
- * device was stepping, peak reserved bytes, peak active requests, seconds
- * since creation */
-int umcp_server_stats(umcp_server *sv, double *stats);
+/* writes the first min(n, 12) of: requests done, chunks, request-steps,
+ * face-steps, seconds the worker spent in chunks (launch to completion),
+ * peak reserved bytes, peak active requests, seconds since creation, seconds
+ * rebuilding the batch's work tables, seconds retiring finished requests,
+ * seconds the worker waited for work, seconds of request set-up summed over
+ * the preparation threads; returns the count written */
+int umcp_server_stats(umcp_server *sv, double *stats, int n);
 const char *umcp_server_last_error(const umcp_server *sv);
 /* finishes admitted requests, drops queued ones, joins the threads */
 void umcp_server_destroy(umcp_server *sv);

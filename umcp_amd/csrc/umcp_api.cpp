@@ -11,7 +11,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <set>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/umcp.h"
@@ -40,16 +43,41 @@ int hip_fail(umcp_surface *s, hipError_t e, const char *where)
     return fail(s, UMCP_ERR_DEVICE, std::string(where) + ": " + hipGetErrorString(e));
 }
 
+// device arrays of the plan are carved from the surface's arena (one
+// hipMalloc of plan_bytes, 256-byte aligned slices); anything beyond it gets
+// an allocation of its own
 template <typename T>
 int dev_alloc(umcp_surface *s, T **p, size_t n)
 {
     if (n == 0) n = 1;
+    const size_t bytes = n * sizeof(T);
+    const size_t off = (s->arena_used + kArenaAlign - 1) & ~(kArenaAlign - 1);
+    if (s->arena && off + bytes <= s->arena_size) {
+        s->arena_used = off + bytes;
+        *p = (T *)(s->arena + off);
+        return UMCP_OK;
+    }
     void *q = nullptr;
-    hipError_t e = hipMalloc(&q, n * sizeof(T));
+    hipError_t e = hipMalloc(&q, bytes);
     if (e != hipSuccess) return fail(s, UMCP_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
     s->allocations.push_back(q);
-    s->device_bytes += n * sizeof(T);
+    s->device_bytes += bytes;
     *p = (T *)q;
+    return UMCP_OK;
+}
+
+// the Gauss-point tables of a face kind are the same for every surface: one
+// upload per device and kind
+int upload_tables_once(umcp_surface *s, const FaceClass &C)
+{
+    static std::mutex mu;
+    static std::set<std::pair<int, int>> done;
+    const int kind = kind_of(C.ncp, C.npt);
+    std::lock_guard<std::mutex> lk(mu);
+    if (done.count({s->device, kind})) return UMCP_OK;
+    hipError_t e = upload_tables(kind, C.table.data(), C.weight.data(), C.corner.data(), C.npt, C.ncp);
+    if (e != hipSuccess) return hip_fail(s, e, "upload Gauss-point tables");
+    done.insert({s->device, kind});
     return UMCP_OK;
 }
 
@@ -127,7 +155,7 @@ FaceArgs face_args(umcp_surface *s, size_t c, const double *d_r)
 // mirrors every dev_alloc of umcp_plan (zero-sized requests allocate one element)
 size_t plan_bytes(const umcp_surface *s)
 {
-    auto sz = [](size_t n, size_t elem) { return (n ? n : 1) * elem; };
+    auto sz = [](size_t n, size_t elem) { return ((n ? n : 1) * elem + kArenaAlign - 1) & ~(kArenaAlign - 1); };
     size_t b = 0;
     int metric_total = 0, ngroups = 0;
     for (const FaceClass &C : s->P.classes) {
@@ -397,6 +425,15 @@ int umcp_plan(umcp_surface *s)
     }
     const int nv = s->T.nv;
     int rc;
+    s->arena_size = plan_bytes(s);
+    s->arena_used = 0;
+    e = hipMalloc((void **)&s->arena, s->arena_size);
+    if (e != hipSuccess) {
+        s->arena = nullptr;
+        return fail(s, UMCP_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    s->allocations.push_back(s->arena);
+    s->device_bytes += s->arena_size;
     s->face_c0.assign(s->P.nfaces, 0.0);
     s->face_Lo.assign(s->P.nfaces, 0.0);
     s->face_Li.assign(s->P.nfaces, 0.0);
@@ -412,8 +449,7 @@ int umcp_plan(umcp_surface *s)
             (rc = dev_alloc(s, &D.Li, nf)) || (rc = dev_alloc(s, &D.c0, nf)) ||
             0)
             return rc;
-        e = upload_tables(kind_of(C.ncp, C.npt), C.table.data(), C.weight.data(), C.corner.data(), C.npt, C.ncp);
-        if (e != hipSuccess) return hip_fail(s, e, "upload Gauss-point tables");
+        if ((rc = upload_tables_once(s, C))) return rc;
         if ((rc = dev_upload(s, D.cp, C.cp.data(), C.cp.size())) ||
             (rc = dev_upload(s, D.pos, C.pos.data(), C.pos.size())) ||
             (rc = dev_upload(s, D.img, C.img.data(), C.img.size())) ||
@@ -495,18 +531,35 @@ int umcp_setg0(umcp_surface *s, const double *r)
                                           s->cls[c].ig0, nullptr, 0, s->stream);
         if (e != hipSuccess) return hip_fail(s, e, "metric kernel launch");
     }
+    // the reference records V0 right after setg0 on the same positions
+    // (src/hd.C:273-279)
+    double h_rs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (!s->has_images) {
+        if ((rc = enqueue_volume(s, s->d_r, s->stream))) return rc;
+        hipError_t e = hipMemcpyAsync(h_rs, s->d_rs, sizeof h_rs, hipMemcpyDeviceToHost, s->stream);
+        if (e != hipSuccess) return hip_fail(s, e, "download volume");
+    }
+    std::vector<std::vector<double>> g0s(s->P.classes.size());
+    for (size_t c = 0; c < s->P.classes.size(); c++) {
+        const FaceClass &C = s->P.classes[c];
+        g0s[c].resize((size_t)C.nf() * C.npt);
+        hipError_t e = hipMemcpyAsync(g0s[c].data(), s->cls[c].g0, g0s[c].size() * sizeof(double),
+                                      hipMemcpyDeviceToHost, s->stream);
+        if (e != hipSuccess) return hip_fail(s, e, "download g0");
+    }
+    if ((rc = sync(s))) return rc;
+    s->V_setg0 = h_rs[4];
     // single-lipid packing per triangle from the reference area of each face
     // (src/interpLib.C:3113-3196): add APL per leaflet until it exceeds A - 65/2
     const double apl = s->prm.lipid_apl;
     double A0 = 0.0;
+    std::vector<std::vector<double>> Ls(s->P.classes.size());
     for (size_t c = 0; c < s->P.classes.size(); c++) {
         const FaceClass &C = s->P.classes[c];
         const int nf = C.nf();
-        std::vector<double> g0((size_t)nf * C.npt), Lo(nf), Li(nf);
-        hipError_t e = hipMemcpyAsync(g0.data(), s->cls[c].g0, g0.size() * sizeof(double),
-                                      hipMemcpyDeviceToHost, s->stream);
-        if (e != hipSuccess) return hip_fail(s, e, "download g0");
-        if ((rc = sync(s))) return rc;
+        const std::vector<double> &g0 = g0s[c];
+        std::vector<double> &L_c = Ls[c];
+        L_c.resize(nf);
         for (int f = 0; f < nf; f++) {
             double A = 0.0;
             for (int p = 0; p < C.npt; p++) A += g0[(size_t)p * nf + f] * C.dudv * C.weight[p];
@@ -515,27 +568,17 @@ int umcp_setg0(umcp_surface *s, const double *r)
                 L += apl;
                 packed += apl;
             }
-            Lo[f] = Li[f] = L;
+            L_c[f] = L;
             s->face_Lo[C.face_id[f]] = L;
             s->face_Li[C.face_id[f]] = L;
         }
-        if ((rc = dev_upload(s, s->cls[c].Lo, Lo.data(), nf)) || (rc = dev_upload(s, s->cls[c].Li, Li.data(), nf)))
+        // both leaflets hold the same packing
+        if ((rc = dev_upload(s, s->cls[c].Lo, L_c.data(), nf)) || (rc = dev_upload(s, s->cls[c].Li, L_c.data(), nf)))
             return rc;
         for (int p = 0; p < C.npt; p++)
             for (int f = 0; f < nf; f++) A0 += C.dudv * g0[(size_t)p * nf + f] * C.weight[p];
     }
     s->A0_setg0 = A0;
-    // the reference records V0 right after setg0 on the same positions
-    // (src/hd.C:273-279)
-    s->V_setg0 = 0.0;
-    if (!s->has_images) {
-        if ((rc = enqueue_volume(s, s->d_r, s->stream))) return rc;
-        double h[8];
-        hipError_t e = hipMemcpyAsync(h, s->d_rs, sizeof h, hipMemcpyDeviceToHost, s->stream);
-        if (e != hipSuccess) return hip_fail(s, e, "download volume");
-        if ((rc = sync(s))) return rc;
-        s->V_setg0 = h[4];
-    }
     if ((rc = sync(s))) return rc;
     s->g0_set = true;
     s->fa_dirty = true;   // restraint targets may default to this state

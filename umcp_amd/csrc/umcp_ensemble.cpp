@@ -59,6 +59,7 @@ struct umcp_ensemble {
     hipGraphExec_t exec = nullptr;
     int graph_steps = 0;
     hipStream_t graph_stream = nullptr;
+    hipStream_t last_stream = nullptr;   // stream of the latest steps (may be the caller's)
 };
 
 namespace {
@@ -72,6 +73,15 @@ int efail(umcp_ensemble *e, int code, const std::string &msg)
 int ehip(umcp_ensemble *e, hipError_t err, const char *where)
 {
     return efail(e, UMCP_ERR_DEVICE, std::string(where) + ": " + hipGetErrorString(err));
+}
+
+// waits for every launch that may still read the work tables or a member
+hipError_t quiesce(umcp_ensemble *e)
+{
+    hipError_t err = hipStreamSynchronize(e->stream);
+    if (err == hipSuccess && e->last_stream && e->last_stream != e->stream)
+        err = hipStreamSynchronize(e->last_stream);
+    return err;
 }
 
 void drop_graph(umcp_ensemble *e)
@@ -115,7 +125,7 @@ int upload_table(umcp_ensemble *e, T **dst, const std::vector<T> &src)
 int build(umcp_ensemble *e)
 {
     drop_graph(e);
-    hipError_t err = hipStreamSynchronize(e->stream);   // old tables may still be read
+    hipError_t err = quiesce(e);   // old tables may still be read
     if (err != hipSuccess) return ehip(e, err, "hipStreamSynchronize");
     drop_tables(e);
     e->dirty = false;
@@ -253,6 +263,8 @@ void umcp_ensemble_destroy(umcp_ensemble *e)
 {
     if (!e) return;
     (void)hipSetDevice(e->device);
+    // the caller's stream may be gone by now; hipFree of the tables waits for
+    // the device
     (void)hipStreamSynchronize(e->stream);
     drop_graph(e);
     drop_tables(e);
@@ -282,7 +294,7 @@ int umcp_ensemble_remove(umcp_ensemble *e, umcp_surface *s)
         if (e->members[i].s == s) {
             // the device tables still point at the member: finish their work first
             (void)hipSetDevice(e->device);
-            hipError_t err = hipStreamSynchronize(e->stream);
+            hipError_t err = quiesce(e);
             if (err != hipSuccess) return ehip(e, err, "hipStreamSynchronize");
             e->members.erase(e->members.begin() + (long)i);
             e->dirty = true;
@@ -335,6 +347,12 @@ int umcp_ensemble_bd_steps(umcp_ensemble *e, int nsteps, int graph_steps, void *
     if (e->dirty && (rc = build(e))) return rc;
     if (e->members.empty() || nsteps == 0) return UMCP_OK;
     hipStream_t st = stream ? (hipStream_t)stream : e->stream;
+    if (e->last_stream && e->last_stream != st && e->last_stream != e->stream) {
+        // steps on another stream still in flight touch the same positions
+        err = hipStreamSynchronize(e->last_stream);
+        if (err != hipSuccess) return ehip(e, err, "hipStreamSynchronize");
+    }
+    e->last_stream = st;
     int done = 0;
     if (graph_steps > 0 && nsteps >= graph_steps) {
         if (!e->exec || e->graph_steps != graph_steps || e->graph_stream != st) {

@@ -5,7 +5,7 @@
 // loop; this is the production front end of the batched ensemble:
 //
 //   submit()    copies the request into the FIFO queue (any thread)
-//   prep thread takes the queue head once a slot is free and its device
+//   prep threads take the queue head once a slot is free and its device
 //               footprint fits the HBM budget: template (parsed and planned
 //               on the host once per mesh file) -> clone -> plan -> setg0 ->
 //               start positions; the footprint is reserved before any device
@@ -15,7 +15,8 @@
 //               every active request, retires the finished ones (results to
 //               host, device memory freed) and repeats
 //
-// Preparation of new requests overlaps the worker's chunks on the device.  A
+// Preparation of new requests (several at once, one per preparation thread,
+// each on the request's own stream) overlaps the worker's chunks.  A
 // request's trajectory is the one it has alone: the ensemble runs each
 // member's own kernels and per-member (seed, step) noise.
 #include <hip/hip_runtime_api.h>
@@ -83,11 +84,13 @@ struct umcp_server {
     std::mutex tmpl_mu;
     umcp_ensemble *ens = nullptr;
     hipStream_t stream = nullptr;
-    std::thread worker, prep;
+    std::thread worker;
+    std::vector<std::thread> prep;
     std::string err;
     // statistics
     long chunks = 0;
     double member_steps = 0, face_steps = 0, busy_s = 0;
+    double build_s = 0, retire_s = 0, wait_s = 0, setup_s = 0;   // where the worker's and prep's time goes
     int n_done = 0;
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
@@ -210,7 +213,12 @@ void prep_loop(umcp_server *sv)
             sv->peak_used = std::max(sv->peak_used, sv->used);
             q->footprint = need;
         }
+        const double t_setup = sv->now();
         const int rc = setup(sv, q, tmpl);
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            sv->setup_s += sv->now() - t_setup;
+        }
         if (rc != UMCP_OK) {
             const std::string msg = q->s ? umcp_last_error(q->s) : "set-up failed";
             {
@@ -262,6 +270,7 @@ void work_loop(umcp_server *sv)
     (void)hipSetDevice(sv->device);
     for (;;) {
         std::vector<Request *> joining;
+        const double t_w = sv->now();
         {
             std::unique_lock<std::mutex> lk(sv->mu);
             sv->cv_work.wait(lk, [&] {
@@ -270,6 +279,7 @@ void work_loop(umcp_server *sv)
             });
             if (sv->ready.empty() && sv->active.empty()) return;   // stopping and drained
             joining.swap(sv->ready);
+            sv->wait_s += sv->now() - t_w;
         }
         for (Request *q : joining) {
             const int rc = umcp_ensemble_add(sv->ens, q->s, q->s->d_r, &q->bd, q->seed, q->d_energies,
@@ -290,11 +300,12 @@ void work_loop(umcp_server *sv)
         if (act.empty()) continue;
         int chunk = sv->cfg.chunk_steps;
         for (Request *q : act) chunk = std::min(chunk, q->remaining);
+        const double t_0 = sv->now();
+        int rc = umcp_ensemble_bd_steps(sv->ens, 0, 0, sv->stream);   // tables of the new membership
         const double t_a = sv->now();
         for (Request *q : act)
             if (q->t_start < 0) q->t_start = t_a;
-        int rc = UMCP_OK;
-        if (chunk > 0) {
+        if (rc == UMCP_OK && chunk > 0) {
             const int g = sv->cfg.graph_steps > 0 ? std::min(sv->cfg.graph_steps, chunk) : 0;
             rc = umcp_ensemble_bd_steps(sv->ens, chunk, g, sv->stream);
             hipError_t e = hipStreamSynchronize(sv->stream);
@@ -306,6 +317,7 @@ void work_loop(umcp_server *sv)
         {
             std::lock_guard<std::mutex> lk(sv->mu);
             sv->chunks++;
+            sv->build_s += t_a - t_0;
             sv->busy_s += t_b - t_a;
             sv->member_steps += (double)chunk * act.size();
             sv->face_steps += (double)chunk * faces;
@@ -321,7 +333,10 @@ void work_loop(umcp_server *sv)
                     (void)umcp_ensemble_remove(sv->ens, q->s);
                     finish_failed(sv, q, rc, umcp_ensemble_last_error(sv->ens));
                 } else {
+                    const double t_r = sv->now();
                     retire(sv, q);
+                    std::lock_guard<std::mutex> lk(sv->mu);
+                    sv->retire_s += sv->now() - t_r;
                 }
             }
         }
@@ -357,7 +372,8 @@ int umcp_server_create(const umcp_server_config *cfg, umcp_server **out)
         return rc;
     }
     sv->worker = std::thread(work_loop, sv);
-    sv->prep = std::thread(prep_loop, sv);
+    const int np = cfg->prep_threads > 0 ? cfg->prep_threads : 1;
+    for (int i = 0; i < np; i++) sv->prep.emplace_back(prep_loop, sv);
     *out = sv;
     return UMCP_OK;
 }
@@ -422,19 +438,16 @@ int umcp_server_drain(umcp_server *sv)
     return UMCP_OK;
 }
 
-int umcp_server_stats(umcp_server *sv, double *stats)
+int umcp_server_stats(umcp_server *sv, double *stats, int n)
 {
-    if (!sv || !stats) return UMCP_ERR_ARG;
+    if (!sv || !stats || n < 0) return UMCP_ERR_ARG;
     std::lock_guard<std::mutex> lk(sv->mu);
-    stats[0] = sv->n_done;
-    stats[1] = (double)sv->chunks;
-    stats[2] = sv->member_steps;
-    stats[3] = sv->face_steps;
-    stats[4] = sv->busy_s;
-    stats[5] = (double)sv->peak_used;
-    stats[6] = sv->peak_active;
-    stats[7] = sv->now();
-    return UMCP_OK;
+    const double v[] = {(double)sv->n_done, (double)sv->chunks, sv->member_steps, sv->face_steps, sv->busy_s,
+                        (double)sv->peak_used, (double)sv->peak_active, sv->now(), sv->build_s, sv->retire_s,
+                        sv->wait_s, sv->setup_s};
+    const int m = std::min(n, (int)(sizeof v / sizeof v[0]));
+    for (int i = 0; i < m; i++) stats[i] = v[i];
+    return m;
 }
 
 const char *umcp_server_last_error(const umcp_server *sv) { return sv ? sv->err.c_str() : "null handle"; }
@@ -456,7 +469,8 @@ void umcp_server_destroy(umcp_server *sv)
         sv->cv_work.notify_all();
         sv->cv_done.notify_all();
     }
-    if (sv->prep.joinable()) sv->prep.join();
+    for (std::thread &t : sv->prep)
+        if (t.joinable()) t.join();
     if (sv->worker.joinable()) sv->worker.join();
     umcp_ensemble_destroy(sv->ens);
     for (auto &kv : sv->templates) umcp_destroy(kv.second);

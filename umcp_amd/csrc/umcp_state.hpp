@@ -34,6 +34,8 @@ struct umcp_surface {
     std::vector<umcp::ClassDev> cls;
     std::vector<void *> allocations;
     size_t device_bytes = 0;              // sum of the allocation sizes
+    char *arena = nullptr;                // the plan's arrays (plan_bytes)
+    size_t arena_size = 0, arena_used = 0;
     // work buffers
     double *d_r = nullptr, *d_g = nullptr, *d_gbuf = nullptr, *d_partial = nullptr;
     float *d_gbuf32 = nullptr;
@@ -75,6 +77,8 @@ struct umcp_surface {
 };
 
 namespace umcp {
+
+constexpr size_t kArenaAlign = 256;
 
 // helpers of the surface object used by both translation units
 bool restraints_on(const umcp_surface *s);

@@ -64,7 +64,7 @@ class Restraints(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("budget_bytes", C.c_double), ("max_active", C.c_int), ("chunk_steps", C.c_int),
                 ("precision", C.c_int), ("point_split", C.c_int), ("flags", C.c_int), ("face_grid", C.c_int),
-                ("order", C.c_int), ("graph_steps", C.c_int)]
+                ("order", C.c_int), ("graph_steps", C.c_int), ("prep_threads", C.c_int)]
 
 
 class Request(C.Structure):
@@ -139,7 +139,7 @@ def load_library(path: str = LIB_PATH):
         "umcp_server_submit": ([vp, C.POINTER(Request), ip], C.c_int),
         "umcp_server_wait": ([vp, C.c_int, dp, dp, dp], C.c_int),
         "umcp_server_drain": ([vp], C.c_int),
-        "umcp_server_stats": ([vp, dp], C.c_int),
+        "umcp_server_stats": ([vp, dp, C.c_int], C.c_int),
         "umcp_server_last_error": ([vp], C.c_char_p),
         "umcp_server_destroy": ([vp], None),
     }
@@ -446,10 +446,10 @@ class Server:
 
     def __init__(self, budget_bytes: float = 200e9, max_active: int = 64, chunk_steps: int = 100,
                  precision: int = PREC_MIXED, point_split: int = 4, flags: int = LAUNCH_OCC2,
-                 face_grid: int = 0, order: int = 0, graph_steps: int = 0):
+                 face_grid: int = 0, order: int = 0, graph_steps: int = 0, prep_threads: int = 4):
         self._L = load_library()
         cfg = ServerConfig(budget_bytes, max_active, chunk_steps, precision, point_split, flags, face_grid,
-                           order, graph_steps)
+                           order, graph_steps, prep_threads)
         h = C.c_void_p()
         rc = self._L.umcp_server_create(C.byref(cfg), C.byref(h))
         if rc != 0:
@@ -492,9 +492,10 @@ class Server:
         self._L.umcp_server_drain(self._h)
 
     def stats(self) -> dict:
-        v = np.zeros(8)
-        self._L.umcp_server_stats(self._h, _dp(v))
-        keys = ("done", "chunks", "request_steps", "face_steps", "busy_s", "peak_bytes", "peak_active", "clock_s")
+        v = np.zeros(12)
+        self._L.umcp_server_stats(self._h, _dp(v), len(v))
+        keys = ("done", "chunks", "request_steps", "face_steps", "busy_s", "peak_bytes", "peak_active", "clock_s",
+                "build_s", "retire_s", "wait_s", "setup_s")
         return dict(zip(keys, v.tolist()))
 
     def close(self):
