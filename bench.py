@@ -55,6 +55,8 @@ FEAT_HELFRICH = 2         # energy-form template argument of the face kernel (mo
 
 def kernel_name(precision: str, split: int, flags: int, feat: int = FEAT_HELFRICH) -> str:
     """The face kernel's symbol as rocprofv3 prints it (one workgroup per item)."""
+    if precision == "mixed" and flags & 4:   # matrix-core path
+        return "faces_batch_mfma<%d, %d, %d, false>" % (split, 2 if flags & 2 else 1, feat)
     occ = (4 if precision == "mixed" else 2) if flags & 2 else 1
     return "faces_batch_kernel<%s, %d, true, %d, %d, false>" % (
         "float" if precision == "mixed" else "double", split, occ, feat)
@@ -135,7 +137,7 @@ def run_leg(args, w, c0, precision, d_xi, dev, torch):
         sp0, fl0 = s.get_launch()
         s.set_launch(args.split or sp0, args.flags if args.flags >= 0 else fl0)
     elif not args.no_tune:
-        best, times = tune.tune(s, d_r, d_E, d_g, stream=sptr)
+        best, times = tune.tune(s, d_r, d_E, d_g, stream=sptr, mixed=precision == "mixed")
         tuned = {"best": list(best), "us": {"%d/%d" % k: round(v, 2) for k, v in times.items()}}
     # energy and gradient at the start configuration (for the error report)
     torch.cuda.synchronize()
@@ -211,7 +213,7 @@ def run_ensemble_leg(args, dev, torch):
     torch.cuda.synchronize()
     tuned = None
     if not args.no_tune:
-        best, times = tune.tune_ensemble(ens, surfs, d_rs, torch, stream)
+        best, times = tune.tune_ensemble(ens, surfs, d_rs, torch, stream, shapes=tune.ENSEMBLE_SHAPES_MIXED)
         tuned = {"best": list(best), "ms_per_step": {"%d/%d/%d/%d" % k: round(v, 4) for k, v in times.items()}}
     nf = sum(m.nfaces for m in members)
     # first W steps (graph capture included), then K timed steps

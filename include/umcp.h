@@ -265,8 +265,12 @@ void umcp_server_destroy(umcp_server *sv);
 /* launch shape of the face kernel (no reference counterpart; set by the
  * tuner): point_split in 1..4 waves share the Gauss points of 64 faces
  * (their control points staged in LDS); flags: UMCP_LAUNCH_OCC2 bounds
- * registers for two waves per SIMD */
+ * registers (two waves per SIMD in FP64, four in the mixed VALU path, two on
+ * the matrix-core path); UMCP_LAUNCH_MFMA (mixed precision only, ignored in
+ * FP64) runs the basis contractions on the matrix cores, with point_split 1,
+ * 2 or 4 */
 #define UMCP_LAUNCH_OCC2 2
+#define UMCP_LAUNCH_MFMA 4
 int umcp_set_launch(umcp_surface *s, int point_split, int flags);
 int umcp_get_launch(const umcp_surface *s, int *point_split, int *flags);
 

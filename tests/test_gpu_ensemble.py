@@ -47,11 +47,13 @@ def specs():
     return out
 
 
-def make_member(sp, precision=S.PREC_FP64):
+def make_member(sp, precision=S.PREC_FP64, launch=None):
     w = sp["w"]
     g = S.Surface(w.mesh_path)
     g.set_params(precision=precision, **sp["params"])
     g.generatePlan()
+    if launch:
+        g.set_launch(*launch)
     g.setg0(w.r0)
     if sp["restraints"]:
         g.set_restraints(**sp["restraints"])
@@ -63,8 +65,8 @@ def coeffs(sp):
     return S.bd_coefficients(1e-12, 1e-9, sp["w"].T)
 
 
-def run_alone(sp, steps, precision=S.PREC_FP64):
-    g = make_member(sp, precision)
+def run_alone(sp, steps, precision=S.PREC_FP64, launch=None):
+    g = make_member(sp, precision, launch)
     d_r = torch.tensor(sp["w"].r, device="cuda", dtype=torch.float64)
     e = torch.zeros(steps, device="cuda", dtype=torch.float64)
     mob, amp = coeffs(sp)
@@ -73,9 +75,10 @@ def run_alone(sp, steps, precision=S.PREC_FP64):
     return d_r.cpu(), e.cpu()
 
 
-def run_ensemble(sps, steps, graph_steps=0, face_grid=0, order=0, precision=S.PREC_FP64, chunks=1):
+def run_ensemble(sps, steps, graph_steps=0, face_grid=0, order=0, precision=S.PREC_FP64, chunks=1,
+                 launch=(4, S.LAUNCH_OCC2)):
     ens = S.Ensemble()
-    ens.set_launch(4, S.LAUNCH_OCC2, face_grid, order)
+    ens.set_launch(launch[0], launch[1], face_grid, order)
     keep = []
     for sp in sps:
         g = make_member(sp, precision)
@@ -179,3 +182,16 @@ def test_mixed_ensemble_close_to_fp64(mix):
         # is ~3e-6 of it
         assert e_rel <= (1e-6 if i < 3 else 1e-5)
         assert dr <= 1e-5
+
+
+@pytest.mark.parametrize("split", [2, 4])
+def test_mfma_ensemble_equals_alone(mix, split):
+    """The matrix-core mixed path runs the same items alone and batched: each
+    member's trajectory is bitwise the one it has alone with that launch."""
+    sps, _ = mix
+    launch = (split, S.LAUNCH_MFMA)
+    say("[ensemble] mixed matrix-core split", split)
+    alone = [run_alone(sp, STEPS, S.PREC_MIXED, launch) for sp in sps]
+    got, _ = run_ensemble(sps, STEPS, graph_steps=4, face_grid=64, precision=S.PREC_MIXED, chunks=2,
+                          launch=launch)
+    check_same("mfma", got, alone)

@@ -55,24 +55,89 @@ def errors(g, o, r, tag):
     return rel_e, rms
 
 
-def test_mixed_vesicle_40962():
+@pytest.fixture(scope="module")
+def vesicle_40962_pair():
     w = workloads.vesicle_40962()
     c0 = workloads.face_c0_uniform(81920, w.extra["c0_seed"])
     g, o = pair(w, c0=c0)
     g.set_restraints(**w.extra["restraints"])
     o.set_restraints(**w.extra["restraints"])
-    rel_e, rms = errors(g, o, w.r, "vesicle_40962")
+    return w, g, o, (o.energy(w.r), o.grad(w.r))
+
+
+# launch shapes: VALU path (flags 0 / OCC2) and matrix-core path (MFMA)
+SHAPES = [(s, f) for s in (1, 2, 3, 4) for f in (0, S.LAUNCH_OCC2)] + \
+         [(s, f) for s in (1, 2, 4) for f in (S.LAUNCH_MFMA, S.LAUNCH_MFMA | S.LAUNCH_OCC2)]
+
+
+def errors_cached(g, ref, r, tag):
+    E_o, g_o = ref
+    E_g, g_g = g.energy_grad(r)
+    rel_e = abs(E_g - E_o) / abs(E_o)
+    rms = np.sqrt(np.mean((g_g[:-3] - g_o[:-3]) ** 2)) / np.sqrt(np.mean(g_o[:-3] ** 2))
+    say("[mixed]", tag, "E rel", rel_e, "grad rms rel", rms)
+    return rel_e, rms
+
+
+@pytest.mark.parametrize("split,flags", [(2, S.LAUNCH_OCC2), (2, S.LAUNCH_MFMA), (4, S.LAUNCH_MFMA)])
+def test_mixed_vesicle_40962(vesicle_40962_pair, split, flags):
+    w, g, o, ref = vesicle_40962_pair
+    g.set_launch(split, flags)
+    rel_e, rms = errors_cached(g, ref, w.r, "vesicle_40962 split %d flags %d" % (split, flags))
     assert rel_e <= E_TOL and rms <= G_RMS_TOL
 
 
 def test_mixed_small_meshes_all_shapes():
     w = workloads.vesicle_642()
     g, o = pair(w)
-    for split in (1, 2, 3, 4):
-        for flags in (0, 2):
-            g.set_launch(split, flags)
-            rel_e, rms = errors(g, o, w.r, "vesicle_642 split %d flags %d" % (split, flags))
-            assert rel_e <= E_TOL and rms <= G_RMS_TOL
+    ref = (o.energy(w.r), o.grad(w.r))
+    for split, flags in SHAPES:
+        g.set_launch(split, flags)
+        rel_e, rms = errors_cached(g, ref, w.r, "vesicle_642 split %d flags %d" % (split, flags))
+        assert rel_e <= E_TOL and rms <= G_RMS_TOL
+
+
+@pytest.mark.parametrize("form", ["lipid", "general"])
+def test_mfma_energy_forms_and_restraints(form):
+    """The matrix-core path in the lipid form and in the general form (kg and
+    the shear regulariser on), with both global restraints, against the
+    oracle; and bitwise repeatable."""
+    w = workloads.vesicle_642()
+    params = {"mode": 0} if form == "lipid" else {"kg": -5.0, "k_reg": 0.5}
+    g, o = pair(w, params=params)
+    rst = {"KA_global": 0.1, "KV": 0.004}
+    g.set_restraints(**rst)
+    o.set_restraints(**rst)
+    rng = np.random.default_rng(3)
+    r = w.r.copy()
+    r[:-3] += 0.3 * rng.standard_normal(r.size - 3)
+    ref = (o.energy(r), o.grad(r))
+    for split in (1, 2, 4):
+        g.set_launch(split, S.LAUNCH_MFMA)
+        rel_e, rms = errors_cached(g, ref, r, "mfma %s split %d" % (form, split))
+        assert rel_e <= E_TOL and rms <= G_RMS_TOL
+        e1, g1 = g.energy_grad(r)
+        e2, g2 = g.energy_grad(r)
+        assert e1 == e2 and np.array_equal(g1, g2)
+
+
+def test_mfma_periodic_sheet():
+    w = workloads.sheet_128()
+    g, o = pair(w)
+    ref = (o.energy(w.r), o.grad(w.r))
+    for split in (2, 4):
+        g.set_launch(split, S.LAUNCH_MFMA | S.LAUNCH_OCC2)
+        rel_e, rms = errors_cached(g, ref, w.r, "mfma sheet_128 split %d" % split)
+        assert rel_e <= E_TOL and rms <= G_RMS_TOL
+
+
+def test_mfma_rejects_split_3():
+    w = workloads.vesicle_642()
+    g = S.Surface(w.mesh_path)
+    g.set_params(precision=S.PREC_MIXED, **w.params)
+    g.generatePlan()
+    with pytest.raises(S.UmcpError):
+        g.set_launch(3, S.LAUNCH_MFMA)
 
 
 def test_mixed_periodic_sheet():

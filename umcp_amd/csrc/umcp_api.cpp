@@ -820,7 +820,9 @@ int umcp_set_launch(umcp_surface *s, int point_split, int flags)
     if (!s || !s->planned) return UMCP_ERR_STATE;
     if (point_split < 1 || point_split > kMaxSplit)
         return fail(s, UMCP_ERR_ARG, "point split must be between 1 and 4 waves");
-    if (flags & ~kLaunchOcc2) return fail(s, UMCP_ERR_ARG, "unknown launch flags");
+    if (flags & ~kLaunchFlags) return fail(s, UMCP_ERR_ARG, "unknown launch flags");
+    if ((flags & kLaunchMfma) && point_split == 3)
+        return fail(s, UMCP_ERR_ARG, "the matrix-core path takes point split 1, 2 or 4");
     s->point_split = point_split;
     s->launch_flags = flags;
     return UMCP_OK;

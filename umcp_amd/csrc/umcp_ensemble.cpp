@@ -316,7 +316,9 @@ int umcp_ensemble_set_launch(umcp_ensemble *e, int point_split, int flags, int f
 {
     if (!e) return UMCP_ERR_ARG;
     if (point_split < 1 || point_split > kMaxSplit) return efail(e, UMCP_ERR_ARG, "point split must be 1..4");
-    if (flags & ~kLaunchOcc2) return efail(e, UMCP_ERR_ARG, "unknown launch flags");
+    if (flags & ~kLaunchFlags) return efail(e, UMCP_ERR_ARG, "unknown launch flags");
+    if ((flags & kLaunchMfma) && point_split == 3)
+        return efail(e, UMCP_ERR_ARG, "the matrix-core path takes point split 1, 2 or 4");
     if (face_grid < 0) return efail(e, UMCP_ERR_ARG, "negative grid");
     if (order < 0 || order > 1) return efail(e, UMCP_ERR_ARG, "order is 0 (member-major) or 1 (round-robin)");
     if (order != e->order) e->dirty = true;

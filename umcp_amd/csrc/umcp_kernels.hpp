@@ -117,6 +117,10 @@ int face_groups(int nf);
 // (whose control points are staged in LDS); flag kLaunchOcc2 bounds registers
 // for two waves per SIMD.  feat: energy-form specialisation (features_of).
 constexpr int kLaunchOcc2 = 2;
+// flag kLaunchMfma (mixed precision only): the basis contractions run on the
+// matrix cores (faces_batch_mfma; point splits 1, 2 and 4)
+constexpr int kLaunchMfma = 4;
+constexpr int kLaunchFlags = kLaunchOcc2 | kLaunchMfma;
 int features_of(const PhysParams &ph);
 hipError_t launch_face_metric(const FaceArgs &a, int kind, int mode, double *g0_out,
                               double *rurv0_out, double *ig0_out, double *partial, int partial_base,

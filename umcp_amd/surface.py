@@ -25,6 +25,7 @@ MODE_HELFRICH = 1
 PREC_FP64 = 0
 PREC_MIXED = 1
 LAUNCH_OCC2 = 2
+LAUNCH_MFMA = 4
 
 # every symbol include/umcp.h declares
 EXPORTS = [

@@ -1,11 +1,12 @@
 """Launch-shape autotuner for the face kernel.
 
-The fused face kernel has two tunable choices: how many waves share the
-Gauss points of a 64-face group (1..4) and whether registers are bounded for
-two waves per SIMD.  ``tune`` times every
-shape on the caller's own surface with the library's HIP-event profiling and
-keeps the fastest; it works on device buffers and never changes ``r`` (it
-evaluates energy + gradient only).
+The fused face kernel has three tunable choices: how many waves share the
+Gauss points of a 64-face group (1..4), whether registers are bounded, and
+(mixed precision) whether the basis contractions run on the matrix cores
+(point splits 1, 2 and 4 there).  ``tune`` times every shape on the caller's
+own surface with the library's HIP-event profiling and keeps the fastest; it
+works on device buffers and never changes ``r`` (it evaluates energy +
+gradient only).
 """
 from __future__ import annotations
 
@@ -13,15 +14,24 @@ import itertools
 
 SPLITS = (1, 2, 3, 4)
 FLAGS = (0, 2)
+MFMA_SPLITS = (1, 2, 4)
+MFMA_FLAGS = (4, 6)
 
 
-def tune(surface, d_r, d_E, d_g, stream=None, reps: int = 10, sync=None):
+def shapes(mixed: bool):
+    out = list(itertools.product(SPLITS, FLAGS))
+    if mixed:
+        out += list(itertools.product(MFMA_SPLITS, MFMA_FLAGS))
+    return out
+
+
+def tune(surface, d_r, d_E, d_g, stream=None, reps: int = 10, sync=None, mixed: bool = False):
     """Returns ((split, flags), {shape: microseconds}) and applies the best."""
     if sync is None:
         import torch
         sync = torch.cuda.synchronize
     times = {}
-    for split, flags in itertools.product(SPLITS, FLAGS):
+    for split, flags in shapes(mixed):
         surface.set_launch(split, flags)
         surface.energy_grad_dev(d_r, d_E, d_g, stream)   # first launch: code load
         sync()
@@ -40,6 +50,10 @@ def tune(surface, d_r, d_E, d_g, stream=None, reps: int = 10, sync=None):
 # (0 = one workgroup per item) and item order
 ENSEMBLE_SHAPES = tuple((sp, fl, grid, order)
                         for sp in (2, 4) for fl in (0, 2) for grid in (0, 1024, 2048) for order in (0, 1))
+# mixed precision adds the matrix-core path
+ENSEMBLE_SHAPES_MIXED = ENSEMBLE_SHAPES + tuple((sp, fl, grid, order)
+                                                for sp in (2, 4) for fl in (4, 6) for grid in (0, 1024)
+                                                for order in (0, 1))
 
 
 def tune_ensemble(ens, surfaces, d_rs, torch, stream, steps: int = 8, shapes=ENSEMBLE_SHAPES):
