@@ -156,6 +156,42 @@ int umcp_set_restraints(umcp_surface *s, const umcp_restraints *rs);
  * polyhedron of the limit positions of the vertices (closed meshes) */
 int umcp_volume(umcp_surface *s, const double *r, double *V);
 
+/* Momentum-space integrators of src/hd.C with the effective-mass matrix
+ * (Cartesian momenta, one per vertex coordinate).
+ * umcp_effm_build (getSparseEffectiveMass, src/modes.C:1719-1960, over the
+ * 66-point lattice of every face and the point mass of src/hd.C:418-481;
+ * after umcp_setg0, which fixes the reference area): dense mass matrix and
+ * Cholesky inverse on the host (O(nv^3); nv <= UMCP_EFFM_MAX_NV), truncated
+ * as the reference does (rel_tol 1e-20), kept on the device as CSR.
+ * umcp_effm_steps(_dev): per step g = grad(r); UMCP_EFFM_BD
+ * (do_bd_membrane): pp = -g Delta + gamma^-1 N(0, s) / Delta, momenta cleared
+ * after the step; UMCP_EFFM_LD (do_ld, OLD_LANGEVIN friction): pp += -Delta
+ * gamma (Mi pp) / alpha^2 - g Delta + N(0, s), momenta carried; then r +=
+ * (Mi pp) / alpha^2 Delta, with Delta = AKMA_TIME dt and s = sqrt(2 gamma kT
+ * Delta) (src/hd.C:1543-1720, 1943-1947).  xi: [nsteps][3nv] standard
+ * normals in (vertex, coordinate) order, or NULL (device entry only) for the
+ * device Philox stream of (seed, step counter) that umcp_bd_steps_rng_dev
+ * uses. */
+#define UMCP_EFFM_BD 0
+#define UMCP_EFFM_LD 1
+#define UMCP_EFFM_MAX_NV 4096
+typedef struct {
+    int mode;              /* UMCP_EFFM_BD or UMCP_EFFM_LD */
+    double dt;             /* time step, s */
+    double kT;             /* kcal/mol */
+    double gamma;          /* gamma_langevin, the reference's units */
+} umcp_effm_params;
+int umcp_effm_build(umcp_surface *s, long *nnz, double *point_mass);
+/* the (truncated) inverse effective mass as a dense nv x nv array */
+int umcp_effm_get(umcp_surface *s, double *Minv);
+/* host arrays: r (3nv+3) and pp (3nv) in/out, xi [nsteps][3nv], energies
+ * (energy at the start of each step) may be NULL */
+int umcp_effm_steps(umcp_surface *s, double *r, double *pp, const double *xi, int nsteps,
+                    const umcp_effm_params *p, double *energies);
+/* device arrays on the surface's device; stream may be NULL */
+int umcp_effm_steps_dev(umcp_surface *s, double *d_r, double *d_pp, const double *d_xi, int nsteps,
+                        const umcp_effm_params *p, unsigned long long seed, double *d_energies, void *stream);
+
 /* Brownian steps drawing their noise on the device: xi of step k is the
  * counter-based stream of (seed, k) that umcp_noise writes (Philox4x64-10
  * keyed by seed, Box-Muller), so a trajectory depends only on (seed, first

@@ -73,6 +73,16 @@ def lib():
         L.orc_volume_grad.argtypes = [vp, dp, dp]
         L.orc_restraint_energy.argtypes = [vp, dp, dp, dp]
         L.orc_restraint_energy.restype = C.c_double
+        L.orc_effm_points_per_face.restype = C.c_int
+        L.orc_effm_point_mass.argtypes = [vp]
+        L.orc_effm_point_mass.restype = C.c_double
+        L.orc_effm_mass.argtypes = [vp, C.c_double, dp]
+        L.orc_effm_invert.argtypes = [C.c_int, dp]
+        L.orc_effm_invert.restype = C.c_int
+        L.orc_effm_truncate.argtypes = [C.c_int, dp, C.c_double]
+        L.orc_effm_truncate.restype = C.c_long
+        L.orc_effm_steps.argtypes = [vp, dp, dp, dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                     dp, dp]
         _LIB = L
     return _LIB
 
@@ -184,6 +194,46 @@ class OracleSurface:
         lib().orc_bd_steps(self._h, _p(r), _p(xi), n, mobility_dt, noise_amp,
                            _p(en) if en is not None else None)
         return (r, en) if want_energies else r
+
+    # effective-mass integrators (src/hd.C, src/modes.C)
+    def effm_point_mass(self) -> float:
+        return lib().orc_effm_point_mass(self._h)
+
+    def effm_mass(self, mass_per_pt: float | None = None) -> np.ndarray:
+        m = self.effm_point_mass() if mass_per_pt is None else mass_per_pt
+        M = np.zeros((self.nv, self.nv))
+        lib().orc_effm_mass(self._h, m, _p(M))
+        return M
+
+    @staticmethod
+    def effm_invert(M: np.ndarray) -> np.ndarray:
+        A = np.ascontiguousarray(M, dtype=np.float64).copy()
+        rc = lib().orc_effm_invert(A.shape[0], _p(A))
+        if rc != 0:
+            raise ValueError("oracle: mass matrix not positive definite (pivot %d)" % (rc - 1))
+        return A
+
+    @staticmethod
+    def effm_truncate(Mi: np.ndarray, rel_tol: float = 1e-20):
+        A = np.ascontiguousarray(Mi, dtype=np.float64).copy()
+        kept = lib().orc_effm_truncate(A.shape[0], _p(A), rel_tol)
+        return A, int(kept)
+
+    def effm_inverse(self) -> np.ndarray:
+        """The truncated inverse effective mass the integrator uses."""
+        return self.effm_truncate(self.effm_invert(self.effm_mass()))[0]
+
+    def effm_steps(self, r: np.ndarray, pp: np.ndarray, xi: np.ndarray, mode: int, dt: float, kT: float,
+                   gamma: float, Mi: np.ndarray, energies: bool = False):
+        r = np.ascontiguousarray(r, dtype=np.float64).copy()
+        pp = np.ascontiguousarray(pp, dtype=np.float64).copy()
+        xi = np.ascontiguousarray(xi, dtype=np.float64)
+        Mi = np.ascontiguousarray(Mi, dtype=np.float64)
+        n = xi.shape[0]
+        e = np.zeros(n)
+        lib().orc_effm_steps(self._h, _p(r), _p(pp), _p(xi), n, int(mode), dt, kT, gamma, _p(Mi),
+                             _p(e) if energies else None)
+        return (r, pp, e) if energies else (r, pp)
 
     def face_info(self, f: int):
         owner, ncp, tri = C.c_int(), C.c_int(), C.c_int()

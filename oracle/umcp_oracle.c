@@ -1761,3 +1761,209 @@ void orc_bd_steps(omesh *m, double *r, const double *xi, int nsteps, double mobi
     }
     free(g);
 }
+
+/* ---- effective-mass integrators (src/hd.C:418-481, 1543-1720, 1943-1947; */
+/* src/modes.C:1719-1960; src/uv_map.C:14-200; src/parallel.C:1028-1075)    */
+
+#define ORC_AMU_PER_KG (1.0 / 1.661e-27)          /* src/units.h */
+#define ORC_AKMA_TIME (1.0 / (0.04888 * (1e-12)))  /* src/units.h */
+#define ORC_PLIM 10                                /* src/hd.C:419 */
+
+int orc_effm_points_per_face(void) { return (ORC_PLIM + 1) * (ORC_PLIM + 2) / 2; }
+
+/* mass of one of the ppf points on each face (src/hd.C:449-455): two leaflets
+ * of POPC (MSCALE 5) over the reference area area0 */
+double orc_effm_point_mass(omesh *m)
+{
+    double A, A0;
+    double *r = malloc(sizeof(double) * (3 * (size_t)m->nv + 3));
+    orc_get_positions(m, r);
+    orc_area(m, r, &A, &A0);
+    free(r);
+    const double MSCALE = 5;
+    const double mass_per_lipid = MSCALE * 1000 * 760.09 / 6.022e23 / 1000;
+    const double area_per_lipid = 65.35;
+    const int ppf = orc_effm_points_per_face();
+    return ORC_AMU_PER_KG * 2 * A0 / (m->nfr + m->nfi) / ppf * mass_per_lipid / area_per_lipid;
+}
+
+/* position coefficients of the limit surface at (u, v) of face f
+ * (surface::get_pt_coeffs, src/uv_map.C:14-200): the box spline on regular
+ * faces, the subdivision route on irregular ones, and at the extraordinary
+ * vertex itself (u = v = 0) its limit mask (1/2 on the vertex, 1/(2 val) on
+ * each neighbour, src/uv_map.C:36-64).  Returns the number of coefficients. */
+static int pt_coeffs(const omesh *m, int f, double u, double v, double *coef, int *list)
+{
+    int owner, ncp, cp[OMAXCP], tri;
+    orc_face_info(m, f, &owner, &ncp, cp, &tri);
+    if (f >= m->nfr) {
+        int val = m->val[owner];
+        if (u < 1e-10 && v < 1e-10) {
+            coef[0] = 0.5;
+            list[0] = owner;
+            for (int e = 0; e < val; e++) {
+                coef[1 + e] = 1.0 / (val * 2);
+                list[1 + e] = m->nb[owner * OMAXV + e];
+            }
+            return 1 + val;
+        }
+        double *tab = malloc(sizeof(double) * 6 * (size_t)ncp);
+        orc_irregular_table(val, u, v, tab);
+        for (int x = 0; x < ncp; x++) {
+            coef[x] = tab[x];
+            list[x] = cp[x];
+        }
+        free(tab);
+        return ncp;
+    }
+    double B[6][12];
+    orc_basis(u, v, B);
+    for (int x = 0; x < ncp; x++) {
+        coef[x] = B[0][x];
+        list[x] = cp[x];
+    }
+    return ncp;
+}
+
+/* M_ij = sum over triangles (src/hd.C:459-478 triangle order) and the
+ * (plim+1)(plim+2)/2 lattice points (i/plim, j/plim) of m c_i c_j
+ * (src/modes.C:1736-1741); M is nv x nv, row-major */
+void orc_effm_mass(omesh *m, double mass_per_pt, double *M)
+{
+    int nv = m->nv;
+    memset(M, 0, sizeof(double) * (size_t)nv * nv);
+    double coef[OMAXCP];
+    int list[OMAXCP];
+    for (int t = 0; t < m->nt; t++) {
+        int f = m->tri_face[t];
+        if (f < 0) continue;
+        for (int fi = 0; fi <= ORC_PLIM; fi++)
+            for (int fj = 0; fj <= ORC_PLIM - fi; fj++) {
+                int n = pt_coeffs(m, f, fi / (double)ORC_PLIM, fj / (double)ORC_PLIM, coef, list);
+                for (int c1 = 0; c1 < n; c1++)
+                    for (int c2 = 0; c2 < n; c2++)
+                        M[(size_t)list[c1] * nv + list[c2]] += coef[c1] * coef[c2] * mass_per_pt;
+            }
+    }
+}
+
+/* in-place inverse of a symmetric positive-definite matrix by Cholesky
+ * (the dpotrf + dpotri of src/modes.C:1827-1847, textbook form: A = L L^T,
+ * A^-1 = L^-T L^-1); returns 0, or the failing pivot + 1 */
+int orc_effm_invert(int n, double *A)
+{
+    double *L = calloc((size_t)n * n, sizeof(double));
+    for (int j = 0; j < n; j++) {
+        double d = A[(size_t)j * n + j];
+        for (int k = 0; k < j; k++) d -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
+        if (!(d > 0.0)) {
+            free(L);
+            return j + 1;
+        }
+        double ljj = sqrt(d);
+        L[(size_t)j * n + j] = ljj;
+        for (int i = j + 1; i < n; i++) {
+            double s = A[(size_t)i * n + j];
+            for (int k = 0; k < j; k++) s -= L[(size_t)i * n + k] * L[(size_t)j * n + k];
+            L[(size_t)i * n + j] = s / ljj;
+        }
+    }
+    /* W = L^-1 (lower triangular), column by column */
+    double *W = calloc((size_t)n * n, sizeof(double));
+    for (int j = 0; j < n; j++) {
+        W[(size_t)j * n + j] = 1.0 / L[(size_t)j * n + j];
+        for (int i = j + 1; i < n; i++) {
+            double s = 0.0;
+            for (int k = j; k < i; k++) s -= L[(size_t)i * n + k] * W[(size_t)k * n + j];
+            W[(size_t)i * n + j] = s / L[(size_t)i * n + i];
+        }
+    }
+    /* A^-1 = W^T W */
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j <= i; j++) {
+            double s = 0.0;
+            for (int k = i; k < n; k++) s += W[(size_t)k * n + i] * W[(size_t)k * n + j];
+            A[(size_t)i * n + j] = A[(size_t)j * n + i] = s;
+        }
+    free(L);
+    free(W);
+    return 0;
+}
+
+/* truncation of the inverse (src/modes.C:1889-1905, rel_tol = 1e-20 at
+ * src/modes.C:16): entries with |Mi_ij| / sqrt(|Mi_ii Mi_jj|) <= rel_tol are
+ * dropped (set to zero); returns the number kept */
+long orc_effm_truncate(int n, double *Mi, double rel_tol)
+{
+    long kept = 0;
+    double *d = malloc(sizeof(double) * (size_t)n);
+    for (int i = 0; i < n; i++) d[i] = Mi[(size_t)i * n + i];
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            double *x = Mi + (size_t)i * n + j;
+            if (fabs(*x) / sqrt(fabs(d[i] * d[j])) > rel_tol) kept++;
+            else *x = 0.0;
+        }
+    free(d);
+    return kept;
+}
+
+/* vout[v][c] = (Mi . vin_c)[v] (src/parallel.C:1028-1075 without the scale) */
+static void effm_mult(int nv, const double *Mi, const double *vin, double *vout)
+{
+    for (int v = 0; v < nv; v++)
+        for (int c = 0; c < 3; c++) {
+            double s = 0.0;
+            for (int u = 0; u < nv; u++) s += Mi[(size_t)v * nv + u] * vin[3 * u + c];
+            vout[3 * v + c] = s;
+        }
+}
+
+/* nsteps of the Cartesian momentum-space integrator of src/hd.C with the
+ * inverse effective mass Mi (nv x nv): mode 0 = do_bd_membrane (momenta reset
+ * after every step, src/hd.C:1943-1947), mode 1 = do_ld with OLD_LANGEVIN
+ * friction through Mi (src/hd.C:40, 1546-1557).  Per step: g = grad(r);
+ * friction (mode 1); half kick; noise (mode 0: gamma^-1 N(0, s) / Delta,
+ * mode 1: N(0, s), s = sqrt(2 gamma kT Delta), xi[step][3nv] in Q = 3v + c
+ * order, src/hd.C:1654-1673); half kick; r += (Mi pp) / alpha^2 Delta
+ * (src/hd.C:1699-1720).  pp[3nv] carries the momenta (mode 1). */
+void orc_effm_steps(omesh *m, double *r, double *pp, const double *xi, int nsteps, int mode, double dt,
+                    double kT, double gamma, const double *Mi, double *energies)
+{
+    int nv = m->nv, n3 = 3 * nv;
+    double *g = malloc(sizeof(double) * (size_t)(n3 + 3));
+    double *next = malloc(sizeof(double) * (size_t)n3);
+    double *fr = malloc(sizeof(double) * (size_t)n3);
+    const double akma = ORC_AKMA_TIME;
+    for (int s = 0; s < nsteps; s++) {
+        if (energies) energies[s] = orc_energy(m, r);
+        memset(g, 0, sizeof(double) * (size_t)(n3 + 3));
+        orc_grad(m, r, g);
+        const double al[3] = {r[n3], r[n3 + 1], r[n3 + 2]};
+        if (mode == 0) memset(pp, 0, sizeof(double) * (size_t)n3);
+        memcpy(next, pp, sizeof(double) * (size_t)n3);
+        if (mode == 1) {
+            const double scale = -akma * dt;
+            effm_mult(nv, Mi, pp, fr);
+            for (int v = 0; v < nv; v++)
+                for (int c = 0; c < 3; c++) next[3 * v + c] += fr[3 * v + c] * (gamma * scale / (al[c] * al[c]));
+        }
+        for (int i = 0; i < n3; i++) next[i] += -g[i] * akma * dt / 2;
+        memcpy(pp, next, sizeof(double) * (size_t)n3);
+        const double *x = xi + (size_t)s * n3;
+        const double sig = sqrt(2 * gamma * kT * akma * dt);
+        for (int i = 0; i < n3; i++) {
+            if (mode == 0) next[i] += (1.0 / gamma) * (sig * x[i]) / (akma * dt);
+            else next[i] += sig * x[i];
+        }
+        for (int i = 0; i < n3; i++) next[i] += -g[i] * akma * dt / 2;
+        memcpy(pp, next, sizeof(double) * (size_t)n3);
+        effm_mult(nv, Mi, pp, fr);
+        for (int v = 0; v < nv; v++)
+            for (int c = 0; c < 3; c++) r[3 * v + c] += fr[3 * v + c] * (1.0 / (al[c] * al[c])) * akma * dt;
+        if (mode == 0) memset(pp, 0, sizeof(double) * (size_t)n3);
+    }
+    free(g);
+    free(next);
+    free(fr);
+}

@@ -54,6 +54,21 @@ double orc_volume(const omesh *m, const double *r);
 void   orc_volume_grad(const omesh *m, const double *r, double *dV);  /* 3 nv */
 double orc_restraint_energy(omesh *m, const double *r, double *lamA, double *lamV);
 
+/* effective-mass integrators of src/hd.C (Cartesian momenta): the point
+ * mass of the ppf lattice points per face, the dense mass matrix M (nv x nv),
+ * its inverse by Cholesky (0 or failing pivot + 1), the reference's
+ * truncation of the inverse, and nsteps of the integrator (mode 0 =
+ * do_bd_membrane, 1 = do_ld with OLD_LANGEVIN friction); pp[3nv] momenta,
+ * xi[nsteps][3nv] standard normals, dt in s, kT in kcal/mol, gamma in the
+ * reference's units (gamma_langevin) */
+int    orc_effm_points_per_face(void);
+double orc_effm_point_mass(omesh *m);
+void   orc_effm_mass(omesh *m, double mass_per_pt, double *M);
+int    orc_effm_invert(int n, double *A);
+long   orc_effm_truncate(int n, double *Mi, double rel_tol);
+void   orc_effm_steps(omesh *m, double *r, double *pp, const double *xi, int nsteps, int mode, double dt,
+                      double kT, double gamma, const double *Mi, double *energies);
+
 /* introspection for tests */
 int    orc_face_info(const omesh *m, int f, int *owner, int *ncp, int *cp, int *tri);
 int    orc_face_table(const omesh *m, int f, int p, double *w6, double *weight,

@@ -738,6 +738,139 @@ int umcp_bd_steps_rng_dev(umcp_surface *s, double *d_r, int nsteps, const umcp_b
     return UMCP_OK;
 }
 
+int umcp_effm_build(umcp_surface *s, long *nnz, double *point_mass)
+{
+    int rc = check_ready(s);
+    if (rc) return rc;
+    if (s->T.nv > UMCP_EFFM_MAX_NV)
+        return fail(s, UMCP_ERR_ARG, "effective-mass integrators are set up for at most UMCP_EFFM_MAX_NV vertices "
+                                     "(dense O(nv^3) inverse, as the reference)");
+    // point mass (src/hd.C:449-455): two leaflets of POPC (MSCALE 5) over the
+    // reference area, spread over the lattice points of all faces
+    const double kAmuPerKg = 1.0 / 1.661e-27;
+    const double mass_per_lipid = 5 * 1000 * 760.09 / 6.022e23 / 1000;
+    const double area_per_lipid = 65.35;
+    const double m = kAmuPerKg * 2 * s->A0_setg0 / s->P.nfaces / effm_points_per_face() * mass_per_lipid /
+                     area_per_lipid;
+    EffmCsr E;
+    std::string err;
+    if (effm_inverse_csr(s->T, s->P, m, 1e-20, E, err)) return fail(s, UMCP_ERR_BAD_MESH, err);
+    // replace an earlier build
+    for (void *p : s->effm_allocs) {
+        (void)hipFree(p);
+        s->allocations.erase(std::find(s->allocations.begin(), s->allocations.end(), p));
+    }
+    s->effm_allocs.clear();
+    s->effm_ready = false;
+    const size_t before = s->allocations.size();
+    const int nv = s->T.nv;
+    if ((rc = dev_alloc(s, &s->d_effm_rowptr, E.rowptr.size())) || (rc = dev_alloc(s, &s->d_effm_col, E.col.size())) ||
+        (rc = dev_alloc(s, &s->d_effm_val, E.val.size())) || (rc = dev_alloc(s, &s->d_pp_next, 3 * (size_t)nv)))
+        return rc;
+    s->effm_allocs.assign(s->allocations.begin() + (long)before, s->allocations.end());
+    if ((rc = dev_upload(s, s->d_effm_rowptr, E.rowptr.data(), E.rowptr.size())) ||
+        (rc = dev_upload(s, s->d_effm_col, E.col.data(), E.col.size())) ||
+        (rc = dev_upload(s, s->d_effm_val, E.val.data(), E.val.size())) || (rc = sync(s)))
+        return rc;
+    s->effm_nnz = (long)E.val.size();
+    s->effm_point_mass = m;
+    s->effm_ready = true;
+    if (nnz) *nnz = s->effm_nnz;
+    if (point_mass) *point_mass = m;
+    return UMCP_OK;
+}
+
+int umcp_effm_get(umcp_surface *s, double *Minv)
+{
+    if (!s || !Minv) return UMCP_ERR_ARG;
+    if (!s->effm_ready) return fail(s, UMCP_ERR_STATE, "effective mass not built (call umcp_effm_build)");
+    // the truncated matrix: the dense inverse with dropped entries zeroed
+    const int nv = s->T.nv;
+    std::memset(Minv, 0, sizeof(double) * (size_t)nv * nv);
+    std::vector<int> rowptr(nv + 1), col((size_t)s->effm_nnz);
+    std::vector<double> val((size_t)s->effm_nnz);
+    hipError_t e = hipMemcpy(rowptr.data(), s->d_effm_rowptr, sizeof(int) * rowptr.size(), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(col.data(), s->d_effm_col, sizeof(int) * col.size(), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(val.data(), s->d_effm_val, sizeof(double) * val.size(), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(s, e, "download effective mass");
+    for (int i = 0; i < nv; i++)
+        for (int k = rowptr[i]; k < rowptr[i + 1]; k++) Minv[(size_t)i * nv + col[k]] = val[k];
+    return UMCP_OK;
+}
+
+int umcp_effm_steps_dev(umcp_surface *s, double *d_r, double *d_pp, const double *d_xi, int nsteps,
+                        const umcp_effm_params *p, unsigned long long seed, double *d_energies, void *stream)
+{
+    int rc = check_ready(s);
+    if (rc) return rc;
+    if (!s->effm_ready) return fail(s, UMCP_ERR_STATE, "effective mass not built (call umcp_effm_build)");
+    if (!d_r || !d_pp || !p || nsteps < 0) return fail(s, UMCP_ERR_ARG, "null argument");
+    if (p->mode != UMCP_EFFM_BD && p->mode != UMCP_EFFM_LD) return fail(s, UMCP_ERR_ARG, "unknown integrator");
+    if (!(p->gamma > 0.0) || !(p->dt > 0.0) || !(p->kT >= 0.0))
+        return fail(s, UMCP_ERR_ARG, "effective-mass integrator needs gamma > 0, dt > 0, kT >= 0");
+    hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+    const size_t n3 = 3 * (size_t)s->T.nv;
+    for (int k = 0; k < nsteps; k++) {
+        GatherArgs ga{};
+        ga.out_mode = 0;
+        ga.g = s->d_g;
+        ga.energies = d_energies;
+        ga.step = k;
+        if ((rc = enqueue_eval(s, d_r, st, ga, s->d_step))) return rc;
+        EffmArgs ea{};
+        ea.nv = s->T.nv;
+        ea.rowptr = s->d_effm_rowptr;
+        ea.col = s->d_effm_col;
+        ea.val = s->d_effm_val;
+        ea.g = s->d_g;
+        ea.pp = d_pp;
+        ea.pp_next = s->d_pp_next;
+        ea.pp_keep = d_pp;
+        ea.r = d_r;
+        ea.xi = d_xi ? d_xi + (size_t)k * n3 : nullptr;
+        ea.seed = seed;
+        ea.step_ctr = s->d_step;
+        ea.mode = p->mode == UMCP_EFFM_LD ? kEffmLangevin : kEffmBrownian;
+        ea.akma = 1.0 / (0.04888 * (1e-12));   // AKMA_TIME, src/units.h
+        ea.dt = p->dt;
+        ea.kT = p->kT;
+        ea.gamma = p->gamma;
+        hipError_t e = launch_effm_step(ea, st);
+        if (e != hipSuccess) return hip_fail(s, e, "effective-mass step launch");
+    }
+    return UMCP_OK;
+}
+
+int umcp_effm_steps(umcp_surface *s, double *r, double *pp, const double *xi, int nsteps, const umcp_effm_params *p,
+                    double *energies)
+{
+    int rc = check_ready(s);
+    if (rc) return rc;
+    if (!r || !pp || !xi || nsteps < 0) return fail(s, UMCP_ERR_ARG, "null argument");
+    const size_t n3 = 3 * (size_t)s->T.nv;
+    double *d_pp = nullptr, *d_xi = nullptr, *d_en = nullptr;
+    hipError_t e = hipMalloc(&d_pp, sizeof(double) * n3);
+    if (e == hipSuccess && nsteps > 0) e = hipMalloc(&d_xi, sizeof(double) * n3 * (size_t)nsteps);
+    if (e == hipSuccess && energies && nsteps > 0) e = hipMalloc(&d_en, sizeof(double) * (size_t)nsteps);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_pp, pp, sizeof(double) * n3, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess && nsteps > 0)
+        e = hipMemcpyAsync(d_xi, xi, sizeof(double) * n3 * (size_t)nsteps, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) rc = upload_r(s, r);
+    if (e == hipSuccess && rc == UMCP_OK) rc = umcp_effm_steps_dev(s, s->d_r, d_pp, d_xi, nsteps, p, 0, d_en, nullptr);
+    if (e == hipSuccess && rc == UMCP_OK) {
+        e = hipMemcpyAsync(r, s->d_r, sizeof(double) * (n3 + 3), hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(pp, d_pp, sizeof(double) * n3, hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess && d_en)
+            e = hipMemcpyAsync(energies, d_en, sizeof(double) * (size_t)nsteps, hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    }
+    (void)hipFree(d_pp);
+    (void)hipFree(d_xi);
+    (void)hipFree(d_en);
+    if (e != hipSuccess) return hip_fail(s, e, "effective-mass steps");
+    return rc;
+}
+
 int umcp_noise(unsigned long long seed, int step, int n, double *d_out, void *stream)
 {
     if (!d_out || n < 0 || step < 0) return UMCP_ERR_ARG;

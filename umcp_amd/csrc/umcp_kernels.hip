@@ -993,6 +993,104 @@ __device__ __forceinline__ void gather_vertices(const GatherArgs &a, int vblock)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Momentum-space integrators of src/hd.C with the inverse effective mass
+// (src/hd.C:1543-1720, 1943-1947; src/parallel.C:1028-1075).  The inverse is
+// a CSR matrix over vertices applied to each Cartesian component; one wave
+// per row, lanes striding over the row's entries and a fixed xor butterfly,
+// so the products are deterministic.  Operation order follows the
+// reference's expressions (contraction off).
+
+__device__ __forceinline__ void effm_row(const EffmArgs &a, int v, const double *x, double out[3])
+{
+    #pragma clang fp contract(off)
+    const int lane = threadIdx.x & (kWave - 1);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    const int e0 = a.rowptr[v], e1 = a.rowptr[v + 1];
+    for (int e = e0 + lane; e < e1; e += kWave) {
+        const double m = a.val[e];
+        const int u = a.col[e];
+        s0 += m * x[3 * (size_t)u];
+        s1 += m * x[3 * (size_t)u + 1];
+        s2 += m * x[3 * (size_t)u + 2];
+    }
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        s0 += __shfl_xor(s0, off, kWave);
+        s1 += __shfl_xor(s1, off, kWave);
+        s2 += __shfl_xor(s2, off, kWave);
+    }
+    out[0] = s0;
+    out[1] = s1;
+    out[2] = s2;
+}
+
+// momenta of the step: friction through the inverse mass (Langevin), half
+// kick, noise, half kick (src/hd.C:1546-1557, 1600-1605, 1650-1693)
+__global__ __launch_bounds__(kEffmBlock) void effm_momentum_kernel(EffmArgs a)
+{
+    #pragma clang fp contract(off)
+    const int v = blockIdx.x * (kEffmBlock / kWave) + (int)threadIdx.x / kWave;
+    if (v >= a.nv) return;   // whole waves leave together
+    const int lane = threadIdx.x & (kWave - 1);
+    const double al[3] = {a.r[3 * a.nv], a.r[3 * a.nv + 1], a.r[3 * a.nv + 2]};
+    double fr[3] = {0.0, 0.0, 0.0};
+    if (a.mode == kEffmLangevin) effm_row(a, v, a.pp, fr);
+    if (lane != 0) return;
+    double z[3];
+    if (a.xi) {
+        z[0] = a.xi[3 * (size_t)v];
+        z[1] = a.xi[3 * (size_t)v + 1];
+        z[2] = a.xi[3 * (size_t)v + 2];
+    } else {
+        normal3(a.seed, a.step_ctr[0] - 1, v, z);
+    }
+    const double Delta = a.akma * a.dt;
+    const double sig = sqrt(2 * a.gamma * a.kT * a.akma * a.dt);
+    const double scale = -a.akma * a.dt;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double g = a.g[3 * (size_t)v + c];
+        double next = a.mode == kEffmLangevin ? a.pp[3 * (size_t)v + c] : 0.0;
+        if (a.mode == kEffmLangevin) next += fr[c] * (a.gamma * scale / (al[c] * al[c]));
+        next += -g * a.akma * a.dt / 2;
+        if (a.mode == kEffmLangevin) next += sig * z[c];
+        else next += (1.0 / a.gamma) * (sig * z[c]) / Delta;
+        next += -g * a.akma * a.dt / 2;
+        a.pp_next[3 * (size_t)v + c] = next;
+    }
+}
+
+// positions: r += (Mi pp) / alpha^2 Delta (src/hd.C:1699-1720); the momenta
+// carried to the next step (Langevin) or cleared (Brownian, src/hd.C:1943-1947)
+__global__ __launch_bounds__(kEffmBlock) void effm_position_kernel(EffmArgs a)
+{
+    #pragma clang fp contract(off)
+    const int v = blockIdx.x * (kEffmBlock / kWave) + (int)threadIdx.x / kWave;
+    if (v >= a.nv) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    double q[3];
+    effm_row(a, v, a.pp_next, q);
+    if (lane != 0) return;
+    const double al[3] = {a.r[3 * a.nv], a.r[3 * a.nv + 1], a.r[3 * a.nv + 2]};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        a.r[3 * (size_t)v + c] += q[c] * (1.0 / (al[c] * al[c])) * a.akma * a.dt;
+        a.pp_keep[3 * (size_t)v + c] = a.mode == kEffmLangevin ? a.pp_next[3 * (size_t)v + c] : 0.0;
+    }
+}
+
+hipError_t launch_effm_step(const EffmArgs &a, hipStream_t st)
+{
+    const int nb = (a.nv + kEffmBlock / kWave - 1) / (kEffmBlock / kWave);
+    if (nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(effm_momentum_kernel, dim3(nb), dim3(kEffmBlock), 0, st, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(effm_position_kernel, dim3(nb), dim3(kEffmBlock), 0, st, a);
+    return hipGetLastError();
+}
+
 template <typename GB>
 __global__ __launch_bounds__(kGatherBlock) void vertex_gather(GatherArgs a)
 {

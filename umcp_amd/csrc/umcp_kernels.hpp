@@ -97,6 +97,28 @@ struct GatherArgs {
     int energies_cap;       // energies[step] is written while step < energies_cap (Philox)
 };
 
+// One step of the momentum-space integrators (src/hd.C) after the gradient
+// of the step is in g: momenta (friction, kicks, noise), then positions.
+constexpr int kEffmBlock = 256;
+constexpr int kEffmBrownian = 0, kEffmLangevin = 1;
+struct EffmArgs {
+    int nv;
+    const int *rowptr;      // nv + 1, CSR of the inverse effective mass
+    const int *col;
+    const double *val;
+    const double *g;        // 3 nv + 3 gradient of this step
+    const double *pp;       // momenta in (Langevin)
+    double *pp_next;        // momenta of this step (scratch, 3 nv)
+    double *pp_keep;        // momenta carried to the next step (the caller's array)
+    double *r;              // 3 nv + 3 positions, alpha last
+    const double *xi;       // 3 nv standard normals of this step, or null: Philox (seed, *step_ctr - 1)
+    unsigned long long seed;
+    const int *step_ctr;
+    int mode;               // kEffmBrownian or kEffmLangevin
+    double akma, dt, kT, gamma;
+};
+hipError_t launch_effm_step(const EffmArgs &a, hipStream_t st);
+
 struct GatherItem {
     int surf, vblock;
 };

@@ -41,6 +41,7 @@ EXPORTS = [
     "umcp_ensemble_set_launch", "umcp_ensemble_get_launch", "umcp_ensemble_bd_steps", "umcp_ensemble_work",
     "umcp_clone", "umcp_device_bytes", "umcp_server_create", "umcp_server_submit", "umcp_server_wait",
     "umcp_server_drain", "umcp_server_stats", "umcp_server_last_error", "umcp_server_destroy",
+    "umcp_effm_build", "umcp_effm_get", "umcp_effm_steps", "umcp_effm_steps_dev",
 ]
 
 
@@ -56,6 +57,15 @@ class Params(C.Structure):
 
 class BDParams(C.Structure):
     _fields_ = [("mobility_dt", C.c_double), ("noise_amp", C.c_double)]
+
+
+EFFM_BD = 0
+EFFM_LD = 1
+EFFM_MAX_NV = 4096
+
+
+class EffmParams(C.Structure):
+    _fields_ = [("mode", C.c_int), ("dt", C.c_double), ("kT", C.c_double), ("gamma", C.c_double)]
 
 
 class Restraints(C.Structure):
@@ -123,6 +133,11 @@ def load_library(path: str = LIB_PATH):
         "umcp_bd_steps_rng_dev": ([vp, vp, C.c_int, C.POINTER(BDParams), C.c_ulonglong, vp, C.c_int, vp],
                                   C.c_int),
         "umcp_noise": ([C.c_ulonglong, C.c_int, C.c_int, vp, vp], C.c_int),
+        "umcp_effm_build": ([vp, C.POINTER(C.c_long), dp], C.c_int),
+        "umcp_effm_get": ([vp, dp], C.c_int),
+        "umcp_effm_steps": ([vp, dp, dp, dp, C.c_int, C.POINTER(EffmParams), dp], C.c_int),
+        "umcp_effm_steps_dev": ([vp, vp, vp, vp, C.c_int, C.POINTER(EffmParams), C.c_ulonglong, vp, vp],
+                                C.c_int),
         "umcp_ensemble_create": ([C.POINTER(vp)], C.c_int),
         "umcp_ensemble_destroy": ([vp], None),
         "umcp_ensemble_last_error": ([vp], C.c_char_p),
@@ -294,6 +309,40 @@ class Surface:
                                            C.c_void_p(d_energies.data_ptr()) if d_energies is not None
                                            else None, cap, C.c_void_p(stream) if stream else None)
         self._check(rc, "bd_steps_rng_dev")
+
+    # momentum-space integrators with the effective-mass matrix (src/hd.C)
+    def effm_build(self):
+        """Builds the truncated inverse effective mass; returns (nnz, point mass)."""
+        nnz, m = C.c_long(), C.c_double()
+        self._check(self._L.umcp_effm_build(self._h, C.byref(nnz), C.byref(m)), "effm_build")
+        return nnz.value, m.value
+
+    def effm_get(self) -> np.ndarray:
+        Mi = np.zeros((self.nv, self.nv))
+        self._check(self._L.umcp_effm_get(self._h, _dp(Mi)), "effm_get")
+        return Mi
+
+    def effm_steps(self, r: np.ndarray, pp: np.ndarray, xi: np.ndarray, mode: int, dt: float, kT: float,
+                   gamma: float, energies: bool = False):
+        """Host arrays: returns (r, pp) or (r, pp, energies) after xi.shape[0] steps."""
+        r = np.ascontiguousarray(r, dtype=np.float64).copy()
+        pp = np.ascontiguousarray(pp, dtype=np.float64).copy()
+        xi = np.ascontiguousarray(xi, dtype=np.float64)
+        n = xi.shape[0]
+        e = np.zeros(max(n, 1))
+        prm = EffmParams(mode, dt, kT, gamma)
+        self._check(self._L.umcp_effm_steps(self._h, _dp(r), _dp(pp), _dp(xi), n, C.byref(prm),
+                                            _dp(e) if energies else None), "effm_steps")
+        return (r, pp, e[:n]) if energies else (r, pp)
+
+    def effm_steps_dev(self, d_r, d_pp, nsteps: int, mode: int, dt: float, kT: float, gamma: float,
+                       d_xi=None, seed: int = 0, d_energies=None, stream=None):
+        """Device arrays (torch tensors); d_xi None: the device noise stream of (seed, step)."""
+        prm = EffmParams(mode, dt, kT, gamma)
+        ptr = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
+        self._check(self._L.umcp_effm_steps_dev(self._h, ptr(d_r), ptr(d_pp), ptr(d_xi), int(nsteps), C.byref(prm),
+                                                C.c_ulonglong(seed), ptr(d_energies),
+                                                C.c_void_p(stream) if stream else None), "effm_steps_dev")
 
     def set_step(self, step: int):
         self._check(self._L.umcp_set_step(self._h, int(step)), "set_step")
