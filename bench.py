@@ -15,8 +15,10 @@ Prints one JSON line: value = FP64 face-evals/s of that loop, the roofline of
 the dominant kernel (the face kernel), the FP32-mixed leg (throughput,
 speedup, error against the FP64 oracle), the batched-ensemble leg (256
 vesicles in one launch per kernel, hipGraph, tuned), the serving leg
-(Poisson request stream, continuous batching against one-at-a-time) and the
-CPU oracle timed on a bounded sample of the headline workload.
+(Poisson request stream, continuous batching against one-at-a-time), the
+effective-mass integrators of src/hd.C on the 642-vertex vesicle (Brownian and
+Langevin forms, set-up time) and the CPU oracle timed on a bounded sample of
+the headline workload.
 """
 from __future__ import annotations
 
@@ -299,6 +301,64 @@ def run_ensemble_leg(args, dev, torch):
     }
 
 
+def run_effm_leg(args, dev, torch):
+    """Momentum-space integrators of src/hd.C with the effective-mass matrix
+    on the 642-vertex vesicle (the size the reference's O(nv^3) set-up
+    serves): set-up time (mass matrix, Cholesky inverse, truncation, upload),
+    then K device-noise steps of the Brownian and of the Langevin form, and
+    the serial oracle on a bounded sample of the same steps."""
+    import numpy as np
+    from umcp_amd import surface as S
+    from umcp_amd import units, workloads
+    w = workloads.vesicle_642()
+    s = S.Surface(w.mesh_path)
+    s.set_params(precision=S.PREC_FP64, **w.params)
+    s.generatePlan()
+    s.setg0(w.r0)
+    t0 = time.perf_counter()
+    nnz, m = s.effm_build()
+    build_s = time.perf_counter() - t0
+    dt, gamma, kT = 1e-11, 0.001, units.kT(298.0)
+    K = args.effm_steps
+    stream = torch.cuda.Stream(device=dev)
+    sptr = stream.cuda_stream
+    out = {"workload": "vesicle_642_effm", "nv": s.nv, "faces": s.nfaces, "precision": "fp64",
+           "inverse_mass_nnz": nnz, "point_mass_amu": m, "setup_s": build_s, "dt_s": dt,
+           "gamma_langevin": gamma, "steps": K}
+    for name, mode in (("brownian", S.EFFM_BD), ("langevin", S.EFFM_LD)):
+        d_r = torch.tensor(w.r, device=dev, dtype=torch.float64)
+        d_pp = torch.zeros(3 * s.nv, device=dev, dtype=torch.float64)
+        s.set_step(0)
+        s.effm_steps_dev(d_r, d_pp, 5, mode, dt, kT, gamma, seed=7, stream=sptr)
+        torch.cuda.synchronize()
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        s.effm_steps_dev(d_r, d_pp, K, mode, dt, kT, gamma, seed=7, stream=sptr)
+        end.record(stream)
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(end) / K
+        out[name] = {"ms_per_step": ms, "steps_per_s": 1e3 / ms, "face_evals_per_s": s.nfaces * 1e3 / ms,
+                     "max_displacement_A": float((d_r - torch.tensor(w.r, device=dev)).abs().max())}
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        o = O.OracleSurface(w.mesh_path)
+        o.set_params(**w.params)
+        o.setg0(w.r0)
+        t0 = time.perf_counter()
+        Mi = o.effm_inverse()
+        t_setup = time.perf_counter() - t0
+        n = 20
+        xi = np.random.default_rng(1).standard_normal((n, 3 * o.nv))
+        t0 = time.perf_counter()
+        o.effm_steps(w.r, np.zeros(3 * o.nv), xi, S.EFFM_LD, dt, kT, gamma, Mi)
+        t_steps = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n / t_steps, "unit": "Langevin steps/s", "cores": 1, "kind": "port",
+                               "setup_s": t_setup,
+                               "sample": "%d Langevin steps of the 642-vertex vesicle, serial oracle" % n}
+        out["langevin"]["speedup_vs_cpu"] = out["langevin"]["steps_per_s"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def run_serving_leg(args, dev, torch):
     """Serving stream: Poisson arrivals of Brownian-dynamics requests
     (icospheres of level 2-6 or periodic sheets of 32^2..256^2 vertices,
@@ -390,6 +450,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=2000)
     ap.add_argument("--graph-steps", type=int, default=50)
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream leg")
+    ap.add_argument("--no-effm", action="store_true", help="skip the effective-mass integrator leg")
+    ap.add_argument("--effm-steps", type=int, default=500)
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-slots", type=int, default=64)
@@ -467,6 +529,8 @@ def main():
         out["ensemble"] = run_ensemble_leg(args, dev, torch)
     if not args.no_serving:
         out["serving"] = run_serving_leg(args, dev, torch)
+    if not args.no_effm:
+        out["effm"] = run_effm_leg(args, dev, torch)
     if not args.no_cpu_baseline:
         base, E_o, g_o = cpu_baseline(w, c0, args.cpu_budget)
         out["cpu_baseline"] = base

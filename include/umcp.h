@@ -192,6 +192,18 @@ int umcp_effm_steps(umcp_surface *s, double *r, double *pp, const double *xi, in
 int umcp_effm_steps_dev(umcp_surface *s, double *d_r, double *d_pp, const double *d_xi, int nsteps,
                         const umcp_effm_params *p, unsigned long long seed, double *d_energies, void *stream);
 
+/* On-disk formats.  umcp_save_mesh: surface::saveSurface
+ * (src/write_structures.C:473-540) with positions r (3nv+3; NULL: the
+ * file's).  Restart files (Simulation::saveRestart / loadRestart,
+ * src/restart.C:7-168): the alpha line, nv + 1 rows of positions (the last is
+ * the alpha row) with the momenta when pp is given (six values per row, as
+ * the reference writes for Cartesian momenta), then "seed N".  Values are
+ * written with %.17g; the reader also takes the reference's %lf files, skips a
+ * complex section and reports has_pp (six-value rows) and seed (-1: none). */
+int umcp_save_mesh(umcp_surface *s, const double *r, const char *path);
+int umcp_write_restart(const char *path, int nv, const double *r, const double *pp, int seed);
+int umcp_read_restart(const char *path, int nv, double *r, double *pp, double *alpha, int *has_pp, int *seed);
+
 /* Brownian steps drawing their noise on the device: xi of step k is the
  * counter-based stream of (seed, k) that umcp_noise writes (Philox4x64-10
  * keyed by seed, Box-Muller), so a trajectory depends only on (seed, first

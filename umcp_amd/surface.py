@@ -42,6 +42,7 @@ EXPORTS = [
     "umcp_clone", "umcp_device_bytes", "umcp_server_create", "umcp_server_submit", "umcp_server_wait",
     "umcp_server_drain", "umcp_server_stats", "umcp_server_last_error", "umcp_server_destroy",
     "umcp_effm_build", "umcp_effm_get", "umcp_effm_steps", "umcp_effm_steps_dev",
+    "umcp_save_mesh", "umcp_write_restart", "umcp_read_restart",
 ]
 
 
@@ -133,6 +134,9 @@ def load_library(path: str = LIB_PATH):
         "umcp_bd_steps_rng_dev": ([vp, vp, C.c_int, C.POINTER(BDParams), C.c_ulonglong, vp, C.c_int, vp],
                                   C.c_int),
         "umcp_noise": ([C.c_ulonglong, C.c_int, C.c_int, vp, vp], C.c_int),
+        "umcp_save_mesh": ([vp, dp, C.c_char_p], C.c_int),
+        "umcp_write_restart": ([C.c_char_p, C.c_int, dp, dp, C.c_int], C.c_int),
+        "umcp_read_restart": ([C.c_char_p, C.c_int, dp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
         "umcp_effm_build": ([vp, C.POINTER(C.c_long), dp], C.c_int),
         "umcp_effm_get": ([vp, dp], C.c_int),
         "umcp_effm_steps": ([vp, dp, dp, dp, C.c_int, C.POINTER(EffmParams), dp], C.c_int),
@@ -310,6 +314,12 @@ class Surface:
                                            else None, cap, C.c_void_p(stream) if stream else None)
         self._check(rc, "bd_steps_rng_dev")
 
+    def save_mesh(self, path: str, r: np.ndarray | None = None):
+        """Writes the surface as a .mesh file (positions r, or the file's)."""
+        rr = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
+        self._check(self._L.umcp_save_mesh(self._h, _dp(rr) if rr is not None else None, path.encode()),
+                    "save_mesh")
+
     # momentum-space integrators with the effective-mass matrix (src/hd.C)
     def effm_build(self):
         """Builds the truncated inverse effective mass; returns (nnz, point mass)."""
@@ -413,6 +423,30 @@ def bd_coefficients(D_m2_per_s: float, dt_s: float, T_K: float = 298.0):
     """(mobility_dt, noise_amp) of r <- r - (D dt/kT) dE/dr + sqrt(2 D dt) xi."""
     D = units.diffusion_A2_per_s(D_m2_per_s)
     return D * dt_s / units.kT(T_K), float(np.sqrt(2.0 * D * dt_s))
+
+
+def write_restart(path: str, r: np.ndarray, pp: np.ndarray | None = None, seed: int = 0):
+    """Restart file of positions r (3nv+3) and optional momenta pp (3nv)."""
+    L = load_library()
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    nv = (r.size - 3) // 3
+    p = None if pp is None else np.ascontiguousarray(pp, dtype=np.float64)
+    rc = L.umcp_write_restart(path.encode(), nv, _dp(r), _dp(p) if p is not None else None, int(seed))
+    if rc != 0:
+        raise UmcpError("write_restart failed (%d)" % rc)
+
+
+def read_restart(path: str, nv: int):
+    """Returns dict(r, pp or None, alpha, seed) of a restart file of nv vertices."""
+    L = load_library()
+    r = np.zeros(3 * nv + 3)
+    pp = np.zeros(3 * nv)
+    al = np.zeros(3)
+    has, seed = C.c_int(), C.c_int()
+    rc = L.umcp_read_restart(path.encode(), nv, _dp(r), _dp(pp), _dp(al), C.byref(has), C.byref(seed))
+    if rc != 0:
+        raise UmcpError("read_restart failed (%d): %s" % (rc, path))
+    return {"r": r, "pp": pp if has.value else None, "alpha": al, "seed": seed.value}
 
 
 def noise(seed: int, step: int, n: int, d_out, stream=None):
