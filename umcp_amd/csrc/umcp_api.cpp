@@ -193,7 +193,7 @@ int enqueue_volume(umcp_surface *s, const double *d_r, hipStream_t st)
     hipError_t e = hipMemcpyAsync(s->d_fa_aux, fa.data(), fa.size() * sizeof(FaceArgs), hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(s, e, "upload face table");
     BatchFaceArgs fb{s->d_fa_aux, s->d_items, s->npartial, d_r, nullptr};
-    e = launch_restraint_prepass(fb, s->d_zero, 1, st);
+    e = launch_restraint_prepass(fb, s->d_zero, 1, false, st);
     if (e != hipSuccess) return hip_fail(s, e, "restraint kernel launch");
     return UMCP_OK;
 }
@@ -248,7 +248,7 @@ int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs 
     fb.r = d_r;
     fb.step_ctr = step_ctr;
     if (rson) {
-        hipError_t e = launch_restraint_prepass(fb, s->d_zero, 1, st);
+        hipError_t e = launch_restraint_prepass(fb, s->d_zero, 1, s->prm.precision == UMCP_PREC_MIXED, st);
         if (e != hipSuccess) return hip_fail(s, e, "restraint kernel launch");
     }
     hipEvent_t ev[2] = {nullptr, nullptr};

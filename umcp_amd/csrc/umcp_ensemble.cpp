@@ -224,7 +224,7 @@ hipError_t enqueue_step(umcp_ensemble *e, hipStream_t st)
     if (e->nrestrained > 0) {
         // over every item: groups of members without restraints return at once
         BatchFaceArgs rb{e->d_fa, e->d_items, e->nitems, nullptr, nullptr};
-        err = launch_restraint_prepass(rb, e->d_first, e->nrestrained, st);
+        err = launch_restraint_prepass(rb, e->d_first, e->nrestrained, e->mixed, st);
         if (err != hipSuccess) return err;
     }
     for (int f = 0; f < 3; f++) {

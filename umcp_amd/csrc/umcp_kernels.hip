@@ -1184,9 +1184,12 @@ __global__ __launch_bounds__(kGatherBlock) void vertex_gather_batch(BatchGatherA
 // faces' corner limit points and volume terms.  (Single-surface and batched
 // evaluations run this same kernel, so contraction choices cannot differ
 // between them.)
-template <int NCP, int NPT, int KIND, int WS>
+// MIXED: the area's per-point arithmetic in FP32 on control points relative
+// to the face's first one (as the mixed face kernel), summed in FP64; the
+// volume stays FP64 on absolute positions.
+template <int NCP, int NPT, int KIND, int WS, bool MIXED>
 __device__ __forceinline__ void restraint_group(const FaceArgs &a, const double *__restrict__ r, int group,
-                                                double *lds, double acc[4])
+                                                double *lds, float *lds32, double acc[4])
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -1206,7 +1209,36 @@ __device__ __forceinline__ void restraint_group(const FaceArgs &a, const double 
     __syncthreads();
 #define PX(c, k) lds[((c) * 3 + (k)) * kWave + lane]
     double A = 0.0, V = 0.0;
-    if (a.do_area) {
+    if (MIXED && a.do_area) {
+        for (int c = w; c < NCP; c += WS)
+#pragma unroll
+            for (int k = 0; k < 3; k++) lds32[(c * 3 + k) * kWave + lane] = (float)(PX(c, k) - PX(0, k));
+        __syncthreads();
+        const float *tab = kind_table<KIND, float>();
+        const float alf[3] = {(float)al[0], (float)al[1], (float)al[2]};
+        const float dudv = (float)a.dudv;
+        for (int p = w; p < NPT; p += WS) {
+            const float *T = tab + p * 5 * NCP;
+            float Ru[3] = {0.f, 0.f, 0.f}, Rv[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int c = 0; c < NCP; c++)
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const float x = lds32[(c * 3 + k) * kWave + lane];
+                    Ru[k] += T[c * 5 + 0] * x;
+                    Rv[k] += T[c * 5 + 1] * x;
+                }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                Ru[k] *= alf[k];
+                Rv[k] *= alf[k];
+            }
+            const float E11 = Ru[0] * Ru[0] + Ru[1] * Ru[1] + Ru[2] * Ru[2];
+            const float E12 = Ru[0] * Rv[0] + Ru[1] * Rv[1] + Ru[2] * Rv[2];
+            const float E22 = Rv[0] * Rv[0] + Rv[1] * Rv[1] + Rv[2] * Rv[2];
+            A += (double)(dudv * __builtin_amdgcn_sqrtf(E11 * E22 - E12 * E12) * tab[NPT * 5 * NCP + p]);
+        }
+    } else if (a.do_area) {
         const double *tab = kind_table<KIND, double>();
         for (int p = w; p < NPT; p += WS) {
             const double *T = tab + p * 5 * NCP;
@@ -1255,10 +1287,11 @@ __device__ __forceinline__ void restraint_group(const FaceArgs &a, const double 
 
 constexpr int kPrepassWaves = 4;   // waves per 64-face group (about four Gauss points each)
 
-template <int WS>
+template <int WS, bool MIXED>
 __global__ __launch_bounds__(kWave *WS) void restraint_prepass_kernel(BatchFaceArgs B)
 {
     __shared__ double lds[13 * 3 * kWave];
+    __shared__ float lds32[MIXED ? 13 * 3 * kWave : 1];
     const int it = xcd_logical_block(blockIdx.x, gridDim.x);
     if (it >= B.nitems) return;
     const FaceItem item = B.items[it];
@@ -1267,9 +1300,9 @@ __global__ __launch_bounds__(kWave *WS) void restraint_prepass_kernel(BatchFaceA
     const double *r = B.r ? B.r : a.r;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     switch (item.kind) {
-    case 0: restraint_group<12, 15, 0, WS>(a, r, item.group, lds, acc); break;
-    case 1: restraint_group<11, 16, 1, WS>(a, r, item.group, lds, acc); break;
-    default: restraint_group<13, 16, 2, WS>(a, r, item.group, lds, acc); break;
+    case 0: restraint_group<12, 15, 0, WS, MIXED>(a, r, item.group, lds, lds32, acc); break;
+    case 1: restraint_group<11, 16, 1, WS, MIXED>(a, r, item.group, lds, lds32, acc); break;
+    default: restraint_group<13, 16, 2, WS, MIXED>(a, r, item.group, lds, lds32, acc); break;
     }
     block_sum4<kWave * WS>(acc);
     if (threadIdx.x == 0) {
@@ -1501,10 +1534,15 @@ hipError_t launch_gather_batch(const BatchGatherArgs &a, int grid, bool mixed, h
     return hipGetLastError();
 }
 
-hipError_t launch_restraint_prepass(const BatchFaceArgs &a, const int *first, int nsurf, hipStream_t st)
+hipError_t launch_restraint_prepass(const BatchFaceArgs &a, const int *first, int nsurf, bool mixed, hipStream_t st)
 {
     if (a.nitems <= 0 || nsurf <= 0) return hipSuccess;
-    hipLaunchKernelGGL(restraint_prepass_kernel<kPrepassWaves>, dim3(a.nitems), dim3(kWave * kPrepassWaves), 0, st, a);
+    if (mixed)
+        hipLaunchKernelGGL((restraint_prepass_kernel<kPrepassWaves, true>), dim3(a.nitems), dim3(kWave * kPrepassWaves),
+                           0, st, a);
+    else
+        hipLaunchKernelGGL((restraint_prepass_kernel<kPrepassWaves, false>), dim3(a.nitems),
+                           dim3(kWave * kPrepassWaves), 0, st, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(restraint_finalize_kernel, dim3(nsurf), dim3(kGatherBlock), 0, st, a.fa, first);

@@ -154,7 +154,7 @@ int gather_blocks(int nv);
 // restraint pre-pass over the same items as the face kernel (per-group area
 // and volume), then one finalize block per surface: first[i] is the FaceArgs
 // entry (in a.fa) of the i-th surface with restraints
-hipError_t launch_restraint_prepass(const BatchFaceArgs &a, const int *first, int nsurf, hipStream_t st);
+hipError_t launch_restraint_prepass(const BatchFaceArgs &a, const int *first, int nsurf, bool mixed, hipStream_t st);
 
 // face kernel over a work-item table (one surface or many); grid = 0: one
 // workgroup per item, else a persistent grid of that many workgroups striding
