@@ -999,7 +999,7 @@ __device__ __forceinline__ void gather_reduce(const GatherArgs &a)
     }
 }
 
-// kGatherLanes lanes per vertex: lane q sums entries s0 + q, s0 + q + 8, ...
+// kGatherLanes lanes per vertex: lane q sums entries s0 + q, s0 + q + kGatherLanes, ...
 // of the vertex's CSR range (neighbouring lanes read neighbouring entries),
 // then a fixed xor butterfly combines the lanes (IEEE addition commutes, so
 // every lane of the group ends with the same sum).
