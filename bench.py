@@ -110,6 +110,20 @@ def cpu_baseline(w, c0, budget_s: float):
     base = {"value": n * o.nfaces / dt, "unit": "face-evals/s", "cores": 1, "kind": "port",
             "sample": "%d energy+gradient evaluations of the 40962-vertex vesicle (%d faces), "
                       "serial oracle, %.1f s" % (n, o.nfaces, dt)}
+    # the same evaluations with the faces split over the host's threads
+    # (OpenMP, per-thread gradient buffers; at most 16, the host share of one GPU)
+    nt = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))))
+    t0 = time.perf_counter()
+    m = 0
+    while True:
+        o.energy_grad_threads(w.r, nt)
+        m += 1
+        if time.perf_counter() - t0 > budget_s / 2:
+            break
+    dt2 = time.perf_counter() - t0
+    base["threads"] = {"value": m * o.nfaces / dt2, "unit": "face-evals/s", "cores": nt, "kind": "port",
+                       "sample": "%d energy+gradient evaluations, faces over %d OpenMP threads, %.1f s"
+                                 % (m, nt, dt2)}
     return base, E, g_ref
 
 

@@ -203,6 +203,12 @@ int umcp_effm_steps_dev(umcp_surface *s, double *d_r, double *d_pp, const double
 int umcp_save_mesh(umcp_surface *s, const double *r, const char *path);
 int umcp_write_restart(const char *path, int nv, const double *r, const double *pp, int seed);
 int umcp_read_restart(const char *path, int nv, double *r, double *pp, double *alpha, int *has_pp, int *seed);
+/* trajectory frame (surface::writeLimitingSurface, src/write_structures.C:
+ * 662-723): XYZ, "C x y z" for the limit positions at the corners (0,0),
+ * (0,1), (1,0) of every face in the reference's face order (the reference's
+ * plim = 1), scaled by alpha; append != 0 adds a frame to the file.  Needs
+ * umcp_prepare. */
+int umcp_write_limit_surface(umcp_surface *s, const double *r, const char *path, int append);
 
 /* Brownian steps drawing their noise on the device: xi of step k is the
  * counter-based stream of (seed, k) that umcp_noise writes (Philox4x64-10

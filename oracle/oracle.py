@@ -73,6 +73,8 @@ def lib():
         L.orc_volume_grad.argtypes = [vp, dp, dp]
         L.orc_restraint_energy.argtypes = [vp, dp, dp, dp]
         L.orc_restraint_energy.restype = C.c_double
+        L.orc_energy_grad_threads.argtypes = [vp, dp, dp, C.c_int]
+        L.orc_energy_grad_threads.restype = C.c_double
         L.orc_effm_points_per_face.restype = C.c_int
         L.orc_effm_point_mass.argtypes = [vp]
         L.orc_effm_point_mass.restype = C.c_double
@@ -194,6 +196,13 @@ class OracleSurface:
         lib().orc_bd_steps(self._h, _p(r), _p(xi), n, mobility_dt, noise_amp,
                            _p(en) if en is not None else None)
         return (r, en) if want_energies else r
+
+    def energy_grad_threads(self, r: np.ndarray, nthreads: int):
+        """(E, g) with the faces split over nthreads OpenMP threads."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        g = np.zeros_like(r)
+        e = lib().orc_energy_grad_threads(self._h, _p(r), _p(g), int(nthreads))
+        return e, g
 
     # effective-mass integrators (src/hd.C, src/modes.C)
     def effm_point_mass(self) -> float:

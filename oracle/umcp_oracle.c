@@ -31,6 +31,9 @@
 
 #include <float.h>
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1584,6 +1587,51 @@ void orc_grad(omesh *m, const double *r, double *g)
         for (int i = 0; i < n3; i++) g[i] += lamV * dV[i];
         free(dV);
     }
+}
+
+/* Energy and gradient with the faces split over nthreads OpenMP threads,
+ * each with its own gradient buffer, summed in thread order afterwards (the
+ * multi-core CPU comparator; the serial functions above are the reference's
+ * single-threaded path).  Same terms as orc_energy + orc_grad; g is
+ * overwritten. */
+double orc_energy_grad_threads(omesh *m, const double *r, double *g, int nthreads)
+{
+    double lamA = 0.0, lamV = 0.0;
+    const double er = orc_restraint_energy(m, r, &lamA, &lamV);
+    const int n3 = 3 * m->nv;
+    if (nthreads < 1) nthreads = 1;
+    double *gbuf = calloc((size_t)nthreads * (size_t)(n3 + 3), sizeof(double));
+    double e = 0.0;
+#pragma omp parallel num_threads(nthreads) reduction(+ : e)
+    {
+#ifdef _OPENMP
+        const int tid = omp_get_thread_num();
+#else
+        const int tid = 0;
+#endif
+        double *gt = gbuf + (size_t)tid * (size_t)(n3 + 3);
+#pragma omp for schedule(static)
+        for (int t = 0; t < m->nt; t++) {
+            int f = m->tri_face[t];
+            if (f < 0) continue;
+            e += orc_face_energy(m, f, r);
+            int npt;
+            double dudv;
+            oform *F = face_forms(m, f, &npt, &dudv);
+            for (int p = 0; p < npt; p++) point_grad(m, f, F + p, dudv, r, gt, lamA);
+        }
+    }
+    memset(g, 0, sizeof(double) * (size_t)(n3 + 3));
+    for (int k = 0; k < nthreads; k++)
+        for (int i = 0; i < n3 + 3; i++) g[i] += gbuf[(size_t)k * (size_t)(n3 + 3) + i];
+    free(gbuf);
+    if (lamV != 0.0) {
+        double *dV = malloc(sizeof(double) * (size_t)n3);
+        orc_volume_grad(m, r, dV);
+        for (int i = 0; i < n3; i++) g[i] += lamV * dV[i];
+        free(dV);
+    }
+    return e + er;
 }
 
 /* geometric area sum(dudv g w) and reference area sum(dudv g0 w) */

@@ -69,6 +69,10 @@ long   orc_effm_truncate(int n, double *Mi, double rel_tol);
 void   orc_effm_steps(omesh *m, double *r, double *pp, const double *xi, int nsteps, int mode, double dt,
                       double kT, double gamma, const double *Mi, double *energies);
 
+/* energy + gradient with the faces split over nthreads OpenMP threads
+ * (per-thread gradient buffers summed in thread order); g overwritten */
+double orc_energy_grad_threads(omesh *m, const double *r, double *g, int nthreads);
+
 /* introspection for tests */
 int    orc_face_info(const omesh *m, int f, int *owner, int *ncp, int *cp, int *tri);
 int    orc_face_table(const omesh *m, int f, int p, double *w6, double *weight,

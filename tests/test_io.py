@@ -3,6 +3,10 @@
 - .mesh (surface::saveSurface, src/write_structures.C:473-540): a saved mesh
   loads back with the same positions (bitwise, %.17g), valences and
   neighbour rings, and saving it again gives the same bytes;
+- limit-surface frames (surface::writeLimitingSurface): three corner points
+  per face; on a closed mesh every point is the limit position of one of the
+  face's vertices (the oracle's Loop-mask limit points) and each vertex's
+  point appears once per incident face;
 - restart (src/restart.C:7-168): positions and momenta round-trip bitwise;
   the reference's own restart fixture (examples/test/energy_conservation/
   start.save: a 120-vertex planar mesh, positions only, one complex section,
@@ -99,3 +103,30 @@ def test_reference_restart_fixture():
     assert np.array_equal(got["r"][:3], [-215.706610, -44.350899, -5.509793])
     with pytest.raises(S.UmcpError):
         S.read_restart(FIXTURE, 200)   # more rows than the file holds
+
+
+def test_limit_surface_frames(tmp_path):
+    from oracle import oracle as O
+    w = workloads.vesicle_642()
+    h = load(w.mesh_path)
+    assert lib().umcp_prepare(h) == 0
+    p = str(tmp_path / "limit.xyz")
+    rng = np.random.default_rng(9)
+    r = w.r.copy()
+    r[:-3] += 0.2 * rng.standard_normal(r.size - 3)
+    assert lib().umcp_write_limit_surface(h, S._dp(r), p.encode(), 0) == 0
+    assert lib().umcp_write_limit_surface(h, S._dp(r), p.encode(), 1) == 0
+    lines = open(p).read().splitlines()
+    nf = 1280
+    assert lines[0] == str(3 * nf) and lines[1] == "limiting surface"
+    assert len(lines) == 2 * (2 + 3 * nf) and lines[2 + 3 * nf] == str(3 * nf)
+    pts = np.array([[float(x) for x in ln.split()[1:]] for ln in lines[2:2 + 3 * nf]])
+    o = O.OracleSurface(w.mesh_path)
+    L = o.limit_points(r).reshape(-1, 3)
+    d = np.linalg.norm(pts[:, None, :] - L[None, :, :], axis=2)
+    nearest = d.argmin(axis=1)
+    assert d.min(axis=1).max() < 1e-5           # %lf keeps six decimals
+    counts = np.bincount(nearest, minlength=o.nv)
+    val = np.array([len(o.neighbors(v)) for v in range(o.nv)])
+    assert np.array_equal(counts, val)
+    lib().umcp_destroy(h)

@@ -178,3 +178,21 @@ def test_noise_moments():
     assert np.all(np.abs(c - np.eye(3)) < 0.03)
     z2 = noise_ref.normals(99, 6, 20000)
     assert abs(np.corrcoef(z.reshape(-1), z2)[0, 1]) < 0.02
+
+
+def test_threaded_energy_grad_equals_serial():
+    """The multi-core comparator (faces split over OpenMP threads) computes
+    the serial oracle's energy and gradient, restraints included."""
+    w = workloads.vesicle_642()
+    o = O.OracleSurface(w.mesh_path)
+    o.set_params(**w.params)
+    o.setg0(w.r0)
+    o.set_restraints(KA_global=0.1, KV=0.004)
+    rng = np.random.default_rng(8)
+    r = w.r.copy()
+    r[:-3] += 0.3 * rng.standard_normal(r.size - 3)
+    E, g = o.energy(r), o.grad(r)
+    for n in (1, 3, 8):
+        Et, gt = o.energy_grad_threads(r, n)
+        assert abs(Et - E) <= 1e-12 * abs(E)
+        assert np.abs(gt - g).max() <= 1e-12 * np.abs(g).max()

@@ -4,6 +4,7 @@
 //   surface::saveSurface          src/write_structures.C:473-540
 //   Simulation::saveRestart       src/restart.C:7-121
 //   Simulation::loadRestart       src/restart.C:123-168
+//   surface::writeLimitingSurface src/write_structures.C:662-723
 //
 // Layouts are the reference's, line for line; floating-point values are
 // written with %.17g (the reference's %lf keeps six decimals, which does not
@@ -11,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/umcp.h"
@@ -54,6 +56,49 @@ int umcp_save_mesh(umcp_surface *s, const double *r, const char *path)
     for (size_t t = 0; t < tri.size(); t += 3) std::fprintf(fp, "%d %d %d\n", tri[t], tri[t + 1], tri[t + 2]);
     const bool ok = std::fclose(fp) == 0;
     return ok ? UMCP_OK : UMCP_ERR_ARG;
+}
+
+// surface::writeLimitingSurface (src/write_structures.C:662-723, plim = 1 at
+// src/write_structures.C:12): one XYZ frame of the limit positions at the
+// three corners (0,0), (0,1), (1,0) of every face, faces in the reference's
+// order, positions scaled by alpha as the surface evaluation does
+int umcp_write_limit_surface(umcp_surface *s, const double *r, const char *path, int append)
+{
+    if (!s || !path) return UMCP_ERR_ARG;
+    if (!s->prepared) return UMCP_ERR_STATE;
+    const Topology &T = s->T;
+    const Plan &P = s->P;
+    const double *x = r ? r : T.r.data();
+    const double al[3] = {r ? r[3 * T.nv] : 1.0, r ? r[3 * T.nv + 1] : 1.0, r ? r[3 * T.nv + 2] : 1.0};
+    // (class, index) of every face in reference order
+    std::vector<std::pair<int, int>> where((size_t)P.nfaces, {-1, -1});
+    for (size_t c = 0; c < P.classes.size(); c++)
+        for (int i = 0; i < P.classes[c].nf(); i++) where[P.classes[c].face_id[i]] = {(int)c, i};
+    FILE *fp = std::fopen(path, append ? "a" : "w");
+    if (!fp) {
+        s->err = std::string("cannot write ") + path;
+        return UMCP_ERR_ARG;
+    }
+    std::fprintf(fp, "%d\n", 3 * P.nfaces);
+    std::fprintf(fp, "limiting surface\n");
+    const int corner_row[3] = {0, 2, 1};   // lattice order (0,0), (0,1), (1,0); corner rows (0,0), (1,0), (0,1)
+    for (int f = 0; f < P.nfaces; f++) {
+        const FaceClass &C = P.classes[where[f].first];
+        const int i = where[f].second, nf = C.nf();
+        for (int q = 0; q < 3; q++) {
+            const double *w = &C.corner[(size_t)corner_row[q] * C.ncp];
+            double p[3] = {0.0, 0.0, 0.0};
+            for (int c = 0; c < C.ncp; c++) {
+                const int v = C.cp[(size_t)c * nf + i];
+                const int code = C.img[(size_t)c * nf + i];
+                const int ix = code / 9 - 1, iy = (code / 3) % 3 - 1, iz = code % 3 - 1;
+                for (int k = 0; k < 3; k++)
+                    p[k] += w[c] * (x[3 * v + k] + ix * T.pbc[0][k] + iy * T.pbc[1][k] + iz * T.pbc[2][k]);
+            }
+            std::fprintf(fp, "C %lf %lf %lf\n", p[0] * al[0], p[1] * al[1], p[2] * al[2]);
+        }
+    }
+    return std::fclose(fp) == 0 ? UMCP_OK : UMCP_ERR_ARG;
 }
 
 int umcp_write_restart(const char *path, int nv, const double *r, const double *pp, int seed)

@@ -42,7 +42,7 @@ EXPORTS = [
     "umcp_clone", "umcp_device_bytes", "umcp_server_create", "umcp_server_submit", "umcp_server_wait",
     "umcp_server_drain", "umcp_server_stats", "umcp_server_last_error", "umcp_server_destroy",
     "umcp_effm_build", "umcp_effm_get", "umcp_effm_steps", "umcp_effm_steps_dev",
-    "umcp_save_mesh", "umcp_write_restart", "umcp_read_restart",
+    "umcp_save_mesh", "umcp_write_restart", "umcp_read_restart", "umcp_write_limit_surface",
 ]
 
 
@@ -135,6 +135,7 @@ def load_library(path: str = LIB_PATH):
                                   C.c_int),
         "umcp_noise": ([C.c_ulonglong, C.c_int, C.c_int, vp, vp], C.c_int),
         "umcp_save_mesh": ([vp, dp, C.c_char_p], C.c_int),
+        "umcp_write_limit_surface": ([vp, dp, C.c_char_p, C.c_int], C.c_int),
         "umcp_write_restart": ([C.c_char_p, C.c_int, dp, dp, C.c_int], C.c_int),
         "umcp_read_restart": ([C.c_char_p, C.c_int, dp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
         "umcp_effm_build": ([vp, C.POINTER(C.c_long), dp], C.c_int),
@@ -319,6 +320,12 @@ class Surface:
         rr = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
         self._check(self._L.umcp_save_mesh(self._h, _dp(rr) if rr is not None else None, path.encode()),
                     "save_mesh")
+
+    def write_limit_surface(self, path: str, r: np.ndarray | None = None, append: bool = False):
+        """One XYZ frame of the limit surface at the face corners (writeLimitingSurface)."""
+        rr = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
+        self._check(self._L.umcp_write_limit_surface(self._h, _dp(rr) if rr is not None else None, path.encode(),
+                                                     1 if append else 0), "write_limit_surface")
 
     # momentum-space integrators with the effective-mass matrix (src/hd.C)
     def effm_build(self):
