@@ -765,7 +765,8 @@ int umcp_effm_build(umcp_surface *s, long *nnz, double *point_mass)
     const size_t before = s->allocations.size();
     const int nv = s->T.nv;
     if ((rc = dev_alloc(s, &s->d_effm_rowptr, E.rowptr.size())) || (rc = dev_alloc(s, &s->d_effm_col, E.col.size())) ||
-        (rc = dev_alloc(s, &s->d_effm_val, E.val.size())) || (rc = dev_alloc(s, &s->d_pp_next, 3 * (size_t)nv)))
+        (rc = dev_alloc(s, &s->d_effm_val, E.val.size())) || (rc = dev_alloc(s, &s->d_pp_next, 3 * (size_t)nv)) ||
+        (rc = dev_alloc(s, &s->d_mp, 3 * (size_t)nv)))
         return rc;
     s->effm_allocs.assign(s->allocations.begin() + (long)before, s->allocations.end());
     if ((rc = dev_upload(s, s->d_effm_rowptr, E.rowptr.data(), E.rowptr.size())) ||
@@ -810,32 +811,50 @@ int umcp_effm_steps_dev(umcp_surface *s, double *d_r, double *d_pp, const double
         return fail(s, UMCP_ERR_ARG, "effective-mass integrator needs gamma > 0, dt > 0, kT >= 0");
     hipStream_t st = stream ? (hipStream_t)stream : s->stream;
     const size_t n3 = 3 * (size_t)s->T.nv;
+    const bool lang = p->mode == UMCP_EFFM_LD;
+    EffmArgs ea{};
+    ea.nv = s->T.nv;
+    ea.rowptr = s->d_effm_rowptr;
+    ea.col = s->d_effm_col;
+    ea.val = s->d_effm_val;
+    ea.pp_next = s->d_pp_next;
+    ea.pp_keep = d_pp;
+    ea.mp = s->d_mp;
+    ea.r = d_r;
+    ea.mode = lang ? kEffmLangevin : kEffmBrownian;
+    ea.akma = 1.0 / (0.04888 * (1e-12));   // AKMA_TIME, src/units.h
+    ea.dt = p->dt;
+    hipError_t e = hipSuccess;
+    if (lang && nsteps > 0) {
+        // the first step's friction product from the caller's momenta; later
+        // steps reuse the product the position kernel forms
+        e = launch_effm_product(ea, d_pp, st);
+        if (e != hipSuccess) return hip_fail(s, e, "effective-mass product launch");
+    }
     for (int k = 0; k < nsteps; k++) {
         GatherArgs ga{};
-        ga.out_mode = 0;
-        ga.g = s->d_g;
+        ga.out_mode = 3;
+        ga.r = d_r;
         ga.energies = d_energies;
         ga.step = k;
+        if (d_xi) {
+            ga.noise = kNoiseArray;
+            ga.xi = d_xi + (size_t)k * n3;
+        } else {
+            ga.noise = kNoisePhilox;
+            ga.seed = seed;
+            ga.step_ctr = s->d_step;
+        }
+        ga.pp_in = d_pp;
+        ga.pp_out = s->d_pp_next;
+        ga.mp = s->d_mp;
+        ga.effm_mode = ea.mode;
+        ga.effm_akma = ea.akma;
+        ga.effm_dt = p->dt;
+        ga.effm_kT = p->kT;
+        ga.effm_gamma = p->gamma;
         if ((rc = enqueue_eval(s, d_r, st, ga, s->d_step))) return rc;
-        EffmArgs ea{};
-        ea.nv = s->T.nv;
-        ea.rowptr = s->d_effm_rowptr;
-        ea.col = s->d_effm_col;
-        ea.val = s->d_effm_val;
-        ea.g = s->d_g;
-        ea.pp = d_pp;
-        ea.pp_next = s->d_pp_next;
-        ea.pp_keep = d_pp;
-        ea.r = d_r;
-        ea.xi = d_xi ? d_xi + (size_t)k * n3 : nullptr;
-        ea.seed = seed;
-        ea.step_ctr = s->d_step;
-        ea.mode = p->mode == UMCP_EFFM_LD ? kEffmLangevin : kEffmBrownian;
-        ea.akma = 1.0 / (0.04888 * (1e-12));   // AKMA_TIME, src/units.h
-        ea.dt = p->dt;
-        ea.kT = p->kT;
-        ea.gamma = p->gamma;
-        hipError_t e = launch_effm_step(ea, st);
+        e = launch_effm_position(ea, st);
         if (e != hipSuccess) return hip_fail(s, e, "effective-mass step launch");
     }
     return UMCP_OK;

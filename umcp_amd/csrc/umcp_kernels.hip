@@ -984,8 +984,11 @@ __device__ __forceinline__ void gather_reduce(const GatherArgs &a)
         if (a.rs) acc[0] += a.rs[2];
         if (a.E_out) a.E_out[0] = acc[0];
         if (a.energies) {
-            const int step = a.noise == kNoisePhilox ? philox_step(a) : a.step;
-            if (a.noise != kNoisePhilox || (step >= 0 && step < a.energies_cap)) a.energies[step] = acc[0];
+            // Brownian steps with device noise index their energies by the
+            // step counter; everything else by the caller's step
+            const bool by_ctr = a.noise == kNoisePhilox && a.out_mode == 1;
+            const int step = by_ctr ? philox_step(a) : a.step;
+            if (!by_ctr || (step >= 0 && step < a.energies_cap)) a.energies[step] = acc[0];
         }
         if (a.out_mode == 0 && a.g) {
             a.g[3 * a.nv + 0] = acc[1];
@@ -1035,6 +1038,35 @@ __device__ __forceinline__ void gather_vertices(const GatherArgs &a, int vblock)
         a.g[3 * v] = g0;
         a.g[3 * v + 1] = g1;
         a.g[3 * v + 2] = g2;
+    } else if (a.out_mode == 3) {
+        // momenta of an effective-mass step: friction through the inverse
+        // mass (Langevin, OLD_LANGEVIN form), half kick, noise, half kick
+        // (src/hd.C:1546-1557, 1600-1605, 1650-1693)
+        double z[3];
+        if (a.noise == kNoisePhilox) {
+            normal3(a.seed, philox_step(a), v, z);
+        } else {
+            const double *xi = a.xi + 3 * (size_t)v;
+            z[0] = xi[0];
+            z[1] = xi[1];
+            z[2] = xi[2];
+        }
+        const double gv[3] = {g0, g1, g2};
+        const double al[3] = {a.r[3 * a.nv], a.r[3 * a.nv + 1], a.r[3 * a.nv + 2]};
+        const double Delta = a.effm_akma * a.effm_dt;
+        const double sig = sqrt(2 * a.effm_gamma * a.effm_kT * a.effm_akma * a.effm_dt);
+        const double scale = -a.effm_akma * a.effm_dt;
+        const bool lang = a.effm_mode == kEffmLangevin;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double next = lang ? a.pp_in[3 * (size_t)v + c] : 0.0;
+            if (lang) next += a.mp[3 * (size_t)v + c] * (a.effm_gamma * scale / (al[c] * al[c]));
+            next += -gv[c] * a.effm_akma * a.effm_dt / 2;
+            if (lang) next += sig * z[c];
+            else next += (1.0 / a.effm_gamma) * (sig * z[c]) / Delta;
+            next += -gv[c] * a.effm_akma * a.effm_dt / 2;
+            a.pp_out[3 * (size_t)v + c] = next;
+        }
     } else {
         double z[3];
         if (a.noise == kNoisePhilox) {
@@ -1084,69 +1116,55 @@ __device__ __forceinline__ void effm_row(const EffmArgs &a, int v, const double 
     out[2] = s2;
 }
 
-// momenta of the step: friction through the inverse mass (Langevin), half
-// kick, noise, half kick (src/hd.C:1546-1557, 1600-1605, 1650-1693)
-__global__ __launch_bounds__(kEffmBlock) void effm_momentum_kernel(EffmArgs a)
+// positions: r += (Mi pp) / alpha^2 Delta (src/hd.C:1699-1720); the momenta
+// carried to the next step (Langevin) or cleared (Brownian, src/hd.C:1943-1947);
+// the product Mi pp kept for the next step's friction (the reference forms
+// the same product again from the same momenta, src/hd.C:1553)
+__global__ __launch_bounds__(kEffmBlock) void effm_position_kernel(EffmArgs a)
 {
     #pragma clang fp contract(off)
     const int v = blockIdx.x * (kEffmBlock / kWave) + (int)threadIdx.x / kWave;
     if (v >= a.nv) return;   // whole waves leave together
     const int lane = threadIdx.x & (kWave - 1);
-    const double al[3] = {a.r[3 * a.nv], a.r[3 * a.nv + 1], a.r[3 * a.nv + 2]};
-    double fr[3] = {0.0, 0.0, 0.0};
-    if (a.mode == kEffmLangevin) effm_row(a, v, a.pp, fr);
+    double q[3];
+    effm_row(a, v, a.pp_next, q);
     if (lane != 0) return;
-    double z[3];
-    if (a.xi) {
-        z[0] = a.xi[3 * (size_t)v];
-        z[1] = a.xi[3 * (size_t)v + 1];
-        z[2] = a.xi[3 * (size_t)v + 2];
-    } else {
-        normal3(a.seed, a.step_ctr[0] - 1, v, z);
-    }
-    const double Delta = a.akma * a.dt;
-    const double sig = sqrt(2 * a.gamma * a.kT * a.akma * a.dt);
-    const double scale = -a.akma * a.dt;
+    const double al[3] = {a.r[3 * a.nv], a.r[3 * a.nv + 1], a.r[3 * a.nv + 2]};
+    const bool lang = a.mode == kEffmLangevin;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        const double g = a.g[3 * (size_t)v + c];
-        double next = a.mode == kEffmLangevin ? a.pp[3 * (size_t)v + c] : 0.0;
-        if (a.mode == kEffmLangevin) next += fr[c] * (a.gamma * scale / (al[c] * al[c]));
-        next += -g * a.akma * a.dt / 2;
-        if (a.mode == kEffmLangevin) next += sig * z[c];
-        else next += (1.0 / a.gamma) * (sig * z[c]) / Delta;
-        next += -g * a.akma * a.dt / 2;
-        a.pp_next[3 * (size_t)v + c] = next;
+        a.r[3 * (size_t)v + c] += q[c] * (1.0 / (al[c] * al[c])) * a.akma * a.dt;
+        a.pp_keep[3 * (size_t)v + c] = lang ? a.pp_next[3 * (size_t)v + c] : 0.0;
+        if (lang) a.mp[3 * (size_t)v + c] = q[c];
     }
 }
 
-// positions: r += (Mi pp) / alpha^2 Delta (src/hd.C:1699-1720); the momenta
-// carried to the next step (Langevin) or cleared (Brownian, src/hd.C:1943-1947)
-__global__ __launch_bounds__(kEffmBlock) void effm_position_kernel(EffmArgs a)
+__global__ __launch_bounds__(kEffmBlock) void effm_product_kernel(EffmArgs a, const double *pp)
 {
     #pragma clang fp contract(off)
     const int v = blockIdx.x * (kEffmBlock / kWave) + (int)threadIdx.x / kWave;
     if (v >= a.nv) return;
     const int lane = threadIdx.x & (kWave - 1);
     double q[3];
-    effm_row(a, v, a.pp_next, q);
+    effm_row(a, v, pp, q);
     if (lane != 0) return;
-    const double al[3] = {a.r[3 * a.nv], a.r[3 * a.nv + 1], a.r[3 * a.nv + 2]};
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        a.r[3 * (size_t)v + c] += q[c] * (1.0 / (al[c] * al[c])) * a.akma * a.dt;
-        a.pp_keep[3 * (size_t)v + c] = a.mode == kEffmLangevin ? a.pp_next[3 * (size_t)v + c] : 0.0;
-    }
+    for (int c = 0; c < 3; c++) a.mp[3 * (size_t)v + c] = q[c];
 }
 
-hipError_t launch_effm_step(const EffmArgs &a, hipStream_t st)
+hipError_t launch_effm_position(const EffmArgs &a, hipStream_t st)
 {
     const int nb = (a.nv + kEffmBlock / kWave - 1) / (kEffmBlock / kWave);
     if (nb == 0) return hipSuccess;
-    hipLaunchKernelGGL(effm_momentum_kernel, dim3(nb), dim3(kEffmBlock), 0, st, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(effm_position_kernel, dim3(nb), dim3(kEffmBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_effm_product(const EffmArgs &a, const double *pp, hipStream_t st)
+{
+    const int nb = (a.nv + kEffmBlock / kWave - 1) / (kEffmBlock / kWave);
+    if (nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(effm_product_kernel, dim3(nb), dim3(kEffmBlock), 0, st, a, pp);
     return hipGetLastError();
 }
 

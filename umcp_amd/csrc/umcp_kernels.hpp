@@ -71,6 +71,7 @@ struct BatchFaceArgs {
 // Counter-based normal deviates: Philox4x64-10 keyed by (seed, 'UMCP') at
 // counter (vertex, step, 0, 0), Box-Muller on the first four outputs.
 constexpr int kNoiseArray = 0, kNoisePhilox = 1;
+constexpr int kEffmBrownian = 0, kEffmLangevin = 1;
 
 struct GatherArgs {
     int nv;
@@ -79,7 +80,7 @@ struct GatherArgs {
     const float *gbuf32;    // FP32 contributions (mixed path) when non-null
     const double *partial;
     int npartial;
-    int out_mode;           // 0 gradient, 1 Brownian update, 2 reductions only
+    int out_mode;           // 0 gradient, 1 Brownian update, 2 reductions only, 3 effective-mass momenta
     double *g;              // out_mode 0
     double *r;              // out_mode 1
     const double *xi;       // out_mode 1, 3 nv standard normals for this step
@@ -95,29 +96,36 @@ struct GatherArgs {
     unsigned long long seed;
     const int *step_ctr;    // Philox: the step is *step_ctr - 1 (advanced by the face kernel)
     int energies_cap;       // energies[step] is written while step < energies_cap (Philox)
+    // out_mode 3: momenta of an effective-mass step (src/hd.C:1546-1693):
+    // pp_out = pp_in [+ friction from mp = Mi pp_in] - g Delta/2 + noise - g Delta/2
+    const double *pp_in;
+    double *pp_out;
+    const double *mp;
+    int effm_mode;          // kEffmBrownian or kEffmLangevin
+    double effm_akma, effm_dt, effm_kT, effm_gamma;
 };
 
-// One step of the momentum-space integrators (src/hd.C) after the gradient
-// of the step is in g: momenta (friction, kicks, noise), then positions.
+// One step of the momentum-space integrators (src/hd.C): the gather
+// (out_mode 3) forms the step's momenta from the gradient, then the position
+// kernel applies r += (Mi pp) / alpha^2 Delta and keeps Mi pp, the next
+// step's friction product.
 constexpr int kEffmBlock = 256;
-constexpr int kEffmBrownian = 0, kEffmLangevin = 1;
 struct EffmArgs {
     int nv;
     const int *rowptr;      // nv + 1, CSR of the inverse effective mass
     const int *col;
     const double *val;
-    const double *g;        // 3 nv + 3 gradient of this step
-    const double *pp;       // momenta in (Langevin)
-    double *pp_next;        // momenta of this step (scratch, 3 nv)
+    const double *pp_next;  // momenta of this step (from the gather)
     double *pp_keep;        // momenta carried to the next step (the caller's array)
+    double *mp;             // out: Mi pp_keep (Langevin), the next step's friction product
     double *r;              // 3 nv + 3 positions, alpha last
-    const double *xi;       // 3 nv standard normals of this step, or null: Philox (seed, *step_ctr - 1)
-    unsigned long long seed;
-    const int *step_ctr;
     int mode;               // kEffmBrownian or kEffmLangevin
-    double akma, dt, kT, gamma;
+    double akma, dt;
 };
-hipError_t launch_effm_step(const EffmArgs &a, hipStream_t st);
+// r += (Mi pp_next) / alpha^2 Delta, pp_keep and mp (one wave per row)
+hipError_t launch_effm_position(const EffmArgs &a, hipStream_t st);
+// mp = Mi pp (the first step's friction product)
+hipError_t launch_effm_product(const EffmArgs &a, const double *pp, hipStream_t st);
 
 struct GatherItem {
     int surf, vblock;

@@ -79,7 +79,7 @@ struct umcp_surface {
     bool effm_ready = false;
     double effm_point_mass = 0.0;
     int *d_effm_rowptr = nullptr, *d_effm_col = nullptr;
-    double *d_effm_val = nullptr, *d_pp_next = nullptr;
+    double *d_effm_val = nullptr, *d_pp_next = nullptr, *d_mp = nullptr;
     long effm_nnz = 0;
     std::vector<void *> effm_allocs;
 };
