@@ -160,9 +160,10 @@ int umcp_volume(umcp_surface *s, const double *r, double *V);
  * (Cartesian momenta, one per vertex coordinate).
  * umcp_effm_build (getSparseEffectiveMass, src/modes.C:1719-1960, over the
  * 66-point lattice of every face and the point mass of src/hd.C:418-481;
- * after umcp_setg0, which fixes the reference area): dense mass matrix and
- * Cholesky inverse on the host (O(nv^3); nv <= UMCP_EFFM_MAX_NV), truncated
- * as the reference does (rel_tol 1e-20), kept on the device as CSR.
+ * after umcp_setg0, which fixes the reference area): the mass matrix (host,
+ * sparse), its Cholesky inverse on the device (rocSOLVER potrf + potri, the
+ * reference's LAPACK pair), truncated as the reference does (rel_tol
+ * 1e-20) and kept dense in HBM (nv x nv doubles; nv <= UMCP_EFFM_MAX_NV).
  * umcp_effm_steps(_dev): per step g = grad(r); UMCP_EFFM_BD
  * (do_bd_membrane): pp = -g Delta + gamma^-1 N(0, s) / Delta, momenta cleared
  * after the step; UMCP_EFFM_LD (do_ld, OLD_LANGEVIN friction): pp += -Delta
@@ -174,7 +175,7 @@ int umcp_volume(umcp_surface *s, const double *r, double *V);
  * uses. */
 #define UMCP_EFFM_BD 0
 #define UMCP_EFFM_LD 1
-#define UMCP_EFFM_MAX_NV 4096
+#define UMCP_EFFM_MAX_NV 65536
 typedef struct {
     int mode;              /* UMCP_EFFM_BD or UMCP_EFFM_LD */
     double dt;             /* time step, s */
@@ -182,7 +183,8 @@ typedef struct {
     double gamma;          /* gamma_langevin, the reference's units */
 } umcp_effm_params;
 int umcp_effm_build(umcp_surface *s, long *nnz, double *point_mass);
-/* the (truncated) inverse effective mass as a dense nv x nv array */
+/* the (truncated) inverse effective mass as a dense nv x nv array; *nnz of
+ * umcp_effm_build counts the entries the truncation keeps */
 int umcp_effm_get(umcp_surface *s, double *Minv);
 /* host arrays: r (3nv+3) and pp (3nv) in/out, xi [nsteps][3nv], energies
  * (energy at the start of each step) may be NULL */

@@ -51,7 +51,10 @@ def build(verbose: bool = False) -> str:
         subprocess.run(cmd, check=True)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), "-shared", "-fPIC", "-pthread", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    # rocSOLVER (potrf + potri of the effective-mass set-up) and rocBLAS
+    libs = ["-L" + os.path.join(ROCM, "lib"), "-lrocsolver", "-lrocblas",
+            "-Wl,-rpath," + os.path.join(ROCM, "lib")]
+    cmd = [hipcc(), "-shared", "-fPIC", "-pthread", "--offload-arch=" + ARCH, "-o", tmp] + objs + libs
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -62,16 +62,11 @@ int parse_mesh(const char *path, Topology &T, std::string &err);
 int orient_mesh(Topology &T, std::string &err);
 int build_plan(const Topology &T, Plan &P, std::string &err);
 
-// effective-mass matrix of the momentum-space integrators (effm_host.cpp)
-struct EffmCsr {
-    std::vector<int> rowptr, col;   // truncated inverse, rows ascending, columns ascending
-    std::vector<double> val;
-    std::vector<double> dense;      // the full inverse, nv x nv
-};
+// effective-mass matrix of the momentum-space integrators (effm_host.cpp):
+// M as CSR, rows ascending, columns ascending
 int effm_points_per_face();
-int effm_mass_matrix(const Topology &T, const Plan &P, double mass_per_pt, std::vector<double> &M, std::string &err);
-int effm_inverse_csr(const Topology &T, const Plan &P, double mass_per_pt, double rel_tol, EffmCsr &out,
-                     std::string &err);
+int effm_mass_csr(const Topology &T, const Plan &P, double mass_per_pt, std::vector<int> &rowptr,
+                  std::vector<int> &col, std::vector<double> &val, std::string &err);
 
 // 12 quartic box-spline basis functions and their first/second derivatives at
 // (u, v), in the control-point slot order of the face plan: out[d * 12 + y].

@@ -22,6 +22,9 @@
 #include "umcp_kernels.hpp"
 #include "umcp_state.hpp"
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 using namespace umcp;
 
 namespace {
@@ -742,9 +745,10 @@ int umcp_effm_build(umcp_surface *s, long *nnz, double *point_mass)
 {
     int rc = check_ready(s);
     if (rc) return rc;
-    if (s->T.nv > UMCP_EFFM_MAX_NV)
-        return fail(s, UMCP_ERR_ARG, "effective-mass integrators are set up for at most UMCP_EFFM_MAX_NV vertices "
-                                     "(dense O(nv^3) inverse, as the reference)");
+    const int n = s->T.nv;
+    if (n > UMCP_EFFM_MAX_NV)
+        return fail(s, UMCP_ERR_ARG, "effective-mass integrators hold a dense nv x nv inverse: at most "
+                                     "UMCP_EFFM_MAX_NV vertices");
     // point mass (src/hd.C:449-455): two leaflets of POPC (MSCALE 5) over the
     // reference area, spread over the lattice points of all faces
     const double kAmuPerKg = 1.0 / 1.661e-27;
@@ -752,9 +756,10 @@ int umcp_effm_build(umcp_surface *s, long *nnz, double *point_mass)
     const double area_per_lipid = 65.35;
     const double m = kAmuPerKg * 2 * s->A0_setg0 / s->P.nfaces / effm_points_per_face() * mass_per_lipid /
                      area_per_lipid;
-    EffmCsr E;
+    std::vector<int> rowptr, col;
+    std::vector<double> val;
     std::string err;
-    if (effm_inverse_csr(s->T, s->P, m, 1e-20, E, err)) return fail(s, UMCP_ERR_BAD_MESH, err);
+    if (effm_mass_csr(s->T, s->P, m, rowptr, col, val, err)) return fail(s, UMCP_ERR_BAD_MESH, err);
     // replace an earlier build
     for (void *p : s->effm_allocs) {
         (void)hipFree(p);
@@ -763,20 +768,80 @@ int umcp_effm_build(umcp_surface *s, long *nnz, double *point_mass)
     s->effm_allocs.clear();
     s->effm_ready = false;
     const size_t before = s->allocations.size();
-    const int nv = s->T.nv;
-    if ((rc = dev_alloc(s, &s->d_effm_rowptr, E.rowptr.size())) || (rc = dev_alloc(s, &s->d_effm_col, E.col.size())) ||
-        (rc = dev_alloc(s, &s->d_effm_val, E.val.size())) || (rc = dev_alloc(s, &s->d_pp_next, 3 * (size_t)nv)) ||
-        (rc = dev_alloc(s, &s->d_mp, 3 * (size_t)nv)))
+    if ((rc = dev_alloc(s, &s->d_effm, (size_t)n * n)) || (rc = dev_alloc(s, &s->d_pp_next, 3 * (size_t)n)) ||
+        (rc = dev_alloc(s, &s->d_mp, 3 * (size_t)n)))
         return rc;
     s->effm_allocs.assign(s->allocations.begin() + (long)before, s->allocations.end());
-    if ((rc = dev_upload(s, s->d_effm_rowptr, E.rowptr.data(), E.rowptr.size())) ||
-        (rc = dev_upload(s, s->d_effm_col, E.col.data(), E.col.size())) ||
-        (rc = dev_upload(s, s->d_effm_val, E.val.data(), E.val.size())) || (rc = sync(s)))
-        return rc;
-    s->effm_nnz = (long)E.val.size();
+    // set-up scratch, freed below
+    int *d_rowptr = nullptr, *d_col = nullptr, *d_info = nullptr, *d_cnt = nullptr;
+    double *d_val = nullptr;
+    hipError_t e = hipMalloc(&d_rowptr, sizeof(int) * rowptr.size());
+    if (e == hipSuccess) e = hipMalloc(&d_col, sizeof(int) * col.size());
+    if (e == hipSuccess) e = hipMalloc(&d_val, sizeof(double) * val.size());
+    if (e == hipSuccess) e = hipMalloc(&d_info, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&d_cnt, sizeof(int) * (size_t)n);
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(s->stream);
+        (void)hipFree(d_rowptr);
+        (void)hipFree(d_col);
+        (void)hipFree(d_val);
+        (void)hipFree(d_info);
+        (void)hipFree(d_cnt);
+    };
+    if (e != hipSuccess) {
+        cleanup();
+        return fail(s, UMCP_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    e = hipMemcpyAsync(d_rowptr, rowptr.data(), sizeof(int) * rowptr.size(), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_col, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_val, val.data(), sizeof(double) * val.size(), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(s->d_effm, 0, sizeof(double) * (size_t)n * n, s->stream);
+    if (e == hipSuccess) e = launch_effm_scatter(n, d_rowptr, d_col, d_val, s->d_effm, s->stream);
+    if (e != hipSuccess) {
+        cleanup();
+        return hip_fail(s, e, "effective mass upload");
+    }
+    // inverse by Cholesky on the device: rocSOLVER's potrf + potri, the
+    // LAPACK pair the reference calls (src/modes.C:1827-1847)
+    rocblas_handle h = nullptr;
+    int info_f = -1, info_i = -1;
+    rocblas_status bs = rocblas_create_handle(&h);
+    if (bs == rocblas_status_success) bs = rocblas_set_stream(h, s->stream);
+    if (bs == rocblas_status_success) bs = rocsolver_dpotrf(h, rocblas_fill_upper, n, s->d_effm, n, d_info);
+    if (bs == rocblas_status_success)
+        e = hipMemcpyAsync(&info_f, d_info, sizeof(int), hipMemcpyDeviceToHost, s->stream);
+    if (bs == rocblas_status_success && e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (bs == rocblas_status_success && e == hipSuccess && info_f == 0)
+        bs = rocsolver_dpotri(h, rocblas_fill_upper, n, s->d_effm, n, d_info);
+    if (bs == rocblas_status_success && e == hipSuccess && info_f == 0) {
+        e = hipMemcpyAsync(&info_i, d_info, sizeof(int), hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    }
+    if (h) (void)rocblas_destroy_handle(h);
+    if (bs != rocblas_status_success || e != hipSuccess) {
+        cleanup();
+        return fail(s, UMCP_ERR_DEVICE, std::string("effective mass inverse: ") +
+                                            (e != hipSuccess ? hipGetErrorString(e) : "rocSOLVER call failed"));
+    }
+    if (info_f != 0 || info_i != 0) {
+        cleanup();
+        return fail(s, UMCP_ERR_BAD_MESH, "effective mass matrix is not positive definite (pivot " +
+                                              std::to_string(info_f ? info_f : info_i) + ")");
+    }
+    std::vector<int> cnt((size_t)n);
+    e = launch_effm_symmetrize(n, s->d_effm, s->stream);
+    if (e == hipSuccess) e = launch_effm_truncate(n, s->d_effm, 1e-20, d_cnt, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), d_cnt, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, s->stream);
+    cleanup();
+    if (e != hipSuccess) return hip_fail(s, e, "effective mass truncation");
+    long kept = 0;
+    for (int c : cnt) kept += c;
+    s->effm_nnz = kept;
     s->effm_point_mass = m;
     s->effm_ready = true;
-    if (nnz) *nnz = s->effm_nnz;
+    if (nnz) *nnz = kept;
     if (point_mass) *point_mass = m;
     return UMCP_OK;
 }
@@ -785,17 +850,9 @@ int umcp_effm_get(umcp_surface *s, double *Minv)
 {
     if (!s || !Minv) return UMCP_ERR_ARG;
     if (!s->effm_ready) return fail(s, UMCP_ERR_STATE, "effective mass not built (call umcp_effm_build)");
-    // the truncated matrix: the dense inverse with dropped entries zeroed
-    const int nv = s->T.nv;
-    std::memset(Minv, 0, sizeof(double) * (size_t)nv * nv);
-    std::vector<int> rowptr(nv + 1), col((size_t)s->effm_nnz);
-    std::vector<double> val((size_t)s->effm_nnz);
-    hipError_t e = hipMemcpy(rowptr.data(), s->d_effm_rowptr, sizeof(int) * rowptr.size(), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(col.data(), s->d_effm_col, sizeof(int) * col.size(), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(val.data(), s->d_effm_val, sizeof(double) * val.size(), hipMemcpyDeviceToHost);
+    const size_t n = (size_t)s->T.nv;
+    hipError_t e = hipMemcpy(Minv, s->d_effm, sizeof(double) * n * n, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(s, e, "download effective mass");
-    for (int i = 0; i < nv; i++)
-        for (int k = rowptr[i]; k < rowptr[i + 1]; k++) Minv[(size_t)i * nv + col[k]] = val[k];
     return UMCP_OK;
 }
 
@@ -814,9 +871,7 @@ int umcp_effm_steps_dev(umcp_surface *s, double *d_r, double *d_pp, const double
     const bool lang = p->mode == UMCP_EFFM_LD;
     EffmArgs ea{};
     ea.nv = s->T.nv;
-    ea.rowptr = s->d_effm_rowptr;
-    ea.col = s->d_effm_col;
-    ea.val = s->d_effm_val;
+    ea.Mi = s->d_effm;
     ea.pp_next = s->d_pp_next;
     ea.pp_keep = d_pp;
     ea.mp = s->d_mp;

@@ -1087,20 +1087,21 @@ __device__ __forceinline__ void gather_vertices(const GatherArgs &a, int vblock)
 // ---------------------------------------------------------------------------
 // Momentum-space integrators of src/hd.C with the inverse effective mass
 // (src/hd.C:1543-1720, 1943-1947; src/parallel.C:1028-1075).  The inverse is
-// a CSR matrix over vertices applied to each Cartesian component; one wave
-// per row, lanes striding over the row's entries and a fixed xor butterfly,
-// so the products are deterministic.  Operation order follows the
-// reference's expressions (contraction off).
+// a dense nv x nv array in HBM (the reference's truncation keeps nearly every
+// entry: the inverse of a mesh mass matrix decays only slowly with distance),
+// applied to each Cartesian component; one wave per row, lanes striding over
+// the row (coalesced) and a fixed xor butterfly, so the products are
+// deterministic.  Operation order follows the reference's expressions
+// (contraction off).
 
 __device__ __forceinline__ void effm_row(const EffmArgs &a, int v, const double *x, double out[3])
 {
     #pragma clang fp contract(off)
     const int lane = threadIdx.x & (kWave - 1);
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    const int e0 = a.rowptr[v], e1 = a.rowptr[v + 1];
-    for (int e = e0 + lane; e < e1; e += kWave) {
-        const double m = a.val[e];
-        const int u = a.col[e];
+    const double *row = a.Mi + (size_t)v * a.nv;
+    for (int u = lane; u < a.nv; u += kWave) {
+        const double m = row[u];
         s0 += m * x[3 * (size_t)u];
         s1 += m * x[3 * (size_t)u + 1];
         s2 += m * x[3 * (size_t)u + 2];
@@ -1114,6 +1115,64 @@ __device__ __forceinline__ void effm_row(const EffmArgs &a, int v, const double 
     out[0] = s0;
     out[1] = s1;
     out[2] = s2;
+}
+
+// set-up of the dense inverse (one block per row)
+__global__ __launch_bounds__(kEffmBlock) void effm_scatter_kernel(int n, const int *rowptr, const int *col,
+                                                                  const double *val, double *A)
+{
+    const int i = blockIdx.x;
+    for (int k = rowptr[i] + (int)threadIdx.x; k < rowptr[i + 1]; k += kEffmBlock) A[(size_t)i * n + col[k]] = val[k];
+}
+
+// potri leaves the inverse in the upper triangle of the column-major array,
+// i.e. at [j][i] with j >= i in row-major terms; copy it to [i][j]
+__global__ __launch_bounds__(kEffmBlock) void effm_symmetrize_kernel(int n, double *A)
+{
+    const int i = blockIdx.x;
+    for (int j = i + 1 + (int)threadIdx.x; j < n; j += kEffmBlock) A[(size_t)i * n + j] = A[(size_t)j * n + i];
+}
+
+// |Mi_ij| / sqrt(|Mi_ii Mi_jj|) <= rel_tol -> 0 (src/modes.C:1889-1905); the
+// diagonal always stays, so reading it while other rows are written is safe
+__global__ __launch_bounds__(kEffmBlock) void effm_truncate_kernel(int n, double *A, double rel_tol, int *rowcount)
+{
+    __shared__ int cnt[kEffmBlock];
+    const int i = blockIdx.x;
+    const double di = A[(size_t)i * n + i];
+    int kept = 0;
+    for (int j = (int)threadIdx.x; j < n; j += kEffmBlock) {
+        double *x = A + (size_t)i * n + j;
+        const double dj = A[(size_t)j * n + j];
+        if (fabs(*x) / sqrt(fabs(di * dj)) > rel_tol) kept++;
+        else *x = 0.0;
+    }
+    cnt[threadIdx.x] = kept;
+    __syncthreads();
+    for (int off = kEffmBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) cnt[threadIdx.x] += cnt[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) rowcount[i] = cnt[0];
+}
+
+hipError_t launch_effm_scatter(int n, const int *rowptr, const int *col, const double *val, double *A,
+                               hipStream_t st)
+{
+    hipLaunchKernelGGL(effm_scatter_kernel, dim3(n), dim3(kEffmBlock), 0, st, n, rowptr, col, val, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_effm_symmetrize(int n, double *A, hipStream_t st)
+{
+    hipLaunchKernelGGL(effm_symmetrize_kernel, dim3(n), dim3(kEffmBlock), 0, st, n, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_effm_truncate(int n, double *A, double rel_tol, int *rowcount, hipStream_t st)
+{
+    hipLaunchKernelGGL(effm_truncate_kernel, dim3(n), dim3(kEffmBlock), 0, st, n, A, rel_tol, rowcount);
+    return hipGetLastError();
 }
 
 // positions: r += (Mi pp) / alpha^2 Delta (src/hd.C:1699-1720); the momenta

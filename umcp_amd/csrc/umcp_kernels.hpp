@@ -112,9 +112,7 @@ struct GatherArgs {
 constexpr int kEffmBlock = 256;
 struct EffmArgs {
     int nv;
-    const int *rowptr;      // nv + 1, CSR of the inverse effective mass
-    const int *col;
-    const double *val;
+    const double *Mi;       // nv x nv inverse effective mass (row-major, truncated entries zero)
     const double *pp_next;  // momenta of this step (from the gather)
     double *pp_keep;        // momenta carried to the next step (the caller's array)
     double *mp;             // out: Mi pp_keep (Langevin), the next step's friction product
@@ -126,6 +124,14 @@ struct EffmArgs {
 hipError_t launch_effm_position(const EffmArgs &a, hipStream_t st);
 // mp = Mi pp (the first step's friction product)
 hipError_t launch_effm_product(const EffmArgs &a, const double *pp, hipStream_t st);
+// set-up of the dense inverse: M from CSR into a zeroed nv x nv array; after
+// the in-place potrf + potri (upper triangle, column-major), the full
+// symmetric inverse; then the reference's truncation with per-row counts of
+// the entries kept
+hipError_t launch_effm_scatter(int n, const int *rowptr, const int *col, const double *val, double *A,
+                               hipStream_t st);
+hipError_t launch_effm_symmetrize(int n, double *A, hipStream_t st);
+hipError_t launch_effm_truncate(int n, double *A, double rel_tol, int *rowcount, hipStream_t st);
 
 struct GatherItem {
     int surf, vblock;

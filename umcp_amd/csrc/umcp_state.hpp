@@ -74,12 +74,12 @@ struct umcp_surface {
     umcp::FaceItem *d_items = nullptr;
     std::vector<umcp::FaceArgs> h_fa;
     bool fa_dirty = true;
-    // momentum-space integrators: truncated inverse effective mass (CSR on
-    // the device) and the step's momenta scratch
+    // momentum-space integrators: truncated inverse effective mass (dense
+    // on the device), the step's momenta and the kept product Mi pp
     bool effm_ready = false;
     double effm_point_mass = 0.0;
-    int *d_effm_rowptr = nullptr, *d_effm_col = nullptr;
-    double *d_effm_val = nullptr, *d_pp_next = nullptr, *d_mp = nullptr;
+    double *d_effm = nullptr;             // nv x nv inverse effective mass (row-major)
+    double *d_pp_next = nullptr, *d_mp = nullptr;
     long effm_nnz = 0;
     std::vector<void *> effm_allocs;
 };

@@ -62,7 +62,7 @@ class BDParams(C.Structure):
 
 EFFM_BD = 0
 EFFM_LD = 1
-EFFM_MAX_NV = 4096
+EFFM_MAX_NV = 65536
 
 
 class EffmParams(C.Structure):
