@@ -384,6 +384,37 @@ def run_effm_leg(args, dev, torch):
         ms = start.elapsed_time(end) / K
         out[name] = {"ms_per_step": ms, "steps_per_s": 1e3 / ms, "face_evals_per_s": s.nfaces * 1e3 / ms,
                      "max_displacement_A": float((d_r - torch.tensor(w.r, device=dev)).abs().max())}
+    # the same integrator on the 40962-vertex vesicle: a set-up the
+    # reference's dense CPU inverse cannot reach (13.4 GB inverse in HBM,
+    # kept as CSR after the truncation)
+    wl = workloads.vesicle_40962()
+    sl = S.Surface(wl.mesh_path)
+    sl.set_params(precision=S.PREC_FP64, **wl.params)
+    sl.generatePlan()
+    sl.setg0(wl.r0)
+    t0 = time.perf_counter()
+    nnz_l, m_l = sl.effm_build()
+    build_l = time.perf_counter() - t0
+    d_r = torch.tensor(wl.r, device=dev, dtype=torch.float64)
+    d_pp = torch.zeros(3 * sl.nv, device=dev, dtype=torch.float64)
+    sl.set_step(0)
+    dt_l, K_l = 1e-12, max(20, K // 5)
+    sl.effm_steps_dev(d_r, d_pp, 3, S.EFFM_LD, dt_l, kT, gamma, seed=7, stream=sptr)
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    sl.effm_steps_dev(d_r, d_pp, K_l, S.EFFM_LD, dt_l, kT, gamma, seed=7, stream=sptr)
+    end.record(stream)
+    torch.cuda.synchronize()
+    ms_l = start.elapsed_time(end) / K_l
+    bytes_l = 12 * nnz_l if nnz_l * 3 < 2 * sl.nv ** 2 else 8 * sl.nv ** 2
+    out["vesicle_40962"] = {"nv": sl.nv, "inverse_mass_kept": nnz_l, "kept_fraction": nnz_l / sl.nv ** 2,
+                            "setup_s": build_l, "dt_s": dt_l, "langevin_steps": K_l,
+                            "langevin_ms_per_step": ms_l, "inverse_bytes_per_product": bytes_l,
+                            "product_GB_per_s_upper_bound": bytes_l / (ms_l * 1e-3) / 1e9,
+                            "finite": bool(torch.isfinite(d_r).all())}
+    sl.close()
+    del d_r, d_pp
     if not args.no_cpu_baseline:
         from oracle import oracle as O
         o = O.OracleSurface(w.mesh_path)

@@ -132,6 +132,13 @@ def test_full_size_vesicle_builds_and_steps():
     nnz, m = g.effm_build()
     say("[effm] 40962: nnz", nnz, "point mass", m)
     assert 0 < nnz <= g.nv ** 2
+    # spot check of the kept (CSR) rows: symmetric pairs, positive diagonal
+    Mi_rows = g.effm_get()
+    for i in (0, 7, 20000, 40961):
+        assert Mi_rows[i, i] > 0
+        j = int(np.argmax(np.abs(Mi_rows[i]) * (np.arange(g.nv) != i)))
+        assert Mi_rows[i, j] == Mi_rows[j, i]
+    del Mi_rows
     kT = units.kT(298.0)
     d_r = torch.tensor(w.r, device="cuda", dtype=torch.float64)
     d_pp = torch.zeros(3 * g.nv, device="cuda", dtype=torch.float64)

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <mutex>
 #include <set>
@@ -838,6 +839,32 @@ int umcp_effm_build(umcp_surface *s, long *nnz, double *point_mass)
     if (e != hipSuccess) return hip_fail(s, e, "effective mass truncation");
     long kept = 0;
     for (int c : cnt) kept += c;
+    s->d_effm_rowptr = nullptr;
+    s->d_effm_col = nullptr;
+    s->d_effm_val = nullptr;
+    // CSR (12 bytes per kept entry) when that reads less than the dense array
+    // (8 bytes per entry): the inverse of a large mesh's mass matrix decays
+    // below the truncation far from the diagonal
+    if (3 * kept < 2 * (long)n * n && kept < (long)INT32_MAX) {
+        std::vector<int> rp((size_t)n + 1, 0);
+        for (int i = 0; i < n; i++) rp[(size_t)i + 1] = rp[(size_t)i] + cnt[(size_t)i];
+        const size_t before2 = s->allocations.size();
+        if ((rc = dev_alloc(s, &s->d_effm_rowptr, rp.size())) || (rc = dev_alloc(s, &s->d_effm_col, (size_t)kept)) ||
+            (rc = dev_alloc(s, &s->d_effm_val, (size_t)kept)))
+            return rc;
+        s->effm_allocs.insert(s->effm_allocs.end(), s->allocations.begin() + (long)before2, s->allocations.end());
+        e = hipMemcpyAsync(s->d_effm_rowptr, rp.data(), sizeof(int) * rp.size(), hipMemcpyHostToDevice, s->stream);
+        if (e == hipSuccess)
+            e = launch_effm_compact(n, s->d_effm, s->d_effm_rowptr, s->d_effm_col, s->d_effm_val, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) return hip_fail(s, e, "effective mass compaction");
+        // the dense array is no longer needed
+        (void)hipFree(s->d_effm);
+        s->allocations.erase(std::find(s->allocations.begin(), s->allocations.end(), (void *)s->d_effm));
+        s->effm_allocs.erase(std::find(s->effm_allocs.begin(), s->effm_allocs.end(), (void *)s->d_effm));
+        s->device_bytes -= sizeof(double) * (size_t)n * n;
+        s->d_effm = nullptr;
+    }
     s->effm_nnz = kept;
     s->effm_point_mass = m;
     s->effm_ready = true;
@@ -851,8 +878,20 @@ int umcp_effm_get(umcp_surface *s, double *Minv)
     if (!s || !Minv) return UMCP_ERR_ARG;
     if (!s->effm_ready) return fail(s, UMCP_ERR_STATE, "effective mass not built (call umcp_effm_build)");
     const size_t n = (size_t)s->T.nv;
-    hipError_t e = hipMemcpy(Minv, s->d_effm, sizeof(double) * n * n, hipMemcpyDeviceToHost);
+    if (s->d_effm) {
+        hipError_t e = hipMemcpy(Minv, s->d_effm, sizeof(double) * n * n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(s, e, "download effective mass");
+        return UMCP_OK;
+    }
+    std::vector<int> rp(n + 1), col((size_t)s->effm_nnz);
+    std::vector<double> val((size_t)s->effm_nnz);
+    hipError_t e = hipMemcpy(rp.data(), s->d_effm_rowptr, sizeof(int) * rp.size(), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(col.data(), s->d_effm_col, sizeof(int) * col.size(), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(val.data(), s->d_effm_val, sizeof(double) * val.size(), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(s, e, "download effective mass");
+    std::memset(Minv, 0, sizeof(double) * n * n);
+    for (size_t i = 0; i < n; i++)
+        for (int k = rp[i]; k < rp[i + 1]; k++) Minv[i * n + (size_t)col[(size_t)k]] = val[(size_t)k];
     return UMCP_OK;
 }
 
@@ -872,6 +911,9 @@ int umcp_effm_steps_dev(umcp_surface *s, double *d_r, double *d_pp, const double
     EffmArgs ea{};
     ea.nv = s->T.nv;
     ea.Mi = s->d_effm;
+    ea.rowptr = s->d_effm_rowptr;
+    ea.col = s->d_effm_col;
+    ea.val = s->d_effm_val;
     ea.pp_next = s->d_pp_next;
     ea.pp_keep = d_pp;
     ea.mp = s->d_mp;

@@ -1099,12 +1099,23 @@ __device__ __forceinline__ void effm_row(const EffmArgs &a, int v, const double 
     #pragma clang fp contract(off)
     const int lane = threadIdx.x & (kWave - 1);
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    const double *row = a.Mi + (size_t)v * a.nv;
-    for (int u = lane; u < a.nv; u += kWave) {
-        const double m = row[u];
-        s0 += m * x[3 * (size_t)u];
-        s1 += m * x[3 * (size_t)u + 1];
-        s2 += m * x[3 * (size_t)u + 2];
+    if (a.rowptr) {
+        const int e1 = a.rowptr[v + 1];
+        for (int e = a.rowptr[v] + lane; e < e1; e += kWave) {
+            const double m = a.val[e];
+            const size_t u = (size_t)a.col[e];
+            s0 += m * x[3 * u];
+            s1 += m * x[3 * u + 1];
+            s2 += m * x[3 * u + 2];
+        }
+    } else {
+        const double *row = a.Mi + (size_t)v * a.nv;
+        for (int u = lane; u < a.nv; u += kWave) {
+            const double m = row[u];
+            s0 += m * x[3 * (size_t)u];
+            s1 += m * x[3 * (size_t)u + 1];
+            s2 += m * x[3 * (size_t)u + 2];
+        }
     }
 #pragma unroll
     for (int off = 1; off < kWave; off <<= 1) {
@@ -1154,6 +1165,43 @@ __global__ __launch_bounds__(kEffmBlock) void effm_truncate_kernel(int n, double
         __syncthreads();
     }
     if (threadIdx.x == 0) rowcount[i] = cnt[0];
+}
+
+// one block per row: chunks of kEffmBlock columns, a block prefix count of
+// the non-zero entries gives each kept entry its place
+__global__ __launch_bounds__(kEffmBlock) void effm_compact_kernel(int n, const double *A, const int *rowptr, int *col,
+                                                                  double *val)
+{
+    __shared__ int pre[kEffmBlock];
+    const int i = blockIdx.x;
+    int base = rowptr[i];
+    for (int j0 = 0; j0 < n; j0 += kEffmBlock) {
+        const int j = j0 + (int)threadIdx.x;
+        const double x = j < n ? A[(size_t)i * n + j] : 0.0;
+        const int keep = x != 0.0 ? 1 : 0;
+        pre[threadIdx.x] = keep;
+        __syncthreads();
+        // inclusive scan (Hillis-Steele) over the chunk
+        for (int off = 1; off < kEffmBlock; off <<= 1) {
+            const int add = (int)threadIdx.x >= off ? pre[threadIdx.x - off] : 0;
+            __syncthreads();
+            pre[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (keep) {
+            const int at = base + pre[threadIdx.x] - 1;
+            col[at] = j;
+            val[at] = x;
+        }
+        base += pre[kEffmBlock - 1];
+        __syncthreads();
+    }
+}
+
+hipError_t launch_effm_compact(int n, const double *A, const int *rowptr, int *col, double *val, hipStream_t st)
+{
+    hipLaunchKernelGGL(effm_compact_kernel, dim3(n), dim3(kEffmBlock), 0, st, n, A, rowptr, col, val);
+    return hipGetLastError();
 }
 
 hipError_t launch_effm_scatter(int n, const int *rowptr, const int *col, const double *val, double *A,

@@ -112,7 +112,12 @@ struct GatherArgs {
 constexpr int kEffmBlock = 256;
 struct EffmArgs {
     int nv;
-    const double *Mi;       // nv x nv inverse effective mass (row-major, truncated entries zero)
+    // the inverse effective mass: dense (Mi, nv x nv row-major, truncated
+    // entries zero) or, when the truncation keeps few enough entries, CSR
+    // (rowptr, col, val; columns ascending)
+    const double *Mi;
+    const int *rowptr, *col;
+    const double *val;
     const double *pp_next;  // momenta of this step (from the gather)
     double *pp_keep;        // momenta carried to the next step (the caller's array)
     double *mp;             // out: Mi pp_keep (Langevin), the next step's friction product
@@ -132,6 +137,9 @@ hipError_t launch_effm_scatter(int n, const int *rowptr, const int *col, const d
                                hipStream_t st);
 hipError_t launch_effm_symmetrize(int n, double *A, hipStream_t st);
 hipError_t launch_effm_truncate(int n, double *A, double rel_tol, int *rowcount, hipStream_t st);
+// the kept entries of the truncated dense array as CSR (rowptr from the
+// per-row counts), columns ascending
+hipError_t launch_effm_compact(int n, const double *A, const int *rowptr, int *col, double *val, hipStream_t st);
 
 struct GatherItem {
     int surf, vblock;

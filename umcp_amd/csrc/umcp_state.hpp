@@ -78,7 +78,9 @@ struct umcp_surface {
     // on the device), the step's momenta and the kept product Mi pp
     bool effm_ready = false;
     double effm_point_mass = 0.0;
-    double *d_effm = nullptr;             // nv x nv inverse effective mass (row-major)
+    double *d_effm = nullptr;             // nv x nv inverse effective mass (row-major), or
+    int *d_effm_rowptr = nullptr, *d_effm_col = nullptr;   // its kept entries as CSR
+    double *d_effm_val = nullptr;
     double *d_pp_next = nullptr, *d_mp = nullptr;
     long effm_nnz = 0;
     std::vector<void *> effm_allocs;
