@@ -150,3 +150,30 @@ def test_full_size_vesicle_builds_and_steps():
     moved = float((d_r - torch.tensor(w.r, device="cuda", dtype=torch.float64)).abs().max())
     say("   3 steps, max displacement", moved)
     assert 0 < moved < 10.0
+
+
+def test_periodic_sheet_against_oracle():
+    """A periodic 32 x 32 sheet (1024 vertices, faces across the box
+    boundary): inverse and a Brownian trajectory against the oracle."""
+    w = workloads.sheet(32, seed=6, name="effm_sheet32")
+    g = S.Surface(w.mesh_path)
+    g.set_params(precision=S.PREC_FP64, **w.params)
+    g.generatePlan()
+    g.setg0(w.r0)
+    o = O.OracleSurface(w.mesh_path)
+    o.set_params(**w.params)
+    o.setg0(w.r0)
+    nnz, m = g.effm_build()
+    assert abs(m - o.effm_point_mass()) <= 1e-12 * m
+    Mi_o = o.effm_inverse()
+    Mi = g.effm_get()
+    err = np.abs(Mi - Mi_o).max() / np.abs(Mi_o).max()
+    say("[effm] sheet 32: nnz", nnz, "max rel diff", err)
+    assert err <= 1e-11
+    kT = units.kT(w.T)
+    xi = np.random.default_rng(41).standard_normal((5, 3 * g.nv))
+    r_g, _ = g.effm_steps(w.r, np.zeros(3 * g.nv), xi, S.EFFM_BD, DT, kT, GAMMA)
+    r_o, _ = o.effm_steps(w.r, np.zeros(3 * g.nv), xi, S.EFFM_BD, DT, kT, GAMMA, Mi_o)
+    disp = np.abs(r_o - w.r).max()
+    say("   trajectory max |dr|", np.abs(r_g - r_o).max(), "displacement", disp)
+    assert np.abs(r_g - r_o).max() <= 1e-9 * disp
