@@ -212,6 +212,30 @@ int umcp_read_restart(const char *path, int nv, double *r, double *pp, double *a
  * umcp_prepare. */
 int umcp_write_limit_surface(umcp_surface *s, const double *r, const char *path, int append);
 
+/* Points on the limit surface, batched; faces in the reference's numbering
+ * (regular faces first, then irregular), (u, v) the face's parameters.
+ * umcp_point_eval replaces surface::evaluateRNRM (src/interpLib.C:7936-8196):
+ * R[3i..] = alpha-scaled position, nrm[3i..] = unit normal of the unscaled
+ * tangents; at an irregular face's extraordinary vertex (u = v = 0) the
+ * reference's mask (1/2 on the vertex, 1/(2 val) on each neighbour) and a NaN
+ * normal.  umcp_point_grad replaces a loop of surface::pointGradient
+ * (src/grad.C:5697-6443) over the points: for each point i it adds to
+ * g (3nv+3) the vertex and alpha gradient of an energy with derivatives
+ * de_dr[3i..] and de_dnrm[3i..] at the point (times frac_mult[i]; NULL: 1),
+ * plus -kT dlog(g)/dr when on_surface, and adds to g_puv[2i..] its derivative
+ * in (u, v).  Like the reference, the gradient on irregular faces forms the
+ * surface from unscaled positions.  Inputs the reference stops on (a
+ * sub-patch level beyond MAX_DOMAIN = 100, (u, v) outside an irregular face,
+ * the gradient at the extraordinary vertex) return UMCP_ERR_ARG naming the
+ * point.  Sums over the points of one vertex are taken in point order.
+ * Needs umcp_plan; host arrays; synchronous. */
+#define UMCP_POINT_MAX_BATCH (1 << 24)
+int umcp_point_eval(umcp_surface *s, const double *r, int n, const int *faces, const double *uv, double *R,
+                    double *nrm);
+int umcp_point_grad(umcp_surface *s, const double *r, int n, const int *faces, const double *uv,
+                    const double *de_dr, const double *de_dnrm, const double *frac_mult, int on_surface, double kT,
+                    double *g, double *g_puv);
+
 /* Brownian steps drawing their noise on the device: xi of step k is the
  * counter-based stream of (seed, k) that umcp_noise writes (Philox4x64-10
  * keyed by seed, Box-Muller), so a trajectory depends only on (seed, first

@@ -85,6 +85,11 @@ def lib():
         L.orc_effm_truncate.restype = C.c_long
         L.orc_effm_steps.argtypes = [vp, dp, dp, dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                      dp, dp]
+        L.orc_point_eval.argtypes = [vp, dp, C.c_int, C.c_double, C.c_double, dp, dp]
+        L.orc_point_eval.restype = C.c_int
+        L.orc_point_grad.argtypes = [vp, dp, C.c_int, C.c_double, C.c_double, dp, dp, C.c_double,
+                                     C.c_int, C.c_double, dp, dp]
+        L.orc_point_grad.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -243,6 +248,45 @@ class OracleSurface:
         lib().orc_effm_steps(self._h, _p(r), _p(pp), _p(xi), n, int(mode), dt, kT, gamma, _p(Mi),
                              _p(e) if energies else None)
         return (r, pp, e) if energies else (r, pp)
+
+    # points on the limit surface (src/interpLib.C:7936, src/grad.C:5697)
+    def point_eval(self, r: np.ndarray, faces, uv):
+        """(R, nrm), each [n, 3], at the points (faces[i], uv[i])."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        uv = np.asarray(uv, dtype=np.float64).reshape(-1, 2)
+        n = uv.shape[0]
+        R, N = np.zeros((n, 3)), np.zeros((n, 3))
+        for i in range(n):
+            a, b = np.zeros(3), np.zeros(3)
+            rc = lib().orc_point_eval(self._h, _p(r), int(faces[i]), float(uv[i, 0]), float(uv[i, 1]),
+                                      _p(a), _p(b))
+            if rc:
+                raise ValueError("oracle: point %d outside the reference's domain" % i)
+            R[i], N[i] = a, b
+        return R, N
+
+    def point_grad(self, r: np.ndarray, faces, uv, de_dr, de_dnrm, frac_mult=None,
+                   on_surface: int = 0, kT: float = 0.0, g: np.ndarray | None = None):
+        """(g, g_puv): pointGradient of every point added, in point order, to
+        g (3nv+3; zeros if None) and to g_puv [n, 2] (zeros)."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        uv = np.asarray(uv, dtype=np.float64).reshape(-1, 2)
+        de_dr = np.asarray(de_dr, dtype=np.float64).reshape(-1, 3)
+        de_dnrm = np.asarray(de_dnrm, dtype=np.float64).reshape(-1, 3)
+        n = uv.shape[0]
+        g = np.zeros_like(r) if g is None else np.ascontiguousarray(g, dtype=np.float64)
+        gp = np.zeros((n, 2))
+        for i in range(n):
+            a = np.ascontiguousarray(de_dr[i])
+            b = np.ascontiguousarray(de_dnrm[i])
+            t = np.zeros(2)
+            fm = 1.0 if frac_mult is None else float(frac_mult[i])
+            rc = lib().orc_point_grad(self._h, _p(r), int(faces[i]), float(uv[i, 0]), float(uv[i, 1]),
+                                      _p(a), _p(b), fm, int(on_surface), float(kT), _p(g), _p(t))
+            if rc:
+                raise ValueError("oracle: point %d outside the reference's domain" % i)
+            gp[i] = t
+        return g, gp
 
     def face_info(self, f: int):
         owner, ncp, tri = C.c_int(), C.c_int(), C.c_int()

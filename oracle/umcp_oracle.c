@@ -2015,3 +2015,186 @@ void orc_effm_steps(omesh *m, double *r, double *pp, const double *xi, int nstep
     free(next);
     free(fr);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Points on the limit surface: surface::evaluateRNRM (src/interpLib.C:     */
+/* 7936-8196) and surface::pointGradient (src/grad.C:5697-6443), the        */
+/* coupling calls of the particle complexes (src/pcomplex.C:501,            */
+/* src/bd.C:1011, src/minimize.C:233).                                      */
+/* ------------------------------------------------------------------------ */
+
+#define ORC_THRESH 1e-4       /* src/interpLib.C:24 */
+#define ORC_MAX_DOMAIN 100    /* src/interp.h:20 */
+
+/* irr_kernel::domain / get_map (src/irr_kernel.C:380-393, 474-541): the
+ * sub-patch level of (u, v) near the extraordinary vertex.  Returns 1 for the
+ * vertex itself (u + v not > 0: the reference's domain < 0 case), -1 for a
+ * level beyond MAX_DOMAIN or local coordinates outside the sub-patch (the
+ * reference's exit(1) paths), else 0. */
+static int irr_locate(double fu, double fv)
+{
+    double f_domain = -log10(fu + fv) / log10(2.0);
+    if (!(f_domain < 1e300) || !(f_domain > -1e300)) return 1;
+    if (fu + fv >= 1.0 - 1e-9) f_domain = 1;
+    int domain = (int)lround(ceil(f_domain));
+    if (domain - 1 >= ORC_MAX_DOMAIN) return -1;
+    double pow2 = pow(2.0, domain - 1.0);
+    fu *= pow2; fv *= pow2;
+    if (fu > 0.5) { fv = 2 * fv; fu = 2.0 * fu - 1.0; }
+    else if (fv > 0.5) { fu = 2 * fu; fv = 2.0 * fv - 1.0; }
+    else { fv = 1 - 2 * fv; fu = 1 - 2 * fu; }
+    if (fu + fv > 1.0 + ORC_THRESH || fu < -ORC_THRESH || fv < -ORC_THRESH) return -1;
+    return 0;
+}
+
+/* the face's six derivative tables at (u, v) and its control points */
+static int point_tables(const omesh *m, int f, double u, double v, double t[6][OMAXCP], int *ncp,
+                        int *cp, double *pbc, int *irregular)
+{
+    int npt;
+    double dudv;
+    const oform *F = face_forms(m, f, &npt, &dudv);
+    *ncp = F[0].ncp;
+    memcpy(cp, F[0].cp, sizeof(int) * (size_t)F[0].ncp);
+    memcpy(pbc, F[0].pbc, sizeof(double) * 3 * (size_t)F[0].ncp);
+    *irregular = f >= m->nfr;
+    if (!*irregular) {
+        double B[6][12];
+        orc_basis(u, v, B);
+        for (int d = 0; d < 6; d++)
+            for (int x = 0; x < 12; x++) t[d][x] = B[d][x];
+        return 0;
+    }
+    int loc = irr_locate(u, v);
+    if (loc) return loc;
+    int val = m->val[m->face_owner[f]];
+    double tab[6 * OMAXCP];
+    orc_irregular_table(val, u, v, tab);
+    for (int d = 0; d < 6; d++)
+        for (int x = 0; x < *ncp; x++) t[d][x] = tab[d * *ncp + x];
+    return 0;
+}
+
+int orc_point_eval(const omesh *m, const double *r, int f, double u, double v, double *rp,
+                   double *nrm)
+{
+    int nv = m->nv, ncp, cp[OMAXCP], irr;
+    double t[6][OMAXCP], pbc[3 * OMAXCP];
+    double al[3] = {r[3 * nv], r[3 * nv + 1], r[3 * nv + 2]};
+    if (f < 0 || f >= m->nfr + m->nfi) return -1;
+    int rc = point_tables(m, f, u, v, t, &ncp, cp, pbc, &irr);
+    if (rc < 0) return -1;
+    if (rc == 1) {
+        /* the extraordinary vertex: 1/2 on it, 1/(2 val) on each neighbour at
+         * its minimum image; normal unknown (src/interpLib.C:7963-8001; the
+         * reference reads its vertex copy, which surface::put keeps equal to r) */
+        int i = m->face_owner[f], val = m->val[i];
+        double lr[3] = {0.5 * r[3 * i], 0.5 * r[3 * i + 1], 0.5 * r[3 * i + 2]};
+        double w = 1.0 / (val * 2);
+        for (int e = 0; e < val; e++) {
+            int j = m->nb[i * OMAXV + e];
+            double dr[3] = {r[3 * j] - r[3 * i], r[3 * j + 1] - r[3 * i + 1], r[3 * j + 2] - r[3 * i + 2]};
+            double put[3];
+            min_image(dr, (double(*)[3])m->PBC, put);
+            for (int c = 0; c < 3; c++) lr[c] += (dr[c] + r[3 * i + c]) * w;
+        }
+        for (int c = 0; c < 3; c++) {
+            rp[c] = lr[c] * al[c];
+            nrm[c] = NAN;
+        }
+        return 0;
+    }
+    double X[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int x = 0; x < ncp; x++)
+        for (int c = 0; c < 3; c++) {
+            double q = r[3 * cp[x] + c] + pbc[3 * x + c];
+            for (int d = 0; d < 3; d++) X[d][c] += t[d][x] * q;
+        }
+    for (int c = 0; c < 3; c++) rp[c] = X[0][c] * al[c];
+    /* the normal of the unscaled tangents, as the reference forms it */
+    cross3(X[1], X[2], nrm);
+    normalize3(nrm);
+    return 0;
+}
+
+/* pointGradient: adds to gr (3nv+3) the gradient of a point energy with
+ * derivatives de_dr and de_dnrm at the surface point (f, u, v), and to
+ * g_puv[2] its derivative in (u, v).  The normal's Jacobians are the
+ * reference's closed forms over fac = |Ru x Rv|^3 (src/grad.C:5876-5893);
+ * on_surface adds -kT dlog(g)/dr (src/grad.C:6016-6044).  On irregular faces
+ * the reference forms R and its derivatives without the alpha scaling
+ * (src/grad.C:6185-6220) and this restatement keeps that. */
+int orc_point_grad(const omesh *m, const double *r, int f, double u, double v, const double *de_dr,
+                   const double *de_dnrm, double frac_mult, int on_surface, double kT, double *gr,
+                   double *g_puv)
+{
+    int nv = m->nv, ncp, cp[OMAXCP], irr;
+    double t[6][OMAXCP], pbc[3 * OMAXCP];
+    double al[3] = {r[3 * nv], r[3 * nv + 1], r[3 * nv + 2]};
+    if (f < 0 || f >= m->nfr + m->nfi) return -1;
+    int rc = point_tables(m, f, u, v, t, &ncp, cp, pbc, &irr);
+    if (rc != 0) return -1;   /* the reference dereferences a NULL map at the vertex */
+    double X[6][3];
+    memset(X, 0, sizeof X);
+    for (int x = 0; x < ncp; x++)
+        for (int c = 0; c < 3; c++) {
+            double q = r[3 * cp[x] + c] + pbc[3 * x + c];
+            if (!irr) q *= al[c];
+            for (int d = 0; d < 6; d++) X[d][c] += t[d][x] * q;
+        }
+    const double *Ru = X[1], *Rv = X[2], *Suu = X[3], *Suv = X[4], *Svv = X[5];
+    double RuRu = Ru[0] * Ru[0] + Ru[1] * Ru[1] + Ru[2] * Ru[2];
+    double RuRv = Ru[0] * Rv[0] + Ru[1] * Rv[1] + Ru[2] * Rv[2];
+    double RvRv = Rv[0] * Rv[0] + Rv[1] * Rv[1] + Rv[2] * Rv[2];
+    double g = sqrt(RuRu * RvRv - RuRv * RuRv);
+    double sq = sqrt(-pow(RuRv, 2) + RuRu * RvRv);
+    double dg_RuRu = RvRv / (2. * sq), dg_RuRv = -(RuRv / sq), dg_RvRv = RuRu / (2. * sq);
+    double dg_ru[3], dg_rv[3];
+    for (int k = 0; k < 3; k++) {
+        dg_rv[k] = dg_RvRv * 2 * Rv[k] + dg_RuRv * Ru[k];
+        dg_ru[k] = dg_RuRv * Rv[k] + dg_RuRu * 2 * Ru[k];
+    }
+    /* J_u[j][k] = d nrm_j / d Ru_k, J_v[j][k] = d nrm_j / d Rv_k */
+    double fac = pow(pow(Ru[1] * Rv[0] - Ru[0] * Rv[1], 2) + pow(Ru[2] * Rv[0] - Ru[0] * Rv[2], 2) +
+                         pow(Ru[2] * Rv[1] - Ru[1] * Rv[2], 2),
+                     1.5);
+    double a = Ru[2] * Rv[1] - Ru[1] * Rv[2];   /* -n_x */
+    double b = Ru[2] * Rv[0] - Ru[0] * Rv[2];   /*  n_y */
+    double c = Ru[1] * Rv[0] - Ru[0] * Rv[1];   /* -n_z */
+    double px = Ru[1] * Rv[0] * Rv[1] + Ru[2] * Rv[0] * Rv[2] - Ru[0] * (pow(Rv[1], 2) + pow(Rv[2], 2));
+    double py = Rv[1] * (Ru[0] * Rv[0] + Ru[2] * Rv[2]) - Ru[1] * (pow(Rv[0], 2) + pow(Rv[2], 2));
+    double pz = Ru[2] * (pow(Rv[0], 2) + pow(Rv[1], 2)) - (Ru[0] * Rv[0] + Ru[1] * Rv[1]) * Rv[2];
+    double qx = pow(Ru[1], 2) * Rv[0] - Ru[0] * Ru[1] * Rv[1] + Ru[2] * (Ru[2] * Rv[0] - Ru[0] * Rv[2]);
+    double qy = -(Ru[0] * Ru[1] * Rv[0]) + pow(Ru[0], 2) * Rv[1] + Ru[2] * (Ru[2] * Rv[1] - Ru[1] * Rv[2]);
+    double qz = -(Ru[0] * Ru[2] * Rv[0]) + pow(Ru[0], 2) * Rv[2] + Ru[1] * (-(Ru[2] * Rv[1]) + Ru[1] * Rv[2]);
+    double Ju[3][3] = {{-(a * px) / fac, b * px / fac, -(c * px) / fac},
+                       {-(a * py) / fac, b * py / fac, -(c * py) / fac},
+                       {a * pz / fac, -(b * pz) / fac, c * pz / fac}};
+    double Jv[3][3] = {{a * qx / fac, -(b * qx) / fac, c * qx / fac},
+                       {a * qy / fac, -(b * qy) / fac, c * qy / fac},
+                       {a * qz / fac, -(b * qz) / fac, c * qz / fac}};
+    double de_ru[3], de_rv[3], Nu[3] = {0, 0, 0}, Nv[3] = {0, 0, 0};
+    for (int k = 0; k < 3; k++) {
+        de_ru[k] = (de_dnrm[0] * Ju[0][k] + de_dnrm[1] * Ju[1][k] + de_dnrm[2] * Ju[2][k]) * frac_mult;
+        de_rv[k] = (de_dnrm[0] * Jv[0][k] + de_dnrm[1] * Jv[1][k] + de_dnrm[2] * Jv[2][k]) * frac_mult;
+    }
+    for (int k = 0; k < 3; k++)
+        for (int j = 0; j < 3; j++) {
+            Nu[j] += Ju[j][k] * Suu[k] + Jv[j][k] * Suv[k];
+            Nv[j] += Ju[j][k] * Suv[k] + Jv[j][k] * Svv[k];
+        }
+    g_puv[0] += de_dr[0] * Ru[0] + de_dr[1] * Ru[1] + de_dr[2] * Ru[2];
+    g_puv[1] += de_dr[0] * Rv[0] + de_dr[1] * Rv[1] + de_dr[2] * Rv[2];
+    g_puv[0] += de_dnrm[0] * Nu[0] + de_dnrm[1] * Nu[1] + de_dnrm[2] * Nu[2];
+    g_puv[1] += de_dnrm[0] * Nv[0] + de_dnrm[1] * Nv[1] + de_dnrm[2] * Nv[2];
+    for (int x = 0; x < ncp; x++) {
+        double dedr[3];
+        for (int k = 0; k < 3; k++) {
+            dedr[k] = de_dr[k] * frac_mult * t[0][x] + de_ru[k] * t[1][x] + de_rv[k] * t[2][x];
+            if (on_surface) dedr[k] -= (1 / g) * kT * (dg_ru[k] * t[1][x] + dg_rv[k] * t[2][x]);
+            gr[3 * cp[x] + k] += dedr[k] * al[k];
+            gr[3 * nv + k] += dedr[k] * (r[3 * cp[x] + k] + pbc[3 * x + k]);
+        }
+    }
+    return 0;
+}

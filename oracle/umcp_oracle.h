@@ -94,6 +94,19 @@ void   orc_basis(double u, double v, double out[6][12]);
 /* irregular (Stam) table for valence val at parameter (u,v): out[6][val+6] */
 int    orc_irregular_table(int val, double u, double v, double *out);
 
+/* points on the limit surface (face in the reference's numbering: regular
+ * faces first).  orc_point_eval: surface::evaluateRNRM, position (alpha
+ * scaled) and unit normal; at an irregular face's extraordinary vertex
+ * (u = v = 0) the reference's limit mask and a NaN normal.  orc_point_grad:
+ * surface::pointGradient, adds to gr[3nv+3] and g_puv[2].  Both return -1 for
+ * the inputs the reference rejects with exit(1) (or, for the gradient at the
+ * extraordinary vertex, dereferences a NULL map). */
+int    orc_point_eval(const omesh *m, const double *r, int f, double u, double v, double *rp,
+                      double *nrm);
+int    orc_point_grad(const omesh *m, const double *r, int f, double u, double v,
+                      const double *de_dr, const double *de_dnrm, double frac_mult, int on_surface,
+                      double kT, double *gr, double *g_puv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,238 @@
+"""GPU: points on the limit surface (surface::evaluateRNRM,
+src/interpLib.C:7936-8196; surface::pointGradient, src/grad.C:5697-6443)
+against the oracle.
+
+The device forms the normal's Jacobian as a projector applied to cross
+products, the oracle with the reference's closed forms over |Ru x Rv|^3, so
+the two agree to rounding only.  Bars, FP64 with per-vertex sums in point
+order on both sides: positions to 1e-12 of the surface's extent everywhere;
+normals to 1e-12 absolute and the (u, v), vertex and alpha gradients to 1e-12
+of their largest entry wherever the evaluation is well conditioned (regular
+faces; irregular faces at u + v >= 0.05).  Closer to an extraordinary vertex
+the reference's sub-patch evaluation (weights of level d scaled by 2^d over
+entries that cancel to (2 lambda)^d) is ill conditioned: one ulp of u moves
+the oracle's own normal by ~5e-10 at u + v = 1e-5 and by ~0.1 at 1e-12.  There
+the bar is 1000 times the oracle's own response to a one-ulp change of u or v.
+Covered: regular faces, sub-patch levels 1-40 on irregular faces, the vertex
+itself (evaluation only), alpha != 1, frac_mult, the on_surface term, a
+periodic sheet (image shifts), accumulation into the caller's arrays,
+repeatability, and the reference's rejections.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import oracle as O  # noqa: E402
+from umcp_amd import surface as S  # noqa: E402
+from umcp_amd import workloads  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _pair(w):
+    g = S.Surface(w.mesh_path)
+    g.set_params(precision=S.PREC_FP64, **w.params)
+    g.generatePlan()
+    o = O.OracleSurface(w.mesh_path)
+    o.set_params(**w.params)
+    return g, o
+
+
+@pytest.fixture(scope="module")
+def ves():
+    w = workloads.vesicle_642()
+    g, o = _pair(w)
+    return w, g, o
+
+
+def random_points(o, n, seed, deep=0):
+    """n points uniform over the faces; the first `deep` points on irregular
+    faces moved towards the extraordinary vertex, u + v = 10^-1 .. 10^-12."""
+    rng = np.random.default_rng(seed)
+    faces = rng.integers(0, o.nfaces, n)
+    u, v = rng.random(n), rng.random(n)
+    flip = u + v > 1
+    u[flip], v[flip] = 1 - u[flip], 1 - v[flip]
+    uv = np.stack([u, v], 1)
+    irr = np.flatnonzero(faces >= o.nreg)
+    for k, i in enumerate(irr[:deep]):
+        s = 10.0 ** -(1 + k % 12)
+        uv[i] *= s / max(uv[i].sum(), 1e-300)
+    return faces.astype(np.int32), uv
+
+
+def well_conditioned(o, faces, uv):
+    return (faces < o.nreg) | (uv.sum(axis=1) >= 0.05)
+
+
+def ulp_response(fn, uv):
+    """Largest change of fn(uv) when u or v of every point moves by one ulp."""
+    base = fn(uv)
+    out = np.zeros_like(base)
+    for k in range(2):
+        q = uv.copy()
+        q[:, k] = np.nextafter(q[:, k], np.inf)
+        out = np.maximum(out, np.abs(fn(q) - base))
+    return out
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("alpha", [(1.0, 1.0, 1.0), (1.1, 0.9, 1.05)])
+def test_eval_matches_oracle(ves, alpha):
+    w, g, o = ves
+    r = w.r.copy()
+    r[-3:] = alpha
+    faces, uv = random_points(o, 3000, 5, deep=120)
+    ok = well_conditioned(o, faces, uv)
+    say("[points] eval n=%d (%d near a vertex) alpha=%s" % (len(faces), (~ok).sum(), alpha))
+    R, N = g.point_eval(r, faces, uv)
+    Ro, No = o.point_eval(r, faces, uv)
+    say("   R rel", rel(R, Ro), "nrm abs (well conditioned)", np.abs(N - No)[ok].max())
+    assert rel(R, Ro) <= 1e-12
+    assert np.abs(N - No)[ok].max() <= 1e-12
+    sens = ulp_response(lambda q: o.point_eval(r, faces[~ok], q)[1], uv[~ok])
+    err = np.abs(N[~ok] - No[~ok])
+    say("   near the vertex: max err", err.max(), "max 1-ulp response", sens.max())
+    assert (err <= 1e-12 + 1000 * sens).all()
+
+
+def test_eval_at_extraordinary_vertex(ves):
+    w, g, o = ves
+    r = w.r.copy()
+    faces = np.array([o.nreg, o.nreg + 5, o.nfaces - 1], np.int32)
+    uv = np.zeros((3, 2))
+    R, N = g.point_eval(r, faces, uv)
+    Ro, No = o.point_eval(r, faces, uv)
+    assert rel(R, Ro) <= 1e-12
+    assert np.isnan(N).all() and np.isnan(No).all()
+
+
+@pytest.mark.parametrize("alpha,on_surface", [((1.0, 1.0, 1.0), False), ((1.0, 1.0, 1.0), True),
+                                              ((1.1, 0.9, 1.05), True)])
+def test_grad_matches_oracle(ves, alpha, on_surface):
+    w, g, o = ves
+    r = w.r.copy()
+    r[-3:] = alpha
+    faces, uv = random_points(o, 2000, 6)
+    rng = np.random.default_rng(7)
+    a, b = rng.normal(size=(len(faces), 3)), rng.normal(size=(len(faces), 3))
+    fm = rng.random(len(faces)) + 0.5
+    kT = 0.616
+    ok = well_conditioned(o, faces, uv)
+    faces, uv, a, b, fm = faces[ok], uv[ok], a[ok], b[ok], fm[ok]
+    say("[points] grad n=%d alpha=%s on_surface=%d" % (len(faces), alpha, on_surface))
+    G, P = g.point_grad(r, faces, uv, a, b, frac_mult=fm, on_surface=on_surface, kT=kT)
+    Go, Po = o.point_grad(r, faces, uv, a, b, frac_mult=fm, on_surface=int(on_surface), kT=kT)
+    say("   g rel", rel(G, Go), "alpha rel", rel(G[-3:], Go[-3:]), "g_puv rel", rel(P, Po))
+    assert rel(G[:-3], Go[:-3]) <= 1e-12
+    assert rel(G[-3:], Go[-3:]) <= 1e-12
+    assert rel(P, Po) <= 1e-12
+
+
+def test_grad_near_extraordinary_vertex(ves):
+    w, g, o = ves
+    r = w.r.copy()
+    faces, uv = random_points(o, 4000, 15, deep=10 ** 6)
+    keep = ~well_conditioned(o, faces, uv)
+    faces, uv = faces[keep][:240], uv[keep][:240]
+    n = len(faces)
+    rng = np.random.default_rng(16)
+    a, b = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+    say("[points] grad near the vertex n=%d, u+v down to %.1e" % (n, uv.sum(axis=1).min()))
+    G, P = g.point_grad(r, faces, uv, a, b, on_surface=True, kT=0.6)
+    Go, Po = o.point_grad(r, faces, uv, a, b, on_surface=1, kT=0.6)
+    sG = ulp_response(lambda q: o.point_grad(r, faces, q, a, b, on_surface=1, kT=0.6)[0], uv)
+    sP = ulp_response(lambda q: o.point_grad(r, faces, q, a, b, on_surface=1, kT=0.6)[1], uv)
+    say("   g err", np.abs(G - Go).max(), "1-ulp response", sG.max(), "g_puv err", np.abs(P - Po).max(),
+        "1-ulp response", sP.max())
+    assert (np.abs(G - Go) <= 1e-12 * np.abs(Go).max() + 1000 * sG.max()).all()
+    assert (np.abs(P - Po) <= 1e-12 * np.abs(Po).max() + 1000 * sP).all()
+
+
+def test_grad_accumulates_and_repeats(ves):
+    w, g, o = ves
+    r = w.r.copy()
+    faces, uv = random_points(o, 500, 8, deep=40)
+    rng = np.random.default_rng(9)
+    a, b = rng.normal(size=(500, 3)), rng.normal(size=(500, 3))
+    G1, P1 = g.point_grad(r, faces, uv, a, b)
+    G2, P2 = g.point_grad(r, faces, uv, a, b)
+    assert np.array_equal(G1, G2) and np.array_equal(P1, P2)
+    base = rng.normal(size=r.size)
+    pb = rng.normal(size=(500, 2))
+    G3, P3 = g.point_grad(r, faces, uv, a, b, g=base.copy(), g_puv=pb.copy())
+    assert np.allclose(G3, base + G1, rtol=0, atol=1e-13 * np.abs(G1).max())
+    assert np.allclose(P3, pb + P1, rtol=0, atol=1e-13 * np.abs(P1).max())
+    # two half batches accumulated give the whole batch
+    Gh, Ph = g.point_grad(r, faces[:250], uv[:250], a[:250], b[:250])
+    g.point_grad(r, faces[250:], uv[250:], a[250:], b[250:], g=Gh)
+    assert rel(Gh, G1) <= 1e-12
+
+
+def test_periodic_sheet_matches_oracle():
+    w = workloads.sheet(16, seed=3)
+    g, o = _pair(w)
+    r = w.r.copy()
+    faces, uv = random_points(o, 1500, 10)
+    say("[points] periodic sheet nv=%d faces=%d" % (o.nv, o.nfaces))
+    R, N = g.point_eval(r, faces, uv)
+    Ro, No = o.point_eval(r, faces, uv)
+    assert rel(R, Ro) <= 1e-12 and np.abs(N - No).max() <= 1e-12
+    rng = np.random.default_rng(12)
+    a, b = rng.normal(size=(1500, 3)), rng.normal(size=(1500, 3))
+    G, P = g.point_grad(r, faces, uv, a, b, on_surface=True, kT=0.6)
+    Go, Po = o.point_grad(r, faces, uv, a, b, on_surface=1, kT=0.6)
+    say("   g rel", rel(G, Go), "g_puv rel", rel(P, Po))
+    assert rel(G[:-3], Go[:-3]) <= 1e-12 and rel(G[-3:], Go[-3:]) <= 1e-12 and rel(P, Po) <= 1e-12
+
+
+def test_large_batch_is_sum_of_parts(ves):
+    w, g, o = ves
+    r = w.r.copy()
+    n = 200000
+    faces, uv = random_points(o, n, 13)
+    ok = well_conditioned(o, faces[:2000], uv[:2000])
+    rng = np.random.default_rng(14)
+    a, b = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+    say("[points] batch n=%d" % n)
+    R, N = g.point_eval(r, faces, uv)
+    Ro, No = o.point_eval(r, faces[:2000], uv[:2000])
+    assert rel(R[:2000], Ro) <= 1e-12 and np.abs(N[:2000] - No)[ok].max() <= 1e-12
+    G, P = g.point_grad(r, faces, uv, a, b)
+    acc = np.zeros_like(r)
+    for k in range(0, n, 50000):
+        g.point_grad(r, faces[k:k + 50000], uv[k:k + 50000], a[k:k + 50000], b[k:k + 50000], g=acc)
+    say("   whole vs parts rel", rel(G, acc))
+    assert rel(G, acc) <= 1e-12
+    _, Po = o.point_grad(r, faces[:2000], uv[:2000], a[:2000], b[:2000])
+    assert rel(P[:2000][ok], Po[ok]) <= 1e-12
+
+
+def test_rejections(ves):
+    w, g, o = ves
+    r = w.r.copy()
+    z = np.zeros((1, 3))
+    irr = np.array([o.nreg], np.int32)
+    with pytest.raises(S.UmcpError):
+        g.point_grad(r, irr, np.zeros((1, 2)), z, z)
+    with pytest.raises(S.UmcpError):
+        g.point_eval(r, irr, np.array([[1e-40, 1e-40]]))
+    with pytest.raises(S.UmcpError):
+        g.point_eval(r, np.array([o.nfaces], np.int32), np.array([[0.2, 0.2]]))
+    with pytest.raises(S.UmcpError):
+        g.point_eval(r, irr, np.array([[0.9, 0.3]]))
+    # the surface still answers afterwards
+    R, _ = g.point_eval(r, irr, np.array([[0.2, 0.2]]))
+    Ro, _ = o.point_eval(r, irr, np.array([[0.2, 0.2]]))
+    assert rel(R, Ro) <= 1e-12

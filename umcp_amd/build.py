@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libumcp_core.so")
 SOURCES = ["umcp_kernels.hip", "surface_host.cpp", "effm_host.cpp", "umcp_api.cpp", "umcp_ensemble.cpp",
-           "umcp_server.cpp", "umcp_io.cpp"]
+           "umcp_server.cpp", "umcp_io.cpp", "umcp_points.cpp"]
 ARCH = os.environ.get("UMCP_OFFLOAD_ARCH", "gfx950")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 

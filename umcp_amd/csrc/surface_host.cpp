@@ -9,6 +9,8 @@
 //   irregular      src/interpLib.C:5011-5161 (Atot), 5687-5809 (sub-patch, A^(d-1))
 #include "surface_host.hpp"
 
+#include "box_spline.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -204,42 +206,16 @@ int orient_mesh(Topology &T, std::string &err)
 
 namespace {
 
-// quartic barycentric monomials u^a v^b w^(4-a-b), indexed by (a, b)
-constexpr int kMono = 15;
-inline int mono_index(int a, int b) { return a * (11 - a) / 2 + b; }  // a in 0..4, b in 0..4-a
-
 struct BoxSpline {
     double coef[12][kMono];  // times 12
 };
-
-// The twelve basis functions n1..n12 as {a, b, coefficient} (w-power implied).
-const int kBoxTerms[12][16][3] = {
-    {{4, 0, 1}, {3, 1, 2}},
-    {{4, 0, 1}, {3, 0, 2}},
-    {{4, 0, 1}, {3, 0, 2}, {3, 1, 6}, {2, 1, 6}, {2, 2, 12}, {1, 2, 6}, {1, 3, 6}, {0, 3, 2}, {0, 4, 1}},
-    {{4, 0, 6}, {3, 0, 24}, {2, 0, 24}, {1, 0, 8}, {0, 0, 1}, {3, 1, 24}, {2, 1, 60}, {1, 1, 36},
-     {0, 1, 6}, {2, 2, 24}, {1, 2, 36}, {0, 2, 12}, {1, 3, 8}, {0, 3, 6}, {0, 4, 1}},
-    {{4, 0, 1}, {3, 0, 6}, {2, 0, 12}, {1, 0, 6}, {0, 0, 1}, {3, 1, 2}, {2, 1, 6}, {1, 1, 6}, {0, 1, 2}},
-    {{1, 3, 2}, {0, 4, 1}},
-    {{4, 0, 1}, {3, 0, 6}, {2, 0, 12}, {1, 0, 6}, {0, 0, 1}, {3, 1, 8}, {2, 1, 36}, {1, 1, 36},
-     {0, 1, 8}, {2, 2, 24}, {1, 2, 60}, {0, 2, 24}, {1, 3, 24}, {0, 3, 24}, {0, 4, 6}},
-    {{4, 0, 1}, {3, 0, 8}, {2, 0, 24}, {1, 0, 24}, {0, 0, 6}, {3, 1, 6}, {2, 1, 36}, {1, 1, 60},
-     {0, 1, 24}, {2, 2, 12}, {1, 2, 36}, {0, 2, 24}, {1, 3, 6}, {0, 3, 8}, {0, 4, 1}},
-    {{1, 0, 2}, {0, 0, 1}},
-    {{0, 3, 2}, {0, 4, 1}},
-    {{1, 0, 2}, {0, 0, 1}, {1, 1, 6}, {0, 1, 6}, {1, 2, 6}, {0, 2, 12}, {1, 3, 2}, {0, 3, 6}, {0, 4, 1}},
-    {{0, 0, 1}, {0, 1, 2}},
-};
-// control-point slot y carries basis n_(kSlotBasis[y])
-const int kSlotBasis[12] = {8, 7, 4, 5, 9, 12, 11, 10, 6, 3, 1, 2};
 
 const BoxSpline &box_spline()
 {
     static BoxSpline B = [] {
         BoxSpline b{};
         for (int n = 0; n < 12; n++)
-            for (int t = 0; t < 16 && kBoxTerms[n][t][2]; t++)
-                b.coef[n][mono_index(kBoxTerms[n][t][0], kBoxTerms[n][t][1])] += kBoxTerms[n][t][2];
+            for (int m = 0; m < kMono; m++) b.coef[n][m] = kBoxTable.c[n][m];
         return b;
     }();
     return B;
@@ -387,6 +363,38 @@ int irregular_patch_table(int val, double fu, double fv, double *out)
             }
             out[d * N + x] = s * sc[d];
         }
+    return 0;
+}
+
+int irregular_domain_maps(int val, int ndomains, double *out)
+{
+    if (val < 3 || val > kMaxValence || ndomains < 1) return -1;
+    const int N = val + 6;
+    std::vector<double> A;
+    loop_stencil(val, A);
+    // rows of Atot picked by the three sub-patches (the slot tables of
+    // irregular_patch_table above)
+    const int pcs[3][12] = {
+        {2, val + 3, val + 4, val + 5, 3, 0, 1, val + 2, val + 8, val + 9, val + 10, val + 11},
+        {1, val + 2, val + 3, 2, 0, val, val + 1, val + 6, val + 7, val + 8, val + 9, val + 4},
+        {val + 3, 2, 1, val + 2, val + 8, val + 9, val + 4, val + 5, 3, 0, val, val + 1}};
+    std::vector<double> P((size_t)N * N, 0.0), tmp((size_t)N * N);
+    for (int i = 0; i < N; i++) P[(size_t)i * N + i] = 1.0;
+    for (int d = 1; d <= ndomains; d++) {
+        if (d > 1) {  // P = A^(d-1), one more factor of the top N x N block
+            matmul(N, A.data(), P.data(), tmp.data());
+            P.swap(tmp);
+        }
+        for (int sub = 0; sub < 3; sub++) {
+            double *o = out + ((size_t)(d - 1) * 3 + sub) * 12 * N;
+            for (int y = 0; y < 12; y++)
+                for (int x = 0; x < N; x++) {
+                    double ax = 0.0;
+                    for (int e = 0; e < N; e++) ax += A[(size_t)pcs[sub][y] * N + e] * P[(size_t)e * N + x];
+                    o[y * N + x] = ax;
+                }
+        }
+    }
     return 0;
 }
 

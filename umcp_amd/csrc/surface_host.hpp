@@ -73,6 +73,12 @@ int effm_mass_csr(const Topology &T, const Plan &P, double mass_per_pt, std::vec
 void box_spline_basis(double u, double v, double *out);
 // Stam evaluation of an irregular patch of valence val at (u, v): out[d*(val+6)+x].
 int irregular_patch_table(int val, double u, double v, double *out);
+// Sub-patch maps of a valence-val irregular face for levels 1..ndomains, the
+// tables irr_kernel::setup precomputes (src/irr_kernel.C:10-260):
+// out[((d - 1) * 3 + sub) * 12 * (val + 6) + y * (val + 6) + x] is the weight
+// of control point x in box-spline slot y of sub-patch sub at level d, i.e.
+// row pc_sub[y] of Atot A^(d-1).
+int irregular_domain_maps(int val, int ndomains, double *out);
 // The 16 (u, v, weight) points of an irregular face of valence val.
 int irregular_points(int val, double pts[kIrrPts][3]);
 

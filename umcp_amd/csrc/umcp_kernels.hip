@@ -14,6 +14,7 @@
 #include <cstdint>
 
 #include "umcp_kernels.hpp"
+#include "box_spline.hpp"
 
 namespace umcp {
 
@@ -1678,6 +1679,374 @@ hipError_t launch_noise(unsigned long long seed, int step, int n, double *out, h
 {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(noise_kernel, dim3((n + kGatherBlock - 1) / kGatherBlock), dim3(kGatherBlock), 0, st, seed, step, n, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// points on the limit surface
+// ---------------------------------------------------------------------------
+
+// The 12 box-spline basis functions and their derivatives d = value, u, v,
+// uu, uv, vv at (u, v) in slot order: B[d][y] = 1/12 of the sum over the 15
+// monomials u^a v^b w^c (c = 4 - a - b, w = 1 - u - v) of the basis's
+// integer coefficient times the monomial's derivative d.  The coefficients
+// are compile-time constants: the zero terms fold away.
+__device__ __forceinline__ void point_basis(double u, double v, double B[6][12])
+{
+    const double w = 1.0 - u - v;
+    double pu[5], pv[5], pw[5];
+    pu[0] = pv[0] = pw[0] = 1.0;
+#pragma unroll
+    for (int i = 1; i < 5; i++) {
+        pu[i] = pu[i - 1] * u;
+        pv[i] = pv[i - 1] * v;
+        pw[i] = pw[i - 1] * w;
+    }
+#pragma unroll
+    for (int d = 0; d < 6; d++)
+#pragma unroll
+        for (int y = 0; y < 12; y++) B[d][y] = 0.0;
+#pragma unroll
+    for (int a = 0; a <= 4; a++)
+#pragma unroll
+        for (int b = 0; a + b <= 4; b++) {
+            const int c = 4 - a - b, m = mono_index(a, b);
+            auto M = [&](int pa, int pb, int pc) -> double {
+                return (pa < 0 || pb < 0 || pc < 0) ? 0.0 : pu[pa < 0 ? 0 : pa] * pv[pb < 0 ? 0 : pb] * pw[pc < 0 ? 0 : pc];
+            };
+            double md[6];
+            md[0] = M(a, b, c);
+            md[1] = a * M(a - 1, b, c) - c * M(a, b, c - 1);
+            md[2] = b * M(a, b - 1, c) - c * M(a, b, c - 1);
+            md[3] = a * (a - 1) * M(a - 2, b, c) - 2.0 * a * c * M(a - 1, b, c - 1) + c * (c - 1) * M(a, b, c - 2);
+            md[4] = a * b * M(a - 1, b - 1, c) - a * c * M(a - 1, b, c - 1) - b * c * M(a, b - 1, c - 1) +
+                    c * (c - 1) * M(a, b, c - 2);
+            md[5] = b * (b - 1) * M(a, b - 2, c) - 2.0 * b * c * M(a, b - 1, c - 1) + c * (c - 1) * M(a, b, c - 2);
+#pragma unroll
+            for (int y = 0; y < 12; y++) {
+                const double cf = kBoxTable.c[kSlotBasis[y] - 1][m];
+                if (cf != 0.0) {
+#pragma unroll
+                    for (int d = 0; d < 6; d++) B[d][y] += cf * md[d];
+                }
+            }
+        }
+#pragma unroll
+    for (int d = 0; d < 6; d++)
+#pragma unroll
+        for (int y = 0; y < 12; y++) B[d][y] /= 12.0;
+}
+
+__device__ __forceinline__ void cross_d(const double a[3], const double b[3], double c[3])
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// weights of irregular control point x for the first ND derivatives
+template <int ND>
+__device__ __forceinline__ void irr_weights(const double *__restrict__ map, int ncp, int x, const double B[6][12],
+                                            const double sc[6], double t[ND])
+{
+#pragma unroll
+    for (int d = 0; d < ND; d++) t[d] = 0.0;
+#pragma unroll
+    for (int y = 0; y < 12; y++) {
+        const double mw = map[y * ncp + x];
+#pragma unroll
+        for (int d = 0; d < ND; d++) t[d] += mw * B[d][y];
+    }
+#pragma unroll
+    for (int d = 0; d < ND; d++) t[d] *= sc[d];
+}
+
+// One point of evaluateRNRM or pointGradient.  IRR: the control-point weights
+// of an irregular face are the sub-patch map applied to the basis, t[d][x] =
+// s_d sum_y map[y][x] B[d][y]; on regular faces t[d][x] = B[d][x].
+template <bool IRR>
+__device__ void point_body(const PointArgs &a, int p, const PointRec &pr, const PointClass &C)
+{
+    const int nf = C.nf, ncp = IRR ? C.ncp : 12, fc = pr.fc;
+    const bool shifted = C.has_img[fc] != 0;
+    const double al[3] = {a.r[3 * a.nv], a.r[3 * a.nv + 1], a.r[3 * a.nv + 2]};
+    auto load_q = [&](int x, double q[3]) {
+        const int v = C.cp[x * nf + fc];
+        q[0] = a.r[3 * v];
+        q[1] = a.r[3 * v + 1];
+        q[2] = a.r[3 * v + 2];
+        if (shifted) {
+            double o[3];
+            image_offset(C.img[x * nf + fc], a.pbc, o);
+            q[0] += o[0];
+            q[1] += o[1];
+            q[2] += o[2];
+        }
+    };
+    double B[6][12];
+    point_basis(pr.u, pr.v, B);
+    const double sc[6] = {1.0, pr.su, pr.sv, pr.su * pr.su, pr.su * pr.sv, pr.sv * pr.sv};
+    const double *map = IRR ? C.map + (size_t)pr.map * 12 * ncp : nullptr;
+    (void)map;
+    (void)sc;
+    // the reference forms the regular gradient's surface from alpha-scaled
+    // positions, the irregular one and the evaluation's tangents from
+    // unscaled ones (src/grad.C:5822-5850, 6185-6220; src/interpLib.C:8140-8196)
+    const bool scale_q = a.grad && !IRR;
+    double X[6][3];
+#pragma unroll
+    for (int d = 0; d < 6; d++) X[d][0] = X[d][1] = X[d][2] = 0.0;
+    auto accumulate = [&](int x, const double t[6]) {
+        double q[3];
+        load_q(x, q);
+        if (scale_q) {
+            q[0] *= al[0];
+            q[1] *= al[1];
+            q[2] *= al[2];
+        }
+#pragma unroll
+        for (int d = 0; d < 6; d++) {
+            X[d][0] += t[d] * q[0];
+            X[d][1] += t[d] * q[1];
+            X[d][2] += t[d] * q[2];
+        }
+    };
+    if constexpr (IRR) {
+        for (int x = 0; x < ncp; x++) {
+            double t[6];
+            irr_weights<6>(map, ncp, x, B, sc, t);
+            accumulate(x, t);
+        }
+    } else {
+#pragma unroll
+        for (int x = 0; x < 12; x++) {
+            const double t[6] = {B[0][x], B[1][x], B[2][x], B[3][x], B[4][x], B[5][x]};
+            accumulate(x, t);
+        }
+    }
+    if (!a.grad) {
+        double n[3];
+        cross_d(X[1], X[2], n);
+        const double ln = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        // a zero normal stays zero, as the reference's normalize leaves it
+        const double inv = ln > 0.0 ? 1.0 / ln : 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            a.R[3 * p + k] = X[0][k] * al[k];
+            a.nrm[3 * p + k] = n[k] * inv;
+        }
+        return;
+    }
+    const double *Ru = X[1], *Rv = X[2], *Suu = X[3], *Suv = X[4], *Svv = X[5];
+    const double RuRu = Ru[0] * Ru[0] + Ru[1] * Ru[1] + Ru[2] * Ru[2];
+    const double RuRv = Ru[0] * Rv[0] + Ru[1] * Rv[1] + Ru[2] * Rv[2];
+    const double RvRv = Rv[0] * Rv[0] + Rv[1] * Rv[1] + Rv[2] * Rv[2];
+    const double g = sqrt(RuRu * RvRv - RuRv * RuRv);
+    double n[3];
+    cross_d(Ru, Rv, n);
+    const double ln = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]), iln = 1.0 / ln;
+    const double nh[3] = {n[0] * iln, n[1] * iln, n[2] * iln};
+    // d nrm / d Ru . s = P (s x Rv) / |n|, d nrm / d Rv . s = P (Ru x s) / |n|
+    // with P = I - nh nh^T; the transposes give dE/dRu = (Rv x m) / |n| and
+    // dE/dRv = (m x Ru) / |n| for m = P de_dnrm
+    auto project = [&](double v[3]) {
+        const double d = nh[0] * v[0] + nh[1] * v[1] + nh[2] * v[2];
+        v[0] = (v[0] - nh[0] * d) * iln;
+        v[1] = (v[1] - nh[1] * d) * iln;
+        v[2] = (v[2] - nh[2] * d) * iln;
+    };
+    const double de_dr[3] = {a.de_dr[3 * p], a.de_dr[3 * p + 1], a.de_dr[3 * p + 2]};
+    const double de_dn[3] = {a.de_dnrm[3 * p], a.de_dnrm[3 * p + 1], a.de_dnrm[3 * p + 2]};
+    const double frac = a.frac ? a.frac[p] : 1.0;
+    double mvec[3] = {de_dn[0], de_dn[1], de_dn[2]};
+    project(mvec);
+    double de_ru[3], de_rv[3];
+    cross_d(Rv, mvec, de_ru);
+    cross_d(mvec, Ru, de_rv);
+    double Nu[3], Nv[3], t1[3], t2[3];
+    cross_d(Suu, Rv, t1);
+    cross_d(Ru, Suv, t2);
+#pragma unroll
+    for (int k = 0; k < 3; k++) Nu[k] = t1[k] + t2[k];
+    project(Nu);
+    cross_d(Suv, Rv, t1);
+    cross_d(Ru, Svv, t2);
+#pragma unroll
+    for (int k = 0; k < 3; k++) Nv[k] = t1[k] + t2[k];
+    project(Nv);
+    a.gpuv[2 * p] = (de_dr[0] * Ru[0] + de_dr[1] * Ru[1] + de_dr[2] * Ru[2]) +
+                    (de_dn[0] * Nu[0] + de_dn[1] * Nu[1] + de_dn[2] * Nu[2]);
+    a.gpuv[2 * p + 1] = (de_dr[0] * Rv[0] + de_dr[1] * Rv[1] + de_dr[2] * Rv[2]) +
+                        (de_dn[0] * Nv[0] + de_dn[1] * Nv[1] + de_dn[2] * Nv[2]);
+    // per-slot coefficients of the weights t0, t1, t2:
+    //   dE/dr_x = frac de_dr t0 + dE/dRu t1 + dE/dRv t2 - on_surface kT / g (dg/dRu t1 + dg/dRv t2)
+    const double os = a.on_surface ? a.kT / g : 0.0;
+    const double ig = 1.0 / g;
+    double c0[3], c1[3], c2[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double dg_ru = (RvRv * Ru[k] - RuRv * Rv[k]) * ig;
+        const double dg_rv = (RuRu * Rv[k] - RuRv * Ru[k]) * ig;
+        c0[k] = frac * de_dr[k];
+        c1[k] = frac * de_ru[k] - os * dg_ru;
+        c2[k] = frac * de_rv[k] - os * dg_rv;
+    }
+    double ga[3] = {0.0, 0.0, 0.0};
+    double *out = a.contrib + (size_t)p * kPointSlots * 3;
+    auto emit = [&](int x, const double t[3]) {
+        double q[3];
+        load_q(x, q);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const double dedr = c0[k] * t[0] + c1[k] * t[1] + c2[k] * t[2];
+            out[3 * x + k] = dedr * al[k];
+            ga[k] += dedr * q[k];
+        }
+    };
+    if constexpr (IRR) {
+        for (int x = 0; x < ncp; x++) {
+            double t[3];
+            irr_weights<3>(map, ncp, x, B, sc, t);
+            emit(x, t);
+        }
+    } else {
+#pragma unroll
+        for (int x = 0; x < 12; x++) {
+            const double t[3] = {B[0][x], B[1][x], B[2][x]};
+            emit(x, t);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) a.alpha[3 * p + k] = ga[k];
+}
+
+// one thread per point of one kind (idx: the batch's points of that kind)
+template <int KIND>
+__global__ __launch_bounds__(kPointBlock) void point_kernel(PointArgs a, const int *__restrict__ idx, int count)
+{
+    const int j = blockIdx.x * kPointBlock + threadIdx.x;
+    if (j >= count) return;
+    const int p = idx[j];
+    const PointRec pr = a.rec[p];
+    const PointClass C = a.cls[pr.cls];
+    if constexpr (KIND == kPointRegular) {
+        point_body<false>(a, p, pr, C);
+    } else if constexpr (KIND == kPointIrregular) {
+        point_body<true>(a, p, pr, C);
+    } else {
+        // the extraordinary vertex (evaluation only): 1/2 on the vertex and
+        // 1/(2 val) on each ring neighbour at its image next to the vertex,
+        // normal unknown (src/interpLib.C:7963-8001)
+        const int nf = C.nf, fc = pr.fc, val = C.valence;
+        const bool shifted = C.has_img[fc] != 0;
+        auto load_q = [&](int x, double q[3]) {
+            const int v = C.cp[x * nf + fc];
+            q[0] = a.r[3 * v];
+            q[1] = a.r[3 * v + 1];
+            q[2] = a.r[3 * v + 2];
+            if (shifted) {
+                double o[3];
+                image_offset(C.img[x * nf + fc], a.pbc, o);
+                q[0] += o[0];
+                q[1] += o[1];
+                q[2] += o[2];
+            }
+        };
+        const int i = C.cp[fc];
+        const double ri[3] = {a.r[3 * i], a.r[3 * i + 1], a.r[3 * i + 2]};
+        double q0[3];
+        load_q(0, q0);
+        const double w = 1.0 / (val * 2);
+        double lr[3] = {0.5 * ri[0], 0.5 * ri[1], 0.5 * ri[2]};
+        for (int x = 1; x <= val; x++) {
+            double q[3];
+            load_q(x, q);
+#pragma unroll
+            for (int k = 0; k < 3; k++) lr[k] += ((q[k] - q0[k]) + ri[k]) * w;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            a.R[3 * p + k] = lr[k] * a.r[3 * a.nv + k];
+            a.nrm[3 * p + k] = __builtin_nan("");
+        }
+    }
+}
+
+__global__ __launch_bounds__(kPointBlock) void point_reduce_kernel(int nt, const int *__restrict__ start,
+                                                                    const int *__restrict__ entries,
+                                                                    const double *__restrict__ contrib,
+                                                                    double *__restrict__ out)
+{
+    const int t = blockIdx.x * kPointBlock + threadIdx.x;
+    if (t >= nt) return;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int e = start[t]; e < start[t + 1]; e++) {
+        const double *c = contrib + (size_t)entries[e] * 3;
+        s0 += c[0];
+        s1 += c[1];
+        s2 += c[2];
+    }
+    out[3 * t] = s0;
+    out[3 * t + 1] = s1;
+    out[3 * t + 2] = s2;
+}
+
+constexpr int kPointAlphaBlock = 256;
+
+__global__ __launch_bounds__(kPointAlphaBlock) void point_alpha_kernel(int n, const double *__restrict__ alpha,
+                                                                       double *__restrict__ out)
+{
+    __shared__ double sh[3][kPointAlphaBlock];
+    const int t = threadIdx.x;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int p = t; p < n; p += kPointAlphaBlock) {
+        s0 += alpha[3 * p];
+        s1 += alpha[3 * p + 1];
+        s2 += alpha[3 * p + 2];
+    }
+    sh[0][t] = s0;
+    sh[1][t] = s1;
+    sh[2][t] = s2;
+    __syncthreads();
+    for (int h = kPointAlphaBlock / 2; h > 0; h >>= 1) {
+        if (t < h) {
+            sh[0][t] += sh[0][t + h];
+            sh[1][t] += sh[1][t + h];
+            sh[2][t] += sh[2][t + h];
+        }
+        __syncthreads();
+    }
+    if (t < 3) out[t] = sh[t][0];
+}
+
+hipError_t launch_points(const PointArgs &a, const int *idx, const int counts[3], hipStream_t st)
+{
+    const int *ix[3] = {idx, idx + counts[0], idx + counts[0] + counts[1]};
+    for (int k = 0; k < 3; k++) {
+        if (counts[k] <= 0) continue;
+        const dim3 grid((counts[k] + kPointBlock - 1) / kPointBlock);
+        if (k == kPointRegular)
+            hipLaunchKernelGGL(point_kernel<kPointRegular>, grid, dim3(kPointBlock), 0, st, a, ix[k], counts[k]);
+        else if (k == kPointIrregular)
+            hipLaunchKernelGGL(point_kernel<kPointIrregular>, grid, dim3(kPointBlock), 0, st, a, ix[k], counts[k]);
+        else
+            hipLaunchKernelGGL(point_kernel<kPointVertex>, grid, dim3(kPointBlock), 0, st, a, ix[k], counts[k]);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_point_reduce(int nt, const int *start, const int *entries, const double *contrib, int n,
+                               const double *alpha, double *out, hipStream_t st)
+{
+    if (nt > 0) {
+        hipLaunchKernelGGL(point_reduce_kernel, dim3((nt + kPointBlock - 1) / kPointBlock), dim3(kPointBlock), 0,
+                           st, nt, start, entries, contrib, out);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(point_alpha_kernel, dim3(1), dim3(kPointAlphaBlock), 0, st, n, alpha, out + 3 * (size_t)nt);
     return hipGetLastError();
 }
 

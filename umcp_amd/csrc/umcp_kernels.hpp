@@ -141,6 +141,58 @@ hipError_t launch_effm_truncate(int n, double *A, double rel_tol, int *rowcount,
 // per-row counts), columns ascending
 hipError_t launch_effm_compact(int n, const double *A, const int *rowptr, int *col, double *val, hipStream_t st);
 
+// ---------------------------------------------------------------------------
+// points on the limit surface: surface::evaluateRNRM (src/interpLib.C:
+// 7936-8196) and surface::pointGradient (src/grad.C:5697-6443)
+// ---------------------------------------------------------------------------
+constexpr int kPointDomains = 100;  // sub-patch levels held (MAX_DOMAIN, src/interp.h:20)
+constexpr int kPointSlots = 18;     // control points per face at the largest valence
+constexpr int kPointBlock = 64;
+enum : int { kPointRegular = 0, kPointIrregular = 1, kPointVertex = 2 };
+
+// One point as the host resolves it: the class of its face and the face's
+// index there, the kind (regular face, irregular sub-patch, or the
+// extraordinary vertex itself), the sub-patch map ((level - 1) * 3 +
+// sub-patch) and the coordinates the basis is evaluated at, with the
+// derivative scale of the sub-patch (d local / d face parameter; 1 on
+// regular faces).
+struct PointRec {
+    int cls, fc, kind, map;
+    double u, v, su, sv;
+};
+
+struct PointClass {
+    const int *cp;           // [ncp][nf]
+    const uint8_t *img;      // [ncp][nf]
+    const uint8_t *has_img;  // [nf]
+    const double *map;       // irregular: [kPointDomains * 3][12][ncp]
+    int nf, ncp, valence, pad;
+};
+
+struct PointArgs {
+    const double *r;         // 3 nv + 3, alpha last
+    int nv, n;
+    int grad;                // 0: position and normal, 1: gradient
+    int on_surface;
+    double kT;
+    double pbc[3][3];
+    const PointRec *rec;     // [n]
+    const PointClass *cls;
+    const double *de_dr, *de_dnrm, *frac;  // [n][3], [n][3], [n] (frac null: 1)
+    double *R, *nrm;         // position and unit normal [n][3]
+    double *contrib;         // vertex gradient per (point, slot) [n][kPointSlots][3]
+    double *alpha;           // alpha gradient per point [n][3]
+    double *gpuv;            // (u, v) gradient per point [n][2]
+};
+// idx: the indices of the regular, irregular and extraordinary-vertex points
+// in that order, counts[kind] of each (the vertex kind: evaluation only)
+hipError_t launch_points(const PointArgs &a, const int *idx, const int counts[3], hipStream_t st);
+// out[t][k] = sum of contrib[entries[e]][k] over e in [start[t], start[t+1]),
+// in that order, for the nt touched vertices; out[nt][k] = the sum of the n
+// points' alpha gradients in a fixed tree order
+hipError_t launch_point_reduce(int nt, const int *start, const int *entries, const double *contrib, int n,
+                               const double *alpha, double *out, hipStream_t st);
+
 struct GatherItem {
     int surf, vblock;
 };

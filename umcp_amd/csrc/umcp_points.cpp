@@ -1,0 +1,398 @@
+// Points on the limit surface: batched surface::evaluateRNRM
+// (src/interpLib.C:7936-8196) and surface::pointGradient (src/grad.C:
+// 5697-6443), the calls through which the reference's particle complexes
+// couple to the membrane (src/pcomplex.C:501, src/bd.C:1011,
+// src/minimize.C:233, src/p_p.C:336-357).
+//
+// The host resolves each point (face -> class, sub-patch level and local
+// coordinates, with the reference's domain rules and rejections) and builds
+// the vertex -> (point, slot) incidence of the batch; the device evaluates one
+// point per thread and sums each touched vertex's contributions in point
+// order, so results do not depend on the launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "umcp_state.hpp"
+
+using namespace umcp;
+
+namespace {
+
+constexpr double kThresh = 1e-4;   // src/interpLib.C:24
+
+int pfail(umcp_surface *s, int code, const std::string &msg)
+{
+    s->err = msg;
+    return code;
+}
+
+int phip(umcp_surface *s, hipError_t e, const char *where)
+{
+    return pfail(s, UMCP_ERR_DEVICE, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+int palloc(umcp_surface *s, void **p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+    if (e != hipSuccess) return pfail(s, UMCP_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    s->allocations.push_back(*p);
+    s->device_bytes += bytes;
+    return UMCP_OK;
+}
+
+void pfree(umcp_surface *s, void *p, size_t bytes)
+{
+    if (!p) return;
+    auto it = std::find(s->allocations.begin(), s->allocations.end(), p);
+    if (it != s->allocations.end()) s->allocations.erase(it);
+    s->device_bytes -= bytes;
+    (void)hipFree(p);
+}
+
+// grow-only device buffer
+int pgrow(umcp_surface *s, char **buf, size_t *cap, size_t need)
+{
+    if (need <= *cap) return UMCP_OK;
+    (void)hipStreamSynchronize(s->stream);
+    pfree(s, *buf, *cap);
+    *buf = nullptr;
+    *cap = 0;
+    size_t want = std::max(need, (size_t)1 << 20);
+    void *p = nullptr;
+    int rc = palloc(s, &p, want);
+    if (rc) return rc;
+    *buf = (char *)p;
+    *cap = want;
+    return UMCP_OK;
+}
+
+size_t align_up(size_t x) { return (x + kArenaAlign - 1) & ~(kArenaAlign - 1); }
+
+// the per-surface tables: face -> (class, index), the class table and each
+// irregular class's sub-patch maps for levels 1..kPointDomains
+int point_setup(umcp_surface *s)
+{
+    if (s->pt_ready) return UMCP_OK;
+    const Plan &P = s->P;
+    s->pt_face_cls.assign((size_t)P.nfaces, -1);
+    s->pt_face_fc.assign((size_t)P.nfaces, -1);
+    for (size_t c = 0; c < P.classes.size(); c++)
+        for (int i = 0; i < P.classes[c].nf(); i++) {
+            s->pt_face_cls[(size_t)P.classes[c].face_id[i]] = (int)c;
+            s->pt_face_fc[(size_t)P.classes[c].face_id[i]] = i;
+        }
+    s->pt_vmark.assign((size_t)s->T.nv, -1);
+    // one buffer: [class table][maps...]
+    std::vector<PointClass> tab(P.classes.size());
+    std::vector<size_t> map_off(P.classes.size(), 0);
+    const size_t tab_off = 0;
+    size_t bytes = align_up(sizeof(PointClass) * tab.size());
+    for (size_t c = 0; c < P.classes.size(); c++)
+        if (P.classes[c].valence != 6) {
+            map_off[c] = bytes;
+            bytes += align_up(sizeof(double) * kPointDomains * 3 * 12 * (size_t)P.classes[c].ncp);
+        }
+    std::vector<char> host(bytes, 0);
+    void *dev = nullptr;
+    int rc = palloc(s, &dev, bytes);
+    if (rc) return rc;
+    char *d = (char *)dev;
+    for (size_t c = 0; c < P.classes.size(); c++) {
+        const FaceClass &C = P.classes[c];
+        PointClass &t = tab[c];
+        t.cp = s->cls[c].cp;
+        t.img = s->cls[c].img;
+        t.has_img = s->cls[c].has_img;
+        t.nf = C.nf();
+        t.ncp = C.ncp;
+        t.valence = C.valence;
+        t.map = nullptr;
+        if (C.valence != 6) {
+            if (irregular_domain_maps(C.valence, kPointDomains, (double *)(host.data() + map_off[c])) != 0)
+                return pfail(s, UMCP_ERR_BAD_MESH, "no sub-patch maps for this valence");
+            t.map = (const double *)(d + map_off[c]);
+        }
+    }
+    std::memcpy(host.data() + tab_off, tab.data(), sizeof(PointClass) * tab.size());
+    hipError_t e = hipMemcpyAsync(d, host.data(), bytes, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return phip(s, e, "upload point tables");
+    s->d_pt_cls = (PointClass *)(d + tab_off);
+    s->pt_ready = true;
+    return UMCP_OK;
+}
+
+// irr_kernel::domain / get_map / get_map_transform (src/irr_kernel.C:
+// 380-541) on the host, where the level is found with the same libm log10
+// the reference uses; the sub-patch arithmetic is exact (powers of two).
+// Returns 0 and fills rec, or a message for the inputs the reference stops on.
+const char *resolve_irregular(double fu, double fv, bool grad, PointRec &rec)
+{
+    double f_domain = -std::log10(fu + fv) / std::log10(2.0);
+    if (!(f_domain < 1e300) || !(f_domain > -1e300)) {
+        // the extraordinary vertex itself (src/interpLib.C:7963)
+        if (grad) return "no gradient at an irregular face's extraordinary vertex (the reference has no map there)";
+        if (!(fu < 1e-10 && fv < 1e-10)) return "(u, v) outside the face";
+        rec.kind = kPointVertex;
+        rec.map = 0;
+        rec.u = rec.v = 0.0;
+        rec.su = rec.sv = 1.0;
+        return nullptr;
+    }
+    if (fu + fv >= 1.0 - 1e-9) f_domain = 1;
+    const int domain = (int)std::lround(std::ceil(f_domain));
+    if (domain - 1 >= kPointDomains) return "requested sub-patch level beyond MAX_DOMAIN (100)";
+    const double pow2 = std::pow(2.0, domain - 1.0);
+    double su = pow2, sv = pow2;
+    fu *= pow2;
+    fv *= pow2;
+    int sub;
+    if (fu > 0.5) {
+        sub = 0;
+        fv = 2 * fv;
+        fu = 2.0 * fu - 1.0;
+        su *= 2.0;
+        sv *= 2.0;
+    } else if (fv > 0.5) {
+        sub = 1;
+        fu = 2 * fu;
+        fv = 2.0 * fv - 1.0;
+        su *= 2;
+        sv *= 2;
+    } else {
+        sub = 2;
+        fv = 1 - 2 * fv;
+        fu = 1 - 2 * fu;
+        su *= -2;
+        sv *= -2;
+    }
+    if (fu + fv > 1.0 + kThresh || fu < -kThresh || fv < -kThresh) return "(u, v) outside the face";
+    rec.kind = kPointIrregular;
+    rec.map = (domain - 1) * 3 + sub;
+    rec.u = fu;
+    rec.v = fv;
+    rec.su = su;
+    rec.sv = sv;
+    return nullptr;
+}
+
+struct Batch {
+    size_t off_r, off_rec, off_idx, off_dr, off_dn, off_frac, off_start, off_ent, in_bytes;
+    size_t off_R, off_N, off_con, off_al, off_gp, off_out, out_bytes;
+    std::vector<int> touched;   // touched vertices in first-touch order
+};
+
+// resolves the points, builds the vertex incidence, stages and launches
+int run_points(umcp_surface *s, const double *r, int n, const int *faces, const double *uv, const double *de_dr,
+               const double *de_dnrm, const double *frac, int on_surface, double kT, bool grad, Batch &b)
+{
+    const int nv = s->T.nv;
+    const Plan &P = s->P;
+    std::vector<PointRec> rec((size_t)n);
+    for (int i = 0; i < n; i++) {
+        const int f = faces[i];
+        if (f < 0 || f >= P.nfaces)
+            return pfail(s, UMCP_ERR_ARG, "point " + std::to_string(i) + ": face index out of range");
+        PointRec &q = rec[(size_t)i];
+        q.cls = s->pt_face_cls[(size_t)f];
+        q.fc = s->pt_face_fc[(size_t)f];
+        const double u = uv[2 * i], v = uv[2 * i + 1];
+        if (!std::isfinite(u) || !std::isfinite(v))
+            return pfail(s, UMCP_ERR_ARG, "point " + std::to_string(i) + ": non-finite (u, v)");
+        if (P.classes[(size_t)q.cls].valence == 6) {
+            q.kind = kPointRegular;
+            q.map = 0;
+            q.u = u;
+            q.v = v;
+            q.su = q.sv = 1.0;
+        } else if (const char *msg = resolve_irregular(u, v, grad, q)) {
+            return pfail(s, UMCP_ERR_ARG, "point " + std::to_string(i) + ": " + msg);
+        }
+    }
+    // the points of each kind, for one launch per kind
+    int counts[3] = {0, 0, 0};
+    for (int i = 0; i < n; i++) counts[rec[(size_t)i].kind]++;
+    std::vector<int> idx((size_t)n);
+    {
+        int fill[3] = {0, counts[0], counts[0] + counts[1]};
+        for (int i = 0; i < n; i++) idx[(size_t)fill[rec[(size_t)i].kind]++] = i;
+    }
+    // vertex -> (point, slot) incidence in point order (gradient only)
+    std::vector<int> count, start, ent;
+    b.touched.clear();
+    if (grad) {
+        for (int i = 0; i < n; i++) {
+            const FaceClass &C = P.classes[(size_t)rec[(size_t)i].cls];
+            const int nf = C.nf(), fc = rec[(size_t)i].fc;
+            for (int x = 0; x < C.ncp; x++) {
+                const int v = C.cp[(size_t)x * nf + fc];
+                int &t = s->pt_vmark[(size_t)v];
+                if (t < 0) {
+                    t = (int)b.touched.size();
+                    b.touched.push_back(v);
+                    count.push_back(0);
+                }
+                count[(size_t)t]++;
+            }
+        }
+        const int nt = (int)b.touched.size();
+        start.assign((size_t)nt + 1, 0);
+        for (int t = 0; t < nt; t++) start[(size_t)t + 1] = start[(size_t)t] + count[(size_t)t];
+        ent.assign((size_t)start[(size_t)nt], 0);
+        std::vector<int> fill(start.begin(), start.end() - 1);
+        for (int i = 0; i < n; i++) {
+            const FaceClass &C = P.classes[(size_t)rec[(size_t)i].cls];
+            const int nf = C.nf(), fc = rec[(size_t)i].fc;
+            for (int x = 0; x < C.ncp; x++) {
+                const int t = s->pt_vmark[(size_t)C.cp[(size_t)x * nf + fc]];
+                ent[(size_t)fill[(size_t)t]++] = i * kPointSlots + x;
+            }
+        }
+        for (int v : b.touched) s->pt_vmark[(size_t)v] = -1;
+    }
+    const int nt = (int)b.touched.size();
+    // staging layout
+    size_t o = 0;
+    b.off_r = o;      o = align_up(o + sizeof(double) * (3 * (size_t)nv + 3));
+    b.off_rec = o;    o = align_up(o + sizeof(PointRec) * (size_t)n);
+    b.off_idx = o;    o = align_up(o + sizeof(int) * (size_t)n);
+    b.off_dr = o;     o = align_up(o + (grad ? sizeof(double) * 3 * (size_t)n : 0));
+    b.off_dn = o;     o = align_up(o + (grad ? sizeof(double) * 3 * (size_t)n : 0));
+    b.off_frac = o;   o = align_up(o + (grad && frac ? sizeof(double) * (size_t)n : 0));
+    b.off_start = o;  o = align_up(o + sizeof(int) * (start.size()));
+    b.off_ent = o;    o = align_up(o + sizeof(int) * ent.size());
+    b.in_bytes = o;
+    o = 0;
+    if (grad) {
+        // [gpuv 2n][touched sums 3 nt][alpha 3] first: one download
+        b.off_gp = o;
+        b.off_out = o + sizeof(double) * 2 * (size_t)n;
+        o = align_up(b.off_out + sizeof(double) * (3 * (size_t)nt + 3));
+        b.off_con = o;  o = align_up(o + sizeof(double) * (size_t)n * kPointSlots * 3);
+        b.off_al = o;   o = align_up(o + sizeof(double) * 3 * (size_t)n);
+        b.off_R = b.off_N = 0;
+    } else {
+        b.off_R = o;
+        b.off_N = o + sizeof(double) * 3 * (size_t)n;
+        o = align_up(b.off_N + sizeof(double) * 3 * (size_t)n);
+        b.off_gp = b.off_out = b.off_con = b.off_al = 0;
+    }
+    b.out_bytes = o;
+    int rc = pgrow(s, &s->d_pt_in, &s->pt_in_cap, b.in_bytes);
+    if (rc == UMCP_OK) rc = pgrow(s, &s->d_pt_out, &s->pt_out_cap, b.out_bytes);
+    if (rc) return rc;
+    if (s->pt_stage.size() < b.in_bytes) s->pt_stage.resize(b.in_bytes);
+    char *h = s->pt_stage.data();
+    std::memcpy(h + b.off_r, r, sizeof(double) * (3 * (size_t)nv + 3));
+    std::memcpy(h + b.off_rec, rec.data(), sizeof(PointRec) * (size_t)n);
+    std::memcpy(h + b.off_idx, idx.data(), sizeof(int) * (size_t)n);
+    if (grad) {
+        std::memcpy(h + b.off_dr, de_dr, sizeof(double) * 3 * (size_t)n);
+        std::memcpy(h + b.off_dn, de_dnrm, sizeof(double) * 3 * (size_t)n);
+        if (frac) std::memcpy(h + b.off_frac, frac, sizeof(double) * (size_t)n);
+        std::memcpy(h + b.off_start, start.data(), sizeof(int) * start.size());
+        std::memcpy(h + b.off_ent, ent.data(), sizeof(int) * ent.size());
+    }
+    hipError_t e = hipMemcpyAsync(s->d_pt_in, h, b.in_bytes, hipMemcpyHostToDevice, s->stream);
+    if (e != hipSuccess) return phip(s, e, "upload points");
+    PointArgs a{};
+    char *di = s->d_pt_in, *dout = s->d_pt_out;
+    a.r = (const double *)(di + b.off_r);
+    a.nv = nv;
+    a.n = n;
+    a.grad = grad ? 1 : 0;
+    a.on_surface = on_surface ? 1 : 0;
+    a.kT = kT;
+    std::memcpy(a.pbc, s->T.pbc, sizeof a.pbc);
+    a.rec = (const PointRec *)(di + b.off_rec);
+    a.cls = s->d_pt_cls;
+    if (grad) {
+        a.de_dr = (const double *)(di + b.off_dr);
+        a.de_dnrm = (const double *)(di + b.off_dn);
+        a.frac = frac ? (const double *)(di + b.off_frac) : nullptr;
+        a.contrib = (double *)(dout + b.off_con);
+        a.alpha = (double *)(dout + b.off_al);
+        a.gpuv = (double *)(dout + b.off_gp);
+    } else {
+        a.R = (double *)(dout + b.off_R);
+        a.nrm = (double *)(dout + b.off_N);
+    }
+    e = launch_points(a, (const int *)(di + b.off_idx), counts, s->stream);
+    if (e != hipSuccess) return phip(s, e, "point kernel launch");
+    if (grad) {
+        e = launch_point_reduce(nt, (const int *)(di + b.off_start), (const int *)(di + b.off_ent), a.contrib, n,
+                                a.alpha, (double *)(dout + b.off_out), s->stream);
+        if (e != hipSuccess) return phip(s, e, "point reduce launch");
+    }
+    return UMCP_OK;
+}
+
+int point_check(umcp_surface *s, const double *r, int n, const int *faces, const double *uv)
+{
+    if (!s) return UMCP_ERR_ARG;
+    if (!s->planned) return pfail(s, UMCP_ERR_STATE, "plan has not been generated (call umcp_plan)");
+    if (n < 0 || (n > 0 && (!r || !faces || !uv))) return pfail(s, UMCP_ERR_ARG, "null point arrays");
+    if (n > UMCP_POINT_MAX_BATCH) return pfail(s, UMCP_ERR_ARG, "more points than UMCP_POINT_MAX_BATCH");
+    hipError_t e = hipSetDevice(s->device);
+    if (e != hipSuccess) return phip(s, e, "hipSetDevice");
+    return point_setup(s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int umcp_point_eval(umcp_surface *s, const double *r, int n, const int *faces, const double *uv, double *R,
+                    double *nrm)
+{
+    int rc = point_check(s, r, n, faces, uv);
+    if (rc) return rc;
+    if (n == 0) return UMCP_OK;
+    if (!R || !nrm) return pfail(s, UMCP_ERR_ARG, "null output arrays");
+    Batch b;
+    rc = run_points(s, r, n, faces, uv, nullptr, nullptr, nullptr, 0, 0.0, false, b);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(R, s->d_pt_out + b.off_R, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost,
+                                  s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(nrm, s->d_pt_out + b.off_N, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost,
+                           s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return phip(s, e, "download points");
+    return UMCP_OK;
+}
+
+int umcp_point_grad(umcp_surface *s, const double *r, int n, const int *faces, const double *uv,
+                    const double *de_dr, const double *de_dnrm, const double *frac_mult, int on_surface, double kT,
+                    double *g, double *g_puv)
+{
+    int rc = point_check(s, r, n, faces, uv);
+    if (rc) return rc;
+    if (n == 0) return UMCP_OK;
+    if (!de_dr || !de_dnrm || !g || !g_puv) return pfail(s, UMCP_ERR_ARG, "null gradient arrays");
+    Batch b;
+    rc = run_points(s, r, n, faces, uv, de_dr, de_dnrm, frac_mult, on_surface, kT, true, b);
+    if (rc) return rc;
+    const int nt = (int)b.touched.size();
+    const size_t cnt = 2 * (size_t)n + 3 * (size_t)nt + 3;   // gpuv, touched sums, alpha: contiguous
+    if (s->pt_host.size() < cnt) s->pt_host.resize(cnt);
+    hipError_t e = hipMemcpyAsync(s->pt_host.data(), s->d_pt_out + b.off_gp, sizeof(double) * cnt,
+                                  hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return phip(s, e, "download point gradient");
+    const double *gp = s->pt_host.data(), *sums = gp + 2 * (size_t)n, *al = sums + 3 * (size_t)nt;
+    // the reference's contract: the caller's arrays accumulate (+=)
+    for (size_t i = 0; i < 2 * (size_t)n; i++) g_puv[i] += gp[i];
+    for (int t = 0; t < nt; t++)
+        for (int k = 0; k < 3; k++) g[3 * (size_t)b.touched[(size_t)t] + k] += sums[3 * (size_t)t + k];
+    const int nv = s->T.nv;
+    for (int k = 0; k < 3; k++) g[3 * (size_t)nv + k] += al[k];
+    return UMCP_OK;
+}
+
+}  // extern "C"

@@ -84,6 +84,17 @@ struct umcp_surface {
     double *d_pp_next = nullptr, *d_mp = nullptr;
     long effm_nnz = 0;
     std::vector<void *> effm_allocs;
+    // points on the limit surface (umcp_point_eval / umcp_point_grad): face ->
+    // (class, index in the class), a vertex mark for the batch incidence, the
+    // device tables (class table, sub-patch maps) and
+    // grow-only staging and result buffers
+    bool pt_ready = false;
+    std::vector<int> pt_face_cls, pt_face_fc, pt_vmark;
+    umcp::PointClass *d_pt_cls = nullptr;
+    char *d_pt_in = nullptr, *d_pt_out = nullptr;
+    size_t pt_in_cap = 0, pt_out_cap = 0;
+    std::vector<char> pt_stage;
+    std::vector<double> pt_host;
 };
 
 namespace umcp {

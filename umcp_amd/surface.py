@@ -43,6 +43,7 @@ EXPORTS = [
     "umcp_server_drain", "umcp_server_stats", "umcp_server_last_error", "umcp_server_destroy",
     "umcp_effm_build", "umcp_effm_get", "umcp_effm_steps", "umcp_effm_steps_dev",
     "umcp_save_mesh", "umcp_write_restart", "umcp_read_restart", "umcp_write_limit_surface",
+    "umcp_point_eval", "umcp_point_grad",
 ]
 
 
@@ -136,6 +137,8 @@ def load_library(path: str = LIB_PATH):
         "umcp_noise": ([C.c_ulonglong, C.c_int, C.c_int, vp, vp], C.c_int),
         "umcp_save_mesh": ([vp, dp, C.c_char_p], C.c_int),
         "umcp_write_limit_surface": ([vp, dp, C.c_char_p, C.c_int], C.c_int),
+        "umcp_point_eval": ([vp, dp, C.c_int, ip, dp, dp, dp], C.c_int),
+        "umcp_point_grad": ([vp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int, C.c_double, dp, dp], C.c_int),
         "umcp_write_restart": ([C.c_char_p, C.c_int, dp, dp, C.c_int], C.c_int),
         "umcp_read_restart": ([C.c_char_p, C.c_int, dp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
         "umcp_effm_build": ([vp, C.POINTER(C.c_long), dp], C.c_int),
@@ -174,6 +177,10 @@ def load_library(path: str = LIB_PATH):
 
 def _dp(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
 
 
 class Surface:
@@ -326,6 +333,37 @@ class Surface:
         rr = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
         self._check(self._L.umcp_write_limit_surface(self._h, _dp(rr) if rr is not None else None, path.encode(),
                                                      1 if append else 0), "write_limit_surface")
+
+    # points on the limit surface (src/interpLib.C:7936, src/grad.C:5697)
+    def point_eval(self, r: np.ndarray, faces, uv):
+        """(R, nrm), each [n, 3]: evaluateRNRM at the points (faces[i], uv[i])."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        f = np.ascontiguousarray(faces, dtype=np.int32)
+        uv = np.ascontiguousarray(uv, dtype=np.float64).reshape(-1, 2)
+        n = f.size
+        R, N = np.zeros((n, 3)), np.zeros((n, 3))
+        self._check(self._L.umcp_point_eval(self._h, _dp(r), n, _ip(f), _dp(uv), _dp(R), _dp(N)), "point_eval")
+        return R, N
+
+    def point_grad(self, r: np.ndarray, faces, uv, de_dr, de_dnrm, frac_mult=None, on_surface: bool = False,
+                   kT: float = 0.0, g: np.ndarray | None = None, g_puv: np.ndarray | None = None):
+        """(g, g_puv): pointGradient of every point added to g (3nv+3) and
+        g_puv [n, 2] (zeros when not given)."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        f = np.ascontiguousarray(faces, dtype=np.int32)
+        uv = np.ascontiguousarray(uv, dtype=np.float64).reshape(-1, 2)
+        n = f.size
+        a = np.ascontiguousarray(de_dr, dtype=np.float64).reshape(n, 3)
+        b = np.ascontiguousarray(de_dnrm, dtype=np.float64).reshape(n, 3)
+        fm = None if frac_mult is None else np.ascontiguousarray(frac_mult, dtype=np.float64).reshape(n)
+        g = np.zeros_like(r) if g is None else g
+        gp = np.zeros((n, 2)) if g_puv is None else g_puv
+        assert g.dtype == np.float64 and g.flags.c_contiguous and g.size == r.size
+        assert gp.dtype == np.float64 and gp.flags.c_contiguous and gp.size == 2 * n
+        self._check(self._L.umcp_point_grad(self._h, _dp(r), n, _ip(f), _dp(uv), _dp(a), _dp(b),
+                                            _dp(fm) if fm is not None else None, 1 if on_surface else 0,
+                                            float(kT), _dp(g), _dp(gp)), "point_grad")
+        return g, gp
 
     # momentum-space integrators with the effective-mass matrix (src/hd.C)
     def effm_build(self):
