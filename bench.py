@@ -346,6 +346,61 @@ def run_sheet_leg(args, dev, torch):
             "energy_finite": bool(np.isfinite(legs["fp64"]["energy_first_last"]).all())}
 
 
+def run_points_leg(args, dev, torch):
+    """Points on the limit surface (evaluateRNRM, pointGradient) batched on
+    the 40962-vertex vesicle: n random points (faces uniform, (u, v) uniform
+    in the triangle), wall time of the host-array calls (staging, the
+    batch's vertex incidence, kernels, results), and the serial oracle on a
+    bounded sample."""
+    import numpy as np
+    from umcp_amd import surface as S
+    from umcp_amd import workloads
+    w = workloads.vesicle_40962()
+    s = S.Surface(w.mesh_path)
+    s.set_params(precision=S.PREC_FP64, **w.params)
+    s.generatePlan()
+    n = args.points_n
+    rng = np.random.default_rng(21)
+    faces = rng.integers(0, s.nfaces, n).astype(np.int32)
+    u, v = rng.random(n), rng.random(n)
+    flip = u + v > 1
+    u[flip], v[flip] = 1 - u[flip], 1 - v[flip]
+    uv = np.stack([u, v], 1)
+    a, b = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+    r = w.r.copy()
+    out = {"workload": "vesicle_40962_points", "nv": s.nv, "faces": s.nfaces, "points": n, "precision": "fp64"}
+    s.point_eval(r, faces[:1000], uv[:1000])
+    s.point_grad(r, faces[:1000], uv[:1000], a[:1000], b[:1000])
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        R, N = s.point_eval(r, faces, uv)
+    te = (time.perf_counter() - t0) / reps
+    g = np.zeros_like(r)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g[:] = 0.0
+        s.point_grad(r, faces, uv, a, b, on_surface=True, kT=0.6, g=g)
+    tg = (time.perf_counter() - t0) / reps
+    out["eval_s"] = te
+    out["eval_points_per_s"] = n / te
+    out["grad_s"] = tg
+    out["grad_points_per_s"] = n / tg
+    out["finite"] = bool(np.isfinite(R).all() and np.isfinite(g).all())
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        o = O.OracleSurface(w.mesh_path)
+        o.set_params(**w.params)
+        m = min(n, 20000)
+        t0 = time.perf_counter()
+        o.point_grad(r, faces[:m], uv[:m], a[:m], b[:m], on_surface=1, kT=0.6)
+        tc = time.perf_counter() - t0
+        out["cpu_oracle_grad_points_per_s"] = m / tc
+        out["cpu_sample_points"] = m
+    s.close()
+    return out
+
+
 def run_effm_leg(args, dev, torch):
     """Momentum-space integrators of src/hd.C with the effective-mass matrix
     on the 642-vertex vesicle (the size the reference's O(nv^3) set-up
@@ -529,6 +584,8 @@ def main():
     ap.add_argument("--no-effm", action="store_true", help="skip the effective-mass integrator leg")
     ap.add_argument("--no-sheet", action="store_true", help="skip the periodic-sheet leg")
     ap.add_argument("--effm-steps", type=int, default=500)
+    ap.add_argument("--no-points", action="store_true", help="skip the surface-points leg")
+    ap.add_argument("--points-n", type=int, default=1000000)
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-slots", type=int, default=64)
@@ -610,6 +667,8 @@ def main():
         out["sheet"] = run_sheet_leg(args, dev, torch)
     if not args.no_effm:
         out["effm"] = run_effm_leg(args, dev, torch)
+    if not args.no_points:
+        out["points"] = run_points_leg(args, dev, torch)
     if not args.no_cpu_baseline:
         base, E_o, g_o = cpu_baseline(w, c0, args.cpu_budget)
         out["cpu_baseline"] = base

@@ -227,8 +227,9 @@ int umcp_write_limit_surface(umcp_surface *s, const double *r, const char *path,
  * surface from unscaled positions.  Inputs the reference stops on (a
  * sub-patch level beyond MAX_DOMAIN = 100, (u, v) outside an irregular face,
  * the gradient at the extraordinary vertex) return UMCP_ERR_ARG naming the
- * point.  Sums over the points of one vertex are taken in point order.
- * Needs umcp_plan; host arrays; synchronous. */
+ * point.  The gradient is summed per (face, slot) over the face's points in
+ * the caller's order, then per vertex in the plan's fixed order: results do
+ * not depend on the launch.  Needs umcp_plan; host arrays; synchronous. */
 #define UMCP_POINT_MAX_BATCH (1 << 24)
 int umcp_point_eval(umcp_surface *s, const double *r, int n, const int *faces, const double *uv, double *R,
                     double *nrm);

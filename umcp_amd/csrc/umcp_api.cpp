@@ -340,6 +340,8 @@ void umcp_destroy(umcp_surface *s)
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (void *p : s->allocations) (void)hipFree(p);
+    if (s->pt_stage) (void)hipHostFree(s->pt_stage);
+    if (s->pt_host) (void)hipHostFree(s->pt_host);
     for (hipEvent_t e : s->prof_events) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;

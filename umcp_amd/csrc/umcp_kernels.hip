@@ -1764,8 +1764,9 @@ __device__ __forceinline__ void irr_weights(const double *__restrict__ map, int 
 // One point of evaluateRNRM or pointGradient.  IRR: the control-point weights
 // of an irregular face are the sub-patch map applied to the basis, t[d][x] =
 // s_d sum_y map[y][x] B[d][y]; on regular faces t[d][x] = B[d][x].
+// p: the caller's index of the point, j: its face-sorted position (gradient)
 template <bool IRR>
-__device__ void point_body(const PointArgs &a, int p, const PointRec &pr, const PointClass &C)
+__device__ void point_body(const PointArgs &a, int p, int j, const PointRec &pr, const PointClass &C)
 {
     const int nf = C.nf, ncp = IRR ? C.ncp : 12, fc = pr.fc;
     const bool shifted = C.has_img[fc] != 0;
@@ -1891,17 +1892,11 @@ __device__ void point_body(const PointArgs &a, int p, const PointRec &pr, const 
         c1[k] = frac * de_ru[k] - os * dg_ru;
         c2[k] = frac * de_rv[k] - os * dg_rv;
     }
-    double ga[3] = {0.0, 0.0, 0.0};
-    double *out = a.contrib + (size_t)p * kPointSlots * 3;
+    // slot-major: lanes of a wave write consecutive 24-byte records
     auto emit = [&](int x, const double t[3]) {
-        double q[3];
-        load_q(x, q);
+        double *out = a.contrib + ((size_t)x * a.n + j) * 3;
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const double dedr = c0[k] * t[0] + c1[k] * t[1] + c2[k] * t[2];
-            out[3 * x + k] = dedr * al[k];
-            ga[k] += dedr * q[k];
-        }
+        for (int k = 0; k < 3; k++) out[k] = c0[k] * t[0] + c1[k] * t[1] + c2[k] * t[2];
     };
     if constexpr (IRR) {
         for (int x = 0; x < ncp; x++) {
@@ -1916,23 +1911,22 @@ __device__ void point_body(const PointArgs &a, int p, const PointRec &pr, const 
             emit(x, t);
         }
     }
-#pragma unroll
-    for (int k = 0; k < 3; k++) a.alpha[3 * p + k] = ga[k];
 }
 
-// one thread per point of one kind (idx: the batch's points of that kind)
+// one thread per point of one kind (idx: the batch's positions of that kind)
 template <int KIND>
 __global__ __launch_bounds__(kPointBlock) void point_kernel(PointArgs a, const int *__restrict__ idx, int count)
 {
-    const int j = blockIdx.x * kPointBlock + threadIdx.x;
-    if (j >= count) return;
-    const int p = idx[j];
+    const int t = blockIdx.x * kPointBlock + threadIdx.x;
+    if (t >= count) return;
+    const int j = idx[t];
+    const int p = a.grad ? a.order[j] : j;
     const PointRec pr = a.rec[p];
     const PointClass C = a.cls[pr.cls];
     if constexpr (KIND == kPointRegular) {
-        point_body<false>(a, p, pr, C);
+        point_body<false>(a, p, j, pr, C);
     } else if constexpr (KIND == kPointIrregular) {
-        point_body<true>(a, p, pr, C);
+        point_body<true>(a, p, j, pr, C);
     } else {
         // the extraordinary vertex (evaluation only): 1/2 on the vertex and
         // 1/(2 val) on each ring neighbour at its image next to the vertex,
@@ -1972,37 +1966,79 @@ __global__ __launch_bounds__(kPointBlock) void point_kernel(PointArgs a, const i
     }
 }
 
-__global__ __launch_bounds__(kPointBlock) void point_reduce_kernel(int nt, const int *__restrict__ start,
-                                                                    const int *__restrict__ entries,
-                                                                    const double *__restrict__ contrib,
-                                                                    double *__restrict__ out)
+// per (face, slot): the face's points summed in order.  Slot-major threads:
+// the lanes of a wave take adjacent faces in one slot plane of contrib.
+__global__ __launch_bounds__(kPointBlock) void point_segment_kernel(PointReduceArgs a)
 {
     const int t = blockIdx.x * kPointBlock + threadIdx.x;
-    if (t >= nt) return;
+    if (t >= a.nseg * kPointSlots) return;
+    const int x = t / a.nseg, sidx = t - x * a.nseg;
+    const PointSeg sg = a.seg[sidx];
+    const PointClass C = a.cls[sg.cls];
+    double *ap = a.apart + (size_t)t * 3;
+    if (x >= C.ncp) {
+        ap[0] = ap[1] = ap[2] = 0.0;
+        return;
+    }
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+    for (int j = sg.start; j < sg.end; j++) {
+        const double *c = a.contrib + ((size_t)x * a.n + j) * 3;
+        d0 += c[0];
+        d1 += c[1];
+        d2 += c[2];
+    }
+    const int v = C.cp[x * C.nf + sg.fc];
+    double q[3] = {a.r[3 * v], a.r[3 * v + 1], a.r[3 * v + 2]};
+    if (C.has_img[sg.fc]) {
+        double o[3];
+        image_offset(C.img[x * C.nf + sg.fc], a.pbc, o);
+        q[0] += o[0];
+        q[1] += o[1];
+        q[2] += o[2];
+    }
+    const double *al = a.r + 3 * a.nv;
+    double *gb = a.gbuf + (size_t)C.pos[x * C.nf + sg.fc] * 3;
+    gb[0] = d0 * al[0];
+    gb[1] = d1 * al[1];
+    gb[2] = d2 * al[2];
+    ap[0] = d0 * q[0];
+    ap[1] = d1 * q[1];
+    ap[2] = d2 * q[2];
+}
+
+// per vertex: its CSR range in order
+__global__ __launch_bounds__(kPointBlock) void point_vertex_kernel(PointReduceArgs a)
+{
+    const int v = blockIdx.x * kPointBlock + threadIdx.x;
+    if (v >= a.nv) return;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int e = start[t]; e < start[t + 1]; e++) {
-        const double *c = contrib + (size_t)entries[e] * 3;
+    for (int e = a.vstart[v]; e < a.vstart[v + 1]; e++) {
+        const double *c = a.gbuf + (size_t)e * 3;
         s0 += c[0];
         s1 += c[1];
         s2 += c[2];
     }
-    out[3 * t] = s0;
-    out[3 * t + 1] = s1;
-    out[3 * t + 2] = s2;
+    a.gout[3 * v] = s0;
+    a.gout[3 * v + 1] = s1;
+    a.gout[3 * v + 2] = s2;
 }
 
 constexpr int kPointAlphaBlock = 256;
 
-__global__ __launch_bounds__(kPointAlphaBlock) void point_alpha_kernel(int n, const double *__restrict__ alpha,
-                                                                       double *__restrict__ out)
+// block b sums its contiguous chunk of src ([len][3]) into dst[b]; the
+// partition and the tree depend on len only
+__global__ __launch_bounds__(kPointAlphaBlock) void point_alpha_kernel(int len, int chunk,
+                                                                       const double *__restrict__ src,
+                                                                       double *__restrict__ dst)
 {
     __shared__ double sh[3][kPointAlphaBlock];
     const int t = threadIdx.x;
+    const int lo = blockIdx.x * chunk, hi = min(len, lo + chunk);
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int p = t; p < n; p += kPointAlphaBlock) {
-        s0 += alpha[3 * p];
-        s1 += alpha[3 * p + 1];
-        s2 += alpha[3 * p + 2];
+    for (int i = lo + t; i < hi; i += kPointAlphaBlock) {
+        s0 += src[3 * (size_t)i];
+        s1 += src[3 * (size_t)i + 1];
+        s2 += src[3 * (size_t)i + 2];
     }
     sh[0][t] = s0;
     sh[1][t] = s1;
@@ -2016,7 +2052,7 @@ __global__ __launch_bounds__(kPointAlphaBlock) void point_alpha_kernel(int n, co
         }
         __syncthreads();
     }
-    if (t < 3) out[t] = sh[t][0];
+    if (t < 3) dst[3 * blockIdx.x + t] = sh[t][0];
 }
 
 hipError_t launch_points(const PointArgs &a, const int *idx, const int counts[3], hipStream_t st)
@@ -2037,16 +2073,32 @@ hipError_t launch_points(const PointArgs &a, const int *idx, const int counts[3]
     return hipSuccess;
 }
 
-hipError_t launch_point_reduce(int nt, const int *start, const int *entries, const double *contrib, int n,
-                               const double *alpha, double *out, hipStream_t st)
+hipError_t launch_point_reduce(const PointReduceArgs &a, hipStream_t st)
 {
-    if (nt > 0) {
-        hipLaunchKernelGGL(point_reduce_kernel, dim3((nt + kPointBlock - 1) / kPointBlock), dim3(kPointBlock), 0,
-                           st, nt, start, entries, contrib, out);
-        hipError_t e = hipGetLastError();
+    hipError_t e = hipMemsetAsync(a.gbuf, 0, sizeof(double) * 3 * (size_t)a.nnz, st);
+    if (e != hipSuccess) return e;
+    const int ns = a.nseg * kPointSlots;
+    if (ns > 0) {
+        hipLaunchKernelGGL(point_segment_kernel, dim3((ns + kPointBlock - 1) / kPointBlock), dim3(kPointBlock), 0, st,
+                           a);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(point_vertex_kernel, dim3((a.nv + kPointBlock - 1) / kPointBlock), dim3(kPointBlock), 0, st,
+                       a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // alpha: fixed chunks of apart, then one block over the chunk sums
+    const int chunk = (ns + kPointAlphaParts - 1) / kPointAlphaParts;
+    const int nb = chunk > 0 ? (ns + chunk - 1) / chunk : 0;
+    if (nb > 0) {
+        hipLaunchKernelGGL(point_alpha_kernel, dim3(nb), dim3(kPointAlphaBlock), 0, st, ns, chunk, a.apart,
+                           a.partial);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        hipLaunchKernelGGL(point_alpha_kernel, dim3(1), dim3(kPointAlphaBlock), 0, st, nb, nb, a.partial,
+                           a.gout + 3 * (size_t)a.nv);
+    } else {
+        e = hipMemsetAsync(a.gout + 3 * (size_t)a.nv, 0, sizeof(double) * 3, st);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(point_alpha_kernel, dim3(1), dim3(kPointAlphaBlock), 0, st, n, alpha, out + 3 * (size_t)nt);
     return hipGetLastError();
 }
 

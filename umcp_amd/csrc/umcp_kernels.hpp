@@ -163,10 +163,16 @@ struct PointRec {
 
 struct PointClass {
     const int *cp;           // [ncp][nf]
+    const int *pos;          // [ncp][nf]   position of (face, slot) in the vertex CSR
     const uint8_t *img;      // [ncp][nf]
     const uint8_t *has_img;  // [nf]
     const double *map;       // irregular: [kPointDomains * 3][12][ncp]
     int nf, ncp, valence, pad;
+};
+
+// the points of one face, [start, end) of the face-sorted order
+struct PointSeg {
+    int cls, fc, start, end;
 };
 
 struct PointArgs {
@@ -176,22 +182,38 @@ struct PointArgs {
     int on_surface;
     double kT;
     double pbc[3][3];
-    const PointRec *rec;     // [n]
+    const PointRec *rec;     // [n], the caller's order
+    const int *order;        // gradient: [n] caller index of face-sorted position j
     const PointClass *cls;
     const double *de_dr, *de_dnrm, *frac;  // [n][3], [n][3], [n] (frac null: 1)
     double *R, *nrm;         // position and unit normal [n][3]
-    double *contrib;         // vertex gradient per (point, slot) [n][kPointSlots][3]
-    double *alpha;           // alpha gradient per point [n][3]
+    double *contrib;         // dE/dr per (slot, sorted position): [kPointSlots][n][3], no alpha
     double *gpuv;            // (u, v) gradient per point [n][2]
 };
-// idx: the indices of the regular, irregular and extraordinary-vertex points
-// in that order, counts[kind] of each (the vertex kind: evaluation only)
+// idx: the positions (eval: caller indices; gradient: face-sorted positions)
+// of the regular, irregular and extraordinary-vertex points in that order,
+// counts[kind] of each (the vertex kind: evaluation only)
 hipError_t launch_points(const PointArgs &a, const int *idx, const int counts[3], hipStream_t st);
-// out[t][k] = sum of contrib[entries[e]][k] over e in [start[t], start[t+1]),
-// in that order, for the nt touched vertices; out[nt][k] = the sum of the n
-// points' alpha gradients in a fixed tree order
-hipError_t launch_point_reduce(int nt, const int *start, const int *entries, const double *contrib, int n,
-                               const double *alpha, double *out, hipStream_t st);
+// gradient reduction of a face-sorted batch: per (face, slot) the sum of its
+// points' contributions in order, alpha-scaled into gbuf at the slot's vertex
+// CSR position (gbuf zeroed first), and its alpha-gradient part into apart;
+// then per vertex the sum of its CSR range in order into gout [nv][3], and
+// the alpha gradient into gout[nv] by a two-stage tree over apart
+struct PointReduceArgs {
+    const double *r;
+    int nv, n, nseg, nnz;
+    double pbc[3][3];
+    const PointSeg *seg;     // [nseg]
+    const PointClass *cls;
+    const double *contrib;
+    const int *vstart;       // [nv + 1]
+    double *gbuf;            // [nnz][3]
+    double *apart;           // [kPointSlots][nseg][3]
+    double *partial;         // [kPointAlphaParts][3]
+    double *gout;            // [nv + 1][3]
+};
+constexpr int kPointAlphaParts = 256;
+hipError_t launch_point_reduce(const PointReduceArgs &a, hipStream_t st);
 
 struct GatherItem {
     int surf, vblock;

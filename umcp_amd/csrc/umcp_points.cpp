@@ -5,10 +5,12 @@
 // src/minimize.C:233, src/p_p.C:336-357).
 //
 // The host resolves each point (face -> class, sub-patch level and local
-// coordinates, with the reference's domain rules and rejections) and builds
-// the vertex -> (point, slot) incidence of the batch; the device evaluates one
-// point per thread and sums each touched vertex's contributions in point
-// order, so results do not depend on the launch.
+// coordinates, with the reference's domain rules and rejections) and, for the
+// gradient, orders the batch by face (a stable counting sort).  The device
+// evaluates one point per thread; the gradient is then summed per (face,
+// slot) over the face's points in the caller's order, written into the
+// surface's vertex-CSR layout and summed per vertex in CSR order, so results
+// do not depend on the launch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -86,7 +88,6 @@ int point_setup(umcp_surface *s)
             s->pt_face_cls[(size_t)P.classes[c].face_id[i]] = (int)c;
             s->pt_face_fc[(size_t)P.classes[c].face_id[i]] = i;
         }
-    s->pt_vmark.assign((size_t)s->T.nv, -1);
     // one buffer: [class table][maps...]
     std::vector<PointClass> tab(P.classes.size());
     std::vector<size_t> map_off(P.classes.size(), 0);
@@ -106,6 +107,7 @@ int point_setup(umcp_surface *s)
         const FaceClass &C = P.classes[c];
         PointClass &t = tab[c];
         t.cp = s->cls[c].cp;
+        t.pos = s->cls[c].pos;
         t.img = s->cls[c].img;
         t.has_img = s->cls[c].has_img;
         t.nf = C.nf();
@@ -119,6 +121,12 @@ int point_setup(umcp_surface *s)
         }
     }
     std::memcpy(host.data() + tab_off, tab.data(), sizeof(PointClass) * tab.size());
+    // (face, slot) sums of the gradient in the vertex-CSR layout, and the
+    // alpha reduction's partials
+    void *gb = nullptr;
+    rc = palloc(s, &gb, sizeof(double) * 3 * ((size_t)P.nnz + kPointAlphaParts));
+    if (rc) return rc;
+    s->d_pt_gbuf = (double *)gb;
     hipError_t e = hipMemcpyAsync(d, host.data(), bytes, hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) return phip(s, e, "upload point tables");
@@ -182,12 +190,28 @@ const char *resolve_irregular(double fu, double fv, bool grad, PointRec &rec)
 }
 
 struct Batch {
-    size_t off_r, off_rec, off_idx, off_dr, off_dn, off_frac, off_start, off_ent, in_bytes;
-    size_t off_R, off_N, off_con, off_al, off_gp, off_out, out_bytes;
-    std::vector<int> touched;   // touched vertices in first-touch order
+    size_t off_r, off_rec, off_order, off_idx, off_dr, off_dn, off_frac, off_seg, in_bytes;
+    size_t off_R, off_N, off_gout, off_gp, off_con, off_apart, out_bytes;
 };
 
-// resolves the points, builds the vertex incidence, stages and launches
+// pinned grow-only host buffer
+int pin_grow(umcp_surface *s, char **buf, size_t *cap, size_t need)
+{
+    if (need <= *cap) return UMCP_OK;
+    (void)hipStreamSynchronize(s->stream);
+    if (*buf) (void)hipHostFree(*buf);
+    *buf = nullptr;
+    *cap = 0;
+    const size_t want = std::max(need, (size_t)1 << 20);
+    void *p = nullptr;
+    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    if (e != hipSuccess) return pfail(s, UMCP_ERR_OOM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    *buf = (char *)p;
+    *cap = want;
+    return UMCP_OK;
+}
+
+// resolves the points, orders a gradient batch by face, stages and launches
 int run_points(umcp_surface *s, const double *r, int n, const int *faces, const double *uv, const double *de_dr,
                const double *de_dnrm, const double *frac, int on_surface, double kT, bool grad, Batch &b)
 {
@@ -214,89 +238,72 @@ int run_points(umcp_surface *s, const double *r, int n, const int *faces, const 
             return pfail(s, UMCP_ERR_ARG, "point " + std::to_string(i) + ": " + msg);
         }
     }
-    // the points of each kind, for one launch per kind
+    // gradient: positions j in face order (stable), one segment per face
+    // with points; evaluation: the caller's order
+    std::vector<int> order, idx((size_t)n);
+    std::vector<PointSeg> seg;
     int counts[3] = {0, 0, 0};
-    for (int i = 0; i < n; i++) counts[rec[(size_t)i].kind]++;
-    std::vector<int> idx((size_t)n);
-    {
+    if (grad) {
+        std::vector<int> start((size_t)P.nfaces + 1, 0);
+        for (int i = 0; i < n; i++) start[(size_t)faces[i] + 1]++;
+        for (int f = 0; f < P.nfaces; f++) {
+            if (start[(size_t)f + 1])
+                seg.push_back({s->pt_face_cls[(size_t)f], s->pt_face_fc[(size_t)f], start[(size_t)f],
+                               start[(size_t)f] + start[(size_t)f + 1]});
+            start[(size_t)f + 1] += start[(size_t)f];
+        }
+        order.resize((size_t)n);
+        for (int i = 0; i < n; i++) order[(size_t)start[(size_t)faces[i]]++] = i;
+        for (int j = 0; j < n; j++) counts[rec[(size_t)order[(size_t)j]].kind]++;
+        int fill[3] = {0, counts[0], counts[0] + counts[1]};
+        for (int j = 0; j < n; j++) idx[(size_t)fill[rec[(size_t)order[(size_t)j]].kind]++] = j;
+    } else {
+        for (int i = 0; i < n; i++) counts[rec[(size_t)i].kind]++;
         int fill[3] = {0, counts[0], counts[0] + counts[1]};
         for (int i = 0; i < n; i++) idx[(size_t)fill[rec[(size_t)i].kind]++] = i;
     }
-    // vertex -> (point, slot) incidence in point order (gradient only)
-    std::vector<int> count, start, ent;
-    b.touched.clear();
-    if (grad) {
-        for (int i = 0; i < n; i++) {
-            const FaceClass &C = P.classes[(size_t)rec[(size_t)i].cls];
-            const int nf = C.nf(), fc = rec[(size_t)i].fc;
-            for (int x = 0; x < C.ncp; x++) {
-                const int v = C.cp[(size_t)x * nf + fc];
-                int &t = s->pt_vmark[(size_t)v];
-                if (t < 0) {
-                    t = (int)b.touched.size();
-                    b.touched.push_back(v);
-                    count.push_back(0);
-                }
-                count[(size_t)t]++;
-            }
-        }
-        const int nt = (int)b.touched.size();
-        start.assign((size_t)nt + 1, 0);
-        for (int t = 0; t < nt; t++) start[(size_t)t + 1] = start[(size_t)t] + count[(size_t)t];
-        ent.assign((size_t)start[(size_t)nt], 0);
-        std::vector<int> fill(start.begin(), start.end() - 1);
-        for (int i = 0; i < n; i++) {
-            const FaceClass &C = P.classes[(size_t)rec[(size_t)i].cls];
-            const int nf = C.nf(), fc = rec[(size_t)i].fc;
-            for (int x = 0; x < C.ncp; x++) {
-                const int t = s->pt_vmark[(size_t)C.cp[(size_t)x * nf + fc]];
-                ent[(size_t)fill[(size_t)t]++] = i * kPointSlots + x;
-            }
-        }
-        for (int v : b.touched) s->pt_vmark[(size_t)v] = -1;
-    }
-    const int nt = (int)b.touched.size();
+    const int nseg = (int)seg.size();
     // staging layout
     size_t o = 0;
     b.off_r = o;      o = align_up(o + sizeof(double) * (3 * (size_t)nv + 3));
     b.off_rec = o;    o = align_up(o + sizeof(PointRec) * (size_t)n);
+    b.off_order = o;  o = align_up(o + sizeof(int) * order.size());
     b.off_idx = o;    o = align_up(o + sizeof(int) * (size_t)n);
     b.off_dr = o;     o = align_up(o + (grad ? sizeof(double) * 3 * (size_t)n : 0));
     b.off_dn = o;     o = align_up(o + (grad ? sizeof(double) * 3 * (size_t)n : 0));
     b.off_frac = o;   o = align_up(o + (grad && frac ? sizeof(double) * (size_t)n : 0));
-    b.off_start = o;  o = align_up(o + sizeof(int) * (start.size()));
-    b.off_ent = o;    o = align_up(o + sizeof(int) * ent.size());
+    b.off_seg = o;    o = align_up(o + sizeof(PointSeg) * seg.size());
     b.in_bytes = o;
     o = 0;
     if (grad) {
-        // [gpuv 2n][touched sums 3 nt][alpha 3] first: one download
-        b.off_gp = o;
-        b.off_out = o + sizeof(double) * 2 * (size_t)n;
-        o = align_up(b.off_out + sizeof(double) * (3 * (size_t)nt + 3));
-        b.off_con = o;  o = align_up(o + sizeof(double) * (size_t)n * kPointSlots * 3);
-        b.off_al = o;   o = align_up(o + sizeof(double) * 3 * (size_t)n);
+        // [vertex sums 3 nv][alpha 3][gpuv 2n] first: one download
+        b.off_gout = o;
+        b.off_gp = o + sizeof(double) * (3 * (size_t)nv + 3);
+        o = align_up(b.off_gp + sizeof(double) * 2 * (size_t)n);
+        b.off_con = o;    o = align_up(o + sizeof(double) * (size_t)n * kPointSlots * 3);
+        b.off_apart = o;  o = align_up(o + sizeof(double) * (size_t)nseg * kPointSlots * 3);
         b.off_R = b.off_N = 0;
     } else {
         b.off_R = o;
         b.off_N = o + sizeof(double) * 3 * (size_t)n;
         o = align_up(b.off_N + sizeof(double) * 3 * (size_t)n);
-        b.off_gp = b.off_out = b.off_con = b.off_al = 0;
+        b.off_gout = b.off_gp = b.off_con = b.off_apart = 0;
     }
     b.out_bytes = o;
     int rc = pgrow(s, &s->d_pt_in, &s->pt_in_cap, b.in_bytes);
     if (rc == UMCP_OK) rc = pgrow(s, &s->d_pt_out, &s->pt_out_cap, b.out_bytes);
+    if (rc == UMCP_OK) rc = pin_grow(s, &s->pt_stage, &s->pt_stage_cap, b.in_bytes);
     if (rc) return rc;
-    if (s->pt_stage.size() < b.in_bytes) s->pt_stage.resize(b.in_bytes);
-    char *h = s->pt_stage.data();
+    char *h = s->pt_stage;
     std::memcpy(h + b.off_r, r, sizeof(double) * (3 * (size_t)nv + 3));
     std::memcpy(h + b.off_rec, rec.data(), sizeof(PointRec) * (size_t)n);
+    if (grad) std::memcpy(h + b.off_order, order.data(), sizeof(int) * order.size());
     std::memcpy(h + b.off_idx, idx.data(), sizeof(int) * (size_t)n);
     if (grad) {
         std::memcpy(h + b.off_dr, de_dr, sizeof(double) * 3 * (size_t)n);
         std::memcpy(h + b.off_dn, de_dnrm, sizeof(double) * 3 * (size_t)n);
         if (frac) std::memcpy(h + b.off_frac, frac, sizeof(double) * (size_t)n);
-        std::memcpy(h + b.off_start, start.data(), sizeof(int) * start.size());
-        std::memcpy(h + b.off_ent, ent.data(), sizeof(int) * ent.size());
+        std::memcpy(h + b.off_seg, seg.data(), sizeof(PointSeg) * seg.size());
     }
     hipError_t e = hipMemcpyAsync(s->d_pt_in, h, b.in_bytes, hipMemcpyHostToDevice, s->stream);
     if (e != hipSuccess) return phip(s, e, "upload points");
@@ -312,11 +319,11 @@ int run_points(umcp_surface *s, const double *r, int n, const int *faces, const 
     a.rec = (const PointRec *)(di + b.off_rec);
     a.cls = s->d_pt_cls;
     if (grad) {
+        a.order = (const int *)(di + b.off_order);
         a.de_dr = (const double *)(di + b.off_dr);
         a.de_dnrm = (const double *)(di + b.off_dn);
         a.frac = frac ? (const double *)(di + b.off_frac) : nullptr;
         a.contrib = (double *)(dout + b.off_con);
-        a.alpha = (double *)(dout + b.off_al);
         a.gpuv = (double *)(dout + b.off_gp);
     } else {
         a.R = (double *)(dout + b.off_R);
@@ -325,8 +332,22 @@ int run_points(umcp_surface *s, const double *r, int n, const int *faces, const 
     e = launch_points(a, (const int *)(di + b.off_idx), counts, s->stream);
     if (e != hipSuccess) return phip(s, e, "point kernel launch");
     if (grad) {
-        e = launch_point_reduce(nt, (const int *)(di + b.off_start), (const int *)(di + b.off_ent), a.contrib, n,
-                                a.alpha, (double *)(dout + b.off_out), s->stream);
+        PointReduceArgs ra{};
+        ra.r = a.r;
+        ra.nv = nv;
+        ra.n = n;
+        ra.nseg = nseg;
+        ra.nnz = P.nnz;
+        std::memcpy(ra.pbc, s->T.pbc, sizeof ra.pbc);
+        ra.seg = (const PointSeg *)(di + b.off_seg);
+        ra.cls = s->d_pt_cls;
+        ra.contrib = a.contrib;
+        ra.vstart = s->d_vstart;
+        ra.gbuf = s->d_pt_gbuf;
+        ra.apart = (double *)(dout + b.off_apart);
+        ra.partial = s->d_pt_gbuf + 3 * (size_t)P.nnz;
+        ra.gout = (double *)(dout + b.off_gout);
+        e = launch_point_reduce(ra, s->stream);
         if (e != hipSuccess) return phip(s, e, "point reduce launch");
     }
     return UMCP_OK;
@@ -378,20 +399,17 @@ int umcp_point_grad(umcp_surface *s, const double *r, int n, const int *faces, c
     Batch b;
     rc = run_points(s, r, n, faces, uv, de_dr, de_dnrm, frac_mult, on_surface, kT, true, b);
     if (rc) return rc;
-    const int nt = (int)b.touched.size();
-    const size_t cnt = 2 * (size_t)n + 3 * (size_t)nt + 3;   // gpuv, touched sums, alpha: contiguous
-    if (s->pt_host.size() < cnt) s->pt_host.resize(cnt);
-    hipError_t e = hipMemcpyAsync(s->pt_host.data(), s->d_pt_out + b.off_gp, sizeof(double) * cnt,
+    const size_t cnt = 3 * (size_t)s->T.nv + 3 + 2 * (size_t)n;   // vertex sums, alpha, gpuv: contiguous
+    rc = pin_grow(s, &s->pt_host, &s->pt_host_cap, sizeof(double) * cnt);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(s->pt_host, s->d_pt_out + b.off_gout, sizeof(double) * cnt,
                                   hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) return phip(s, e, "download point gradient");
-    const double *gp = s->pt_host.data(), *sums = gp + 2 * (size_t)n, *al = sums + 3 * (size_t)nt;
+    const double *gs = (const double *)s->pt_host, *gp = gs + 3 * (size_t)s->T.nv + 3;
     // the reference's contract: the caller's arrays accumulate (+=)
+    for (size_t i = 0; i < 3 * (size_t)s->T.nv + 3; i++) g[i] += gs[i];
     for (size_t i = 0; i < 2 * (size_t)n; i++) g_puv[i] += gp[i];
-    for (int t = 0; t < nt; t++)
-        for (int k = 0; k < 3; k++) g[3 * (size_t)b.touched[(size_t)t] + k] += sums[3 * (size_t)t + k];
-    const int nv = s->T.nv;
-    for (int k = 0; k < 3; k++) g[3 * (size_t)nv + k] += al[k];
     return UMCP_OK;
 }
 

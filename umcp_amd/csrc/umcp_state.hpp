@@ -85,16 +85,17 @@ struct umcp_surface {
     long effm_nnz = 0;
     std::vector<void *> effm_allocs;
     // points on the limit surface (umcp_point_eval / umcp_point_grad): face ->
-    // (class, index in the class), a vertex mark for the batch incidence, the
-    // device tables (class table, sub-patch maps) and
-    // grow-only staging and result buffers
+    // (class, index in the class), the device tables (class table, sub-patch
+    // maps), the gradient's (face, slot) buffer and grow-only staging and
+    // result buffers
     bool pt_ready = false;
-    std::vector<int> pt_face_cls, pt_face_fc, pt_vmark;
+    std::vector<int> pt_face_cls, pt_face_fc;
     umcp::PointClass *d_pt_cls = nullptr;
+    double *d_pt_gbuf = nullptr;          // [nnz][3] (face, slot) sums, then the alpha partials
     char *d_pt_in = nullptr, *d_pt_out = nullptr;
     size_t pt_in_cap = 0, pt_out_cap = 0;
-    std::vector<char> pt_stage;
-    std::vector<double> pt_host;
+    char *pt_stage = nullptr, *pt_host = nullptr;   // pinned (hipHostMalloc)
+    size_t pt_stage_cap = 0, pt_host_cap = 0;
 };
 
 namespace umcp {
