@@ -211,6 +211,11 @@ int umcp_read_restart(const char *path, int nv, double *r, double *pp, double *a
  * plim = 1), scaled by alpha; append != 0 adds a frame to the file.  Needs
  * umcp_prepare. */
 int umcp_write_limit_surface(umcp_surface *s, const double *r, const char *path, int append);
+/* the topology of those frames (surface::writeLimitingSurfacePSF,
+ * src/write_structures.C:544-655 over writePSF, src/mutil.C:601-676, without
+ * complexes): PSF EXT, one "C" atom per frame point, three bonds per face
+ * joining its corners.  Needs umcp_prepare. */
+int umcp_write_limit_psf(umcp_surface *s, const char *path);
 
 /* Points on the limit surface, batched; faces in the reference's numbering
  * (regular faces first, then irregular), (u, v) the face's parameters.

@@ -130,3 +130,43 @@ def test_limit_surface_frames(tmp_path):
     val = np.array([len(o.neighbors(v)) for v in range(o.nv)])
     assert np.array_equal(counts, val)
     lib().umcp_destroy(h)
+
+
+def test_limit_surface_psf(tmp_path):
+    """The PSF topology of the limit frames (writeLimitingSurfacePSF at plim = 1,
+    src/write_structures.C:544-655; writePSF, src/mutil.C:601-676): the
+    column layout, one atom per frame point, and bonds that join the three
+    corners of each face in the frame written next to it."""
+    w = workloads.vesicle_642()
+    h = load(w.mesh_path)
+    assert lib().umcp_prepare(h) == 0
+    p = str(tmp_path / "limit.psf")
+    x = str(tmp_path / "limit.xyz")
+    assert lib().umcp_write_limit_psf(h, p.encode()) == 0
+    r = w.r.copy()
+    assert lib().umcp_write_limit_surface(h, S._dp(r), x.encode(), 0) == 0
+    lines = open(p).read().split("\n")
+    nf = 1280
+    n = 3 * nf
+    assert lines[0] == "PSF EXT" and lines[1] == ""
+    assert lines[2] == "    3840 !NATOM"
+    tail = "C        C        C        0.000000        1.0000           0   0.00000      0.000000E-01"
+    assert lines[3] == "         1 SEGN     1        " + tail
+    assert lines[3 + 1233] == "      1234 SEGN     1234     " + tail
+    for i in range(n):
+        f = lines[3 + i].split()
+        assert f[0] == f[2] == str(i + 1) and f[1] == "SEGN"
+    assert lines[3 + n] == "    3840 !NBOND: nbonds"
+    body = lines[4 + n:]
+    assert all(len(ln) == 80 for ln in body[:-1]) and body[-1] == ""   # 4 bonds of 2 x 10 columns per line
+    tok = np.array("".join(body).split(), dtype=int).reshape(-1, 2)
+    assert len(tok) == 3 * nf
+    t = np.arange(nf)
+    want = np.stack([np.stack([3 * t + 1, 3 * t + 3], 1), np.stack([3 * t + 1, 3 * t + 2], 1),
+                     np.stack([3 * t + 2, 3 * t + 3], 1)], 1).reshape(-1, 2)
+    assert np.array_equal(tok, want)
+    # each bond joins two distinct corners of one face of the frame: an edge
+    pts = np.array([[float(v) for v in ln.split()[1:]] for ln in open(x).read().splitlines()[2:2 + n]])
+    d = np.linalg.norm(pts[tok[:, 0] - 1] - pts[tok[:, 1] - 1], axis=1)
+    assert d.min() > 0 and d.max() < 3 * np.median(d)
+    lib().umcp_destroy(h)

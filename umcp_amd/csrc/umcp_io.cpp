@@ -101,6 +101,38 @@ int umcp_write_limit_surface(umcp_surface *s, const double *r, const char *path,
     return std::fclose(fp) == 0 ? UMCP_OK : UMCP_ERR_ARG;
 }
 
+int umcp_write_limit_psf(umcp_surface *s, const char *path)
+{
+    if (!s || !path) return UMCP_ERR_ARG;
+    if (!s->prepared) return UMCP_ERR_STATE;
+    const int nt = s->P.nfaces;
+    // three corners per face, lattice order (0,0), (0,1), (1,0); within a
+    // face the bonds (0,0)-(1,0), (0,0)-(0,1), (0,1)-(1,0), the order the
+    // reference's lattice walk at plim = 1 produces
+    FILE *fp = std::fopen(path, "w");
+    if (!fp) {
+        s->err = std::string("cannot write ") + path;
+        return UMCP_ERR_ARG;
+    }
+    const long natom = 3L * nt, nbond = 3L * nt;
+    std::fprintf(fp, "PSF EXT\n\n%8ld !NATOM\n", natom);
+    for (long i = 1; i <= natom; i++) {
+        char res[32];
+        std::snprintf(res, sizeof res, "%ld", i);
+        std::fprintf(fp, "%10ld SEGN     %-9sC        C        C        0.000000        1.0000           0   0.00000      0.000000E-01\n",
+                     i, res);
+    }
+    std::fprintf(fp, "%8ld !NBOND: nbonds\n", nbond);
+    const int pair[3][2] = {{0, 2}, {0, 1}, {1, 2}};
+    long b = 0;
+    for (int t = 0; t < nt; t++)
+        for (int k = 0; k < 3; k++) {
+            std::fprintf(fp, "%10ld%10ld", 3L * t + pair[k][0] + 1, 3L * t + pair[k][1] + 1);
+            if (++b % 4 == 0) std::fprintf(fp, "\n");
+        }
+    return std::fclose(fp) == 0 ? UMCP_OK : UMCP_ERR_ARG;
+}
+
 int umcp_write_restart(const char *path, int nv, const double *r, const double *pp, int seed)
 {
     if (!path || !r || nv <= 0) return UMCP_ERR_ARG;

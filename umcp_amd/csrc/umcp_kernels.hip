@@ -1980,9 +1980,22 @@ __global__ __launch_bounds__(kPointBlock) void point_segment_kernel(PointReduceA
         ap[0] = ap[1] = ap[2] = 0.0;
         return;
     }
+    // four records in flight per step, added in order
     double d0 = 0.0, d1 = 0.0, d2 = 0.0;
-    for (int j = sg.start; j < sg.end; j++) {
-        const double *c = a.contrib + ((size_t)x * a.n + j) * 3;
+    const double *c = a.contrib + ((size_t)x * a.n + sg.start) * 3;
+    int j = sg.start;
+    for (; j + 4 <= sg.end; j += 4, c += 12) {
+        double v[12];
+#pragma unroll
+        for (int i = 0; i < 12; i++) v[i] = c[i];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            d0 += v[3 * i];
+            d1 += v[3 * i + 1];
+            d2 += v[3 * i + 2];
+        }
+    }
+    for (; j < sg.end; j++, c += 3) {
         d0 += c[0];
         d1 += c[1];
         d2 += c[2];

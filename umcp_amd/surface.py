@@ -43,7 +43,7 @@ EXPORTS = [
     "umcp_server_drain", "umcp_server_stats", "umcp_server_last_error", "umcp_server_destroy",
     "umcp_effm_build", "umcp_effm_get", "umcp_effm_steps", "umcp_effm_steps_dev",
     "umcp_save_mesh", "umcp_write_restart", "umcp_read_restart", "umcp_write_limit_surface",
-    "umcp_point_eval", "umcp_point_grad",
+    "umcp_point_eval", "umcp_point_grad", "umcp_write_limit_psf",
 ]
 
 
@@ -138,6 +138,7 @@ def load_library(path: str = LIB_PATH):
         "umcp_save_mesh": ([vp, dp, C.c_char_p], C.c_int),
         "umcp_write_limit_surface": ([vp, dp, C.c_char_p, C.c_int], C.c_int),
         "umcp_point_eval": ([vp, dp, C.c_int, ip, dp, dp, dp], C.c_int),
+        "umcp_write_limit_psf": ([vp, C.c_char_p], C.c_int),
         "umcp_point_grad": ([vp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int, C.c_double, dp, dp], C.c_int),
         "umcp_write_restart": ([C.c_char_p, C.c_int, dp, dp, C.c_int], C.c_int),
         "umcp_read_restart": ([C.c_char_p, C.c_int, dp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
@@ -364,6 +365,10 @@ class Surface:
                                             _dp(fm) if fm is not None else None, 1 if on_surface else 0,
                                             float(kT), _dp(g), _dp(gp)), "point_grad")
         return g, gp
+
+    def write_limit_psf(self, path: str):
+        """The PSF topology of the limit-surface frames (writeLimitingSurfacePSF)."""
+        self._check(self._L.umcp_write_limit_psf(self._h, path.encode()), "write_limit_psf")
 
     # momentum-space integrators with the effective-mass matrix (src/hd.C)
     def effm_build(self):
