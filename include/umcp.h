@@ -365,6 +365,9 @@ int umcp_get_launch(const umcp_surface *s, int *point_split, int *flags);
  * begin, run work, then end returns the summed milliseconds and the number
  * of launches (all face classes share one launch, reported as entry 0) */
 int umcp_profile_begin(umcp_surface *s);
+/* while profiling, time one evaluation's face launch in every `every`
+ * (default 1), so the events' own cost stays off the other steps */
+int umcp_profile_sample(umcp_surface *s, int every);
 int umcp_profile_end(umcp_surface *s, int max_cls, double *ms, int *launches);
 
 /* introspection (tests): the shared table of a face class and its points */

@@ -256,7 +256,8 @@ int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs 
         if (e != hipSuccess) return hip_fail(s, e, "restraint kernel launch");
     }
     hipEvent_t ev[2] = {nullptr, nullptr};
-    if (s->prof_on) {
+    const bool timed = s->prof_on && (s->prof_tick++ % s->prof_every) == 0;
+    if (timed) {
         int rc = prof_pair(s, 0, ev);
         if (rc) return rc;
         (void)hipEventRecord(ev[0], st);
@@ -264,7 +265,7 @@ int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs 
     const bool mixed = s->prm.precision == UMCP_PREC_MIXED;
     hipError_t e = launch_faces_batch(fb, 0, s->point_split, s->launch_flags, mixed, features_of(phys(s)), st);
     if (e != hipSuccess) return hip_fail(s, e, "face kernel launch");
-    if (s->prof_on) (void)hipEventRecord(ev[1], st);
+    if (timed) (void)hipEventRecord(ev[1], st);
     ga.nv = s->T.nv;
     ga.vstart = s->d_vstart;
     ga.gbuf = s->d_gbuf;
@@ -1092,6 +1093,15 @@ int umcp_profile_begin(umcp_surface *s)
     if (!s || !s->planned) return UMCP_ERR_STATE;
     s->prof_on = true;
     s->prof_used = 0;
+    s->prof_tick = 0;
+    return UMCP_OK;
+}
+
+int umcp_profile_sample(umcp_surface *s, int every)
+{
+    if (!s) return UMCP_ERR_ARG;
+    if (every < 1) return fail(s, UMCP_ERR_ARG, "sampling stride must be >= 1");
+    s->prof_every = every;
     return UMCP_OK;
 }
 

@@ -58,6 +58,8 @@ struct umcp_surface {
     std::vector<hipEvent_t> prof_events;  // pairs: start, end
     std::vector<int> prof_class;          // class of each pair
     size_t prof_used = 0;
+    int prof_every = 1;                   // time one evaluation in prof_every
+    long prof_tick = 0;
     // global area / volume restraints
     umcp_restraints rsp{};
     bool has_images = false;              // periodic mesh: no enclosed volume

@@ -34,7 +34,7 @@ EXPORTS = [
     "umcp_setg0", "umcp_energy", "umcp_grad", "umcp_energy_grad", "umcp_area", "umcp_bd_steps",
     "umcp_energy_grad_dev", "umcp_bd_steps_dev", "umcp_class_count", "umcp_class_info",
     "umcp_class_table", "umcp_face_points", "umcp_face_lipids", "umcp_set_launch",
-    "umcp_get_launch", "umcp_profile_begin", "umcp_profile_end",
+    "umcp_get_launch", "umcp_profile_begin", "umcp_profile_end", "umcp_profile_sample",
     "umcp_set_restraints", "umcp_volume", "umcp_set_step", "umcp_get_step", "umcp_bd_steps_rng_dev",
     "umcp_noise", "umcp_ensemble_create", "umcp_ensemble_destroy", "umcp_ensemble_last_error",
     "umcp_ensemble_add", "umcp_ensemble_remove", "umcp_ensemble_size", "umcp_ensemble_refresh",
@@ -127,6 +127,7 @@ def load_library(path: str = LIB_PATH):
         "umcp_set_launch": ([vp, C.c_int, C.c_int], C.c_int),
         "umcp_get_launch": ([vp, ip, ip], C.c_int),
         "umcp_profile_begin": ([vp], C.c_int),
+        "umcp_profile_sample": ([vp, C.c_int], C.c_int),
         "umcp_profile_end": ([vp, C.c_int, dp, ip], C.c_int),
         "umcp_set_restraints": ([vp, C.POINTER(Restraints)], C.c_int),
         "umcp_volume": ([vp, dp, dp], C.c_int),
@@ -447,7 +448,9 @@ class Surface:
         self._check(self._L.umcp_get_launch(self._h, C.byref(sp), C.byref(fl)), "get_launch")
         return sp.value, fl.value
 
-    def profile_begin(self):
+    def profile_begin(self, every: int = 1):
+        """Times the face launch of one evaluation in every `every` until profile_end."""
+        self._check(self._L.umcp_profile_sample(self._h, int(every)), "profile_sample")
         self._check(self._L.umcp_profile_begin(self._h), "profile_begin")
 
     def profile_end(self):
