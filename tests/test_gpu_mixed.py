@@ -87,13 +87,14 @@ def test_mixed_vesicle_40962(vesicle_40962_pair, split, flags):
     assert rel_e <= E_TOL and rms <= G_RMS_TOL
 
 
-def test_mixed_small_meshes_all_shapes():
-    w = workloads.vesicle_642()
+@pytest.mark.parametrize("mesh", ["vesicle_642", "defect_5_7"])
+def test_mixed_small_meshes_all_shapes(mesh):
+    w = workloads.vesicle_642() if mesh == "vesicle_642" else workloads.defect_vesicle()
     g, o = pair(w)
     ref = (o.energy(w.r), o.grad(w.r))
     for split, flags in SHAPES:
         g.set_launch(split, flags)
-        rel_e, rms = errors_cached(g, ref, w.r, "vesicle_642 split %d flags %d" % (split, flags))
+        rel_e, rms = errors_cached(g, ref, w.r, "%s split %d flags %d" % (mesh, split, flags))
         assert rel_e <= E_TOL and rms <= G_RMS_TOL
 
 

@@ -96,6 +96,30 @@ def test_all_terms_helfrich():
     check_eval(g, o, w.r, "vesicle_2562 helfrich kg kreg KA")
 
 
+@pytest.mark.parametrize("form", ["lipid", "helfrich"])
+def test_valence_5_and_7(form):
+    """A sphere with valence-5 and valence-7 vertices (four edge flips on an
+    icosphere, one Loop step): the valence-7 face class against the oracle,
+    in every launch shape, with both restraints in the Helfrich form."""
+    w = workloads.defect_vesicle()
+    if form == "lipid":
+        g, o = make_pair(w)
+    else:
+        g, o = make_pair(w, params=dict(kc=6.0, kg=-2.0, KA=0.5, k_reg=1.5, mode=1))
+        c0 = workloads.face_c0_uniform(g.nfaces, 6, -0.02, 0.02)
+        g.set_face_c0(c0)
+        o.set_face_c0(c0)
+        rst = {"KA_global": 0.1, "KV": 0.004}
+        g.set_restraints(**rst)
+        o.set_restraints(**rst)
+    vals = sorted(c["valence"] for c in g.classes())
+    assert vals == [5, 6, 7], vals
+    for split in (1, 2, 3, 4):
+        for flags in (0, S.LAUNCH_OCC2):
+            g.set_launch(split, flags)
+            check_eval(g, o, w.r, "defect %s split %d flags %d" % (form, split, flags))
+
+
 def test_periodic_sheet_small():
     w = workloads.sheet(32, Lx=250.0, seed=5, name="sheet_32_parity")
     g, o = make_pair(w)

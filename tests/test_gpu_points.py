@@ -197,6 +197,27 @@ def test_periodic_sheet_matches_oracle():
     assert rel(G[:-3], Go[:-3]) <= 1e-12 and rel(G[-3:], Go[-3:]) <= 1e-12 and rel(P, Po) <= 1e-12
 
 
+def test_valence_7_faces_match_oracle():
+    """Points on a sphere with valence-5 and valence-7 vertices: the
+    valence-7 sub-patch maps (13 control points) against the oracle."""
+    w = workloads.defect_vesicle()
+    g, o = _pair(w)
+    r = w.r.copy()
+    faces, uv = random_points(o, 3000, 17, deep=60)
+    ok = well_conditioned(o, faces, uv)
+    say("[points] valence 5/7 sphere: %d irregular points" % int((faces >= o.nreg).sum()))
+    R, N = g.point_eval(r, faces, uv)
+    Ro, No = o.point_eval(r, faces, uv)
+    assert rel(R, Ro) <= 1e-12 and np.abs(N - No)[ok].max() <= 1e-12
+    rng = np.random.default_rng(18)
+    f2, uv2 = faces[ok], uv[ok]
+    a, b = rng.normal(size=(len(f2), 3)), rng.normal(size=(len(f2), 3))
+    G, P = g.point_grad(r, f2, uv2, a, b, on_surface=True, kT=0.6)
+    Go, Po = o.point_grad(r, f2, uv2, a, b, on_surface=1, kT=0.6)
+    say("   g rel", rel(G, Go), "g_puv rel", rel(P, Po))
+    assert rel(G[:-3], Go[:-3]) <= 1e-12 and rel(G[-3:], Go[-3:]) <= 1e-12 and rel(P, Po) <= 1e-12
+
+
 def test_large_batch_is_sum_of_parts(ves):
     w, g, o = ves
     r = w.r.copy()

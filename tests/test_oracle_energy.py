@@ -66,6 +66,31 @@ def test_fd_gradient_all_terms_helfrich(mesh_dir):
         assert abs(_rich(s, w.r, i, 1e-3) - g[i]) < 2e-7 * scale, i
 
 
+@pytest.mark.parametrize("mode", [O.MODE_LIPID, O.MODE_HELFRICH])
+def test_fd_gradient_valence_7(mesh_dir, mode):
+    """A sphere with valence-5 and valence-7 vertices (edge flips): the
+    gradient at every control point of the valence-7 faces matches Richardson
+    differences of the energy, in both energy forms."""
+    w = workloads.defect_vesicle()
+    s = O.OracleSurface(w.mesh_path)
+    if mode == O.MODE_HELFRICH:
+        s.set_params(kc=5.0, kg=-1.3, KA=0.4, k_reg=2.0, mode=mode)
+        s.set_face_c0(np.random.default_rng(5).uniform(-0.02, 0.02, s.nfaces))
+    else:
+        s.set_params(**w.params)
+    s.setg0(w.r0)
+    g = s.grad(w.r)
+    scale = np.abs(g).max()
+    v7 = [v for v in range(s.nv) if len(s.neighbors(v)) == 7]
+    assert len(v7) == 8
+    idx = set()
+    for v in v7[:3]:
+        for u in [v] + s.neighbors(v):
+            idx.update(3 * u + k for k in range(3))
+    for i in sorted(idx):
+        assert abs(_rich(s, w.r, i, 2e-3) - g[i]) < 2e-7 * scale + 1e-8 * abs(g[i]), i
+
+
 def test_translation_rotation_invariance(ves):
     w, s = ves
     s.set_params(**w.params)

@@ -50,6 +50,7 @@ MESHES = {
     "ico2": lambda: meshgen.icosphere(2, 50.0),
     "ico3": lambda: meshgen.icosphere(3, 100.0),
     "sheet12": lambda: meshgen.hex_sheet(12, 12, 100.0),
+    "defect57": lambda: meshgen.defect_sphere(2, 100.0, 4, 7),   # valence 5 and 7
 }
 
 

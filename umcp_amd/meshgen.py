@@ -197,6 +197,60 @@ def icosphere(level: int, R: float) -> Mesh:
     return _orient_like_faces(Mesh(r, m.rings, np.diag([3.0 * R] * 3)))
 
 
+def flip_edge(mesh: Mesh, a: int, b: int) -> Mesh:
+    """Replaces edge a-b by the edge between its two opposite vertices.
+
+    With triangles (a, b, c) and (a, d, b) on the edge, the new triangles are
+    (a, d, c) and (b, c, d): a and b lose a neighbour, c and d gain one.
+    """
+    rings = [list(x) for x in mesh.rings]
+    ra = rings[a]
+    e = ra.index(b)
+    c, d = ra[(e + 1) % len(ra)], ra[(e - 1) % len(ra)]
+    ra.remove(b)
+    rings[b].remove(a)
+    rc = rings[c]
+    rc.insert(rc.index(a) + 1, d)        # c: ... a, d, b ...
+    rd = rings[d]
+    rd.insert(rd.index(b) + 1, c)        # d: ... b, c, a ...
+    return Mesh(mesh.r.copy(), rings, mesh.pbc.copy())
+
+
+def defect_sphere(level: int, R: float, nflip: int, seed: int) -> Mesh:
+    """A sphere with valence-5 and valence-7 vertices: nflip edges between
+    valence-6 vertices of icosphere(level) are flipped (each makes two
+    valence-5 and two valence-7 vertices), chosen at random among edges whose
+    four vertices are untouched and of valence 6; one Loop step then
+    separates the irregular vertices, and the result is projected to R."""
+    m = icosphere(level, 1.0)
+    rng = np.random.default_rng(seed)
+    touched = set()
+    edges = [(i, j) for i in range(m.nv) for j in m.rings[i] if j > i]
+    order = rng.permutation(len(edges))
+    done = 0
+    for idx in order:
+        if done == nflip:
+            break
+        a, b = edges[idx]
+        ra = m.rings[a]
+        e = ra.index(b)
+        c, d = ra[(e + 1) % len(ra)], ra[(e - 1) % len(ra)]
+        quad = (a, b, c, d)
+        near = set(quad)
+        for v in quad:
+            near.update(m.rings[v])
+        if any(len(m.rings[v]) != 6 for v in quad) or near & touched:
+            continue
+        m = flip_edge(m, a, b)
+        touched.update(near)
+        done += 1
+    if done < nflip:
+        raise ValueError("could not place %d separated flips" % nflip)
+    m = loop_subdivide(m)
+    r = m.r / np.linalg.norm(m.r, axis=1, keepdims=True) * R
+    return _orient_like_faces(Mesh(r, m.rings, np.diag([3.0 * R] * 3)))
+
+
 # --------------------------------------------------------------------------
 # periodic hex sheet
 # --------------------------------------------------------------------------

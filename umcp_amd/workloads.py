@@ -76,6 +76,22 @@ def vesicle(level: int, R: float, sigma: float, seed: int, radial: bool = False,
     return Workload(name or path, path, mesh.nv, r0, r, params, T=T)
 
 
+def defect_vesicle(level: int = 2, R: float = 100.0, nflip: int = 4, sigma: float = 0.5, seed: int = 7,
+                   kc_kT: float = 20.0, KA_kT: float = 0.215, T: float = 298.0) -> Workload:
+    """A vesicle with valence-5 and valence-7 vertices (meshgen.defect_sphere:
+    nflip edge flips on icosphere(level), one Loop step), isotropic noise."""
+    mesh = meshgen.defect_sphere(level, R, nflip, seed)
+    name = "defect_L%d_f%d_s%d" % (level, nflip, seed)
+    path = _write_once(mesh, name)
+    r0 = meshgen.flat_coords(mesh)
+    rng = np.random.default_rng(seed + 1)
+    r = r0.copy()
+    r[:-3] += rng.normal(0.0, sigma, 3 * mesh.nv)
+    params = dict(kc=units.kc_from_kT(kc_kT, T), kg=0.0, KA=units.KA_from_kT(KA_kT, T), k_reg=0.0,
+                  mode=0)
+    return Workload(name, path, mesh.nv, r0, r, params, T=T)
+
+
 def vesicle_642(seed: int = 1) -> Workload:
     return vesicle(3, 100.0, 0.5, seed, name="vesicle_642")
 
