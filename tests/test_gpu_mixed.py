@@ -87,9 +87,10 @@ def test_mixed_vesicle_40962(vesicle_40962_pair, split, flags):
     assert rel_e <= E_TOL and rms <= G_RMS_TOL
 
 
-@pytest.mark.parametrize("mesh", ["vesicle_642", "defect_5_7"])
+@pytest.mark.parametrize("mesh", ["vesicle_642", "defect_5_7", "defect_sheet"])
 def test_mixed_small_meshes_all_shapes(mesh):
-    w = workloads.vesicle_642() if mesh == "vesicle_642" else workloads.defect_vesicle()
+    w = {"vesicle_642": workloads.vesicle_642, "defect_5_7": workloads.defect_vesicle,
+         "defect_sheet": workloads.defect_sheet}[mesh]()
     g, o = pair(w)
     ref = (o.energy(w.r), o.grad(w.r))
     for split, flags in SHAPES:

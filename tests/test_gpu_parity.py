@@ -96,13 +96,14 @@ def test_all_terms_helfrich():
     check_eval(g, o, w.r, "vesicle_2562 helfrich kg kreg KA")
 
 
-@pytest.mark.parametrize("form", ["lipid", "helfrich"])
+@pytest.mark.parametrize("form", ["lipid", "helfrich", "sheet"])
 def test_valence_5_and_7(form):
     """A sphere with valence-5 and valence-7 vertices (four edge flips on an
     icosphere, one Loop step): the valence-7 face class against the oracle,
-    in every launch shape, with both restraints in the Helfrich form."""
-    w = workloads.defect_vesicle()
-    if form == "lipid":
+    in every launch shape, with both restraints in the Helfrich form; and a
+    periodic sheet whose irregular faces reach across the box."""
+    w = workloads.defect_sheet() if form == "sheet" else workloads.defect_vesicle()
+    if form in ("lipid", "sheet"):
         g, o = make_pair(w)
     else:
         g, o = make_pair(w, params=dict(kc=6.0, kg=-2.0, KA=0.5, k_reg=1.5, mode=1))

@@ -197,18 +197,25 @@ def test_periodic_sheet_matches_oracle():
     assert rel(G[:-3], Go[:-3]) <= 1e-12 and rel(G[-3:], Go[-3:]) <= 1e-12 and rel(P, Po) <= 1e-12
 
 
-def test_valence_7_faces_match_oracle():
+@pytest.mark.parametrize("mesh", ["sphere", "sheet"])
+def test_valence_7_faces_match_oracle(mesh):
     """Points on a sphere with valence-5 and valence-7 vertices: the
-    valence-7 sub-patch maps (13 control points) against the oracle."""
-    w = workloads.defect_vesicle()
+    valence-7 sub-patch maps (13 control points) against the oracle; and on
+    a periodic sheet whose irregular faces reach across the box."""
+    w = workloads.defect_vesicle() if mesh == "sphere" else workloads.defect_sheet()
     g, o = _pair(w)
     r = w.r.copy()
     faces, uv = random_points(o, 3000, 17, deep=60)
     ok = well_conditioned(o, faces, uv)
-    say("[points] valence 5/7 sphere: %d irregular points" % int((faces >= o.nreg).sum()))
+    say("[points] valence 5/7 %s: %d irregular points" % (mesh, int((faces >= o.nreg).sum())))
     R, N = g.point_eval(r, faces, uv)
     Ro, No = o.point_eval(r, faces, uv)
     assert rel(R, Ro) <= 1e-12 and np.abs(N - No)[ok].max() <= 1e-12
+    # every irregular face at its extraordinary vertex
+    irr = np.arange(o.nreg, o.nfaces, dtype=np.int32)
+    R0, _ = g.point_eval(r, irr, np.zeros((len(irr), 2)))
+    R0o, _ = o.point_eval(r, irr, np.zeros((len(irr), 2)))
+    assert rel(R0, R0o) <= 1e-12
     rng = np.random.default_rng(18)
     f2, uv2 = faces[ok], uv[ok]
     a, b = rng.normal(size=(len(f2), 3)), rng.normal(size=(len(f2), 3))

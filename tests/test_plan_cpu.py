@@ -51,6 +51,7 @@ MESHES = {
     "ico3": lambda: meshgen.icosphere(3, 100.0),
     "sheet12": lambda: meshgen.hex_sheet(12, 12, 100.0),
     "defect57": lambda: meshgen.defect_sphere(2, 100.0, 4, 7),   # valence 5 and 7
+    "defect_sheet": lambda: meshgen.defect_sheet(16, 400.0, 3, 9),   # ... across periodic images
 }
 
 

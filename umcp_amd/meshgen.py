@@ -138,8 +138,10 @@ def _ccw_ring(r: np.ndarray, v: int, ring: list) -> list:
     return order[s:] + order[:s]
 
 
-def loop_subdivide(mesh: Mesh) -> Mesh:
-    """One Loop step: old vertices keep their ids; edge vertices appended."""
+def loop_subdivide(mesh: Mesh, periodic: bool = False) -> Mesh:
+    """One Loop step: old vertices keep their ids; edge vertices appended.
+    periodic: the mesh tiles its (rectangular) box, and stencils take each
+    neighbour at its nearest image."""
     nvo = mesh.nv
     rings = mesh.rings
     edge_id = {}
@@ -154,17 +156,33 @@ def loop_subdivide(mesh: Mesh) -> Mesh:
         return edge_id[(a, b) if a < b else (b, a)]
 
     r = np.zeros((nv, 3))
+    if periodic:
+        # rectangular periodic box: neighbours at their image nearest the
+        # vertex the stencil is centred on, results wrapped into the box
+        L = np.diag(mesh.pbc)
+
+        def near(a, b):
+            d = mesh.r[b] - mesh.r[a]
+            return mesh.r[a] + (d - L * np.round(d / L))
     for v in range(nvo):
         val = len(rings[v])
         w = 3.0 / (8.0 * val)
-        r[v] = (1 - val * w) * mesh.r[v] + w * mesh.r[rings[v]].sum(axis=0)
+        if periodic:
+            r[v] = (1 - val * w) * mesh.r[v] + w * sum(near(v, j) for j in rings[v])
+        else:
+            r[v] = (1 - val * w) * mesh.r[v] + w * mesh.r[rings[v]].sum(axis=0)
     # triangle-fan successor: in ring order, (i, ring[e], ring[e+1]) is oriented
     for (a, b), m in edge_id.items():
         ra = rings[a]
         e = ra.index(b)
         c = ra[(e + 1) % len(ra)]
         d = ra[(e - 1) % len(ra)]
-        r[m] = 3.0 / 8.0 * (mesh.r[a] + mesh.r[b]) + 1.0 / 8.0 * (mesh.r[c] + mesh.r[d])
+        if periodic:
+            r[m] = 3.0 / 8.0 * (mesh.r[a] + near(a, b)) + 1.0 / 8.0 * (near(a, c) + near(a, d))
+        else:
+            r[m] = 3.0 / 8.0 * (mesh.r[a] + mesh.r[b]) + 1.0 / 8.0 * (mesh.r[c] + mesh.r[d])
+    if periodic:
+        r = r - L * np.floor(r / L)
     new_rings = [None] * nv
     for v in range(nvo):
         new_rings[v] = [mid(v, j) for j in rings[v]]
@@ -249,6 +267,43 @@ def defect_sphere(level: int, R: float, nflip: int, seed: int) -> Mesh:
     m = loop_subdivide(m)
     r = m.r / np.linalg.norm(m.r, axis=1, keepdims=True) * R
     return _orient_like_faces(Mesh(r, m.rings, np.diag([3.0 * R] * 3)))
+
+
+def defect_sheet(n: int, Lx: float, nflip: int, seed: int) -> Mesh:
+    """A periodic n x n hex sheet with valence-5 and valence-7 vertices: nflip
+    separated edge flips, then one periodic Loop step (2n x 2n vertices)."""
+    m = hex_sheet(n, n, Lx)
+    rng = np.random.default_rng(seed)
+    touched = set()
+    edges = [(i, j) for i in range(m.nv) for j in m.rings[i] if j > i]
+    done = 0
+    for idx in rng.permutation(len(edges)):
+        if done == nflip:
+            break
+        a, b = edges[idx]
+        ra = m.rings[a]
+        e = ra.index(b)
+        c, d = ra[(e + 1) % len(ra)], ra[(e - 1) % len(ra)]
+        near = {a, b, c, d}
+        for v in (a, b, c, d):
+            near.update(m.rings[v])
+        if near & touched:
+            continue
+        m = flip_edge(m, a, b)
+        touched.update(near)
+        done += 1
+    if done < nflip:
+        raise ValueError("could not place %d separated flips" % nflip)
+    m = loop_subdivide(m, periodic=True)
+    # move the first valence-7 vertex next to the box corner, so that its
+    # faces reach across both periodic boundaries
+    L = np.diag(m.pbc)
+    v7 = int(np.flatnonzero(m.valence() == 7)[0])
+    shift = 0.15 * (Lx / (2 * n)) - m.r[v7]
+    shift[2] = 0.0
+    r = m.r + shift
+    r = r - L * np.floor(r / L)
+    return Mesh(r, m.rings, m.pbc.copy())
 
 
 # --------------------------------------------------------------------------

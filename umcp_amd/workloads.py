@@ -92,6 +92,22 @@ def defect_vesicle(level: int = 2, R: float = 100.0, nflip: int = 4, sigma: floa
     return Workload(name, path, mesh.nv, r0, r, params, T=T)
 
 
+def defect_sheet(n: int = 16, Lx: float = 400.0, nflip: int = 3, seed: int = 9, kc_kT: float = 20.0,
+                 T: float = 310.0, KA_kT: float = 0.215) -> Workload:
+    """A periodic sheet with valence-5 and valence-7 vertices (meshgen.defect_sheet)
+    under a Fourier height field: irregular faces with periodic images."""
+    mesh = meshgen.defect_sheet(n, Lx, nflip, seed)
+    name = "defect_sheet_%d_f%d_s%d" % (n, nflip, seed)
+    path = _write_once(mesh, name)
+    r0 = meshgen.flat_coords(mesh)
+    h = meshgen.fourier_heights(mesh, kc_kT, 4, seed)
+    r = r0.copy()
+    r[2:-3:3] += h
+    params = dict(kc=units.kc_from_kT(kc_kT, T), kg=0.0, KA=units.KA_from_kT(KA_kT, T), k_reg=0.0,
+                  mode=0)
+    return Workload(name, path, mesh.nv, r0, r, params, T=T)
+
+
 def vesicle_642(seed: int = 1) -> Workload:
     return vesicle(3, 100.0, 0.5, seed, name="vesicle_642")
 
