@@ -120,6 +120,33 @@ def test_larger_mesh_against_numpy_inverse():
     assert np.abs(r_g - r_o).max() <= 1e-9 * disp
 
 
+def test_valence_7_mesh():
+    """The sphere with valence-5 and valence-7 vertices: the mass matrix's
+    lattice points on valence-7 faces (and the vertex mask at their corner)
+    give the oracle's inverse, and a Brownian trajectory matches."""
+    w = workloads.defect_vesicle()
+    g = S.Surface(w.mesh_path)
+    g.set_params(precision=S.PREC_FP64, **w.params)
+    g.generatePlan()
+    g.setg0(w.r0)
+    o = O.OracleSurface(w.mesh_path)
+    o.set_params(**w.params)
+    o.setg0(w.r0)
+    nnz, m = g.effm_build()
+    assert abs(m - o.effm_point_mass()) <= 1e-12 * m
+    Mi_o = o.effm_inverse()
+    Mi = g.effm_get()
+    err = np.abs(Mi - Mi_o).max() / np.abs(Mi_o).max()
+    say("[effm] valence 5/7 sphere: nnz", nnz, "max rel diff", err)
+    assert err <= 1e-12 and nnz == int((Mi_o != 0).sum())
+    kT = units.kT(298.0)
+    xi = np.random.default_rng(29).standard_normal((6, 3 * g.nv))
+    r_g, pp_g = g.effm_steps(w.r, np.zeros(3 * g.nv), xi, S.EFFM_BD, DT, kT, GAMMA)
+    r_o, pp_o = o.effm_steps(w.r, np.zeros(3 * g.nv), xi, S.EFFM_BD, DT, kT, GAMMA, Mi_o)
+    disp = np.abs(r_o - w.r).max()
+    assert np.abs(r_g - r_o).max() <= 1e-9 * disp
+
+
 def test_full_size_vesicle_builds_and_steps():
     """The 40962-vertex vesicle: a 13.4 GB inverse in HBM (far beyond the
     reference's CPU set-up), symmetric with a positive diagonal, and Langevin
