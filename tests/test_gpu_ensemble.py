@@ -44,6 +44,11 @@ def specs():
     out[2]["restraints"] = dict(KA_global=0.3, KV=0.02, A0=0.98 * A, V0=1.02 * V)
     ws = workloads.sheet(32, seed=6, name="ens_sheet32")
     out.append(dict(w=ws, params=dict(ws.params), restraints=None, seed=1003))
+    # valence-5/7 members: a sphere and a periodic sheet (the valence-7 class)
+    wd = workloads.defect_vesicle()
+    out.append(dict(w=wd, params=dict(wd.params), restraints=None, seed=1004))
+    wsd = workloads.defect_sheet()
+    out.append(dict(w=wsd, params=dict(wsd.params), restraints=None, seed=1005))
     return out
 
 
