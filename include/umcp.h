@@ -241,6 +241,18 @@ int umcp_point_eval(umcp_surface *s, const double *r, int n, const int *faces, c
 int umcp_point_grad(umcp_surface *s, const double *r, int n, const int *faces, const double *uv,
                     const double *de_dr, const double *de_dnrm, const double *frac_mult, int on_surface, double kT,
                     double *g, double *g_puv);
+/* Local geometry at the points, replacing surface::ru, surface::rv,
+ * surface::g and surface::c (src/uv_map.C:1347-1700, 884-1106, 2361-2990):
+ * geom[15 i ..] = Ru[3], Rv[3] (alpha-scaled tangents along the face's u and
+ * v), g (the reference's area element: |lru x lrv| of unscaled positions, on
+ * irregular faces with the sub-patch derivatives scaled by the face's own u
+ * and v as the reference does), c1 + c2, c1 c2, c1, c2 (principal curvatures
+ * of the alpha-scaled surface, SAFETY-regularised as in src/uv_map.C:2580) and
+ * the principal directions vec1[2], vec2[2] in (u, v) coordinates, unit
+ * length.  The extraordinary vertex itself is rejected (the reference has no
+ * sub-patch there). */
+#define UMCP_POINT_GEOM 15
+int umcp_point_geometry(umcp_surface *s, const double *r, int n, const int *faces, const double *uv, double *geom);
 
 /* Brownian steps drawing their noise on the device: xi of step k is the
  * counter-based stream of (seed, k) that umcp_noise writes (Philox4x64-10

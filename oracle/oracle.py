@@ -90,6 +90,8 @@ def lib():
         L.orc_point_grad.argtypes = [vp, dp, C.c_int, C.c_double, C.c_double, dp, dp, C.c_double,
                                      C.c_int, C.c_double, dp, dp]
         L.orc_point_grad.restype = C.c_int
+        L.orc_point_geometry.argtypes = [vp, dp, C.c_int, C.c_double, C.c_double, dp]
+        L.orc_point_geometry.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -287,6 +289,18 @@ class OracleSurface:
                 raise ValueError("oracle: point %d outside the reference's domain" % i)
             gp[i] = t
         return g, gp
+
+    def point_geometry(self, r: np.ndarray, faces, uv) -> np.ndarray:
+        """[n, 15]: Ru, Rv, g, c1 + c2, c1 c2, c1, c2, vec1, vec2 at the points."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        uv = np.asarray(uv, dtype=np.float64).reshape(-1, 2)
+        out = np.zeros((uv.shape[0], 15))
+        for i in range(uv.shape[0]):
+            t = np.zeros(15)
+            if lib().orc_point_geometry(self._h, _p(r), int(faces[i]), float(uv[i, 0]), float(uv[i, 1]), _p(t)):
+                raise ValueError("oracle: point %d outside the reference's domain" % i)
+            out[i] = t
+        return out
 
     def face_info(self, f: int):
         owner, ncp, tri = C.c_int(), C.c_int(), C.c_int()

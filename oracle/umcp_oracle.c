@@ -2198,3 +2198,99 @@ int orc_point_grad(const omesh *m, const double *r, int f, double u, double v, c
     }
     return 0;
 }
+
+/* Point geometry: surface::ru / surface::rv (src/uv_map.C:1347-1700: the
+ * alpha-scaled tangents in the face's parameters), surface::g
+ * (src/uv_map.C:884-1106: |lru x lrv| of unscaled positions; on irregular
+ * faces the reference scales the sub-patch derivatives by the face's own u and
+ * v instead of the sub-patch factors, kept here as it computes it) and
+ * surface::c (src/uv_map.C:2361-2990: principal curvatures from the shape
+ * operator of the alpha-scaled surface with SAFETY, their sum and product,
+ * and the principal directions in (u, v)).  out[15] = Ru[3], Rv[3], g,
+ * c1 + c2, c1 c2, c1, c2, vec1[2], vec2[2].  Returns -1 where the reference
+ * stops (the extraordinary vertex itself, levels beyond MAX_DOMAIN, (u, v)
+ * outside an irregular face). */
+int orc_point_geometry(const omesh *m, const double *r, int f, double u, double v, double *out)
+{
+    int nv = m->nv, ncp, cp[OMAXCP], irr;
+    double t[6][OMAXCP], pbc[3 * OMAXCP];
+    double al[3] = {r[3 * nv], r[3 * nv + 1], r[3 * nv + 2]};
+    if (f < 0 || f >= m->nfr + m->nfi) return -1;
+    if (point_tables(m, f, u, v, t, &ncp, cp, pbc, &irr) != 0) return -1;
+    double X[6][3];
+    memset(X, 0, sizeof X);
+    for (int x = 0; x < ncp; x++)
+        for (int c = 0; c < 3; c++) {
+            double q = r[3 * cp[x] + c] + pbc[3 * x + c];
+            for (int d = 0; d < 6; d++) X[d][c] += t[d][x] * q;
+        }
+    /* g(): unscaled tangents; irregular: the sub-patch derivative (the
+     * table's divided by the sub-patch factor, 2^level in magnitude) times
+     * the face's u and v */
+    double gu[3], gv[3], gn[3];
+    double su = 1.0, sv = 1.0;
+    if (irr) {
+        double f_domain = -log10(u + v) / log10(2.0);
+        if (u + v >= 1.0 - 1e-9) f_domain = 1;
+        int domain = (int)lround(ceil(f_domain));
+        double pow2 = pow(2.0, domain - 1.0);
+        double fu = u * pow2, fv = v * pow2;
+        double s = (fu > 0.5 || fv > 0.5) ? 2.0 : -2.0;
+        su = sv = pow2 * s;
+    }
+    for (int c = 0; c < 3; c++) {
+        gu[c] = X[1][c] / su * (irr ? u : 1.0);
+        gv[c] = X[2][c] / sv * (irr ? v : 1.0);
+    }
+    cross3(gu, gv, gn);
+    double g_ref = sqrt(gn[0] * gn[0] + gn[1] * gn[1] + gn[2] * gn[2]);
+    double Ru[3], Rv[3], Suu[3], Suv[3], Svv[3], nrm[3];
+    for (int c = 0; c < 3; c++) {
+        Ru[c] = al[c] * X[1][c];
+        Rv[c] = al[c] * X[2][c];
+        Suu[c] = al[c] * X[3][c];
+        Suv[c] = al[c] * X[4][c];
+        Svv[c] = al[c] * X[5][c];
+    }
+    cross3(Ru, Rv, nrm);
+    normalize3(nrm);
+    double RuRu = Ru[0] * Ru[0] + Ru[1] * Ru[1] + Ru[2] * Ru[2];
+    double RuRv = Ru[0] * Rv[0] + Ru[1] * Rv[1] + Ru[2] * Rv[2];
+    double RvRv = Rv[0] * Rv[0] + Rv[1] * Rv[1] + Rv[2] * Rv[2];
+    double g = sqrt(RuRu * RvRv - RuRv * RuRv);
+    double nsuu = Suu[0] * nrm[0] + Suu[1] * nrm[1] + Suu[2] * nrm[2];
+    double nsuv = Suv[0] * nrm[0] + Suv[1] * nrm[1] + Suv[2] * nrm[2];
+    double nsvv = Svv[0] * nrm[0] + Svv[1] * nrm[1] + Svv[2] * nrm[2];
+    double a = 1.0 / (g * g) * (nsuu * RvRv - nsuv * RuRv);
+    double b = (1.0 / (g * g)) * (nsuv * RvRv - nsvv * RuRv);
+    double cc = 1.0 / (g * g) * (nsuv * RuRu - nsuu * RuRv);
+    double d = (1.0 / (g * g)) * (nsvv * RuRu - nsuv * RuRv);
+    double fac = sqrt(SAFETY_ + a * a + 4 * b * cc - 2 * a * d + d * d);
+    double c1 = -0.5 * (a + d - fac), c2 = -0.5 * (a + d + fac);
+    double v1[2], v2[2];
+    if (fabs(cc) > 1e-6) {
+        v1[0] = -(-a + d + fac) / (2 * cc); v1[1] = 1.0;
+        v2[0] = -(-a + d - fac) / (2 * cc); v2[1] = 1.0;
+    } else if (fabs(b) > 1e-6) {
+        v1[0] = -(b) / (a - d); v1[1] = 1.0;
+        v2[0] = 1; v2[1] = 0;
+    } else {
+        v1[0] = 1; v1[1] = 0;
+        v2[0] = -RuRv / sqrt(RuRu) / sqrt(RvRv); v2[1] = 1;
+    }
+    double l1 = sqrt(v1[0] * v1[0] + v1[1] * v1[1]), l2 = sqrt(v2[0] * v2[0] + v2[1] * v2[1]);
+    for (int c = 0; c < 3; c++) {
+        out[c] = Ru[c];
+        out[3 + c] = Rv[c];
+    }
+    out[6] = g_ref;
+    out[7] = c1 + c2;
+    out[8] = c1 * c2;
+    out[9] = c1;
+    out[10] = c2;
+    out[11] = v1[0] / l1;
+    out[12] = v1[1] / l1;
+    out[13] = v2[0] / l2;
+    out[14] = v2[1] / l2;
+    return 0;
+}

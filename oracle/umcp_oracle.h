@@ -107,6 +107,10 @@ int    orc_point_grad(const omesh *m, const double *r, int f, double u, double v
                       const double *de_dr, const double *de_dnrm, double frac_mult, int on_surface,
                       double kT, double *gr, double *g_puv);
 
+/* point geometry (surface::ru, rv, g, c): out[15] = Ru[3], Rv[3], g,
+ * c1 + c2, c1 c2, c1, c2, principal directions vec1[2], vec2[2] in (u, v) */
+int    orc_point_geometry(const omesh *m, const double *r, int f, double u, double v, double *out);
+
 #ifdef __cplusplus
 }
 #endif

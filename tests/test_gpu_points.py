@@ -264,3 +264,61 @@ def test_rejections(ves):
     R, _ = g.point_eval(r, irr, np.array([[0.2, 0.2]]))
     Ro, _ = o.point_eval(r, irr, np.array([[0.2, 0.2]]))
     assert rel(R, Ro) <= 1e-12
+
+
+@pytest.mark.parametrize("mesh,alpha", [("vesicle", (1.0, 1.0, 1.0)), ("vesicle", (1.1, 0.9, 1.05)),
+                                        ("defect_sphere", (1.0, 1.0, 1.0)), ("defect_sheet", (1.0, 1.0, 1.0))])
+def test_geometry_matches_oracle(ves, mesh, alpha):
+    """surface::ru, rv, g and c at random well-conditioned points.  Bars:
+    tangents, the reference's area element and c1 + c2 to 1e-12 of their
+    scale s.  The individual principal curvatures are
+    -(a + d)/2 -+ sqrt(SAFETY + (a - d)^2 + 4bc)/2: a rounding change dD of the
+    discriminant moves them by dD / (4 |c1 - c2|), which grows without bound at
+    umbilic points (a sphere is umbilic everywhere), and the principal
+    directions are undefined there.  So c1, c2 and c1 c2 are held to
+    1e-12 s + 1e-14 s^2 / |c1 - c2| and the unit directions to
+    1e-10 + 1e-14 s / |c1 - c2|, per point.  On an irregular face at
+    sub-patch level d the second-derivative weights are the level's map
+    entries times 4^d over sums that cancel, so the two implementations'
+    rounding of the maps reaches the curvatures multiplied by 4^d (measured:
+    regular faces 1e-13, levels 2-4 up to 6e-12): every curvature bar carries
+    that factor."""
+    if mesh == "vesicle":
+        w, g, o = ves
+    else:
+        w = workloads.defect_vesicle() if mesh == "defect_sphere" else workloads.defect_sheet()
+        g, o = _pair(w)
+    r = w.r.copy()
+    r[-3:] = alpha
+    faces, uv = random_points(o, 2000, 31)
+    ok = well_conditioned(o, faces, uv)
+    faces, uv = faces[ok], uv[ok]
+    say("[points] geometry %s alpha=%s n=%d" % (mesh, alpha, len(faces)))
+    G = g.point_geometry(r, faces, uv)
+    Go = o.point_geometry(r, faces, uv)
+    tang = np.abs(G[:, :6] - Go[:, :6]).max() / np.abs(Go[:, :6]).max()
+    area = np.abs(G[:, 6] - Go[:, 6]).max() / np.abs(Go[:, 6]).max()
+    s = np.abs(Go[:, 9:11]).max()
+    t = uv.sum(axis=1)
+    level = np.where(faces >= o.nreg, np.where(t >= 1 - 1e-9, 1, np.ceil(-np.log2(t))), 0)
+    cond = 4.0 ** level
+    dH = np.abs(G[:, 7] - Go[:, 7])
+    split = np.maximum(np.abs(Go[:, 9] - Go[:, 10]), 1e-300)
+    dc = np.abs(G[:, 9:11] - Go[:, 9:11]).max(axis=1)
+    dk = np.abs(G[:, 8] - Go[:, 8])
+    dv = np.abs(G[:, 11:] - Go[:, 11:]).max(axis=1)
+    reg = faces < o.nreg
+    say("   tangents", tang, "g", area, "c1+c2 regular", dH[reg].max() / s, "irregular",
+        (dH[~reg].max() / s) if (~reg).any() else 0.0, "min |c1-c2|/s", split.min() / s,
+        "directions max", dv.max())
+    assert tang <= 1e-12 and area <= 1e-12
+    assert (dH <= 1e-12 * s * cond).all()
+    assert (dc <= (1e-12 * s + 1e-14 * s * s / split) * cond).all()
+    assert (dk <= (1e-12 * s * s + 1e-14 * s ** 3 / split) * cond).all()
+    assert (dv <= 1e-10 + 1e-14 * s / split * cond).all()
+
+
+def test_geometry_rejects_the_vertex(ves):
+    w, g, o = ves
+    with pytest.raises(S.UmcpError):
+        g.point_geometry(w.r, np.array([o.nreg], np.int32), np.zeros((1, 2)))

@@ -17,7 +17,14 @@ pinned by what the mathematics fixes rather than by retyping its formulas:
   gradient of de_dr.R (regular faces, alpha != 1); on_surface adds the
   gradient of -kT log g; frac_mult scales the vertex gradient only;
 - the reference's rejections: the extraordinary vertex itself for the
-  gradient (a NULL map there), levels beyond MAX_DOMAIN = 100.
+  gradient (a NULL map there), levels beyond MAX_DOMAIN = 100;
+- point geometry (surface::ru, rv, g, c): the tangents are central
+  differences of R; with alpha != 1 everything but g equals the geometry of
+  alpha-scaled positions at alpha = 1; g is |Ru x Rv| on regular faces and
+  the reference's u v / 4^level scaling of it on irregular ones; the
+  Weingarten map from central differences of the normal gives c1 + c2, c1 c2
+  and the principal directions (its eigenvectors), which map to orthogonal
+  tangents.
 """
 import numpy as np
 import pytest
@@ -218,3 +225,98 @@ def test_reference_rejections(ves):
     ref = 0.5 * q[i] + q[nb].sum(axis=0) / (2 * len(nb))
     assert np.abs(R[0] - ref).max() < 1e-12 * np.abs(ref).max()
     assert np.isnan(N).all()
+
+
+# ---------------------------------------------------------------------------
+# point geometry (surface::ru, rv, g, c; src/uv_map.C:884-2990)
+# ---------------------------------------------------------------------------
+
+GEOM_CASES = [("regular", 1, (0.2, 0.3)), ("regular", 2, (0.6, 0.1)), ("irregular", 0, (0.3, 0.3)),
+              ("irregular", 1, (0.01, 0.13)), ("irregular", 2, (0.7, 0.2)), ("irregular", 3, (0.15, 0.05))]
+
+
+def _case(o, kind, k):
+    reg, irr = _faces(o)
+    return (reg if kind == "regular" else irr)[k]
+
+
+def test_geometry_tangents_are_derivatives(ves):
+    """Ru, Rv are central differences of the evaluated (alpha-scaled) R."""
+    w, o = ves
+    r = w.r.copy()
+    r[-3:] = [1.1, 0.9, 1.05]
+    for kind, k, (u, v) in GEOM_CASES:
+        f = _case(o, kind, k)
+        G = o.point_geometry(r, [f], [(u, v)])[0]
+        Ru_fd, Rv_fd, _, _ = _fd_uv(o, r, f, u, v)
+        s = np.abs(Ru_fd).max() + np.abs(Rv_fd).max()
+        assert np.abs(G[0:3] - Ru_fd).max() < 1e-6 * s and np.abs(G[3:6] - Rv_fd).max() < 1e-6 * s, (kind, k)
+
+
+def test_geometry_alpha_is_a_scaling_of_positions(ves):
+    """Everything but g is the geometry of the surface through alpha-scaled
+    positions with alpha = 1 (g, like the reference's, ignores alpha)."""
+    w, o = ves
+    al = np.array([1.1, 0.9, 1.05])
+    r = w.r.copy()
+    r[-3:] = al
+    r1 = np.concatenate([(w.r[:-3].reshape(-1, 3) * al).reshape(-1), [1.0, 1.0, 1.0]])
+    for kind, k, uv in GEOM_CASES:
+        f = _case(o, kind, k)
+        A = o.point_geometry(r, [f], [uv])[0]
+        B = o.point_geometry(r1, [f], [uv])[0]
+        assert np.allclose(A[[0, 1, 2, 3, 4, 5, 7, 8, 9, 10]], B[[0, 1, 2, 3, 4, 5, 7, 8, 9, 10]], rtol=1e-12,
+                           atol=1e-14), (kind, k)
+        G1 = o.point_geometry(w.r, [f], [uv])[0]
+        assert A[6] == G1[6]
+
+
+def test_geometry_area_element(ves):
+    """g = |Ru x Rv| on regular faces; on irregular faces the reference's
+    scaling by the face's own u v over the squared sub-patch factor 4^level."""
+    w, o = ves
+    r = w.r.copy()
+    for kind, k, (u, v) in GEOM_CASES:
+        f = _case(o, kind, k)
+        G = o.point_geometry(r, [f], [(u, v)])[0]
+        gn = np.linalg.norm(np.cross(G[0:3], G[3:6]))
+        if kind == "regular":
+            assert abs(G[6] - gn) < 1e-12 * gn
+        else:
+            level = int(np.ceil(-np.log2(u + v))) if u + v < 1 - 1e-9 else 1
+            assert abs(G[6] - gn * u * v / 4.0 ** level) < 1e-12 * gn * u * v / 4.0 ** level, (k, level)
+
+
+def test_geometry_curvature_from_the_normal(ves):
+    """With alpha = 1 the Weingarten map from central differences of the
+    evaluated normal, W = -(J^T J)^-1 J^T [n_u n_v] with J = [Ru Rv], gives
+    c1 + c2 = -tr W and c1 c2 = det W; vec1 and vec2 are its eigenvectors for
+    -c1 and -c2 and map to orthogonal tangent directions."""
+    w, o = ves
+    r = w.r.copy()
+    for kind, k, (u, v) in GEOM_CASES:
+        f = _case(o, kind, k)
+        G = o.point_geometry(r, [f], [(u, v)])[0]
+        _, _, Nu, Nv = _fd_uv(o, r, f, u, v, h=1e-5)
+        J = np.stack([G[0:3], G[3:6]], 1)
+        W = -np.linalg.solve(J.T @ J, J.T @ np.stack([Nu, Nv], 1))
+        c1, c2 = G[9], G[10]
+        sc = abs(c1) + abs(c2)
+        assert abs(G[7] - (c1 + c2)) < 1e-14 * sc and abs(G[8] - c1 * c2) < 1e-14 * sc * sc
+        assert abs(-np.trace(W) - G[7]) < 1e-6 * sc, (kind, k)
+        assert abs(np.linalg.det(W) - G[8]) < 1e-6 * sc * sc, (kind, k)
+        v1, v2 = G[11:13], G[13:15]
+        assert abs(np.linalg.norm(v1) - 1) < 1e-14 and abs(np.linalg.norm(v2) - 1) < 1e-14
+        assert np.abs(W @ v1 + c1 * v1).max() < 1e-5 * sc, (kind, k)
+        assert np.abs(W @ v2 + c2 * v2).max() < 1e-5 * sc, (kind, k)
+        t1, t2 = J @ v1, J @ v2
+        assert abs(t1 @ t2) < 1e-6 * np.linalg.norm(t1) * np.linalg.norm(t2)
+
+
+def test_geometry_rejections(ves):
+    w, o = ves
+    _, irr = _faces(o)
+    with pytest.raises(ValueError):
+        o.point_geometry(w.r, [irr[0]], [(0.0, 0.0)])
+    with pytest.raises(ValueError):
+        o.point_geometry(w.r, [irr[0]], [(1e-40, 1e-40)])

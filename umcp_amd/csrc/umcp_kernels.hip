@@ -1764,6 +1764,85 @@ __device__ __forceinline__ void irr_weights(const double *__restrict__ map, int 
 // One point of evaluateRNRM or pointGradient.  IRR: the control-point weights
 // of an irregular face are the sub-patch map applied to the basis, t[d][x] =
 // s_d sum_y map[y][x] B[d][y]; on regular faces t[d][x] = B[d][x].
+// surface::ru, rv, g and c at one point from its contractions X (unscaled
+// positions; derivatives in the face's parameters): the tangents, shape
+// operator and principal curvatures of the alpha-scaled surface, and the
+// reference's g — |lru x lrv| of unscaled positions, on irregular faces with
+// the sub-patch derivatives scaled by the face's own u and v
+// (src/uv_map.C:884-1106, 1347-1700, 2361-2990)
+__device__ void point_geometry_out(const PointArgs &a, int p, const PointRec &pr, bool irr, const double al[3],
+                                   const double X[6][3])
+{
+    double gu[3], gv[3], gn[3];
+    const double mu = irr ? pr.fu / pr.su : 1.0, mv = irr ? pr.fv / pr.sv : 1.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        gu[k] = X[1][k] * mu;
+        gv[k] = X[2][k] * mv;
+    }
+    cross_d(gu, gv, gn);
+    double Ru[3], Rv[3], Suu[3], Suv[3], Svv[3], n[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        Ru[k] = al[k] * X[1][k];
+        Rv[k] = al[k] * X[2][k];
+        Suu[k] = al[k] * X[3][k];
+        Suv[k] = al[k] * X[4][k];
+        Svv[k] = al[k] * X[5][k];
+    }
+    cross_d(Ru, Rv, n);
+    const double ln = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (ln > 0.0) {
+        n[0] /= ln;
+        n[1] /= ln;
+        n[2] /= ln;
+    }
+    const double E = Ru[0] * Ru[0] + Ru[1] * Ru[1] + Ru[2] * Ru[2];
+    const double F = Ru[0] * Rv[0] + Ru[1] * Rv[1] + Ru[2] * Rv[2];
+    const double G = Rv[0] * Rv[0] + Rv[1] * Rv[1] + Rv[2] * Rv[2];
+    const double g2 = E * G - F * F;
+    const double L = Suu[0] * n[0] + Suu[1] * n[1] + Suu[2] * n[2];
+    const double M = Suv[0] * n[0] + Suv[1] * n[1] + Suv[2] * n[2];
+    const double N = Svv[0] * n[0] + Svv[1] * n[1] + Svv[2] * n[2];
+    // shape operator I^-1 II in the (u, v) basis
+    const double sa = (L * G - M * F) / g2, sb = (M * G - N * F) / g2;
+    const double sc = (M * E - L * F) / g2, sd = (N * E - M * F) / g2;
+    const double fac = sqrt(1e-12 + sa * sa + 4.0 * sb * sc - 2.0 * sa * sd + sd * sd);   // SAFETY, src/interp.h:14
+    const double c1 = -0.5 * (sa + sd - fac), c2 = -0.5 * (sa + sd + fac);
+    double v1[2], v2[2];
+    if (fabs(sc) > 1e-6) {
+        v1[0] = (sa - sd - fac) / (2.0 * sc);
+        v2[0] = (sa - sd + fac) / (2.0 * sc);
+        v1[1] = v2[1] = 1.0;
+    } else if (fabs(sb) > 1e-6) {
+        v1[0] = -sb / (sa - sd);
+        v1[1] = 1.0;
+        v2[0] = 1.0;
+        v2[1] = 0.0;
+    } else {
+        v1[0] = 1.0;
+        v1[1] = 0.0;
+        v2[0] = -F / sqrt(E) / sqrt(G);
+        v2[1] = 1.0;
+    }
+    const double l1 = sqrt(v1[0] * v1[0] + v1[1] * v1[1]), l2 = sqrt(v2[0] * v2[0] + v2[1] * v2[1]);
+    double *o = a.geom + (size_t)p * kPointGeom;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        o[k] = Ru[k];
+        o[3 + k] = Rv[k];
+    }
+    o[6] = sqrt(gn[0] * gn[0] + gn[1] * gn[1] + gn[2] * gn[2]);
+    o[7] = c1 + c2;
+    o[8] = c1 * c2;
+    o[9] = c1;
+    o[10] = c2;
+    o[11] = v1[0] / l1;
+    o[12] = v1[1] / l1;
+    o[13] = v2[0] / l2;
+    o[14] = v2[1] / l2;
+}
+
 // p: the caller's index of the point, j: its face-sorted position (gradient)
 template <bool IRR>
 __device__ void point_body(const PointArgs &a, int p, int j, const PointRec &pr, const PointClass &C)
@@ -1793,7 +1872,7 @@ __device__ void point_body(const PointArgs &a, int p, int j, const PointRec &pr,
     // the reference forms the regular gradient's surface from alpha-scaled
     // positions, the irregular one and the evaluation's tangents from
     // unscaled ones (src/grad.C:5822-5850, 6185-6220; src/interpLib.C:8140-8196)
-    const bool scale_q = a.grad && !IRR;
+    const bool scale_q = a.grad == 1 && !IRR;
     double X[6][3];
 #pragma unroll
     for (int d = 0; d < 6; d++) X[d][0] = X[d][1] = X[d][2] = 0.0;
@@ -1824,6 +1903,10 @@ __device__ void point_body(const PointArgs &a, int p, int j, const PointRec &pr,
             const double t[6] = {B[0][x], B[1][x], B[2][x], B[3][x], B[4][x], B[5][x]};
             accumulate(x, t);
         }
+    }
+    if (a.grad == 2) {
+        point_geometry_out(a, p, pr, IRR, al, X);
+        return;
     }
     if (!a.grad) {
         double n[3];
@@ -1920,7 +2003,7 @@ __global__ __launch_bounds__(kPointBlock) void point_kernel(PointArgs a, const i
     const int t = blockIdx.x * kPointBlock + threadIdx.x;
     if (t >= count) return;
     const int j = idx[t];
-    const int p = a.grad ? a.order[j] : j;
+    const int p = a.grad == 1 ? a.order[j] : j;
     const PointRec pr = a.rec[p];
     const PointClass C = a.cls[pr.cls];
     if constexpr (KIND == kPointRegular) {

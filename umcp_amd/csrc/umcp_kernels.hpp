@@ -159,6 +159,7 @@ enum : int { kPointRegular = 0, kPointIrregular = 1, kPointVertex = 2 };
 struct PointRec {
     int cls, fc, kind, map;
     double u, v, su, sv;
+    double fu, fv;           // the face parameters as given
 };
 
 struct PointClass {
@@ -178,7 +179,7 @@ struct PointSeg {
 struct PointArgs {
     const double *r;         // 3 nv + 3, alpha last
     int nv, n;
-    int grad;                // 0: position and normal, 1: gradient
+    int grad;                // 0: position and normal, 1: gradient, 2: geometry
     int on_surface;
     double kT;
     double pbc[3][3];
@@ -189,7 +190,10 @@ struct PointArgs {
     double *R, *nrm;         // position and unit normal [n][3]
     double *contrib;         // dE/dr per (slot, sorted position): [kPointSlots][n][3], no alpha
     double *gpuv;            // (u, v) gradient per point [n][2]
+    double *geom;            // geometry per point [n][kPointGeom]
 };
+// geometry record: Ru[3], Rv[3], g, c1 + c2, c1 c2, c1, c2, vec1[2], vec2[2]
+constexpr int kPointGeom = 15;
 // idx: the positions (eval: caller indices; gradient: face-sorted positions)
 // of the regular, irregular and extraordinary-vertex points in that order,
 // counts[kind] of each (the vertex kind: evaluation only)

@@ -139,12 +139,13 @@ int point_setup(umcp_surface *s)
 // 380-541) on the host, where the level is found with the same libm log10
 // the reference uses; the sub-patch arithmetic is exact (powers of two).
 // Returns 0 and fills rec, or a message for the inputs the reference stops on.
-const char *resolve_irregular(double fu, double fv, bool grad, PointRec &rec)
+const char *resolve_irregular(double fu, double fv, bool vertex_ok, PointRec &rec)
 {
     double f_domain = -std::log10(fu + fv) / std::log10(2.0);
     if (!(f_domain < 1e300) || !(f_domain > -1e300)) {
-        // the extraordinary vertex itself (src/interpLib.C:7963)
-        if (grad) return "no gradient at an irregular face's extraordinary vertex (the reference has no map there)";
+        // the extraordinary vertex itself (src/interpLib.C:7963); only the
+        // evaluation has a rule there, the reference's map is NULL otherwise
+        if (!vertex_ok) return "the extraordinary vertex of an irregular face has no sub-patch (the reference has no map there)";
         if (!(fu < 1e-10 && fv < 1e-10)) return "(u, v) outside the face";
         rec.kind = kPointVertex;
         rec.map = 0;
@@ -211,10 +212,12 @@ int pin_grow(umcp_surface *s, char **buf, size_t *cap, size_t need)
     return UMCP_OK;
 }
 
-// resolves the points, orders a gradient batch by face, stages and launches
+// resolves the points, orders a gradient batch by face, stages and launches;
+// mode 0 evaluation, 1 gradient, 2 geometry
 int run_points(umcp_surface *s, const double *r, int n, const int *faces, const double *uv, const double *de_dr,
-               const double *de_dnrm, const double *frac, int on_surface, double kT, bool grad, Batch &b)
+               const double *de_dnrm, const double *frac, int on_surface, double kT, int mode, Batch &b)
 {
+    const bool grad = mode == 1;
     const int nv = s->T.nv;
     const Plan &P = s->P;
     std::vector<PointRec> rec((size_t)n);
@@ -226,6 +229,8 @@ int run_points(umcp_surface *s, const double *r, int n, const int *faces, const 
         q.cls = s->pt_face_cls[(size_t)f];
         q.fc = s->pt_face_fc[(size_t)f];
         const double u = uv[2 * i], v = uv[2 * i + 1];
+        q.fu = u;
+        q.fv = v;
         if (!std::isfinite(u) || !std::isfinite(v))
             return pfail(s, UMCP_ERR_ARG, "point " + std::to_string(i) + ": non-finite (u, v)");
         if (P.classes[(size_t)q.cls].valence == 6) {
@@ -234,7 +239,7 @@ int run_points(umcp_surface *s, const double *r, int n, const int *faces, const 
             q.u = u;
             q.v = v;
             q.su = q.sv = 1.0;
-        } else if (const char *msg = resolve_irregular(u, v, grad, q)) {
+        } else if (const char *msg = resolve_irregular(u, v, mode == 0, q)) {
             return pfail(s, UMCP_ERR_ARG, "point " + std::to_string(i) + ": " + msg);
         }
     }
@@ -283,6 +288,10 @@ int run_points(umcp_surface *s, const double *r, int n, const int *faces, const 
         b.off_con = o;    o = align_up(o + sizeof(double) * (size_t)n * kPointSlots * 3);
         b.off_apart = o;  o = align_up(o + sizeof(double) * (size_t)nseg * kPointSlots * 3);
         b.off_R = b.off_N = 0;
+    } else if (mode == 2) {
+        b.off_R = o;   // the geometry records
+        o = align_up(o + sizeof(double) * kPointGeom * (size_t)n);
+        b.off_N = b.off_gout = b.off_gp = b.off_con = b.off_apart = 0;
     } else {
         b.off_R = o;
         b.off_N = o + sizeof(double) * 3 * (size_t)n;
@@ -312,7 +321,7 @@ int run_points(umcp_surface *s, const double *r, int n, const int *faces, const 
     a.r = (const double *)(di + b.off_r);
     a.nv = nv;
     a.n = n;
-    a.grad = grad ? 1 : 0;
+    a.grad = mode;
     a.on_surface = on_surface ? 1 : 0;
     a.kT = kT;
     std::memcpy(a.pbc, s->T.pbc, sizeof a.pbc);
@@ -325,6 +334,8 @@ int run_points(umcp_surface *s, const double *r, int n, const int *faces, const 
         a.frac = frac ? (const double *)(di + b.off_frac) : nullptr;
         a.contrib = (double *)(dout + b.off_con);
         a.gpuv = (double *)(dout + b.off_gp);
+    } else if (mode == 2) {
+        a.geom = (double *)(dout + b.off_R);
     } else {
         a.R = (double *)(dout + b.off_R);
         a.nrm = (double *)(dout + b.off_N);
@@ -376,7 +387,7 @@ int umcp_point_eval(umcp_surface *s, const double *r, int n, const int *faces, c
     if (n == 0) return UMCP_OK;
     if (!R || !nrm) return pfail(s, UMCP_ERR_ARG, "null output arrays");
     Batch b;
-    rc = run_points(s, r, n, faces, uv, nullptr, nullptr, nullptr, 0, 0.0, false, b);
+    rc = run_points(s, r, n, faces, uv, nullptr, nullptr, nullptr, 0, 0.0, 0, b);
     if (rc) return rc;
     hipError_t e = hipMemcpyAsync(R, s->d_pt_out + b.off_R, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost,
                                   s->stream);
@@ -397,7 +408,7 @@ int umcp_point_grad(umcp_surface *s, const double *r, int n, const int *faces, c
     if (n == 0) return UMCP_OK;
     if (!de_dr || !de_dnrm || !g || !g_puv) return pfail(s, UMCP_ERR_ARG, "null gradient arrays");
     Batch b;
-    rc = run_points(s, r, n, faces, uv, de_dr, de_dnrm, frac_mult, on_surface, kT, true, b);
+    rc = run_points(s, r, n, faces, uv, de_dr, de_dnrm, frac_mult, on_surface, kT, 1, b);
     if (rc) return rc;
     const size_t cnt = 3 * (size_t)s->T.nv + 3 + 2 * (size_t)n;   // vertex sums, alpha, gpuv: contiguous
     rc = pin_grow(s, &s->pt_host, &s->pt_host_cap, sizeof(double) * cnt);
@@ -410,6 +421,22 @@ int umcp_point_grad(umcp_surface *s, const double *r, int n, const int *faces, c
     // the reference's contract: the caller's arrays accumulate (+=)
     for (size_t i = 0; i < 3 * (size_t)s->T.nv + 3; i++) g[i] += gs[i];
     for (size_t i = 0; i < 2 * (size_t)n; i++) g_puv[i] += gp[i];
+    return UMCP_OK;
+}
+
+int umcp_point_geometry(umcp_surface *s, const double *r, int n, const int *faces, const double *uv, double *geom)
+{
+    int rc = point_check(s, r, n, faces, uv);
+    if (rc) return rc;
+    if (n == 0) return UMCP_OK;
+    if (!geom) return pfail(s, UMCP_ERR_ARG, "null output array");
+    Batch b;
+    rc = run_points(s, r, n, faces, uv, nullptr, nullptr, nullptr, 0, 0.0, 2, b);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(geom, s->d_pt_out + b.off_R, sizeof(double) * kPointGeom * (size_t)n,
+                                  hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return phip(s, e, "download point geometry");
     return UMCP_OK;
 }
 

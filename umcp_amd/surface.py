@@ -43,7 +43,7 @@ EXPORTS = [
     "umcp_server_drain", "umcp_server_stats", "umcp_server_last_error", "umcp_server_destroy",
     "umcp_effm_build", "umcp_effm_get", "umcp_effm_steps", "umcp_effm_steps_dev",
     "umcp_save_mesh", "umcp_write_restart", "umcp_read_restart", "umcp_write_limit_surface",
-    "umcp_point_eval", "umcp_point_grad", "umcp_write_limit_psf",
+    "umcp_point_eval", "umcp_point_grad", "umcp_point_geometry", "umcp_write_limit_psf",
 ]
 
 
@@ -140,6 +140,7 @@ def load_library(path: str = LIB_PATH):
         "umcp_write_limit_surface": ([vp, dp, C.c_char_p, C.c_int], C.c_int),
         "umcp_point_eval": ([vp, dp, C.c_int, ip, dp, dp, dp], C.c_int),
         "umcp_write_limit_psf": ([vp, C.c_char_p], C.c_int),
+        "umcp_point_geometry": ([vp, dp, C.c_int, ip, dp, dp], C.c_int),
         "umcp_point_grad": ([vp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int, C.c_double, dp, dp], C.c_int),
         "umcp_write_restart": ([C.c_char_p, C.c_int, dp, dp, C.c_int], C.c_int),
         "umcp_read_restart": ([C.c_char_p, C.c_int, dp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
@@ -366,6 +367,16 @@ class Surface:
                                             _dp(fm) if fm is not None else None, 1 if on_surface else 0,
                                             float(kT), _dp(g), _dp(gp)), "point_grad")
         return g, gp
+
+    def point_geometry(self, r: np.ndarray, faces, uv) -> np.ndarray:
+        """[n, 15]: Ru, Rv, g, c1 + c2, c1 c2, c1, c2, vec1, vec2 (ru, rv, g, c)."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        f = np.ascontiguousarray(faces, dtype=np.int32)
+        uv = np.ascontiguousarray(uv, dtype=np.float64).reshape(-1, 2)
+        out = np.zeros((f.size, 15))
+        self._check(self._L.umcp_point_geometry(self._h, _dp(r), f.size, _ip(f), _dp(uv), _dp(out)),
+                    "point_geometry")
+        return out
 
     def write_limit_psf(self, path: str):
         """The PSF topology of the limit-surface frames (writeLimitingSurfacePSF)."""
