@@ -388,7 +388,14 @@ def run_points_leg(args, dev, torch):
     out["eval_points_per_s"] = n / te
     out["grad_s"] = tg
     out["grad_points_per_s"] = n / tg
-    out["finite"] = bool(np.isfinite(R).all() and np.isfinite(g).all())
+    # the geometry query (ru, rv, g, c)
+    s.point_geometry(r, faces[:1000], uv[:1000])
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        Gm = s.point_geometry(r, faces, uv)
+    out["geometry_s"] = (time.perf_counter() - t0) / reps
+    out["geometry_points_per_s"] = n / out["geometry_s"]
+    out["finite"] = bool(np.isfinite(R).all() and np.isfinite(g).all() and np.isfinite(Gm).all())
     if not args.no_cpu_baseline:
         from oracle import oracle as O
         o = O.OracleSurface(w.mesh_path)
