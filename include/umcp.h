@@ -121,6 +121,15 @@ int umcp_energy_grad(umcp_surface *s, const double *r, double *E, double *g);
 /* surface::area (src/interp.h:301): geometric area sum(dudv g w) and
  * reference area sum(dudv g0 w) */
 int umcp_area(umcp_surface *s, const double *r, double *A, double *A0);
+/* Energies of single faces and of the faces a vertex moves: umcp_face_energies
+ * writes surface::faceEnergy (src/interpLib.C:3405-3424) of every face, in the
+ * reference's face order, into E[nfaces]; umcp_energy_vertex replaces
+ * surface::energy(r, NULL, do_vertex) with do_vertex >= 0
+ * (src/interpLib.C:4262-4295): the energy of every face that has the vertex
+ * among its control points, summed in computeModifiedVertices' order (faces
+ * ascending, one term per occurrence), without the global restraint terms. */
+int umcp_face_energies(umcp_surface *s, const double *r, double *E);
+int umcp_energy_vertex(umcp_surface *s, const double *r, int vertex, double *E);
 
 /* nsteps Brownian steps from host arrays; xi is [nsteps][3nv] standard
  * normals; r (3nv+3) is updated in place; energies (nsteps, may be NULL)

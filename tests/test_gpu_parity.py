@@ -195,3 +195,45 @@ def test_device_entry_points(ves642):
     torch.cuda.synchronize()           # device-wide: waits for that stream too
     assert float(d_E.cpu()[0]) == E_h
     np.testing.assert_array_equal(d_g.cpu().numpy(), g_h)
+
+
+@pytest.mark.parametrize("case", ["vesicle_lipid", "defect_helfrich"])
+def test_face_and_vertex_energies(case):
+    """surface::faceEnergy of every face and energy(r, NULL, do_vertex): each
+    face against the oracle's to 1e-12 of the largest; the faces sum to the
+    total energy; a vertex's energy is the sum over the faces that have it as
+    a control point (computeModifiedVertices' list, duplicates included); and
+    moving that vertex alone changes the total energy by exactly the change
+    of its vertex energy (a face it moves but the list missed would break it)."""
+    if case == "vesicle_lipid":
+        w = workloads.vesicle_642()
+        g, o = make_pair(w)
+    else:
+        w = workloads.defect_vesicle()
+        g, o = make_pair(w, params=dict(kc=6.0, kg=-2.0, KA=0.5, k_reg=1.5, mode=1))
+        c0 = workloads.face_c0_uniform(g.nfaces, 8, -0.02, 0.02)
+        g.set_face_c0(c0)
+        o.set_face_c0(c0)
+    r = w.r
+    Ef = g.face_energies(r)
+    Eo = np.array([o.face_energy(f, r) for f in range(o.nfaces)])
+    say("[parity] face energies", case, "max rel", np.abs(Ef - Eo).max() / np.abs(Eo).max())
+    assert np.abs(Ef - Eo).max() <= 1e-12 * np.abs(Eo).max()
+    E_tot = g.energy(r)
+    assert abs(Ef.sum() - E_tot) <= 1e-12 * abs(E_tot)
+    faces_of = [[] for _ in range(o.nv)]
+    for f in range(o.nfaces):
+        for v in o.face_info(f)["cp"]:
+            faces_of[v].append(f)
+    rng = np.random.default_rng(41)
+    for v in list(rng.choice(o.nv, 12, replace=False)) + [0, o.nv - 1]:
+        Ev = g.energy_vertex(r, int(v))
+        ref = sum(o.face_energy(f, r) for f in faces_of[v])
+        assert abs(Ev - ref) <= 1e-12 * max(abs(ref), 1e-300) + 1e-14 * abs(E_tot), v
+        r2 = r.copy()
+        r2[3 * v:3 * v + 3] += rng.normal(0.0, 0.3, 3)
+        dE = g.energy(r2) - E_tot
+        dEv = g.energy_vertex(r2, int(v)) - Ev
+        assert abs(dE - dEv) <= 1e-11 * abs(E_tot), (v, dE, dEv)
+    with pytest.raises(S.UmcpError):
+        g.energy_vertex(r, o.nv)

@@ -234,17 +234,23 @@ int prof_pair(umcp_surface *s, int cls, hipEvent_t *ev)
     return UMCP_OK;
 }
 
+// the per-class face tables on the device, refreshed when parameters changed
+int sync_face_table(umcp_surface *s, hipStream_t st)
+{
+    if (!s->fa_dirty) return UMCP_OK;
+    s->h_fa.resize(s->P.classes.size());
+    for (size_t c = 0; c < s->P.classes.size(); c++) s->h_fa[c] = face_args(s, c, s->d_r);
+    hipError_t e = hipMemcpyAsync(s->d_fa_tab, s->h_fa.data(), s->h_fa.size() * sizeof(FaceArgs),
+                                  hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(s, e, "upload face table");
+    s->fa_dirty = false;
+    return UMCP_OK;
+}
+
 int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs ga, int *step_ctr = nullptr)
 {
     const bool rson = restraints_on(s);
-    if (s->fa_dirty) {
-        s->h_fa.resize(s->P.classes.size());
-        for (size_t c = 0; c < s->P.classes.size(); c++) s->h_fa[c] = face_args(s, c, s->d_r);
-        hipError_t e = hipMemcpyAsync(s->d_fa_tab, s->h_fa.data(), s->h_fa.size() * sizeof(FaceArgs),
-                                      hipMemcpyHostToDevice, st);
-        if (e != hipSuccess) return hip_fail(s, e, "upload face table");
-        s->fa_dirty = false;
-    }
+    if (int rc = sync_face_table(s, st)) return rc;
     BatchFaceArgs fb{};
     fb.fa = s->d_fa_tab;
     fb.items = s->d_items;
@@ -641,6 +647,93 @@ int umcp_energy_grad(umcp_surface *s, const double *r, double *E, double *g)
 {
     if (!g || !E) return fail(s, UMCP_ERR_ARG, "null output");
     return eval_host(s, r, E, g, false);
+}
+
+namespace {
+
+// faces in the reference's order as (class, index), the classes' kernel
+// kinds, and every vertex's faces the way computeModifiedVertices lists them
+// (src/interpLib.C:4040-4088: faces ascending, one entry per control-point
+// occurrence); built once per surface
+int face_energy_setup(umcp_surface *s)
+{
+    if (s->fe_ready) return UMCP_OK;
+    const Plan &P = s->P;
+    std::vector<int2> all((size_t)P.nfaces);
+    for (size_t c = 0; c < P.classes.size(); c++)
+        for (int i = 0; i < P.classes[c].nf(); i++) all[(size_t)P.classes[c].face_id[i]] = make_int2((int)c, i);
+    std::vector<int> kinds;
+    for (const FaceClass &C : P.classes) kinds.push_back(kind_of(C.ncp, C.npt));
+    const int nv = s->T.nv;
+    std::vector<int> cnt((size_t)nv + 1, 0);
+    for (int f = 0; f < P.nfaces; f++) {
+        const FaceClass &C = P.classes[(size_t)all[(size_t)f].x];
+        for (int x = 0; x < C.ncp; x++) cnt[(size_t)C.cp[(size_t)x * C.nf() + all[(size_t)f].y] + 1]++;
+    }
+    for (int v = 0; v < nv; v++) cnt[(size_t)v + 1] += cnt[(size_t)v];
+    s->fe_vstart = cnt;
+    s->fe_vfaces.assign((size_t)cnt[(size_t)nv], 0);
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int f = 0; f < P.nfaces; f++) {
+        const FaceClass &C = P.classes[(size_t)all[(size_t)f].x];
+        for (int x = 0; x < C.ncp; x++) {
+            const int v = C.cp[(size_t)x * C.nf() + all[(size_t)f].y];
+            s->fe_vfaces[(size_t)fill[(size_t)v]++] = f;
+        }
+    }
+    s->fe_items = all;
+    int rc;
+    if ((rc = dev_alloc(s, &s->d_fe_kinds, kinds.size())) || (rc = dev_alloc(s, &s->d_fe_items, all.size())) ||
+        (rc = dev_alloc(s, &s->d_fe_sub, (size_t)kFeSub)) || (rc = dev_alloc(s, &s->d_fe_out, all.size())) ||
+        (rc = dev_upload(s, s->d_fe_kinds, kinds.data(), kinds.size())) ||
+        (rc = dev_upload(s, s->d_fe_items, all.data(), all.size())))
+        return rc;
+    s->fe_ready = true;
+    return UMCP_OK;
+}
+
+}  // namespace
+
+int umcp_face_energies(umcp_surface *s, const double *r, double *E)
+{
+    int rc = check_ready(s);
+    if (rc) return rc;
+    if (!r || !E) return fail(s, UMCP_ERR_ARG, "null argument");
+    if ((rc = face_energy_setup(s)) || (rc = upload_r(s, r)) || (rc = sync_face_table(s, s->stream))) return rc;
+    const int n = s->P.nfaces;
+    hipError_t e = launch_face_energies(s->d_fa_tab, s->d_fe_kinds, (int)s->P.classes.size(), s->d_r,
+                                        s->d_fe_items, n, s->d_fe_out, s->stream);
+    if (e != hipSuccess) return hip_fail(s, e, "face energy launch");
+    e = hipMemcpyAsync(E, s->d_fe_out, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, s->stream);
+    if (e != hipSuccess) return hip_fail(s, e, "download face energies");
+    return sync(s);
+}
+
+int umcp_energy_vertex(umcp_surface *s, const double *r, int vertex, double *E)
+{
+    int rc = check_ready(s);
+    if (rc) return rc;
+    if (!r || !E) return fail(s, UMCP_ERR_ARG, "null argument");
+    if (vertex < 0 || vertex >= s->T.nv) return fail(s, UMCP_ERR_ARG, "vertex index out of range");
+    if ((rc = face_energy_setup(s)) || (rc = upload_r(s, r)) || (rc = sync_face_table(s, s->stream))) return rc;
+    const int lo = s->fe_vstart[(size_t)vertex], n = s->fe_vstart[(size_t)vertex + 1] - lo;
+    if (n > kFeSub) return fail(s, UMCP_ERR_ARG, "vertex is a control point of too many faces");
+    std::vector<int2> items((size_t)n);
+    for (int i = 0; i < n; i++) items[(size_t)i] = s->fe_items[(size_t)s->fe_vfaces[(size_t)(lo + i)]];
+    if ((rc = dev_upload(s, s->d_fe_sub, items.data(), items.size()))) return rc;
+    hipError_t e = launch_face_energies(s->d_fa_tab, s->d_fe_kinds, (int)s->P.classes.size(), s->d_r, s->d_fe_sub,
+                                        n, s->d_fe_out, s->stream);
+    if (e != hipSuccess) return hip_fail(s, e, "face energy launch");
+    std::vector<double> h((size_t)n);
+    e = hipMemcpyAsync(h.data(), s->d_fe_out, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, s->stream);
+    if (e != hipSuccess) return hip_fail(s, e, "download face energies");
+    if ((rc = sync(s))) return rc;
+    // surface::energy with do_vertex >= 0 (src/interpLib.C:4262-4295): the
+    // faces' energies summed in that list's order, no global terms
+    double sum = 0.0;
+    for (double x : h) sum += x;
+    *E = sum;
+    return UMCP_OK;
 }
 
 int umcp_area(umcp_surface *s, const double *r, double *A, double *A0)

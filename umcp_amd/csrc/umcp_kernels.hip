@@ -1470,6 +1470,72 @@ __global__ __launch_bounds__(kGatherBlock) void restraint_finalize_kernel(const 
     }
 }
 
+// One face's energy (FP64, every energy form; no global restraint terms, as
+// surface::faceEnergy), one thread per listed face.
+template <int NCP, int NPT, int KIND>
+__device__ double face_energy_one(const FaceArgs &a, const double *__restrict__ r, int fc)
+{
+    const int nf = a.nf;
+    const double al[3] = {r[3 * a.nv], r[3 * a.nv + 1], r[3 * a.nv + 2]};
+    double P[NCP][3];
+#pragma unroll
+    for (int c = 0; c < NCP; c++) {
+        const int v = a.cp[c * nf + fc];
+        double o[3] = {0.0, 0.0, 0.0};
+        if (a.has_img[fc]) image_offset(a.img[c * nf + fc], a.pbc, o);
+#pragma unroll
+        for (int k = 0; k < 3; k++) P[c][k] = r[3 * v + k] + o[k];
+    }
+    const double *tab = kind_table<KIND, double>();
+    const double *wts = tab + NPT * 5 * NCP;
+    double e = 0.0;
+    for (int p = 0; p < NPT; p++) {
+        const double *T = tab + p * 5 * NCP;
+        double X[5][3];
+#pragma unroll
+        for (int d = 0; d < 5; d++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                double s = 0.0;
+#pragma unroll
+                for (int c = 0; c < NCP; c++) s += T[c * 5 + d] * P[c][k];
+                X[d][k] = s * al[k];
+            }
+        PointGrad<double> pg;
+        e += point_energy_grad<double, kFeatGeneral>(X, wts[p], a.g0[p * nf + fc], a.ig0[p * nf + fc],
+                                                     a.rurv0[p * nf + fc], a.Lo[fc], a.Li[fc], a.c0[fc], a.dudv, 0.0,
+                                                     a.ph, pg);
+    }
+    return e;
+}
+
+__global__ __launch_bounds__(kMetricBlock) void face_energy_kernel(const FaceArgs *__restrict__ fa,
+                                                                   const int *__restrict__ kinds,
+                                                                   const double *__restrict__ r,
+                                                                   const int2 *__restrict__ items, int n,
+                                                                   double *__restrict__ E)
+{
+    const int i = blockIdx.x * kMetricBlock + threadIdx.x;
+    if (i >= n) return;
+    const int2 it = items[i];
+    const FaceArgs &a = fa[it.x];
+    switch (kinds[it.x]) {
+    case 0: E[i] = face_energy_one<12, 15, 0>(a, r, it.y); break;
+    case 1: E[i] = face_energy_one<11, 16, 1>(a, r, it.y); break;
+    default: E[i] = face_energy_one<13, 16, 2>(a, r, it.y); break;
+    }
+}
+
+hipError_t launch_face_energies(const FaceArgs *fa, const int *kinds, int nclass, const double *r, const int2 *items,
+                                int n, double *E, hipStream_t st)
+{
+    (void)nclass;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(face_energy_kernel, dim3((n + kMetricBlock - 1) / kMetricBlock), dim3(kMetricBlock), 0, st,
+                       fa, kinds, r, items, n, E);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------

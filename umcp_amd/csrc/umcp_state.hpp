@@ -86,6 +86,15 @@ struct umcp_surface {
     double *d_pp_next = nullptr, *d_mp = nullptr;
     long effm_nnz = 0;
     std::vector<void *> effm_allocs;
+    // per-face energies (umcp_face_energies, umcp_energy_vertex): faces in
+    // reference order as (class, index), each vertex's faces as
+    // computeModifiedVertices lists them, and the device lists and outputs
+    bool fe_ready = false;
+    std::vector<int2> fe_items;
+    std::vector<int> fe_vstart, fe_vfaces;
+    int *d_fe_kinds = nullptr;
+    int2 *d_fe_items = nullptr, *d_fe_sub = nullptr;
+    double *d_fe_out = nullptr;
     // points on the limit surface (umcp_point_eval / umcp_point_grad): face ->
     // (class, index in the class), the device tables (class table, sub-patch
     // maps), the gradient's (face, slot) buffer and grow-only staging and
@@ -103,6 +112,7 @@ struct umcp_surface {
 namespace umcp {
 
 constexpr size_t kArenaAlign = 256;
+constexpr int kFeSub = 1024;   // faces one vertex's energy may span
 
 // helpers of the surface object used by both translation units
 bool restraints_on(const umcp_surface *s);

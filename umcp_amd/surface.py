@@ -44,6 +44,7 @@ EXPORTS = [
     "umcp_effm_build", "umcp_effm_get", "umcp_effm_steps", "umcp_effm_steps_dev",
     "umcp_save_mesh", "umcp_write_restart", "umcp_read_restart", "umcp_write_limit_surface",
     "umcp_point_eval", "umcp_point_grad", "umcp_point_geometry", "umcp_write_limit_psf",
+    "umcp_face_energies", "umcp_energy_vertex",
 ]
 
 
@@ -141,6 +142,8 @@ def load_library(path: str = LIB_PATH):
         "umcp_point_eval": ([vp, dp, C.c_int, ip, dp, dp, dp], C.c_int),
         "umcp_write_limit_psf": ([vp, C.c_char_p], C.c_int),
         "umcp_point_geometry": ([vp, dp, C.c_int, ip, dp, dp], C.c_int),
+        "umcp_face_energies": ([vp, dp, dp], C.c_int),
+        "umcp_energy_vertex": ([vp, dp, C.c_int, dp], C.c_int),
         "umcp_point_grad": ([vp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int, C.c_double, dp, dp], C.c_int),
         "umcp_write_restart": ([C.c_char_p, C.c_int, dp, dp, C.c_int], C.c_int),
         "umcp_read_restart": ([C.c_char_p, C.c_int, dp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
@@ -367,6 +370,20 @@ class Surface:
                                             _dp(fm) if fm is not None else None, 1 if on_surface else 0,
                                             float(kT), _dp(g), _dp(gp)), "point_grad")
         return g, gp
+
+    def face_energies(self, r: np.ndarray) -> np.ndarray:
+        """faceEnergy of every face, in the reference's face order."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        E = np.zeros(self.nfaces)
+        self._check(self._L.umcp_face_energies(self._h, _dp(r), _dp(E)), "face_energies")
+        return E
+
+    def energy_vertex(self, r: np.ndarray, vertex: int) -> float:
+        """energy(r, NULL, do_vertex): the faces the vertex moves, no global terms."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        E = C.c_double()
+        self._check(self._L.umcp_energy_vertex(self._h, _dp(r), int(vertex), C.byref(E)), "energy_vertex")
+        return E.value
 
     def point_geometry(self, r: np.ndarray, faces, uv) -> np.ndarray:
         """[n, 15]: Ru, Rv, g, c1 + c2, c1 c2, c1, c2, vec1, vec2 (ru, rv, g, c)."""
