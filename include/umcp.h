@@ -123,13 +123,18 @@ int umcp_energy_grad(umcp_surface *s, const double *r, double *E, double *g);
 int umcp_area(umcp_surface *s, const double *r, double *A, double *A0);
 /* Energies of single faces and of the faces a vertex moves: umcp_face_energies
  * writes surface::faceEnergy (src/interpLib.C:3405-3424) of every face, in the
- * reference's face order, into E[nfaces]; umcp_energy_vertex replaces
+ * reference's face order, into E[nfaces], and (when not NULL) each face's
+ * terms of surface::area into A (sum dudv g w) and A0 (sum dudv g0 w);
+ * umcp_energy_vertex replaces
  * surface::energy(r, NULL, do_vertex) with do_vertex >= 0
  * (src/interpLib.C:4262-4295): the energy of every face that has the vertex
  * among its control points, summed in computeModifiedVertices' order (faces
  * ascending, one term per occurrence), without the global restraint terms. */
-int umcp_face_energies(umcp_surface *s, const double *r, double *E);
+int umcp_face_energies(umcp_surface *s, const double *r, double *E, double *A, double *A0);
 int umcp_energy_vertex(umcp_surface *s, const double *r, int vertex, double *E);
+/* surface::area(r, do_vertex, &A, &A0) with do_vertex >= 0
+ * (src/interpLib.C:4420-4560): area and reference area of the same faces */
+int umcp_area_vertex(umcp_surface *s, const double *r, int vertex, double *A, double *A0);
 
 /* nsteps Brownian steps from host arrays; xi is [nsteps][3nv] standard
  * normals; r (3nv+3) is updated in place; energies (nsteps, may be NULL)

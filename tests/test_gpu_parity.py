@@ -199,12 +199,14 @@ def test_device_entry_points(ves642):
 
 @pytest.mark.parametrize("case", ["vesicle_lipid", "defect_helfrich"])
 def test_face_and_vertex_energies(case):
-    """surface::faceEnergy of every face and energy(r, NULL, do_vertex): each
-    face against the oracle's to 1e-12 of the largest; the faces sum to the
-    total energy; a vertex's energy is the sum over the faces that have it as
-    a control point (computeModifiedVertices' list, duplicates included); and
-    moving that vertex alone changes the total energy by exactly the change
-    of its vertex energy (a face it moves but the list missed would break it)."""
+    """surface::faceEnergy of every face, energy(r, NULL, do_vertex) and
+    area(r, do_vertex): each face's energy against the oracle's to 1e-12 of
+    the largest; the faces' energies sum to the total energy and their area
+    terms to surface::area's; a vertex's energy is the sum over the faces that
+    have it as a control point (computeModifiedVertices' list, duplicates
+    included), and its area the same faces' area terms; moving that vertex
+    alone changes the total energy by exactly the change of its vertex energy
+    (a face it moves but the list missed would break it)."""
     if case == "vesicle_lipid":
         w = workloads.vesicle_642()
         g, o = make_pair(w)
@@ -221,6 +223,10 @@ def test_face_and_vertex_energies(case):
     assert np.abs(Ef - Eo).max() <= 1e-12 * np.abs(Eo).max()
     E_tot = g.energy(r)
     assert abs(Ef.sum() - E_tot) <= 1e-12 * abs(E_tot)
+    # the faces' area terms sum to surface::area's totals (oracle-checked)
+    _, Af, A0f = g.face_energies(r, areas=True)
+    At, A0t = o.area(r)
+    assert abs(Af.sum() - At) <= 1e-12 * At and abs(A0f.sum() - A0t) <= 1e-12 * A0t
     faces_of = [[] for _ in range(o.nv)]
     for f in range(o.nfaces):
         for v in o.face_info(f)["cp"]:
@@ -235,5 +241,8 @@ def test_face_and_vertex_energies(case):
         dE = g.energy(r2) - E_tot
         dEv = g.energy_vertex(r2, int(v)) - Ev
         assert abs(dE - dEv) <= 1e-11 * abs(E_tot), (v, dE, dEv)
+        # area(r, do_vertex): the same faces' area and reference area
+        A, A0 = g.area_vertex(r, int(v))
+        assert abs(A - Af[faces_of[v]].sum()) <= 1e-13 * A and abs(A0 - A0f[faces_of[v]].sum()) <= 1e-13 * A0
     with pytest.raises(S.UmcpError):
         g.energy_vertex(r, o.nv)

@@ -684,7 +684,7 @@ int face_energy_setup(umcp_surface *s)
     s->fe_items = all;
     int rc;
     if ((rc = dev_alloc(s, &s->d_fe_kinds, kinds.size())) || (rc = dev_alloc(s, &s->d_fe_items, all.size())) ||
-        (rc = dev_alloc(s, &s->d_fe_sub, (size_t)kFeSub)) || (rc = dev_alloc(s, &s->d_fe_out, all.size())) ||
+        (rc = dev_alloc(s, &s->d_fe_sub, (size_t)kFeSub)) || (rc = dev_alloc(s, &s->d_fe_out, 3 * all.size())) ||
         (rc = dev_upload(s, s->d_fe_kinds, kinds.data(), kinds.size())) ||
         (rc = dev_upload(s, s->d_fe_items, all.data(), all.size())))
         return rc;
@@ -694,7 +694,7 @@ int face_energy_setup(umcp_surface *s)
 
 }  // namespace
 
-int umcp_face_energies(umcp_surface *s, const double *r, double *E)
+int umcp_face_energies(umcp_surface *s, const double *r, double *E, double *A, double *A0)
 {
     int rc = check_ready(s);
     if (rc) return rc;
@@ -705,15 +705,24 @@ int umcp_face_energies(umcp_surface *s, const double *r, double *E)
                                         s->d_fe_items, n, s->d_fe_out, s->stream);
     if (e != hipSuccess) return hip_fail(s, e, "face energy launch");
     e = hipMemcpyAsync(E, s->d_fe_out, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess && A)
+        e = hipMemcpyAsync(A, s->d_fe_out + n, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess && A0)
+        e = hipMemcpyAsync(A0, s->d_fe_out + 2 * (size_t)n, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost,
+                           s->stream);
     if (e != hipSuccess) return hip_fail(s, e, "download face energies");
     return sync(s);
 }
 
-int umcp_energy_vertex(umcp_surface *s, const double *r, int vertex, double *E)
+namespace {
+
+// energy, area and reference area of the faces a vertex moves, each summed in
+// computeModifiedVertices' order
+int vertex_sums(umcp_surface *s, const double *r, int vertex, double out[3])
 {
     int rc = check_ready(s);
     if (rc) return rc;
-    if (!r || !E) return fail(s, UMCP_ERR_ARG, "null argument");
+    if (!r) return fail(s, UMCP_ERR_ARG, "null argument");
     if (vertex < 0 || vertex >= s->T.nv) return fail(s, UMCP_ERR_ARG, "vertex index out of range");
     if ((rc = face_energy_setup(s)) || (rc = upload_r(s, r)) || (rc = sync_face_table(s, s->stream))) return rc;
     const int lo = s->fe_vstart[(size_t)vertex], n = s->fe_vstart[(size_t)vertex + 1] - lo;
@@ -724,16 +733,42 @@ int umcp_energy_vertex(umcp_surface *s, const double *r, int vertex, double *E)
     hipError_t e = launch_face_energies(s->d_fa_tab, s->d_fe_kinds, (int)s->P.classes.size(), s->d_r, s->d_fe_sub,
                                         n, s->d_fe_out, s->stream);
     if (e != hipSuccess) return hip_fail(s, e, "face energy launch");
-    std::vector<double> h((size_t)n);
-    e = hipMemcpyAsync(h.data(), s->d_fe_out, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, s->stream);
+    std::vector<double> h(3 * (size_t)n);
+    e = hipMemcpyAsync(h.data(), s->d_fe_out, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost, s->stream);
     if (e != hipSuccess) return hip_fail(s, e, "download face energies");
     if ((rc = sync(s))) return rc;
-    // surface::energy with do_vertex >= 0 (src/interpLib.C:4262-4295): the
-    // faces' energies summed in that list's order, no global terms
-    double sum = 0.0;
-    for (double x : h) sum += x;
-    *E = sum;
+    for (int k = 0; k < 3; k++) {
+        double sum = 0.0;
+        for (int i = 0; i < n; i++) sum += h[(size_t)k * n + i];
+        out[k] = sum;
+    }
     return UMCP_OK;
+}
+
+}  // namespace
+
+int umcp_energy_vertex(umcp_surface *s, const double *r, int vertex, double *E)
+{
+    // surface::energy with do_vertex >= 0 (src/interpLib.C:4262-4295): no
+    // global terms
+    if (!E) return fail(s, UMCP_ERR_ARG, "null argument");
+    double out[3];
+    int rc = vertex_sums(s, r, vertex, out);
+    if (rc == UMCP_OK) *E = out[0];
+    return rc;
+}
+
+int umcp_area_vertex(umcp_surface *s, const double *r, int vertex, double *A, double *A0)
+{
+    // surface::area with do_vertex >= 0 (src/interpLib.C:4420-4560)
+    if (!A || !A0) return fail(s, UMCP_ERR_ARG, "null argument");
+    double out[3];
+    int rc = vertex_sums(s, r, vertex, out);
+    if (rc == UMCP_OK) {
+        *A = out[1];
+        *A0 = out[2];
+    }
+    return rc;
 }
 
 int umcp_area(umcp_surface *s, const double *r, double *A, double *A0)

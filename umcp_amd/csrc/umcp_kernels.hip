@@ -1473,7 +1473,8 @@ __global__ __launch_bounds__(kGatherBlock) void restraint_finalize_kernel(const 
 // One face's energy (FP64, every energy form; no global restraint terms, as
 // surface::faceEnergy), one thread per listed face.
 template <int NCP, int NPT, int KIND>
-__device__ double face_energy_one(const FaceArgs &a, const double *__restrict__ r, int fc)
+__device__ double face_energy_one(const FaceArgs &a, const double *__restrict__ r, int fc, double &area,
+                                  double &area0)
 {
     const int nf = a.nf;
     const double al[3] = {r[3 * a.nv], r[3 * a.nv + 1], r[3 * a.nv + 2]};
@@ -1489,6 +1490,7 @@ __device__ double face_energy_one(const FaceArgs &a, const double *__restrict__ 
     const double *tab = kind_table<KIND, double>();
     const double *wts = tab + NPT * 5 * NCP;
     double e = 0.0;
+    area = area0 = 0.0;
     for (int p = 0; p < NPT; p++) {
         const double *T = tab + p * 5 * NCP;
         double X[5][3];
@@ -1505,6 +1507,12 @@ __device__ double face_energy_one(const FaceArgs &a, const double *__restrict__ 
         e += point_energy_grad<double, kFeatGeneral>(X, wts[p], a.g0[p * nf + fc], a.ig0[p * nf + fc],
                                                      a.rurv0[p * nf + fc], a.Lo[fc], a.Li[fc], a.c0[fc], a.dudv, 0.0,
                                                      a.ph, pg);
+        // surface::area's terms (src/interpLib.C:4480-4560)
+        const double E11 = X[0][0] * X[0][0] + X[0][1] * X[0][1] + X[0][2] * X[0][2];
+        const double E12 = X[0][0] * X[1][0] + X[0][1] * X[1][1] + X[0][2] * X[1][2];
+        const double E22 = X[1][0] * X[1][0] + X[1][1] * X[1][1] + X[1][2] * X[1][2];
+        area += a.dudv * sqrt(E11 * E22 - E12 * E12) * wts[p];
+        area0 += a.dudv * a.g0[p * nf + fc] * wts[p];
     }
     return e;
 }
@@ -1519,11 +1527,15 @@ __global__ __launch_bounds__(kMetricBlock) void face_energy_kernel(const FaceArg
     if (i >= n) return;
     const int2 it = items[i];
     const FaceArgs &a = fa[it.x];
+    double e, ar, ar0;
     switch (kinds[it.x]) {
-    case 0: E[i] = face_energy_one<12, 15, 0>(a, r, it.y); break;
-    case 1: E[i] = face_energy_one<11, 16, 1>(a, r, it.y); break;
-    default: E[i] = face_energy_one<13, 16, 2>(a, r, it.y); break;
+    case 0: e = face_energy_one<12, 15, 0>(a, r, it.y, ar, ar0); break;
+    case 1: e = face_energy_one<11, 16, 1>(a, r, it.y, ar, ar0); break;
+    default: e = face_energy_one<13, 16, 2>(a, r, it.y, ar, ar0); break;
     }
+    E[i] = e;
+    E[n + i] = ar;
+    E[2 * n + i] = ar0;
 }
 
 hipError_t launch_face_energies(const FaceArgs *fa, const int *kinds, int nclass, const double *r, const int2 *items,

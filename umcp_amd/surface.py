@@ -44,7 +44,7 @@ EXPORTS = [
     "umcp_effm_build", "umcp_effm_get", "umcp_effm_steps", "umcp_effm_steps_dev",
     "umcp_save_mesh", "umcp_write_restart", "umcp_read_restart", "umcp_write_limit_surface",
     "umcp_point_eval", "umcp_point_grad", "umcp_point_geometry", "umcp_write_limit_psf",
-    "umcp_face_energies", "umcp_energy_vertex",
+    "umcp_face_energies", "umcp_energy_vertex", "umcp_area_vertex",
 ]
 
 
@@ -142,8 +142,9 @@ def load_library(path: str = LIB_PATH):
         "umcp_point_eval": ([vp, dp, C.c_int, ip, dp, dp, dp], C.c_int),
         "umcp_write_limit_psf": ([vp, C.c_char_p], C.c_int),
         "umcp_point_geometry": ([vp, dp, C.c_int, ip, dp, dp], C.c_int),
-        "umcp_face_energies": ([vp, dp, dp], C.c_int),
+        "umcp_face_energies": ([vp, dp, dp, dp, dp], C.c_int),
         "umcp_energy_vertex": ([vp, dp, C.c_int, dp], C.c_int),
+        "umcp_area_vertex": ([vp, dp, C.c_int, dp, dp], C.c_int),
         "umcp_point_grad": ([vp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int, C.c_double, dp, dp], C.c_int),
         "umcp_write_restart": ([C.c_char_p, C.c_int, dp, dp, C.c_int], C.c_int),
         "umcp_read_restart": ([C.c_char_p, C.c_int, dp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
@@ -371,12 +372,16 @@ class Surface:
                                             float(kT), _dp(g), _dp(gp)), "point_grad")
         return g, gp
 
-    def face_energies(self, r: np.ndarray) -> np.ndarray:
-        """faceEnergy of every face, in the reference's face order."""
+    def face_energies(self, r: np.ndarray, areas: bool = False):
+        """faceEnergy of every face, in the reference's face order; with
+        areas=True also each face's area and reference area: (E, A, A0)."""
         r = np.ascontiguousarray(r, dtype=np.float64)
         E = np.zeros(self.nfaces)
-        self._check(self._L.umcp_face_energies(self._h, _dp(r), _dp(E)), "face_energies")
-        return E
+        A = np.zeros(self.nfaces) if areas else None
+        A0 = np.zeros(self.nfaces) if areas else None
+        self._check(self._L.umcp_face_energies(self._h, _dp(r), _dp(E), _dp(A) if areas else None,
+                                               _dp(A0) if areas else None), "face_energies")
+        return (E, A, A0) if areas else E
 
     def energy_vertex(self, r: np.ndarray, vertex: int) -> float:
         """energy(r, NULL, do_vertex): the faces the vertex moves, no global terms."""
@@ -384,6 +389,13 @@ class Surface:
         E = C.c_double()
         self._check(self._L.umcp_energy_vertex(self._h, _dp(r), int(vertex), C.byref(E)), "energy_vertex")
         return E.value
+
+    def area_vertex(self, r: np.ndarray, vertex: int):
+        """area(r, do_vertex): (A, A0) of the faces the vertex moves."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        A, A0 = C.c_double(), C.c_double()
+        self._check(self._L.umcp_area_vertex(self._h, _dp(r), int(vertex), C.byref(A), C.byref(A0)), "area_vertex")
+        return A.value, A0.value
 
     def point_geometry(self, r: np.ndarray, faces, uv) -> np.ndarray:
         """[n, 15]: Ru, Rv, g, c1 + c2, c1 c2, c1, c2, vec1, vec2 (ru, rv, g, c)."""
