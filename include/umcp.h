@@ -267,6 +267,18 @@ int umcp_point_grad(umcp_surface *s, const double *r, int n, const int *faces, c
  * sub-patch there). */
 #define UMCP_POINT_GEOM 15
 int umcp_point_geometry(umcp_surface *s, const double *r, int n, const int *faces, const double *uv, double *geom);
+/* One step of surface::nextFace (src/uv_map.C:1695-1961) for each point:
+ * moves (u, v) by (du, dv) within the face, or to the border it reaches
+ * first and onto the face across it (faces[i], uv updated), leaving the rest
+ * of the displacement transformed so that du r_u + dv r_v is kept against the
+ * old face's tangents (duv updated; 0 once inside).  mom (2 per point, may be
+ * NULL) is transformed with the same matrix and, when given, coord_transform
+ * (4 per point, row-major) is multiplied by it, as the reference does.  Call
+ * again while any displacement remains.  The reference's random nudge for a
+ * point moving along the edge it sits on (rand(), src/uv_map.C:1711-1718) is
+ * not made; its deterministic 1e-12 nudges are. */
+int umcp_next_face(umcp_surface *s, const double *r, int n, int *faces, double *uv, double *duv, double *mom,
+                   double *coord_transform);
 
 /* Brownian steps drawing their noise on the device: xi of step k is the
  * counter-based stream of (seed, k) that umcp_noise writes (Philox4x64-10

@@ -27,7 +27,7 @@ torch = pytest.importorskip("torch")
 
 from oracle import oracle as O  # noqa: E402
 from umcp_amd import surface as S  # noqa: E402
-from umcp_amd import workloads  # noqa: E402
+from umcp_amd import meshgen, workloads  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -322,3 +322,72 @@ def test_geometry_rejects_the_vertex(ves):
     w, g, o = ves
     with pytest.raises(S.UmcpError):
         g.point_geometry(w.r, np.array([o.nreg], np.int32), np.zeros((1, 2)))
+
+
+@pytest.mark.parametrize("mesh", ["vesicle", "defect_sheet"])
+def test_next_face_walk(ves, mesh):
+    """surface::nextFace (src/uv_map.C:1695-1961), one step per call: a move
+    that stays inside the face just adds; a move that leaves it stops at the
+    border it reaches first, where the new face evaluates to the same surface
+    point (the reference's MOVE_CHECK condition, here to 1e-10 A), and the
+    rest (du', dv') satisfies the reference's two equations
+    (du r_u + dv r_v) . r_u = (du' r_u' + dv' r_v') . r_u (and with r_v) with
+    the oracle's tangents; the momentum goes through the same matrix; a walk
+    repeated until no displacement remains ends on a face."""
+    if mesh == "vesicle":
+        w, g, o = ves
+    else:
+        w = workloads.defect_sheet()
+        g, o = _pair(w)
+    r = w.r.copy()
+    rng = np.random.default_rng(51)
+    n = 600
+    faces, uv = random_points(o, n, 52)
+    duv = rng.normal(0.0, 0.4, (n, 2))
+    duv[:50] *= 1e-3                                    # these stay inside
+    mom = duv.copy()
+    f1, uv1, d1, m1, _ = g.next_face(r, faces, uv, duv, mom=mom)
+    inside = (uv + duv).min(axis=1) > 0
+    inside &= (uv + duv).sum(axis=1) < 1
+    assert np.array_equal(f1[inside], faces[inside])
+    assert np.allclose(uv1[inside], (uv + duv)[inside], rtol=0, atol=1e-15) and not d1[inside].any()
+    moved = ~inside
+    say("[points] next_face %s: %d crossings" % (mesh, int(moved.sum())))
+    # the crossing on the old face: the smallest positive t to a border
+    u, v = uv[moved, 0], uv[moved, 1]
+    du, dv = duv[moved, 0], duv[moved, 1]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t = np.stack([np.where(dv < 0, -v / dv, np.inf), np.where(du + dv > 0, (1 - u - v) / (du + dv), np.inf),
+                      np.where(du < 0, -u / du, np.inf)], 1).min(axis=1)
+    uc, vc = u + t * du, v + t * dv
+    border = np.minimum(np.minimum(uc, vc), 1 - uc - vc)
+    assert np.abs(border).max() < 1e-12
+    Ro, _ = o.point_eval(r, faces[moved], np.stack([uc, vc], 1))
+    Rn, _ = o.point_eval(r, f1[moved], uv1[moved])
+    dR = Ro - Rn
+    if mesh == "defect_sheet":   # the same point of a periodic surface, at its minimum image
+        L = np.array(meshgen.read_mesh(w.mesh_path).pbc).diagonal()
+        dR -= L * np.round(dR / L)
+    say("   continuity max |dR|", np.abs(dR).max())
+    assert np.abs(dR).max() < 1e-10
+    assert (f1[moved] != faces[moved]).all()
+    Go = o.point_geometry(r, faces[moved], np.stack([uc, vc], 1))
+    Gn = o.point_geometry(r, f1[moved], uv1[moved])
+    rem = np.stack([du * (1 - t), dv * (1 - t)], 1)
+    lhs = rem[:, :1] * Go[:, 0:3] + rem[:, 1:] * Go[:, 3:6]
+    rhs = d1[moved, :1] * Gn[:, 0:3] + d1[moved, 1:] * Gn[:, 3:6]
+    for k in (0, 3):
+        a = (lhs * Go[:, k:k + 3]).sum(axis=1)
+        b = (rhs * Go[:, k:k + 3]).sum(axis=1)
+        assert np.abs(a - b).max() <= 1e-9 * np.abs(lhs).max() * np.abs(Go[:, k:k + 3]).max()
+    # mom was set to the displacement: it comes out as the new displacement
+    # times 1/(1 - t) (the map is linear and took the remaining part)
+    assert np.allclose(m1[moved] * (1 - t)[:, None], d1[moved], rtol=1e-9, atol=1e-12)
+    # walk to the end
+    f, q, d = f1, uv1, d1
+    for _ in range(50):
+        if not d.any():
+            break
+        f, q, d, _, _ = g.next_face(r, f, q, d)
+    assert not d.any()
+    assert (q.min(axis=1) >= -1e-12).all() and (q.sum(axis=1) <= 1 + 1e-12).all()

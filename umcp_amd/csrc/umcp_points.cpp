@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -437,6 +438,173 @@ int umcp_point_geometry(umcp_surface *s, const double *r, int n, const int *face
                                   hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) return phip(s, e, "download point geometry");
+    return UMCP_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// face corners (slot 0 at (u, v) = (0, 0), slot 2 at (1, 0), slot 1 at
+// (0, 1)) and the face across each edge, built once
+int walk_setup(umcp_surface *s)
+{
+    if (s->nf_ready) return UMCP_OK;
+    const Plan &P = s->P;
+    s->nf_corner.assign(3 * (size_t)P.nfaces, -1);
+    for (const FaceClass &C : P.classes)
+        for (int i = 0; i < C.nf(); i++) {
+            const int f = C.face_id[(size_t)i];
+            s->nf_corner[3 * (size_t)f] = C.cp[(size_t)0 * C.nf() + i];
+            s->nf_corner[3 * (size_t)f + 1] = C.cp[(size_t)2 * C.nf() + i];
+            s->nf_corner[3 * (size_t)f + 2] = C.cp[(size_t)1 * C.nf() + i];
+        }
+    std::map<std::pair<int, int>, std::vector<int>> edges;
+    for (int f = 0; f < P.nfaces; f++)
+        for (int k = 0; k < 3; k++) {
+            int a = s->nf_corner[3 * (size_t)f + k], b = s->nf_corner[3 * (size_t)f + (k + 1) % 3];
+            edges[{std::min(a, b), std::max(a, b)}].push_back(f);
+        }
+    s->nf_across.assign(3 * (size_t)P.nfaces, -1);
+    for (int f = 0; f < P.nfaces; f++) {
+        // edge 0: v = 0 (corners 0, 1); edge 1: u + v = 1 (1, 2); edge 2: u = 0 (0, 2)
+        const int pairs[3][2] = {{0, 1}, {1, 2}, {0, 2}};
+        for (int k = 0; k < 3; k++) {
+            int a = s->nf_corner[3 * (size_t)f + pairs[k][0]], b = s->nf_corner[3 * (size_t)f + pairs[k][1]];
+            const auto &lst = edges[{std::min(a, b), std::max(a, b)}];
+            for (int g : lst)
+                if (g != f) s->nf_across[3 * (size_t)f + k] = g;
+            if (lst.size() != 2 || s->nf_across[3 * (size_t)f + k] < 0)
+                return pfail(s, UMCP_ERR_BAD_MESH, "an edge is not shared by exactly two faces");
+        }
+    }
+    s->nf_ready = true;
+    return UMCP_OK;
+}
+
+// (u, v) of vertex a among the corners of face f
+bool corner_uv(const umcp_surface *s, int f, int a, double uv[2])
+{
+    const double at[3][2] = {{0.0, 0.0}, {1.0, 0.0}, {0.0, 1.0}};
+    for (int k = 0; k < 3; k++)
+        if (s->nf_corner[3 * (size_t)f + k] == a) {
+            uv[0] = at[k][0];
+            uv[1] = at[k][1];
+            return true;
+        }
+    return false;
+}
+
+}  // namespace
+
+extern "C" {
+
+int umcp_next_face(umcp_surface *s, const double *r, int n, int *faces, double *uv, double *duv, double *mom,
+                   double *coord_transform)
+{
+    int rc = point_check(s, r, n, faces, uv);
+    if (rc) return rc;
+    if (n == 0) return UMCP_OK;
+    if (!duv) return pfail(s, UMCP_ERR_ARG, "null displacement array");
+    if ((rc = walk_setup(s))) return rc;
+    const int nfaces = s->P.nfaces;
+    // per crossing point: the old face at the crossing and the new face there
+    std::vector<int> cross_idx, gf;
+    std::vector<double> guv, rem;
+    for (int i = 0; i < n; i++) {
+        const int f = faces[i];
+        if (f < 0 || f >= nfaces) return pfail(s, UMCP_ERR_ARG, "point " + std::to_string(i) + ": face index out of range");
+        double u = uv[2 * i], v = uv[2 * i + 1], du = duv[2 * i], dv = duv[2 * i + 1];
+        if (du == 0 && dv == 0) continue;
+        if (u + v > 1 + kThresh || u < -kThresh || v < -kThresh)
+            return pfail(s, UMCP_ERR_ARG, "point " + std::to_string(i) + ": not on its face");
+        const double nu = u + du, nv = v + dv;
+        if (nu > 0 && nv > 0 && nu + nv < 1) {
+            uv[2 * i] += du;
+            uv[2 * i + 1] += dv;
+            duv[2 * i] = duv[2 * i + 1] = 0.0;
+            continue;
+        }
+        // moving along an edge: the reference's nudges (src/uv_map.C:1743-1749)
+        if (std::fabs(du + dv) < 1e-10 && std::fabs(du) + std::fabs(dv) > 1e-10) du += 1e-12;
+        if (std::fabs(du) < 1e-10 && std::fabs(dv) > 1e-10) du += 1e-12;
+        if (std::fabs(dv) < 1e-10 && std::fabs(du) > 1e-10) du += 1e-12;
+        double t1 = 1e10, t2 = 1e10, t3 = 1e10;
+        if (dv < 0) t1 = -v / dv;
+        if (du + dv > 0) t2 = (1 - u - v) / (du + dv);
+        if (du < 0) t3 = -u / du;
+        int edge;
+        double t;
+        if (t1 < t2 && t1 < t3) { edge = 0; t = t1; }
+        else if (t2 < t3) { edge = 1; t = t2; }
+        else { edge = 2; t = t3; }
+        u += t * du;
+        v += t * dv;
+        du -= t * du;
+        dv -= t * dv;
+        // the shared edge a -> b and the crossing's parameter along it
+        const int c0 = s->nf_corner[3 * (size_t)f], c1 = s->nf_corner[3 * (size_t)f + 1],
+                  c2 = s->nf_corner[3 * (size_t)f + 2];
+        int a, b;
+        double sp;
+        if (edge == 0) { a = c0; b = c1; sp = u; }
+        else if (edge == 1) { a = c1; b = c2; sp = v; }
+        else { a = c0; b = c2; sp = v; }
+        const int g = s->nf_across[3 * (size_t)f + edge];
+        double ua[2], ub[2];
+        if (!corner_uv(s, g, a, ua) || !corner_uv(s, g, b, ub))
+            return pfail(s, UMCP_ERR_BAD_MESH, "inconsistent face corners");
+        const double nu2 = (1 - sp) * ua[0] + sp * ub[0], nv2 = (1 - sp) * ua[1] + sp * ub[1];
+        cross_idx.push_back(i);
+        gf.push_back(f);
+        gf.push_back(g);
+        guv.insert(guv.end(), {u, v, nu2, nv2});
+        rem.insert(rem.end(), {du, dv});
+    }
+    const int m = (int)cross_idx.size();
+    if (m == 0) return UMCP_OK;
+    // tangents at the crossings, on both sides (surface::ru / rv)
+    std::vector<double> geom((size_t)2 * m * kPointGeom);
+    Batch b;
+    rc = run_points(s, r, 2 * m, gf.data(), guv.data(), nullptr, nullptr, nullptr, 0, 0.0, 2, b);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(geom.data(), s->d_pt_out + b.off_R, sizeof(double) * geom.size(),
+                                  hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return phip(s, e, "download tangents");
+    for (int x = 0; x < m; x++) {
+        const int i = cross_idx[(size_t)x];
+        const double *go = &geom[(size_t)(2 * x) * kPointGeom], *gn = &geom[(size_t)(2 * x + 1) * kPointGeom];
+        const double *ruo = go, *rvo = go + 3, *run = gn, *rvn = gn + 3;
+        auto dot = [](const double *p, const double *q) { return p[0] * q[0] + p[1] * q[1] + p[2] * q[2]; };
+        // (du r_u + dv r_v = du' r_u' + dv' r_v') projected on r_u and r_v of
+        // the old face (src/uv_map.C:1876-1893)
+        const double ruo1 = dot(ruo, ruo), run1 = dot(run, ruo), ruo2 = dot(ruo, rvo), run2 = dot(run, rvo);
+        const double rvo1 = dot(rvo, ruo), rvn1 = dot(rvn, ruo), rvo2 = dot(rvo, rvo), rvn2 = dot(rvn, rvo);
+        const double du = rem[2 * (size_t)x], dv = rem[2 * (size_t)x + 1];
+        const double den = run2 * rvn1 - run1 * rvn2;
+        const double dup = -(-(du * ruo2 * rvn1) + du * ruo1 * rvn2 + dv * rvn2 * rvo1 - dv * rvn1 * rvo2) / den;
+        const double dvp = -(-(du * run2 * ruo1) + du * run1 * ruo2 - dv * run2 * rvo1 + dv * run1 * rvo2) / den;
+        if (mom) {
+            const double rot[4] = {(ruo2 * rvn1 - ruo1 * rvn2) / den, -(rvn2 * rvo1 - rvn1 * rvo2) / den,
+                                   (run2 * ruo1 - run1 * ruo2) / den, (run2 * rvo1 - run1 * rvo2) / den};
+            if (coord_transform) {
+                double *ct = coord_transform + 4 * (size_t)i;
+                const double tt[4] = {rot[0] * ct[0] + rot[1] * ct[2], rot[0] * ct[1] + rot[1] * ct[3],
+                                      rot[2] * ct[0] + rot[3] * ct[2], rot[2] * ct[1] + rot[3] * ct[3]};
+                std::memcpy(ct, tt, sizeof tt);
+            }
+            const double mu = mom[2 * i] * rot[0] + mom[2 * i + 1] * rot[1];
+            const double mv = mom[2 * i] * rot[2] + mom[2 * i + 1] * rot[3];
+            mom[2 * i] = mu;
+            mom[2 * i + 1] = mv;
+        }
+        faces[i] = gf[2 * (size_t)x + 1];
+        uv[2 * i] = guv[4 * (size_t)x + 2];
+        uv[2 * i + 1] = guv[4 * (size_t)x + 3];
+        duv[2 * i] = dup;
+        duv[2 * i + 1] = dvp;
+    }
     return UMCP_OK;
 }
 

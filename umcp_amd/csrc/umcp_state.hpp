@@ -107,6 +107,10 @@ struct umcp_surface {
     size_t pt_in_cap = 0, pt_out_cap = 0;
     char *pt_stage = nullptr, *pt_host = nullptr;   // pinned (hipHostMalloc)
     size_t pt_stage_cap = 0, pt_host_cap = 0;
+    // moving points across faces (umcp_next_face): each face's corner
+    // vertices and the face across each of its three edges
+    bool nf_ready = false;
+    std::vector<int> nf_corner, nf_across;
 };
 
 namespace umcp {

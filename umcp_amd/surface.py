@@ -44,7 +44,7 @@ EXPORTS = [
     "umcp_effm_build", "umcp_effm_get", "umcp_effm_steps", "umcp_effm_steps_dev",
     "umcp_save_mesh", "umcp_write_restart", "umcp_read_restart", "umcp_write_limit_surface",
     "umcp_point_eval", "umcp_point_grad", "umcp_point_geometry", "umcp_write_limit_psf",
-    "umcp_face_energies", "umcp_energy_vertex", "umcp_area_vertex",
+    "umcp_face_energies", "umcp_energy_vertex", "umcp_area_vertex", "umcp_next_face",
 ]
 
 
@@ -145,6 +145,7 @@ def load_library(path: str = LIB_PATH):
         "umcp_face_energies": ([vp, dp, dp, dp, dp], C.c_int),
         "umcp_energy_vertex": ([vp, dp, C.c_int, dp], C.c_int),
         "umcp_area_vertex": ([vp, dp, C.c_int, dp, dp], C.c_int),
+        "umcp_next_face": ([vp, dp, C.c_int, ip, dp, dp, dp, dp], C.c_int),
         "umcp_point_grad": ([vp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int, C.c_double, dp, dp], C.c_int),
         "umcp_write_restart": ([C.c_char_p, C.c_int, dp, dp, C.c_int], C.c_int),
         "umcp_read_restart": ([C.c_char_p, C.c_int, dp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
@@ -389,6 +390,20 @@ class Surface:
         E = C.c_double()
         self._check(self._L.umcp_energy_vertex(self._h, _dp(r), int(vertex), C.byref(E)), "energy_vertex")
         return E.value
+
+    def next_face(self, r: np.ndarray, faces, uv, duv, mom=None, coord_transform=None):
+        """One nextFace step per point: returns (faces, uv, duv, mom, coord_transform)
+        updated (copies of the inputs)."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        f = np.array(faces, dtype=np.int32)
+        u = np.array(uv, dtype=np.float64).reshape(-1, 2)
+        d = np.array(duv, dtype=np.float64).reshape(-1, 2)
+        m = None if mom is None else np.array(mom, dtype=np.float64).reshape(-1, 2)
+        ct = None if coord_transform is None else np.array(coord_transform, dtype=np.float64).reshape(-1, 4)
+        self._check(self._L.umcp_next_face(self._h, _dp(r), f.size, _ip(f), _dp(u), _dp(d),
+                                           _dp(m) if m is not None else None,
+                                           _dp(ct) if ct is not None else None), "next_face")
+        return f, u, d, m, ct
 
     def area_vertex(self, r: np.ndarray, vertex: int):
         """area(r, do_vertex): (A, A0) of the faces the vertex moves."""
