@@ -267,6 +267,15 @@ int umcp_point_grad(umcp_surface *s, const double *r, int n, const int *faces, c
  * sub-patch there). */
 #define UMCP_POINT_GEOM 15
 int umcp_point_geometry(umcp_surface *s, const double *r, int n, const int *faces, const double *uv, double *geom);
+/* Control-point weights at the points, replacing surface::get_pt_coeffs and
+ * get_pt_dcoeffs (src/uv_map.C:14-420): ids[18 i + x] the vertex of control
+ * point x (-1 past the face's last), w[54 i + 18 d + x] its weight in the
+ * position (d = 0) and in the derivatives along the face's u (1) and v (2),
+ * so that R = alpha * sum w_0x (r_x + image) and likewise Ru, Rv.  At an
+ * irregular face's extraordinary vertex the reference's mask: 1/2 on the
+ * vertex, 1/(2 val) on its ring, derivative weights zero. */
+#define UMCP_POINT_SLOTS 18
+int umcp_point_coeffs(umcp_surface *s, int n, const int *faces, const double *uv, int *ids, double *w);
 /* One step of surface::nextFace (src/uv_map.C:1695-1961) for each point:
  * moves (u, v) by (du, dv) within the face, or to the border it reaches
  * first and onto the face across it (faces[i], uv updated), leaving the rest

@@ -16,7 +16,9 @@ the bar is 1000 times the oracle's own response to a one-ulp change of u or v.
 Covered: regular faces, sub-patch levels 1-40 on irregular faces, the vertex
 itself (evaluation only), alpha != 1, frac_mult, the on_surface term, a
 periodic sheet (image shifts), accumulation into the caller's arrays,
-repeatability, and the reference's rejections.
+repeatability, and the reference's rejections; the control-point weights of
+get_pt_coeffs / get_pt_dcoeffs (src/uv_map.C:14-420) against the oracle's
+position and tangents and against pointGradient's coupling.
 """
 import sys
 
@@ -391,3 +393,99 @@ def test_next_face_walk(ves, mesh):
         f, q, d, _, _ = g.next_face(r, f, q, d)
     assert not d.any()
     assert (q.min(axis=1) >= -1e-12).all() and (q.sum(axis=1) <= 1 + 1e-12).all()
+
+
+def _coupled(r, ids, w, d):
+    """alpha * sum_x w[:, d, x] r[ids[:, x]] (no image shifts: closed surfaces)."""
+    nv = (r.size - 3) // 3
+    X = r[:3 * nv].reshape(nv, 3)
+    m = ids >= 0
+    out = np.einsum("nx,nxk->nk", np.where(m, w[:, d], 0.0), X[np.where(m, ids, 0)])
+    return out * r[-3:]
+
+
+@pytest.mark.parametrize("alpha", [(1.0, 1.0, 1.0), (1.1, 0.9, 1.05)])
+def test_point_coeffs_match_oracle(ves, alpha):
+    """get_pt_coeffs / get_pt_dcoeffs (src/uv_map.C:14-420): the weights
+    reproduce the oracle's position and tangents, sum to one (position) and
+    zero (derivatives), and list the face's control points."""
+    w, g, o = ves
+    r = w.r.copy()
+    r[-3:] = alpha
+    faces, uv = random_points(o, 3000, 11, deep=120)
+    ok = well_conditioned(o, faces, uv)
+    say("[points] coeffs n=%d (%d near a vertex) alpha=%s" % (len(faces), (~ok).sum(), alpha))
+    ids, W = g.point_coeffs(faces, uv)
+    ncp = (ids >= 0).sum(axis=1)
+    val = np.array([len(x) for x in meshgen.read_mesh(w.mesh_path).rings])
+    assert (ncp[faces < o.nreg] == 12).all()
+    irr = faces >= o.nreg
+    assert (ncp[irr] == val[ids[irr, 0]] + 6).all()
+    assert (np.where(ids[:, None, :] < 0, W, 0) == 0).all()
+    s = W.sum(axis=2)
+    scale = np.abs(W).sum(axis=2)
+    say("   sum w0 - 1", np.abs(s[:, 0] - 1).max(), "sum dw / sum |dw|", (np.abs(s[:, 1:]) / scale[:, 1:]).max())
+    assert np.abs(s[:, 0] - 1).max() <= 1e-13
+    # near the vertex the derivative weights are sums of 2^d-scaled entries
+    # that cancel: the sum rule holds to the evaluation's conditioning there,
+    # which the tangents below check against the oracle's own response
+    assert (np.abs(s[ok, 1:]) <= 1e-13 * scale[ok, 1:]).all()
+    Ro, _ = o.point_eval(r, faces, uv)
+    R = _coupled(r, ids, W, 0)
+    say("   R rel", rel(R, Ro))
+    assert rel(R, Ro) <= 1e-12
+    Go = o.point_geometry(r, faces, uv)[:, :6]
+    T = np.concatenate([_coupled(r, ids, W, 1), _coupled(r, ids, W, 2)], 1)
+    say("   Ru, Rv rel (well conditioned)", rel(T[ok], Go[ok]))
+    assert rel(T[ok], Go[ok]) <= 1e-12
+    sens = ulp_response(lambda q: o.point_geometry(r, faces[~ok], q)[:, :6], uv[~ok])
+    err = np.abs(T[~ok] - Go[~ok])
+    say("   near the vertex: max err", err.max(), "max 1-ulp response", sens.max())
+    assert (err <= 1e-12 * np.abs(Go[ok]).max() + 1000 * sens).all()
+
+
+def test_point_coeffs_at_extraordinary_vertex(ves):
+    """The reference's mask at the vertex itself: 1/2 on the vertex, 1/(2 val)
+    on its ring in the mesh's ring order, derivative weights zero."""
+    w, g, o = ves
+    rings = meshgen.read_mesh(w.mesh_path).rings
+    faces = np.arange(o.nreg, o.nfaces, 7, dtype=np.int32)
+    uv = np.zeros((faces.size, 2))
+    ids, W = g.point_coeffs(faces, uv)
+    for k in range(faces.size):
+        i = ids[k, 0]
+        val = len(rings[i])
+        assert val != 6
+        assert list(ids[k, 1:1 + val]) == list(rings[i])
+        assert (ids[k, 1 + val:] == -1).all()
+        assert W[k, 0, 0] == 0.5 and (W[k, 0, 1:1 + val] == 1.0 / (2 * val)).all()
+        assert (W[k, 1:] == 0).all() and (W[k, 0, 1 + val:] == 0).all()
+    R = _coupled(w.r, ids, W, 0)
+    Ro, _ = o.point_eval(w.r, faces, uv)
+    assert rel(R, Ro) <= 1e-12
+
+
+@pytest.mark.parametrize("mesh", ["vesicle", "defect_sheet"])
+def test_point_coeffs_couple_the_gradient(ves, mesh):
+    """The gradient the reference's particle couplings build from the weights
+    (dE/dr_x = alpha w_0x dE/dR; src/pcomplex.C:160-200) equals pointGradient's
+    with no normal term, periodic images and valence-7 faces included."""
+    if mesh == "vesicle":
+        w, g, o = ves
+    else:
+        w = workloads.defect_sheet()
+        g, o = _pair(w)
+    r = w.r.copy()
+    r[-3:] = (1.1, 0.9, 1.05)
+    faces, uv = random_points(o, 1500, 12)
+    say("[points] coeffs gradient coupling mesh=%s n=%d" % (mesh, faces.size))
+    rng = np.random.default_rng(13)
+    a = rng.normal(size=(faces.size, 3))
+    G, _ = g.point_grad(r, faces, uv, a, np.zeros_like(a))
+    ids, W = g.point_coeffs(faces, uv)
+    nv = (r.size - 3) // 3
+    Gc = np.zeros((nv, 3))
+    m = ids >= 0
+    np.add.at(Gc, ids[m], (W[:, 0][m])[:, None] * (a * r[-3:])[np.nonzero(m)[0]])
+    say("   vertex gradient rel", rel(Gc.ravel(), G[:-3]))
+    assert rel(Gc.ravel(), G[:-3]) <= 1e-12

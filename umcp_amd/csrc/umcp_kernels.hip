@@ -2074,6 +2074,59 @@ __device__ void point_body(const PointArgs &a, int p, int j, const PointRec &pr,
     }
 }
 
+// surface::get_pt_coeffs / get_pt_dcoeffs (src/uv_map.C:14-420): the
+// control points' vertices and their weights in the position and its
+// derivatives along the face's u and v; at the extraordinary vertex the
+// reference's mask on the vertex and its ring (derivative weights zero)
+template <int KIND>
+__device__ void point_weights_out(const PointArgs &a, int p, const PointRec &pr, const PointClass &C)
+{
+    double *w = a.wts + (size_t)p * 3 * kPointSlots;
+    int *id = a.ids + (size_t)p * kPointSlots;
+    const int nf = C.nf, fc = pr.fc;
+#pragma unroll
+    for (int x = 0; x < kPointSlots; x++) {
+        w[x] = w[kPointSlots + x] = w[2 * kPointSlots + x] = 0.0;
+        id[x] = -1;
+    }
+    if constexpr (KIND == kPointVertex) {
+        const int val = C.valence;
+        const double wr = 1.0 / (val * 2);
+        id[0] = C.cp[fc];
+        w[0] = 0.5;
+        for (int x = 1; x <= val; x++) {
+            id[x] = C.cp[x * nf + fc];
+            w[x] = wr;
+        }
+    } else {
+        double B[6][12];
+        point_basis(pr.u, pr.v, B);
+        const double sc[6] = {1.0, pr.su, pr.sv, pr.su * pr.su, pr.su * pr.sv, pr.sv * pr.sv};
+        const int ncp = KIND == kPointIrregular ? C.ncp : 12;
+        const double *map = KIND == kPointIrregular ? C.map + (size_t)pr.map * 12 * ncp : nullptr;
+        for (int x = 0; x < ncp; x++) {
+            double t[3];
+            if constexpr (KIND == kPointIrregular) {
+                irr_weights<3>(map, ncp, x, B, sc, t);
+            } else {
+                // x runs over the 12 slots: pick the basis column without an
+                // indexed register array
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int y = 0; y < 12; y++) s = (y == x) ? B[d][y] : s;
+                    t[d] = s;
+                }
+            }
+            id[x] = C.cp[x * nf + fc];
+            w[x] = t[0];
+            w[kPointSlots + x] = t[1];
+            w[2 * kPointSlots + x] = t[2];
+        }
+    }
+}
+
 // one thread per point of one kind (idx: the batch's positions of that kind)
 template <int KIND>
 __global__ __launch_bounds__(kPointBlock) void point_kernel(PointArgs a, const int *__restrict__ idx, int count)
@@ -2084,6 +2137,10 @@ __global__ __launch_bounds__(kPointBlock) void point_kernel(PointArgs a, const i
     const int p = a.grad == 1 ? a.order[j] : j;
     const PointRec pr = a.rec[p];
     const PointClass C = a.cls[pr.cls];
+    if (a.grad == 3) {
+        point_weights_out<KIND>(a, p, pr, C);
+        return;
+    }
     if constexpr (KIND == kPointRegular) {
         point_body<false>(a, p, j, pr, C);
     } else if constexpr (KIND == kPointIrregular) {

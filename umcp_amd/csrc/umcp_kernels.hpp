@@ -179,7 +179,7 @@ struct PointSeg {
 struct PointArgs {
     const double *r;         // 3 nv + 3, alpha last
     int nv, n;
-    int grad;                // 0: position and normal, 1: gradient, 2: geometry
+    int grad;                // 0: position and normal, 1: gradient, 2: geometry, 3: weights
     int on_surface;
     double kT;
     double pbc[3][3];
@@ -191,6 +191,8 @@ struct PointArgs {
     double *contrib;         // dE/dr per (slot, sorted position): [kPointSlots][n][3], no alpha
     double *gpuv;            // (u, v) gradient per point [n][2]
     double *geom;            // geometry per point [n][kPointGeom]
+    double *wts;             // weights per point [n][3][kPointSlots] (value, d/du, d/dv)
+    int *ids;                // their vertices [n][kPointSlots] (-1 past the last)
 };
 // geometry record: Ru[3], Rv[3], g, c1 + c2, c1 c2, c1, c2, vec1[2], vec2[2]
 constexpr int kPointGeom = 15;

@@ -240,7 +240,7 @@ int run_points(umcp_surface *s, const double *r, int n, const int *faces, const 
             q.u = u;
             q.v = v;
             q.su = q.sv = 1.0;
-        } else if (const char *msg = resolve_irregular(u, v, mode == 0, q)) {
+        } else if (const char *msg = resolve_irregular(u, v, mode == 0 || mode == 3, q)) {
             return pfail(s, UMCP_ERR_ARG, "point " + std::to_string(i) + ": " + msg);
         }
     }
@@ -293,6 +293,11 @@ int run_points(umcp_surface *s, const double *r, int n, const int *faces, const 
         b.off_R = o;   // the geometry records
         o = align_up(o + sizeof(double) * kPointGeom * (size_t)n);
         b.off_N = b.off_gout = b.off_gp = b.off_con = b.off_apart = 0;
+    } else if (mode == 3) {
+        b.off_R = o;   // weights, then vertex ids
+        b.off_N = align_up(o + sizeof(double) * 3 * kPointSlots * (size_t)n);
+        o = align_up(b.off_N + sizeof(int) * kPointSlots * (size_t)n);
+        b.off_gout = b.off_gp = b.off_con = b.off_apart = 0;
     } else {
         b.off_R = o;
         b.off_N = o + sizeof(double) * 3 * (size_t)n;
@@ -305,7 +310,8 @@ int run_points(umcp_surface *s, const double *r, int n, const int *faces, const 
     if (rc == UMCP_OK) rc = pin_grow(s, &s->pt_stage, &s->pt_stage_cap, b.in_bytes);
     if (rc) return rc;
     char *h = s->pt_stage;
-    std::memcpy(h + b.off_r, r, sizeof(double) * (3 * (size_t)nv + 3));
+    if (r) std::memcpy(h + b.off_r, r, sizeof(double) * (3 * (size_t)nv + 3));
+    else std::memset(h + b.off_r, 0, sizeof(double) * (3 * (size_t)nv + 3));
     std::memcpy(h + b.off_rec, rec.data(), sizeof(PointRec) * (size_t)n);
     if (grad) std::memcpy(h + b.off_order, order.data(), sizeof(int) * order.size());
     std::memcpy(h + b.off_idx, idx.data(), sizeof(int) * (size_t)n);
@@ -337,6 +343,9 @@ int run_points(umcp_surface *s, const double *r, int n, const int *faces, const 
         a.gpuv = (double *)(dout + b.off_gp);
     } else if (mode == 2) {
         a.geom = (double *)(dout + b.off_R);
+    } else if (mode == 3) {
+        a.wts = (double *)(dout + b.off_R);
+        a.ids = (int *)(dout + b.off_N);
     } else {
         a.R = (double *)(dout + b.off_R);
         a.nrm = (double *)(dout + b.off_N);
@@ -365,11 +374,11 @@ int run_points(umcp_surface *s, const double *r, int n, const int *faces, const 
     return UMCP_OK;
 }
 
-int point_check(umcp_surface *s, const double *r, int n, const int *faces, const double *uv)
+int point_check(umcp_surface *s, const double *r, int n, const int *faces, const double *uv, bool need_r = true)
 {
     if (!s) return UMCP_ERR_ARG;
     if (!s->planned) return pfail(s, UMCP_ERR_STATE, "plan has not been generated (call umcp_plan)");
-    if (n < 0 || (n > 0 && (!r || !faces || !uv))) return pfail(s, UMCP_ERR_ARG, "null point arrays");
+    if (n < 0 || (n > 0 && ((need_r && !r) || !faces || !uv))) return pfail(s, UMCP_ERR_ARG, "null point arrays");
     if (n > UMCP_POINT_MAX_BATCH) return pfail(s, UMCP_ERR_ARG, "more points than UMCP_POINT_MAX_BATCH");
     hipError_t e = hipSetDevice(s->device);
     if (e != hipSuccess) return phip(s, e, "hipSetDevice");
@@ -422,6 +431,35 @@ int umcp_point_grad(umcp_surface *s, const double *r, int n, const int *faces, c
     // the reference's contract: the caller's arrays accumulate (+=)
     for (size_t i = 0; i < 3 * (size_t)s->T.nv + 3; i++) g[i] += gs[i];
     for (size_t i = 0; i < 2 * (size_t)n; i++) g_puv[i] += gp[i];
+    return UMCP_OK;
+}
+
+int umcp_point_coeffs(umcp_surface *s, int n, const int *faces, const double *uv, int *ids, double *w)
+{
+    int rc = point_check(s, nullptr, n, faces, uv, false);
+    if (rc) return rc;
+    if (n == 0) return UMCP_OK;
+    if (!ids || !w) return pfail(s, UMCP_ERR_ARG, "null output arrays");
+    Batch b;
+    rc = run_points(s, nullptr, n, faces, uv, nullptr, nullptr, nullptr, 0, 0.0, 3, b);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(w, s->d_pt_out + b.off_R, sizeof(double) * 3 * kPointSlots * (size_t)n,
+                                  hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(ids, s->d_pt_out + b.off_N, sizeof(int) * kPointSlots * (size_t)n, hipMemcpyDeviceToHost,
+                           s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return phip(s, e, "download point weights");
+    // at the extraordinary vertex the reference lists the ring in the
+    // vertex's own (oriented) edge order, src/uv_map.C:46-52; the face's
+    // control-point table holds the same vertices rotated to the face's edge
+    const PointRec *rec = (const PointRec *)(s->pt_stage + b.off_rec);
+    for (int i = 0; i < n; i++) {
+        if (rec[i].kind != kPointVertex) continue;
+        int *id = ids + (size_t)i * kPointSlots;
+        const int v = id[0], val = s->T.valence[v];
+        for (int k = 0; k < val; k++) id[1 + k] = s->T.ring[(size_t)v * kMaxValence + k];
+    }
     return UMCP_OK;
 }
 

@@ -44,7 +44,7 @@ EXPORTS = [
     "umcp_effm_build", "umcp_effm_get", "umcp_effm_steps", "umcp_effm_steps_dev",
     "umcp_save_mesh", "umcp_write_restart", "umcp_read_restart", "umcp_write_limit_surface",
     "umcp_point_eval", "umcp_point_grad", "umcp_point_geometry", "umcp_write_limit_psf",
-    "umcp_face_energies", "umcp_energy_vertex", "umcp_area_vertex", "umcp_next_face",
+    "umcp_face_energies", "umcp_energy_vertex", "umcp_area_vertex", "umcp_next_face", "umcp_point_coeffs",
 ]
 
 
@@ -146,6 +146,7 @@ def load_library(path: str = LIB_PATH):
         "umcp_energy_vertex": ([vp, dp, C.c_int, dp], C.c_int),
         "umcp_area_vertex": ([vp, dp, C.c_int, dp, dp], C.c_int),
         "umcp_next_face": ([vp, dp, C.c_int, ip, dp, dp, dp, dp], C.c_int),
+        "umcp_point_coeffs": ([vp, C.c_int, ip, dp, ip, dp], C.c_int),
         "umcp_point_grad": ([vp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int, C.c_double, dp, dp], C.c_int),
         "umcp_write_restart": ([C.c_char_p, C.c_int, dp, dp, C.c_int], C.c_int),
         "umcp_read_restart": ([C.c_char_p, C.c_int, dp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
@@ -390,6 +391,15 @@ class Surface:
         E = C.c_double()
         self._check(self._L.umcp_energy_vertex(self._h, _dp(r), int(vertex), C.byref(E)), "energy_vertex")
         return E.value
+
+    def point_coeffs(self, faces, uv):
+        """(ids [n, 18], weights [n, 3, 18]): get_pt_coeffs / get_pt_dcoeffs."""
+        f = np.ascontiguousarray(faces, dtype=np.int32)
+        uv = np.ascontiguousarray(uv, dtype=np.float64).reshape(-1, 2)
+        ids = np.zeros((f.size, 18), dtype=np.int32)
+        w = np.zeros((f.size, 3, 18))
+        self._check(self._L.umcp_point_coeffs(self._h, f.size, _ip(f), _dp(uv), _ip(ids), _dp(w)), "point_coeffs")
+        return ids, w
 
     def next_face(self, r: np.ndarray, faces, uv, duv, mom=None, coord_transform=None):
         """One nextFace step per point: returns (faces, uv, duv, mom, coord_transform)
