@@ -489,3 +489,37 @@ def test_point_coeffs_couple_the_gradient(ves, mesh):
     np.add.at(Gc, ids[m], (W[:, 0][m])[:, None] * (a * r[-3:])[np.nonzero(m)[0]])
     say("   vertex gradient rel", rel(Gc.ravel(), G[:-3]))
     assert rel(Gc.ravel(), G[:-3]) <= 1e-12
+
+
+def test_threaded_staging_matches_serial(ves):
+    """Batches of 32768 points and more are resolved and staged over host
+    threads: per-point results equal those of small (single-threaded)
+    batches bitwise, and a rejection names the first bad point."""
+    w, g, o = ves
+    r = w.r.copy()
+    r[-3:] = (1.1, 0.9, 1.05)
+    n = 150000
+    faces, uv = random_points(o, n, 21, deep=40)
+    say("[points] threaded staging n=%d" % n)
+    R, N = g.point_eval(r, faces, uv)
+    Gm = g.point_geometry(r, faces, uv)
+    ids, W = g.point_coeffs(faces, uv)
+    rng = np.random.default_rng(22)
+    a, b = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+    _, P = g.point_grad(r, faces, uv, a, b)
+    for k in range(0, n, 20000):
+        s = slice(k, k + 20000)
+        R2, N2 = g.point_eval(r, faces[s], uv[s])
+        assert np.array_equal(R2, R[s]) and np.array_equal(N2, N[s], equal_nan=True)
+        assert np.array_equal(g.point_geometry(r, faces[s], uv[s]), Gm[s], equal_nan=True)
+        i2, W2 = g.point_coeffs(faces[s], uv[s])
+        assert np.array_equal(i2, ids[s]) and np.array_equal(W2, W[s])
+        _, P2 = g.point_grad(r, faces[s], uv[s], a[s], b[s])
+        assert np.array_equal(P2, P[s])
+    bad_f, bad_uv = faces.copy(), uv.copy()
+    bad_f[120000] = o.nfaces
+    bad_uv[70000] = np.nan
+    with pytest.raises(S.UmcpError, match="point 70000: non-finite"):
+        g.point_eval(r, bad_f, bad_uv)
+    with pytest.raises(S.UmcpError, match="point 120000: face index out of range"):
+        g.point_grad(r, bad_f, uv, a, b)

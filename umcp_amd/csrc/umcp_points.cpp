@@ -18,6 +18,8 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 #include "umcp_state.hpp"
@@ -213,73 +215,176 @@ int pin_grow(umcp_surface *s, char **buf, size_t *cap, size_t need)
     return UMCP_OK;
 }
 
+// Host threads for a batch of n points: one below 32768 points, else up to
+// 16 (the host work is memory bound: records, copies, the face sort).
+int host_threads(size_t n)
+{
+    if (n < 32768) return 1;
+    const unsigned hc = std::thread::hardware_concurrency();
+    const int t = (int)std::min<unsigned>(hc ? hc : 1u, 16u);
+    return std::max(1, std::min(t, (int)(n / 16384)));
+}
+
+// f(t, lo, hi) over nt contiguous chunks of [0, n), chunk t on thread t
+// (chunk 0 on the caller); a thread that cannot be started runs its chunk
+// on the caller instead
+template <class F>
+void par_chunks(int nt, size_t n, F &&f)
+{
+    if (nt <= 1) {
+        f(0, (size_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve((size_t)nt - 1);
+    for (int t = 1; t < nt; t++) {
+        const size_t lo = n * (size_t)t / (size_t)nt, hi = n * (size_t)(t + 1) / (size_t)nt;
+        try {
+            th.emplace_back([&f, t, lo, hi] { f(t, lo, hi); });
+        } catch (const std::system_error &) {
+            f(t, lo, hi);
+        }
+    }
+    f(0, (size_t)0, n / (size_t)nt);
+    for (auto &x : th) x.join();
+}
+
+void par_copy(int nt, void *dst, const void *src, size_t bytes)
+{
+    par_chunks(nt, bytes, [&](int, size_t lo, size_t hi) {
+        std::memcpy((char *)dst + lo, (const char *)src + lo, hi - lo);
+    });
+}
+
 // resolves the points, orders a gradient batch by face, stages and launches;
-// mode 0 evaluation, 1 gradient, 2 geometry
+// mode 0 evaluation, 1 gradient, 2 geometry, 3 control-point weights.
+// Records go straight into the pinned staging buffer, over host threads in
+// contiguous chunks of the batch; a rejection names the first bad point, as
+// a serial pass would.
 int run_points(umcp_surface *s, const double *r, int n, const int *faces, const double *uv, const double *de_dr,
                const double *de_dnrm, const double *frac, int on_surface, double kT, int mode, Batch &b)
 {
     const bool grad = mode == 1;
     const int nv = s->T.nv;
     const Plan &P = s->P;
-    std::vector<PointRec> rec((size_t)n);
-    for (int i = 0; i < n; i++) {
-        const int f = faces[i];
-        if (f < 0 || f >= P.nfaces)
-            return pfail(s, UMCP_ERR_ARG, "point " + std::to_string(i) + ": face index out of range");
-        PointRec &q = rec[(size_t)i];
-        q.cls = s->pt_face_cls[(size_t)f];
-        q.fc = s->pt_face_fc[(size_t)f];
-        const double u = uv[2 * i], v = uv[2 * i + 1];
-        q.fu = u;
-        q.fv = v;
-        if (!std::isfinite(u) || !std::isfinite(v))
-            return pfail(s, UMCP_ERR_ARG, "point " + std::to_string(i) + ": non-finite (u, v)");
-        if (P.classes[(size_t)q.cls].valence == 6) {
-            q.kind = kPointRegular;
-            q.map = 0;
-            q.u = u;
-            q.v = v;
-            q.su = q.sv = 1.0;
-        } else if (const char *msg = resolve_irregular(u, v, mode == 0 || mode == 3, q)) {
-            return pfail(s, UMCP_ERR_ARG, "point " + std::to_string(i) + ": " + msg);
-        }
-    }
-    // gradient: positions j in face order (stable), one segment per face
-    // with points; evaluation: the caller's order
-    std::vector<int> order, idx((size_t)n);
-    std::vector<PointSeg> seg;
-    int counts[3] = {0, 0, 0};
-    if (grad) {
-        std::vector<int> start((size_t)P.nfaces + 1, 0);
-        for (int i = 0; i < n; i++) start[(size_t)faces[i] + 1]++;
-        for (int f = 0; f < P.nfaces; f++) {
-            if (start[(size_t)f + 1])
-                seg.push_back({s->pt_face_cls[(size_t)f], s->pt_face_fc[(size_t)f], start[(size_t)f],
-                               start[(size_t)f] + start[(size_t)f + 1]});
-            start[(size_t)f + 1] += start[(size_t)f];
-        }
-        order.resize((size_t)n);
-        for (int i = 0; i < n; i++) order[(size_t)start[(size_t)faces[i]]++] = i;
-        for (int j = 0; j < n; j++) counts[rec[(size_t)order[(size_t)j]].kind]++;
-        int fill[3] = {0, counts[0], counts[0] + counts[1]};
-        for (int j = 0; j < n; j++) idx[(size_t)fill[rec[(size_t)order[(size_t)j]].kind]++] = j;
-    } else {
-        for (int i = 0; i < n; i++) counts[rec[(size_t)i].kind]++;
-        int fill[3] = {0, counts[0], counts[0] + counts[1]};
-        for (int i = 0; i < n; i++) idx[(size_t)fill[rec[(size_t)i].kind]++] = i;
-    }
-    const int nseg = (int)seg.size();
-    // staging layout
+    const int nfaces = P.nfaces;
+    // staging layout; the segment table (at most one per face with points)
+    // goes last so that only its used part is uploaded
+    const size_t seg_cap = grad ? std::min<size_t>((size_t)n, (size_t)nfaces) : 0;
     size_t o = 0;
     b.off_r = o;      o = align_up(o + sizeof(double) * (3 * (size_t)nv + 3));
     b.off_rec = o;    o = align_up(o + sizeof(PointRec) * (size_t)n);
-    b.off_order = o;  o = align_up(o + sizeof(int) * order.size());
+    b.off_order = o;  o = align_up(o + (grad ? sizeof(int) * (size_t)n : 0));
     b.off_idx = o;    o = align_up(o + sizeof(int) * (size_t)n);
     b.off_dr = o;     o = align_up(o + (grad ? sizeof(double) * 3 * (size_t)n : 0));
     b.off_dn = o;     o = align_up(o + (grad ? sizeof(double) * 3 * (size_t)n : 0));
     b.off_frac = o;   o = align_up(o + (grad && frac ? sizeof(double) * (size_t)n : 0));
-    b.off_seg = o;    o = align_up(o + sizeof(PointSeg) * seg.size());
-    b.in_bytes = o;
+    b.off_seg = o;    o = align_up(o + sizeof(PointSeg) * seg_cap);
+    const size_t in_cap = o;
+    int rc = pin_grow(s, &s->pt_stage, &s->pt_stage_cap, in_cap);
+    if (rc) return rc;
+    char *h = s->pt_stage;
+    PointRec *rec = (PointRec *)(h + b.off_rec);
+    int *order = (int *)(h + b.off_order), *idx = (int *)(h + b.off_idx);
+    PointSeg *seg = (PointSeg *)(h + b.off_seg);
+
+    const int nt = host_threads((size_t)n);
+    struct Chunk {
+        int counts[3] = {0, 0, 0};
+        int bad = -1;              // first rejected point of the chunk
+        const char *msg = nullptr;
+        std::vector<int> fcount;   // gradient: points per face in the chunk
+    };
+    std::vector<Chunk> ch((size_t)nt);
+    const std::vector<int> &fcls = s->pt_face_cls, &ffc = s->pt_face_fc;
+    par_chunks(nt, (size_t)n, [&](int t, size_t lo, size_t hi) {
+        Chunk &c = ch[(size_t)t];
+        if (grad) c.fcount.assign((size_t)nfaces, 0);
+        for (size_t i = lo; i < hi; i++) {
+            const int f = faces[i];
+            if (f < 0 || f >= nfaces) {
+                c.bad = (int)i;
+                c.msg = "face index out of range";
+                return;
+            }
+            PointRec &q = rec[i];
+            q.cls = fcls[(size_t)f];
+            q.fc = ffc[(size_t)f];
+            const double u = uv[2 * i], v = uv[2 * i + 1];
+            q.fu = u;
+            q.fv = v;
+            if (!std::isfinite(u) || !std::isfinite(v)) {
+                c.bad = (int)i;
+                c.msg = "non-finite (u, v)";
+                return;
+            }
+            if (P.classes[(size_t)q.cls].valence == 6) {
+                q.kind = kPointRegular;
+                q.map = 0;
+                q.u = u;
+                q.v = v;
+                q.su = q.sv = 1.0;
+            } else if (const char *m = resolve_irregular(u, v, mode == 0 || mode == 3, q)) {
+                c.bad = (int)i;
+                c.msg = m;
+                return;
+            }
+            c.counts[q.kind]++;
+            if (grad) c.fcount[(size_t)f]++;
+        }
+    });
+    for (const Chunk &c : ch)
+        if (c.bad >= 0) return pfail(s, UMCP_ERR_ARG, "point " + std::to_string(c.bad) + ": " + c.msg);
+    int counts[3] = {0, 0, 0};
+    for (const Chunk &c : ch)
+        for (int k = 0; k < 3; k++) counts[k] += c.counts[k];
+
+    int nseg = 0;
+    if (grad) {
+        // stable counting sort by face: chunk t's points of face f start at
+        // the face's start plus the earlier chunks' counts of f, so the order
+        // is that of one serial pass.  One segment per face with points; the
+        // kind is the face's (regular or irregular; the vertex itself is
+        // rejected above), so the kind lists are runs of segments.
+        int fill[3] = {0, counts[0], counts[0] + counts[1]};
+        int pos = 0;
+        for (int f = 0; f < nfaces; f++) {
+            int cnt = 0;
+            for (Chunk &c : ch) {
+                const int k = c.fcount[(size_t)f];
+                c.fcount[(size_t)f] = pos + cnt;
+                cnt += k;
+            }
+            if (!cnt) continue;
+            seg[nseg++] = {fcls[(size_t)f], ffc[(size_t)f], pos, pos + cnt};
+            const int kind = P.classes[(size_t)fcls[(size_t)f]].valence == 6 ? kPointRegular : kPointIrregular;
+            for (int j = pos; j < pos + cnt; j++) idx[fill[kind]++] = j;
+            pos += cnt;
+        }
+        par_chunks(nt, (size_t)n, [&](int t, size_t lo, size_t hi) {
+            std::vector<int> &next = ch[(size_t)t].fcount;
+            for (size_t i = lo; i < hi; i++) order[next[(size_t)faces[i]]++] = (int)i;
+            std::memcpy(h + b.off_dr + sizeof(double) * 3 * lo, de_dr + 3 * lo, sizeof(double) * 3 * (hi - lo));
+            std::memcpy(h + b.off_dn + sizeof(double) * 3 * lo, de_dnrm + 3 * lo, sizeof(double) * 3 * (hi - lo));
+            if (frac) std::memcpy(h + b.off_frac + sizeof(double) * lo, frac + lo, sizeof(double) * (hi - lo));
+        });
+    } else {
+        // positions grouped by kind, each group in the caller's order
+        std::vector<int> base((size_t)nt * 3);
+        int fill[3] = {0, counts[0], counts[0] + counts[1]};
+        for (int t = 0; t < nt; t++)
+            for (int k = 0; k < 3; k++) {
+                base[(size_t)t * 3 + k] = fill[k];
+                fill[k] += ch[(size_t)t].counts[k];
+            }
+        par_chunks(nt, (size_t)n, [&](int t, size_t lo, size_t hi) {
+            int *nx = &base[(size_t)t * 3];
+            for (size_t i = lo; i < hi; i++) idx[nx[rec[i].kind]++] = (int)i;
+        });
+    }
+    if (r) std::memcpy(h + b.off_r, r, sizeof(double) * (3 * (size_t)nv + 3));
+    else std::memset(h + b.off_r, 0, sizeof(double) * (3 * (size_t)nv + 3));
+    b.in_bytes = b.off_seg + sizeof(PointSeg) * (size_t)nseg;
     o = 0;
     if (grad) {
         // [vertex sums 3 nv][alpha 3][gpuv 2n] first: one download
@@ -305,22 +410,9 @@ int run_points(umcp_surface *s, const double *r, int n, const int *faces, const 
         b.off_gout = b.off_gp = b.off_con = b.off_apart = 0;
     }
     b.out_bytes = o;
-    int rc = pgrow(s, &s->d_pt_in, &s->pt_in_cap, b.in_bytes);
+    rc = pgrow(s, &s->d_pt_in, &s->pt_in_cap, b.in_bytes);
     if (rc == UMCP_OK) rc = pgrow(s, &s->d_pt_out, &s->pt_out_cap, b.out_bytes);
-    if (rc == UMCP_OK) rc = pin_grow(s, &s->pt_stage, &s->pt_stage_cap, b.in_bytes);
     if (rc) return rc;
-    char *h = s->pt_stage;
-    if (r) std::memcpy(h + b.off_r, r, sizeof(double) * (3 * (size_t)nv + 3));
-    else std::memset(h + b.off_r, 0, sizeof(double) * (3 * (size_t)nv + 3));
-    std::memcpy(h + b.off_rec, rec.data(), sizeof(PointRec) * (size_t)n);
-    if (grad) std::memcpy(h + b.off_order, order.data(), sizeof(int) * order.size());
-    std::memcpy(h + b.off_idx, idx.data(), sizeof(int) * (size_t)n);
-    if (grad) {
-        std::memcpy(h + b.off_dr, de_dr, sizeof(double) * 3 * (size_t)n);
-        std::memcpy(h + b.off_dn, de_dnrm, sizeof(double) * 3 * (size_t)n);
-        if (frac) std::memcpy(h + b.off_frac, frac, sizeof(double) * (size_t)n);
-        std::memcpy(h + b.off_seg, seg.data(), sizeof(PointSeg) * seg.size());
-    }
     hipError_t e = hipMemcpyAsync(s->d_pt_in, h, b.in_bytes, hipMemcpyHostToDevice, s->stream);
     if (e != hipSuccess) return phip(s, e, "upload points");
     PointArgs a{};
@@ -374,6 +466,19 @@ int run_points(umcp_surface *s, const double *r, int n, const int *faces, const 
     return UMCP_OK;
 }
 
+// copies [off, off + bytes) of the batch's device output into pinned host
+// memory (the caller's arrays may be pageable) and returns it there
+int fetch_out(umcp_surface *s, size_t off, size_t bytes, const char **host, const char *what)
+{
+    int rc = pin_grow(s, &s->pt_host, &s->pt_host_cap, bytes);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(s->pt_host, s->d_pt_out + off, bytes, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return phip(s, e, what);
+    *host = s->pt_host;
+    return UMCP_OK;
+}
+
 int point_check(umcp_surface *s, const double *r, int n, const int *faces, const double *uv, bool need_r = true)
 {
     if (!s) return UMCP_ERR_ARG;
@@ -399,13 +504,13 @@ int umcp_point_eval(umcp_surface *s, const double *r, int n, const int *faces, c
     Batch b;
     rc = run_points(s, r, n, faces, uv, nullptr, nullptr, nullptr, 0, 0.0, 0, b);
     if (rc) return rc;
-    hipError_t e = hipMemcpyAsync(R, s->d_pt_out + b.off_R, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost,
-                                  s->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(nrm, s->d_pt_out + b.off_N, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost,
-                           s->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    if (e != hipSuccess) return phip(s, e, "download points");
+    const size_t bytes = sizeof(double) * 3 * (size_t)n;   // R, then nrm
+    const char *hst = nullptr;
+    rc = fetch_out(s, b.off_R, b.off_N - b.off_R + bytes, &hst, "download points");
+    if (rc) return rc;
+    const int nt = host_threads((size_t)n);
+    par_copy(nt, R, hst, bytes);
+    par_copy(nt, nrm, hst + (b.off_N - b.off_R), bytes);
     return UMCP_OK;
 }
 
@@ -420,17 +525,16 @@ int umcp_point_grad(umcp_surface *s, const double *r, int n, const int *faces, c
     Batch b;
     rc = run_points(s, r, n, faces, uv, de_dr, de_dnrm, frac_mult, on_surface, kT, 1, b);
     if (rc) return rc;
-    const size_t cnt = 3 * (size_t)s->T.nv + 3 + 2 * (size_t)n;   // vertex sums, alpha, gpuv: contiguous
-    rc = pin_grow(s, &s->pt_host, &s->pt_host_cap, sizeof(double) * cnt);
+    const size_t ng = 3 * (size_t)s->T.nv + 3;   // vertex sums, alpha, then gpuv: contiguous
+    const char *hst = nullptr;
+    rc = fetch_out(s, b.off_gout, sizeof(double) * (ng + 2 * (size_t)n), &hst, "download point gradient");
     if (rc) return rc;
-    hipError_t e = hipMemcpyAsync(s->pt_host, s->d_pt_out + b.off_gout, sizeof(double) * cnt,
-                                  hipMemcpyDeviceToHost, s->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    if (e != hipSuccess) return phip(s, e, "download point gradient");
-    const double *gs = (const double *)s->pt_host, *gp = gs + 3 * (size_t)s->T.nv + 3;
+    const double *gs = (const double *)hst, *gp = gs + ng;
     // the reference's contract: the caller's arrays accumulate (+=)
-    for (size_t i = 0; i < 3 * (size_t)s->T.nv + 3; i++) g[i] += gs[i];
-    for (size_t i = 0; i < 2 * (size_t)n; i++) g_puv[i] += gp[i];
+    for (size_t i = 0; i < ng; i++) g[i] += gs[i];
+    par_chunks(host_threads((size_t)n), 2 * (size_t)n, [&](int, size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; i++) g_puv[i] += gp[i];
+    });
     return UMCP_OK;
 }
 
@@ -443,13 +547,13 @@ int umcp_point_coeffs(umcp_surface *s, int n, const int *faces, const double *uv
     Batch b;
     rc = run_points(s, nullptr, n, faces, uv, nullptr, nullptr, nullptr, 0, 0.0, 3, b);
     if (rc) return rc;
-    hipError_t e = hipMemcpyAsync(w, s->d_pt_out + b.off_R, sizeof(double) * 3 * kPointSlots * (size_t)n,
-                                  hipMemcpyDeviceToHost, s->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(ids, s->d_pt_out + b.off_N, sizeof(int) * kPointSlots * (size_t)n, hipMemcpyDeviceToHost,
-                           s->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    if (e != hipSuccess) return phip(s, e, "download point weights");
+    const size_t wb = sizeof(double) * 3 * kPointSlots * (size_t)n, ib = sizeof(int) * kPointSlots * (size_t)n;
+    const char *hst = nullptr;
+    rc = fetch_out(s, b.off_R, b.off_N - b.off_R + ib, &hst, "download point weights");
+    if (rc) return rc;
+    const int nt = host_threads((size_t)n);
+    par_copy(nt, w, hst, wb);
+    par_copy(nt, ids, hst + (b.off_N - b.off_R), ib);
     // at the extraordinary vertex the reference lists the ring in the
     // vertex's own (oriented) edge order, src/uv_map.C:46-52; the face's
     // control-point table holds the same vertices rotated to the face's edge
@@ -472,10 +576,11 @@ int umcp_point_geometry(umcp_surface *s, const double *r, int n, const int *face
     Batch b;
     rc = run_points(s, r, n, faces, uv, nullptr, nullptr, nullptr, 0, 0.0, 2, b);
     if (rc) return rc;
-    hipError_t e = hipMemcpyAsync(geom, s->d_pt_out + b.off_R, sizeof(double) * kPointGeom * (size_t)n,
-                                  hipMemcpyDeviceToHost, s->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    if (e != hipSuccess) return phip(s, e, "download point geometry");
+    const size_t bytes = sizeof(double) * kPointGeom * (size_t)n;
+    const char *hst = nullptr;
+    rc = fetch_out(s, b.off_R, bytes, &hst, "download point geometry");
+    if (rc) return rc;
+    par_copy(host_threads((size_t)n), geom, hst, bytes);
     return UMCP_OK;
 }
 
