@@ -276,6 +276,18 @@ int umcp_point_geometry(umcp_surface *s, const double *r, int n, const int *face
  * vertex, 1/(2 val) on its ring, derivative weights zero. */
 #define UMCP_POINT_SLOTS 18
 int umcp_point_coeffs(umcp_surface *s, int n, const int *faces, const double *uv, int *ids, double *w);
+/* surface::penergy (src/interpLib.C:8330-8790) for n particles on the
+ * limit surface, batched: particle i on face faces[i] at uv[2i], uv[2i+1],
+ * with curvature preference c0[i] and area area[i]:
+ *   out[i] = 1/2 kc area (c1 + c2 - c0)^2 - area * rho_f,
+ * c1, c2 the principal curvatures at the point and rho_f the face's
+ * curvature-energy density as faceEnergy forms it (regular faces
+ * 1/2 kc (c1 + c2 - c0_f)^2, irregular faces the point energy density, over
+ * dudv g0 w; src/interpLib.C:3612, 3850).  faceEnergy of a face carrying
+ * particles adds the sum of its particles' out[i] (src/interpLib.C:3636,
+ * 3878).  The Monge-gauge form (do_monge) is not provided. */
+int umcp_particle_energy(umcp_surface *s, const double *r, int n, const int *faces, const double *uv,
+                         const double *c0, const double *area, double *out);
 /* One step of surface::nextFace (src/uv_map.C:1695-1961) for each point:
  * moves (u, v) by (du, dv) within the face, or to the border it reaches
  * first and onto the face across it (faces[i], uv updated), leaving the rest

@@ -92,6 +92,11 @@ def lib():
         L.orc_point_grad.restype = C.c_int
         L.orc_point_geometry.argtypes = [vp, dp, C.c_int, C.c_double, C.c_double, dp]
         L.orc_point_geometry.restype = C.c_int
+        L.orc_face_density.argtypes = [vp, C.c_int, dp, dp, dp]
+        L.orc_face_density.restype = C.c_int
+        L.orc_particle_energy.argtypes = [vp, dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                          C.c_double, dp]
+        L.orc_particle_energy.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -300,6 +305,33 @@ class OracleSurface:
             if lib().orc_point_geometry(self._h, _p(r), int(faces[i]), float(uv[i, 0]), float(uv[i, 1]), _p(t)):
                 raise ValueError("oracle: point %d outside the reference's domain" % i)
             out[i] = t
+        return out
+
+    def face_density(self, r: np.ndarray, f: int):
+        """(numerator, area) of the face's curvature-energy density (penergy's
+        face_energy_density before the division)."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        num, area = C.c_double(), C.c_double()
+        if lib().orc_face_density(self._h, int(f), _p(r), C.byref(num), C.byref(area)):
+            raise ValueError("oracle: face out of range")
+        return num.value, area.value
+
+    def particle_energy(self, r: np.ndarray, faces, uv, c0, area) -> np.ndarray:
+        """penergy's term of every particle: 1/2 kc a (c1 + c2 - c0)^2 - a rho_f."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        uv = np.asarray(uv, dtype=np.float64).reshape(-1, 2)
+        dens = {}
+        out = np.zeros(uv.shape[0])
+        for i in range(uv.shape[0]):
+            f = int(faces[i])
+            if f not in dens:
+                num, ar = self.face_density(r, f)
+                dens[f] = num / ar
+            t = C.c_double()
+            if lib().orc_particle_energy(self._h, _p(r), f, float(uv[i, 0]), float(uv[i, 1]), float(c0[i]),
+                                         float(area[i]), dens[f], C.byref(t)):
+                raise ValueError("oracle: particle %d outside the reference's domain" % i)
+            out[i] = t.value
         return out
 
     def face_info(self, f: int):

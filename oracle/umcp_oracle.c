@@ -1412,6 +1412,72 @@ double orc_face_energy(omesh *m, int f, const double *r)
     return e;
 }
 
+/* The face's curvature-energy density as fenergy / ifenergy form it for
+ * penergy (src/interpLib.C:3850-3851 regular, 3612-3613 irregular, FIXED_A
+ * so dAf = g0 w): regular faces sum dudv dAf 1/2 kc (c1 + c2 - c0)^2, irregular
+ * faces dudv dAf en with en the point energy density (kg c1 c2 under
+ * LOCAL_LIPID_ENERGY, else 1/2 kc (c1 + c2 - c0)^2 + kg c1 c2); area the sum
+ * of dudv dAf. */
+int orc_face_density(omesh *m, int f, const double *r, double *num, double *area)
+{
+    int npt;
+    double dudv;
+    if (f < 0 || f >= m->nfr + m->nfi) return -1;
+    oform *F = face_forms(m, f, &npt, &dudv);
+    const oparams *P = &m->prm;
+    const double c0 = m->face_c0[f];
+    double s = 0.0, a = 0.0;
+    for (int p = 0; p < npt; p++) {
+        double X[6][3];
+        contract(m, F + p, r, X);
+        double *Ru = X[1], *Rv = X[2], *Suu = X[3], *Suv = X[4], *Svv = X[5];
+        double nrm[3];
+        cross3(Ru, Rv, nrm);
+        normalize3(nrm);
+        double RuRu = Ru[0] * Ru[0] + Ru[1] * Ru[1] + Ru[2] * Ru[2];
+        double RuRv = Ru[0] * Rv[0] + Ru[1] * Rv[1] + Ru[2] * Rv[2];
+        double RvRv = Rv[0] * Rv[0] + Rv[1] * Rv[1] + Rv[2] * Rv[2];
+        double g = sqrt(RuRu * RvRv - RuRv * RuRv);
+        double nsuu = Suu[0] * nrm[0] + Suu[1] * nrm[1] + Suu[2] * nrm[2];
+        double nsuv = Suv[0] * nrm[0] + Suv[1] * nrm[1] + Suv[2] * nrm[2];
+        double nsvv = Svv[0] * nrm[0] + Svv[1] * nrm[1] + Svv[2] * nrm[2];
+        double sa = 1.0 / (g * g) * (nsuu * RvRv - nsuv * RuRv);
+        double sb = (1.0 / (g * g)) * (nsuv * RvRv - nsvv * RuRv);
+        double sc = 1.0 / (g * g) * (nsuv * RuRu - nsuu * RuRv);
+        double sd = (1.0 / (g * g)) * (nsvv * RuRu - nsuv * RuRv);
+        double c1 = -0.5 * (sa + sd - sqrt(SAFETY_ + sa * sa + 4 * sb * sc - 2 * sa * sd + sd * sd));
+        double c2 = -0.5 * (sa + sd + sqrt(SAFETY_ + sa * sa + 4 * sb * sc - 2 * sa * sd + sd * sd));
+        double dAf = F[p].g0 * F[p].weight;
+        double x;
+        if (f < m->nfr)
+            x = 0.5 * P->kc * (c1 + c2 - c0) * (c1 + c2 - c0);
+        else
+            x = (P->mode == 0) ? P->kg * c1 * c2 : 0.5 * P->kc * (c1 + c2 - c0) * (c1 + c2 - c0) + P->kg * c1 * c2;
+        s += dudv * dAf * x;
+        a += dudv * dAf;
+    }
+    *num = s;
+    *area = a;
+    return 0;
+}
+
+/* surface::penergy for one particle (src/interpLib.C:8330-8790, do_monge 0):
+ * 1/2 kc pa (c1 + c2 - pc0)^2 - pa fed, c1 and c2 at the particle's (u, v)
+ * (the curvatures do not change with the sub-patch's power-of-two scale, so
+ * the point geometry's serve), fed the face's density over its area */
+int orc_particle_energy(const omesh *m, const double *r, int f, double u, double v, double pc0, double pa,
+                        double fed, double *out)
+{
+    double gq[15];
+    if (orc_point_geometry(m, r, f, u, v, gq) != 0) return -1;
+    const double c1 = gq[9], c2 = gq[10];
+    double en = 0.0;
+    en += 0.5 * m->prm.kc * pa * (c1 + c2 - pc0) * (c1 + c2 - pc0);
+    en -= pa * fed;
+    *out = en;
+    return 0;
+}
+
 /* surface::energy: faces summed in triangle order (src/interpLib.C:4315-4320,
  * src/parallel.C:319-326) */
 double orc_energy(omesh *m, const double *r)

@@ -110,6 +110,9 @@ int    orc_point_grad(const omesh *m, const double *r, int f, double u, double v
 /* point geometry (surface::ru, rv, g, c): out[15] = Ru[3], Rv[3], g,
  * c1 + c2, c1 c2, c1, c2, principal directions vec1[2], vec2[2] in (u, v) */
 int    orc_point_geometry(const omesh *m, const double *r, int f, double u, double v, double *out);
+int    orc_face_density(omesh *m, int f, const double *r, double *num, double *area);
+int    orc_particle_energy(const omesh *m, const double *r, int f, double u, double v, double pc0, double pa,
+                           double fed, double *out);
 
 #ifdef __cplusplus
 }

@@ -684,7 +684,7 @@ int face_energy_setup(umcp_surface *s)
     s->fe_items = all;
     int rc;
     if ((rc = dev_alloc(s, &s->d_fe_kinds, kinds.size())) || (rc = dev_alloc(s, &s->d_fe_items, all.size())) ||
-        (rc = dev_alloc(s, &s->d_fe_sub, (size_t)kFeSub)) || (rc = dev_alloc(s, &s->d_fe_out, 3 * all.size())) ||
+        (rc = dev_alloc(s, &s->d_fe_sub, (size_t)kFeSub)) || (rc = dev_alloc(s, &s->d_fe_out, 4 * all.size())) ||
         (rc = dev_upload(s, s->d_fe_kinds, kinds.data(), kinds.size())) ||
         (rc = dev_upload(s, s->d_fe_items, all.data(), all.size())))
         return rc;
@@ -693,6 +693,31 @@ int face_energy_setup(umcp_surface *s)
 }
 
 }  // namespace
+
+}  // extern "C"
+
+// the numerator of every face's curvature-energy density and its area
+// (sum dudv g0 w), global face order: penergy's face_energy_density and
+// face_area (src/interpLib.C:3612-3613, 3850-3851)
+int umcp::face_density(umcp_surface *s, const double *r, double *num, double *area0)
+{
+    int rc = check_ready(s);
+    if (rc) return rc;
+    if ((rc = face_energy_setup(s)) || (rc = upload_r(s, r)) || (rc = sync_face_table(s, s->stream))) return rc;
+    const int n = s->P.nfaces;
+    hipError_t e = launch_face_energies(s->d_fa_tab, s->d_fe_kinds, (int)s->P.classes.size(), s->d_r,
+                                        s->d_fe_items, n, s->d_fe_out, s->stream);
+    if (e != hipSuccess) return hip_fail(s, e, "face energy launch");
+    e = hipMemcpyAsync(num, s->d_fe_out + 3 * (size_t)n, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost,
+                       s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(area0, s->d_fe_out + 2 * (size_t)n, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost,
+                           s->stream);
+    if (e != hipSuccess) return hip_fail(s, e, "download face densities");
+    return sync(s);
+}
+
+extern "C" {
 
 int umcp_face_energies(umcp_surface *s, const double *r, double *E, double *A, double *A0)
 {

@@ -1474,7 +1474,7 @@ __global__ __launch_bounds__(kGatherBlock) void restraint_finalize_kernel(const 
 // surface::faceEnergy), one thread per listed face.
 template <int NCP, int NPT, int KIND>
 __device__ double face_energy_one(const FaceArgs &a, const double *__restrict__ r, int fc, double &area,
-                                  double &area0)
+                                  double &area0, double &dens)
 {
     const int nf = a.nf;
     const double al[3] = {r[3 * a.nv], r[3 * a.nv + 1], r[3 * a.nv + 2]};
@@ -1490,7 +1490,7 @@ __device__ double face_energy_one(const FaceArgs &a, const double *__restrict__ 
     const double *tab = kind_table<KIND, double>();
     const double *wts = tab + NPT * 5 * NCP;
     double e = 0.0;
-    area = area0 = 0.0;
+    area = area0 = dens = 0.0;
     for (int p = 0; p < NPT; p++) {
         const double *T = tab + p * 5 * NCP;
         double X[5][3];
@@ -1513,6 +1513,26 @@ __device__ double face_energy_one(const FaceArgs &a, const double *__restrict__ 
         const double E22 = X[1][0] * X[1][0] + X[1][1] * X[1][1] + X[1][2] * X[1][2];
         area += a.dudv * sqrt(E11 * E22 - E12 * E12) * wts[p];
         area0 += a.dudv * a.g0[p * nf + fc] * wts[p];
+        // the curvature-energy density penergy subtracts (src/interpLib.C:
+        // 3850 regular: 1/2 kc (c1 + c2 - c0)^2 in either form; 3612
+        // irregular: the point's energy density, kg c1 c2 in the lipid form)
+        // over dudv g0 w; c1 + c2 = -(L E22 - 2 M E12 + N E11) / det and
+        // c1 c2 = (L N - M^2) / det - SAFETY / 4 as the reference's roots give
+        const double det = E11 * E22 - E12 * E12;
+        const double nv[3] = {X[0][1] * X[1][2] - X[0][2] * X[1][1], X[0][2] * X[1][0] - X[0][0] * X[1][2],
+                              X[0][0] * X[1][1] - X[0][1] * X[1][0]};
+        const double iln = 1.0 / sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+        const double L = (nv[0] * X[2][0] + nv[1] * X[2][1] + nv[2] * X[2][2]) * iln;
+        const double M = (nv[0] * X[3][0] + nv[1] * X[3][1] + nv[2] * X[3][2]) * iln;
+        const double N = (nv[0] * X[4][0] + nv[1] * X[4][1] + nv[2] * X[4][2]) * iln;
+        const double H = -(L * E22 - 2.0 * M * E12 + N * E11) / det;
+        const double hd = H - a.c0[fc];
+        double x = 0.5 * a.ph.kc * hd * hd;
+        if (KIND != 0) {
+            const double K = (L * N - M * M) / det - 0.25e-12;
+            x = a.ph.mode == 0 ? a.ph.kg * K : x + a.ph.kg * K;
+        }
+        dens += a.dudv * a.g0[p * nf + fc] * wts[p] * x;
     }
     return e;
 }
@@ -1527,15 +1547,16 @@ __global__ __launch_bounds__(kMetricBlock) void face_energy_kernel(const FaceArg
     if (i >= n) return;
     const int2 it = items[i];
     const FaceArgs &a = fa[it.x];
-    double e, ar, ar0;
+    double e, ar, ar0, dn;
     switch (kinds[it.x]) {
-    case 0: e = face_energy_one<12, 15, 0>(a, r, it.y, ar, ar0); break;
-    case 1: e = face_energy_one<11, 16, 1>(a, r, it.y, ar, ar0); break;
-    default: e = face_energy_one<13, 16, 2>(a, r, it.y, ar, ar0); break;
+    case 0: e = face_energy_one<12, 15, 0>(a, r, it.y, ar, ar0, dn); break;
+    case 1: e = face_energy_one<11, 16, 1>(a, r, it.y, ar, ar0, dn); break;
+    default: e = face_energy_one<13, 16, 2>(a, r, it.y, ar, ar0, dn); break;
     }
     E[i] = e;
     E[n + i] = ar;
     E[2 * n + i] = ar0;
+    E[3 * n + i] = dn;
 }
 
 hipError_t launch_face_energies(const FaceArgs *fa, const int *kinds, int nclass, const double *r, const int2 *items,

@@ -223,8 +223,10 @@ hipError_t launch_point_reduce(const PointReduceArgs &a, hipStream_t st);
 
 // energies of listed faces (surface::faceEnergy, src/interpLib.C:3405-3424):
 // item i = (class, index in the class) of the face whose energy lands in E[i],
-// its area (sum dudv g w) in E[n + i] and reference area (sum dudv g0 w) in
-// E[2n + i]; fa: the surface's per-class face tables, r its positions
+// its area (sum dudv g w) in E[n + i], reference area (sum dudv g0 w) in
+// E[2n + i] and the numerator of its curvature-energy density (penergy's
+// face_energy_density) in E[3n + i]; fa: the surface's per-class face
+// tables, r its positions
 hipError_t launch_face_energies(const FaceArgs *fa, const int *kinds, int nclass, const double *r, const int2 *items,
                                 int n, double *E, hipStream_t st);
 

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <new>
 #include <string>
 #include <system_error>
 #include <thread>
@@ -296,10 +297,16 @@ int run_points(umcp_surface *s, const double *r, int n, const int *faces, const 
         std::vector<int> fcount;   // gradient: points per face in the chunk
     };
     std::vector<Chunk> ch((size_t)nt);
+    try {
+        // allocated here, not in the workers, where a failure could not be reported
+        if (grad)
+            for (Chunk &c : ch) c.fcount.assign((size_t)nfaces, 0);
+    } catch (const std::bad_alloc &) {
+        return pfail(s, UMCP_ERR_OOM, "host memory for the face sort");
+    }
     const std::vector<int> &fcls = s->pt_face_cls, &ffc = s->pt_face_fc;
     par_chunks(nt, (size_t)n, [&](int t, size_t lo, size_t hi) {
         Chunk &c = ch[(size_t)t];
-        if (grad) c.fcount.assign((size_t)nfaces, 0);
         for (size_t i = lo; i < hi; i++) {
             const int f = faces[i];
             if (f < 0 || f >= nfaces) {
@@ -564,6 +571,40 @@ int umcp_point_coeffs(umcp_surface *s, int n, const int *faces, const double *uv
         const int v = id[0], val = s->T.valence[v];
         for (int k = 0; k < val; k++) id[1 + k] = s->T.ring[(size_t)v * kMaxValence + k];
     }
+    return UMCP_OK;
+}
+
+int umcp_particle_energy(umcp_surface *s, const double *r, int n, const int *faces, const double *uv,
+                         const double *c0, const double *area, double *out)
+{
+    int rc = point_check(s, r, n, faces, uv);
+    if (rc) return rc;
+    if (n == 0) return UMCP_OK;
+    if (!c0 || !area || !out) return pfail(s, UMCP_ERR_ARG, "null particle arrays");
+    const int nf = s->P.nfaces;
+    std::vector<double> num((size_t)nf), a0((size_t)nf);
+    if ((rc = umcp::face_density(s, r, num.data(), a0.data()))) return rc;
+    Batch b;
+    rc = run_points(s, r, n, faces, uv, nullptr, nullptr, nullptr, 0, 0.0, 2, b);
+    if (rc) return rc;
+    const char *hst = nullptr;
+    rc = fetch_out(s, b.off_R, sizeof(double) * kPointGeom * (size_t)n, &hst, "download particle curvatures");
+    if (rc) return rc;
+    const double *G = (const double *)hst;
+    const double kc = s->prm.kc;
+    // the reference's order: en += 1/2 kc a (c1 + c2 - c0)^2; en -= a fed with
+    // fed = face_energy_density / face_area (src/interpLib.C:8580-8584, 3636)
+    par_chunks(host_threads((size_t)n), (size_t)n, [&](int, size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; i++) {
+            const double c1 = G[i * kPointGeom + 9], c2 = G[i * kPointGeom + 10];
+            const int f = faces[i];
+            const double fed = num[(size_t)f] / a0[(size_t)f];
+            double en = 0.0;
+            en += 0.5 * kc * area[i] * (c1 + c2 - c0[i]) * (c1 + c2 - c0[i]);
+            en -= area[i] * fed;
+            out[i] = en;
+        }
+    });
     return UMCP_OK;
 }
 

@@ -21,6 +21,13 @@ struct ClassDev {
 
 }  // namespace umcp
 
+struct umcp_surface;
+namespace umcp {
+// per-face curvature-energy density numerators and reference areas in global
+// face order, from the face-energy kernel (umcp_api.cpp)
+int face_density(umcp_surface *s, const double *r, double *num, double *area0);
+}  // namespace umcp
+
 struct umcp_surface {
     int device = 0;
     hipStream_t stream = nullptr;
