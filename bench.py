@@ -349,11 +349,11 @@ def run_sheet_leg(args, dev, torch):
 
 
 def run_points_leg(args, dev, torch):
-    """Points on the limit surface (evaluateRNRM, pointGradient) batched on
-    the 40962-vertex vesicle: n random points (faces uniform, (u, v) uniform
-    in the triangle), wall time of the host-array calls (staging, the
-    batch's vertex incidence, kernels, results), and the serial oracle on a
-    bounded sample."""
+    """Points on the limit surface (evaluateRNRM, pointGradient, the point
+    geometry, penergy's particle terms) batched on the 40962-vertex vesicle:
+    n random points (faces uniform, (u, v) uniform in the triangle), wall
+    time of the host-array calls (staging, the batch's vertex incidence,
+    kernels, results), and the serial oracle on a bounded sample."""
     import numpy as np
     from umcp_amd import surface as S
     from umcp_amd import workloads
@@ -395,7 +395,19 @@ def run_points_leg(args, dev, torch):
         Gm = s.point_geometry(r, faces, uv)
     out["geometry_s"] = (time.perf_counter() - t0) / reps
     out["geometry_points_per_s"] = n / out["geometry_s"]
-    out["finite"] = bool(np.isfinite(R).all() and np.isfinite(g).all() and np.isfinite(Gm).all())
+    # penergy's terms for n particles (c0 and area per particle), including
+    # the face densities of every face
+    s.setg0(w.r0)
+    pc0 = rng.uniform(-0.03, 0.03, n)
+    pa = rng.uniform(20.0, 80.0, n)
+    s.particle_energy(r, faces[:1000], uv[:1000], pc0[:1000], pa[:1000])
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        Ep = s.particle_energy(r, faces, uv, pc0, pa)
+    out["particle_energy_s"] = (time.perf_counter() - t0) / reps
+    out["particle_energy_per_s"] = n / out["particle_energy_s"]
+    out["finite"] = bool(np.isfinite(R).all() and np.isfinite(g).all() and np.isfinite(Gm).all()
+                         and np.isfinite(Ep).all())
     if not args.no_cpu_baseline:
         from oracle import oracle as O
         o = O.OracleSurface(w.mesh_path)
