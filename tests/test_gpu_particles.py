@@ -141,3 +141,10 @@ def test_threaded_batch_and_rejections():
         g.particle_energy(w.r, bad, uv, c0, area)
     # the surface still answers afterwards
     assert np.array_equal(g.particle_energy(w.r, faces[:10], uv[:10], c0[:10], area[:10]), E[:10])
+    # the densities need the reference state (setg0)
+    h = S.Surface(w.mesh_path)
+    h.set_params(precision=S.PREC_FP64, **w.params)
+    h.generatePlan()
+    with pytest.raises(S.UmcpError, match="setg0"):
+        h.particle_energy(w.r, faces[:10], uv[:10], c0[:10], area[:10])
+    h.close()
