@@ -160,6 +160,27 @@ def test_grad_accumulates_and_is_deterministic(ves642):
     assert g.energy(r) == E1
 
 
+def test_nan_energy_is_an_error(ves642):
+    """A NaN position makes the energy NaN; the reference prints "nan en." and
+    exits (src/interpLib.C:3546-3550).  Here the energy and gradient entry points return
+    UMCP_ERR_NAN_ENERGY, leave the caller's gradient untouched, and the handle
+    evaluates finite positions afterwards exactly as before."""
+    g, o = make_pair(ves642)
+    r = ves642.r
+    E1, g1 = g.energy_grad(r)
+    bad = r.copy()
+    bad[3 * 17 + 1] = np.nan
+    say("[parity] nan position at vertex 17")
+    with pytest.raises(S.UmcpError, match=r"\(-2\).*nan energy"):
+        g.energy(bad)
+    acc = np.full_like(r, 2.5)
+    with pytest.raises(S.UmcpError, match=r"\(-2\)"):
+        g.grad(bad, acc)
+    assert np.all(acc == 2.5)
+    E2, g2 = g.energy_grad(r)
+    assert E2 == E1 and np.array_equal(g2, g1)
+
+
 def test_area(ves642):
     g, o = make_pair(ves642)
     A, A0 = g.area(ves642.r)
