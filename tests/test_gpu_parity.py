@@ -202,6 +202,24 @@ def test_bd_trajectory(ves642):
     assert np.abs(e_g / e_o - 1).max() <= 1e-11
 
 
+def test_bd_nan_energy_is_an_error(ves642):
+    """Brownian steps that report energies stop with UMCP_ERR_NAN_ENERGY once a
+    step's energy is NaN (the reference's src/interpLib.C:3546-3550 check, met
+    through the energy it evaluates every step); the same handle then steps
+    finite positions exactly as a fresh run does."""
+    g, o = make_pair(ves642)
+    mob, amp = S.bd_coefficients(1e-12, 1e-9, 298.0)
+    xi = np.random.default_rng(7).standard_normal((4, 3 * g.nv))
+    r1, e1 = g.bd_steps(ves642.r, xi, mob, amp, want_energies=True)
+    bad = ves642.r.copy()
+    bad[3 * 300 + 2] = np.nan
+    say("[parity] bd with a nan position at vertex 300")
+    with pytest.raises(S.UmcpError, match=r"\(-2\).*nan energy"):
+        g.bd_steps(bad, xi, mob, amp, want_energies=True)
+    r2, e2 = g.bd_steps(ves642.r, xi, mob, amp, want_energies=True)
+    assert np.array_equal(r2, r1) and np.array_equal(e2, e1)
+
+
 def test_device_entry_points(ves642):
     if torch is None:
         pytest.skip("torch is needed for device buffers")
