@@ -9,6 +9,7 @@ device budget that holds only a few requests still serves them all without
 exceeding it; a malformed request fails alone.
 """
 import sys
+import time
 
 import numpy as np
 import pytest
@@ -91,6 +92,25 @@ def test_budget_admission(reqs):
     assert st["done"] == len(reqs)
     assert st["peak_bytes"] <= budget
     del one
+
+
+def test_destroy_right_after_a_failing_request(reqs, tmp_path):
+    """A request whose preparation fails (missing mesh file, positions of
+    another mesh) while the server is being destroyed: destroy returns
+    promptly instead of leaving the worker waiting."""
+    say("[serving] destroy after failing requests")
+    q = reqs[0]
+    other = next(x for x in reqs if x.nv != q.nv)
+    mob, amp = S.bd_coefficients(1e-12, q.dt, q.T)
+    for case in range(3):
+        sv = S.Server(precision=S.PREC_FP64, max_active=2, chunk_steps=20)
+        if case != 1:
+            sv.submit(str(tmp_path / "missing.mesh"), q.nv, q.params, q.c0, 10, 1, mob, amp, q.r0, q.r)
+        if case != 0:
+            sv.submit(q.mesh_path, other.nv, q.params, q.c0, 10, 1, mob, amp, other.r0, other.r)
+        t0 = time.time()
+        sv.close()
+        assert time.time() - t0 < 30.0
 
 
 def test_bad_request_fails_alone(reqs):

@@ -131,6 +131,9 @@ void finish_failed(umcp_server *sv, Request *q, int code, const std::string &msg
     sv->n_done++;
     sv->cv_done.notify_all();
     sv->cv_prep.notify_all();
+    // a failed preparation lowers `preparing`, which the worker's exit
+    // condition at shutdown waits on
+    sv->cv_work.notify_all();
 }
 
 // device set-up of one request (prep thread); footprint already reserved
@@ -471,6 +474,11 @@ void umcp_server_destroy(umcp_server *sv)
     }
     for (std::thread &t : sv->prep)
         if (t.joinable()) t.join();
+    {
+        // no preparation is left: wake the worker to check its exit condition
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->cv_work.notify_all();
+    }
     if (sv->worker.joinable()) sv->worker.join();
     umcp_ensemble_destroy(sv->ens);
     for (auto &kv : sv->templates) umcp_destroy(kv.second);
