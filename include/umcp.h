@@ -36,6 +36,16 @@ extern "C" {
 #define UMCP_PREC_FP64 0
 #define UMCP_PREC_MIXED 1     /* FP32 per-face arithmetic, FP64 reductions and positions */
 
+/* gradient of the Helfrich form (UMCP_MODE_HELFRICH; the lipid form has one
+ * gradient).  UMCP_GRAD_REFERENCE (default) is what surface::grad computes:
+ * on regular faces dE/dg also carries 1/2 kc (c1+c2-c0)^2 dudv w
+ * (src/grad.C:448) although the energy's area element is the fixed g0;
+ * irregular faces carry no such term (igrad keeps it under #ifndef FIXED_A,
+ * src/grad.C:6822-6824).  UMCP_GRAD_EXACT is the exact derivative of the
+ * energy instead. */
+#define UMCP_GRAD_REFERENCE 0
+#define UMCP_GRAD_EXACT 1
+
 typedef struct umcp_surface umcp_surface;
 
 /* Physical constants of one surface; replaces the process globals kc, kg,
@@ -51,6 +61,7 @@ typedef struct {
     int precision;
     double lipid_c0;
     double lipid_apl;
+    int grad_form;         /* UMCP_GRAD_REFERENCE (0) or UMCP_GRAD_EXACT */
 } umcp_params;
 
 /* Per-vertex overdamped Brownian step, r <- r - mobility_dt * dE/dr + noise_amp * xi

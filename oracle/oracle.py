@@ -17,12 +17,14 @@ _LIB = None
 
 MODE_LIPID = 0   # FIXED_A + LOCAL_LIPID_ENERGY (reference default)
 MODE_HELFRICH = 1
+GRAD_REFERENCE = 0   # surface::grad's Helfrich gradient (src/grad.C:448)
+GRAD_EXACT = 1       # the exact derivative of the Helfrich energy
 
 
 class OParams(C.Structure):
     _fields_ = [("kc", C.c_double), ("kg", C.c_double), ("KA", C.c_double),
                 ("k_reg", C.c_double), ("mode", C.c_int), ("lipid_c0", C.c_double),
-                ("lipid_apl", C.c_double)]
+                ("lipid_apl", C.c_double), ("grad_form", C.c_int)]
 
 
 def build() -> str:
@@ -54,6 +56,8 @@ def lib():
         L.orc_face_energy.argtypes = [vp, C.c_int, dp]
         L.orc_face_energy.restype = C.c_double
         L.orc_grad.argtypes = [vp, dp, dp]
+        L.orc_frozen_curvature_area.argtypes = [vp, dp, dp]
+        L.orc_frozen_curvature_area.restype = C.c_double
         L.orc_area.argtypes = [vp, dp, dp, dp]
         L.orc_bd_steps.argtypes = [vp, dp, dp, C.c_int, C.c_double, C.c_double, dp]
         L.orc_face_info.argtypes = [vp, C.c_int, ip, ip, ip, ip]
@@ -136,9 +140,15 @@ class OracleSurface:
         return r
 
     def set_params(self, kc=14.0, kg=0.0, KA=0.0, k_reg=0.0, mode=MODE_LIPID,
-                   lipid_c0=0.0, lipid_apl=68.3):
-        p = OParams(kc, kg, KA, k_reg, mode, lipid_c0, lipid_apl)
+                   lipid_c0=0.0, lipid_apl=68.3, grad_form=GRAD_REFERENCE):
+        p = OParams(kc, kg, KA, k_reg, mode, lipid_c0, lipid_apl, grad_form)
         lib().orc_set_params(self._h, C.byref(p))
+
+    def frozen_curvature_area(self, r_star: np.ndarray, r: np.ndarray) -> float:
+        """sum over regular-face Gauss points of 1/2 kc (H(r_star) - c0)^2 dudv w g(r)"""
+        r_star = np.ascontiguousarray(r_star, dtype=np.float64)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        return lib().orc_frozen_curvature_area(self._h, _p(r_star), _p(r))
 
     def set_face_c0(self, c0: np.ndarray):
         c0 = np.ascontiguousarray(c0, dtype=np.float64)

@@ -1490,6 +1490,46 @@ double orc_energy(omesh *m, const double *r)
     return e + orc_restraint_energy(m, r, NULL, NULL);
 }
 
+/* sum over the regular faces' Gauss points of 1/2 kc (c1+c2-c0)^2 dudv w g(r)
+ * with c1 + c2 from r_star: its gradient at r = r_star is the dE/dg term
+ * that surface::grad adds in the Helfrich form (src/grad.C:448).  Used only
+ * to pin the gradient by finite differences in r with the curvature frozen. */
+double orc_frozen_curvature_area(omesh *m, const double *r_star, const double *r)
+{
+    const oparams *P = &m->prm;
+    double s = 0.0;
+    for (int f = 0; f < m->nfr; f++) {
+        int npt;
+        double dudv;
+        oform *F = face_forms(m, f, &npt, &dudv);
+        const double c0 = m->face_c0[f];
+        for (int p = 0; p < npt; p++) {
+            double X[6][3], Y[6][3];
+            contract(m, F + p, r_star, X);
+            contract(m, F + p, r, Y);
+            double *Ru = X[1], *Rv = X[2], *Suu = X[3], *Suv = X[4], *Svv = X[5];
+            double nrm[3];
+            cross3(Ru, Rv, nrm);
+            normalize3(nrm);
+            double RuRu = Ru[0] * Ru[0] + Ru[1] * Ru[1] + Ru[2] * Ru[2];
+            double RuRv = Ru[0] * Rv[0] + Ru[1] * Rv[1] + Ru[2] * Rv[2];
+            double RvRv = Rv[0] * Rv[0] + Rv[1] * Rv[1] + Rv[2] * Rv[2];
+            double det = RuRu * RvRv - RuRv * RuRv;
+            double nsuu = Suu[0] * nrm[0] + Suu[1] * nrm[1] + Suu[2] * nrm[2];
+            double nsuv = Suv[0] * nrm[0] + Suv[1] * nrm[1] + Suv[2] * nrm[2];
+            double nsvv = Svv[0] * nrm[0] + Svv[1] * nrm[1] + Svv[2] * nrm[2];
+            /* c1 + c2 = -tr(I^-1 II) */
+            double H = -(nsuu * RvRv - 2.0 * nsuv * RuRv + nsvv * RuRu) / det;
+            double *Su = Y[1], *Sv = Y[2];
+            double n[3];
+            cross3(Su, Sv, n);
+            double g = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            s += 0.5 * P->kc * (H - c0) * (H - c0) * dudv * F[p].weight * g;
+        }
+    }
+    return s;
+}
+
 /* one Gauss point of grad/igrad (src/grad.C:246-930, 6605-7450), through the
  * reference's graph: E(c1, c2, g) -> (a,b,c,d,g) -> (nsuu,nsuv,nsvv,RuRu,RuRv,
  * RvRv) -> (Ru, Rv, Suu, Suv, Svv) -> control points */
@@ -1539,6 +1579,15 @@ static void point_grad(omesh *m, int f, const oform *F, double dudv, const doubl
     }
     de_c1 += P->kg * e2 * dudv * dAf;
     de_c2 += P->kg * e1 * dudv * dAf;
+    /* Helfrich form, regular faces: surface::grad adds the area element's
+     * derivative of 1/2 kc (c1+c2-c0)^2 dudv g w although the energy holds g0
+     * fixed (the #else of LOCAL_LIPID_ENERGY, src/grad.C:448, outside its
+     * #ifndef FIXED_A); igrad keeps the same line inside #ifndef FIXED_A
+     * (src/grad.C:6822-6824), so irregular faces do not add it */
+    if (P->mode != 0 && f < m->nfr && P->grad_form == 0) {
+        double c0 = m->face_c0[f];
+        de_g += (0.5 * P->kc * (e1 + e2 - c0) * (e1 + e2 - c0)) * dudv * w;
+    }
     de_g += 2 * P->KA * ((A - A0) / A0) * w * dudv;
     /* global area restraint: dE/dA_tot * dA_tot/dg, A_tot = sum dudv g w */
     de_g += lamA * w * dudv;

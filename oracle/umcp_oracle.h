@@ -22,11 +22,15 @@ typedef struct omesh omesh;
 
 /* energy modes: 0 = reference default (FIXED_A + LOCAL_LIPID_ENERGY,
  * src/local_config.h:21-22); 1 = per-face c0 Helfrich with g0 area element
- * (the #else branches of src/interpLib.C:3766-3844). */
+ * (the #else branches of src/interpLib.C:3766-3844).
+ * grad_form (mode 1 only): 0 = surface::grad's gradient, whose regular faces
+ * add 1/2 kc (c1+c2-c0)^2 dudv w to dE/dg (src/grad.C:448; igrad does not,
+ * src/grad.C:6822-6824); 1 = the exact derivative of the energy. */
 typedef struct {
     double kc, kg, KA, k_reg;
     int mode;
     double lipid_c0, lipid_apl;   /* single lipid type, default "ZERO": 0, 68.3 */
+    int grad_form;
 } oparams;
 
 int    orc_load(const char *path, omesh **out);        /* loadLattice + orientSurface */
@@ -53,6 +57,13 @@ void   orc_limit_points(const omesh *m, const double *r, double *L);   /* 3 nv *
 double orc_volume(const omesh *m, const double *r);
 void   orc_volume_grad(const omesh *m, const double *r, double *dV);  /* 3 nv */
 double orc_restraint_energy(omesh *m, const double *r, double *lamA, double *lamV);
+
+/* sum over the Gauss points of the regular faces of
+ * 1/2 kc (c1 + c2 - c0)^2 dudv w g(r), with c1 + c2 evaluated at r_star and
+ * g(r) the area element at r (Helfrich form): the function whose gradient in
+ * r at r = r_star is what surface::grad's dE/dg term of src/grad.C:448 adds
+ * to the exact gradient */
+double orc_frozen_curvature_area(omesh *m, const double *r_star, const double *r);
 
 /* effective-mass integrators of src/hd.C (Cartesian momenta): the point
  * mass of the ppf lattice points per face, the dense mass matrix M (nv x nv),

@@ -87,26 +87,54 @@ def test_launch_shapes(ves642):
             check_eval(g, o, ves642.r, "vesicle_642 split %d flags %d" % (split, flags))
 
 
-def test_all_terms_helfrich():
+GRAD_FORMS = [S.GRAD_REFERENCE, S.GRAD_EXACT]
+
+
+@pytest.mark.parametrize("grad_form", GRAD_FORMS)
+def test_all_terms_helfrich(grad_form):
+    """Every term of the Helfrich form, in both gradient forms: surface::grad's
+    (src/grad.C:448) and the exact derivative."""
     w = workloads.vesicle(4, 300.0, 1.5, seed=21, name="vesicle_2562_terms")
-    g, o = make_pair(w, params=dict(kc=6.0, kg=-2.0, KA=0.5, k_reg=1.5, mode=1))
+    g, o = make_pair(w, params=dict(kc=6.0, kg=-2.0, KA=0.5, k_reg=1.5, mode=1, grad_form=grad_form))
     c0 = workloads.face_c0_uniform(g.nfaces, 4, -0.02, 0.02)
     g.set_face_c0(c0)
     o.set_face_c0(c0)
-    check_eval(g, o, w.r, "vesicle_2562 helfrich kg kreg KA")
+    check_eval(g, o, w.r, "vesicle_2562 helfrich kg kreg KA grad_form %d" % grad_form)
 
 
-@pytest.mark.parametrize("form", ["lipid", "helfrich", "sheet"])
+def test_gradient_forms_differ():
+    """The two Helfrich gradient forms give the same energy and gradients
+    that differ by surface::grad's dE/dg term, by as much on the GPU as in the
+    oracle."""
+    w = workloads.vesicle(4, 300.0, 1.5, seed=21, name="vesicle_2562_terms")
+    out = {}
+    for form in GRAD_FORMS:
+        g, o = make_pair(w, params=dict(kc=6.0, mode=1, grad_form=form))
+        c0 = workloads.face_c0_uniform(g.nfaces, 4, -0.02, 0.02)
+        g.set_face_c0(c0)
+        o.set_face_c0(c0)
+        out[form] = (g.energy_grad(w.r), o.grad(w.r))
+    (E_r, g_r), o_r = out[S.GRAD_REFERENCE]
+    (E_x, g_x), o_x = out[S.GRAD_EXACT]
+    assert E_r == E_x
+    d_g, d_o = g_r - g_x, o_r - o_x
+    assert np.linalg.norm(d_o) > 1e-3 * np.linalg.norm(o_x)
+    assert np.linalg.norm(d_g - d_o) <= 1e-11 * np.linalg.norm(d_o)
+
+
+@pytest.mark.parametrize("form", ["lipid", "helfrich", "helfrich_exact", "sheet"])
 def test_valence_5_and_7(form):
     """A sphere with valence-5 and valence-7 vertices (four edge flips on an
     icosphere, one Loop step): the valence-7 face class against the oracle,
-    in every launch shape, with both restraints in the Helfrich form; and a
-    periodic sheet whose irregular faces reach across the box."""
+    in every launch shape, with both restraints in the Helfrich form (both
+    gradient forms); and a periodic sheet whose irregular faces reach across
+    the box."""
     w = workloads.defect_sheet() if form == "sheet" else workloads.defect_vesicle()
     if form in ("lipid", "sheet"):
         g, o = make_pair(w)
     else:
-        g, o = make_pair(w, params=dict(kc=6.0, kg=-2.0, KA=0.5, k_reg=1.5, mode=1))
+        gf = S.GRAD_EXACT if form == "helfrich_exact" else S.GRAD_REFERENCE
+        g, o = make_pair(w, params=dict(kc=6.0, kg=-2.0, KA=0.5, k_reg=1.5, mode=1, grad_form=gf))
         c0 = workloads.face_c0_uniform(g.nfaces, 6, -0.02, 0.02)
         g.set_face_c0(c0)
         o.set_face_c0(c0)
@@ -133,9 +161,10 @@ def test_sheet_128(tmp_path):
     check_eval(g, o, w.r, "sheet_128")
 
 
-def test_vesicle_40962_full_size():
+@pytest.mark.parametrize("grad_form", GRAD_FORMS)
+def test_vesicle_40962_full_size(grad_form):
     w = workloads.vesicle_40962()
-    g, o = make_pair(w, params=dict(mode=1))
+    g, o = make_pair(w, params=dict(mode=1, grad_form=grad_form))
     c0 = workloads.face_c0_uniform(g.nfaces, w.extra["c0_seed"])
     g.set_face_c0(c0)
     o.set_face_c0(c0)
@@ -145,7 +174,8 @@ def test_vesicle_40962_full_size():
     # 2 KA_g (A - A0)/A0.  A - A0 is ~4e-6 of A here, so the ~1e-13 relative
     # rounding of the two 1.2M-point area sums (GPU tree order vs the serial
     # oracle) becomes ~1e-8 in lamA: the alpha bound is that conditioning.
-    check_eval(g, o, w.r, "vesicle_40962 helfrich c0, area and volume restraints", alpha_tol=2e-8)
+    check_eval(g, o, w.r, "vesicle_40962 helfrich c0, area and volume restraints, grad_form %d" % grad_form,
+               alpha_tol=2e-8)
 
 
 def test_grad_accumulates_and_is_deterministic(ves642):

@@ -4,7 +4,11 @@ Checks that a dropped term, a wrong sign/index or a transposed operand would
 fail:
 * central finite differences with Richardson extrapolation (the reference's
   own check is src/grad.C:25-63) for every energy term at once (mode A with
-  KA, mode B with per-face c0, kg and k_reg), including the alpha gradient;
+  KA, mode B with per-face c0, kg and k_reg, the exact gradient form),
+  including the alpha gradient;
+* mode B's reference gradient form: its difference from the exact form is
+  the finite-difference gradient of the area elements weighted by the
+  curvature energy frozen at the point (src/grad.C:448, 6822-6824);
 * translation invariance (sum of vertex gradients = 0) and rotation
   invariance (E unchanged, zero net torque) on a closed vesicle;
 * closed forms: flat periodic sheet E = 0, g = 0; the area term vanishes at
@@ -57,13 +61,79 @@ def test_fd_gradient_mode_lipid(ves):
 def test_fd_gradient_all_terms_helfrich(mesh_dir):
     w = workloads.vesicle(2, 60.0, 0.8, seed=11, name="ves162_fd")
     s = O.OracleSurface(w.mesh_path)
-    s.set_params(kc=5.0, kg=-1.3, KA=0.4, k_reg=2.0, mode=O.MODE_HELFRICH)
+    s.set_params(kc=5.0, kg=-1.3, KA=0.4, k_reg=2.0, mode=O.MODE_HELFRICH, grad_form=O.GRAD_EXACT)
     s.set_face_c0(np.random.default_rng(3).uniform(-0.02, 0.02, s.nfaces))
     s.setg0(w.r0)
     g = s.grad(w.r)
     scale = np.abs(g).max()
     for i in range(0, 3 * s.nv, 7):
         assert abs(_rich(s, w.r, i, 1e-3) - g[i]) < 2e-7 * scale, i
+
+
+def _rich_frozen(s, r_star, i, h):
+    def fd(hh):
+        rp, rm = r_star.copy(), r_star.copy()
+        rp[i] += hh
+        rm[i] -= hh
+        return (s.frozen_curvature_area(r_star, rp) - s.frozen_curvature_area(r_star, rm)) / (2 * hh)
+    return (4 * fd(h / 2) - fd(h)) / 3
+
+
+def _ref_form_cases():
+    return [("ves162", lambda: workloads.vesicle(2, 60.0, 0.8, seed=11, name="ves162_fd")),
+            ("defect", workloads.defect_vesicle),
+            ("defect_sheet", workloads.defect_sheet)]
+
+
+@pytest.mark.parametrize("case", [c[0] for c in _ref_form_cases()])
+def test_reference_helfrich_gradient_term(mesh_dir, case):
+    """surface::grad's Helfrich gradient adds 1/2 kc (c1+c2-c0)^2 dudv w to
+    dE/dg on regular faces (src/grad.C:448) and nothing on irregular faces
+    (src/grad.C:6822-6824).  So grad_reference - grad_exact must be the
+    gradient in r, at r = r*, of sum_p 1/2 kc (H(r*) - c0)^2 dudv w g(r) over
+    the regular faces' points with the curvature frozen at r*: checked by
+    Richardson differences at every coordinate of the control points around
+    an extraordinary vertex (where regular and irregular faces meet) and at a
+    random sample, alpha included.  Adding the term on irregular faces, using
+    g0 for g, dropping dudv or w, or freezing the wrong curvature fails it."""
+    w = dict(_ref_form_cases())[case]()
+    s = O.OracleSurface(w.mesh_path)
+    prm = dict(kc=5.0, kg=-1.3, KA=0.4, k_reg=2.0, mode=O.MODE_HELFRICH)
+    s.set_params(**prm, grad_form=O.GRAD_EXACT)
+    s.set_face_c0(np.random.default_rng(3).uniform(-0.02, 0.02, s.nfaces))
+    s.setg0(w.r0)
+    r = w.r.copy()
+    r[-3:] = [1.02, 0.99, 1.01]
+    g_exact = s.grad(r)
+    s.set_params(**prm, grad_form=O.GRAD_REFERENCE)
+    g_ref = s.grad(r)
+    diff = g_ref - g_exact
+    # the term is not small: same order as the bending gradient itself
+    assert np.abs(diff[:-3]).max() > 1e-3 * np.abs(g_exact[:-3]).max()
+    irregular = [v for v in range(s.nv) if len(s.neighbors(v)) != 6]
+    assert irregular
+    idx = set()
+    for v in irregular[:2]:
+        for u in [v] + s.neighbors(v):
+            idx.update(3 * u + k for k in range(3))
+    rng = np.random.default_rng(1)
+    idx.update(int(i) for i in rng.integers(0, 3 * s.nv, 24))
+    idx.update([3 * s.nv, 3 * s.nv + 1, 3 * s.nv + 2])
+    scale = np.abs(diff).max()
+    for i in sorted(idx):
+        h = 2e-3 if i < 3 * s.nv else 1e-4
+        fd = _rich_frozen(s, r, i, h)
+        assert abs(fd - diff[i]) < 1e-8 * scale + 1e-8 * abs(diff[i]), (case, i, fd, diff[i])
+
+
+def test_grad_form_does_not_touch_lipid_mode(ves):
+    """The lipid form (FIXED_A + LOCAL_LIPID_ENERGY) has one gradient."""
+    w, s = ves
+    s.set_params(**w.params, grad_form=O.GRAD_REFERENCE)
+    s.setg0(w.r0)
+    g0 = s.grad(w.r)
+    s.set_params(**w.params, grad_form=O.GRAD_EXACT)
+    assert np.array_equal(s.grad(w.r), g0)
 
 
 @pytest.mark.parametrize("mode", [O.MODE_LIPID, O.MODE_HELFRICH])
@@ -74,7 +144,7 @@ def test_fd_gradient_valence_7(mesh_dir, mode):
     w = workloads.defect_vesicle()
     s = O.OracleSurface(w.mesh_path)
     if mode == O.MODE_HELFRICH:
-        s.set_params(kc=5.0, kg=-1.3, KA=0.4, k_reg=2.0, mode=mode)
+        s.set_params(kc=5.0, kg=-1.3, KA=0.4, k_reg=2.0, mode=mode, grad_form=O.GRAD_EXACT)
         s.set_face_c0(np.random.default_rng(5).uniform(-0.02, 0.02, s.nfaces))
     else:
         s.set_params(**w.params)

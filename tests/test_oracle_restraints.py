@@ -139,7 +139,7 @@ def test_restraint_energy_closed_form(ves):
 def test_fd_gradient_with_restraints(mesh_dir):
     w = workloads.vesicle(2, 60.0, 0.8, seed=5, name="ves162_rs")
     s = O.OracleSurface(w.mesh_path)
-    s.set_params(kc=5.0, kg=0.0, KA=0.2, k_reg=0.0, mode=O.MODE_HELFRICH)
+    s.set_params(kc=5.0, kg=0.0, KA=0.2, k_reg=0.0, mode=O.MODE_HELFRICH, grad_form=O.GRAD_EXACT)
     s.set_face_c0(np.linspace(-0.02, 0.02, s.nfaces))
     s.setg0(w.r0)
     A, _ = s.area(w.r)

@@ -8,6 +8,7 @@ that it does not know the feature, which is harmless.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -17,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libumcp_core.so")
-SOURCES = ["umcp_kernels.hip", "surface_host.cpp", "effm_host.cpp", "umcp_api.cpp", "umcp_ensemble.cpp",
+SOURCES = ["umcp_kernels.hip", "umcp_faces64.hip", "umcp_faces32.hip", "umcp_faces_mx.hip", "surface_host.cpp", "effm_host.cpp", "umcp_api.cpp", "umcp_ensemble.cpp",
            "umcp_server.cpp", "umcp_io.cpp", "umcp_points.cpp"]
 ARCH = os.environ.get("UMCP_OFFLOAD_ARCH", "gfx950")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
@@ -32,6 +33,7 @@ def hipcc() -> str:
 
 def build(verbose: bool = False) -> str:
     objs = []
+    cmds = []
     build_dir = os.path.join(HERE, "_build")
     os.makedirs(build_dir, exist_ok=True)
     common = ["-O3", "-fPIC", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
@@ -48,8 +50,13 @@ def build(verbose: bool = False) -> str:
                 "-pthread", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include")]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append(cmd)
         objs.append(obj)
+    # the translation units are independent: compile them side by side
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "8"))))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
+        for f in [ex.submit(subprocess.run, c, check=True) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     # rocSOLVER (potrf + potri of the effective-mass set-up) and rocBLAS
     libs = ["-L" + os.path.join(ROCM, "lib"), "-lrocsolver", "-lrocblas",

@@ -110,6 +110,7 @@ PhysParams phys(const umcp_surface *s)
     ph.k_reg = s->prm.k_reg;
     ph.lipid_c0 = s->prm.lipid_c0;
     ph.mode = s->prm.mode;
+    ph.grad_exact = s->prm.grad_form == UMCP_GRAD_EXACT;
     return ph;
 }
 
@@ -337,6 +338,7 @@ int umcp_load_mesh(const char *path, umcp_surface **out)
     s->prm.precision = UMCP_PREC_FP64;
     s->prm.lipid_c0 = 0.0;
     s->prm.lipid_apl = 68.3;
+    s->prm.grad_form = UMCP_GRAD_REFERENCE;
     *out = s;
     return UMCP_OK;
 }
@@ -382,6 +384,8 @@ int umcp_set_params(umcp_surface *s, const umcp_params *p)
         return fail(s, UMCP_ERR_ARG, "unknown energy mode");
     if (p->precision != UMCP_PREC_FP64 && p->precision != UMCP_PREC_MIXED)
         return fail(s, UMCP_ERR_ARG, "unknown precision");
+    if (p->grad_form != UMCP_GRAD_REFERENCE && p->grad_form != UMCP_GRAD_EXACT)
+        return fail(s, UMCP_ERR_ARG, "unknown gradient form");
     if (!(p->lipid_apl > 0)) return fail(s, UMCP_ERR_ARG, "lipid area must be positive");
     bool repack = s->g0_set && p->lipid_apl != s->prm.lipid_apl;
     s->prm = *p;

@@ -1,843 +1,30 @@
-// HIP kernels for gfx950 (MI355X): fused limit-surface energy + gradient per
-// face class, the deterministic vertex gather (which also finishes the global
-// reductions and can apply the Brownian update), and the metric kernel used
-// by setg0 and area.
+// HIP kernels for gfx950 (MI355X) besides the face kernels: the
+// deterministic vertex gather (which also finishes the global reductions and
+// can apply the Brownian update), the metric kernel used by setg0 and area,
+// the restraint pre-pass, single-face energies, the effective-mass kernels,
+// device noise and the points on the limit surface; and the launchers.  The
+// fused face kernels live in umcp_faces64.hip, umcp_faces32.hip and
+// umcp_faces_mx.hip (device code shared through face_device.hpp).
 //
 // Reference computations, alexsodt/UMCP:
-//   per-point energy      src/interpLib.C:3641-3881 (fenergy), 3426-3639 (ifenergy)
-//   per-point gradient    src/grad.C:246-930 (grad), 6605-7450 (igrad)
 //   reference metric      src/interpLib.C:2969-3111 (setg0)
 //   area                  src/interpLib.C:4355-4611
 //   Brownian step         docs/timescales.tex:59-67, src/bd.C:1035-1036
-#include <hip/hip_runtime.h>
-
-#include <cstdint>
-
-#include "umcp_kernels.hpp"
-#include "box_spline.hpp"
+#include "face_device.hpp"
 
 namespace umcp {
 
-// The Gauss-point tables depend only on the valence of the face's owner, so
-// every surface on the device shares one copy in constant memory; wave-
-// uniform reads from it become scalar loads whose values feed the FMAs as
-// SGPR operands.  Layout per kind: [npt][ncp][5] (the five derivative rows
-// Ru, Rv, Ruu, Ruv, Rvv of each control point contiguous, so one scalar load
-// brings a control point's coefficients) then [npt] weights.  The gradient
-// contraction reads a second copy, so the compiler cannot keep the forward
-// pass's coefficients alive in SGPRs across the pointwise math.
-constexpr int kTabReg = 15 * 5 * 12, kTabV5 = 16 * 5 * 11, kTabV7 = 16 * 5 * 13;
-__constant__ double c_tab_reg[kTabReg + 15];
-__constant__ double c_tab_v5[kTabV5 + 16];
-__constant__ double c_tab_v7[kTabV7 + 16];
-__constant__ double c_tab_reg_b[kTabReg];
-__constant__ double c_tab_v5_b[kTabV5];
-__constant__ double c_tab_v7_b[kTabV7];
-// FP32 copies for the mixed path
-__constant__ float c_tab32_reg[kTabReg + 15];
-__constant__ float c_tab32_v5[kTabV5 + 16];
-__constant__ float c_tab32_v7[kTabV7 + 16];
-__constant__ float c_tab32_reg_b[kTabReg];
-__constant__ float c_tab32_v5_b[kTabV5];
-__constant__ float c_tab32_v7_b[kTabV7];
-
-// position weights of the three corners of a face (the limit points of its
-// vertices i, ring[e], ring[e+1]): [3][ncp] per kind
-__constant__ double c_corner_reg[3 * 12];
-__constant__ double c_corner_v5[3 * 11];
-__constant__ double c_corner_v7[3 * 13];
-
-template <int KIND>
-__device__ __forceinline__ const double *kind_corner()
-{
-    return KIND == 0 ? c_corner_reg : (KIND == 1 ? c_corner_v5 : c_corner_v7);
-}
-
-template <int KIND, typename R>
-__device__ __forceinline__ const R *kind_table()
-{
-    if constexpr (sizeof(R) == 8)
-        return (const R *)(KIND == 0 ? c_tab_reg : (KIND == 1 ? c_tab_v5 : c_tab_v7));
-    else
-        return (const R *)(KIND == 0 ? c_tab32_reg : (KIND == 1 ? c_tab32_v5 : c_tab32_v7));
-}
-template <int KIND, typename R>
-__device__ __forceinline__ const R *kind_table_b()
-{
-    if constexpr (sizeof(R) == 8)
-        return (const R *)(KIND == 0 ? c_tab_reg_b : (KIND == 1 ? c_tab_v5_b : c_tab_v7_b));
-    else
-        return (const R *)(KIND == 0 ? c_tab32_reg_b : (KIND == 1 ? c_tab32_v5_b : c_tab32_v7_b));
-}
-
-// Workgroups are dispatched round-robin over the 8 XCDs; give every XCD a
-// contiguous run of logical blocks so neighbouring faces (which share
-// vertices) hit the same L2.
-__device__ __forceinline__ int xcd_logical_block(int bid, int nb)
-{
-    constexpr int kXcd = 8;
-    int q = nb / kXcd, rem = nb % kXcd;
-    int x = bid % kXcd, i = bid / kXcd;
-    int start = (x < rem) ? x * (q + 1) : rem * (q + 1) + (x - rem) * q;
-    return start + i;
-}
-
-// Deterministic reduction of four doubles over a block of kBlock threads;
-// thread 0 receives the sums.
-template <int kBlock>
-__device__ __forceinline__ void block_sum4(double v[4])
-{
-    __shared__ double red[kBlock / 64][4];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-        for (int i = 0; i < 4; i++) v[i] += __shfl_down(v[i], off, 64);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0)
-#pragma unroll
-        for (int i = 0; i < 4; i++) red[wave][i] = v[i];
-    __syncthreads();
-    if (threadIdx.x == 0)
-        for (int w = 1; w < kBlock / 64; w++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) v[i] += red[w][i];
-}
-
-__device__ __forceinline__ void image_offset(int code, const double pbc[3][3], double out[3])
-{
-    const int ix = code / 9 - 1, iy = (code / 3) % 3 - 1, iz = code % 3 - 1;
-#pragma unroll
-    for (int k = 0; k < 3; k++) out[k] = ix * pbc[0][k] + iy * pbc[1][k] + iz * pbc[2][k];
-}
-
-// Reciprocal and square root: IEEE-exact in FP64; the hardware
-// single-instruction forms (about 1 ulp) in the FP32 path, whose error budget
-// is set by FP32 rounding anyway.
-template <typename R>
-__device__ __forceinline__ R recip(R x)
-{
-    if constexpr (sizeof(R) == 4) return __builtin_amdgcn_rcpf(x);
-    else return R(1) / x;
-}
-template <typename R>
-__device__ __forceinline__ R root(R x)
-{
-    if constexpr (sizeof(R) == 4) return __builtin_amdgcn_sqrtf(x);
-    else return sqrt(x);
-}
-
-// Energy-form feature sets, fixed at compile time so the common cases skip
-// unused terms: general (runtime mode, kg and k_reg), lipid mode with kg =
-// k_reg = 0 (the reference default), Helfrich mode with kg = k_reg = 0.
-constexpr int kFeatGeneral = 0, kFeatLipid = 1, kFeatHelfrich = 2;
-
-// One Gauss point: energy and dE/d(Ru, Rv, Ruu, Ruv, Rvv).
-//
-// The reference builds the principal curvatures c1,2 = -(a+d -/+ sqrt(SAFETY
-// + (a-d)^2 + 4bc))/2 of the shape operator S = I^-1 II and differentiates
-// through both; every energy term depends on c1 + c2 = -(a + d) = -tr S and
-// c1 c2 = det S - SAFETY/4 only, so this evaluates tr S and det S directly:
-//   tr S  = (L E22 - 2 M E12 + N E11) / det I,   det S = (L N - M^2) / det I
-// with E = first fundamental form, (L, M, N) = n . (Ruu, Ruv, Rvv) and
-// n = (Ru x Rv)/g, g = sqrt(det I) = |Ru x Rv|; the gradient is the exact
-// reverse of this chain.  ig0 = 1/g0 of this point.
-// The gradient of a point in factored form: dE/dRu, dE/dRv, and the rows
-// Ruu, Ruv, Rvv as (dL, dM, dN) times the unit normal n.
-template <typename R>
-struct PointGrad {
-    R du[3], dv[3];
-    R dl, dm, dn;
-    R n[3];
-};
-
-template <typename R, int FEAT>
-__device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R ig0, R rurv0, R Lo, R Li,
-                                               R c0f, R dudv, R lamA, const PhysParams &ph, PointGrad<R> &out)
-{
-    constexpr bool kAnyMode = FEAT == kFeatGeneral;
-    constexpr bool kExtras = FEAT == kFeatGeneral;   // kg and k_reg terms
-    const R *Ru = X[0], *Rv = X[1], *Suu = X[2], *Suv = X[3], *Svv = X[4];
-    const R E11 = Ru[0] * Ru[0] + Ru[1] * Ru[1] + Ru[2] * Ru[2];
-    const R E12 = Ru[0] * Rv[0] + Ru[1] * Rv[1] + Ru[2] * Rv[2];
-    const R E22 = Rv[0] * Rv[0] + Rv[1] * Rv[1] + Rv[2] * Rv[2];
-    const R det = E11 * E22 - E12 * E12;
-    const R g = root<R>(det);
-    const R idet = recip<R>(det);
-    const R ig = g * idet;                       // 1/g
-    R Nv[3];
-    Nv[0] = Ru[1] * Rv[2] - Ru[2] * Rv[1];
-    Nv[1] = Ru[2] * Rv[0] - Ru[0] * Rv[2];
-    Nv[2] = Ru[0] * Rv[1] - Ru[1] * Rv[0];
-    const R n[3] = {Nv[0] * ig, Nv[1] * ig, Nv[2] * ig};
-    const R L = n[0] * Suu[0] + n[1] * Suu[1] + n[2] * Suu[2];
-    const R M = n[0] * Suv[0] + n[1] * Suv[1] + n[2] * Suv[2];
-    const R N = n[0] * Svv[0] + n[1] * Svv[1] + n[2] * Svv[2];
-    const R T = L * E22 - R(2) * M * E12 + N * E11;
-    const R H = -T * idet;                       // c1 + c2
-
-    const R kc = (R)ph.kc, KA = (R)ph.KA;
-    const R g0w = g0 * w;
-    R e, dH;
-    const bool lipid = kAnyMode ? (ph.mode == 0) : (FEAT == kFeatLipid);
-    if (lipid) {
-        // lipid-area weighted leaflets, no dudv (src/interpLib.C:3832-3834)
-        const R lc0 = (R)ph.lipid_c0;
-        const R ho = H - lc0, hi = H + lc0;
-        const R q = R(0.25) * kc * w;
-        e = q * (ho * ho * Lo + hi * hi * Li);
-        dH = R(2) * q * (ho * Lo + hi * Li);
-    } else {
-        const R hd = H - c0f;
-        e = R(0.5) * kc * dudv * g0w * hd * hd;
-        dH = kc * dudv * g0w * hd;
-    }
-    // per-point area strain (A - A0)/A0 with A = dudv g w, A0 = dudv g0 w;
-    // lamA = dE/dA_tot of the global area restraint (A_tot = sum dudv g w)
-    const R strain = g * ig0 - R(1);
-    e += KA * dudv * g0w * strain * strain;
-    const R dg = (R(2) * KA * strain + lamA) * dudv * w;
-
-    R dL = -dH * E22 * idet, dM = R(2) * dH * E12 * idet, dN = -dH * E11 * idet;
-    R ddet = dH * T * idet * idet + dg * R(0.5) * ig;
-    R dE11 = -dH * N * idet + ddet * E22;
-    R dE22 = -dH * L * idet + ddet * E11;
-    R dE12 = R(2) * dH * M * idet - R(2) * ddet * E12;
-    if constexpr (kExtras) {
-        // Gaussian curvature term kg c1 c2 = kg (det S - SAFETY/4)
-        const R K = (L * N - M * M) * idet;
-        const R dK = (R)ph.kg * dudv * g0w;
-        e += dK * (K - R(0.25e-12));
-        dL += dK * N * idet;
-        dM -= R(2) * dK * M * idet;
-        dN += dK * L * idet;
-        const R dd = -dK * K * idet;
-        dE11 += dd * E22;
-        dE22 += dd * E11;
-        dE12 -= R(2) * dd * E12;
-        // shear regulariser on cos(angle(Ru, Rv)) (src/interpLib.C:3797-3801)
-        const R kreg = (R)ph.k_reg;
-        const R s1122 = sqrt(E11 * E22);
-        const R val = E12 / s1122;
-        const R dv_ = val - rurv0;
-        e += kreg * dv_ * dv_;
-        const R dval = R(2) * kreg * dv_;
-        dE11 -= dval * val * R(0.5) / E11;
-        dE22 -= dval * val * R(0.5) / E22;
-        dE12 += dval / s1122;
-    }
-    R Q[3], dNv[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) Q[k] = dL * Suu[k] + dM * Suv[k] + dN * Svv[k];
-    const R qn = Q[0] * n[0] + Q[1] * n[1] + Q[2] * n[2];
-#pragma unroll
-    for (int k = 0; k < 3; k++) dNv[k] = (Q[k] - qn * n[k]) * ig;
-    // dE/dRu += Rv x dN ; dE/dRv += dN x Ru
-    out.du[0] = R(2) * dE11 * Ru[0] + dE12 * Rv[0] + (Rv[1] * dNv[2] - Rv[2] * dNv[1]);
-    out.du[1] = R(2) * dE11 * Ru[1] + dE12 * Rv[1] + (Rv[2] * dNv[0] - Rv[0] * dNv[2]);
-    out.du[2] = R(2) * dE11 * Ru[2] + dE12 * Rv[2] + (Rv[0] * dNv[1] - Rv[1] * dNv[0]);
-    out.dv[0] = R(2) * dE22 * Rv[0] + dE12 * Ru[0] + (dNv[1] * Ru[2] - dNv[2] * Ru[1]);
-    out.dv[1] = R(2) * dE22 * Rv[1] + dE12 * Ru[1] + (dNv[2] * Ru[0] - dNv[0] * Ru[2]);
-    out.dv[2] = R(2) * dE22 * Rv[2] + dE12 * Ru[2] + (dNv[0] * Ru[1] - dNv[1] * Ru[0]);
-    out.dl = dL;
-    out.dm = dM;
-    out.dn = dN;
-#pragma unroll
-    for (int k = 0; k < 3; k++) out.n[k] = n[k];
-    return e;
-}
-
-// the same with the five rows expanded: D[d][k] = dE/dX[d][k]
-template <typename R, int FEAT>
-__device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R ig0, R rurv0, R Lo, R Li,
-                                               R c0f, R dudv, R lamA, const PhysParams &ph, R D[5][3])
-{
-    PointGrad<R> pg;
-    const R e = point_energy_grad<R, FEAT>(X, w, g0, ig0, rurv0, Lo, Li, c0f, dudv, lamA, ph, pg);
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        D[0][k] = pg.du[k];
-        D[1][k] = pg.dv[k];
-        D[2][k] = pg.dl * pg.n[k];
-        D[3][k] = pg.dm * pg.n[k];
-        D[4][k] = pg.dn * pg.n[k];
-    }
-    return e;
-}
-
-// Fused energy + gradient of 64 faces by one workgroup of WS waves.
-// Lane l of every wave owns face group*64 + l; wave w evaluates the Gauss
-// points p = w, w + WS, ... so the table row of each point is wave-uniform
-// (scalar loads feed the FMAs as SGPR operands).  The waves' partial
-// gradients are summed in LDS in a fixed order, then wave 0 writes each
-// (face, control point) gradient into that vertex's CSR slot.
-//
-// R = double: the FP64 path, on absolute coordinates.
-// R = float:  the mixed path.  Control points are formed relative to the
-// face's first control point in FP64 and rounded to FP32 (the derivative
-// rows of every table sum to zero, so the contraction is translation
-// invariant); contraction, curvature and chain rule run in FP32; the energy
-// and dE/dalpha accumulate in FP64; the gradient leaves as FP32 and is summed
-// per vertex in FP64 by the gather.
-template <typename R, int NCP, int NPT, int KIND, int WS, bool PLDS, int FEAT>
-__device__ __forceinline__ void face_group(const FaceArgs &a, const double *__restrict__ r, int group, R *lds,
-                                           double (*s_corr)[kWave], double acc[4])
-{
-    constexpr bool kRel = sizeof(R) == 4;
-    const R *tab = kind_table<KIND, R>();
-    const R *wts = tab + NPT * 5 * NCP;
-    const int lane = threadIdx.x & (kWave - 1);
-    // wave index made provably wave-uniform so every table address below is
-    // too (the compiler then reads the tables with scalar loads)
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int nf = a.nf;
-    const int f = group * kWave + lane;
-    const bool live = f < nf;
-    const int fc = live ? f : nf - 1;
-    const double al64[3] = {r[3 * a.nv], r[3 * a.nv + 1], r[3 * a.nv + 2]};
-    const R al[3] = {(R)al64[0], (R)al64[1], (R)al64[2]};
-    const bool shifted = a.has_img[fc] != 0;
-
-    auto load_cp = [&](int c, double out[3]) {
-        const int v = a.cp[c * nf + fc];
-        out[0] = r[3 * v];
-        out[1] = r[3 * v + 1];
-        out[2] = r[3 * v + 2];
-        if (shifted) {
-            double o[3];
-            image_offset(a.img[c * nf + fc], a.pbc, o);
-            out[0] += o[0];
-            out[1] += o[1];
-            out[2] += o[2];
-        }
-    };
-    double base[3] = {0.0, 0.0, 0.0};
-    if (kRel) load_cp(0, base);
-    R P[PLDS ? 1 : NCP][3];
-    if (PLDS) {
-        for (int c = w; c < NCP; c += WS) {
-            double t[3];
-            load_cp(c, t);
-#pragma unroll
-            for (int k = 0; k < 3; k++) lds[(c * 3 + k) * kWave + lane] = (R)(t[k] - base[k]);
-        }
-        __syncthreads();
-    } else {
-#pragma unroll
-        for (int c = 0; c < NCP; c++) {
-            double t[3];
-            load_cp(c, t);
-#pragma unroll
-            for (int k = 0; k < 3; k++) P[c][k] = (R)(t[k] - base[k]);
-        }
-    }
-#define PV(c, k) (PLDS ? lds[((c) * 3 + (k)) * kWave + lane] : P[PLDS ? 0 : (c)][k])
-
-    R G[NCP][3];
-#pragma unroll
-    for (int c = 0; c < NCP; c++) G[c][0] = G[c][1] = G[c][2] = R(0);
-    // s_corr[3][64]: the volume seed's share of dE/dalpha (wave 0), in LDS
-    const bool seeded = a.rs && a.do_volume;
-    if (seeded && w == 0) {
-        // Volume restraint, seeded into wave 0's accumulators before the
-        // points (few registers are live here): dE/dV dV/dP of this face's
-        // flat triangle of corner limit points L0, L1, L2 (V_face = L0 . (L1 x
-        // L2) / 6), on absolute positions in FP64 (the faces' terms cancel only
-        // about a common origin).  The write-out multiplies G by alpha, so the
-        // seed is divided by it; the volume does not depend on alpha, so the
-        // seed's share of the alpha gradient is taken off acc here.
-        const double *W = kind_corner<KIND>();
-        double L[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
-#pragma unroll
-        for (int c = 0; c < NCP; c++)
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const double x = base[k] + (double)PV(c, k);
-#pragma unroll
-                for (int q = 0; q < 3; q++) L[q][k] += W[q * NCP + c] * x;
-            }
-        const double lamV = a.rs[1] * (1.0 / 6.0);
-        double D[3][3];
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            const double *A = L[(q + 1) % 3], *B = L[(q + 2) % 3];
-            D[q][0] = lamV * (A[1] * B[2] - A[2] * B[1]);
-            D[q][1] = lamV * (A[2] * B[0] - A[0] * B[2]);
-            D[q][2] = lamV * (A[0] * B[1] - A[1] * B[0]);
-        }
-        double corr[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-        for (int c = 0; c < NCP; c++)
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const double v = W[c] * D[0][k] + W[NCP + c] * D[1][k] + W[2 * NCP + c] * D[2][k];
-                G[c][k] = (R)(v / al64[k]);
-                corr[k] += (double)G[c][k] * (base[k] + (double)PV(c, k));
-            }
-        // kept in LDS over the points (registers are what the points need)
-#pragma unroll
-        for (int k = 0; k < 3; k++) s_corr[k][lane] = corr[k];
-    }
-    const R Lo = (R)a.Lo[fc], Li = (R)a.Li[fc], c0f = (R)a.c0[fc];
-    const R lamA = a.rs ? (R)a.rs[0] : R(0);
-    double e = 0.0;
-    for (int p = w; p < NPT; p += WS) {
-        const R *T = tab + p * 5 * NCP;
-        const R *TB = kind_table_b<KIND, R>() + p * 5 * NCP;
-        R X[5][3];
-#pragma unroll
-        for (int d = 0; d < 5; d++)
-#pragma unroll
-            for (int k = 0; k < 3; k++) X[d][k] = R(0);
-#pragma unroll
-        for (int c = 0; c < NCP; c++) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const R x = PV(c, k);
-#pragma unroll
-                for (int d = 0; d < 5; d++) X[d][k] += T[c * 5 + d] * x;
-            }
-        }
-#pragma unroll
-        for (int d = 0; d < 5; d++)
-#pragma unroll
-            for (int k = 0; k < 3; k++) X[d][k] *= al[k];
-        PointGrad<R> pg;
-        const R rr0 = FEAT == kFeatGeneral ? (R)a.rurv0[p * nf + fc] : R(0);
-        e += (double)point_energy_grad<R, FEAT>(X, wts[p], (R)a.g0[p * nf + fc], (R)a.ig0[p * nf + fc], rr0,
-                                                Lo, Li, c0f, (R)a.dudv, lamA, a.ph, pg);
-        if constexpr (sizeof(R) == 4) {
-            // FP32: the second-derivative rows factored, sum_d TB[c][d]
-            // D[d][k] = TB0 du_k + TB1 dv_k + (TB2 dL + TB3 dM + TB4 dN) n_k
-            // (12 multiply-adds per control point instead of 15; measured 5 %
-            // faster on the vesicle, 14 % on the sheet)
-#pragma unroll
-            for (int c = 0; c < NCP; c++) {
-                const R sn = TB[c * 5 + 2] * pg.dl + TB[c * 5 + 3] * pg.dm + TB[c * 5 + 4] * pg.dn;
-#pragma unroll
-                for (int k = 0; k < 3; k++)
-                    G[c][k] += TB[c * 5] * pg.du[k] + TB[c * 5 + 1] * pg.dv[k] + sn * pg.n[k];
-            }
-        } else {
-            // FP64: the five rows expanded (the factored form removes the
-            // register spills but measured no faster here)
-            R D[5][3];
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                D[0][k] = pg.du[k];
-                D[1][k] = pg.dv[k];
-                D[2][k] = pg.dl * pg.n[k];
-                D[3][k] = pg.dm * pg.n[k];
-                D[4][k] = pg.dn * pg.n[k];
-            }
-#pragma unroll
-            for (int c = 0; c < NCP; c++) {
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    R s = G[c][k];
-#pragma unroll
-                    for (int d = 0; d < 5; d++) s += TB[c * 5 + d] * D[d][k];
-                    G[c][k] = s;
-                }
-            }
-        }
-    }
-    if (live) {
-        acc[0] += e;
-        // dE/dalpha_k = sum_c dE/dR_ck (r + shift)_ck is linear in G: each
-        // wave adds its own points' share (in FP64)
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            double sg = 0.0, sgp = 0.0;
-#pragma unroll
-            for (int c = 0; c < NCP; c++) {
-                sg += (double)G[c][k];
-                sgp += (double)G[c][k] * (double)PV(c, k);
-            }
-            acc[1 + k] += sgp + sg * base[k];
-            if (seeded && w == 0) acc[1 + k] -= s_corr[k][lane];
-        }
-    }
-#undef PV
-    if (WS > 1) {
-        // fixed-order cross-wave sum of G through LDS
-        if (PLDS) __syncthreads();
-        for (int src = 1; src < WS; src++) {
-            if (w == src)
-#pragma unroll
-                for (int c = 0; c < NCP; c++)
-#pragma unroll
-                    for (int k = 0; k < 3; k++) lds[(c * 3 + k) * kWave + lane] = G[c][k];
-            __syncthreads();
-            if (w == 0)
-#pragma unroll
-                for (int c = 0; c < NCP; c++)
-#pragma unroll
-                    for (int k = 0; k < 3; k++) G[c][k] += lds[(c * 3 + k) * kWave + lane];
-            __syncthreads();
-        }
-    }
-    if (w == 0 && live) {
-        R *gb = (R *)(kRel ? (void *)a.gbuf32 : (void *)a.gbuf);
-#pragma unroll
-        for (int c = 0; c < NCP; c++) {
-            R *dst = gb + 3 * (size_t)a.pos[c * nf + f];
-            dst[0] = G[c][0] * al[0];
-            dst[1] = G[c][1] * al[1];
-            dst[2] = G[c][2] * al[2];
-        }
-    }
-}
-
-// Fused face kernel over a table of work items, one item per 64-face group of
-// one face class of one surface.  A surface evaluated alone and every member
-// of a batched ensemble run this same kernel; an item writes its partial sums
-// to the slot its group has in its surface's own launch, so a surface's
-// results do not depend on what else shares the launch.  PERSIST = false: one
-// workgroup per item; PERSIST = true: a smaller grid strides over the items.
-// B.r / B.step_ctr (non-null for a single surface's own launch) override the
-// table's positions and step counter.
-template <typename R, int WS, bool PLDS, int OCC, int FEAT, bool PERSIST>
-__global__ __launch_bounds__(kWave *WS, OCC) void faces_batch_kernel(BatchFaceArgs B)
-{
-    __shared__ R lds[13 * 3 * kWave];
-    __shared__ double s_corr[3][kWave];
-    for (int it = xcd_logical_block(blockIdx.x, gridDim.x); it < B.nitems; it += gridDim.x) {
-        const FaceItem item = B.items[it];
-        const FaceArgs &a = B.fa[item.fa];
-        const double *r = B.r ? B.r : a.r;
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        switch (item.kind) {
-        case 0: face_group<R, 12, 15, 0, WS, PLDS, FEAT>(a, r, item.group, lds, s_corr, acc); break;
-        case 1: face_group<R, 11, 16, 1, WS, PLDS, FEAT>(a, r, item.group, lds, s_corr, acc); break;
-        default: face_group<R, 13, 16, 2, WS, PLDS, FEAT>(a, r, item.group, lds, s_corr, acc); break;
-        }
-        block_sum4<kWave * WS>(acc);
-        if (threadIdx.x == 0) {
-            double *dst = a.partial + 4 * (size_t)item.slot;
-#pragma unroll
-            for (int i = 0; i < 4; i++) dst[i] = acc[i];
-            int *ctr = B.step_ctr ? B.step_ctr : a.step_ctr;
-            if (item.slot == 0 && ctr) ctr[0] += 1;
-        }
-        if (!PERSIST) break;
-        __syncthreads();   // LDS and the reduction scratch are reused by the next item
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Mixed path with the basis contractions on the matrix cores.
-//
-// The two contractions of a Gauss point, X[d][k] = sum_c T[p][c][d] P[c][k]
-// (forward: the five derivative rows Ru, Rv, Ruu, Ruv, Rvv) and G[c][k] +=
-// sum_d TB[p][c][d] D[d][k] (gradient), are about 70 % of the VALU work of
-// face_group<float>.  Here they run as v_mfma_f32_16x16x4f32 products over 64
-// faces and four Gauss points at a time, in layouts chosen so that no data
-// moves between the matrix products and the pointwise math:
-//
-//   operand layouts of v_mfma_f32_16x16x4f32 (wave64, lane l, q = l / 16,
-//   j = l % 16): A[16][4] supplies A[j][q], B[4][16] supplies B[q][j], and
-//   the 16 x 16 result leaves rows 4q .. 4q + 3 of column j in the lane.
-//
-//   forward, per 16-face tile t and coordinate k:  rows = (point q, d),
-//     M-tile 0 holds d = 0..3, M-tile 1 holds d = 4 in row 4q (rows 4q+1..3
-//     zero); K = control points in steps of four; columns = faces.  Lane
-//     (q, j) then holds all five rows X[d][k] of point 4b + q of face 16t + j.
-//   pointwise: lane (q, j) evaluates that (face, point) pair.
-//   gradient: K = the four points of the batch (one step per row d), B =
-//     D[d][k] exactly as the lane holds it; rows = control points; the lane
-//     accumulates G[c = 4q .. 4q + 3][k] of face 16t + j.
-//
-// Products accumulate in FP32 in a fixed order, so results are deterministic
-// (not bitwise equal to face_group<float>, whose FMA order differs).
-typedef float mx4 __attribute__((ext_vector_type(4)));
-
-constexpr int kMxRow = 80;                  // LDS row stride of the staged control points (bank spread)
-constexpr int kMxP = 16 * 3 * kMxRow;       // [16 cp][3][kMxRow] floats
-constexpr int kMxBase = 3 * kWave * 2;      // [3][64] doubles
-constexpr int kMxGv = 13 * 3 * kWave;       // [ncp][3][64] volume seed
-constexpr int kMxLds = kMxP + kMxBase + kMxGv;
-static_assert(48 * kWave <= kMxP, "cross-wave sums reuse the control-point rows");
-
-__device__ __forceinline__ mx4 mfma4(float a, float b, mx4 c)
-{
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-}
-
-template <int NCP, int NPT, int KIND, int WS, int FEAT>
-__device__ __forceinline__ void face_group_mx(const FaceArgs &a, const double *__restrict__ r, int group,
-                                              float *lds, double acc[4])
-{
-    constexpr int NKS = (NCP + 3) / 4;   // control-point steps of four
-    constexpr int NB = (NPT + 3) / 4;    // batches of four Gauss points
-    static_assert(4 * NKS <= 16 && NCP <= 13, "staging holds 16 control-point rows");
-    const float *tab = kind_table<KIND, float>();
-    const float *tabb = kind_table_b<KIND, float>();
-    const float *wts = tab + NPT * 5 * NCP;
-    float *sP = lds;                                   // [16][3][kMxRow] relative control points
-    double *sBase = (double *)(lds + kMxP);            // [3][64] face origin (control point 0)
-    float *sGv = lds + kMxP + kMxBase;                 // [NCP][3][64] volume seed / alpha
-    float *sRed = lds;                                 // [48][64] cross-wave sums (after the points)
-    const int lane = threadIdx.x & (kWave - 1);
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int q = lane >> 4, j = lane & 15;
-    const int nf = a.nf;
-    const double al64[3] = {r[3 * a.nv], r[3 * a.nv + 1], r[3 * a.nv + 2]};
-    const float al[3] = {(float)al64[0], (float)al64[1], (float)al64[2]};
-
-    // 1. staging: lane = face, the waves split the control points
-    {
-        const int f = group * kWave + lane;
-        const int fc = f < nf ? f : nf - 1;
-        const bool shifted = a.has_img[fc] != 0;
-        auto load_cp = [&](int c, double out[3]) {
-            const int v = a.cp[c * nf + fc];
-            out[0] = r[3 * v];
-            out[1] = r[3 * v + 1];
-            out[2] = r[3 * v + 2];
-            if (shifted) {
-                double o[3];
-                image_offset(a.img[c * nf + fc], a.pbc, o);
-                out[0] += o[0];
-                out[1] += o[1];
-                out[2] += o[2];
-            }
-        };
-        double base[3];
-        load_cp(0, base);
-        if (w == 0)
-#pragma unroll
-            for (int k = 0; k < 3; k++) sBase[k * kWave + lane] = base[k];
-        for (int c = w; c < 4 * NKS; c += WS) {
-            double t[3] = {base[0], base[1], base[2]};
-            if (c < NCP) load_cp(c, t);
-#pragma unroll
-            for (int k = 0; k < 3; k++) sP[(c * 3 + k) * kMxRow + lane] = (float)(t[k] - base[k]);
-        }
-    }
-    __syncthreads();
-    const bool vol = a.rs && a.do_volume;
-    if (vol) {
-        // volume restraint of this face (see face_group): dE/dV dV/dP of the
-        // flat triangle of corner limit points, FP64 on absolute positions,
-        // kept in LDS divided by alpha until the write-out; every wave forms
-        // the corners, then writes its share of the control points
-        const double *W = kind_corner<KIND>();
-        double L[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
-#pragma unroll
-        for (int c = 0; c < NCP; c++)
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const double x = sBase[k * kWave + lane] + (double)sP[(c * 3 + k) * kMxRow + lane];
-#pragma unroll
-                for (int qq = 0; qq < 3; qq++) L[qq][k] += W[qq * NCP + c] * x;
-            }
-        const double lamV = a.rs[1] * (1.0 / 6.0);
-        double D[3][3];
-#pragma unroll
-        for (int qq = 0; qq < 3; qq++) {
-            const double *A = L[(qq + 1) % 3], *B = L[(qq + 2) % 3];
-            D[qq][0] = lamV * (A[1] * B[2] - A[2] * B[1]) / al64[0];
-            D[qq][1] = lamV * (A[2] * B[0] - A[0] * B[2]) / al64[1];
-            D[qq][2] = lamV * (A[0] * B[1] - A[1] * B[0]) / al64[2];
-        }
-        for (int c = w; c < NCP; c += WS)
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const double v = W[c] * D[0][k] + W[NCP + c] * D[1][k] + W[2 * NCP + c] * D[2][k];
-                sGv[(c * 3 + k) * kWave + lane] = (float)v;
-            }
-    }
-
-    // 2. Gauss points, four at a time
-    mx4 G[4][3];
-#pragma unroll
-    for (int t = 0; t < 4; t++)
-#pragma unroll
-        for (int k = 0; k < 3; k++) G[t][k] = mx4{0.f, 0.f, 0.f, 0.f};
-    const float lamA = a.rs ? (float)a.rs[0] : 0.f;
-    const float dudv = (float)a.dudv;
-    // per-face inputs of the lane's four faces (16 t + j), loaded up front
-    constexpr bool kLip = FEAT != kFeatHelfrich, kC0 = FEAT != kFeatLipid;
-    float fLo[4], fLi[4], fc0[4];
-    int fcs[4];
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-        const int f = group * kWave + 16 * t + j;
-        fcs[t] = f < nf ? f : nf - 1;
-        fLo[t] = kLip ? (float)a.Lo[fcs[t]] : 0.f;
-        fLi[t] = kLip ? (float)a.Li[fcs[t]] : 0.f;
-        fc0[t] = kC0 ? (float)a.c0[fcs[t]] : 0.f;
-    }
-    double e = 0.0;
-#pragma unroll 1
-    for (int b = w; b < NB; b += WS) {
-        // A operands: forward row j = (point j / 4, row j % 4), column q =
-        // control point 4s + q; gradient row j = control point, column q = point
-        float A0[NKS], A1[NKS], AB[5];
-        const int pf = 4 * b + (j >> 2);
-#pragma unroll
-        for (int s2 = 0; s2 < NKS; s2++) {
-            const int c = 4 * s2 + q;
-            const bool ok = pf < NPT && c < NCP;
-            const int i0 = ok ? (pf * NCP + c) * 5 : 0;
-            A0[s2] = ok ? tab[i0 + (j & 3)] : 0.f;
-            A1[s2] = (ok && (j & 3) == 0) ? tab[i0 + 4] : 0.f;
-        }
-        const int p = 4 * b + q;
-        const bool pt = p < NPT;
-        {
-            const bool ok = pt && j < NCP;
-            const int i0 = ok ? (p * NCP + j) * 5 : 0;
-#pragma unroll
-            for (int d = 0; d < 5; d++) AB[d] = ok ? tabb[i0 + d] : 0.f;
-        }
-        const float wp = pt ? wts[p] : 0.f;
-        // per-point inputs of the four faces (clamped point index: the pad
-        // point of a 15-point rule reads point 14 and is then skipped)
-        const int pc = pt ? p : NPT - 1;
-        float pg0[4], pig0[4], prr0[4];
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            pg0[t] = (float)a.g0[pc * nf + fcs[t]];
-            pig0[t] = (float)a.ig0[pc * nf + fcs[t]];
-            prr0[t] = FEAT == kFeatGeneral ? (float)a.rurv0[pc * nf + fcs[t]] : 0.f;
-        }
-        // one 16-face tile at a time: forward products, pointwise math,
-        // gradient products (holding all four tiles' products at once costs
-        // more in occupancy than it gains in independent work)
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const int fl = 16 * t + j;
-            mx4 X0[3], X1[3];
-#pragma unroll
-            for (int k = 0; k < 3; k++) X0[k] = X1[k] = mx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int s2 = 0; s2 < NKS; s2++)
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    const float bv = sP[((4 * s2 + q) * 3 + k) * kMxRow + fl];
-                    X0[k] = mfma4(A0[s2], bv, X0[k]);
-                    X1[k] = mfma4(A1[s2], bv, X1[k]);
-                }
-            float D[5][3];
-            if (pt) {
-                float X[5][3];
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    X[0][k] = X0[k][0] * al[k];
-                    X[1][k] = X0[k][1] * al[k];
-                    X[2][k] = X0[k][2] * al[k];
-                    X[3][k] = X0[k][3] * al[k];
-                    X[4][k] = X1[k][0] * al[k];
-                }
-                const float ee = point_energy_grad<float, FEAT>(X, wp, pg0[t], pig0[t], prr0[t], fLo[t], fLi[t],
-                                                                fc0[t], dudv, lamA, a.ph, D);
-                if (group * kWave + fl < nf) e += (double)ee;
-            } else {
-#pragma unroll
-                for (int d = 0; d < 5; d++) D[d][0] = D[d][1] = D[d][2] = 0.f;
-            }
-#pragma unroll
-            for (int d = 0; d < 5; d++)
-#pragma unroll
-                for (int k = 0; k < 3; k++) G[t][k] = mfma4(AB[d], D[d][k], G[t][k]);
-        }
-    }
-
-    // 3. energy and dE/dalpha_k = sum_c G[c][k] (r + shift)_ck of this wave's points
-    acc[0] += e;
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-        const int fl = 16 * t + j;
-        if (group * kWave + fl >= nf) continue;
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            double sg = 0.0, sgp = 0.0;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int c = 4 * q + i;
-                if (c < NCP) {
-                    sg += (double)G[t][k][i];
-                    sgp += (double)G[t][k][i] * (double)sP[(c * 3 + k) * kMxRow + fl];
-                }
-            }
-            acc[1 + k] += sgp + sg * sBase[k * kWave + fl];
-        }
-    }
-    __syncthreads();   // sP is reused for the cross-wave sums below
-    if (WS > 1) {
-        for (int src = 1; src < WS; src++) {
-            if (w == src)
-#pragma unroll
-                for (int t = 0; t < 4; t++)
-#pragma unroll
-                    for (int k = 0; k < 3; k++)
-#pragma unroll
-                        for (int i = 0; i < 4; i++) sRed[((t * 3 + k) * 4 + i) * kWave + lane] = G[t][k][i];
-            __syncthreads();
-            if (w == 0)
-#pragma unroll
-                for (int t = 0; t < 4; t++)
-#pragma unroll
-                    for (int k = 0; k < 3; k++)
-#pragma unroll
-                        for (int i = 0; i < 4; i++) G[t][k][i] += sRed[((t * 3 + k) * 4 + i) * kWave + lane];
-            __syncthreads();
-        }
-    }
-    if (w == 0) {
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const int fl = 16 * t + j;
-            const int f = group * kWave + fl;
-            if (f >= nf) continue;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int c = 4 * q + i;
-                if (c >= NCP) continue;
-                float v[3];
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    v[k] = G[t][k][i];
-                    if (vol) v[k] += sGv[(c * 3 + k) * kWave + fl];
-                }
-                float *dst = a.gbuf32 + 3 * (size_t)a.pos[c * nf + f];
-                dst[0] = v[0] * al[0];
-                dst[1] = v[1] * al[1];
-                dst[2] = v[2] * al[2];
-            }
-        }
-    }
-}
-
-// The batched face kernel of the matrix-core mixed path: the same work items,
-// partial slots and step counter as faces_batch_kernel.
-template <int WS, int OCC, int FEAT, bool PERSIST>
-__global__ __launch_bounds__(kWave *WS, OCC) void faces_batch_mfma(BatchFaceArgs B)
-{
-    __shared__ __align__(16) float lds[kMxLds];
-    for (int it = xcd_logical_block(blockIdx.x, gridDim.x); it < B.nitems; it += gridDim.x) {
-        const FaceItem item = B.items[it];
-        const FaceArgs &a = B.fa[item.fa];
-        const double *r = B.r ? B.r : a.r;
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        switch (item.kind) {
-        case 0: face_group_mx<12, 15, 0, WS, FEAT>(a, r, item.group, lds, acc); break;
-        case 1: face_group_mx<11, 16, 1, WS, FEAT>(a, r, item.group, lds, acc); break;
-        default: face_group_mx<13, 16, 2, WS, FEAT>(a, r, item.group, lds, acc); break;
-        }
-        block_sum4<kWave * WS>(acc);
-        if (threadIdx.x == 0) {
-            double *dst = a.partial + 4 * (size_t)item.slot;
-#pragma unroll
-            for (int i = 0; i < 4; i++) dst[i] = acc[i];
-            int *ctr = B.step_ctr ? B.step_ctr : a.step_ctr;
-            if (item.slot == 0 && ctr) ctr[0] += 1;
-        }
-        if (!PERSIST) break;
-        __syncthreads();
-    }
-}
+// the face kernels' translation units (umcp_faces64.hip, umcp_faces32.hip,
+// umcp_faces_mx.hip)
+hipError_t launch_faces64(const BatchFaceArgs &a, int nb, bool persist, int ws, int occ2, int feat, hipStream_t st);
+hipError_t launch_faces32(const BatchFaceArgs &a, int nb, bool persist, int ws, int occ2, int feat, hipStream_t st);
+hipError_t launch_faces_mx(const BatchFaceArgs &a, int nb, bool persist, int ws, int occ2, int feat, hipStream_t st);
+hipError_t copy_tables_faces64(int kind, const double *buf, const float *b32, const double *corner, size_t n, int npt,
+                               int ncp);
+hipError_t copy_tables_faces32(int kind, const double *buf, const float *b32, const double *corner, size_t n, int npt,
+                               int ncp);
+hipError_t copy_tables_faces_mx(int kind, const double *buf, const float *b32, const double *corner, size_t n,
+                                int npt, int ncp);
 
 // Per-point metric: mode 0 writes g0 and RuRv0 (no alpha, as setg0 does);
 // mode 1 accumulates sum(dudv g w) and sum(dudv g0 w) into block partials.
@@ -1504,7 +691,7 @@ __device__ double face_energy_one(const FaceArgs &a, const double *__restrict__ 
                 X[d][k] = s * al[k];
             }
         PointGrad<double> pg;
-        e += point_energy_grad<double, kFeatGeneral>(X, wts[p], a.g0[p * nf + fc], a.ig0[p * nf + fc],
+        e += point_energy_grad<double, kFeatGeneral, KIND == 0>(X, wts[p], a.g0[p * nf + fc], a.ig0[p * nf + fc],
                                                      a.rurv0[p * nf + fc], a.Lo[fc], a.Li[fc], a.c0[fc], a.dudv, 0.0,
                                                      a.ph, pg);
         // surface::area's terms (src/interpLib.C:4480-4560)
@@ -1594,23 +781,12 @@ hipError_t upload_tables(int kind, const double *table, const double *weights, c
         for (int c = 0; c < ncp; c++)
             for (int d = 0; d < 5; d++) buf[(size_t)(p * ncp + c) * 5 + d] = table[(size_t)(p * 6 + d + 1) * ncp + c];
     for (int p = 0; p < npt; p++) buf[n + p] = weights[p];
-    const void *fwd = kind == 0 ? (const void *)HIP_SYMBOL(c_tab_reg)
-                                : (kind == 1 ? (const void *)HIP_SYMBOL(c_tab_v5) : (const void *)HIP_SYMBOL(c_tab_v7));
-    const void *bwd = kind == 0 ? (const void *)HIP_SYMBOL(c_tab_reg_b)
-                                : (kind == 1 ? (const void *)HIP_SYMBOL(c_tab_v5_b) : (const void *)HIP_SYMBOL(c_tab_v7_b));
-    hipError_t e = hipMemcpyToSymbol(fwd, buf, (n + npt) * sizeof(double), 0, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpyToSymbol(bwd, buf, n * sizeof(double), 0, hipMemcpyHostToDevice);
     float *b32 = new float[n + npt];
     for (size_t i = 0; i < n + npt; i++) b32[i] = (float)buf[i];
-    const void *fwd32 = kind == 0 ? (const void *)HIP_SYMBOL(c_tab32_reg)
-                                  : (kind == 1 ? (const void *)HIP_SYMBOL(c_tab32_v5) : (const void *)HIP_SYMBOL(c_tab32_v7));
-    const void *bwd32 = kind == 0 ? (const void *)HIP_SYMBOL(c_tab32_reg_b)
-                                  : (kind == 1 ? (const void *)HIP_SYMBOL(c_tab32_v5_b) : (const void *)HIP_SYMBOL(c_tab32_v7_b));
-    if (e == hipSuccess) e = hipMemcpyToSymbol(fwd32, b32, (n + npt) * sizeof(float), 0, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpyToSymbol(bwd32, b32, n * sizeof(float), 0, hipMemcpyHostToDevice);
-    const void *cor = kind == 0 ? (const void *)HIP_SYMBOL(c_corner_reg)
-                                : (kind == 1 ? (const void *)HIP_SYMBOL(c_corner_v5) : (const void *)HIP_SYMBOL(c_corner_v7));
-    if (e == hipSuccess) e = hipMemcpyToSymbol(cor, corner, 3 * (size_t)ncp * sizeof(double), 0, hipMemcpyHostToDevice);
+    hipError_t e = copy_tables_here(kind, buf, b32, corner, n, npt, ncp);
+    if (e == hipSuccess) e = copy_tables_faces64(kind, buf, b32, corner, n, npt, ncp);
+    if (e == hipSuccess) e = copy_tables_faces32(kind, buf, b32, corner, n, npt, ncp);
+    if (e == hipSuccess) e = copy_tables_faces_mx(kind, buf, b32, corner, n, npt, ncp);
     delete[] b32;
     delete[] buf;
     return e;
@@ -1618,82 +794,6 @@ hipError_t upload_tables(int kind, const double *table, const double *weights, c
 
 int face_groups(int nf) { return (nf + kWave - 1) / kWave; }
 int metric_blocks(int nf) { return (nf + kMetricBlock - 1) / kMetricBlock; }
-
-template <typename R, int WS, int OCC, int FEAT>
-static hipError_t launch_ws(const BatchFaceArgs &a, int nb, bool persist, hipStream_t st)
-{
-    if (persist)
-        hipLaunchKernelGGL((faces_batch_kernel<R, WS, true, OCC, FEAT, true>), dim3(nb), dim3(kWave * WS), 0, st, a);
-    else
-        hipLaunchKernelGGL((faces_batch_kernel<R, WS, true, OCC, FEAT, false>), dim3(nb), dim3(kWave * WS), 0, st, a);
-    return hipGetLastError();
-}
-
-// the register-bound flag asks for 2 waves per SIMD in FP64 and 4 in FP32
-template <typename R, int FEAT>
-static hipError_t launch_shape(const BatchFaceArgs &a, int nb, bool persist, int ws, int occ2, hipStream_t st)
-{
-    constexpr int kB = sizeof(R) == 4 ? 4 : 2;
-    switch (ws * 2 + (occ2 ? 1 : 0)) {
-    case 2: return launch_ws<R, 1, 1, FEAT>(a, nb, persist, st);
-    case 3: return launch_ws<R, 1, kB, FEAT>(a, nb, persist, st);
-    case 4: return launch_ws<R, 2, 1, FEAT>(a, nb, persist, st);
-    case 5: return launch_ws<R, 2, kB, FEAT>(a, nb, persist, st);
-    case 6: return launch_ws<R, 3, 1, FEAT>(a, nb, persist, st);
-    case 7: return launch_ws<R, 3, kB, FEAT>(a, nb, persist, st);
-    case 8: return launch_ws<R, 4, 1, FEAT>(a, nb, persist, st);
-    case 9: return launch_ws<R, 4, kB, FEAT>(a, nb, persist, st);
-    default: return hipErrorInvalidValue;
-    }
-}
-
-template <typename R>
-static hipError_t launch_prec(const BatchFaceArgs &a, int nb, bool persist, int ws, int occ2, int feat,
-                              hipStream_t st)
-{
-    switch (feat) {
-    case kFeatLipid: return launch_shape<R, kFeatLipid>(a, nb, persist, ws, occ2, st);
-    case kFeatHelfrich: return launch_shape<R, kFeatHelfrich>(a, nb, persist, ws, occ2, st);
-    default: return launch_shape<R, kFeatGeneral>(a, nb, persist, ws, occ2, st);
-    }
-}
-
-// matrix-core mixed path: point splits 1, 2 and 4 (one, two or four batches
-// of four Gauss points per wave); the register-bound flag asks for 2 waves
-// per SIMD
-template <int WS, int OCC, int FEAT>
-static hipError_t launch_mx_ws(const BatchFaceArgs &a, int nb, bool persist, hipStream_t st)
-{
-    if (persist)
-        hipLaunchKernelGGL((faces_batch_mfma<WS, OCC, FEAT, true>), dim3(nb), dim3(kWave * WS), 0, st, a);
-    else
-        hipLaunchKernelGGL((faces_batch_mfma<WS, OCC, FEAT, false>), dim3(nb), dim3(kWave * WS), 0, st, a);
-    return hipGetLastError();
-}
-
-template <int FEAT>
-static hipError_t launch_mx(const BatchFaceArgs &a, int nb, bool persist, int ws, int occ2, hipStream_t st)
-{
-    switch (ws * 2 + (occ2 ? 1 : 0)) {
-    case 2: return launch_mx_ws<1, 1, FEAT>(a, nb, persist, st);
-    case 3: return launch_mx_ws<1, 2, FEAT>(a, nb, persist, st);
-    case 4: return launch_mx_ws<2, 1, FEAT>(a, nb, persist, st);
-    case 5: return launch_mx_ws<2, 2, FEAT>(a, nb, persist, st);
-    case 8: return launch_mx_ws<4, 1, FEAT>(a, nb, persist, st);
-    case 9: return launch_mx_ws<4, 2, FEAT>(a, nb, persist, st);
-    default: return hipErrorInvalidValue;
-    }
-}
-
-static hipError_t launch_mx_feat(const BatchFaceArgs &a, int nb, bool persist, int ws, int occ2, int feat,
-                                 hipStream_t st)
-{
-    switch (feat) {
-    case kFeatLipid: return launch_mx<kFeatLipid>(a, nb, persist, ws, occ2, st);
-    case kFeatHelfrich: return launch_mx<kFeatHelfrich>(a, nb, persist, ws, occ2, st);
-    default: return launch_mx<kFeatGeneral>(a, nb, persist, ws, occ2, st);
-    }
-}
 
 int features_of(const PhysParams &ph)
 {
@@ -1743,9 +843,9 @@ hipError_t launch_faces_batch(const BatchFaceArgs &a, int grid, int ws, int flag
     const bool persist = grid > 0 && grid < a.nitems;
     const int nb = persist ? grid : a.nitems;
     const int occ2 = (flags & kLaunchOcc2) ? 1 : 0;
-    if (mixed && (flags & kLaunchMfma)) return launch_mx_feat(a, nb, persist, ws, occ2, feat, st);
-    return mixed ? launch_prec<float>(a, nb, persist, ws, occ2, feat, st)
-                 : launch_prec<double>(a, nb, persist, ws, occ2, feat, st);
+    if (mixed && (flags & kLaunchMfma)) return launch_faces_mx(a, nb, persist, ws, occ2, feat, st);
+    return mixed ? launch_faces32(a, nb, persist, ws, occ2, feat, st)
+                 : launch_faces64(a, nb, persist, ws, occ2, feat, st);
 }
 
 hipError_t launch_gather_batch(const BatchGatherArgs &a, int grid, bool mixed, hipStream_t st)

@@ -19,6 +19,7 @@ constexpr int kMaxSplit = 4;
 struct PhysParams {
     double kc, kg, KA, k_reg, lipid_c0;
     int mode;
+    int grad_exact;   // Helfrich form: exact dE/dg instead of surface::grad's
 };
 
 // One face class (all faces whose owner vertex has the same valence).

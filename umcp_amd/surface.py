@@ -24,6 +24,8 @@ MODE_LIPID = 0
 MODE_HELFRICH = 1
 PREC_FP64 = 0
 PREC_MIXED = 1
+GRAD_REFERENCE = 0   # surface::grad's Helfrich gradient (src/grad.C:448)
+GRAD_EXACT = 1       # the exact derivative of the Helfrich energy
 LAUNCH_OCC2 = 2
 LAUNCH_MFMA = 4
 
@@ -56,7 +58,7 @@ class UmcpError(RuntimeError):
 class Params(C.Structure):
     _fields_ = [("kc", C.c_double), ("kg", C.c_double), ("KA", C.c_double), ("k_reg", C.c_double),
                 ("mode", C.c_int), ("precision", C.c_int), ("lipid_c0", C.c_double),
-                ("lipid_apl", C.c_double)]
+                ("lipid_apl", C.c_double), ("grad_form", C.c_int)]
 
 
 class BDParams(C.Structure):
@@ -239,8 +241,8 @@ class Surface:
 
     # -- setup -----------------------------------------------------------
     def set_params(self, kc=14.0, kg=0.0, KA=0.0, k_reg=0.0, mode=MODE_LIPID,
-                   precision=PREC_FP64, lipid_c0=0.0, lipid_apl=68.3):
-        p = Params(kc, kg, KA, k_reg, mode, precision, lipid_c0, lipid_apl)
+                   precision=PREC_FP64, lipid_c0=0.0, lipid_apl=68.3, grad_form=GRAD_REFERENCE):
+        p = Params(kc, kg, KA, k_reg, mode, precision, lipid_c0, lipid_apl, grad_form)
         self._check(self._L.umcp_set_params(self._h, C.byref(p)), "set_params")
 
     def generatePlan(self):
@@ -677,7 +679,7 @@ class Server:
     def submit(self, mesh_path: str, nv: int, params: dict, c0: float, steps: int, seed: int,
                mobility_dt: float, noise_amp: float, r0: np.ndarray, r: np.ndarray) -> int:
         p = dict(kc=14.0, kg=0.0, KA=0.0, k_reg=0.0, mode=MODE_LIPID, precision=PREC_FP64, lipid_c0=0.0,
-                 lipid_apl=68.3)
+                 lipid_apl=68.3, grad_form=GRAD_REFERENCE)
         p.update(params)
         r0 = np.ascontiguousarray(r0, dtype=np.float64)
         r = np.ascontiguousarray(r, dtype=np.float64)
@@ -685,7 +687,7 @@ class Server:
             raise UmcpError("positions must hold 3nv+3 doubles")
         rq = Request(mesh_path.encode(), nv,
                      Params(p["kc"], p["kg"], p["KA"], p["k_reg"], p["mode"], p["precision"], p["lipid_c0"],
-                            p["lipid_apl"]), c0, int(steps), C.c_ulonglong(seed), BDParams(mobility_dt, noise_amp),
+                            p["lipid_apl"], p["grad_form"]), c0, int(steps), C.c_ulonglong(seed), BDParams(mobility_dt, noise_amp),
                      _dp(r0), _dp(r))
         rid = C.c_int()
         rc = self._L.umcp_server_submit(self._h, C.byref(rq), C.byref(rid))
