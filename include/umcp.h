@@ -161,6 +161,21 @@ int umcp_energy_grad_dev(umcp_surface *s, const double *d_r, double *d_E, double
 int umcp_bd_steps_dev(umcp_surface *s, double *d_r, const double *d_xi, int nsteps,
                       const umcp_bd_params *bd, double *d_energies, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under stream capture, serving loops that forbid allocation):
+ * umcp_energy_grad_dev on a planned, setg0'd surface, calling no hipMalloc or
+ * hipFree.  Its work buffers (face tables, (face, slot) gradient records,
+ * block and restraint partials) live in one static device array of
+ * UMCP_NOALLOC_SCRATCH_BYTES per device instead of the handle's buffers;
+ * results are bitwise those of umcp_energy_grad_dev.  A surface whose buffers
+ * do not fit (about 3 nnz doubles, nnz = sum of control points over faces:
+ * roughly 56k vertices in FP64, twice that mixed) returns UMCP_ERR_ARG.  All
+ * calls on one device share its array and keep the last call's contents:
+ * calls that overlap on one device (other streams or threads) race. */
+#define UMCP_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+int umcp_energy_grad_dev_noalloc(umcp_surface *s, const double *d_r, double *d_E, double *d_g,
+                                 void *stream);
+
 /* Global restraints: area KA_global*A0*((A-A0)/A0)^2 with A = sum dudv g w
  * (the GLOBAL_AREA form, src/interpLib.C:4322-4326) and volume
  * KV/2*V0*((V-V0)/V0)^2 (src/minimize.C:90-114) of the limit-point

@@ -83,6 +83,7 @@ struct umcp_surface {
     umcp::FaceItem *d_items = nullptr;
     std::vector<umcp::FaceArgs> h_fa;
     bool fa_dirty = true;
+    std::vector<umcp::FaceArgs> h_fa_noalloc;   // umcp_energy_grad_dev_noalloc's face table (host copy)
     // momentum-space integrators: truncated inverse effective mass (dense
     // on the device), the step's momenta and the kept product Mi pp
     bool effm_ready = false;

@@ -34,7 +34,7 @@ EXPORTS = [
     "umcp_link_check", "umcp_set_device", "umcp_load_mesh", "umcp_destroy", "umcp_last_error",
     "umcp_nv", "umcp_nfaces", "umcp_get", "umcp_set_params", "umcp_prepare", "umcp_plan", "umcp_set_face_c0",
     "umcp_setg0", "umcp_energy", "umcp_grad", "umcp_energy_grad", "umcp_area", "umcp_bd_steps",
-    "umcp_energy_grad_dev", "umcp_bd_steps_dev", "umcp_class_count", "umcp_class_info",
+    "umcp_energy_grad_dev", "umcp_energy_grad_dev_noalloc", "umcp_bd_steps_dev", "umcp_class_count", "umcp_class_info",
     "umcp_class_table", "umcp_face_points", "umcp_face_lipids", "umcp_set_launch",
     "umcp_get_launch", "umcp_profile_begin", "umcp_profile_end", "umcp_profile_sample",
     "umcp_set_restraints", "umcp_volume", "umcp_set_step", "umcp_get_step", "umcp_bd_steps_rng_dev",
@@ -122,6 +122,7 @@ def load_library(path: str = LIB_PATH):
         "umcp_area": ([vp, dp, dp, dp], C.c_int),
         "umcp_bd_steps": ([vp, dp, dp, C.c_int, C.POINTER(BDParams), dp], C.c_int),
         "umcp_energy_grad_dev": ([vp, vp, vp, vp, vp], C.c_int),
+        "umcp_energy_grad_dev_noalloc": ([vp, vp, vp, vp, vp], C.c_int),
         "umcp_bd_steps_dev": ([vp, vp, vp, C.c_int, C.POINTER(BDParams), vp, vp], C.c_int),
         "umcp_class_count": ([vp], C.c_int),
         "umcp_class_info": ([vp, C.c_int, ip, ip, ip, ip], C.c_int),
@@ -314,6 +315,15 @@ class Surface:
                                           C.c_void_p(d_g.data_ptr()) if d_g is not None else None,
                                           C.c_void_p(stream) if stream else None)
         self._check(rc, "energy_grad_dev")
+
+    def energy_grad_dev_noalloc(self, d_r, d_E, d_g, stream=None):
+        """energy_grad_dev with the library's static scratch array as work space
+        (no device allocation)"""
+        rc = self._L.umcp_energy_grad_dev_noalloc(self._h, C.c_void_p(d_r.data_ptr()),
+                                                  C.c_void_p(d_E.data_ptr()),
+                                                  C.c_void_p(d_g.data_ptr()) if d_g is not None else None,
+                                                  C.c_void_p(stream) if stream else None)
+        self._check(rc, "energy_grad_dev_noalloc")
 
     def bd_steps_dev(self, d_r, d_xi, nsteps: int, mobility_dt: float, noise_amp: float,
                      d_energies=None, stream=None):
