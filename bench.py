@@ -175,7 +175,7 @@ def run_leg(args, w, c0, precision, d_xi, dev, torch, D=1e-12, dt=1e-9):
     s.bd_steps_dev(d_r, d_xi[W:W + K], K, mob, amp, d_en[W:], stream=sptr)
     end.record(stream)
     torch.cuda.synchronize()
-    face_ms, face_n = s.profile_end()
+    (face_ms, gather_ms, rs_ms), (face_n, gather_n, rs_n) = s.profile_end(3)
     ms = start.elapsed_time(end)
     energies = d_en.cpu().numpy()
     if not np.all(np.isfinite(energies)):
@@ -186,6 +186,20 @@ def run_leg(args, w, c0, precision, d_xi, dev, torch, D=1e-12, dt=1e-9):
         "bd_steps_per_s": K / (ms / 1e3),
         "ms_per_step": ms / K,
         "face_kernel_us": 1e3 * face_ms / face_n,
+        "gather_us": 1e3 * gather_ms / max(gather_n, 1),
+        "restraint_us": 1e3 * rs_ms / rs_n if rs_n else None,
+        # the Brownian update itself: per vertex r read and written, the
+        # gradient and the deviate read (96 B algorithmic, src/bd.C:1035); the
+        # gather also streams the (face, slot) gradient records back
+        "bd_update": {
+            "algorithmic_bytes": 96 * nv,
+            "record_bytes": 3 * (4 if precision == "mixed" else 8) * sum(c["nf"] * c["ncp"] for c in classes),
+            "achieved_GB_per_s": 96 * nv / (gather_ms / max(gather_n, 1) / 1e3) / 1e9,
+            "frac_of_8TB_s": 96 * nv / (gather_ms / max(gather_n, 1) / 1e3) / 8e12,
+            "with_records_frac_of_8TB_s": (96 * nv + 3 * (4 if precision == "mixed" else 8) *
+                                           sum(c["nf"] * c["ncp"] for c in classes)) /
+                                          (gather_ms / max(gather_n, 1) / 1e3) / 8e12,
+        },
         "launch": {"point_split": split, "flags": flags},
         "autotune": tuned,
         "flop_per_launch": sum(c["nf"] * face_flops(c["ncp"], c["npt"]) for c in classes),
@@ -664,7 +678,7 @@ def main():
                    "D_m2_per_s": 1e-12, "dt_s": 1e-9},
         "bd_steps_per_s": leg64["bd_steps_per_s"],
         "fp64": leg64,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+        "roofline": {"bound": "valu-fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
                      "traffic": measured_traffic(kname),
                      "traffic_unit": "bytes/launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",

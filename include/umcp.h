@@ -445,10 +445,12 @@ void umcp_server_destroy(umcp_server *sv);
 int umcp_set_launch(umcp_surface *s, int point_split, int flags);
 int umcp_get_launch(const umcp_surface *s, int *point_split, int *flags);
 
-/* per-launch timing of the face kernel with HIP events recorded on the
- * stream each launch uses (no reference counterpart; benchmark support):
- * begin, run work, then end returns the summed milliseconds and the number
- * of launches (all face classes share one launch, reported as entry 0) */
+/* per-launch timing with HIP events recorded on the stream each launch uses
+ * (no reference counterpart; benchmark support): begin, run work, then end
+ * returns the summed milliseconds and the number of launches per entry
+ * (max_cls entries): 0 the face kernel (all face classes share one launch),
+ * 1 the vertex gather (gradient or Brownian update), 2 the restraint
+ * pre-pass and finalize (surfaces with global restraints) */
 int umcp_profile_begin(umcp_surface *s);
 /* while profiling, time one evaluation's face launch in every `every`
  * (default 1), so the events' own cost stays off the other steps */

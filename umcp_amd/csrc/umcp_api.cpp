@@ -258,12 +258,19 @@ int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs 
     fb.nitems = s->npartial;
     fb.r = d_r;
     fb.step_ctr = step_ctr;
-    if (rson) {
-        hipError_t e = launch_restraint_prepass(fb, s->d_zero, 1, s->prm.precision == UMCP_PREC_MIXED, st);
-        if (e != hipSuccess) return hip_fail(s, e, "restraint kernel launch");
-    }
     hipEvent_t ev[2] = {nullptr, nullptr};
     const bool timed = s->prof_on && (s->prof_tick++ % s->prof_every) == 0;
+    if (rson) {
+        // timed evaluations time the restraint pre-pass and finalize too (entry 2)
+        hipEvent_t evr[2] = {nullptr, nullptr};
+        if (timed) {
+            if (int rc = prof_pair(s, 2, evr)) return rc;
+            (void)hipEventRecord(evr[0], st);
+        }
+        hipError_t e = launch_restraint_prepass(fb, s->d_zero, 1, s->prm.precision == UMCP_PREC_MIXED, st);
+        if (e != hipSuccess) return hip_fail(s, e, "restraint kernel launch");
+        if (timed) (void)hipEventRecord(evr[1], st);
+    }
     if (timed) {
         int rc = prof_pair(s, 0, ev);
         if (rc) return rc;
@@ -280,8 +287,14 @@ int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs 
     ga.partial = s->d_partial;
     ga.npartial = s->npartial;
     if (rson) ga.rs = s->d_rs;
+    hipEvent_t evg[2] = {nullptr, nullptr};
+    if (timed) {   // the vertex gather (entry 1)
+        if (int rc = prof_pair(s, 1, evg)) return rc;
+        (void)hipEventRecord(evg[0], st);
+    }
     e = launch_vertex_gather(ga, st);
     if (e != hipSuccess) return hip_fail(s, e, "gather kernel launch");
+    if (timed) (void)hipEventRecord(evg[1], st);
     return UMCP_OK;
 }
 

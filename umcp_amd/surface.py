@@ -545,12 +545,16 @@ class Surface:
         self._check(self._L.umcp_profile_sample(self._h, int(every)), "profile_sample")
         self._check(self._L.umcp_profile_begin(self._h), "profile_begin")
 
-    def profile_end(self):
-        """(summed milliseconds, launches) of the face kernel since profile_begin."""
-        ms = np.zeros(1)
-        cnt = (C.c_int * 1)()
-        self._check(self._L.umcp_profile_end(self._h, 1, _dp(ms), cnt), "profile_end")
-        return float(ms[0]), int(cnt[0])
+    def profile_end(self, entries: int = 1):
+        """(summed milliseconds, launches) of the face kernel since
+        profile_begin; with entries=3 two lists: face kernel, vertex gather,
+        restraint pre-pass and finalize."""
+        ms = np.zeros(entries)
+        cnt = (C.c_int * entries)()
+        self._check(self._L.umcp_profile_end(self._h, entries, _dp(ms), cnt), "profile_end")
+        if entries == 1:
+            return float(ms[0]), int(cnt[0])
+        return [float(x) for x in ms], [int(x) for x in cnt]
 
     def face_points(self, f: int):
         ncp = C.c_int()
