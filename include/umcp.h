@@ -314,6 +314,22 @@ int umcp_point_coeffs(umcp_surface *s, int n, const int *faces, const double *uv
  * 3878).  The Monge-gauge form (do_monge) is not provided. */
 int umcp_particle_energy(umcp_surface *s, const double *r, int n, const int *faces, const double *uv,
                          const double *c0, const double *area, double *out);
+/* The particle terms of surface::grad for n particles on the limit surface
+ * (particle i on face faces[i] at uv[2i], uv[2i+1], curvature preference
+ * c0[i], area area[i]), adding into g (3nv+3, with dE/dalpha) and g_puv (2n):
+ * surface::pgrad (src/grad.C:3487-5336) -- the gradient of
+ * 1/2 kc area (c1 + c2 - c0)^2 at each particle in the vertices, alpha and the
+ * particle's (u, v) -- and the term grad and igrad add on every face
+ * carrying particles whose c0 differs from the face's by 1e-7 or more
+ * (src/grad.C:445-446, 458-459, 6819-6820): per Gauss point dE/d(c1 + c2) =
+ * -P dAf kc (c1 + c2 - c0_face) / A with P their summed area, dAf = g0 w and
+ * A = sum g w of the current surface.  With umcp_grad this gives the
+ * reference's grad(r, g, puv, pg) of a surface carrying particles.  The
+ * extraordinary vertex itself and points outside an irregular face's
+ * sub-patches return UMCP_ERR_ARG naming the particle.  Needs umcp_plan and
+ * umcp_setg0; host arrays; synchronous. */
+int umcp_particle_grad(umcp_surface *s, const double *r, int n, const int *faces, const double *uv,
+                       const double *c0, const double *area, double *g, double *g_puv);
 /* One step of surface::nextFace (src/uv_map.C:1695-1961) for each point:
  * moves (u, v) by (du, dv) within the face, or to the border it reaches
  * first and onto the face across it (faces[i], uv updated), leaving the rest

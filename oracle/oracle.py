@@ -101,6 +101,12 @@ def lib():
         L.orc_particle_energy.argtypes = [vp, dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                           C.c_double, dp]
         L.orc_particle_energy.restype = C.c_int
+        L.orc_particle_grad.argtypes = [vp, dp, C.c_int, ip, dp, dp, dp, dp, dp]
+        L.orc_particle_grad.restype = C.c_int
+        L.orc_particle_curvature_energy.argtypes = [vp, dp, C.c_int, ip, dp, dp, dp]
+        L.orc_particle_curvature_energy.restype = C.c_double
+        L.orc_particle_density_frozen.argtypes = [vp, dp, dp, C.c_int, ip, dp, dp]
+        L.orc_particle_density_frozen.restype = C.c_double
         _LIB = L
     return _LIB
 
@@ -325,6 +331,38 @@ class OracleSurface:
         if lib().orc_face_density(self._h, int(f), _p(r), C.byref(num), C.byref(area)):
             raise ValueError("oracle: face out of range")
         return num.value, area.value
+
+    @staticmethod
+    def _particles(faces, uv, c0, area):
+        faces = np.ascontiguousarray(faces, dtype=np.int32)
+        uv = np.ascontiguousarray(uv, dtype=np.float64).reshape(-1)
+        c0 = np.ascontiguousarray(c0, dtype=np.float64)
+        area = np.ascontiguousarray(area, dtype=np.float64)
+        return faces, uv, c0, area, faces.ctypes.data_as(C.POINTER(C.c_int))
+
+    def particle_grad(self, r: np.ndarray, faces, uv, c0, area, g=None, g_puv=None):
+        """pgrad plus grad's particle-area terms: (g, g_puv), accumulated into
+        the given arrays (zeros when None)."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        faces, uv, c0, area, fp = self._particles(faces, uv, c0, area)
+        n = faces.size
+        g = np.zeros_like(r) if g is None else g
+        g_puv = np.zeros(2 * n) if g_puv is None else g_puv
+        rc = lib().orc_particle_grad(self._h, _p(r), n, fp, _p(uv), _p(c0), _p(area), _p(g), _p(g_puv))
+        if rc:
+            raise ValueError("oracle: particle %d outside the reference's domain" % (-1 - rc))
+        return g, g_puv
+
+    def particle_curvature_energy(self, r: np.ndarray, faces, uv, c0, area) -> float:
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        faces, uv, c0, area, fp = self._particles(faces, uv, c0, area)
+        return lib().orc_particle_curvature_energy(self._h, _p(r), faces.size, fp, _p(uv), _p(c0), _p(area))
+
+    def particle_density_frozen(self, r_star: np.ndarray, r: np.ndarray, faces, c0, area) -> float:
+        r_star = np.ascontiguousarray(r_star, dtype=np.float64)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        faces, _, c0, area, fp = self._particles(faces, np.zeros(2 * len(faces)), c0, area)
+        return lib().orc_particle_density_frozen(self._h, _p(r_star), _p(r), faces.size, fp, _p(c0), _p(area))
 
     def particle_energy(self, r: np.ndarray, faces, uv, c0, area) -> np.ndarray:
         """penergy's term of every particle: 1/2 kc a (c1 + c2 - c0)^2 - a rho_f."""

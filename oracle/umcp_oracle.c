@@ -623,28 +623,44 @@ static void box_monomials(int which, double m[5][5])
     }
 }
 
-/* out[d][y]: d = 0 value, 1 u, 2 v, 3 uu, 4 uv, 5 vv at (u,v) in slot order */
-void orc_basis(double u, double v, double out[6][12])
+/* out[d][y]: d = 0 value, 1 u, 2 v, 3 uu, 4 uv, 5 vv, 6 uuu, 7 uuv, 8 uvv,
+ * 9 vvv at (u,v) in slot order (the third derivatives are what pgrad's
+ * (u, v) gradient needs, src/grad.C:3620-3700) */
+void orc_basis10(double u, double v, double out[10][12])
 {
     double up[5] = {1, u, u * u, u * u * u, u * u * u * u};
     double vp[5] = {1, v, v * v, v * v * v, v * v * v * v};
+    /* derivative orders (du, dv) of the ten rows */
+    static const int DU[10] = {0, 1, 0, 2, 1, 0, 3, 2, 1, 0};
+    static const int DV[10] = {0, 0, 1, 0, 1, 2, 0, 1, 2, 3};
     for (int y = 0; y < 12; y++) {
         double m[5][5];
         box_monomials(CEFF[y] - 1, m);
-        double s[6] = {0, 0, 0, 0, 0, 0};
+        double s[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (int a = 0; a <= 4; a++)
             for (int b = 0; a + b <= 4; b++) {
                 double c = m[a][b] / 12.0;
                 if (c == 0.0) continue;
-                s[0] += c * up[a] * vp[b];
-                if (a >= 1) s[1] += c * a * up[a - 1] * vp[b];
-                if (b >= 1) s[2] += c * b * up[a] * vp[b - 1];
-                if (a >= 2) s[3] += c * a * (a - 1) * up[a - 2] * vp[b];
-                if (a >= 1 && b >= 1) s[4] += c * a * b * up[a - 1] * vp[b - 1];
-                if (b >= 2) s[5] += c * b * (b - 1) * up[a] * vp[b - 2];
+                for (int d = 0; d < 10; d++) {
+                    int i = DU[d], j = DV[d];
+                    if (a < i || b < j) continue;
+                    double fa = 1.0, fb = 1.0;
+                    for (int t = 0; t < i; t++) fa *= (a - t);
+                    for (int t = 0; t < j; t++) fb *= (b - t);
+                    s[d] += c * fa * fb * up[a - i] * vp[b - j];
+                }
             }
-        for (int d = 0; d < 6; d++) out[d][y] = s[d];
+        for (int d = 0; d < 10; d++) out[d][y] = s[d];
     }
+}
+
+/* out[d][y]: d = 0 value, 1 u, 2 v, 3 uu, 4 uv, 5 vv at (u,v) in slot order */
+void orc_basis(double u, double v, double out[6][12])
+{
+    double b[10][12];
+    orc_basis10(u, v, b);
+    for (int d = 0; d < 6; d++)
+        for (int y = 0; y < 12; y++) out[d][y] = b[d][y];
 }
 
 /* ------------------------------------------------------------------------ */
@@ -695,7 +711,7 @@ static void subdivision_matrix(int val, double *A)
 
 /* out is [6][val+6]: derivative tables r_w, r_u, r_v, r_uu, r_uv, r_vv at the
  * original parameters (u, v) of an irregular face (src/interpLib.C:5687-5918) */
-int orc_irregular_table(int val, double fu, double fv, double *out)
+static int irregular_table_nd(int val, double fu, double fv, int nd, double *out)
 {
     int N = val + 6;
     if (val < 3 || N > OMAXCP) return -1;
@@ -747,17 +763,24 @@ int orc_irregular_table(int val, double fu, double fv, double *out)
             for (int e = 0; e < N; e++) acc += Atot[f * N + e] * P[e * N + ii];
             Ause[f * N + ii] = acc;
         }
-    double B[6][12];
-    orc_basis(fu, fv, B);
-    double sc[6] = {1.0, scale_u, scale_v, scale_u * scale_u, scale_u * scale_v, scale_v * scale_v};
+    double B[10][12];
+    orc_basis10(fu, fv, B);
+    double sc[10] = {1.0, scale_u, scale_v, scale_u * scale_u, scale_u * scale_v, scale_v * scale_v,
+                     scale_u * scale_u * scale_u, scale_u * scale_u * scale_v, scale_u * scale_v * scale_v,
+                     scale_v * scale_v * scale_v};
     for (int x = 0; x < N; x++)
-        for (int d = 0; d < 6; d++) {
+        for (int d = 0; d < nd; d++) {
             double acc = 0.0;
             for (int y = 0; y < 12; y++) acc += Ause[pc[y] * N + x] * B[d][y] * sc[d];
             out[d * N + x] = acc;
         }
     free(Atot); free(P); free(T); free(Ause);
     return 0;
+}
+
+int orc_irregular_table(int val, double fu, double fv, double *out)
+{
+    return irregular_table_nd(val, fu, fv, 6, out);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -2408,4 +2431,364 @@ int orc_point_geometry(const omesh *m, const double *r, int f, double u, double 
     out[13] = v2[0] / l2;
     out[14] = v2[1] / l2;
     return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Particle terms of the gradient: surface::pgrad (src/grad.C:3487-5336) and */
+/* the p_face_area terms grad / igrad add on faces carrying particles       */
+/* (src/grad.C:445-446, 458-459, 6819-6820)                                 */
+/* ------------------------------------------------------------------------ */
+
+/* the face's ten derivative tables at (u, v) (position, two first, three
+ * second and four third derivatives, in the face's own parameters) and its
+ * control points; irregular faces through the sub-patch map, derivative rows
+ * scaled by the sub-patch factor as pgrad does with get_map_transform
+ * (src/grad.C:4417-4418, 4614-4655) */
+static int point_tables10(const omesh *m, int f, double u, double v, double t[10][OMAXCP], int *ncp, int *cp,
+                          double *pbc)
+{
+    int npt;
+    double dudv;
+    const oform *F = face_forms(m, f, &npt, &dudv);
+    *ncp = F[0].ncp;
+    memcpy(cp, F[0].cp, sizeof(int) * (size_t)F[0].ncp);
+    memcpy(pbc, F[0].pbc, sizeof(double) * 3 * (size_t)F[0].ncp);
+    if (f < m->nfr) {
+        double B[10][12];
+        orc_basis10(u, v, B);
+        for (int d = 0; d < 10; d++)
+            for (int x = 0; x < 12; x++) t[d][x] = B[d][x];
+        return 0;
+    }
+    if (irr_locate(u, v) != 0) return -1;   /* the vertex itself or outside */
+    int val = m->val[m->face_owner[f]];
+    double tab[10 * OMAXCP];
+    if (irregular_table_nd(val, u, v, 10, tab) != 0) return -1;
+    for (int d = 0; d < 10; d++)
+        for (int x = 0; x < *ncp; x++) t[d][x] = tab[d * *ncp + x];
+    return 0;
+}
+
+/* The backward chain of grad's energy terms, E(c1, c2, g) -> (a, b, c, d, g)
+ * -> (nsuu, nsuv, nsvv, RuRu, RuRv, RvRv) -> (Ru, Rv, Suu, Suv, Svv)
+ * (src/grad.C:344-494): dE[0..4] = dE/dRu, dE/dRv, dE/dSuu, dE/dSuv, dE/dSvv */
+static void shape_back(const double *Ru, const double *Rv, const double *Suu, const double *Suv, const double *Svv,
+                       double de_c1, double de_c2, double de_g, double dE[5][3])
+{
+    double nrm[3];
+    cross3(Ru, Rv, nrm);
+    normalize3(nrm);
+    double RuRu = Ru[0] * Ru[0] + Ru[1] * Ru[1] + Ru[2] * Ru[2];
+    double RuRv = Ru[0] * Rv[0] + Ru[1] * Rv[1] + Ru[2] * Rv[2];
+    double RvRv = Rv[0] * Rv[0] + Rv[1] * Rv[1] + Rv[2] * Rv[2];
+    double g = sqrt(RuRu * RvRv - RuRv * RuRv);
+    double nsuu = Suu[0] * nrm[0] + Suu[1] * nrm[1] + Suu[2] * nrm[2];
+    double nsuv = Suv[0] * nrm[0] + Suv[1] * nrm[1] + Suv[2] * nrm[2];
+    double nsvv = Svv[0] * nrm[0] + Svv[1] * nrm[1] + Svv[2] * nrm[2];
+    double g2 = g * g, g3 = g * g * g;
+    double a = 1.0 / g2 * (nsuu * RvRv - nsuv * RuRv);
+    double b = (1.0 / g2) * (nsuv * RvRv - nsvv * RuRv);
+    double c = 1.0 / g2 * (nsuv * RuRu - nsuu * RuRv);
+    double d = (1.0 / g2) * (nsvv * RuRu - nsuv * RuRv);
+    double S = sqrt(SAFETY_ + a * a + 4 * b * c - 2 * a * d + d * d);
+    double dc1_a = -0.5 * (1 - (2 * a - 2 * d) / (2. * S));
+    double dc1_b = c / S, dc1_c = b / S;
+    double dc1_d = -0.5 * (1 - (-2 * a + 2 * d) / (2. * S));
+    double dc2_a = -0.5 * (1 + (2 * a - 2 * d) / (2. * S));
+    double dc2_b = -c / S, dc2_c = -b / S;
+    double dc2_d = -0.5 * (1 + (-2 * a + 2 * d) / (2. * S));
+    double de_a = de_c1 * dc1_a + de_c2 * dc2_a;
+    double de_b = de_c1 * dc1_b + de_c2 * dc2_b;
+    double de_c = de_c1 * dc1_c + de_c2 * dc2_c;
+    double de_d = de_c1 * dc1_d + de_c2 * dc2_d;
+    de_g += de_a * (-2 * (-(nsuv * RuRv) + nsuu * RvRv) / g3);
+    de_g += de_b * (-2 * (-(nsvv * RuRv) + nsuv * RvRv) / g3);
+    de_g += de_c * (-2 * (nsuv * RuRu - nsuu * RuRv) / g3);
+    de_g += de_d * (-2 * (nsvv * RuRu - nsuv * RuRv) / g3);
+    double de_nsuu = de_a * (RvRv / g2) + de_c * (-RuRv / g2);
+    double de_nsuv = de_a * (-RuRv / g2) + de_b * (RvRv / g2) + de_c * (RuRu / g2) + de_d * (-RuRv / g2);
+    double de_nsvv = de_b * (-RuRv / g2) + de_d * (RuRu / g2);
+    double de_RuRu = de_c * (nsuv / g2) + de_d * (nsvv / g2);
+    double de_RvRv = de_a * (nsuu / g2) + de_b * (nsuv / g2);
+    double de_RuRv = de_a * (-nsuv / g2) + de_b * (-nsvv / g2) + de_c * (-nsuu / g2) + de_d * (-nsuv / g2);
+    de_RuRu += de_g * RvRv / (2. * sqrt(-RuRv * RuRv + RuRu * RvRv));
+    de_RuRv += de_g * (-(RuRv / sqrt(-RuRv * RuRv + RuRu * RvRv)));
+    de_RvRv += de_g * RuRu / (2. * sqrt(-RuRv * RuRv + RuRu * RvRv));
+    double de_n[3];
+    for (int k = 0; k < 3; k++) {
+        de_n[k] = de_nsuu * Suu[k] + de_nsuv * Suv[k] + de_nsvv * Svv[k];
+        dE[0][k] = 2 * Ru[k] * de_RuRu + Rv[k] * de_RuRv;
+        dE[1][k] = 2 * Rv[k] * de_RvRv + Ru[k] * de_RuRv;
+        dE[2][k] = de_nsuu * nrm[k];
+        dE[3][k] = de_nsuv * nrm[k];
+        dE[4][k] = de_nsvv * nrm[k];
+    }
+    double N[3];
+    cross3(Ru, Rv, N);
+    double n2 = N[0] * N[0] + N[1] * N[1] + N[2] * N[2];
+    double fac = pow(n2, 1.5);
+    double J[3][3];
+    for (int i2 = 0; i2 < 3; i2++)
+        for (int j2 = 0; j2 < 3; j2++) J[i2][j2] = ((i2 == j2) * n2 - N[i2] * N[j2]) / fac;
+    double de_N[3];
+    for (int j2 = 0; j2 < 3; j2++) de_N[j2] = de_n[0] * J[0][j2] + de_n[1] * J[1][j2] + de_n[2] * J[2][j2];
+    double t1[3], t2[3];
+    cross3(Rv, de_N, t1);
+    cross3(de_N, Ru, t2);
+    for (int k = 0; k < 3; k++) {
+        dE[0][k] += t1[k];
+        dE[1][k] += t2[k];
+    }
+}
+
+/* pgrad for one particle on face f at (u, v): E = 1/2 kc pa (c1 + c2 - pc0)^2
+ * at the point; adds dE/dr (and dE/dalpha) into g and dE/du, dE/dv into pg
+ * (src/grad.C:3700-3935 regular, 4614-5335 irregular) */
+static int particle_pgrad(omesh *m, const double *r, int f, double u, double v, double pc0, double pa, double *g,
+                          double *pg)
+{
+    int nv = m->nv, ncp, cp[OMAXCP];
+    double t[10][OMAXCP], pbc[3 * OMAXCP];
+    double al[3] = {r[3 * nv], r[3 * nv + 1], r[3 * nv + 2]};
+    if (point_tables10(m, f, u, v, t, &ncp, cp, pbc) != 0) return -1;
+    double X[10][3];
+    memset(X, 0, sizeof X);
+    for (int x = 0; x < ncp; x++)
+        for (int c = 0; c < 3; c++) {
+            double q = al[c] * (r[3 * cp[x] + c] + pbc[3 * x + c]);
+            for (int d = 0; d < 10; d++) X[d][c] += t[d][x] * q;
+        }
+    double *Ru = X[1], *Rv = X[2], *Suu = X[3], *Suv = X[4], *Svv = X[5];
+    double *Suuu = X[6], *Suuv = X[7], *Suvv = X[8], *Svvv = X[9];
+    double nrm[3];
+    cross3(Ru, Rv, nrm);
+    normalize3(nrm);
+    double RuRu = Ru[0] * Ru[0] + Ru[1] * Ru[1] + Ru[2] * Ru[2];
+    double RuRv = Ru[0] * Rv[0] + Ru[1] * Rv[1] + Ru[2] * Rv[2];
+    double RvRv = Rv[0] * Rv[0] + Rv[1] * Rv[1] + Rv[2] * Rv[2];
+    double g_ = sqrt(RuRu * RvRv - RuRv * RuRv);
+    double nsuu = Suu[0] * nrm[0] + Suu[1] * nrm[1] + Suu[2] * nrm[2];
+    double nsuv = Suv[0] * nrm[0] + Suv[1] * nrm[1] + Suv[2] * nrm[2];
+    double nsvv = Svv[0] * nrm[0] + Svv[1] * nrm[1] + Svv[2] * nrm[2];
+    double g2 = g_ * g_, g3 = g_ * g_ * g_;
+    double a = 1.0 / g2 * (nsuu * RvRv - nsuv * RuRv);
+    double b = (1.0 / g2) * (nsuv * RvRv - nsvv * RuRv);
+    double c = 1.0 / g2 * (nsuv * RuRu - nsuu * RuRv);
+    double d = (1.0 / g2) * (nsvv * RuRu - nsuv * RuRv);
+    double S = sqrt(SAFETY_ + a * a + 4 * b * c - 2 * a * d + d * d);
+    double e1 = -0.5 * (a + d - S), e2 = -0.5 * (a + d + S);
+    /* en = 1/2 kc pa (c1 + c2 - pc0)^2 (src/grad.C:3837-3840) */
+    double de_c1 = m->prm.kc * pa * (e1 + e2 - pc0);
+    double de_c2 = m->prm.kc * pa * (e1 + e2 - pc0);
+    double dE[5][3];
+    shape_back(Ru, Rv, Suu, Suv, Svv, de_c1, de_c2, 0.0, dE);
+    for (int x = 0; x < ncp; x++)
+        for (int k = 0; k < 3; k++) {
+            double s = dE[0][k] * t[1][x] + dE[1][k] * t[2][x] + dE[2][k] * t[3][x] + dE[3][k] * t[4][x] +
+                       dE[4][k] * t[5][x];
+            g[3 * cp[x] + k] += s * al[k];
+            g[3 * nv + k] += s * (r[3 * cp[x] + k] + pbc[3 * x + k]);
+        }
+    /* the (u, v) gradient (src/grad.C:3856-3933): every quantity's
+     * derivative along u and v through the third derivatives */
+    double N[3];
+    cross3(Ru, Rv, N);
+    double n2 = N[0] * N[0] + N[1] * N[1] + N[2] * N[2];
+    double fac = pow(n2, 1.5);
+    double Jn[3][3];   /* d nrm_i / d N_j */
+    for (int i2 = 0; i2 < 3; i2++)
+        for (int j2 = 0; j2 < 3; j2++) Jn[i2][j2] = ((i2 == j2) * n2 - N[i2] * N[j2]) / fac;
+    /* dN/du = Suu x Rv + Ru x Suv, dN/dv = Suv x Rv + Ru x Svv */
+    double a1[3], a2[3], dNu[3], dNv[3];
+    cross3(Suu, Rv, a1);
+    cross3(Ru, Suv, a2);
+    for (int k = 0; k < 3; k++) dNu[k] = a1[k] + a2[k];
+    cross3(Suv, Rv, a1);
+    cross3(Ru, Svv, a2);
+    for (int k = 0; k < 3; k++) dNv[k] = a1[k] + a2[k];
+    double dnu[3], dnv[3];
+    for (int i2 = 0; i2 < 3; i2++) {
+        dnu[i2] = Jn[i2][0] * dNu[0] + Jn[i2][1] * dNu[1] + Jn[i2][2] * dNu[2];
+        dnv[i2] = Jn[i2][0] * dNv[0] + Jn[i2][1] * dNv[1] + Jn[i2][2] * dNv[2];
+    }
+#define DOT(p, q) ((p)[0] * (q)[0] + (p)[1] * (q)[1] + (p)[2] * (q)[2])
+    double d_nsuu_u = DOT(Suuu, nrm) + DOT(Suu, dnu), d_nsuu_v = DOT(Suuv, nrm) + DOT(Suu, dnv);
+    double d_nsuv_u = DOT(Suuv, nrm) + DOT(Suv, dnu), d_nsuv_v = DOT(Suvv, nrm) + DOT(Suv, dnv);
+    double d_nsvv_u = DOT(Suvv, nrm) + DOT(Svv, dnu), d_nsvv_v = DOT(Svvv, nrm) + DOT(Svv, dnv);
+    double d_RuRu_u = 2 * DOT(Ru, Suu), d_RuRu_v = 2 * DOT(Ru, Suv);
+    double d_RuRv_u = DOT(Ru, Suv) + DOT(Rv, Suu), d_RuRv_v = DOT(Ru, Svv) + DOT(Rv, Suv);
+    double d_RvRv_u = 2 * DOT(Rv, Suv), d_RvRv_v = 2 * DOT(Rv, Svv);
+#undef DOT
+    double d_g_RuRu = RvRv / (2. * g_), d_g_RuRv = -(RuRv / g_), d_g_RvRv = RuRu / (2. * g_);
+    double d_g_u = d_g_RuRu * d_RuRu_u + d_g_RuRv * d_RuRv_u + d_g_RvRv * d_RvRv_u;
+    double d_g_v = d_g_RuRu * d_RuRu_v + d_g_RuRv * d_RuRv_v + d_g_RvRv * d_RvRv_v;
+    /* partials of the shape operator entries (src/grad.C:364-386) */
+    double a_g = -2 * (-(nsuv * RuRv) + nsuu * RvRv) / g3, a_nsuu = RvRv / g2, a_RvRv = nsuu / g2;
+    double a_nsuv = -(RuRv / g2), a_RuRv = -(nsuv / g2);
+    double b_g = -2 * (-(nsvv * RuRv) + nsuv * RvRv) / g3, b_nsvv = -(RuRv / g2), b_RvRv = nsuv / g2;
+    double b_nsuv = RvRv / g2, b_RuRv = -(nsvv / g2);
+    double c_g = -2 * (nsuv * RuRu - nsuu * RuRv) / g3, c_nsuu = -(RuRv / g2), c_RuRu = nsuv / g2;
+    double c_nsuv = RuRu / g2, c_RuRv = -(nsuu / g2);
+    double d_g = -2 * (nsvv * RuRu - nsuv * RuRv) / g3, d_nsvv = RuRu / g2, d_RuRu = nsvv / g2;
+    double d_nsuv = -(RuRv / g2), d_RuRv = -(nsuv / g2);
+    double da[2], db[2], dc[2], dd[2];
+    double dnsuu[2] = {d_nsuu_u, d_nsuu_v}, dnsuv[2] = {d_nsuv_u, d_nsuv_v}, dnsvv[2] = {d_nsvv_u, d_nsvv_v};
+    double dRuRu[2] = {d_RuRu_u, d_RuRu_v}, dRuRv[2] = {d_RuRv_u, d_RuRv_v}, dRvRv[2] = {d_RvRv_u, d_RvRv_v};
+    double dg_[2] = {d_g_u, d_g_v};
+    for (int q = 0; q < 2; q++) {
+        da[q] = a_nsuu * dnsuu[q] + a_nsuv * dnsuv[q] + a_RuRv * dRuRv[q] + a_RvRv * dRvRv[q] + a_g * dg_[q];
+        db[q] = b_nsvv * dnsvv[q] + b_nsuv * dnsuv[q] + b_RuRv * dRuRv[q] + b_RvRv * dRvRv[q] + b_g * dg_[q];
+        dc[q] = c_nsuu * dnsuu[q] + c_nsuv * dnsuv[q] + c_RuRv * dRuRv[q] + c_RuRu * dRuRu[q] + c_g * dg_[q];
+        dd[q] = d_nsvv * dnsvv[q] + d_nsuv * dnsuv[q] + d_RuRv * dRuRv[q] + d_RuRu * dRuRu[q] + d_g * dg_[q];
+    }
+    double dc1_a = -0.5 * (1 - (2 * a - 2 * d) / (1e-100 + 2. * S));
+    double dc1_b = -(-1. * c) / S, dc1_c = -(-1. * b) / S;
+    double dc1_d = -0.5 * (1 - (-2 * a + 2 * d) / (1e-100 + 2. * S));
+    double dc2_a = -0.5 * (1 + (2 * a - 2 * d) / (1e-100 + 2. * S));
+    double dc2_b = -(1. * c) / S, dc2_c = -(1. * b) / S;
+    double dc2_d = -0.5 * (1 + (-2 * a + 2 * d) / (1e-100 + 2. * S));
+    for (int q = 0; q < 2; q++) {
+        double dc1 = dc1_a * da[q] + dc1_b * db[q] + dc1_c * dc[q] + dc1_d * dd[q];
+        double dc2 = dc2_a * da[q] + dc2_b * db[q] + dc2_c * dc[q] + dc2_d * dd[q];
+        pg[q] += de_c1 * dc1 + de_c2 * dc2;
+    }
+    return 0;
+}
+
+/* grad's term for a face carrying particle area pfa (particles whose c0
+ * differs from the face's by 1e-7 or more): per Gauss point dE/dc1 = dE/dc2 =
+ * -pfa dAf kc (c1 + c2 - c0) / face_area with dAf = g0 w and face_area = sum
+ * g w of the current surface (src/grad.C:93-103, 212-224, 445-446, 458-459;
+ * igrad 6477-6487, 6584, 6819-6820) */
+static void face_particle_density_grad(omesh *m, int f, double pfa, const double *r, double *g)
+{
+    int npt, nv = m->nv;
+    double dudv;
+    oform *F = face_forms(m, f, &npt, &dudv);
+    double al[3] = {r[3 * nv], r[3 * nv + 1], r[3 * nv + 2]};
+    double face_area = 0.0;
+    for (int p = 0; p < npt; p++) {
+        double X[6][3];
+        contract(m, F + p, r, X);
+        double RuRu = X[1][0] * X[1][0] + X[1][1] * X[1][1] + X[1][2] * X[1][2];
+        double RuRv = X[1][0] * X[2][0] + X[1][1] * X[2][1] + X[1][2] * X[2][2];
+        double RvRv = X[2][0] * X[2][0] + X[2][1] * X[2][1] + X[2][2] * X[2][2];
+        face_area += sqrt(RuRu * RvRv - RuRv * RuRv) * F[p].weight;
+    }
+    const double c0 = m->face_c0[f];
+    for (int p = 0; p < npt; p++) {
+        double X[6][3];
+        contract(m, F + p, r, X);
+        double *Ru = X[1], *Rv = X[2], *Suu = X[3], *Suv = X[4], *Svv = X[5];
+        double nrm[3];
+        cross3(Ru, Rv, nrm);
+        normalize3(nrm);
+        double RuRu = Ru[0] * Ru[0] + Ru[1] * Ru[1] + Ru[2] * Ru[2];
+        double RuRv = Ru[0] * Rv[0] + Ru[1] * Rv[1] + Ru[2] * Rv[2];
+        double RvRv = Rv[0] * Rv[0] + Rv[1] * Rv[1] + Rv[2] * Rv[2];
+        double gg = sqrt(RuRu * RvRv - RuRv * RuRv);
+        double nsuu = Suu[0] * nrm[0] + Suu[1] * nrm[1] + Suu[2] * nrm[2];
+        double nsuv = Suv[0] * nrm[0] + Suv[1] * nrm[1] + Suv[2] * nrm[2];
+        double nsvv = Svv[0] * nrm[0] + Svv[1] * nrm[1] + Svv[2] * nrm[2];
+        double a = 1.0 / (gg * gg) * (nsuu * RvRv - nsuv * RuRv);
+        double d = (1.0 / (gg * gg)) * (nsvv * RuRu - nsuv * RuRv);
+        double sum = -(a + d);   /* e1 + e2 */
+        double dAf = F[p].g0 * F[p].weight;
+        double de = -pfa * dAf * m->prm.kc * (sum - c0) / face_area;
+        double dE[5][3];
+        shape_back(Ru, Rv, Suu, Suv, Svv, de, de, 0.0, dE);
+        for (int x = 0; x < F[p].ncp; x++)
+            for (int k = 0; k < 3; k++) {
+                double s = dE[0][k] * F[p].t[1][x] + dE[1][k] * F[p].t[2][x] + dE[2][k] * F[p].t[3][x] +
+                           dE[3][k] * F[p].t[4][x] + dE[4][k] * F[p].t[5][x];
+                g[3 * F[p].cp[x] + k] += s * al[k];
+                g[3 * nv + k] += s * (r[3 * F[p].cp[x] + k] + F[p].pbc[3 * x + k]);
+            }
+    }
+}
+
+/* the particle area each face carries in grad's density term */
+static double *particle_face_area(const omesh *m, int n, const int *faces, const double *c0, const double *area)
+{
+    int nf = m->nfr + m->nfi;
+    double *pfa = calloc((size_t)nf, sizeof(double));
+    for (int i = 0; i < n; i++)
+        if (fabs(c0[i] - m->face_c0[faces[i]]) >= 1e-7) pfa[faces[i]] += area[i];
+    return pfa;
+}
+
+int orc_particle_grad(omesh *m, const double *r, int n, const int *faces, const double *uv, const double *c0,
+                      const double *area, double *g, double *g_puv)
+{
+    int nf = m->nfr + m->nfi;
+    for (int i = 0; i < n; i++)
+        if (faces[i] < 0 || faces[i] >= nf) return -1 - i;
+    double *pfa = particle_face_area(m, n, faces, c0, area);
+    for (int f = 0; f < nf; f++)
+        if (pfa[f] != 0.0) face_particle_density_grad(m, f, pfa[f], r, g);
+    free(pfa);
+    for (int i = 0; i < n; i++)
+        if (particle_pgrad(m, r, faces[i], uv[2 * i], uv[2 * i + 1], c0[i], area[i], g, g_puv + 2 * i) != 0)
+            return -1 - i;
+    return 0;
+}
+
+/* sum over the particles of 1/2 kc pa (c1 + c2 - pc0)^2 at each particle, the
+ * curvatures from orc_point_geometry: the energy whose gradient pgrad is */
+double orc_particle_curvature_energy(const omesh *m, const double *r, int n, const int *faces, const double *uv,
+                                     const double *c0, const double *area)
+{
+    double e = 0.0;
+    for (int i = 0; i < n; i++) {
+        double gq[15];
+        if (orc_point_geometry(m, r, faces[i], uv[2 * i], uv[2 * i + 1], gq) != 0) return NAN;
+        double h = gq[9] + gq[10] - c0[i];
+        e += 0.5 * m->prm.kc * area[i] * h * h;
+    }
+    return e;
+}
+
+/* sum over the faces carrying particle area pfa of -(pfa / face_area(r_star))
+ * sum_p g0 w 1/2 kc (c1 + c2 - c0)^2 at r: the function whose gradient at r =
+ * r_star is grad's density term (face_area frozen at r_star) */
+double orc_particle_density_frozen(omesh *m, const double *r_star, const double *r, int n, const int *faces,
+                                   const double *c0, const double *area)
+{
+    int nf = m->nfr + m->nfi;
+    double *pfa = particle_face_area(m, n, faces, c0, area);
+    double e = 0.0;
+    for (int f = 0; f < nf; f++) {
+        if (pfa[f] == 0.0) continue;
+        int npt;
+        double dudv;
+        oform *F = face_forms(m, f, &npt, &dudv);
+        double face_area = 0.0, s = 0.0;
+        for (int p = 0; p < npt; p++) {
+            double X[6][3];
+            contract(m, F + p, r_star, X);
+            double n_[3];
+            cross3(X[1], X[2], n_);
+            face_area += sqrt(n_[0] * n_[0] + n_[1] * n_[1] + n_[2] * n_[2]) * F[p].weight;
+        }
+        for (int p = 0; p < npt; p++) {
+            double X[6][3];
+            contract(m, F + p, r, X);
+            double *Ru = X[1], *Rv = X[2];
+            double E11 = Ru[0] * Ru[0] + Ru[1] * Ru[1] + Ru[2] * Ru[2];
+            double E12 = Ru[0] * Rv[0] + Ru[1] * Rv[1] + Ru[2] * Rv[2];
+            double E22 = Rv[0] * Rv[0] + Rv[1] * Rv[1] + Rv[2] * Rv[2];
+            double nn[3];
+            cross3(Ru, Rv, nn);
+            normalize3(nn);
+            double L = X[3][0] * nn[0] + X[3][1] * nn[1] + X[3][2] * nn[2];
+            double M = X[4][0] * nn[0] + X[4][1] * nn[1] + X[4][2] * nn[2];
+            double N = X[5][0] * nn[0] + X[5][1] * nn[1] + X[5][2] * nn[2];
+            /* c1 + c2 = -tr(I^-1 II) */
+            double H = -(L * E22 - 2 * M * E12 + N * E11) / (E11 * E22 - E12 * E12);
+            s += F[p].g0 * F[p].weight * 0.5 * m->prm.kc * (H - m->face_c0[f]) * (H - m->face_c0[f]);
+        }
+        e += -pfa[f] / face_area * s;
+    }
+    free(pfa);
+    return e;
 }

@@ -121,6 +121,22 @@ int    orc_point_grad(const omesh *m, const double *r, int f, double u, double v
 /* point geometry (surface::ru, rv, g, c): out[15] = Ru[3], Rv[3], g,
  * c1 + c2, c1 c2, c1, c2, principal directions vec1[2], vec2[2] in (u, v) */
 int    orc_point_geometry(const omesh *m, const double *r, int f, double u, double v, double *out);
+
+/* Particle terms of the gradient: surface::pgrad (src/grad.C:3487-5336) for
+ * n particles (face, (u, v), c0, area) -- E_i = 1/2 kc a_i (c1 + c2 - c0_i)^2
+ * at the particle; dE/dr and dE/dalpha added into g (3nv+3), dE/du, dE/dv into
+ * g_puv (2n) -- plus the terms grad and igrad add on faces carrying particle
+ * area (src/grad.C:445-446, 458-459, 6819-6820).  Returns 0, or -1-i for a
+ * particle i the reference would stop on. */
+int    orc_particle_grad(omesh *m, const double *r, int n, const int *faces, const double *uv, const double *c0,
+                         const double *area, double *g, double *g_puv);
+/* pins: the particles' curvature energy (whose gradient pgrad is), and the
+ * frozen-area density function whose gradient at r_star is grad's term */
+double orc_particle_curvature_energy(const omesh *m, const double *r, int n, const int *faces, const double *uv,
+                                     const double *c0, const double *area);
+double orc_particle_density_frozen(omesh *m, const double *r_star, const double *r, int n, const int *faces,
+                                   const double *c0, const double *area);
+void   orc_basis10(double u, double v, double out[10][12]);
 int    orc_face_density(omesh *m, int f, const double *r, double *num, double *area);
 int    orc_particle_energy(const omesh *m, const double *r, int f, double u, double v, double pc0, double pa,
                            double fed, double *out);

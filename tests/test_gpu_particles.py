@@ -12,6 +12,15 @@ of u or v.  Covered: both energy
 forms, random face c0, the vesicle and the periodic sheet with valence-5 and
 -7 vertices, particles at a face's Gauss points cancelling its density, the
 batch split over host threads, and rejections.
+
+The particle terms of the gradient (umcp_particle_grad: surface::pgrad and
+grad's particle-area term) against the oracle's restatement, pinned there by
+finite differences: vertex and alpha gradient norm-wise 1e-12 relative with
+particles on regular faces only; with particles on irregular faces at
+sub-patch level d 1e-12 times 8^d (the third derivatives' conditioning;
+measured 1.3e-12 at levels up to 3), the (u, v) gradient per particle 1e-12
+times 8^d of the largest; bitwise repeatable, accumulating, and the
+extraordinary vertex rejected.
 """
 import sys
 
@@ -148,3 +157,84 @@ def test_threaded_batch_and_rejections():
     with pytest.raises(S.UmcpError, match="setg0"):
         h.particle_energy(w.r, faces[:10], uv[:10], c0[:10], area[:10])
     h.close()
+
+
+# --- particle terms of the gradient (surface::pgrad, src/grad.C:3487-5336, and
+# grad's particle-area term, src/grad.C:445-446, 458-459, 6819-6820) ---------
+
+def _grad_check(g, o, r, faces, uv, c0, area, tag):
+    gg, gp = g.particle_grad(r, faces, uv, c0, area)
+    og, op = o.particle_grad(r, faces, uv, c0, area)
+    rel = np.linalg.norm(gg[:-3] - og[:-3]) / np.linalg.norm(og[:-3])
+    rel_a = np.abs(gg[-3:] - og[-3:]).max() / np.abs(og[-3:]).max()
+    gp, op = gp.reshape(-1, 2), op.reshape(-1, 2)
+    t = uv.sum(axis=1)
+    reg = faces < o.nreg
+    level = np.where(reg, 0, np.where(t >= 1 - 1e-9, 1, np.ceil(-np.log2(np.maximum(t, 1e-300)))))
+    scale = np.abs(op).max()
+    err = np.abs(gp - op).max(axis=1)
+    say("   %s: n=%d vertex grad rel %.3g alpha rel %.3g, (u,v) max err %.3g of %.3g (levels %s)" % (
+        tag, len(faces), rel, rel_a, err.max(), scale, sorted(set(level.astype(int).tolist()))))
+    # third derivatives of a level-d sub-patch are formed from map entries
+    # times 8^d over sums that cancel
+    bar = 1e-12 * 8.0 ** level.max()
+    assert rel <= bar and rel_a <= bar
+    assert (err <= 1e-12 * scale * 8.0 ** level).all()
+    return gg, gp
+
+
+def _grad_particles(o, n, seed):
+    """random particles; irregular ones at sub-patch levels 1-4 (u + v >= 1/16)"""
+    faces, uv, c0, area = _particles(o, n, seed)
+    irr = faces >= o.nreg
+    t = uv[irr].sum(axis=1, keepdims=True)
+    uv[irr] *= np.maximum(t, 1.0 / 15.0) / np.maximum(t, 1e-300)
+    return faces, uv, c0, area
+
+
+@pytest.mark.parametrize("mode", [O.MODE_LIPID, O.MODE_HELFRICH])
+def test_particle_grad_vesicle(mode):
+    w = workloads.vesicle_642()
+    g, o, _ = _pair(w, mode)
+    faces, uv, c0, area = _grad_particles(o, 600, 21)
+    irr = np.flatnonzero(faces >= o.nreg)
+    assert irr.size >= 10
+    r = w.r.copy()
+    r[-3:] = [1.01, 0.99, 1.02]
+    say("[particle grad] vesicle 642 mode", mode)
+    _grad_check(g, o, r, faces, uv, c0, area, "random c0 (particle-area term on)")
+    reg = faces < o.nreg
+    _grad_check(g, o, r, faces[reg], uv[reg], c0[reg], area[reg], "regular faces only (bar 1e-12)")
+    # particles whose c0 equals their face's: pgrad alone
+    fc0 = np.random.default_rng(3).uniform(-0.02, 0.02, o.nfaces)
+    _grad_check(g, o, r, faces, uv, fc0[faces], area, "face c0 (pgrad alone)")
+
+
+def test_particle_grad_defect_sheet():
+    """valence-5 and -7 faces reaching across the periodic box"""
+    w = workloads.defect_sheet()
+    g, o, _ = _pair(w, O.MODE_HELFRICH)
+    faces, uv, c0, area = _grad_particles(o, 500, 5)
+    say("[particle grad] defect sheet")
+    _grad_check(g, o, w.r, faces, uv, c0, area, "defect sheet")
+
+
+def test_particle_grad_accumulates_repeats_and_rejects():
+    w = workloads.vesicle_642()
+    g, o, _ = _pair(w, O.MODE_HELFRICH)
+    faces, uv, c0, area = _grad_particles(o, 300, 9)
+    g1, p1 = g.particle_grad(w.r, faces, uv, c0, area)
+    g2, p2 = g.particle_grad(w.r, faces, uv, c0, area)
+    assert np.array_equal(g1, g2) and np.array_equal(p1, p2)
+    acc, accp = np.ones_like(g1), np.full(p1.shape, 2.0)
+    g.particle_grad(w.r, faces, uv, c0, area, acc, accp)
+    assert np.array_equal(acc, 1.0 + g1) and np.array_equal(accp, 2.0 + p1)
+    # the extraordinary vertex itself has no sub-patch (the reference stops)
+    irr = int(np.flatnonzero(faces >= o.nreg)[0])
+    bad_uv = uv.copy()
+    bad_uv[irr] = (0.0, 0.0)
+    with pytest.raises(S.UmcpError, match="point %d" % irr):
+        g.particle_grad(w.r, faces, bad_uv, c0, area)
+    # the handle still works and gives the same result
+    g3, p3 = g.particle_grad(w.r, faces, uv, c0, area)
+    assert np.array_equal(g3, g1) and np.array_equal(p3, p1)

@@ -936,6 +936,71 @@ __device__ __forceinline__ void point_basis(double u, double v, double B[6][12])
         for (int y = 0; y < 12; y++) B[d][y] /= 12.0;
 }
 
+// Third derivatives of the 12 box-spline basis functions at (u, v) in slot
+// order: B3[d][y], d = uuu, uuv, uvv, vvv (pgrad's (u, v) gradient,
+// src/grad.C:3620-3700).  On the barycentric monomials u^a v^b w^c (w = 1 - u
+// - v) d/du acts as del_u - del_w and d/dv as del_v - del_w.
+__device__ __forceinline__ double falling(int n, int k)
+{
+    double r = 1.0;
+    for (int t = 0; t < k; t++) r *= (double)(n - t);
+    return r;
+}
+
+__device__ __forceinline__ void point_basis3(double u, double v, double B3[4][12])
+{
+    const double w = 1.0 - u - v;
+    double pu[5], pv[5], pw[5];
+    pu[0] = pv[0] = pw[0] = 1.0;
+#pragma unroll
+    for (int i = 1; i < 5; i++) {
+        pu[i] = pu[i - 1] * u;
+        pv[i] = pv[i - 1] * v;
+        pw[i] = pw[i - 1] * w;
+    }
+#pragma unroll
+    for (int d = 0; d < 4; d++)
+#pragma unroll
+        for (int y = 0; y < 12; y++) B3[d][y] = 0.0;
+    constexpr int kI[4] = {3, 2, 1, 0};   // orders along u (the rest along v)
+    constexpr double kBin[4][4] = {{1, 0, 0, 0}, {1, 1, 0, 0}, {1, 2, 1, 0}, {1, 3, 3, 1}};
+#pragma unroll
+    for (int a = 0; a <= 4; a++)
+#pragma unroll
+        for (int b = 0; a + b <= 4; b++) {
+            const int c = 4 - a - b, m = mono_index(a, b);
+            double md[4];
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                const int I = kI[d], J = 3 - kI[d];
+                double s = 0.0;
+#pragma unroll
+                for (int k = 0; k <= I; k++)
+#pragma unroll
+                    for (int l = 0; l <= J; l++) {
+                        const int e = (I - k) + (J - l);   // order along w
+                        if (k > a || l > b || e > c) continue;
+                        const double sg = (e & 1) ? -1.0 : 1.0;
+                        s += sg * kBin[I][k] * kBin[J][l] * falling(a, k) * falling(b, l) * falling(c, e) *
+                             pu[a - k] * pv[b - l] * pw[c - e];
+                    }
+                md[d] = s;
+            }
+#pragma unroll
+            for (int y = 0; y < 12; y++) {
+                const double cf = kBoxTable.c[kSlotBasis[y] - 1][m];
+                if (cf != 0.0) {
+#pragma unroll
+                    for (int d = 0; d < 4; d++) B3[d][y] += cf * md[d];
+                }
+            }
+        }
+#pragma unroll
+    for (int d = 0; d < 4; d++)
+#pragma unroll
+        for (int y = 0; y < 12; y++) B3[d][y] /= 12.0;
+}
+
 __device__ __forceinline__ void cross_d(const double a[3], const double b[3], double c[3])
 {
     c[0] = a[1] * b[2] - a[2] * b[1];
@@ -1042,6 +1107,142 @@ __device__ void point_geometry_out(const PointArgs &a, int p, const PointRec &pr
     o[14] = v2[1] / l2;
 }
 
+// Mean curvature H = c1 + c2 = -(L E22 - 2 M E12 + N E11) / det I of an
+// alpha-scaled point (Ru, Rv, Ruu, Ruv, Rvv) and the exact reverse of it for a
+// given dE/dH (no area-element term): D[d][k] = dE/dX_d,k, X = Ru, Rv, Ruu,
+// Ruv, Rvv.  c1 + c2 of the reference's roots is -(a + d) exactly, whatever
+// SAFETY (src/grad.C:381-399), so the trace form is the reference's H.
+struct CurvPoint {
+    double E11, E12, E22, idet, iln, n[3], L, M, N, T, H;
+};
+
+__device__ __forceinline__ CurvPoint curv_setup(const double Ru[3], const double Rv[3], const double Suu[3],
+                                               const double Suv[3], const double Svv[3])
+{
+    CurvPoint c;
+    c.E11 = Ru[0] * Ru[0] + Ru[1] * Ru[1] + Ru[2] * Ru[2];
+    c.E12 = Ru[0] * Rv[0] + Ru[1] * Rv[1] + Ru[2] * Rv[2];
+    c.E22 = Rv[0] * Rv[0] + Rv[1] * Rv[1] + Rv[2] * Rv[2];
+    c.idet = 1.0 / (c.E11 * c.E22 - c.E12 * c.E12);
+    double Nv[3];
+    cross_d(Ru, Rv, Nv);
+    c.iln = 1.0 / sqrt(Nv[0] * Nv[0] + Nv[1] * Nv[1] + Nv[2] * Nv[2]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) c.n[k] = Nv[k] * c.iln;
+    c.L = c.n[0] * Suu[0] + c.n[1] * Suu[1] + c.n[2] * Suu[2];
+    c.M = c.n[0] * Suv[0] + c.n[1] * Suv[1] + c.n[2] * Suv[2];
+    c.N = c.n[0] * Svv[0] + c.n[1] * Svv[1] + c.n[2] * Svv[2];
+    c.T = c.L * c.E22 - 2.0 * c.M * c.E12 + c.N * c.E11;
+    c.H = -c.T * c.idet;
+    return c;
+}
+
+__device__ __forceinline__ void curv_back(const CurvPoint &c, const double Ru[3], const double Rv[3],
+                                          const double Suu[3], const double Suv[3], const double Svv[3], double dH,
+                                          double D[5][3])
+{
+    const double dL = -dH * c.E22 * c.idet, dM = 2.0 * dH * c.E12 * c.idet, dN = -dH * c.E11 * c.idet;
+    const double ddet = dH * c.T * c.idet * c.idet;
+    const double dE11 = -dH * c.N * c.idet + ddet * c.E22;
+    const double dE22 = -dH * c.L * c.idet + ddet * c.E11;
+    const double dE12 = 2.0 * dH * c.M * c.idet - 2.0 * ddet * c.E12;
+    double Q[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) Q[k] = dL * Suu[k] + dM * Suv[k] + dN * Svv[k];
+    const double qn = Q[0] * c.n[0] + Q[1] * c.n[1] + Q[2] * c.n[2];
+    double dNv[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) dNv[k] = (Q[k] - qn * c.n[k]) * c.iln;
+    double t1[3], t2[3];
+    cross_d(Rv, dNv, t1);
+    cross_d(dNv, Ru, t2);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        D[0][k] = 2.0 * dE11 * Ru[k] + dE12 * Rv[k] + t1[k];
+        D[1][k] = 2.0 * dE22 * Rv[k] + dE12 * Ru[k] + t2[k];
+        D[2][k] = dL * c.n[k];
+        D[3][k] = dM * c.n[k];
+        D[4][k] = dN * c.n[k];
+    }
+}
+
+// d H / d s along a parameter s of the point: dX holds the derivatives of
+// Ru and Rv along s (dRu, dRv) and of the second-derivative rows (dSuu, dSuv,
+// dSvv), e.g. for s = u: Ruu, Ruv, Ruuu, Ruuv, Ruvv
+__device__ __forceinline__ double curv_dir(const CurvPoint &c, const double Ru[3], const double Rv[3],
+                                           const double Suu[3], const double Suv[3], const double Svv[3],
+                                           const double dRu[3], const double dRv[3], const double dSuu[3],
+                                           const double dSuv[3], const double dSvv[3])
+{
+    // unit normal along s: P (dRu x Rv + Ru x dRv) / |N|, P = I - n n^T
+    double a1[3], a2[3], dn[3];
+    cross_d(dRu, Rv, a1);
+    cross_d(Ru, dRv, a2);
+#pragma unroll
+    for (int k = 0; k < 3; k++) dn[k] = a1[k] + a2[k];
+    const double pn = dn[0] * c.n[0] + dn[1] * c.n[1] + dn[2] * c.n[2];
+#pragma unroll
+    for (int k = 0; k < 3; k++) dn[k] = (dn[k] - pn * c.n[k]) * c.iln;
+    auto dot = [](const double x[3], const double y[3]) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+    const double dE11 = 2.0 * dot(Ru, dRu), dE12 = dot(dRu, Rv) + dot(Ru, dRv), dE22 = 2.0 * dot(Rv, dRv);
+    const double dL = dot(dn, Suu) + dot(c.n, dSuu);
+    const double dM = dot(dn, Suv) + dot(c.n, dSuv);
+    const double dN = dot(dn, Svv) + dot(c.n, dSvv);
+    const double dT = dL * c.E22 + c.L * dE22 - 2.0 * (dM * c.E12 + c.M * dE12) + dN * c.E11 + c.N * dE11;
+    const double ddet = dE11 * c.E22 + c.E11 * dE22 - 2.0 * c.E12 * dE12;
+    return -dT * c.idet + c.T * ddet * c.idet * c.idet;
+}
+
+// pgrad for one particle (src/grad.C:3700-3935 regular, 4614-5335
+// irregular): E = 1/2 kc a (c1 + c2 - c0)^2 at the particle on the
+// alpha-scaled surface, kc a and c0 in de_dr[3p], de_dr[3p + 1].  Writes the
+// (u, v) gradient and, per control point, the contribution sum_d t_d dE/dX_d
+// (the reduction multiplies by alpha).  X: contractions of unscaled positions
+// with the weights of the position and its five derivatives, X3 with the
+// third derivatives (uuu, uuv, uvv, vvv).
+template <bool IRR>
+__device__ void particle_out(const PointArgs &a, int p, int j, const double al[3], const double X[6][3],
+                             const double X3[4][3], const double *map, int ncp, const double B[6][12],
+                             const double sc[6])
+{
+    double R[10][3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+#pragma unroll
+        for (int d = 1; d < 6; d++) R[d - 1][k] = al[k] * X[d][k];
+#pragma unroll
+        for (int d = 0; d < 4; d++) R[5 + d][k] = al[k] * X3[d][k];
+    }
+    const double *Ru = R[0], *Rv = R[1], *Suu = R[2], *Suv = R[3], *Svv = R[4];
+    const double *Suuu = R[5], *Suuv = R[6], *Suvv = R[7], *Svvv = R[8];
+    const CurvPoint c = curv_setup(Ru, Rv, Suu, Suv, Svv);
+    const double kca = a.de_dr[3 * p], pc0 = a.de_dr[3 * p + 1];
+    const double dH = kca * (c.H - pc0);
+    a.gpuv[2 * p] = dH * curv_dir(c, Ru, Rv, Suu, Suv, Svv, Suu, Suv, Suuu, Suuv, Suvv);
+    a.gpuv[2 * p + 1] = dH * curv_dir(c, Ru, Rv, Suu, Suv, Svv, Suv, Svv, Suuv, Suvv, Svvv);
+    double D[5][3];
+    curv_back(c, Ru, Rv, Suu, Suv, Svv, dH, D);
+    auto emit = [&](int x, const double t[6]) {
+        double *out = a.contrib + ((size_t)x * a.n + j) * 3;
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            out[k] = t[1] * D[0][k] + t[2] * D[1][k] + t[3] * D[2][k] + t[4] * D[3][k] + t[5] * D[4][k];
+    };
+    if constexpr (IRR) {
+        for (int x = 0; x < ncp; x++) {
+            double t[6];
+            irr_weights<6>(map, ncp, x, B, sc, t);
+            emit(x, t);
+        }
+    } else {
+#pragma unroll
+        for (int x = 0; x < 12; x++) {
+            const double t[6] = {B[0][x], B[1][x], B[2][x], B[3][x], B[4][x], B[5][x]};
+            emit(x, t);
+        }
+    }
+}
+
 // p: the caller's index of the point, j: its face-sorted position (gradient)
 template <bool IRR>
 __device__ void point_body(const PointArgs &a, int p, int j, const PointRec &pr, const PointClass &C)
@@ -1105,6 +1306,46 @@ __device__ void point_body(const PointArgs &a, int p, int j, const PointRec &pr,
     }
     if (a.grad == 2) {
         point_geometry_out(a, p, pr, IRR, al, X);
+        return;
+    }
+    if (a.grad == 4) {
+        // the third derivatives for the particle's (u, v) gradient
+        double B3[4][12];
+        point_basis3(pr.u, pr.v, B3);
+        const double sc3[4] = {pr.su * pr.su * pr.su, pr.su * pr.su * pr.sv, pr.su * pr.sv * pr.sv,
+                               pr.sv * pr.sv * pr.sv};
+        double X3[4][3];
+#pragma unroll
+        for (int d = 0; d < 4; d++) X3[d][0] = X3[d][1] = X3[d][2] = 0.0;
+        for (int x = 0; x < ncp; x++) {
+            double t3[4];
+            if constexpr (IRR) {
+#pragma unroll
+                for (int d = 0; d < 4; d++) t3[d] = 0.0;
+                for (int y = 0; y < 12; y++) {
+                    const double mw = map[y * ncp + x];
+#pragma unroll
+                    for (int d = 0; d < 4; d++) t3[d] += mw * B3[d][y];
+                }
+#pragma unroll
+                for (int d = 0; d < 4; d++) t3[d] *= sc3[d];
+            } else {
+#pragma unroll
+                for (int d = 0; d < 4; d++) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int y = 0; y < 12; y++) s = (y == x) ? B3[d][y] : s;
+                    t3[d] = s;
+                }
+            }
+            double q[3];
+            load_q(x, q);
+#pragma unroll
+            for (int d = 0; d < 4; d++)
+#pragma unroll
+                for (int k = 0; k < 3; k++) X3[d][k] += t3[d] * q[k];
+        }
+        particle_out<IRR>(a, p, j, al, X, X3, map, ncp, B, sc);
         return;
     }
     if (!a.grad) {
@@ -1255,7 +1496,7 @@ __global__ __launch_bounds__(kPointBlock) void point_kernel(PointArgs a, const i
     const int t = blockIdx.x * kPointBlock + threadIdx.x;
     if (t >= count) return;
     const int j = idx[t];
-    const int p = a.grad == 1 ? a.order[j] : j;
+    const int p = (a.grad == 1 || a.grad == 4) ? a.order[j] : j;
     const PointRec pr = a.rec[p];
     const PointClass C = a.cls[pr.cls];
     if (a.grad == 3) {
@@ -1405,6 +1646,97 @@ __global__ __launch_bounds__(kPointAlphaBlock) void point_alpha_kernel(int len, 
         __syncthreads();
     }
     if (t < 3) dst[3 * blockIdx.x + t] = sh[t][0];
+}
+
+// grad's particle-area term of one face carrying particles (src/grad.C:93-103,
+// 212-224, 445-446, 458-459; igrad 6477-6487, 6584, 6819-6820): per Gauss
+// point dE/dH = -pfa dAf kc (H - c0_f) / face_area, dAf = g0 w, face_area =
+// sum g w of the current (alpha-scaled) surface, no area-element term; the
+// face's (face, slot) contributions go to sorted position it.j.
+template <int NCP, int NPT, int KIND>
+__device__ void density_face(const PointArgs &a, const DensityItem &it, const PointClass &C)
+{
+    const int nf = C.nf, fc = it.fc;
+    const bool shifted = C.has_img[fc] != 0;
+    const double al[3] = {a.r[3 * a.nv], a.r[3 * a.nv + 1], a.r[3 * a.nv + 2]};
+    double P[NCP][3];
+#pragma unroll
+    for (int x = 0; x < NCP; x++) {
+        const int v = C.cp[x * nf + fc];
+        double o[3] = {0.0, 0.0, 0.0};
+        if (shifted) image_offset(C.img[x * nf + fc], a.pbc, o);
+#pragma unroll
+        for (int k = 0; k < 3; k++) P[x][k] = a.r[3 * v + k] + o[k];
+    }
+    const double *tab = kind_table<KIND, double>();
+    const double *wts = tab + NPT * 5 * NCP;
+    auto contract = [&](int p, double X[5][3]) {
+        const double *T = tab + p * 5 * NCP;
+#pragma unroll
+        for (int d = 0; d < 5; d++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                double s = 0.0;
+#pragma unroll
+                for (int x = 0; x < NCP; x++) s += T[x * 5 + d] * P[x][k];
+                X[d][k] = s * al[k];
+            }
+    };
+    double area = 0.0;
+    for (int p = 0; p < NPT; p++) {
+        double X[5][3], n[3];
+        contract(p, X);
+        cross_d(X[0], X[1], n);
+        area += sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]) * wts[p];
+    }
+    double G[NCP][3];
+#pragma unroll
+    for (int x = 0; x < NCP; x++) G[x][0] = G[x][1] = G[x][2] = 0.0;
+    for (int p = 0; p < NPT; p++) {
+        double X[5][3];
+        contract(p, X);
+        const CurvPoint c = curv_setup(X[0], X[1], X[2], X[3], X[4]);
+        const double dAf = C.g0[p * nf + fc] * wts[p];
+        const double dH = -it.pfa * dAf * a.kc * (c.H - it.c0f) / area;
+        double D[5][3];
+        curv_back(c, X[0], X[1], X[2], X[3], X[4], dH, D);
+        const double *T = tab + p * 5 * NCP;
+#pragma unroll
+        for (int x = 0; x < NCP; x++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                double s = G[x][k];
+#pragma unroll
+                for (int d = 0; d < 5; d++) s += T[x * 5 + d] * D[d][k];
+                G[x][k] = s;
+            }
+    }
+#pragma unroll
+    for (int x = 0; x < NCP; x++) {
+        double *out = a.contrib + ((size_t)x * a.n + it.j) * 3;
+        out[0] = G[x][0];
+        out[1] = G[x][1];
+        out[2] = G[x][2];
+    }
+}
+
+__global__ __launch_bounds__(kPointBlock) void particle_density_kernel(PointArgs a, const DensityItem *items, int count)
+{
+    const int t = blockIdx.x * kPointBlock + threadIdx.x;
+    if (t >= count) return;
+    const DensityItem it = items[t];
+    const PointClass C = a.cls[it.cls];
+    if (C.valence == 6) density_face<12, 15, 0>(a, it, C);
+    else if (C.ncp == 11) density_face<11, 16, 1>(a, it, C);
+    else density_face<13, 16, 2>(a, it, C);
+}
+
+hipError_t launch_particle_density(const PointArgs &a, const DensityItem *items, int count, hipStream_t st)
+{
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(particle_density_kernel, dim3((count + kPointBlock - 1) / kPointBlock), dim3(kPointBlock), 0,
+                       st, a, items, count);
+    return hipGetLastError();
 }
 
 hipError_t launch_points(const PointArgs &a, const int *idx, const int counts[3], hipStream_t st)

@@ -169,6 +169,7 @@ struct PointClass {
     const uint8_t *img;      // [ncp][nf]
     const uint8_t *has_img;  // [nf]
     const double *map;       // irregular: [kPointDomains * 3][12][ncp]
+    const double *g0;        // [npt][nf] reference metric of the face's Gauss points
     int nf, ncp, valence, pad;
 };
 
@@ -180,9 +181,11 @@ struct PointSeg {
 struct PointArgs {
     const double *r;         // 3 nv + 3, alpha last
     int nv, n;
-    int grad;                // 0: position and normal, 1: gradient, 2: geometry, 3: weights
+    int grad;                // 0: position and normal, 1: gradient, 2: geometry, 3: weights,
+                             // 4: particle curvature gradient (pgrad; de_dr[3p] = kc a, c0)
     int on_surface;
     double kT;
+    double kc;               // particle gradient: the surface's bending modulus
     double pbc[3][3];
     const PointRec *rec;     // [n], the caller's order
     const int *order;        // gradient: [n] caller index of face-sorted position j
@@ -201,6 +204,15 @@ constexpr int kPointGeom = 15;
 // of the regular, irregular and extraordinary-vertex points in that order,
 // counts[kind] of each (the vertex kind: evaluation only)
 hipError_t launch_points(const PointArgs &a, const int *idx, const int counts[3], hipStream_t st);
+// grad's particle-area term of one face (src/grad.C:445-446, 458-459,
+// 6819-6820): its (face, slot) contributions land at sorted position j of the
+// face's segment, after the face's particles
+struct DensityItem {
+    int cls, fc, j, pad;
+    double pfa;              // particle area of the face's particles whose c0 differs from the face's
+    double c0f;              // the face's spontaneous curvature
+};
+hipError_t launch_particle_density(const PointArgs &a, const DensityItem *items, int count, hipStream_t st);
 // gradient reduction of a face-sorted batch: per (face, slot) the sum of its
 // points' contributions in order, alpha-scaled into gbuf at the slot's vertex
 // CSR position (gbuf zeroed first), and its alpha-gradient part into apart;

@@ -118,6 +118,7 @@ int point_setup(umcp_surface *s)
         t.ncp = C.ncp;
         t.valence = C.valence;
         t.map = nullptr;
+        t.g0 = s->cls[c].g0;
         if (C.valence != 6) {
             if (irregular_domain_maps(C.valence, kPointDomains, (double *)(host.data() + map_off[c])) != 0)
                 return pfail(s, UMCP_ERR_BAD_MESH, "no sub-patch maps for this valence");
@@ -195,7 +196,7 @@ const char *resolve_irregular(double fu, double fv, bool vertex_ok, PointRec &re
 }
 
 struct Batch {
-    size_t off_r, off_rec, off_order, off_idx, off_dr, off_dn, off_frac, off_seg, in_bytes;
+    size_t off_r, off_rec, off_order, off_idx, off_dr, off_dn, off_frac, off_dens, off_seg, in_bytes;
     size_t off_R, off_N, off_gout, off_gp, off_con, off_apart, out_bytes;
 };
 
@@ -262,10 +263,15 @@ void par_copy(int nt, void *dst, const void *src, size_t bytes)
 // Records go straight into the pinned staging buffer, over host threads in
 // contiguous chunks of the batch; a rejection names the first bad point, as
 // a serial pass would.
+// mode 0 evaluation, 1 point gradient, 2 geometry, 3 weights, 4 particle
+// gradient (de_dr = de_dnrm: kc a, c0, 0 per particle; face_pfa and face_c0
+// per face in global order: the particle area of grad's density term and the
+// face's spontaneous curvature)
 int run_points(umcp_surface *s, const double *r, int n, const int *faces, const double *uv, const double *de_dr,
-               const double *de_dnrm, const double *frac, int on_surface, double kT, int mode, Batch &b)
+               const double *de_dnrm, const double *frac, int on_surface, double kT, int mode, Batch &b,
+               const double *face_pfa = nullptr, const double *face_c0 = nullptr)
 {
-    const bool grad = mode == 1;
+    const bool grad = mode == 1 || mode == 4;
     const int nv = s->T.nv;
     const Plan &P = s->P;
     const int nfaces = P.nfaces;
@@ -275,11 +281,14 @@ int run_points(umcp_surface *s, const double *r, int n, const int *faces, const 
     size_t o = 0;
     b.off_r = o;      o = align_up(o + sizeof(double) * (3 * (size_t)nv + 3));
     b.off_rec = o;    o = align_up(o + sizeof(PointRec) * (size_t)n);
-    b.off_order = o;  o = align_up(o + (grad ? sizeof(int) * (size_t)n : 0));
+    // the face-sorted positions run past n when faces get a density record
+    const size_t order_cap = grad ? (size_t)n + (mode == 4 ? seg_cap : 0) : 0;
+    b.off_order = o;  o = align_up(o + sizeof(int) * order_cap);
     b.off_idx = o;    o = align_up(o + sizeof(int) * (size_t)n);
     b.off_dr = o;     o = align_up(o + (grad ? sizeof(double) * 3 * (size_t)n : 0));
     b.off_dn = o;     o = align_up(o + (grad ? sizeof(double) * 3 * (size_t)n : 0));
     b.off_frac = o;   o = align_up(o + (grad && frac ? sizeof(double) * (size_t)n : 0));
+    b.off_dens = o;   o = align_up(o + (mode == 4 ? sizeof(DensityItem) * seg_cap : 0));
     b.off_seg = o;    o = align_up(o + sizeof(PointSeg) * seg_cap);
     const size_t in_cap = o;
     int rc = pin_grow(s, &s->pt_stage, &s->pt_stage_cap, in_cap);
@@ -346,7 +355,8 @@ int run_points(umcp_surface *s, const double *r, int n, const int *faces, const 
     for (const Chunk &c : ch)
         for (int k = 0; k < 3; k++) counts[k] += c.counts[k];
 
-    int nseg = 0;
+    int nseg = 0, ndens = 0;
+    DensityItem *dens = (DensityItem *)(h + b.off_dens);
     if (grad) {
         // stable counting sort by face: chunk t's points of face f start at
         // the face's start plus the earlier chunks' counts of f, so the order
@@ -363,10 +373,16 @@ int run_points(umcp_surface *s, const double *r, int n, const int *faces, const 
                 cnt += k;
             }
             if (!cnt) continue;
-            seg[nseg++] = {fcls[(size_t)f], ffc[(size_t)f], pos, pos + cnt};
+            // particle gradient: a face carrying particle area in grad's
+            // density term gets one more record after its particles
+            const int extra = (mode == 4 && face_pfa[(size_t)f] != 0.0) ? 1 : 0;
+            seg[nseg++] = {fcls[(size_t)f], ffc[(size_t)f], pos, pos + cnt + extra};
+            if (extra)
+                dens[ndens++] = {fcls[(size_t)f], ffc[(size_t)f], pos + cnt, 0, face_pfa[(size_t)f],
+                                 face_c0[(size_t)f]};
             const int kind = P.classes[(size_t)fcls[(size_t)f]].valence == 6 ? kPointRegular : kPointIrregular;
             for (int j = pos; j < pos + cnt; j++) idx[fill[kind]++] = j;
-            pos += cnt;
+            pos += cnt + extra;
         }
         par_chunks(nt, (size_t)n, [&](int t, size_t lo, size_t hi) {
             std::vector<int> &next = ch[(size_t)t].fcount;
@@ -392,13 +408,14 @@ int run_points(umcp_surface *s, const double *r, int n, const int *faces, const 
     if (r) std::memcpy(h + b.off_r, r, sizeof(double) * (3 * (size_t)nv + 3));
     else std::memset(h + b.off_r, 0, sizeof(double) * (3 * (size_t)nv + 3));
     b.in_bytes = b.off_seg + sizeof(PointSeg) * (size_t)nseg;
+    const int nrec = n + ndens;   // gradient records per slot
     o = 0;
     if (grad) {
         // [vertex sums 3 nv][alpha 3][gpuv 2n] first: one download
         b.off_gout = o;
         b.off_gp = o + sizeof(double) * (3 * (size_t)nv + 3);
         o = align_up(b.off_gp + sizeof(double) * 2 * (size_t)n);
-        b.off_con = o;    o = align_up(o + sizeof(double) * (size_t)n * kPointSlots * 3);
+        b.off_con = o;    o = align_up(o + sizeof(double) * (size_t)nrec * kPointSlots * 3);
         b.off_apart = o;  o = align_up(o + sizeof(double) * (size_t)nseg * kPointSlots * 3);
         b.off_R = b.off_N = 0;
     } else if (mode == 2) {
@@ -426,10 +443,11 @@ int run_points(umcp_surface *s, const double *r, int n, const int *faces, const 
     char *di = s->d_pt_in, *dout = s->d_pt_out;
     a.r = (const double *)(di + b.off_r);
     a.nv = nv;
-    a.n = n;
+    a.n = nrec;
     a.grad = mode;
     a.on_surface = on_surface ? 1 : 0;
     a.kT = kT;
+    a.kc = s->prm.kc;
     std::memcpy(a.pbc, s->T.pbc, sizeof a.pbc);
     a.rec = (const PointRec *)(di + b.off_rec);
     a.cls = s->d_pt_cls;
@@ -451,11 +469,15 @@ int run_points(umcp_surface *s, const double *r, int n, const int *faces, const 
     }
     e = launch_points(a, (const int *)(di + b.off_idx), counts, s->stream);
     if (e != hipSuccess) return phip(s, e, "point kernel launch");
+    if (mode == 4) {
+        e = launch_particle_density(a, (const DensityItem *)(di + b.off_dens), ndens, s->stream);
+        if (e != hipSuccess) return phip(s, e, "particle density kernel launch");
+    }
     if (grad) {
         PointReduceArgs ra{};
         ra.r = a.r;
         ra.nv = nv;
-        ra.n = n;
+        ra.n = nrec;
         ra.nseg = nseg;
         ra.nnz = P.nnz;
         std::memcpy(ra.pbc, s->T.pbc, sizeof ra.pbc);
@@ -542,6 +564,38 @@ int umcp_point_grad(umcp_surface *s, const double *r, int n, const int *faces, c
     par_chunks(host_threads((size_t)n), 2 * (size_t)n, [&](int, size_t lo, size_t hi) {
         for (size_t i = lo; i < hi; i++) g_puv[i] += gp[i];
     });
+    return UMCP_OK;
+}
+
+int umcp_particle_grad(umcp_surface *s, const double *r, int n, const int *faces, const double *uv,
+                       const double *c0, const double *area, double *g, double *g_puv)
+{
+    int rc = point_check(s, r, n, faces, uv);
+    if (rc) return rc;
+    if (n == 0) return UMCP_OK;
+    if (!c0 || !area || !g || !g_puv) return pfail(s, UMCP_ERR_ARG, "null particle arrays");
+    const int nfaces = s->P.nfaces;
+    // grad's density term counts the particles whose c0 differs from their
+    // face's by 1e-7 or more (src/grad.C:93-103, 6483-6487)
+    std::vector<double> pfa((size_t)nfaces, 0.0), cf((size_t)nfaces, 0.0), pc(3 * (size_t)n, 0.0);
+    if (!s->face_c0.empty()) cf = s->face_c0;
+    for (int i = 0; i < n; i++) {
+        const int f = faces[i];
+        if (f < 0 || f >= nfaces) return pfail(s, UMCP_ERR_ARG, "point " + std::to_string(i) + ": face index out of range");
+        if (std::fabs(c0[i] - cf[(size_t)f]) >= 1e-7) pfa[(size_t)f] += area[i];
+        pc[3 * (size_t)i] = s->prm.kc * area[i];
+        pc[3 * (size_t)i + 1] = c0[i];
+    }
+    Batch b;
+    rc = run_points(s, r, n, faces, uv, pc.data(), pc.data(), nullptr, 0, 0.0, 4, b, pfa.data(), cf.data());
+    if (rc) return rc;
+    const size_t ng = 3 * (size_t)s->T.nv + 3;
+    const char *hst = nullptr;
+    rc = fetch_out(s, b.off_gout, sizeof(double) * (ng + 2 * (size_t)n), &hst, "download particle gradient");
+    if (rc) return rc;
+    const double *gs = (const double *)hst, *gp = gs + ng;
+    for (size_t i = 0; i < ng; i++) g[i] += gs[i];
+    for (size_t i = 0; i < 2 * (size_t)n; i++) g_puv[i] += gp[i];
     return UMCP_OK;
 }
 

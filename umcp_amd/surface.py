@@ -47,7 +47,7 @@ EXPORTS = [
     "umcp_save_mesh", "umcp_write_restart", "umcp_read_restart", "umcp_write_limit_surface",
     "umcp_point_eval", "umcp_point_grad", "umcp_point_geometry", "umcp_write_limit_psf",
     "umcp_face_energies", "umcp_energy_vertex", "umcp_area_vertex", "umcp_next_face", "umcp_point_coeffs",
-    "umcp_particle_energy",
+    "umcp_particle_energy", "umcp_particle_grad",
 ]
 
 
@@ -152,6 +152,7 @@ def load_library(path: str = LIB_PATH):
         "umcp_next_face": ([vp, dp, C.c_int, ip, dp, dp, dp, dp], C.c_int),
         "umcp_point_coeffs": ([vp, C.c_int, ip, dp, ip, dp], C.c_int),
         "umcp_particle_energy": ([vp, dp, C.c_int, ip, dp, dp, dp, dp], C.c_int),
+        "umcp_particle_grad": ([vp, dp, C.c_int, ip, dp, dp, dp, dp, dp], C.c_int),
         "umcp_point_grad": ([vp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int, C.c_double, dp, dp], C.c_int),
         "umcp_write_restart": ([C.c_char_p, C.c_int, dp, dp, C.c_int], C.c_int),
         "umcp_read_restart": ([C.c_char_p, C.c_int, dp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
@@ -418,6 +419,23 @@ class Surface:
         self._check(self._L.umcp_particle_energy(self._h, _dp(r), f.size, _ip(f), _dp(uv), _dp(c0), _dp(a),
                                                  _dp(out)), "particle_energy")
         return out
+
+    def particle_grad(self, r: np.ndarray, faces, uv, c0, area, g=None, g_puv=None):
+        """The particle terms of surface::grad (pgrad and the particle-area
+        term of faces carrying particles), accumulated into g (3nv+3) and g_puv
+        (2n); returns (g, g_puv), zeros first when not given."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        f = np.ascontiguousarray(faces, dtype=np.int32)
+        uv = np.ascontiguousarray(uv, dtype=np.float64).reshape(-1, 2)
+        c0 = np.ascontiguousarray(c0, dtype=np.float64)
+        a = np.ascontiguousarray(area, dtype=np.float64)
+        assert c0.size == f.size and a.size == f.size
+        g = np.zeros(r.size) if g is None else g
+        g_puv = np.zeros(2 * f.size) if g_puv is None else g_puv
+        assert g.flags.c_contiguous and g_puv.flags.c_contiguous
+        self._check(self._L.umcp_particle_grad(self._h, _dp(r), f.size, _ip(f), _dp(uv), _dp(c0), _dp(a), _dp(g),
+                                               _dp(g_puv)), "particle_grad")
+        return g, g_puv
 
     def point_coeffs(self, faces, uv):
         """(ids [n, 18], weights [n, 3, 18]): get_pt_coeffs / get_pt_dcoeffs."""
