@@ -137,6 +137,106 @@ __device__ __forceinline__ void image_offset(int code, const double pbc[3][3], d
     for (int k = 0; k < 3; k++) out[k] = ix * pbc[0][k] + iy * pbc[1][k] + iz * pbc[2][k];
 }
 
+// Control points c = c0, c0 + STRIDE, ... of face fc, image-shifted, into
+// x[i] (cp c0 + i STRIDE; c0 wave-uniform; entries past NCP - 1 repeat the
+// last control point and are the caller's to ignore).  Every index load is
+// issued, then every position load, before any of them is used, with no
+// branch between them, and the periodic fix-up (faces with has_img, a
+// divergent branch) comes last: a wave waits for two dependent memory round
+// trips rather than one per control point.
+template <int NCP, int STRIDE>
+__device__ __forceinline__ void gather_cps(const FaceArgs &a, const double *__restrict__ r, int fc, int c0,
+                                           double x[(NCP + STRIDE - 1) / STRIDE][3])
+{
+    constexpr int kN = (NCP + STRIDE - 1) / STRIDE;
+    const int nf = a.nf;
+    const bool shifted = a.has_img[fc] != 0;
+    int v[kN];
+#pragma unroll
+    for (int i = 0; i < kN; i++) v[i] = a.cp[min(c0 + i * STRIDE, NCP - 1) * nf + fc];
+#pragma unroll
+    for (int i = 0; i < kN; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) x[i][k] = r[3 * v[i] + k];
+    if (shifted) {
+#pragma unroll
+        for (int i = 0; i < kN; i++) {
+            double o[3];
+            image_offset(a.img[min(c0 + i * STRIDE, NCP - 1) * nf + fc], a.pbc, o);
+#pragma unroll
+            for (int k = 0; k < 3; k++) x[i][k] += o[k];
+        }
+    }
+}
+
+// The global restraints' energy and multipliers from the pre-pass's group
+// partials (area: src/interpLib.C:4322-4326; volume: src/minimize.C:90-114):
+// A_tot and V summed in one fixed order -- lane l takes groups l, l + 64, ...
+// in turn, then an xor butterfly over the wave, whose every lane ends with
+// the same bits since IEEE addition commutes -- so any wave of any block gets
+// the same values.  out = {dE/dA_tot, dE/dV, restraint energy, A_tot, V}.
+__device__ __forceinline__ void restraint_scalars(const FaceArgs &a, double out[5])
+{
+    #pragma clang fp contract(off)
+    const int lane = threadIdx.x & (kWave - 1);
+    constexpr int kU = 16;   // loads in flight per lane
+    const double2 *part = reinterpret_cast<const double2 *>(a.rpart);
+    double sA = 0.0, sV = 0.0;
+    for (int i0 = 0; i0 < a.ngroups; i0 += kU * kWave) {
+        double2 v[kU];
+#pragma unroll
+        for (int j = 0; j < kU; j++) v[j] = part[min(i0 + j * kWave + lane, a.ngroups - 1)];
+#pragma unroll
+        for (int j = 0; j < kU; j++)
+            if (i0 + j * kWave + lane < a.ngroups) {
+                sA += v[j].x;
+                sV += v[j].y;
+            }
+    }
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        sA += __shfl_xor(sA, off, kWave);
+        sV += __shfl_xor(sV, off, kWave);
+    }
+    double lamA = 0.0, lamV = 0.0, E = 0.0;
+    if (a.do_area && a.KA_g != 0.0) {
+        const double st = (sA - a.A0) / a.A0;
+        E += a.KA_g * a.A0 * st * st;
+        lamA = 2.0 * a.KA_g * st;
+    }
+    if (a.do_volume && a.KV != 0.0) {
+        const double sv = (sV - a.V0) / a.V0;
+        E += 0.5 * a.KV * a.V0 * sv * sv;
+        lamV = a.KV * sv;
+    }
+    out[0] = lamA;
+    out[1] = lamV;
+    out[2] = E;
+    out[3] = sA;
+    out[4] = sV;
+}
+
+// The multipliers a face block uses: formed from the partials when the
+// surface folds its finalize (the slot-0 group also publishes all five
+// scalars in rs for the gather and the host), else read from rs.
+__device__ __forceinline__ void restraint_multipliers(const FaceArgs &a, int slot, double lam[2])
+{
+    lam[0] = lam[1] = 0.0;
+    if (!a.rs) return;
+    if (a.rs_fold) {
+        double o[5];
+        restraint_scalars(a, o);
+        if (slot == 0 && threadIdx.x == 0)
+#pragma unroll
+            for (int i = 0; i < 5; i++) a.rs[i] = o[i];
+        lam[0] = o[0];
+        lam[1] = o[1];
+    } else {
+        lam[0] = a.rs[0];
+        lam[1] = a.rs[1];
+    }
+}
+
 // Reciprocal and square root: IEEE-exact in FP64; the hardware
 // single-instruction forms (about 1 ulp) in the FP32 path, whose error budget
 // is set by FP32 rounding anyway.
@@ -184,7 +284,7 @@ struct PointGrad {
 
 template <typename R, int FEAT, bool REG>
 __device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R ig0, R rurv0, R Lo, R Li,
-                                               R c0f, R dudv, R lamA, const PhysParams &ph, PointGrad<R> &out)
+                                               R c0f, R dudv, double lamA, const PhysParams &ph, PointGrad<R> &out)
 {
     constexpr bool kAnyMode = FEAT == kFeatGeneral;
     constexpr bool kExtras = FEAT == kFeatGeneral;   // kg and k_reg terms
@@ -226,10 +326,15 @@ __device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R ig0
         if (REG && !ph.grad_exact) hg = eh;   // src/grad.C:448
     }
     // per-point area strain (A - A0)/A0 with A = dudv g w, A0 = dudv g0 w;
-    // lamA = dE/dA_tot of the global area restraint (A_tot = sum dudv g w)
+    // lamA = dE/dA_tot of the global area restraint (A_tot = sum dudv g w),
+    // an FP64 multiplier that the FP32 path adds in FP64 (one rounding)
     const R strain = g * ig0 - R(1);
     e += KA * dudv * g0w * strain * strain;
-    const R dg = (R(2) * KA * strain + lamA + hg) * dudv * w;
+    R dg;
+    if constexpr (sizeof(R) == 8)
+        dg = (R(2) * KA * strain + lamA + hg) * dudv * w;
+    else
+        dg = (R)((double)(R(2) * KA * strain + hg) + lamA) * dudv * w;
 
     R dL = -dH * E22 * idet, dM = R(2) * dH * E12 * idet, dN = -dH * E11 * idet;
     R ddet = dH * T * idet * idet + dg * R(0.5) * ig;
@@ -283,7 +388,7 @@ __device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R ig0
 // the same with the five rows expanded: D[d][k] = dE/dX[d][k]
 template <typename R, int FEAT, bool REG>
 __device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R ig0, R rurv0, R Lo, R Li,
-                                               R c0f, R dudv, R lamA, const PhysParams &ph, R D[5][3])
+                                               R c0f, R dudv, double lamA, const PhysParams &ph, R D[5][3])
 {
     PointGrad<R> pg;
     const R e = point_energy_grad<R, FEAT, REG>(X, w, g0, ig0, rurv0, Lo, Li, c0f, dudv, lamA, ph, pg);
@@ -313,8 +418,8 @@ __device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R ig0
 // and dE/dalpha accumulate in FP64; the gradient leaves as FP32 and is summed
 // per vertex in FP64 by the gather.
 template <typename R, int NCP, int NPT, int KIND, int WS, bool PLDS, int FEAT>
-__device__ __forceinline__ void face_group(const FaceArgs &a, const double *__restrict__ r, int group, R *lds,
-                                           double (*s_corr)[kWave], double acc[4])
+__device__ __forceinline__ void face_group(const FaceArgs &a, const double *__restrict__ r, int group, int slot,
+                                           R *lds, double (*s_corr)[kWave], double acc[4])
 {
     constexpr bool kRel = sizeof(R) == 4;
     const R *tab = kind_table<KIND, R>();
@@ -329,40 +434,40 @@ __device__ __forceinline__ void face_group(const FaceArgs &a, const double *__re
     const int fc = live ? f : nf - 1;
     const double al64[3] = {r[3 * a.nv], r[3 * a.nv + 1], r[3 * a.nv + 2]};
     const R al[3] = {(R)al64[0], (R)al64[1], (R)al64[2]};
-    const bool shifted = a.has_img[fc] != 0;
-
-    auto load_cp = [&](int c, double out[3]) {
-        const int v = a.cp[c * nf + fc];
-        out[0] = r[3 * v];
-        out[1] = r[3 * v + 1];
-        out[2] = r[3 * v + 2];
-        if (shifted) {
-            double o[3];
-            image_offset(a.img[c * nf + fc], a.pbc, o);
-            out[0] += o[0];
-            out[1] += o[1];
-            out[2] += o[2];
-        }
-    };
+    // the restraint multipliers first, while few registers are live
+    double lam[2];   // dE/dA_tot, dE/dV
+    restraint_multipliers(a, slot, lam);
     double base[3] = {0.0, 0.0, 0.0};
-    if (kRel) load_cp(0, base);
     R P[PLDS ? 1 : NCP][3];
     if (PLDS) {
-        for (int c = w; c < NCP; c += WS) {
-            double t[3];
-            load_cp(c, t);
+        double t[(NCP + WS - 1) / WS][3];
+        gather_cps<NCP, WS>(a, r, fc, WS == 1 ? 0 : w, t);
+        if (kRel) {
+            const int v0 = a.cp[fc];
+            for (int k = 0; k < 3; k++) base[k] = r[3 * v0 + k];
+            if (a.has_img[fc]) {
+                double o[3];
+                image_offset(a.img[fc], a.pbc, o);
+                for (int k = 0; k < 3; k++) base[k] += o[k];
+            }
+        }
 #pragma unroll
-            for (int k = 0; k < 3; k++) lds[(c * 3 + k) * kWave + lane] = (R)(t[k] - base[k]);
+        for (int i = 0; i < (NCP + WS - 1) / WS; i++) {
+            const int c = w + i * WS;
+            if (c < NCP)
+#pragma unroll
+                for (int k = 0; k < 3; k++) lds[(c * 3 + k) * kWave + lane] = (R)(t[i][k] - base[k]);
         }
         __syncthreads();
     } else {
+        double t[NCP][3];
+        gather_cps<NCP, 1>(a, r, fc, 0, t);
+        if (kRel)
+            for (int k = 0; k < 3; k++) base[k] = t[0][k];
 #pragma unroll
-        for (int c = 0; c < NCP; c++) {
-            double t[3];
-            load_cp(c, t);
+        for (int c = 0; c < NCP; c++)
 #pragma unroll
-            for (int k = 0; k < 3; k++) P[c][k] = (R)(t[k] - base[k]);
-        }
+            for (int k = 0; k < 3; k++) P[c][k] = (R)(t[c][k] - base[k]);
     }
 #define PV(c, k) (PLDS ? lds[((c) * 3 + (k)) * kWave + lane] : P[PLDS ? 0 : (c)][k])
 
@@ -389,7 +494,7 @@ __device__ __forceinline__ void face_group(const FaceArgs &a, const double *__re
 #pragma unroll
                 for (int q = 0; q < 3; q++) L[q][k] += W[q * NCP + c] * x;
             }
-        const double lamV = a.rs[1] * (1.0 / 6.0);
+        const double lamV = lam[1] * (1.0 / 6.0);
         double D[3][3];
 #pragma unroll
         for (int q = 0; q < 3; q++) {
@@ -412,7 +517,7 @@ __device__ __forceinline__ void face_group(const FaceArgs &a, const double *__re
         for (int k = 0; k < 3; k++) s_corr[k][lane] = corr[k];
     }
     const R Lo = (R)a.Lo[fc], Li = (R)a.Li[fc], c0f = (R)a.c0[fc];
-    const R lamA = a.rs ? (R)a.rs[0] : R(0);
+    const double lamA = lam[0];
     double e = 0.0;
     for (int p = w; p < NPT; p += WS) {
         const R *T = tab + p * 5 * NCP;
@@ -541,9 +646,9 @@ __global__ __launch_bounds__(kWave *WS, OCC) void faces_batch_kernel(BatchFaceAr
         const double *r = B.r ? B.r : a.r;
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         switch (item.kind) {
-        case 0: face_group<R, 12, 15, 0, WS, PLDS, FEAT>(a, r, item.group, lds, s_corr, acc); break;
-        case 1: face_group<R, 11, 16, 1, WS, PLDS, FEAT>(a, r, item.group, lds, s_corr, acc); break;
-        default: face_group<R, 13, 16, 2, WS, PLDS, FEAT>(a, r, item.group, lds, s_corr, acc); break;
+        case 0: face_group<R, 12, 15, 0, WS, PLDS, FEAT>(a, r, item.group, item.slot, lds, s_corr, acc); break;
+        case 1: face_group<R, 11, 16, 1, WS, PLDS, FEAT>(a, r, item.group, item.slot, lds, s_corr, acc); break;
+        default: face_group<R, 13, 16, 2, WS, PLDS, FEAT>(a, r, item.group, item.slot, lds, s_corr, acc); break;
         }
         block_sum4<kWave * WS>(acc);
         if (threadIdx.x == 0) {

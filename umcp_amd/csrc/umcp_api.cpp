@@ -148,6 +148,7 @@ FaceArgs face_args(umcp_surface *s, size_t c, const double *d_r)
     a.rs = restraints_on(s) ? s->d_rs : nullptr;
     a.rpart = s->d_rpart;
     a.ngroups = s->npartial;
+    a.rs_fold = a.rs && a.ngroups <= kFoldMaxGroups ? 1 : 0;
     a.KA_g = s->rsp.KA_global;
     a.KV = s->rsp.KV;
     a.A0 = s->rsp.A0 > 0.0 ? s->rsp.A0 : s->A0_setg0;
@@ -190,6 +191,7 @@ int enqueue_volume(umcp_surface *s, const double *d_r, hipStream_t st)
     for (size_t c = 0; c < fa.size(); c++) {
         fa[c] = face_args(s, c, d_r);
         fa[c].rs = s->d_rs;
+        fa[c].rs_fold = 0;   // no face kernel follows: the finalize writes d_rs
         fa[c].do_area = 0;
         fa[c].do_volume = 1;
         fa[c].KA_g = 0.0;
@@ -198,7 +200,7 @@ int enqueue_volume(umcp_surface *s, const double *d_r, hipStream_t st)
     hipError_t e = hipMemcpyAsync(s->d_fa_aux, fa.data(), fa.size() * sizeof(FaceArgs), hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(s, e, "upload face table");
     BatchFaceArgs fb{s->d_fa_aux, s->d_items, s->npartial, d_r, nullptr};
-    e = launch_restraint_prepass(fb, s->d_zero, 1, false, st);
+    e = launch_restraint_prepass(fb, s->d_zero, 1, st);
     if (e != hipSuccess) return hip_fail(s, e, "restraint kernel launch");
     return UMCP_OK;
 }
@@ -267,7 +269,7 @@ int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs 
             if (int rc = prof_pair(s, 2, evr)) return rc;
             (void)hipEventRecord(evr[0], st);
         }
-        hipError_t e = launch_restraint_prepass(fb, s->d_zero, 1, s->prm.precision == UMCP_PREC_MIXED, st);
+        hipError_t e = launch_restraint_prepass(fb, s->d_zero, s->h_fa[0].rs_fold ? 0 : 1, st);
         if (e != hipSuccess) return hip_fail(s, e, "restraint kernel launch");
         if (timed) (void)hipEventRecord(evr[1], st);
     }

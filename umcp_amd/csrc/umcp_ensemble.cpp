@@ -51,8 +51,8 @@ struct umcp_ensemble {
     FaceItem *d_items = nullptr;
     GatherArgs *d_ga = nullptr;
     GatherItem *d_gitems = nullptr;
-    int *d_first = nullptr;       // FaceArgs entry (class 0) of each member with restraints
-    int nitems = 0, ngitems = 0, nrestrained = 0;
+    int *d_first = nullptr;       // FaceArgs entry (class 0) of each member whose restraint finalize is a launch
+    int nitems = 0, ngitems = 0, nrestrained = 0, nfinalize = 0;
     std::vector<void *> tables;   // allocations of the current build
     // captured step sequence
     hipGraph_t graph = nullptr;
@@ -103,7 +103,7 @@ void drop_tables(umcp_ensemble *e)
     e->d_ga = nullptr;
     e->d_gitems = nullptr;
     e->d_first = nullptr;
-    e->nitems = e->ngitems = e->nrestrained = 0;
+    e->nitems = e->ngitems = e->nrestrained = e->nfinalize = 0;
 }
 
 template <typename T>
@@ -145,11 +145,13 @@ int build(umcp_ensemble *e)
     std::vector<GatherArgs> ga;
     std::vector<GatherItem> gitems;
     std::vector<int> first;
+    int nrestrained = 0;
     for (size_t mi = 0; mi < e->members.size(); mi++) {
         const auto &m = e->members[mi];
         umcp_surface *s = m.s;
         const bool rson = restraints_on(s);
-        if (rson) first.push_back((int)fa.size());
+        if (rson) nrestrained++;
+        if (rson && face_args(s, 0, m.d_r).rs_fold == 0) first.push_back((int)fa.size());
         std::vector<int> fa_of_class(s->P.classes.size());
         for (size_t c = 0; c < s->P.classes.size(); c++) {
             FaceArgs a = face_args(s, c, m.d_r);
@@ -212,7 +214,8 @@ int build(umcp_ensemble *e)
         return rc;
     e->nitems = (int)items.size();
     e->ngitems = (int)gitems.size();
-    e->nrestrained = (int)first.size();
+    e->nrestrained = nrestrained;
+    e->nfinalize = (int)first.size();
     err = hipStreamSynchronize(e->stream);
     if (err != hipSuccess) return ehip(e, err, "hipStreamSynchronize");
     return UMCP_OK;
@@ -224,7 +227,7 @@ hipError_t enqueue_step(umcp_ensemble *e, hipStream_t st)
     if (e->nrestrained > 0) {
         // over every item: groups of members without restraints return at once
         BatchFaceArgs rb{e->d_fa, e->d_items, e->nitems, nullptr, nullptr};
-        err = launch_restraint_prepass(rb, e->d_first, e->nrestrained, e->mixed, st);
+        err = launch_restraint_prepass(rb, e->d_first, e->nfinalize, st);
         if (err != hipSuccess) return err;
     }
     for (int f = 0; f < 3; f++) {

@@ -44,7 +44,7 @@ __device__ __forceinline__ mx4 mfma4(float a, float b, mx4 c)
 }
 
 template <int NCP, int NPT, int KIND, int WS, int FEAT>
-__device__ __forceinline__ void face_group_mx(const FaceArgs &a, const double *__restrict__ r, int group,
+__device__ __forceinline__ void face_group_mx(const FaceArgs &a, const double *__restrict__ r, int group, int slot,
                                               float *lds, double acc[4])
 {
     constexpr int NKS = (NCP + 3) / 4;   // control-point steps of four
@@ -64,34 +64,36 @@ __device__ __forceinline__ void face_group_mx(const FaceArgs &a, const double *_
     const double al64[3] = {r[3 * a.nv], r[3 * a.nv + 1], r[3 * a.nv + 2]};
     const float al[3] = {(float)al64[0], (float)al64[1], (float)al64[2]};
 
-    // 1. staging: lane = face, the waves split the control points
+    // 1. the restraint multipliers, then staging: lane = face, the waves
+    // split the control points
+    double lam[2];   // dE/dA_tot, dE/dV
+    restraint_multipliers(a, slot, lam);
     {
         const int f = group * kWave + lane;
         const int fc = f < nf ? f : nf - 1;
-        const bool shifted = a.has_img[fc] != 0;
-        auto load_cp = [&](int c, double out[3]) {
-            const int v = a.cp[c * nf + fc];
-            out[0] = r[3 * v];
-            out[1] = r[3 * v + 1];
-            out[2] = r[3 * v + 2];
-            if (shifted) {
-                double o[3];
-                image_offset(a.img[c * nf + fc], a.pbc, o);
-                out[0] += o[0];
-                out[1] += o[1];
-                out[2] += o[2];
-            }
-        };
+        constexpr int kN = (NCP + WS - 1) / WS, kRows = (4 * NKS + WS - 1) / WS;
+        double t[kN][3];
+        gather_cps<NCP, WS>(a, r, fc, WS == 1 ? 0 : w, t);
         double base[3];
-        load_cp(0, base);
+        {
+            const int v0 = a.cp[fc];
+            for (int k = 0; k < 3; k++) base[k] = r[3 * v0 + k];
+            if (a.has_img[fc]) {
+                double o[3];
+                image_offset(a.img[fc], a.pbc, o);
+                for (int k = 0; k < 3; k++) base[k] += o[k];
+            }
+        }
         if (w == 0)
 #pragma unroll
             for (int k = 0; k < 3; k++) sBase[k * kWave + lane] = base[k];
-        for (int c = w; c < 4 * NKS; c += WS) {
-            double t[3] = {base[0], base[1], base[2]};
-            if (c < NCP) load_cp(c, t);
 #pragma unroll
-            for (int k = 0; k < 3; k++) sP[(c * 3 + k) * kMxRow + lane] = (float)(t[k] - base[k]);
+        for (int i = 0; i < kRows; i++) {
+            const int c = w + i * WS;
+            if (c < 4 * NKS)
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+                    sP[(c * 3 + k) * kMxRow + lane] = (i < kN && c < NCP) ? (float)(t[i < kN ? i : 0][k] - base[k]) : 0.f;
         }
     }
     __syncthreads();
@@ -111,7 +113,7 @@ __device__ __forceinline__ void face_group_mx(const FaceArgs &a, const double *_
 #pragma unroll
                 for (int qq = 0; qq < 3; qq++) L[qq][k] += W[qq * NCP + c] * x;
             }
-        const double lamV = a.rs[1] * (1.0 / 6.0);
+        const double lamV = lam[1] * (1.0 / 6.0);
         double D[3][3];
 #pragma unroll
         for (int qq = 0; qq < 3; qq++) {
@@ -134,7 +136,7 @@ __device__ __forceinline__ void face_group_mx(const FaceArgs &a, const double *_
     for (int t = 0; t < 4; t++)
 #pragma unroll
         for (int k = 0; k < 3; k++) G[t][k] = mx4{0.f, 0.f, 0.f, 0.f};
-    const float lamA = a.rs ? (float)a.rs[0] : 0.f;
+    const double lamA = lam[0];
     const float dudv = (float)a.dudv;
     // per-face inputs of the lane's four faces (16 t + j), loaded up front
     constexpr bool kLip = FEAT != kFeatHelfrich, kC0 = FEAT != kFeatLipid;
@@ -302,9 +304,9 @@ __global__ __launch_bounds__(kWave *WS, OCC) void faces_batch_mfma(BatchFaceArgs
         const double *r = B.r ? B.r : a.r;
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         switch (item.kind) {
-        case 0: face_group_mx<12, 15, 0, WS, FEAT>(a, r, item.group, lds, acc); break;
-        case 1: face_group_mx<11, 16, 1, WS, FEAT>(a, r, item.group, lds, acc); break;
-        default: face_group_mx<13, 16, 2, WS, FEAT>(a, r, item.group, lds, acc); break;
+        case 0: face_group_mx<12, 15, 0, WS, FEAT>(a, r, item.group, item.slot, lds, acc); break;
+        case 1: face_group_mx<11, 16, 1, WS, FEAT>(a, r, item.group, item.slot, lds, acc); break;
+        default: face_group_mx<13, 16, 2, WS, FEAT>(a, r, item.group, item.slot, lds, acc); break;
         }
         block_sum4<kWave * WS>(acc);
         if (threadIdx.x == 0) {

@@ -193,7 +193,9 @@ __device__ __forceinline__ void gather_reduce(const GatherArgs &a)
 // kGatherLanes lanes per vertex: lane q sums entries s0 + q, s0 + q + kGatherLanes, ...
 // of the vertex's CSR range (neighbouring lanes read neighbouring entries),
 // then a fixed xor butterfly combines the lanes (IEEE addition commutes, so
-// every lane of the group ends with the same sum).
+// every lane of the group ends with the same sum).  The entries are read
+// kGatherUnroll at a time, every load issued before the first add (clamped
+// to the range; the adds skip what lies beyond it), in the same order.
 template <typename GB>
 __device__ __forceinline__ void gather_vertices(const GatherArgs &a, int vblock)
 {
@@ -208,10 +210,23 @@ __device__ __forceinline__ void gather_vertices(const GatherArgs &a, int vblock)
     if (mine) {
         const int s0 = a.vstart[v], s1 = a.vstart[v + 1];
         const GB *src = (const GB *)(sizeof(GB) == 4 ? (const void *)a.gbuf32 : (const void *)a.gbuf);
-        for (int e = s0 + q; e < s1; e += kGatherLanes) {
-            g0 += (double)src[3 * (size_t)e];
-            g1 += (double)src[3 * (size_t)e + 1];
-            g2 += (double)src[3 * (size_t)e + 2];
+        constexpr int kU = kGatherUnroll;
+        for (int e0 = s0 + q; e0 < s1; e0 += kU * kGatherLanes) {
+            GB x[kU][3];
+#pragma unroll
+            for (int j = 0; j < kU; j++) {
+                const size_t e = (size_t)min(e0 + j * kGatherLanes, s1 - 1);
+                x[j][0] = src[3 * e];
+                x[j][1] = src[3 * e + 1];
+                x[j][2] = src[3 * e + 2];
+            }
+#pragma unroll
+            for (int j = 0; j < kU; j++)
+                if (e0 + j * kGatherLanes < s1) {
+                    g0 += (double)x[j][0];
+                    g1 += (double)x[j][1];
+                    g2 += (double)x[j][2];
+                }
         }
     }
 #pragma unroll
@@ -287,22 +302,51 @@ __device__ __forceinline__ void effm_row(const EffmArgs &a, int v, const double 
     #pragma clang fp contract(off)
     const int lane = threadIdx.x & (kWave - 1);
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    // kU entries per lane in flight (indices clamped to the row, products
+    // beyond it skipped), summed in the same order as one at a time
+    constexpr int kU = 8;
     if (a.rowptr) {
         const int e1 = a.rowptr[v + 1];
-        for (int e = a.rowptr[v] + lane; e < e1; e += kWave) {
-            const double m = a.val[e];
-            const size_t u = (size_t)a.col[e];
-            s0 += m * x[3 * u];
-            s1 += m * x[3 * u + 1];
-            s2 += m * x[3 * u + 2];
+        for (int e0 = a.rowptr[v] + lane; e0 < e1; e0 += kU * kWave) {
+            double m[kU];
+            int u[kU];
+#pragma unroll
+            for (int j = 0; j < kU; j++) {
+                const int e = min(e0 + j * kWave, e1 - 1);
+                m[j] = a.val[e];
+                u[j] = a.col[e];
+            }
+            double xx[kU][3];
+#pragma unroll
+            for (int j = 0; j < kU; j++)
+#pragma unroll
+                for (int k = 0; k < 3; k++) xx[j][k] = x[3 * (size_t)u[j] + k];
+#pragma unroll
+            for (int j = 0; j < kU; j++)
+                if (e0 + j * kWave < e1) {
+                    s0 += m[j] * xx[j][0];
+                    s1 += m[j] * xx[j][1];
+                    s2 += m[j] * xx[j][2];
+                }
         }
     } else {
         const double *row = a.Mi + (size_t)v * a.nv;
-        for (int u = lane; u < a.nv; u += kWave) {
-            const double m = row[u];
-            s0 += m * x[3 * (size_t)u];
-            s1 += m * x[3 * (size_t)u + 1];
-            s2 += m * x[3 * (size_t)u + 2];
+        for (int u0 = lane; u0 < a.nv; u0 += kU * kWave) {
+            double m[kU], xx[kU][3];
+#pragma unroll
+            for (int j = 0; j < kU; j++) {
+                const size_t u = (size_t)min(u0 + j * kWave, a.nv - 1);
+                m[j] = row[u];
+#pragma unroll
+                for (int k = 0; k < 3; k++) xx[j][k] = x[3 * u + k];
+            }
+#pragma unroll
+            for (int j = 0; j < kU; j++)
+                if (u0 + j * kWave < a.nv) {
+                    s0 += m[j] * xx[j][0];
+                    s1 += m[j] * xx[j][1];
+                    s2 += m[j] * xx[j][2];
+                }
         }
     }
 #pragma unroll
@@ -492,66 +536,43 @@ __global__ __launch_bounds__(kGatherBlock) void vertex_gather_batch(BatchGatherA
 // finalize sums the groups' partials in fixed order and writes the
 // multipliers the face kernel reads.
 // ---------------------------------------------------------------------------
-// One 64-face group: the waves stage the group's control points in LDS and
-// take the Gauss points p = w, w + WS, ...; the last wave also forms the
-// faces' corner limit points and volume terms.  (Single-surface and batched
-// evaluations run this same kernel, so contraction choices cannot differ
-// between them.)
-// MIXED: the area's per-point arithmetic in FP32 on control points relative
-// to the face's first one (as the mixed face kernel), summed in FP64; the
-// volume stays FP64 on absolute positions.
-template <int NCP, int NPT, int KIND, int WS, bool MIXED>
+// One 64-face group, lane = face: the waves gather the group's control
+// points once (wave w takes c = w, w + WS, ...) into LDS, then each wave
+// reads all of them into registers and takes the Gauss points p = w, w + WS,
+// ...; the last wave also forms the corner limit points and the volume term.
+// (Single-surface and batched evaluations run this same kernel, so
+// contraction choices cannot differ between them.)
+// FP64 throughout, for the mixed path too: the multipliers depend on
+// A_tot - A0 and V - V0, small differences of the sums.
+template <int NCP, int NPT, int KIND, int WS>
 __device__ __forceinline__ void restraint_group(const FaceArgs &a, const double *__restrict__ r, int group,
-                                                double *lds, float *lds32, double acc[4])
+                                                double *lds, double acc[4])
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int nf = a.nf;
     const int f = group * kWave + lane;
-    const bool live = f < nf;
-    const int fc = live ? f : nf - 1;
+    const bool live = f < a.nf;
+    const int fc = live ? f : a.nf - 1;
     const double al[3] = {r[3 * a.nv], r[3 * a.nv + 1], r[3 * a.nv + 2]};
-    const bool shifted = a.has_img[fc] != 0;
-    for (int c = w; c < NCP; c += WS) {
-        const int v = a.cp[c * nf + fc];
-        double o[3] = {0.0, 0.0, 0.0};
-        if (shifted) image_offset(a.img[c * nf + fc], a.pbc, o);
+    {
+        double t[(NCP + WS - 1) / WS][3];
+        gather_cps<NCP, WS>(a, r, fc, WS == 1 ? 0 : w, t);
 #pragma unroll
-        for (int k = 0; k < 3; k++) lds[(c * 3 + k) * kWave + lane] = r[3 * v + k] + o[k];
+        for (int i = 0; i < (NCP + WS - 1) / WS; i++) {
+            const int c = w + i * WS;
+            if (c < NCP)
+#pragma unroll
+                for (int k = 0; k < 3; k++) lds[(c * 3 + k) * kWave + lane] = t[i][k];
+        }
     }
     __syncthreads();
-#define PX(c, k) lds[((c) * 3 + (k)) * kWave + lane]
+    double x[NCP][3];
+#pragma unroll
+    for (int c = 0; c < NCP; c++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) x[c][k] = lds[(c * 3 + k) * kWave + lane];
     double A = 0.0, V = 0.0;
-    if (MIXED && a.do_area) {
-        for (int c = w; c < NCP; c += WS)
-#pragma unroll
-            for (int k = 0; k < 3; k++) lds32[(c * 3 + k) * kWave + lane] = (float)(PX(c, k) - PX(0, k));
-        __syncthreads();
-        const float *tab = kind_table<KIND, float>();
-        const float alf[3] = {(float)al[0], (float)al[1], (float)al[2]};
-        const float dudv = (float)a.dudv;
-        for (int p = w; p < NPT; p += WS) {
-            const float *T = tab + p * 5 * NCP;
-            float Ru[3] = {0.f, 0.f, 0.f}, Rv[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-            for (int c = 0; c < NCP; c++)
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    const float x = lds32[(c * 3 + k) * kWave + lane];
-                    Ru[k] += T[c * 5 + 0] * x;
-                    Rv[k] += T[c * 5 + 1] * x;
-                }
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                Ru[k] *= alf[k];
-                Rv[k] *= alf[k];
-            }
-            const float E11 = Ru[0] * Ru[0] + Ru[1] * Ru[1] + Ru[2] * Ru[2];
-            const float E12 = Ru[0] * Rv[0] + Ru[1] * Rv[1] + Ru[2] * Rv[2];
-            const float E22 = Rv[0] * Rv[0] + Rv[1] * Rv[1] + Rv[2] * Rv[2];
-            A += (double)(dudv * __builtin_amdgcn_sqrtf(E11 * E22 - E12 * E12) * tab[NPT * 5 * NCP + p]);
-        }
-    } else if (a.do_area) {
+    if (a.do_area) {
         const double *tab = kind_table<KIND, double>();
         for (int p = w; p < NPT; p += WS) {
             const double *T = tab + p * 5 * NCP;
@@ -560,9 +581,8 @@ __device__ __forceinline__ void restraint_group(const FaceArgs &a, const double 
             for (int c = 0; c < NCP; c++)
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
-                    const double x = PX(c, k);
-                    Ru[k] += T[c * 5 + 0] * x;
-                    Rv[k] += T[c * 5 + 1] * x;
+                    Ru[k] += T[c * 5 + 0] * x[c][k];
+                    Rv[k] += T[c * 5 + 1] * x[c][k];
                 }
 #pragma unroll
             for (int k = 0; k < 3; k++) {
@@ -581,41 +601,37 @@ __device__ __forceinline__ void restraint_group(const FaceArgs &a, const double 
 #pragma unroll
         for (int c = 0; c < NCP; c++)
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const double x = PX(c, k);
+            for (int k = 0; k < 3; k++)
 #pragma unroll
-                for (int q = 0; q < 3; q++) L[q][k] += W[q * NCP + c] * x;
-            }
+                for (int q = 0; q < 3; q++) L[q][k] += W[q * NCP + c] * x[c][k];
         const double cx = L[1][1] * L[2][2] - L[1][2] * L[2][1];
         const double cy = L[1][2] * L[2][0] - L[1][0] * L[2][2];
         const double cz = L[1][0] * L[2][1] - L[1][1] * L[2][0];
         V = (L[0][0] * cx + L[0][1] * cy + L[0][2] * cz) * (1.0 / 6.0);
     }
-#undef PX
     if (live) {
         acc[0] += A;
         acc[1] += V;
     }
 }
 
-constexpr int kPrepassWaves = 4;   // waves per 64-face group (about four Gauss points each)
+constexpr int kPrepassWaves = 2;   // waves per 64-face group
 
-template <int WS, bool MIXED>
+template <int WS>
 __global__ __launch_bounds__(kWave *WS) void restraint_prepass_kernel(BatchFaceArgs B)
 {
-    __shared__ double lds[13 * 3 * kWave];
-    __shared__ float lds32[MIXED ? 13 * 3 * kWave : 1];
     const int it = xcd_logical_block(blockIdx.x, gridDim.x);
     if (it >= B.nitems) return;
     const FaceItem item = B.items[it];
     const FaceArgs &a = B.fa[item.fa];
     if (!a.rs) return;   // a surface without restraints (uniform over the block)
+    __shared__ double lds[13 * 3 * kWave];
     const double *r = B.r ? B.r : a.r;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     switch (item.kind) {
-    case 0: restraint_group<12, 15, 0, WS, MIXED>(a, r, item.group, lds, lds32, acc); break;
-    case 1: restraint_group<11, 16, 1, WS, MIXED>(a, r, item.group, lds, lds32, acc); break;
-    default: restraint_group<13, 16, 2, WS, MIXED>(a, r, item.group, lds, lds32, acc); break;
+    case 0: restraint_group<12, 15, 0, WS>(a, r, item.group, lds, acc); break;
+    case 1: restraint_group<11, 16, 1, WS>(a, r, item.group, lds, acc); break;
+    default: restraint_group<13, 16, 2, WS>(a, r, item.group, lds, acc); break;
     }
     block_sum4<kWave * WS>(acc);
     if (threadIdx.x == 0) {
@@ -624,37 +640,16 @@ __global__ __launch_bounds__(kWave *WS) void restraint_prepass_kernel(BatchFaceA
     }
 }
 
-// one block per surface (the surface's FaceArgs entry of class 0): the
-// groups' partials summed in fixed order, then the restraint energy and its
-// multipliers
-__global__ __launch_bounds__(kGatherBlock) void restraint_finalize_kernel(const FaceArgs *fa, const int *first)
+// one wave per surface that does not fold its finalize into the face kernel
+// (the surface's FaceArgs entry of class 0): the scalars the face kernel
+// would form, written to rs
+__global__ __launch_bounds__(kWave) void restraint_finalize_kernel(const FaceArgs *fa, const int *first)
 {
-    #pragma clang fp contract(off)
     const FaceArgs &a = fa[first[blockIdx.x]];
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int i = threadIdx.x; i < a.ngroups; i += kGatherBlock) {
-        s[0] += a.rpart[2 * (size_t)i];
-        s[1] += a.rpart[2 * (size_t)i + 1];
-    }
-    block_sum4<kGatherBlock>(s);
-    if (threadIdx.x == 0) {
-        double lamA = 0.0, lamV = 0.0, E = 0.0;
-        if (a.do_area && a.KA_g != 0.0) {
-            const double sa = (s[0] - a.A0) / a.A0;
-            E += a.KA_g * a.A0 * sa * sa;
-            lamA = 2.0 * a.KA_g * sa;
-        }
-        if (a.do_volume && a.KV != 0.0) {
-            const double sv = (s[1] - a.V0) / a.V0;
-            E += 0.5 * a.KV * a.V0 * sv * sv;
-            lamV = a.KV * sv;
-        }
-        a.rs[0] = lamA;
-        a.rs[1] = lamV;
-        a.rs[2] = E;
-        a.rs[3] = s[0];
-        a.rs[4] = s[1];
-    }
+    double o[5];
+    restraint_scalars(a, o);
+    if (threadIdx.x == 0)
+        for (int i = 0; i < 5; i++) a.rs[i] = o[i];
 }
 
 // One face's energy (FP64, every energy form; no global restraint terms, as
@@ -859,18 +854,13 @@ hipError_t launch_gather_batch(const BatchGatherArgs &a, int grid, bool mixed, h
     return hipGetLastError();
 }
 
-hipError_t launch_restraint_prepass(const BatchFaceArgs &a, const int *first, int nsurf, bool mixed, hipStream_t st)
+hipError_t launch_restraint_prepass(const BatchFaceArgs &a, const int *first, int nsurf, hipStream_t st)
 {
-    if (a.nitems <= 0 || nsurf <= 0) return hipSuccess;
-    if (mixed)
-        hipLaunchKernelGGL((restraint_prepass_kernel<kPrepassWaves, true>), dim3(a.nitems), dim3(kWave * kPrepassWaves),
-                           0, st, a);
-    else
-        hipLaunchKernelGGL((restraint_prepass_kernel<kPrepassWaves, false>), dim3(a.nitems),
-                           dim3(kWave * kPrepassWaves), 0, st, a);
+    if (a.nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL((restraint_prepass_kernel<kPrepassWaves>), dim3(a.nitems), dim3(kWave * kPrepassWaves), 0, st, a);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(restraint_finalize_kernel, dim3(nsurf), dim3(kGatherBlock), 0, st, a.fa, first);
+    if (e != hipSuccess || nsurf <= 0) return e;
+    hipLaunchKernelGGL(restraint_finalize_kernel, dim3(nsurf), dim3(kWave), 0, st, a.fa, first);
     return hipGetLastError();
 }
 

@@ -12,6 +12,7 @@ constexpr int kWave = 64;
 constexpr int kGatherBlock = 256;
 constexpr int kGatherLanes = 4;                          // lanes summing one vertex's CSR range (4 measured fastest of 2, 4, 8, 16)
 constexpr int kGatherVerts = kGatherBlock / kGatherLanes;  // vertices per gather block
+constexpr int kGatherUnroll = 8;                          // CSR entries per lane in flight
 constexpr int kMetricBlock = 256;
 constexpr int kMaxClasses = 4;
 constexpr int kMaxSplit = 4;
@@ -49,6 +50,9 @@ struct FaceArgs {
     double *rs;
     double *rpart;
     int ngroups;
+    // 1: the face kernel forms the restraint scalars from rpart itself (and
+    // its slot-0 group writes rs); 0: a finalize launch writes rs first
+    int rs_fold;
     double KA_g, KV, A0, V0;
     int do_area, do_volume;
 };
@@ -278,7 +282,12 @@ int gather_blocks(int nv);
 // restraint pre-pass over the same items as the face kernel (per-group area
 // and volume), then one finalize block per surface: first[i] is the FaceArgs
 // entry (in a.fa) of the i-th surface with restraints
-hipError_t launch_restraint_prepass(const BatchFaceArgs &a, const int *first, int nsurf, bool mixed, hipStream_t st);
+// Surfaces of at most this many 64-face groups fold the restraint finalize
+// into the face kernel (every face block sums the group partials itself).
+constexpr int kFoldMaxGroups = 2048;
+// pre-pass over a's items, then a finalize for the nsurf surfaces whose
+// class-0 FaceArgs entries first[] names (those that do not fold)
+hipError_t launch_restraint_prepass(const BatchFaceArgs &a, const int *first, int nsurf, hipStream_t st);
 
 // face kernel over a work-item table (one surface or many); grid = 0: one
 // workgroup per item, else a persistent grid of that many workgroups striding

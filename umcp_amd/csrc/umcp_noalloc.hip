@@ -122,7 +122,7 @@ extern "C" int umcp_energy_grad_dev_noalloc(umcp_surface *s, const double *d_r, 
     fb.r = d_r;
     fb.step_ctr = nullptr;
     if (rson) {
-        e = launch_restraint_prepass(fb, s->d_zero, 1, mixed, st);
+        e = launch_restraint_prepass(fb, s->d_zero, fa[0].rs_fold ? 0 : 1, st);
         if (e != hipSuccess) return fail(s, UMCP_ERR_DEVICE, std::string("restraint kernel launch: ") + hipGetErrorString(e));
     }
     e = launch_faces_batch(fb, 0, s->point_split, s->launch_flags, mixed, features_of(fa[0].ph), st);
