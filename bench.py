@@ -53,6 +53,7 @@ def face_bytes(ncp: int, npt: int, nv: int, nf: int) -> float:
 
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
+AMU_KG = 1.661e-27         # atomic mass unit in kg, as the reference converts masses (src/units.h)
 FEAT_HELFRICH = 2         # energy-form template argument of the face kernel (mode B, kg = k_reg = 0)
 
 
@@ -316,6 +317,32 @@ def run_ensemble_leg(args, dev, torch):
     dr = np.array([np.abs(a - b).max() for a, b in zip(r32, r64)])
 
     levels = {str(L): sum(1 for m in members if m.level == L) for L in sorted({m.level for m in members})}
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the serial oracle's Brownian step, member after member in the
+        # ensemble's order until the budget is spent; the ensemble step's CPU
+        # time is projected from the measured face-evals/s
+        from oracle import oracle as O
+        rng = np.random.default_rng(4)
+        t_cpu, n_m, n_f = 0.0, 0, 0
+        for m in members:
+            o = O.OracleSurface(m.mesh_path)
+            o.set_params(**m.params)
+            o.setg0(m.r0)
+            mob, amp = S.bd_coefficients(1e-12, m.dt, m.T)
+            xi = rng.standard_normal((1, 3 * o.nv))
+            t0 = time.perf_counter()
+            o.bd_steps(m.r, xi, mob, amp)
+            t_cpu += time.perf_counter() - t0
+            n_m, n_f = n_m + 1, n_f + o.nfaces
+            del o
+            if t_cpu > args.cpu_budget / 2:
+                break
+        rate = n_f / t_cpu
+        cpu = {"value": rate, "unit": "face-evals/s", "cores": 1, "kind": "port",
+               "ensemble_ms_per_step": 1e3 * nf / rate,
+               "sample": "one Brownian step of each of the first %d members (%d faces), serial oracle, %.1f s"
+                         % (n_m, n_f, t_cpu)}
     return {
         "workload": "ensemble_%d_levels2-5" % len(members),
         "members": len(members), "levels": levels, "faces": nf, "vertices": sum(m.nv for m in members),
@@ -332,34 +359,185 @@ def run_ensemble_leg(args, dev, torch):
         "vs_fp64": {"steps": Kc, "energy_rel_max": float(e_rel.max()), "energy_rel_median": float(np.median(e_rel)),
                     "max_dr_A": float(dr.max())},
         "setup_s": setup_s,
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (cpu["ensemble_ms_per_step"] / (ms / K)) if cpu else None,
     }
 
 
-def run_sheet_leg(args, dev, torch):
-    """The periodic 128 x 128 hex sheet (16384 vertices, 32768 regular faces,
-    lipid form, Fourier-mode heights), FP64 and FP32-mixed, Brownian steps
-    with the friction gamma = 1e-10 kg/s as D = kT/gamma.  dt = 3e-11 s: at the
-    stated 1e-9 s (and at 1e-10 s) the explicit step diverges (the serial
-    oracle reaches NaN within 20 steps); throughput does not depend on dt."""
+def run_c1_leg(args, dev, torch):
+    """The 642-vertex vesicle (kc 20 kT, c0 0, KA 0.215 kT/A^2): one energy +
+    gradient evaluation, then overdamped Brownian steps at dt 1e-9 s and
+    D 1e-12 m^2/s with device N(0,1) noise, FP64 and FP32-mixed; beside them
+    the oracle's serial and threaded steps (host normals) on a bounded sample
+    of the same steps."""
     import numpy as np
+    from umcp_amd import surface as S
     from umcp_amd import workloads
+    w = workloads.vesicle_642()
+    mob, amp = S.bd_coefficients(1e-12, 1e-9, w.T)
+    K = args.c1_steps
+    stream = torch.cuda.Stream(device=dev)
+    sptr = stream.cuda_stream
+    out = {"workload": "vesicle_642_bd", "nv": w.nv, "faces": 1280, "dt_s": 1e-9, "D_m2_per_s": 1e-12,
+           "steps": K}
+    ends = {}
+    for prec, code in (("fp64", S.PREC_FP64), ("mixed", S.PREC_MIXED)):
+        s = S.Surface(w.mesh_path)
+        s.set_params(precision=code, **w.params)
+        s.generatePlan()
+        s.setg0(w.r0)
+        d_r = torch.tensor(w.r, device=dev, dtype=torch.float64)
+        d_E = torch.zeros(1, device=dev, dtype=torch.float64)
+        d_g = torch.zeros_like(d_r)
+        d_en = torch.zeros(K, device=dev, dtype=torch.float64)
+        s.energy_grad_dev(d_r, d_E, d_g, stream=sptr)     # warm the kernels
+        torch.cuda.synchronize()
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        s.energy_grad_dev(d_r, d_E, d_g, stream=sptr)
+        end.record(stream)
+        torch.cuda.synchronize()
+        eval_ms = start.elapsed_time(end)
+        s.set_step(0)
+        start.record(stream)
+        s.bd_steps_rng_dev(d_r, K, mob, amp, 21, d_en, sptr)
+        end.record(stream)
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(end) / K
+        e = d_en.cpu().numpy()
+        ends[prec] = d_r.cpu().numpy()
+        out[prec] = {"energy_grad_ms": eval_ms, "ms_per_step": ms, "steps_per_s": 1e3 / ms,
+                     "face_evals_per_s": s.nfaces * 1e3 / ms, "energy_first_last": [float(e[0]), float(e[-1])],
+                     "finite": bool(np.isfinite(e).all())}
+        s.close()
+    out["mixed"]["max_dr_vs_fp64_A"] = float(np.abs(ends["mixed"] - ends["fp64"]).max())
+    out["mixed"]["displacement_A"] = float(np.abs(ends["fp64"] - w.r).max())
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        o = O.OracleSurface(w.mesh_path)
+        o.set_params(**w.params)
+        o.setg0(w.r0)
+        nt = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))))
+        chunk = 25
+        xi = np.random.default_rng(2).standard_normal((chunk, 3 * o.nv))
+        base = {}
+        for name, cores in (("serial", 1), ("threads", nt)):
+            r = w.r.copy()
+            n, t0 = 0, time.perf_counter()
+            while n < K and time.perf_counter() - t0 < args.cpu_budget / 2:
+                r = o.bd_steps(r, xi, mob, amp) if cores == 1 else o.bd_steps_threads(r, xi, mob, amp, cores)
+                n += chunk
+            dt = time.perf_counter() - t0
+            base[name] = {"value": n / dt, "unit": "BD steps/s", "cores": cores, "kind": "port",
+                          "sample": "%d of the %d Brownian steps, %s oracle, %.1f s"
+                                    % (n, K, "serial" if cores == 1 else "%d-thread OpenMP" % cores, dt)}
+        out["cpu_baseline"] = dict(base["serial"], threads=base["threads"])
+        out["fp64"]["speedup_vs_cpu"] = out["fp64"]["steps_per_s"] / base["serial"]["value"]
+        out["fp64"]["speedup_vs_cpu_threads"] = out["fp64"]["steps_per_s"] / base["threads"]["value"]
+    return out
+
+
+def run_c2_leg(args, dev, torch):
+    """The periodic 128 x 128 hex sheet (16384 vertices, 32768 regular faces,
+    box 1000 x 866 A, Fourier heights of kc 20 kT at 310 K, lipid form): the
+    Langevin integrator of src/hd.C with the effective mass (OLD_LANGEVIN
+    friction -Delta gamma Mi pp / alpha^2, src/hd.C:1546-1557) at friction
+    gamma = 1e-10 kg/s, i.e. 1e-10 kg/s in amu per AKMA time unit (the
+    reference's unit of gamma_langevin: mass over time), then 10000 steps in
+    FP64 and in FP32-mixed.  The stated dt = 1e-9 s is tried first, then
+    shorter steps: each trial runs 200 steps from the start state and counts
+    as stable when every energy stays finite and within 100x the start
+    energy plus 3 nv kT; the timed runs use the largest stable dt.  The CPU
+    comparator is the serial oracle's Langevin step with the inverse from the
+    GPU set-up (the reference's dense O(nv^3) serial inversion is not timed)."""
+    import numpy as np
+    from umcp_amd import surface as S
+    from umcp_amd import units, workloads
     w = workloads.sheet_128()
-    D = 1.380649e-23 * w.T / 1e-10
-    dt = 3e-11
-    n3 = 3 * w.nv
-    gen = torch.Generator(device=dev)
-    gen.manual_seed(5)
-    d_xi = torch.randn((args.warmup + args.steps, n3), device=dev, dtype=torch.float64, generator=gen)
-    legs = {}
-    for prec in ("fp64", "mixed"):
-        leg, _, _ = run_leg(args, w, None, prec, d_xi, dev, torch, D=D, dt=dt)
-        leg["kernel"] = kernel_name(prec, leg["launch"]["point_split"], leg["launch"]["flags"], feat=1)
-        legs[prec] = leg
-    del d_xi
-    return {"workload": "sheet_128_periodic", "nv": w.nv, "faces": 2 * w.nv, "D_m2_per_s": D, "dt_s": dt,
-            "fp64": legs["fp64"], "mixed": legs["mixed"],
-            "mixed_speedup_vs_fp64": legs["mixed"]["face_evals_per_s"] / legs["fp64"]["face_evals_per_s"],
-            "energy_finite": bool(np.isfinite(legs["fp64"]["energy_first_last"]).all())}
+    gamma = 1e-10 / AMU_KG / units.AKMA_TIME      # amu per AKMA time unit
+    kT = units.kT(w.T)
+    K = args.c2_steps
+    stream = torch.cuda.Stream(device=dev)
+    sptr = stream.cuda_stream
+    out = {"workload": "sheet_128_periodic_langevin", "nv": w.nv, "faces": 2 * w.nv, "box_A": [1000.0, 866.0],
+           "gamma_kg_per_s": 1e-10, "gamma_langevin": gamma, "T_K": w.T, "dt_requested_s": 1e-9, "steps": K}
+    legs, trials, dt = {}, [], None
+    for prec, code in (("fp64", S.PREC_FP64), ("mixed", S.PREC_MIXED)):
+        s = S.Surface(w.mesh_path)
+        s.set_params(precision=code, **w.params)
+        s.generatePlan()
+        s.setg0(w.r0)
+        t0 = time.perf_counter()
+        nnz, m = s.effm_build()
+        setup_s = time.perf_counter() - t0
+        if dt is None:
+            e_cap = None
+            for cand in (1e-9, 3e-10, 1e-10, 3e-11, 1e-11, 3e-12, 1e-12):
+                d_r = torch.tensor(w.r, device=dev, dtype=torch.float64)
+                d_pp = torch.zeros(3 * s.nv, device=dev, dtype=torch.float64)
+                d_e = torch.zeros(200, device=dev, dtype=torch.float64)
+                s.set_step(0)
+                s.effm_steps_dev(d_r, d_pp, 200, S.EFFM_LD, cand, kT, gamma, seed=11, d_energies=d_e, stream=sptr)
+                torch.cuda.synchronize()
+                e = d_e.cpu().numpy()
+                if e_cap is None:
+                    e_cap = 100.0 * (abs(float(e[0])) + 3 * s.nv * kT)
+                ok = bool(np.isfinite(e).all() and torch.isfinite(d_r).all() and np.abs(e).max() < e_cap)
+                trials.append({"dt_s": cand, "stable": ok, "energy_last": float(e[-1])})
+                if ok:
+                    dt = cand
+                    break
+            if dt is None:
+                raise SystemExit("sheet Langevin: no stable time step down to 1e-12 s")
+        d_r = torch.tensor(w.r, device=dev, dtype=torch.float64)
+        d_pp = torch.zeros(3 * s.nv, device=dev, dtype=torch.float64)
+        d_e = torch.zeros(K, device=dev, dtype=torch.float64)
+        s.set_step(0)
+        s.effm_steps_dev(d_r, d_pp, 3, S.EFFM_LD, dt, kT, gamma, seed=12, stream=sptr)   # warm-up
+        d_r.copy_(torch.tensor(w.r, device=dev, dtype=torch.float64))
+        d_pp.zero_()
+        s.set_step(0)
+        torch.cuda.synchronize()
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        s.effm_steps_dev(d_r, d_pp, K, S.EFFM_LD, dt, kT, gamma, seed=12, d_energies=d_e, stream=sptr)
+        end.record(stream)
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(end) / K
+        e = d_e.cpu().numpy()
+        legs[prec] = {"ms_per_step": ms, "steps_per_s": 1e3 / ms, "face_evals_per_s": 2 * w.nv * 1e3 / ms,
+                      "setup_s": setup_s, "inverse_mass_kept": nnz, "point_mass_amu": m,
+                      "energy_first_last": [float(e[0]), float(e[-1])], "finite": bool(np.isfinite(e).all()),
+                      "r_end": d_r.cpu().numpy()}
+        if prec == "fp64" and not args.no_cpu_baseline:
+            Mi = s.effm_get()
+        s.close()
+        del d_r, d_pp, d_e
+    out["dt_trials"] = trials
+    out["dt_s"] = dt
+    r64, r32 = legs["fp64"].pop("r_end"), legs["mixed"].pop("r_end")
+    legs["mixed"]["max_dr_vs_fp64_A"] = float(np.abs(r32 - r64).max())
+    legs["mixed"]["energy_rel_vs_fp64_last"] = abs(legs["mixed"]["energy_first_last"][1] /
+                                                   legs["fp64"]["energy_first_last"][1] - 1)
+    legs["fp64"]["displacement_A"] = float(np.abs(r64 - w.r).max())
+    out["fp64"], out["mixed"] = legs["fp64"], legs["mixed"]
+    out["mixed_speedup_vs_fp64"] = legs["fp64"]["ms_per_step"] / legs["mixed"]["ms_per_step"]
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        o = O.OracleSurface(w.mesh_path)
+        o.set_params(**w.params)
+        o.setg0(w.r0)
+        n = 4
+        xi = np.random.default_rng(3).standard_normal((n, 3 * o.nv))
+        t0 = time.perf_counter()
+        o.effm_steps(w.r, np.zeros(3 * o.nv), xi, S.EFFM_LD, dt, kT, gamma, Mi)
+        t = time.perf_counter() - t0
+        del Mi
+        out["cpu_baseline"] = {"value": n / t, "unit": "Langevin steps/s", "cores": 1, "kind": "port",
+                               "sample": "%d Langevin steps of the sheet, serial oracle, inverse effective mass "
+                                         "from the GPU set-up, %.1f s" % (n, t)}
+        out["fp64"]["speedup_vs_cpu"] = out["fp64"]["steps_per_s"] / out["cpu_baseline"]["value"]
+    return out
 
 
 def run_points_leg(args, dev, torch):
@@ -529,17 +707,20 @@ def run_serving_leg(args, dev, torch):
     """Serving stream: Poisson arrivals of Brownian-dynamics requests
     (icospheres of level 2-6 or periodic sheets of 32^2..256^2 vertices,
     log-uniform steps in [100, 10000], per-request kc and c0), FP32-mixed,
-    through the serving loop with continuous batching, then through the same
-    loop one request at a time.  Reports latency percentiles (submit to done),
-    aggregate request and face-eval throughput over the makespan, and that
-    every request's final positions are the same in both runs."""
+    through the serving loop with continuous batching.  The stream of
+    --serve-requests at --serve-rate gives the latency percentiles (submit to
+    done) and aggregate throughput over the makespan; its first
+    --serve-alone requests also run through the same loop one at a time (the
+    comparator: face-evals/s of the two runs, since the subset's mix differs
+    from the whole stream's; and every request's result equal in both).  A
+    latency-versus-load curve repeats the stream's first --serve-curve
+    requests at 10, 20, 50 and 200 requests/s, and a stream of small requests
+    (levels 2-3, sheets of 32^2..64^2, 100-1000 steps) runs both ways."""
     import numpy as np
     from umcp_amd import surface as S
     from umcp_amd import workloads
 
-    reqs = workloads.serving(n=args.serve_requests, rate=args.serve_rate)
-
-    def run(slots):
+    def run(reqs, slots):
         sv = S.Server(max_active=slots, chunk_steps=args.serve_chunk, precision=S.PREC_MIXED)
         # the mesh library is loaded before the stream starts: one zero-step
         # request per distinct mesh file parses and plans its template
@@ -567,8 +748,10 @@ def run_serving_leg(args, dev, torch):
         face_steps = float(sum(q.nfaces * q.steps for q in reqs))
         return {
             "slots": slots,
+            "requests": len(reqs),
             "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                          "mean": float(lat.mean()), "max": float(lat.max())},
+                          "mean": float(lat.mean()), "max": float(lat.max()), "count": len(lat),
+                          "above_p99": int((lat > np.percentile(lat, 99)).sum())},
             "queue_wait_s_p50": float(np.percentile(np.array([t[1] - t[0] for _, _, t in res]), 50)),
             "makespan_s": float(makespan),
             "requests_per_s": len(reqs) / makespan,
@@ -580,10 +763,27 @@ def run_serving_leg(args, dev, torch):
             "peak_reserved_GB": st["peak_bytes"] / 1e9,
         }, res
 
-    batched, res_b = run(args.serve_slots)
-    single, res_1 = run(1)
-    dr = max(float(np.abs(a[0] - b[0]).max()) for a, b in zip(res_b, res_1))
-    same_e = all(np.array_equal(a[1], b[1]) for a, b in zip(res_b, res_1))
+    def same(ra, rb):
+        return {"max_abs_dr_A": max(float(np.abs(a[0] - b[0]).max()) for a, b in zip(ra, rb)),
+                "energies_bitwise_equal": bool(all(np.array_equal(a[1], b[1]) for a, b in zip(ra, rb)))}
+
+    reqs = workloads.serving(n=args.serve_requests, rate=args.serve_rate)
+    batched, res_b = run(reqs, args.serve_slots)
+    sub = reqs[:args.serve_alone]
+    single, res_1 = run(sub, 1)
+    curve = []
+    for rate in (10.0, 20.0, 50.0, 200.0):
+        rq = workloads.serving(n=args.serve_curve, rate=rate)
+        c, _ = run(rq, args.serve_slots)
+        curve.append({"offered_requests_per_s": rate, "requests": c["requests"],
+                      "p50_s": c["latency_s"]["p50"], "p99_s": c["latency_s"]["p99"],
+                      "max_s": c["latency_s"]["max"], "requests_per_s": c["requests_per_s"],
+                      "face_evals_per_s": c["face_evals_per_s"], "queue_wait_s_p50": c["queue_wait_s_p50"],
+                      "peak_active": c["peak_active"]})
+    small = workloads.serving(n=args.serve_small, rate=args.serve_rate, seed=9, levels=(2, 3),
+                              sheet_sides=(32, 64), steps_range=(100, 1000))
+    sm_b, rsb = run(small, args.serve_slots)
+    sm_1, rs1 = run(small, 1)
     return {
         "workload": "serving_%d_requests_poisson_%g_per_s" % (len(reqs), args.serve_rate),
         "precision": "mixed",
@@ -592,16 +792,21 @@ def run_serving_leg(args, dev, torch):
         "sheets": sum(1 for q in reqs if q.kind == "sheet"),
         "face_steps": float(sum(q.nfaces * q.steps for q in reqs)),
         "continuous_batching": batched,
-        "one_at_a_time": single,
-        "throughput_ratio": batched["requests_per_s"] / single["requests_per_s"],
-        "same_result_alone_and_in_mix": {"max_abs_dr_A": dr, "energies_bitwise_equal": bool(same_e)},
+        "one_at_a_time": dict(single, note="the stream's first %d requests" % len(sub)),
+        "throughput_ratio": batched["face_evals_per_s"] / single["face_evals_per_s"],
+        "throughput_ratio_unit": "face-evals/s of the stream over face-evals/s of its first requests one at a time",
+        "same_result_alone_and_in_mix": same(res_b[:len(sub)], res_1),
+        "curve": curve,
+        "small_requests": {"requests": len(small), "continuous_batching": sm_b, "one_at_a_time": sm_1,
+                           "throughput_ratio": sm_b["requests_per_s"] / sm_1["requests_per_s"],
+                           "same_result_alone_and_in_mix": same(rsb, rs1)},
     }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
@@ -617,11 +822,17 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=50)
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream leg")
     ap.add_argument("--no-effm", action="store_true", help="skip the effective-mass integrator leg")
-    ap.add_argument("--no-sheet", action="store_true", help="skip the periodic-sheet leg")
+    ap.add_argument("--no-sheet", action="store_true", help="skip the periodic-sheet Langevin leg")
+    ap.add_argument("--c2-steps", type=int, default=10000)
+    ap.add_argument("--no-c1", action="store_true", help="skip the 642-vertex Brownian leg")
+    ap.add_argument("--c1-steps", type=int, default=1000)
     ap.add_argument("--effm-steps", type=int, default=500)
     ap.add_argument("--no-points", action="store_true", help="skip the surface-points leg")
     ap.add_argument("--points-n", type=int, default=1000000)
-    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-requests", type=int, default=512)
+    ap.add_argument("--serve-alone", type=int, default=128, help="requests of the stream also run one at a time")
+    ap.add_argument("--serve-curve", type=int, default=128, help="requests per point of the load curve")
+    ap.add_argument("--serve-small", type=int, default=256, help="requests of the small-request stream")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-slots", type=int, default=64)
     ap.add_argument("--serve-chunk", type=int, default=100)
@@ -698,8 +909,10 @@ def main():
         out["ensemble"] = run_ensemble_leg(args, dev, torch)
     if not args.no_serving:
         out["serving"] = run_serving_leg(args, dev, torch)
+    if not args.no_c1:
+        out["c1"] = run_c1_leg(args, dev, torch)
     if not args.no_sheet:
-        out["sheet"] = run_sheet_leg(args, dev, torch)
+        out["sheet"] = run_c2_leg(args, dev, torch)
     if not args.no_effm:
         out["effm"] = run_effm_leg(args, dev, torch)
     if not args.no_points:

@@ -79,6 +79,7 @@ def lib():
         L.orc_restraint_energy.restype = C.c_double
         L.orc_energy_grad_threads.argtypes = [vp, dp, dp, C.c_int]
         L.orc_energy_grad_threads.restype = C.c_double
+        L.orc_bd_steps_threads.argtypes = [vp, dp, dp, C.c_int, C.c_double, C.c_double, dp, C.c_int]
         L.orc_effm_points_per_face.restype = C.c_int
         L.orc_effm_point_mass.argtypes = [vp]
         L.orc_effm_point_mass.restype = C.c_double
@@ -224,6 +225,15 @@ class OracleSurface:
         lib().orc_bd_steps(self._h, _p(r), _p(xi), n, mobility_dt, noise_amp,
                            _p(en) if en is not None else None)
         return (r, en) if want_energies else r
+
+    def bd_steps_threads(self, r: np.ndarray, xi: np.ndarray, mobility_dt: float, noise_amp: float,
+                         nthreads: int) -> np.ndarray:
+        """bd_steps with each energy + gradient over nthreads OpenMP threads."""
+        r = np.ascontiguousarray(r, dtype=np.float64).copy()
+        xi = np.ascontiguousarray(xi, dtype=np.float64)
+        n = xi.size // (3 * self.nv)
+        lib().orc_bd_steps_threads(self._h, _p(r), _p(xi), n, mobility_dt, noise_amp, None, int(nthreads))
+        return r
 
     def energy_grad_threads(self, r: np.ndarray, nthreads: int):
         """(E, g) with the faces split over nthreads OpenMP threads."""

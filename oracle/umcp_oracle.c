@@ -1948,6 +1948,22 @@ void orc_bd_steps(omesh *m, double *r, const double *xi, int nsteps, double mobi
     free(g);
 }
 
+/* the same steps with each energy + gradient split over nthreads OpenMP
+ * threads (orc_energy_grad_threads): the threaded CPU comparator */
+void orc_bd_steps_threads(omesh *m, double *r, const double *xi, int nsteps, double mobility_dt,
+                          double noise_amp, double *energies, int nthreads)
+{
+    int n3 = 3 * m->nv;
+    double *g = malloc(sizeof(double) * (size_t)(n3 + 3));
+    for (int s = 0; s < nsteps; s++) {
+        const double e = orc_energy_grad_threads(m, r, g, nthreads);
+        if (energies) energies[s] = e;
+        const double *x = xi + (size_t)s * n3;
+        for (int i = 0; i < n3; i++) r[i] += -mobility_dt * g[i] + noise_amp * x[i];
+    }
+    free(g);
+}
+
 /* ---- effective-mass integrators (src/hd.C:418-481, 1543-1720, 1943-1947; */
 /* src/modes.C:1719-1960; src/uv_map.C:14-200; src/parallel.C:1028-1075)    */
 

@@ -83,6 +83,8 @@ void   orc_effm_steps(omesh *m, double *r, double *pp, const double *xi, int nst
 /* energy + gradient with the faces split over nthreads OpenMP threads
  * (per-thread gradient buffers summed in thread order); g overwritten */
 double orc_energy_grad_threads(omesh *m, const double *r, double *g, int nthreads);
+void   orc_bd_steps_threads(omesh *m, double *r, const double *xi, int nsteps, double mobility_dt,
+                            double noise_amp, double *energies, int nthreads);
 
 /* introspection for tests */
 int    orc_face_info(const omesh *m, int f, int *owner, int *ncp, int *cp, int *tri);

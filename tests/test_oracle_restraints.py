@@ -196,3 +196,17 @@ def test_threaded_energy_grad_equals_serial():
         Et, gt = o.energy_grad_threads(r, n)
         assert abs(Et - E) <= 1e-12 * abs(E)
         assert np.abs(gt - g).max() <= 1e-12 * np.abs(g).max()
+
+
+def test_threaded_bd_steps_equal_serial():
+    """The threaded Brownian steps (the multi-core comparator of the step
+    benchmarks) follow the serial oracle's trajectory."""
+    w = workloads.vesicle_642()
+    o = O.OracleSurface(w.mesh_path)
+    o.set_params(**w.params)
+    o.setg0(w.r0)
+    xi = np.random.default_rng(3).standard_normal((5, 3 * o.nv))
+    r1 = o.bd_steps(w.r, xi, 0.05, 0.02)
+    for n in (1, 4):
+        rn = o.bd_steps_threads(w.r, xi, 0.05, 0.02, n)
+        assert np.abs(rn - r1).max() <= 1e-12 * np.abs(r1 - w.r).max()
