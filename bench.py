@@ -42,6 +42,14 @@ METRIC = "energy+grad face-evals/s & BD steps/s, FP64 vs FP32-mixed; p99 request
 C_PT = 700
 
 
+_T0 = time.perf_counter()
+
+
+def progress(msg: str):
+    """One progress line on stderr per leg (stdout carries only the JSON)."""
+    print("[bench %6.1f s] %s" % (time.perf_counter() - _T0, msg), file=sys.stderr, flush=True)
+
+
 def face_flops(ncp: int, npt: int) -> int:
     return npt * (2 * 6 * ncp * 3 + 2 * 5 * ncp * 3 + 2 * 5 * ncp * 3 + C_PT)
 
@@ -250,6 +258,7 @@ def run_ensemble_leg(args, dev, torch):
     torch.cuda.synchronize()
     tuned = None
     if not args.no_tune:
+        progress("  ensemble launch tuner")
         best, times = tune.tune_ensemble(ens, surfs, d_rs, torch, stream, shapes=tune.ENSEMBLE_SHAPES_MIXED)
         tuned = {"best": list(best), "ms_per_step": {"%d/%d/%d/%d" % k: round(v, 4) for k, v in times.items()}}
     nf = sum(m.nfaces for m in members)
@@ -296,6 +305,7 @@ def run_ensemble_leg(args, dev, torch):
     seq_ms_per_step = sum(rate_alone[m.level] for m in members)
 
     # mixed against FP64, per member, over the first Kc steps from the start
+    progress("  ensemble: mixed against FP64")
     Kc = 20
     res = {}
     for prec in (S.PREC_FP64, S.PREC_MIXED):
@@ -319,6 +329,7 @@ def run_ensemble_leg(args, dev, torch):
     levels = {str(L): sum(1 for m in members if m.level == L) for L in sorted({m.level for m in members})}
     cpu = None
     if not args.no_cpu_baseline:
+        progress("  ensemble: CPU baseline")
         # the serial oracle's Brownian step, member after member in the
         # ensemble's order until the budget is spent; the ensemble step's CPU
         # time is projected from the measured face-evals/s
@@ -675,7 +686,7 @@ def run_effm_leg(args, dev, torch):
     end.record(stream)
     torch.cuda.synchronize()
     ms_l = start.elapsed_time(end) / K_l
-    bytes_l = 12 * nnz_l if nnz_l * 3 < 2 * sl.nv ** 2 else 8 * sl.nv ** 2
+    bytes_l = 10 * nnz_l if 5 * nnz_l < 4 * sl.nv ** 2 else 8 * sl.nv ** 2
     out["vesicle_40962"] = {"nv": sl.nv, "inverse_mass_kept": nnz_l, "kept_fraction": nnz_l / sl.nv ** 2,
                             "setup_s": build_l, "dt_s": dt_l, "langevin_steps": K_l,
                             "langevin_ms_per_step": ms_l, "inverse_bytes_per_product": bytes_l,
@@ -715,7 +726,8 @@ def run_serving_leg(args, dev, torch):
     from the whole stream's; and every request's result equal in both).  A
     latency-versus-load curve repeats the stream's first --serve-curve
     requests at 10, 20, 50 and 200 requests/s, and a stream of small requests
-    (levels 2-3, sheets of 32^2..64^2, 100-1000 steps) runs both ways."""
+    (levels 2-3, sheets of 32^2..64^2, 100-1000 steps) offered at 2000/s runs
+    both ways."""
     import numpy as np
     from umcp_amd import surface as S
     from umcp_amd import workloads
@@ -768,20 +780,25 @@ def run_serving_leg(args, dev, torch):
                 "energies_bitwise_equal": bool(all(np.array_equal(a[1], b[1]) for a, b in zip(ra, rb)))}
 
     reqs = workloads.serving(n=args.serve_requests, rate=args.serve_rate)
+    progress("  %d requests at %g/s, continuous batching" % (len(reqs), args.serve_rate))
     batched, res_b = run(reqs, args.serve_slots)
     sub = reqs[:args.serve_alone]
+    progress("  the first %d one at a time" % len(sub))
     single, res_1 = run(sub, 1)
     curve = []
     for rate in (10.0, 20.0, 50.0, 200.0):
         rq = workloads.serving(n=args.serve_curve, rate=rate)
+        progress("  load curve: %d requests at %g/s" % (len(rq), rate))
         c, _ = run(rq, args.serve_slots)
         curve.append({"offered_requests_per_s": rate, "requests": c["requests"],
                       "p50_s": c["latency_s"]["p50"], "p99_s": c["latency_s"]["p99"],
                       "max_s": c["latency_s"]["max"], "requests_per_s": c["requests_per_s"],
                       "face_evals_per_s": c["face_evals_per_s"], "queue_wait_s_p50": c["queue_wait_s_p50"],
                       "peak_active": c["peak_active"]})
-    small = workloads.serving(n=args.serve_small, rate=args.serve_rate, seed=9, levels=(2, 3),
+    # offered faster than either way serves them, so both runs measure capacity
+    small = workloads.serving(n=args.serve_small, rate=2000.0, seed=9, levels=(2, 3),
                               sheet_sides=(32, 64), steps_range=(100, 1000))
+    progress("  %d small requests, both ways" % len(small))
     sm_b, rsb = run(small, args.serve_slots)
     sm_1, rs1 = run(small, 1)
     return {
@@ -864,6 +881,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    progress("40962-vertex vesicle, FP64: tuning and %d Brownian steps" % K)
     leg64, E64, g64 = run_leg(args, w, c0, "fp64", d_xi, dev, torch)
     face_s = leg64["face_kernel_us"] / 1e6
     achieved = leg64["flop_per_launch"] / face_s / 1e12
@@ -899,6 +917,7 @@ def main():
                      "algorithmic_bytes_per_launch": leg64["bytes_per_launch"]},
     }
     if not args.no_mixed:
+        progress("40962-vertex vesicle, FP32-mixed")
         legm, Em, gm = run_leg(args, w, c0, "mixed", d_xi, dev, torch)
         legm["speedup_vs_fp64"] = legm["face_evals_per_s"] / leg64["face_evals_per_s"]
         legm["face_kernel_speedup_vs_fp64"] = leg64["face_kernel_us"] / legm["face_kernel_us"]
@@ -906,18 +925,25 @@ def main():
         legm["traffic"] = measured_traffic(legm["kernel"])
         out["mixed"] = legm
     if not args.no_ensemble:
+        progress("batched ensemble")
         out["ensemble"] = run_ensemble_leg(args, dev, torch)
     if not args.no_serving:
+        progress("serving stream")
         out["serving"] = run_serving_leg(args, dev, torch)
     if not args.no_c1:
+        progress("642-vertex vesicle, Brownian steps")
         out["c1"] = run_c1_leg(args, dev, torch)
     if not args.no_sheet:
+        progress("128^2 sheet, Langevin steps")
         out["sheet"] = run_c2_leg(args, dev, torch)
     if not args.no_effm:
+        progress("effective-mass integrators")
         out["effm"] = run_effm_leg(args, dev, torch)
     if not args.no_points:
+        progress("surface points")
         out["points"] = run_points_leg(args, dev, torch)
     if not args.no_cpu_baseline:
+        progress("CPU baseline of the 40962-vertex vesicle")
         base, E_o, g_o = cpu_baseline(w, c0, args.cpu_budget)
         out["cpu_baseline"] = base
 
@@ -928,6 +954,7 @@ def main():
         if "mixed" in out:
             out["mixed"]["vs_oracle"] = {"energy_rel": abs(Em - E_o) / abs(E_o),
                                          "grad_rms_rel": rms(gm[:-3] - g_o[:-3]) / rms(g_o[:-3])}
+    progress("done")
     print(json.dumps(out), flush=True)
 
 

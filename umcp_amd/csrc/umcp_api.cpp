@@ -1017,10 +1017,10 @@ int umcp_effm_build(umcp_surface *s, long *nnz, double *point_mass)
     s->d_effm_rowptr = nullptr;
     s->d_effm_col = nullptr;
     s->d_effm_val = nullptr;
-    // CSR (12 bytes per kept entry) when that reads less than the dense array
-    // (8 bytes per entry): the inverse of a large mesh's mass matrix decays
-    // below the truncation far from the diagonal
-    if (3 * kept < 2 * (long)n * n && kept < (long)INT32_MAX) {
+    // CSR (10 bytes per kept entry: 16-bit columns) when that reads less than
+    // the dense array (8 bytes per entry): the inverse of a large mesh's mass
+    // matrix decays below the truncation far from the diagonal
+    if (5 * kept < 4 * (long)n * n && kept < (long)INT32_MAX) {
         std::vector<int> rp((size_t)n + 1, 0);
         for (int i = 0; i < n; i++) rp[(size_t)i + 1] = rp[(size_t)i] + cnt[(size_t)i];
         const size_t before2 = s->allocations.size();
@@ -1058,10 +1058,11 @@ int umcp_effm_get(umcp_surface *s, double *Minv)
         if (e != hipSuccess) return hip_fail(s, e, "download effective mass");
         return UMCP_OK;
     }
-    std::vector<int> rp(n + 1), col((size_t)s->effm_nnz);
+    std::vector<int> rp(n + 1);
+    std::vector<uint16_t> col((size_t)s->effm_nnz);
     std::vector<double> val((size_t)s->effm_nnz);
     hipError_t e = hipMemcpy(rp.data(), s->d_effm_rowptr, sizeof(int) * rp.size(), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(col.data(), s->d_effm_col, sizeof(int) * col.size(), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(col.data(), s->d_effm_col, sizeof(uint16_t) * col.size(), hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(val.data(), s->d_effm_val, sizeof(double) * val.size(), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(s, e, "download effective mass");
     std::memset(Minv, 0, sizeof(double) * n * n);

@@ -401,8 +401,8 @@ __global__ __launch_bounds__(kEffmBlock) void effm_truncate_kernel(int n, double
 
 // one block per row: chunks of kEffmBlock columns, a block prefix count of
 // the non-zero entries gives each kept entry its place
-__global__ __launch_bounds__(kEffmBlock) void effm_compact_kernel(int n, const double *A, const int *rowptr, int *col,
-                                                                  double *val)
+__global__ __launch_bounds__(kEffmBlock) void effm_compact_kernel(int n, const double *A, const int *rowptr,
+                                                                  uint16_t *col, double *val)
 {
     __shared__ int pre[kEffmBlock];
     const int i = blockIdx.x;
@@ -422,7 +422,7 @@ __global__ __launch_bounds__(kEffmBlock) void effm_compact_kernel(int n, const d
         }
         if (keep) {
             const int at = base + pre[threadIdx.x] - 1;
-            col[at] = j;
+            col[at] = (uint16_t)j;   // n <= 65536
             val[at] = x;
         }
         base += pre[kEffmBlock - 1];
@@ -430,8 +430,9 @@ __global__ __launch_bounds__(kEffmBlock) void effm_compact_kernel(int n, const d
     }
 }
 
-hipError_t launch_effm_compact(int n, const double *A, const int *rowptr, int *col, double *val, hipStream_t st)
+hipError_t launch_effm_compact(int n, const double *A, const int *rowptr, uint16_t *col, double *val, hipStream_t st)
 {
+    if (n > 65536) return hipErrorInvalidValue;   // 16-bit column indices
     hipLaunchKernelGGL(effm_compact_kernel, dim3(n), dim3(kEffmBlock), 0, st, n, A, rowptr, col, val);
     return hipGetLastError();
 }

@@ -121,7 +121,8 @@ struct EffmArgs {
     // entries zero) or, when the truncation keeps few enough entries, CSR
     // (rowptr, col, val; columns ascending)
     const double *Mi;
-    const int *rowptr, *col;
+    const int *rowptr;
+    const uint16_t *col;    // column indices (nv <= 65536)
     const double *val;
     const double *pp_next;  // momenta of this step (from the gather)
     double *pp_keep;        // momenta carried to the next step (the caller's array)
@@ -144,7 +145,7 @@ hipError_t launch_effm_symmetrize(int n, double *A, hipStream_t st);
 hipError_t launch_effm_truncate(int n, double *A, double rel_tol, int *rowcount, hipStream_t st);
 // the kept entries of the truncated dense array as CSR (rowptr from the
 // per-row counts), columns ascending
-hipError_t launch_effm_compact(int n, const double *A, const int *rowptr, int *col, double *val, hipStream_t st);
+hipError_t launch_effm_compact(int n, const double *A, const int *rowptr, uint16_t *col, double *val, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // points on the limit surface: surface::evaluateRNRM (src/interpLib.C:

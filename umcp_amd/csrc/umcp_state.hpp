@@ -89,7 +89,8 @@ struct umcp_surface {
     bool effm_ready = false;
     double effm_point_mass = 0.0;
     double *d_effm = nullptr;             // nv x nv inverse effective mass (row-major), or
-    int *d_effm_rowptr = nullptr, *d_effm_col = nullptr;   // its kept entries as CSR
+    int *d_effm_rowptr = nullptr;                          // its kept entries as CSR
+    uint16_t *d_effm_col = nullptr;                        // (16-bit columns: nv <= 65536)
     double *d_effm_val = nullptr;
     double *d_pp_next = nullptr, *d_mp = nullptr;
     long effm_nnz = 0;
