@@ -164,15 +164,15 @@ int umcp_bd_steps_dev(umcp_surface *s, double *d_r, const double *d_xi, int nste
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under stream capture, serving loops that forbid allocation):
  * umcp_energy_grad_dev on a planned, setg0'd surface, calling no hipMalloc or
- * hipFree.  Its work buffers (face tables, (face, slot) gradient records,
- * block and restraint partials) live in one static device array of
+ * hipFree.  Its work buffers (face tables, gradient records, block and
+ * restraint partials) live in one static device array of
  * UMCP_NOALLOC_SCRATCH_BYTES per device instead of the handle's buffers;
  * results are bitwise those of umcp_energy_grad_dev.  A surface whose buffers
- * do not fit (about 3 nnz doubles, nnz = sum of control points over faces:
- * roughly 56k vertices in FP64, twice that mixed) returns UMCP_ERR_ARG.  All
- * calls on one device share its array and keep the last call's contents:
- * calls that overlap on one device (other streams or threads) race. */
-#define UMCP_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+ * do not fit (about three records of 3 doubles per vertex: roughly 110k
+ * vertices in FP64, twice that mixed) returns UMCP_ERR_ARG.  All calls on one
+ * device share its array and keep the last call's contents: calls that
+ * overlap on one device (other streams or threads) race. */
+#define UMCP_NOALLOC_SCRATCH_BYTES ((size_t)8 << 20)
 int umcp_energy_grad_dev_noalloc(umcp_surface *s, const double *d_r, double *d_E, double *d_g,
                                  void *stream);
 
@@ -455,9 +455,13 @@ void umcp_server_destroy(umcp_server *sv);
  * registers (two waves per SIMD in FP64, four in the mixed VALU path, two on
  * the matrix-core path); UMCP_LAUNCH_MFMA (mixed precision only, ignored in
  * FP64) runs the basis contractions on the matrix cores, with point_split 1,
- * 2 or 4 */
+ * 2 or 4; UMCP_LAUNCH_CPS (point_split 2, 3 or 4, not with MFMA): the waves
+ * of a group also split the control points, each accumulating the gradient
+ * of its own share over every Gauss point (the points' reverse passes
+ * through LDS), which needs about half the registers */
 #define UMCP_LAUNCH_OCC2 2
 #define UMCP_LAUNCH_MFMA 4
+#define UMCP_LAUNCH_CPS 8
 int umcp_set_launch(umcp_surface *s, int point_split, int flags);
 int umcp_get_launch(const umcp_surface *s, int *point_split, int *flags);
 

@@ -189,6 +189,19 @@ def test_mixed_ensemble_close_to_fp64(mix):
         assert dr <= 1e-5
 
 
+@pytest.mark.parametrize("split,precision", [(2, S.PREC_FP64), (3, S.PREC_FP64), (4, S.PREC_MIXED)])
+def test_cps_ensemble_equals_alone(mix, split, precision):
+    """The control-point split form runs the same items alone and batched
+    (persistent grid, graph replay): each member's trajectory is bitwise the
+    one it has alone with that launch."""
+    sps, _ = mix
+    launch = (split, S.LAUNCH_CPS)
+    say("[ensemble] control-point split", split, "precision", precision)
+    alone = [run_alone(sp, STEPS, precision, launch) for sp in sps]
+    got, _ = run_ensemble(sps, STEPS, graph_steps=4, face_grid=48, precision=precision, chunks=2, launch=launch)
+    check_same("cps", got, alone)
+
+
 @pytest.mark.parametrize("split", [2, 4])
 def test_mfma_ensemble_equals_alone(mix, split):
     """The matrix-core mixed path runs the same items alone and batched: each

@@ -67,7 +67,8 @@ def vesicle_40962_pair():
 
 # launch shapes: VALU path (flags 0 / OCC2) and matrix-core path (MFMA)
 SHAPES = [(s, f) for s in (1, 2, 3, 4) for f in (0, S.LAUNCH_OCC2)] + \
-         [(s, f) for s in (1, 2, 4) for f in (S.LAUNCH_MFMA, S.LAUNCH_MFMA | S.LAUNCH_OCC2)]
+         [(s, f) for s in (1, 2, 4) for f in (S.LAUNCH_MFMA, S.LAUNCH_MFMA | S.LAUNCH_OCC2)] + \
+         [(s, S.LAUNCH_CPS) for s in (2, 3, 4)]
 
 
 def errors_cached(g, ref, r, tag):
@@ -79,7 +80,8 @@ def errors_cached(g, ref, r, tag):
     return rel_e, rms
 
 
-@pytest.mark.parametrize("split,flags", [(2, S.LAUNCH_OCC2), (2, S.LAUNCH_MFMA), (4, S.LAUNCH_MFMA)])
+@pytest.mark.parametrize("split,flags", [(2, S.LAUNCH_OCC2), (2, S.LAUNCH_MFMA), (4, S.LAUNCH_MFMA),
+                                         (3, S.LAUNCH_CPS)])
 def test_mixed_vesicle_40962(vesicle_40962_pair, split, flags):
     w, g, o, ref = vesicle_40962_pair
     g.set_launch(split, flags)

@@ -77,14 +77,17 @@ def test_vesicle_642_lipid_mode(ves642):
         assert g.face_lipids(f) == o.tri_lipids(t)
 
 
+# every FP64 launch shape: point split across 1-4 waves with or without the
+# register bound, and the control-point split over 2-4 waves
+SHAPES = [(s, f) for s in (1, 2, 3, 4) for f in (0, S.LAUNCH_OCC2)] + [(s, S.LAUNCH_CPS) for s in (2, 3, 4)]
+
+
 def test_launch_shapes(ves642):
-    """Every launch shape (point split across 1-4 waves, register bound for 1
-    or 2 waves/SIMD) meets the bar."""
+    """Every launch shape meets the bar."""
     g, o = make_pair(ves642)
-    for split in (1, 2, 3, 4):
-        for flags in (0, 2):
-            g.set_launch(split, flags)
-            check_eval(g, o, ves642.r, "vesicle_642 split %d flags %d" % (split, flags))
+    for split, flags in SHAPES:
+        g.set_launch(split, flags)
+        check_eval(g, o, ves642.r, "vesicle_642 split %d flags %d" % (split, flags))
 
 
 GRAD_FORMS = [S.GRAD_REFERENCE, S.GRAD_EXACT]
@@ -143,10 +146,9 @@ def test_valence_5_and_7(form):
         o.set_restraints(**rst)
     vals = sorted(c["valence"] for c in g.classes())
     assert vals == [5, 6, 7], vals
-    for split in (1, 2, 3, 4):
-        for flags in (0, S.LAUNCH_OCC2):
-            g.set_launch(split, flags)
-            check_eval(g, o, w.r, "defect %s split %d flags %d" % (form, split, flags))
+    for split, flags in SHAPES:
+        g.set_launch(split, flags)
+        check_eval(g, o, w.r, "defect %s split %d flags %d" % (form, split, flags))
 
 
 def test_periodic_sheet_small():

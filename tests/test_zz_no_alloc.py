@@ -66,21 +66,22 @@ def test_noalloc_equals_allocating_entry_point(mesh_dir, precision):
 
 
 def test_noalloc_rejects_a_surface_beyond_the_array(mesh_dir):
-    """The 256^2 periodic sheet needs 3 nnz = 4.7M doubles of gradient records
-    (37.7 MB) in FP64, beyond the 32 MiB array: rejected; in mixed precision
-    (4-byte records, 18.9 MB) it fits and matches the allocating call."""
-    w = workloads.sheet(256, name="sheet_256_noalloc")
+    """The 163842-vertex icosphere has about 480k gradient records (one per
+    distinct vertex of each 64-face group): 11.5 MB in FP64, beyond the 8 MiB
+    array: rejected; in mixed precision (4-byte records, 5.8 MB) it fits and
+    matches the allocating call."""
+    w = workloads.vesicle(7, 2000.0, 2.0, seed=5, name="vesicle_163842_noalloc")
     g = _surface(w, S.PREC_FP64)
     d_r = torch.tensor(w.r, device="cuda", dtype=torch.float64)
     d_E = torch.zeros(1, device="cuda", dtype=torch.float64)
     d_g = torch.zeros(w.r.size, device="cuda", dtype=torch.float64)
-    say("[noalloc] 256^2 sheet FP64 (rejected)")
+    say("[noalloc] 163842-vertex icosphere FP64 (rejected)")
     with pytest.raises(S.UmcpError, match="scratch"):
         g.energy_grad_dev_noalloc(d_r, d_E, d_g)
     g.energy_grad_dev(d_r, d_E, d_g)
     torch.cuda.synchronize()
     assert np.isfinite(float(d_E.cpu()[0]))
-    say("[noalloc] 256^2 sheet mixed")
+    say("[noalloc] 163842-vertex icosphere mixed")
     gm = _surface(w, S.PREC_MIXED)
     (E1, g1), (E2, g2) = _both(gm, w.r)
     assert E1 == E2 and np.array_equal(g1, g2)

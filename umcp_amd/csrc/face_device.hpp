@@ -173,8 +173,9 @@ __device__ __forceinline__ void gather_cps(const FaceArgs &a, const double *__re
 // partials (area: src/interpLib.C:4322-4326; volume: src/minimize.C:90-114):
 // A_tot and V summed in one fixed order -- lane l takes groups l, l + 64, ...
 // in turn, then an xor butterfly over the wave, whose every lane ends with
-// the same bits since IEEE addition commutes -- so any wave of any block gets
-// the same values.  out = {dE/dA_tot, dE/dV, restraint energy, A_tot, V}.
+// the same bits since IEEE addition commutes -- so the result does not depend
+// on which block forms it.  out = {dE/dA_tot, dE/dV, restraint energy, A_tot,
+// V}.
 __device__ __forceinline__ void restraint_scalars(const FaceArgs &a, double out[5])
 {
     #pragma clang fp contract(off)
@@ -216,26 +217,98 @@ __device__ __forceinline__ void restraint_scalars(const FaceArgs &a, double out[
     out[4] = sV;
 }
 
-// The multipliers a face block uses: formed from the partials when the
-// surface folds its finalize (the slot-0 group also publishes all five
-// scalars in rs for the gather and the host), else read from rs.
-__device__ __forceinline__ void restraint_multipliers(const FaceArgs &a, int slot, double lam[2])
+// The multipliers a face block uses, written by the restraint pre-pass.
+__device__ __forceinline__ void restraint_multipliers(const FaceArgs &a, double lam[2])
 {
     lam[0] = lam[1] = 0.0;
     if (!a.rs) return;
-    if (a.rs_fold) {
-        double o[5];
-        restraint_scalars(a, o);
-        if (slot == 0 && threadIdx.x == 0)
-#pragma unroll
-            for (int i = 0; i < 5; i++) a.rs[i] = o[i];
-        lam[0] = o[0];
-        lam[1] = o[1];
-    } else {
-        lam[0] = a.rs[0];
-        lam[1] = a.rs[1];
-    }
+    lam[0] = a.rs[0];
+    lam[1] = a.rs[1];
 }
+
+// The gradient records of one 64-face group: T[(c * 3 + k) * 64 + lane] holds
+// dE/dP of slot c of the group's face lane (LDS); record L of the group's
+// range sums its vertex's incidences in their fixed (slot-major) order in
+// FP64 and is scaled by alpha once.  NT threads share the work in two parts:
+// RecordLists::load issues the loads of the group's incidence lists (early,
+// so their latency overlaps the caller's last LDS work), RecordLists::sum
+// stores them in LDS scratch (kRecScratchBytes, disjoint from T), waits for
+// the block and forms and writes the records.  T must be complete and
+// visible to the block when sum is called.
+constexpr int kRecIncMax = 13 * kWave;        // incidences of a group
+constexpr int kRecLocMax = 13 * kWave + 1;    // records of a group + 1
+constexpr int kRecScratchBytes = kRecIncMax * 2 + kRecLocMax * 4;
+template <int NT>
+struct RecordLists {
+    static constexpr int kI = (kRecIncMax + NT - 1) / NT, kL = (kRecLocMax + NT - 1) / NT;
+    uint16_t vi[kI];
+    int vl[kL];
+    int L0, n, m;
+
+    __device__ __forceinline__ void load(const FaceArgs &a, int slot)
+    {
+        L0 = a.gl_start[slot];
+        n = a.gl_start[slot + 1] - L0;
+        const int base = a.linc_start[L0];
+        m = a.linc_start[L0 + n] - base;
+#pragma unroll
+        for (int j = 0; j < kI; j++) {
+            const int i = (int)threadIdx.x + j * NT;
+            vi[j] = i < m ? a.linc[base + i] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < kL; j++) {
+            const int i = (int)threadIdx.x + j * NT;
+            vl[j] = i <= n ? a.linc_start[L0 + i] - base : 0;
+        }
+    }
+
+    template <typename R>
+    __device__ __forceinline__ void sum(const FaceArgs &a, const R *T, const double al[3], void *scratch)
+    {
+        uint16_t *sInc = (uint16_t *)scratch;
+        int *sStart = (int *)((char *)scratch + kRecIncMax * 2);
+#pragma unroll
+        for (int j = 0; j < kI; j++) {
+            const int i = (int)threadIdx.x + j * NT;
+            if (i < m) sInc[i] = vi[j];
+        }
+#pragma unroll
+        for (int j = 0; j < kL; j++) {
+            const int i = (int)threadIdx.x + j * NT;
+            if (i <= n) sStart[i] = vl[j];
+        }
+        __syncthreads();
+        for (int l = (int)threadIdx.x; l < n; l += NT) {
+            const int i0 = sStart[l], i1 = sStart[l + 1];
+            double s[3] = {0.0, 0.0, 0.0};
+            // four incidences in flight, added in list order
+            for (int i = i0; i < i1; i += 4) {
+                R t[4][3];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int e = sInc[min(i + q, i1 - 1)];
+                    const R *p = T + (e >> 6) * 3 * kWave + (e & (kWave - 1));
+#pragma unroll
+                    for (int k = 0; k < 3; k++) t[q][k] = p[k * kWave];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    if (i + q < i1)
+#pragma unroll
+                        for (int k = 0; k < 3; k++) s[k] += (double)t[q][k];
+            }
+            const size_t L = (size_t)(L0 + l);
+            if constexpr (sizeof(R) == 4) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) a.gbuf32[3 * L + k] = (float)(s[k] * al[k]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; k++) a.gbuf[3 * L + k] = s[k] * al[k];
+            }
+        }
+    }
+};
 
 // Reciprocal and square root: IEEE-exact in FP64; the hardware
 // single-instruction forms (about 1 ulp) in the FP32 path, whose error budget
@@ -407,8 +480,10 @@ __device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R ig0
 // Lane l of every wave owns face group*64 + l; wave w evaluates the Gauss
 // points p = w, w + WS, ... so the table row of each point is wave-uniform
 // (scalar loads feed the FMAs as SGPR operands).  The waves' partial
-// gradients are summed in LDS in a fixed order, then wave 0 writes each
-// (face, control point) gradient into that vertex's CSR slot.
+// gradients are summed in LDS in a fixed order, and each (face, control
+// point) gradient is written to its record: records are slot-major per class
+// (slot c of face f at c nf + f), so a wave's stores are contiguous; the
+// vertex gather finds a vertex's records through the CSR's record index.
 //
 // R = double: the FP64 path, on absolute coordinates.
 // R = float:  the mixed path.  Control points are formed relative to the
@@ -436,7 +511,7 @@ __device__ __forceinline__ void face_group(const FaceArgs &a, const double *__re
     const R al[3] = {(R)al64[0], (R)al64[1], (R)al64[2]};
     // the restraint multipliers first, while few registers are live
     double lam[2];   // dE/dA_tot, dE/dV
-    restraint_multipliers(a, slot, lam);
+    restraint_multipliers(a, lam);
     double base[3] = {0.0, 0.0, 0.0};
     R P[PLDS ? 1 : NCP][3];
     if (PLDS) {
@@ -597,34 +672,233 @@ __device__ __forceinline__ void face_group(const FaceArgs &a, const double *__re
         }
     }
 #undef PV
+    // Every wave puts its G in its own LDS buffer (the control points' buffer
+    // is dead once every wave has finished its points); with several waves,
+    // wave w sums the buffers of control points c = w, w + WS, ... in wave
+    // order (the order of a sequential sum over the waves) into buffer 0.
+    // Then the group's records: one per distinct vertex.
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NCP; c++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) lds[((w * NCP + c) * 3 + k) * kWave + lane] = G[c][k];
+    RecordLists<WS * kWave> rl;
+    rl.load(a, slot);
+    __syncthreads();
     if (WS > 1) {
-        // fixed-order cross-wave sum of G through LDS
-        if (PLDS) __syncthreads();
-        for (int src = 1; src < WS; src++) {
-            if (w == src)
+        for (int c = w; c < NCP; c += WS)
 #pragma unroll
-                for (int c = 0; c < NCP; c++)
+            for (int k = 0; k < 3; k++) {
+                R s = lds[(c * 3 + k) * kWave + lane];
 #pragma unroll
-                    for (int k = 0; k < 3; k++) lds[(c * 3 + k) * kWave + lane] = G[c][k];
-            __syncthreads();
-            if (w == 0)
+                for (int q = 1; q < WS; q++) s += lds[((q * NCP + c) * 3 + k) * kWave + lane];
+                lds[(c * 3 + k) * kWave + lane] = s;
+            }
+        __syncthreads();
+    }
+    // record scratch: past buffer 0 (the other waves' buffers are dead; with
+    // one wave the kernel's LDS array has room past it)
+    rl.template sum<R>(a, lds, al64, lds + 13 * 3 * kWave);
+}
+
+// Fused energy + gradient of 64 faces by WS waves that split the Gauss points
+// AND the control points (CPS form).  Lane l owns face group*64 + l in every
+// wave.  The control points are staged in LDS relative to control point 0 (in
+// FP64 for the FP64 path, rounded to FP32 for the mixed one: the derivative
+// rows of every table sum to zero, so the contraction is translation
+// invariant and point 0's row drops out).  Round j: wave w evaluates Gauss
+// point j WS + w and stages its reverse -- dE/dRu, dE/dRv, the normal and
+// (dL, dM, dN) -- in LDS; then every wave applies the round's points to its
+// own control points c = w, w + WS, ... (table rows wave-uniform: scalar
+// loads feed the FMAs):
+//   dE/dP_c += TB0 dE/dRu + TB1 dE/dRv + (TB2 dL + TB3 dM + TB4 dN) n.
+// A wave keeps the gradient of only its NCP/WS control points in registers,
+// so no cross-wave gradient sum is needed and the footprint allows three to
+// four waves per SIMD.  The volume seed (FP64, absolute positions) is added
+// to each wave's control points at write-out; dE/dalpha takes only the
+// Gauss-point terms (the volume does not depend on alpha).
+template <typename R, int NCP, int NPT, int KIND, int WS, int FEAT>
+__device__ __forceinline__ void face_group_cps(const FaceArgs &a, const double *__restrict__ r, int group, int slot,
+                                               R *sP, R *sD, double acc[4])
+{
+    constexpr int kOwn = (NCP + WS - 1) / WS;    // control points per wave
+    constexpr int kRounds = (NPT + WS - 1) / WS;
+    constexpr int kDRows = 12;                   // staged reverse of one point
+    const R *tab = kind_table<KIND, R>();
+    const R *wts = tab + NPT * 5 * NCP;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int nf = a.nf;
+    const int f = group * kWave + lane;
+    const bool live = f < nf;
+    const int fc = live ? f : nf - 1;
+    const double al64[3] = {r[3 * a.nv], r[3 * a.nv + 1], r[3 * a.nv + 2]};
+    const R al[3] = {(R)al64[0], (R)al64[1], (R)al64[2]};
+    double lam[2];   // dE/dA_tot, dE/dV
+    restraint_multipliers(a, lam);
+    // control point 0 (the origin) and this wave's own control points
+    double base[3];
+    {
+        double t[1][3];
+        gather_cps<1, 1>(a, r, fc, 0, t);
+        for (int k = 0; k < 3; k++) base[k] = t[0][k];
+    }
+    {
+        double t[kOwn][3];
+        gather_cps<NCP, WS>(a, r, fc, w, t);
 #pragma unroll
-                for (int c = 0; c < NCP; c++)
+        for (int i = 0; i < kOwn; i++) {
+            const int c = w + i * WS;
+            if (c >= 1 && c < NCP)
 #pragma unroll
-                    for (int k = 0; k < 3; k++) G[c][k] += lds[(c * 3 + k) * kWave + lane];
-            __syncthreads();
+                for (int k = 0; k < 3; k++) sP[((c - 1) * 3 + k) * kWave + lane] = (R)(t[i][k] - base[k]);
         }
     }
-    if (w == 0 && live) {
-        R *gb = (R *)(kRel ? (void *)a.gbuf32 : (void *)a.gbuf);
+    __syncthreads();
+#define PV(c, k) sP[(((c) - 1) * 3 + (k)) * kWave + lane]
+    const R Lo = (R)a.Lo[fc], Li = (R)a.Li[fc], c0f = (R)a.c0[fc];
+    const double lamA = lam[0];
+    R G[kOwn][3];
 #pragma unroll
-        for (int c = 0; c < NCP; c++) {
-            R *dst = gb + 3 * (size_t)a.pos[c * nf + f];
-            dst[0] = G[c][0] * al[0];
-            dst[1] = G[c][1] * al[1];
-            dst[2] = G[c][2] * al[2];
+    for (int i = 0; i < kOwn; i++) G[i][0] = G[i][1] = G[i][2] = R(0);
+    double e = 0.0;
+    for (int j = 0; j < kRounds; j++) {
+        const int p = j * WS + w;
+        if (p < NPT) {
+            const R *T = tab + p * 5 * NCP;
+            R X[5][3];
+#pragma unroll
+            for (int d = 0; d < 5; d++)
+#pragma unroll
+                for (int k = 0; k < 3; k++) X[d][k] = R(0);
+#pragma unroll
+            for (int c = 1; c < NCP; c++)
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const R x = PV(c, k);
+#pragma unroll
+                    for (int d = 0; d < 5; d++) X[d][k] += T[c * 5 + d] * x;
+                }
+#pragma unroll
+            for (int d = 0; d < 5; d++)
+#pragma unroll
+                for (int k = 0; k < 3; k++) X[d][k] *= al[k];
+            PointGrad<R> pg;
+            const R rr0 = FEAT == kFeatGeneral ? (R)a.rurv0[p * nf + fc] : R(0);
+            e += (double)point_energy_grad<R, FEAT, KIND == 0>(X, wts[p], (R)a.g0[p * nf + fc],
+                                                               (R)a.ig0[p * nf + fc], rr0, Lo, Li, c0f, (R)a.dudv,
+                                                               lamA, a.ph, pg);
+            R *d = sD + w * kDRows * kWave + lane;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                d[k * kWave] = pg.du[k];
+                d[(3 + k) * kWave] = pg.dv[k];
+                d[(6 + k) * kWave] = pg.n[k];
+            }
+            d[9 * kWave] = pg.dl;
+            d[10 * kWave] = pg.dm;
+            d[11 * kWave] = pg.dn;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < WS; q++) {
+            const int pq = j * WS + q;
+            if (pq < NPT) {
+                const R *TB = kind_table_b<KIND, R>() + pq * 5 * NCP;
+                const R *d = sD + q * kDRows * kWave + lane;
+                R du[3], dv[3], nn[3];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    du[k] = d[k * kWave];
+                    dv[k] = d[(3 + k) * kWave];
+                    nn[k] = d[(6 + k) * kWave];
+                }
+                const R dl = d[9 * kWave], dm = d[10 * kWave], dn = d[11 * kWave];
+#pragma unroll
+                for (int i = 0; i < kOwn; i++) {
+                    const int c = w + i * WS;
+                    if (c < NCP) {
+                        const R sn = TB[c * 5 + 2] * dl + TB[c * 5 + 3] * dm + TB[c * 5 + 4] * dn;
+#pragma unroll
+                        for (int k = 0; k < 3; k++)
+                            G[i][k] += TB[c * 5] * du[k] + TB[c * 5 + 1] * dv[k] + sn * nn[k];
+                    }
+                }
+            }
+        }
+        if (j + 1 < kRounds) __syncthreads();
+    }
+    // dE/dalpha_k = sum_c dE/dR_ck (r + shift)_ck over this wave's control
+    // points (FP64), the Gauss-point terms only
+    if (live) {
+        acc[0] += e;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            double sg = 0.0, sgp = 0.0;
+#pragma unroll
+            for (int i = 0; i < kOwn; i++) {
+                const int c = w + i * WS;
+                if (c < NCP) {
+                    sg += (double)G[i][k];
+                    if (c > 0) sgp += (double)G[i][k] * (double)PV(c, k);
+                }
+            }
+            acc[1 + k] += sgp + sg * base[k];
         }
     }
+    // Volume restraint: dE/dV dV/dP of this face's flat triangle of corner
+    // limit points L0, L1, L2 (V_face = L0 . (L1 x L2) / 6), on absolute
+    // positions in FP64, added to the wave's own control points; the
+    // write-out multiplies by alpha, so it enters divided by alpha
+    double seed[kOwn][3];
+#pragma unroll
+    for (int i = 0; i < kOwn; i++) seed[i][0] = seed[i][1] = seed[i][2] = 0.0;
+    if (a.rs && a.do_volume) {
+        const double *W = kind_corner<KIND>();
+        double L[3][3];
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) L[q][k] = W[q * NCP] * base[k];
+#pragma unroll
+        for (int c = 1; c < NCP; c++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double x = base[k] + (double)PV(c, k);
+#pragma unroll
+                for (int q = 0; q < 3; q++) L[q][k] += W[q * NCP + c] * x;
+            }
+        const double lamV = lam[1] * (1.0 / 6.0);
+        double D[3][3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const double *A = L[(q + 1) % 3], *B = L[(q + 2) % 3];
+            D[q][0] = lamV * (A[1] * B[2] - A[2] * B[1]);
+            D[q][1] = lamV * (A[2] * B[0] - A[0] * B[2]);
+            D[q][2] = lamV * (A[0] * B[1] - A[1] * B[0]);
+        }
+#pragma unroll
+        for (int i = 0; i < kOwn; i++) {
+            const int c = min(w + i * WS, NCP - 1);
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                seed[i][k] = (W[c] * D[0][k] + W[NCP + c] * D[1][k] + W[2 * NCP + c] * D[2][k]) / al64[k];
+        }
+    }
+#undef PV
+    // the group's per-(face, slot) gradients into LDS (over the staged control
+    // points, once every wave is done with them), then its records
+    RecordLists<WS * kWave> rl;
+    rl.load(a, slot);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kOwn; i++) {
+        const int c = w + i * WS;
+        if (c < NCP)
+#pragma unroll
+            for (int k = 0; k < 3; k++) sP[(c * 3 + k) * kWave + lane] = (R)((double)G[i][k] + seed[i][k]);
+    }
+    rl.template sum<R>(a, sP, al64, sP + 13 * 3 * kWave);
 }
 
 // Fused face kernel over a table of work items, one item per 64-face group of
@@ -635,20 +909,36 @@ __device__ __forceinline__ void face_group(const FaceArgs &a, const double *__re
 // workgroup per item; PERSIST = true: a smaller grid strides over the items.
 // B.r / B.step_ctr (non-null for a single surface's own launch) override the
 // table's positions and step counter.
-template <typename R, int WS, bool PLDS, int OCC, int FEAT, bool PERSIST>
+template <typename R, int WS, bool PLDS, int OCC, int FEAT, bool PERSIST, bool CPS = false>
 __global__ __launch_bounds__(kWave *WS, OCC) void faces_batch_kernel(BatchFaceArgs B)
 {
-    __shared__ R lds[13 * 3 * kWave];
-    __shared__ double s_corr[3][kWave];
+    // CPS: 12 relative control points, then one staged point reverse (12
+    // rows) per wave; else 13 control points, then one gradient buffer per
+    // wave for the cross-wave sum; both hold the group records' scratch past
+    // the first 13 control points' rows
+    constexpr int kRecR = (kRecScratchBytes + sizeof(R) - 1) / sizeof(R);
+    constexpr int kLdsA = CPS ? (12 * 3 + WS * 12) * kWave : (WS > 1 ? WS : 1) * 13 * 3 * kWave;
+    constexpr int kLds = kLdsA > 13 * 3 * kWave + kRecR ? kLdsA : 13 * 3 * kWave + kRecR;
+    __shared__ R lds[kLds];
     for (int it = xcd_logical_block(blockIdx.x, gridDim.x); it < B.nitems; it += gridDim.x) {
         const FaceItem item = B.items[it];
         const FaceArgs &a = B.fa[item.fa];
         const double *r = B.r ? B.r : a.r;
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        switch (item.kind) {
-        case 0: face_group<R, 12, 15, 0, WS, PLDS, FEAT>(a, r, item.group, item.slot, lds, s_corr, acc); break;
-        case 1: face_group<R, 11, 16, 1, WS, PLDS, FEAT>(a, r, item.group, item.slot, lds, s_corr, acc); break;
-        default: face_group<R, 13, 16, 2, WS, PLDS, FEAT>(a, r, item.group, item.slot, lds, s_corr, acc); break;
+        if constexpr (CPS) {
+            R *sD = lds + 12 * 3 * kWave;
+            switch (item.kind) {
+            case 0: face_group_cps<R, 12, 15, 0, WS, FEAT>(a, r, item.group, item.slot, lds, sD, acc); break;
+            case 1: face_group_cps<R, 11, 16, 1, WS, FEAT>(a, r, item.group, item.slot, lds, sD, acc); break;
+            default: face_group_cps<R, 13, 16, 2, WS, FEAT>(a, r, item.group, item.slot, lds, sD, acc); break;
+            }
+        } else {
+            __shared__ double s_corr[3][kWave];
+            switch (item.kind) {
+            case 0: face_group<R, 12, 15, 0, WS, PLDS, FEAT>(a, r, item.group, item.slot, lds, s_corr, acc); break;
+            case 1: face_group<R, 11, 16, 1, WS, PLDS, FEAT>(a, r, item.group, item.slot, lds, s_corr, acc); break;
+            default: face_group<R, 13, 16, 2, WS, PLDS, FEAT>(a, r, item.group, item.slot, lds, s_corr, acc); break;
+            }
         }
         block_sum4<kWave * WS>(acc);
         if (threadIdx.x == 0) {

@@ -557,13 +557,53 @@ int build_plan(const Topology &T, Plan &P, std::string &err)
     P.nfaces = nf;
     P.nreg = nreg;
     for (int f = 0; f < nf; f++) P.classes[faces[f].cls].face_id.push_back(gid[f]);
-    // SoA per class, faces in global-id order within the class
+    // SoA per class, faces in Morton order of their corner centroids (ties by
+    // global id): the 64 faces of a group then form a compact patch whose
+    // control points are few distinct vertices (about 95 for 768 (face, slot)
+    // incidences on an icosphere), so the face kernel's per-group vertex sums
+    // write few gradient records
     std::vector<std::vector<int>> members(P.classes.size());
     for (int f = 0; f < nf; f++) members[faces[f].cls].push_back(f);
+    std::vector<uint64_t> key(nf);
+    {
+        std::vector<double> cen(3 * (size_t)nf);
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int f = 0; f < nf; f++)
+            for (int k = 0; k < 3; k++) {
+                double s = 0.0;
+                for (int q = 0; q < 3; q++) {
+                    const int code = faces[f].img[q];
+                    const int ix = code / 9 - 1, iy = (code / 3) % 3 - 1, iz = code % 3 - 1;
+                    s += T.r[3 * (size_t)faces[f].cp[q] + k] + ix * T.pbc[0][k] + iy * T.pbc[1][k] + iz * T.pbc[2][k];
+                }
+                cen[3 * (size_t)f + k] = s / 3.0;
+                lo[k] = std::min(lo[k], s / 3.0);
+                hi[k] = std::max(hi[k], s / 3.0);
+            }
+        auto spread = [](uint64_t x) {   // 21 bits -> every third bit
+            x &= 0x1fffff;
+            x = (x | x << 32) & 0x1f00000000ffffULL;
+            x = (x | x << 16) & 0x1f0000ff0000ffULL;
+            x = (x | x << 8) & 0x100f00f00f00f00fULL;
+            x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
+            x = (x | x << 2) & 0x1249249249249249ULL;
+            return x;
+        };
+        for (int f = 0; f < nf; f++) {
+            uint64_t m = 0;
+            for (int k = 0; k < 3; k++) {
+                const double span = hi[k] - lo[k];
+                const double t = span > 0 ? (cen[3 * (size_t)f + k] - lo[k]) / span : 0.0;
+                m |= spread((uint64_t)(std::min(std::max(t, 0.0), 1.0) * 2097151.0)) << k;
+            }
+            key[f] = m;
+        }
+    }
     for (size_t c = 0; c < P.classes.size(); c++) {
         FaceClass &C = P.classes[c];
         auto &mem = members[c];
-        std::sort(mem.begin(), mem.end(), [&](int a, int b) { return gid[a] < gid[b]; });
+        std::sort(mem.begin(), mem.end(),
+                  [&](int a, int b) { return key[a] != key[b] ? key[a] < key[b] : gid[a] < gid[b]; });
         int n = (int)mem.size();
         C.face_id.resize(n);
         C.cp.resize((size_t)C.ncp * n);
@@ -594,6 +634,40 @@ int build_plan(const Topology &T, Plan &P, std::string &err)
         FaceClass &C = P.classes[c];
         C.pos.resize(C.cp.size());
         for (size_t s = 0; s < C.cp.size(); s++) C.pos[s] = next[C.cp[s]]++;
+    }
+    // per 64-face group: its distinct control-point vertices (in order of
+    // first appearance, slot-major) and each one's incidences slot * 64 + lane
+    // in ascending order
+    for (size_t c = 0; c < P.classes.size(); c++) {
+        FaceClass &C = P.classes[c];
+        const int n = C.nf();
+        C.gl_off.assign(1, 0);
+        C.gl_vert.clear();
+        C.gl_inc_off.assign(1, 0);
+        C.gl_inc.clear();
+        std::vector<int> local(T.nv, -1);
+        for (int g0 = 0; g0 < n; g0 += 64) {
+            const int g1 = std::min(n, g0 + 64);
+            std::vector<int> verts;
+            std::vector<std::vector<uint16_t>> inc;
+            for (int q = 0; q < C.ncp; q++)
+                for (int x = g0; x < g1; x++) {
+                    const int v = C.cp[(size_t)q * n + x];
+                    if (local[v] < 0) {
+                        local[v] = (int)verts.size();
+                        verts.push_back(v);
+                        inc.emplace_back();
+                    }
+                    inc[local[v]].push_back((uint16_t)(q * 64 + (x - g0)));
+                }
+            for (size_t l = 0; l < verts.size(); l++) {
+                C.gl_vert.push_back(verts[l]);
+                C.gl_inc.insert(C.gl_inc.end(), inc[l].begin(), inc[l].end());
+                C.gl_inc_off.push_back((int)C.gl_inc.size());
+                local[verts[l]] = -1;
+            }
+            C.gl_off.push_back((int)C.gl_vert.size());
+        }
     }
     return 0;
 }

@@ -45,6 +45,16 @@ struct FaceClass {
     // limit points of the owner i, of ring[e] and of ring[e+1]  [3][ncp]
     std::vector<double> corner;
     std::vector<int> pos;              // [ncp][nf]   CSR position of (face, slot)
+    // Per group of 64 faces (faces 64 g .. 64 g + 63): the distinct vertices
+    // among its control points, gl_vert[gl_off[g] .. gl_off[g + 1]), and for
+    // the class-wide local vertex l its incidences gl_inc[gl_inc_off[l] ..
+    // gl_inc_off[l + 1]) as slot * 64 + (face - 64 g), ascending.  The face
+    // kernel writes one gradient record per local vertex (the sum over its
+    // incidences in that order).
+    std::vector<int> gl_off;           // ngroups + 1
+    std::vector<int> gl_vert;          // local vertices
+    std::vector<int> gl_inc_off;       // local vertices + 1
+    std::vector<uint16_t> gl_inc;      // ncp * nf
     int nf() const { return (int)face_id.size(); }
 };
 
@@ -52,8 +62,8 @@ struct Plan {
     int nfaces = 0;
     int nreg = 0;
     std::vector<FaceClass> classes;    // classes[0] is the regular class
-    // vertex -> (face, slot) incidence as CSR: the gradient contribution of
-    // slot c of face f lands at position pos[c][f] inside its vertex's range
+    // vertex -> (face, slot) incidence as CSR, entries of a vertex ordered by
+    // class, slot, face: entry pos[c][f] is slot c of face f
     std::vector<int> vstart;           // nv + 1
     int nnz = 0;
 };

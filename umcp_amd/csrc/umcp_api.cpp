@@ -137,18 +137,19 @@ FaceArgs face_args(umcp_surface *s, size_t c, const double *d_r)
     a.Lo = D.Lo;
     a.Li = D.Li;
     a.c0 = D.c0;
-    a.pos = D.pos;
     a.dudv = C.dudv;
     std::memcpy(a.pbc, s->T.pbc, sizeof a.pbc);
     a.gbuf = s->d_gbuf;
     a.gbuf32 = s->d_gbuf32;
+    a.gl_start = s->d_gl_start;
+    a.linc_start = s->d_linc_start;
+    a.linc = s->d_linc;
     a.ph = phys(s);
     a.step_ctr = nullptr;
     a.partial = s->d_partial;
     a.rs = restraints_on(s) ? s->d_rs : nullptr;
     a.rpart = s->d_rpart;
     a.ngroups = s->npartial;
-    a.rs_fold = a.rs && a.ngroups <= kFoldMaxGroups ? 1 : 0;
     a.KA_g = s->rsp.KA_global;
     a.KV = s->rsp.KV;
     a.A0 = s->rsp.A0 > 0.0 ? s->rsp.A0 : s->A0_setg0;
@@ -173,8 +174,12 @@ size_t plan_bytes(const umcp_surface *s)
     }
     const size_t nv = s->T.nv;
     b += 2 * sz(3 * nv + 3, 8) + sz(3 * (size_t)s->P.nnz, 8) + sz(3 * (size_t)s->P.nnz, 4);
+    size_t nloc = 0;
+    for (const FaceClass &C : s->P.classes) nloc += C.gl_vert.size();
     b += sz(4 * (size_t)std::max(ngroups, metric_total), 8) + sz(8, 8) + sz(s->P.vstart.size(), 4);
-    b += sz(8, 8) + sz(2 * (size_t)ngroups, 8) + sz(1, 4) + sz(1, 4);   // rs, rpart, zero, step
+    // group records: per-group bases, incidence ranges and lists, vertex CSR
+    b += sz(ngroups + 1, 4) + sz(nloc + 1, 4) + sz(s->P.nnz, 2) + sz(nv + 1, 4) + sz(nloc, 4);
+    b += sz(8, 8) + sz(2 * (size_t)ngroups, 8) + sz(1, 4);   // rs, rpart, step
     b += 2 * sz(s->P.classes.size(), sizeof(FaceArgs)) + sz(ngroups, sizeof(FaceItem));
     return b;
 }
@@ -191,7 +196,6 @@ int enqueue_volume(umcp_surface *s, const double *d_r, hipStream_t st)
     for (size_t c = 0; c < fa.size(); c++) {
         fa[c] = face_args(s, c, d_r);
         fa[c].rs = s->d_rs;
-        fa[c].rs_fold = 0;   // no face kernel follows: the finalize writes d_rs
         fa[c].do_area = 0;
         fa[c].do_volume = 1;
         fa[c].KA_g = 0.0;
@@ -200,7 +204,7 @@ int enqueue_volume(umcp_surface *s, const double *d_r, hipStream_t st)
     hipError_t e = hipMemcpyAsync(s->d_fa_aux, fa.data(), fa.size() * sizeof(FaceArgs), hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(s, e, "upload face table");
     BatchFaceArgs fb{s->d_fa_aux, s->d_items, s->npartial, d_r, nullptr};
-    e = launch_restraint_prepass(fb, s->d_zero, 1, st);
+    e = launch_restraint_prepass(fb, st);
     if (e != hipSuccess) return hip_fail(s, e, "restraint kernel launch");
     return UMCP_OK;
 }
@@ -269,7 +273,7 @@ int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs 
             if (int rc = prof_pair(s, 2, evr)) return rc;
             (void)hipEventRecord(evr[0], st);
         }
-        hipError_t e = launch_restraint_prepass(fb, s->d_zero, s->h_fa[0].rs_fold ? 0 : 1, st);
+        hipError_t e = launch_restraint_prepass(fb, st);
         if (e != hipSuccess) return hip_fail(s, e, "restraint kernel launch");
         if (timed) (void)hipEventRecord(evr[1], st);
     }
@@ -283,7 +287,8 @@ int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs 
     if (e != hipSuccess) return hip_fail(s, e, "face kernel launch");
     if (timed) (void)hipEventRecord(ev[1], st);
     ga.nv = s->T.nv;
-    ga.vstart = s->d_vstart;
+    ga.vstart = s->d_lvstart;
+    ga.vrec = s->d_lvrec;
     ga.gbuf = s->d_gbuf;
     ga.gbuf32 = mixed ? s->d_gbuf32 : nullptr;
     ga.partial = s->d_partial;
@@ -505,12 +510,44 @@ int umcp_plan(umcp_surface *s)
         (rc = dev_alloc(s, &s->d_vstart, s->P.vstart.size())))
         return rc;
     if ((rc = dev_upload(s, s->d_vstart, s->P.vstart.data(), s->P.vstart.size()))) return rc;
-    // restraint work: per-group partials, completion count, scalars
+    // group records in launch-slot order: slot -> first record, record ->
+    // incidence range, and the vertex -> records CSR of the gather
+    {
+        std::vector<int> gl_start(1, 0), linc_start(1, 0), lvert;
+        std::vector<uint16_t> linc;
+        for (size_t i = 0; i < s->launch_order.size(); i++) {
+            const FaceClass &C = s->P.classes[s->launch_order[i]];
+            for (int g = 0; g + 1 < (int)C.gl_off.size(); g++) {
+                for (int l = C.gl_off[g]; l < C.gl_off[g + 1]; l++) {
+                    lvert.push_back(C.gl_vert[l]);
+                    linc.insert(linc.end(), C.gl_inc.begin() + C.gl_inc_off[l], C.gl_inc.begin() + C.gl_inc_off[l + 1]);
+                    linc_start.push_back((int)linc.size());
+                }
+                gl_start.push_back((int)lvert.size());
+            }
+        }
+        std::vector<int> lvstart(nv + 1, 0), lvrec(lvert.size());
+        for (int v : lvert) lvstart[v + 1]++;
+        for (int v = 0; v < nv; v++) lvstart[v + 1] += lvstart[v];
+        std::vector<int> fill(lvstart.begin(), lvstart.end() - 1);
+        for (size_t L = 0; L < lvert.size(); L++) lvrec[fill[lvert[L]]++] = (int)L;
+        s->nrecords = (int)lvert.size();
+        if ((rc = dev_alloc(s, &s->d_gl_start, gl_start.size())) || (rc = dev_alloc(s, &s->d_linc_start, linc_start.size())) ||
+            (rc = dev_alloc(s, &s->d_linc, linc.size())) || (rc = dev_alloc(s, &s->d_lvstart, lvstart.size())) ||
+            (rc = dev_alloc(s, &s->d_lvrec, lvrec.size())) ||
+            (rc = dev_upload(s, s->d_gl_start, gl_start.data(), gl_start.size())) ||
+            (rc = dev_upload(s, s->d_linc_start, linc_start.data(), linc_start.size())) ||
+            (rc = dev_upload(s, s->d_linc, linc.data(), linc.size())) ||
+            (rc = dev_upload(s, s->d_lvstart, lvstart.data(), lvstart.size())) ||
+            (rc = dev_upload(s, s->d_lvrec, lvrec.data(), lvrec.size())))
+            return rc;
+    }
+    // restraint work: per-group partials, scalars
     s->has_images = false;
     for (const FaceClass &C : s->P.classes)
         for (uint8_t h : C.has_img) s->has_images = s->has_images || h;
     if ((rc = dev_alloc(s, &s->d_rs, 8)) || (rc = dev_alloc(s, &s->d_rpart, 2 * (size_t)s->npartial)) ||
-        (rc = dev_alloc(s, &s->d_zero, 1)) || (rc = dev_alloc(s, &s->d_step, 1)))
+        (rc = dev_alloc(s, &s->d_step, 1)))
         return rc;
     // face work tables
     std::vector<FaceItem> items;
@@ -527,7 +564,7 @@ int umcp_plan(umcp_surface *s)
     s->fa_dirty = true;
     const int zero_step = 0;
     const double zeros8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if ((rc = dev_upload(s, s->d_step, &zero_step, 1)) || (rc = dev_upload(s, s->d_zero, &zero_step, 1)) ||
+    if ((rc = dev_upload(s, s->d_step, &zero_step, 1)) ||
         (rc = dev_upload(s, s->d_rs, zeros8, 8)))
         return rc;
     if ((rc = sync(s))) return rc;
@@ -1243,11 +1280,7 @@ int umcp_bd_steps(umcp_surface *s, double *r, const double *xi, int nsteps, cons
 int umcp_set_launch(umcp_surface *s, int point_split, int flags)
 {
     if (!s || !s->planned) return UMCP_ERR_STATE;
-    if (point_split < 1 || point_split > kMaxSplit)
-        return fail(s, UMCP_ERR_ARG, "point split must be between 1 and 4 waves");
-    if (flags & ~kLaunchFlags) return fail(s, UMCP_ERR_ARG, "unknown launch flags");
-    if ((flags & kLaunchMfma) && point_split == 3)
-        return fail(s, UMCP_ERR_ARG, "the matrix-core path takes point split 1, 2 or 4");
+    if (const char *why = launch_shape_error(point_split, flags)) return fail(s, UMCP_ERR_ARG, why);
     s->point_split = point_split;
     s->launch_flags = flags;
     return UMCP_OK;

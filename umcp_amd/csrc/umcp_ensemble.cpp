@@ -51,8 +51,7 @@ struct umcp_ensemble {
     FaceItem *d_items = nullptr;
     GatherArgs *d_ga = nullptr;
     GatherItem *d_gitems = nullptr;
-    int *d_first = nullptr;       // FaceArgs entry (class 0) of each member whose restraint finalize is a launch
-    int nitems = 0, ngitems = 0, nrestrained = 0, nfinalize = 0;
+    int nitems = 0, ngitems = 0, nrestrained = 0;
     std::vector<void *> tables;   // allocations of the current build
     // captured step sequence
     hipGraph_t graph = nullptr;
@@ -102,8 +101,7 @@ void drop_tables(umcp_ensemble *e)
     e->d_items = nullptr;
     e->d_ga = nullptr;
     e->d_gitems = nullptr;
-    e->d_first = nullptr;
-    e->nitems = e->ngitems = e->nrestrained = e->nfinalize = 0;
+    e->nitems = e->ngitems = e->nrestrained = 0;
 }
 
 template <typename T>
@@ -144,14 +142,12 @@ int build(umcp_ensemble *e)
     std::vector<std::vector<FaceItem>> per_member(e->members.size());
     std::vector<GatherArgs> ga;
     std::vector<GatherItem> gitems;
-    std::vector<int> first;
     int nrestrained = 0;
     for (size_t mi = 0; mi < e->members.size(); mi++) {
         const auto &m = e->members[mi];
         umcp_surface *s = m.s;
         const bool rson = restraints_on(s);
         if (rson) nrestrained++;
-        if (rson && face_args(s, 0, m.d_r).rs_fold == 0) first.push_back((int)fa.size());
         std::vector<int> fa_of_class(s->P.classes.size());
         for (size_t c = 0; c < s->P.classes.size(); c++) {
             FaceArgs a = face_args(s, c, m.d_r);
@@ -169,7 +165,8 @@ int build(umcp_ensemble *e)
         }
         GatherArgs g{};
         g.nv = s->T.nv;
-        g.vstart = s->d_vstart;
+        g.vstart = s->d_lvstart;
+        g.vrec = s->d_lvrec;
         g.gbuf = s->d_gbuf;
         g.gbuf32 = e->mixed ? s->d_gbuf32 : nullptr;
         g.partial = s->d_partial;
@@ -209,13 +206,11 @@ int build(umcp_ensemble *e)
     e->feat_start[3] = (int)items.size();
     int rc;
     if ((rc = upload_table(e, &e->d_fa, fa)) || (rc = upload_table(e, &e->d_items, items)) ||
-        (rc = upload_table(e, &e->d_ga, ga)) || (rc = upload_table(e, &e->d_gitems, gitems)) ||
-        (rc = upload_table(e, &e->d_first, first)))
+        (rc = upload_table(e, &e->d_ga, ga)) || (rc = upload_table(e, &e->d_gitems, gitems)))
         return rc;
     e->nitems = (int)items.size();
     e->ngitems = (int)gitems.size();
     e->nrestrained = nrestrained;
-    e->nfinalize = (int)first.size();
     err = hipStreamSynchronize(e->stream);
     if (err != hipSuccess) return ehip(e, err, "hipStreamSynchronize");
     return UMCP_OK;
@@ -227,7 +222,7 @@ hipError_t enqueue_step(umcp_ensemble *e, hipStream_t st)
     if (e->nrestrained > 0) {
         // over every item: groups of members without restraints return at once
         BatchFaceArgs rb{e->d_fa, e->d_items, e->nitems, nullptr, nullptr};
-        err = launch_restraint_prepass(rb, e->d_first, e->nfinalize, st);
+        err = launch_restraint_prepass(rb, st);
         if (err != hipSuccess) return err;
     }
     for (int f = 0; f < 3; f++) {
@@ -318,10 +313,7 @@ int umcp_ensemble_refresh(umcp_ensemble *e)
 int umcp_ensemble_set_launch(umcp_ensemble *e, int point_split, int flags, int face_grid, int order)
 {
     if (!e) return UMCP_ERR_ARG;
-    if (point_split < 1 || point_split > kMaxSplit) return efail(e, UMCP_ERR_ARG, "point split must be 1..4");
-    if (flags & ~kLaunchFlags) return efail(e, UMCP_ERR_ARG, "unknown launch flags");
-    if ((flags & kLaunchMfma) && point_split == 3)
-        return efail(e, UMCP_ERR_ARG, "the matrix-core path takes point split 1, 2 or 4");
+    if (const char *why = launch_shape_error(point_split, flags)) return efail(e, UMCP_ERR_ARG, why);
     if (face_grid < 0) return efail(e, UMCP_ERR_ARG, "negative grid");
     if (order < 0 || order > 1) return efail(e, UMCP_ERR_ARG, "order is 0 (member-major) or 1 (round-robin)");
     if (order != e->order) e->dirty = true;

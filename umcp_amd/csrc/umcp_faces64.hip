@@ -16,11 +16,34 @@ static hipError_t launch_ws64(const BatchFaceArgs &a, int nb, bool persist, hipS
     return hipGetLastError();
 }
 
-// the register-bound flag asks for 2 waves per SIMD
-template <int FEAT>
-static hipError_t launch_shape64(const BatchFaceArgs &a, int nb, bool persist, int ws, int occ2, hipStream_t st)
+// the control-point split (CPS form) of the kernel
+template <int WS, int FEAT>
+static hipError_t launch_cps64(const BatchFaceArgs &a, int nb, bool persist, hipStream_t st)
 {
-    switch (ws * 2 + (occ2 ? 1 : 0)) {
+    if (persist)
+        hipLaunchKernelGGL((faces_batch_kernel<double, WS, true, 1, FEAT, true, true>), dim3(nb), dim3(kWave * WS), 0,
+                           st, a);
+    else
+        hipLaunchKernelGGL((faces_batch_kernel<double, WS, true, 1, FEAT, false, true>), dim3(nb), dim3(kWave * WS), 0,
+                           st, a);
+    return hipGetLastError();
+}
+
+// the register-bound flag asks for 2 waves per SIMD (not in the CPS form)
+template <int FEAT>
+static hipError_t launch_shape64(const BatchFaceArgs &a, int nb, bool persist, int ws, int flags, hipStream_t st)
+{
+    if (flags & kLaunchCps) {
+        // the control-point split needs its registers only (no bound)
+        switch (ws) {
+        case 2: return launch_cps64<2, FEAT>(a, nb, persist, st);
+        case 3: return launch_cps64<3, FEAT>(a, nb, persist, st);
+        case 4: return launch_cps64<4, FEAT>(a, nb, persist, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
+    const int occ2 = (flags & kLaunchOcc2) ? 1 : 0;
+    switch (ws * 2 + occ2) {
     case 2: return launch_ws64<1, 1, FEAT>(a, nb, persist, st);
     case 3: return launch_ws64<1, 2, FEAT>(a, nb, persist, st);
     case 4: return launch_ws64<2, 1, FEAT>(a, nb, persist, st);
@@ -33,12 +56,12 @@ static hipError_t launch_shape64(const BatchFaceArgs &a, int nb, bool persist, i
     }
 }
 
-hipError_t launch_faces64(const BatchFaceArgs &a, int nb, bool persist, int ws, int occ2, int feat, hipStream_t st)
+hipError_t launch_faces64(const BatchFaceArgs &a, int nb, bool persist, int ws, int flags, int feat, hipStream_t st)
 {
     switch (feat) {
-    case kFeatLipid: return launch_shape64<kFeatLipid>(a, nb, persist, ws, occ2, st);
-    case kFeatHelfrich: return launch_shape64<kFeatHelfrich>(a, nb, persist, ws, occ2, st);
-    default: return launch_shape64<kFeatGeneral>(a, nb, persist, ws, occ2, st);
+    case kFeatLipid: return launch_shape64<kFeatLipid>(a, nb, persist, ws, flags, st);
+    case kFeatHelfrich: return launch_shape64<kFeatHelfrich>(a, nb, persist, ws, flags, st);
+    default: return launch_shape64<kFeatGeneral>(a, nb, persist, ws, flags, st);
     }
 }
 

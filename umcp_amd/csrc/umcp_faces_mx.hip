@@ -37,6 +37,7 @@ constexpr int kMxBase = 3 * kWave * 2;      // [3][64] doubles
 constexpr int kMxGv = 13 * 3 * kWave;       // [ncp][3][64] volume seed
 constexpr int kMxLds = kMxP + kMxBase + kMxGv;
 static_assert(48 * kWave <= kMxP, "cross-wave sums reuse the control-point rows");
+static_assert(13 * 3 * kWave * 4 + kRecScratchBytes <= kMxP * 4, "group records reuse the control-point rows");
 
 __device__ __forceinline__ mx4 mfma4(float a, float b, mx4 c)
 {
@@ -67,7 +68,7 @@ __device__ __forceinline__ void face_group_mx(const FaceArgs &a, const double *_
     // 1. the restraint multipliers, then staging: lane = face, the waves
     // split the control points
     double lam[2];   // dE/dA_tot, dE/dV
-    restraint_multipliers(a, slot, lam);
+    restraint_multipliers(a, lam);
     {
         const int f = group * kWave + lane;
         const int fc = f < nf ? f : nf - 1;
@@ -267,29 +268,29 @@ __device__ __forceinline__ void face_group_mx(const FaceArgs &a, const double *_
             __syncthreads();
         }
     }
+    // the group's per-(face, slot) gradients into LDS over the control-point
+    // rows, then its records (one per distinct vertex)
+    float *T = sP;
+    RecordLists<WS * kWave> rl;
+    rl.load(a, slot);
     if (w == 0) {
 #pragma unroll
         for (int t = 0; t < 4; t++) {
             const int fl = 16 * t + j;
-            const int f = group * kWave + fl;
-            if (f >= nf) continue;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const int c = 4 * q + i;
                 if (c >= NCP) continue;
-                float v[3];
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
-                    v[k] = G[t][k][i];
-                    if (vol) v[k] += sGv[(c * 3 + k) * kWave + fl];
+                    float v = G[t][k][i];
+                    if (vol) v += sGv[(c * 3 + k) * kWave + fl];
+                    T[(c * 3 + k) * kWave + fl] = v;
                 }
-                float *dst = a.gbuf32 + 3 * (size_t)a.pos[c * nf + f];
-                dst[0] = v[0] * al[0];
-                dst[1] = v[1] * al[1];
-                dst[2] = v[2] * al[2];
             }
         }
     }
+    rl.template sum<float>(a, T, al64, T + 13 * 3 * kWave);
 }
 
 // The batched face kernel of the matrix-core mixed path: the same work items,

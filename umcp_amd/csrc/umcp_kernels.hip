@@ -16,8 +16,8 @@ namespace umcp {
 
 // the face kernels' translation units (umcp_faces64.hip, umcp_faces32.hip,
 // umcp_faces_mx.hip)
-hipError_t launch_faces64(const BatchFaceArgs &a, int nb, bool persist, int ws, int occ2, int feat, hipStream_t st);
-hipError_t launch_faces32(const BatchFaceArgs &a, int nb, bool persist, int ws, int occ2, int feat, hipStream_t st);
+hipError_t launch_faces64(const BatchFaceArgs &a, int nb, bool persist, int ws, int flags, int feat, hipStream_t st);
+hipError_t launch_faces32(const BatchFaceArgs &a, int nb, bool persist, int ws, int flags, int feat, hipStream_t st);
 hipError_t launch_faces_mx(const BatchFaceArgs &a, int nb, bool persist, int ws, int occ2, int feat, hipStream_t st);
 hipError_t copy_tables_faces64(int kind, const double *buf, const float *b32, const double *corner, size_t n, int npt,
                                int ncp);
@@ -190,12 +190,14 @@ __device__ __forceinline__ void gather_reduce(const GatherArgs &a)
     }
 }
 
-// kGatherLanes lanes per vertex: lane q sums entries s0 + q, s0 + q + kGatherLanes, ...
-// of the vertex's CSR range (neighbouring lanes read neighbouring entries),
-// then a fixed xor butterfly combines the lanes (IEEE addition commutes, so
-// every lane of the group ends with the same sum).  The entries are read
-// kGatherUnroll at a time, every load issued before the first add (clamped
-// to the range; the adds skip what lies beyond it), in the same order.
+// kGatherLanes lanes per vertex: lane q sums the records of entries s0 + q,
+// s0 + q + kGatherLanes, ... of the vertex's CSR range (neighbouring lanes read
+// neighbouring entries of vrec; the records themselves, written slot-major by
+// the face kernel, are scattered), then a fixed xor butterfly combines the
+// lanes (IEEE addition commutes, so every lane of the group ends with the same
+// sum).  The entries are read kGatherUnroll at a time, every load issued
+// before the first add (clamped to the range; the adds skip what lies beyond
+// it), in the same order.
 template <typename GB>
 __device__ __forceinline__ void gather_vertices(const GatherArgs &a, int vblock)
 {
@@ -212,10 +214,13 @@ __device__ __forceinline__ void gather_vertices(const GatherArgs &a, int vblock)
         const GB *src = (const GB *)(sizeof(GB) == 4 ? (const void *)a.gbuf32 : (const void *)a.gbuf);
         constexpr int kU = kGatherUnroll;
         for (int e0 = s0 + q; e0 < s1; e0 += kU * kGatherLanes) {
+            int rec[kU];
+#pragma unroll
+            for (int j = 0; j < kU; j++) rec[j] = a.vrec[min(e0 + j * kGatherLanes, s1 - 1)];
             GB x[kU][3];
 #pragma unroll
             for (int j = 0; j < kU; j++) {
-                const size_t e = (size_t)min(e0 + j * kGatherLanes, s1 - 1);
+                const size_t e = (size_t)rec[j];
                 x[j][0] = src[3 * e];
                 x[j][1] = src[3 * e + 1];
                 x[j][2] = src[3 * e + 2];
@@ -533,9 +538,9 @@ __global__ __launch_bounds__(kGatherBlock) void vertex_gather_batch(BatchGatherA
 // src/interpLib.C:997-1098 / src/volume.C:350-430), as a pre-pass over the
 // face kernel's items: A_tot = sum dudv g w over every Gauss point (alpha-
 // scaled, as the energy's g) and V = sum over faces of L0 . (L1 x L2) / 6 of
-// the face's corner limit points, per 64-face group; a one-block-per-surface
-// finalize sums the groups' partials in fixed order and writes the
-// multipliers the face kernel reads.
+// the face's corner limit points, per 64-face group; a finalize launch sums
+// each surface's partials in fixed order and writes the multipliers the face
+// kernel reads.
 // ---------------------------------------------------------------------------
 // One 64-face group, lane = face: the waves gather the group's control
 // points once (wave w takes c = w, w + WS, ...) into LDS, then each wave
@@ -641,12 +646,17 @@ __global__ __launch_bounds__(kWave *WS) void restraint_prepass_kernel(BatchFaceA
     }
 }
 
-// one wave per surface that does not fold its finalize into the face kernel
-// (the surface's FaceArgs entry of class 0): the scalars the face kernel
-// would form, written to rs
-__global__ __launch_bounds__(kWave) void restraint_finalize_kernel(const FaceArgs *fa, const int *first)
+// The restraint scalars of every surface with restraints: the block of the
+// surface's slot-0 item forms them from the pre-pass partials; every other
+// block returns at once.  (A separate launch: a pre-pass block cannot see the
+// other blocks' partials without a device-wide cache write-back, which cost
+// far more than this launch.)
+__global__ __launch_bounds__(kWave) void restraint_finalize_kernel(BatchFaceArgs B)
 {
-    const FaceArgs &a = fa[first[blockIdx.x]];
+    const FaceItem item = B.items[blockIdx.x];
+    if (item.slot != 0) return;
+    const FaceArgs &a = B.fa[item.fa];
+    if (!a.rs) return;
     double o[5];
     restraint_scalars(a, o);
     if (threadIdx.x == 0)
@@ -840,8 +850,8 @@ hipError_t launch_faces_batch(const BatchFaceArgs &a, int grid, int ws, int flag
     const int nb = persist ? grid : a.nitems;
     const int occ2 = (flags & kLaunchOcc2) ? 1 : 0;
     if (mixed && (flags & kLaunchMfma)) return launch_faces_mx(a, nb, persist, ws, occ2, feat, st);
-    return mixed ? launch_faces32(a, nb, persist, ws, occ2, feat, st)
-                 : launch_faces64(a, nb, persist, ws, occ2, feat, st);
+    return mixed ? launch_faces32(a, nb, persist, ws, flags, feat, st)
+                 : launch_faces64(a, nb, persist, ws, flags, feat, st);
 }
 
 hipError_t launch_gather_batch(const BatchGatherArgs &a, int grid, bool mixed, hipStream_t st)
@@ -855,13 +865,13 @@ hipError_t launch_gather_batch(const BatchGatherArgs &a, int grid, bool mixed, h
     return hipGetLastError();
 }
 
-hipError_t launch_restraint_prepass(const BatchFaceArgs &a, const int *first, int nsurf, hipStream_t st)
+hipError_t launch_restraint_prepass(const BatchFaceArgs &a, hipStream_t st)
 {
     if (a.nitems <= 0) return hipSuccess;
     hipLaunchKernelGGL((restraint_prepass_kernel<kPrepassWaves>), dim3(a.nitems), dim3(kWave * kPrepassWaves), 0, st, a);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess || nsurf <= 0) return e;
-    hipLaunchKernelGGL(restraint_finalize_kernel, dim3(nsurf), dim3(kWave), 0, st, a.fa, first);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(restraint_finalize_kernel, dim3(a.nitems), dim3(kWave), 0, st, a);
     return hipGetLastError();
 }
 

@@ -29,7 +29,6 @@ struct FaceArgs {
     int nv;
     int nf;
     const int *cp;          // [ncp][nf]   control-point vertex ids
-    const int *pos;         // [ncp][nf]   position of (face, slot) in the vertex CSR
     const uint8_t *img;     // [ncp][nf]   periodic image codes
     const uint8_t *has_img; // [nf]
     const double *g0;       // [npt][nf]
@@ -39,8 +38,15 @@ struct FaceArgs {
     const double *c0;       // [nf]
     double dudv;
     double pbc[3][3];
-    double *gbuf;           // [nnz][3] per-(face, slot) gradient, CSR order (FP64 path)
-    float *gbuf32;          // [nnz][3] the same for the mixed path
+    // gradient records of the surface, one per distinct vertex of a 64-face
+    // group: group slot s owns records gl_start[s] .. gl_start[s + 1]; record
+    // L sums the (face, slot) gradients linc[linc_start[L] .. linc_start[L +
+    // 1]) (each slot * 64 + face within the group)
+    double *gbuf;           // [records][3] FP64 path
+    float *gbuf32;          // [records][3] mixed path
+    const int *gl_start;
+    const int *linc_start;
+    const uint16_t *linc;
     PhysParams ph;
     int *step_ctr;          // Brownian step counter, advanced once per evaluation, or null
     double *partial;        // the surface's block partials [ngroups][4]
@@ -50,9 +56,6 @@ struct FaceArgs {
     double *rs;
     double *rpart;
     int ngroups;
-    // 1: the face kernel forms the restraint scalars from rpart itself (and
-    // its slot-0 group writes rs); 0: a finalize launch writes rs first
-    int rs_fold;
     double KA_g, KV, A0, V0;
     int do_area, do_volume;
 };
@@ -80,7 +83,8 @@ constexpr int kEffmBrownian = 0, kEffmLangevin = 1;
 
 struct GatherArgs {
     int nv;
-    const int *vstart;      // nv + 1
+    const int *vstart;      // nv + 1: each vertex's range of vrec
+    const int *vrec;        // the records of each vertex (the face kernel's group records)
     const double *gbuf;     // FP64 contributions, or
     const float *gbuf32;    // FP32 contributions (mixed path) when non-null
     const double *partial;
@@ -271,7 +275,23 @@ constexpr int kLaunchOcc2 = 2;
 // flag kLaunchMfma (mixed precision only): the basis contractions run on the
 // matrix cores (faces_batch_mfma; point splits 1, 2 and 4)
 constexpr int kLaunchMfma = 4;
-constexpr int kLaunchFlags = kLaunchOcc2 | kLaunchMfma;
+// flag kLaunchCps (point splits 2..4): the waves of a group also split the
+// control points -- each wave reverses its own Gauss points into LDS and
+// accumulates the gradient of its share of the control points over every
+// point (faces_batch_kernel's CPS form); with kLaunchOcc2 the registers are
+// bounded for three waves per SIMD
+constexpr int kLaunchCps = 8;
+constexpr int kLaunchFlags = kLaunchOcc2 | kLaunchMfma | kLaunchCps;
+// null if (point_split, flags) names a launch shape, else the reason it does not
+inline const char *launch_shape_error(int point_split, int flags)
+{
+    if (point_split < 1 || point_split > kMaxSplit) return "point split must be between 1 and 4 waves";
+    if (flags & ~kLaunchFlags) return "unknown launch flags";
+    if ((flags & kLaunchMfma) && (flags & kLaunchCps)) return "the matrix-core path has no control-point split";
+    if ((flags & kLaunchMfma) && point_split == 3) return "the matrix-core path takes point split 1, 2 or 4";
+    if ((flags & kLaunchCps) && point_split == 1) return "the control-point split takes point split 2, 3 or 4";
+    return nullptr;
+}
 int features_of(const PhysParams &ph);
 hipError_t launch_face_metric(const FaceArgs &a, int kind, int mode, double *g0_out,
                               double *rurv0_out, double *ig0_out, double *partial, int partial_base,
@@ -281,14 +301,9 @@ hipError_t launch_vertex_gather(const GatherArgs &a, hipStream_t st);
 int gather_blocks(int nv);
 
 // restraint pre-pass over the same items as the face kernel (per-group area
-// and volume), then one finalize block per surface: first[i] is the FaceArgs
-// entry (in a.fa) of the i-th surface with restraints
-// Surfaces of at most this many 64-face groups fold the restraint finalize
-// into the face kernel (every face block sums the group partials itself).
-constexpr int kFoldMaxGroups = 2048;
-// pre-pass over a's items, then a finalize for the nsurf surfaces whose
-// class-0 FaceArgs entries first[] names (those that do not fold)
-hipError_t launch_restraint_prepass(const BatchFaceArgs &a, const int *first, int nsurf, hipStream_t st);
+// and volume), then the finalize: per surface, the partials summed in fixed
+// order into the restraint scalars
+hipError_t launch_restraint_prepass(const BatchFaceArgs &a, hipStream_t st);
 
 // face kernel over a work-item table (one surface or many); grid = 0: one
 // workgroup per item, else a persistent grid of that many workgroups striding

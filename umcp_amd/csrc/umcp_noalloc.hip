@@ -26,7 +26,7 @@
 
 namespace {
 
-constexpr size_t kScratchBytes = size_t(32) << 20;   // UMCP_NOALLOC_SCRATCH_BYTES
+constexpr size_t kScratchBytes = size_t(8) << 20;   // UMCP_NOALLOC_SCRATCH_BYTES
 static_assert(kScratchBytes == UMCP_NOALLOC_SCRATCH_BYTES, "header and library disagree on the scratch size");
 constexpr size_t kAlign = 256;
 constexpr int kMaxDevices = 64;
@@ -79,7 +79,6 @@ extern "C" int umcp_energy_grad_dev_noalloc(umcp_surface *s, const double *d_r, 
     const bool mixed = s->prm.precision == UMCP_PREC_MIXED;
     const bool rson = restraints_on(s);
     const size_t ncls = s->P.classes.size();
-    const size_t nnz = (size_t)s->P.nnz;
     const size_t ngroups = (size_t)s->npartial;
 
     // layout of this call's buffers in the scratch array
@@ -88,7 +87,7 @@ extern "C" int umcp_energy_grad_dev_noalloc(umcp_surface *s, const double *d_r, 
     const size_t o_partial = carve(off, 4 * ngroups * sizeof(double));
     const size_t o_rs = carve(off, 8 * sizeof(double));
     const size_t o_rpart = carve(off, 2 * ngroups * sizeof(double));
-    const size_t o_gbuf = carve(off, 3 * nnz * (mixed ? sizeof(float) : sizeof(double)));
+    const size_t o_gbuf = carve(off, 3 * (size_t)s->nrecords * (mixed ? sizeof(float) : sizeof(double)));
     if (off > kScratchBytes)
         return fail(s, UMCP_ERR_ARG,
                     "surface needs " + std::to_string(off) + " bytes of scratch; the no-allocation entry point holds " +
@@ -122,7 +121,7 @@ extern "C" int umcp_energy_grad_dev_noalloc(umcp_surface *s, const double *d_r, 
     fb.r = d_r;
     fb.step_ctr = nullptr;
     if (rson) {
-        e = launch_restraint_prepass(fb, s->d_zero, fa[0].rs_fold ? 0 : 1, st);
+        e = launch_restraint_prepass(fb, st);
         if (e != hipSuccess) return fail(s, UMCP_ERR_DEVICE, std::string("restraint kernel launch: ") + hipGetErrorString(e));
     }
     e = launch_faces_batch(fb, 0, s->point_split, s->launch_flags, mixed, features_of(fa[0].ph), st);
@@ -132,7 +131,8 @@ extern "C" int umcp_energy_grad_dev_noalloc(umcp_surface *s, const double *d_r, 
     ga.g = d_g;
     ga.E_out = d_E;
     ga.nv = s->T.nv;
-    ga.vstart = s->d_vstart;
+    ga.vstart = s->d_lvstart;
+    ga.vrec = s->d_lvrec;
     ga.gbuf = mixed ? nullptr : (const double *)(base + o_gbuf);
     ga.gbuf32 = mixed ? (const float *)(base + o_gbuf) : nullptr;
     ga.partial = partial;

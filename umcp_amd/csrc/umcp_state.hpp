@@ -48,6 +48,15 @@ struct umcp_surface {
     float *d_gbuf32 = nullptr;
     double *d_scalar = nullptr;    // [8]: E, sums[4], ...
     int *d_vstart = nullptr;
+    // the face kernel's group records (one per distinct vertex of a 64-face
+    // group): slot -> first record, record -> incidence range and list, and
+    // the gather's vertex -> records CSR
+    int nrecords = 0;
+    int *d_gl_start = nullptr;
+    int *d_linc_start = nullptr;
+    uint16_t *d_linc = nullptr;
+    int *d_lvstart = nullptr;
+    int *d_lvrec = nullptr;
     int npartial = 0;
     // launch shape of the face kernel: waves sharing 64 faces' Gauss points,
     // and whether the faces' control points are staged in LDS
@@ -73,7 +82,6 @@ struct umcp_surface {
     double A0_setg0 = 0.0, V_setg0 = 0.0;
     double *d_rs = nullptr;               // [8] dE/dA, dE/dV, E, A, V
     double *d_rpart = nullptr;            // [npartial][2] per-group area and volume
-    int *d_zero = nullptr;                // [1] = 0: the finalize's table entry (class 0)
     umcp::FaceArgs *d_fa_aux = nullptr;   // a second face table (volume-only passes)
     // Brownian step counter of the device-noise path
     int *d_step = nullptr;

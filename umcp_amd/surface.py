@@ -28,6 +28,7 @@ GRAD_REFERENCE = 0   # surface::grad's Helfrich gradient (src/grad.C:448)
 GRAD_EXACT = 1       # the exact derivative of the Helfrich energy
 LAUNCH_OCC2 = 2
 LAUNCH_MFMA = 4
+LAUNCH_CPS = 8
 
 # every symbol include/umcp.h declares
 EXPORTS = [

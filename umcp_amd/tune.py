@@ -1,7 +1,8 @@
 """Launch-shape autotuner for the face kernel.
 
-The fused face kernel has three tunable choices: how many waves share the
-Gauss points of a 64-face group (1..4), whether registers are bounded, and
+The fused face kernel has four tunable choices: how many waves share the
+Gauss points of a 64-face group (1..4), whether those waves also split the
+control points (CPS form, splits 2..4), whether registers are bounded, and
 (mixed precision) whether the basis contractions run on the matrix cores
 (point splits 1, 2 and 4 there).  ``tune`` times every shape on the caller's
 own surface with the library's HIP-event profiling and keeps the fastest; it
@@ -14,12 +15,14 @@ import itertools
 
 SPLITS = (1, 2, 3, 4)
 FLAGS = (0, 2)
+CPS_SPLITS = (2, 3, 4)
+CPS_FLAGS = (8,)
 MFMA_SPLITS = (1, 2, 4)
 MFMA_FLAGS = (4, 6)
 
 
 def shapes(mixed: bool):
-    out = list(itertools.product(SPLITS, FLAGS))
+    out = list(itertools.product(SPLITS, FLAGS)) + list(itertools.product(CPS_SPLITS, CPS_FLAGS))
     if mixed:
         out += list(itertools.product(MFMA_SPLITS, MFMA_FLAGS))
     return out
@@ -61,8 +64,10 @@ def tune(surface, d_r, d_E, d_g, stream=None, reps: int = 10, sync=None, mixed: 
 
 # batched ensembles: waves per face group, register bound, persistent grid
 # (0 = one workgroup per item) and item order
-ENSEMBLE_SHAPES = tuple((sp, fl, grid, order)
-                        for sp in (2, 4) for fl in (0, 2) for grid in (0, 1024, 2048) for order in (0, 1))
+ENSEMBLE_SHAPES = (tuple((sp, fl, grid, order)
+                         for sp in (2, 4) for fl in (0, 2) for grid in (0, 1024, 2048) for order in (0, 1))
+                   + tuple((sp, 8, grid, order)
+                           for sp in (2, 3, 4) for grid in (0, 1024, 2048) for order in (0, 1)))
 # mixed precision adds the matrix-core path
 ENSEMBLE_SHAPES_MIXED = ENSEMBLE_SHAPES + tuple((sp, fl, grid, order)
                                                 for sp in (2, 4) for fl in (4, 6) for grid in (0, 1024)
