@@ -169,35 +169,92 @@ __device__ __forceinline__ void gather_cps(const FaceArgs &a, const double *__re
     }
 }
 
-// The global restraints' energy and multipliers from the pre-pass's group
-// partials (area: src/interpLib.C:4322-4326; volume: src/minimize.C:90-114):
-// A_tot and V summed in one fixed order -- lane l takes groups l, l + 64, ...
-// in turn, then an xor butterfly over the wave, whose every lane ends with
-// the same bits since IEEE addition commutes -- so the result does not depend
-// on which block forms it.  out = {dE/dA_tot, dE/dV, restraint energy, A_tot,
-// V}.
-__device__ __forceinline__ void restraint_scalars(const FaceArgs &a, double out[5])
+// Control points through the group's distinct vertices: the block stages the
+// positions of the group's vertices (at most kStageMax of them) in LDS once,
+// then every lane reads its control points through their local indices, so a
+// 64-face group makes about 95 scattered position loads instead of 768.  The
+// local indices and the vertex list load together, so the positions still
+// wait for one round trip only.  A group with more distinct vertices reads
+// its control points directly (the choice is uniform over the block).  Every
+// thread of the block calls this (it synchronises the block; with NT = 64
+// and WAVE, one wave stages its own group and synchronises only itself); sV is
+// LDS scratch of kStageMax * 3 doubles.  Output as gather_cps<NCP, STRIDE>.
+constexpr int kStageMax = 192;
+// LDS written by some lanes of a wave, then read by others of the same wave
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+template <int NCP, int STRIDE, int NT, bool WAVE = false>
+__device__ __forceinline__ void stage_cps(const FaceArgs &a, const double *__restrict__ r, int slot, int fc, int c0,
+                                          double *sV, double x[(NCP + STRIDE - 1) / STRIDE][3])
+{
+    static_assert(!WAVE || NT == kWave, "a wave stages with its own 64 lanes");
+    constexpr int kN = (NCP + STRIDE - 1) / STRIDE;
+    constexpr int kJ = (kStageMax + NT - 1) / NT;
+    const int tid = WAVE ? (int)(threadIdx.x & (kWave - 1)) : (int)threadIdx.x;
+    const int nf = a.nf;
+    const int L0 = a.gl_start[slot];
+    const int n = a.gl_start[slot + 1] - L0;
+    if (n > kStageMax) {
+        if (!WAVE) __syncthreads();
+        gather_cps<NCP, STRIDE>(a, r, fc, c0, x);
+        return;
+    }
+    int vj[kJ];
+#pragma unroll
+    for (int j = 0; j < kJ; j++) {
+        const int t = tid + j * NT;
+        vj[j] = t < n ? a.lvert[L0 + t] : 0;
+    }
+    int l[kN];
+#pragma unroll
+    for (int i = 0; i < kN; i++) l[i] = a.loc[min(c0 + i * STRIDE, NCP - 1) * nf + fc];
+    double p[kJ][3];
+#pragma unroll
+    for (int j = 0; j < kJ; j++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) p[j][k] = r[3 * vj[j] + k];
+#pragma unroll
+    for (int j = 0; j < kJ; j++) {
+        const int t = tid + j * NT;
+        if (t < n)
+#pragma unroll
+            for (int k = 0; k < 3; k++) sV[3 * t + k] = p[j][k];
+    }
+    if (WAVE)
+        wave_sync();
+    else
+        __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kN; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) x[i][k] = sV[3 * l[i] + k];
+    if (a.has_img[fc]) {
+#pragma unroll
+        for (int i = 0; i < kN; i++) {
+            double o[3];
+            image_offset(a.img[min(c0 + i * STRIDE, NCP - 1) * nf + fc], a.pbc, o);
+#pragma unroll
+            for (int k = 0; k < 3; k++) x[i][k] += o[k];
+        }
+    }
+}
+
+// The global restraints' energy and multipliers (area: src/interpLib.C:
+// 4322-4326; volume: src/minimize.C:90-114) from the per-wave sums of the
+// group partials, red[w] = {A, V} of wave w of the finalize block, added in
+// wave order: out = {dE/dA_tot, dE/dV, restraint energy, A_tot, V}.
+constexpr int kFinalizeThreads = 256;
+__device__ __forceinline__ void restraint_form(const FaceArgs &a, const double (*red)[2], double out[5])
 {
     #pragma clang fp contract(off)
-    const int lane = threadIdx.x & (kWave - 1);
-    constexpr int kU = 16;   // loads in flight per lane
-    const double2 *part = reinterpret_cast<const double2 *>(a.rpart);
-    double sA = 0.0, sV = 0.0;
-    for (int i0 = 0; i0 < a.ngroups; i0 += kU * kWave) {
-        double2 v[kU];
-#pragma unroll
-        for (int j = 0; j < kU; j++) v[j] = part[min(i0 + j * kWave + lane, a.ngroups - 1)];
-#pragma unroll
-        for (int j = 0; j < kU; j++)
-            if (i0 + j * kWave + lane < a.ngroups) {
-                sA += v[j].x;
-                sV += v[j].y;
-            }
-    }
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        sA += __shfl_xor(sA, off, kWave);
-        sV += __shfl_xor(sV, off, kWave);
+    double sA = red[0][0], sV = red[0][1];
+    for (int q = 1; q < kFinalizeThreads / kWave; q++) {
+        sA += red[q][0];
+        sV += red[q][1];
     }
     double lamA = 0.0, lamV = 0.0, E = 0.0;
     if (a.do_area && a.KA_g != 0.0) {
@@ -217,13 +274,106 @@ __device__ __forceinline__ void restraint_scalars(const FaceArgs &a, double out[
     out[4] = sV;
 }
 
-// The multipliers a face block uses, written by the restraint pre-pass.
-__device__ __forceinline__ void restraint_multipliers(const FaceArgs &a, double lam[2])
+// The restraint scalars by one block of kFinalizeThreads threads: thread t
+// sums groups t, t + kFinalizeThreads, ... in turn (every load issued
+// first), then xor butterflies in each wave and the waves in order, so the
+// result does not depend on the launch.  Thread 0 receives out.
+__device__ __forceinline__ void restraint_scalars(const FaceArgs &a, double out[5])
+{
+    #pragma clang fp contract(off)
+    constexpr int NT = kFinalizeThreads, kU = 8;
+    const double2 *part = reinterpret_cast<const double2 *>(a.rpart);
+    double sA = 0.0, sV = 0.0;
+    for (int i0 = 0; i0 < a.ngroups; i0 += kU * NT) {
+        double2 v[kU];
+#pragma unroll
+        for (int j = 0; j < kU; j++) v[j] = part[min(i0 + j * NT + (int)threadIdx.x, a.ngroups - 1)];
+#pragma unroll
+        for (int j = 0; j < kU; j++)
+            if (i0 + j * NT + (int)threadIdx.x < a.ngroups) {
+                sA += v[j].x;
+                sV += v[j].y;
+            }
+    }
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        sA += __shfl_xor(sA, off, kWave);
+        sV += __shfl_xor(sV, off, kWave);
+    }
+    __shared__ double red[NT / kWave][2];
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        red[threadIdx.x / kWave][0] = sA;
+        red[threadIdx.x / kWave][1] = sV;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) restraint_form(a, red, out);
+}
+
+// The restraint scalars formed by a face block (see restraint_multipliers).
+template <int NT>
+__device__ __forceinline__ void restraint_fold(const FaceArgs &a, int slot, double lam[2])
+{
+    #pragma clang fp contract(off)
+    constexpr int NF = kFinalizeThreads, kU = 8, kR = (NF + NT - 1) / NT;
+    const double2 *part = reinterpret_cast<const double2 *>(a.rpart);
+    // finalize thread t = threadIdx.x + j NT (t < NF) sums groups t, t + NF, ...
+    double sA[kR], sV[kR];
+#pragma unroll
+    for (int j = 0; j < kR; j++) {
+        const int t = (int)threadIdx.x + j * NT;
+        sA[j] = sV[j] = 0.0;
+        if (t >= NF) continue;
+        for (int i0 = 0; i0 < a.ngroups; i0 += kU * NF) {
+            double2 v[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) v[u] = part[min(i0 + u * NF + t, a.ngroups - 1)];
+#pragma unroll
+            for (int u = 0; u < kU; u++)
+                if (i0 + u * NF + t < a.ngroups) {
+                    sA[j] += v[u].x;
+                    sV[j] += v[u].y;
+                }
+        }
+    }
+    __shared__ double red[NF / kWave][2];
+#pragma unroll
+    for (int j = 0; j < kR; j++) {
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            sA[j] += __shfl_xor(sA[j], off, kWave);
+            sV[j] += __shfl_xor(sV[j], off, kWave);
+        }
+        if ((threadIdx.x & (kWave - 1)) == 0 && (int)threadIdx.x + j * NT < NF) {
+            red[(threadIdx.x + j * NT) / kWave][0] = sA[j];
+            red[(threadIdx.x + j * NT) / kWave][1] = sV[j];
+        }
+    }
+    __syncthreads();
+    double o[5];
+    restraint_form(a, red, o);
+    lam[0] = o[0];
+    lam[1] = o[1];
+    if (slot == 0 && threadIdx.x == 0)
+#pragma unroll
+        for (int i = 0; i < 5; i++) a.rs[i] = o[i];
+}
+
+// The multipliers a face block uses: written by the restraint finalize, or
+// (a.rs_fold) formed by the block itself from the pre-pass partials, the
+// finalize's sums in the finalize's order (its block size NF), so both give
+// the same bits; the slot-0 block then also writes rs for the gather and the
+// host.  Every thread of the block calls this.
+template <int NT>
+__device__ __forceinline__ void restraint_multipliers(const FaceArgs &a, int slot, double lam[2])
 {
     lam[0] = lam[1] = 0.0;
     if (!a.rs) return;
-    lam[0] = a.rs[0];
-    lam[1] = a.rs[1];
+    if (!a.rs_fold) {
+        lam[0] = a.rs[0];
+        lam[1] = a.rs[1];
+        return;
+    }
+    restraint_fold<NT>(a, slot, lam);
 }
 
 // The gradient records of one 64-face group: T[(c * 3 + k) * 64 + lane] holds
@@ -511,12 +661,14 @@ __device__ __forceinline__ void face_group(const FaceArgs &a, const double *__re
     const R al[3] = {(R)al64[0], (R)al64[1], (R)al64[2]};
     // the restraint multipliers first, while few registers are live
     double lam[2];   // dE/dA_tot, dE/dV
-    restraint_multipliers(a, lam);
+    restraint_multipliers<WS * kWave>(a, slot, lam);
     double base[3] = {0.0, 0.0, 0.0};
     R P[PLDS ? 1 : NCP][3];
     if (PLDS) {
+        // the staging scratch lies past the control points' rows (in wave 1's
+        // gradient buffer, or the record scratch of a one-wave group)
         double t[(NCP + WS - 1) / WS][3];
-        gather_cps<NCP, WS>(a, r, fc, WS == 1 ? 0 : w, t);
+        stage_cps<NCP, WS, WS * kWave>(a, r, slot, fc, WS == 1 ? 0 : w, (double *)(lds + 13 * 3 * kWave), t);
         if (kRel) {
             const int v0 = a.cp[fc];
             for (int k = 0; k < 3; k++) base[k] = r[3 * v0 + k];
@@ -735,8 +887,9 @@ __device__ __forceinline__ void face_group_cps(const FaceArgs &a, const double *
     const double al64[3] = {r[3 * a.nv], r[3 * a.nv + 1], r[3 * a.nv + 2]};
     const R al[3] = {(R)al64[0], (R)al64[1], (R)al64[2]};
     double lam[2];   // dE/dA_tot, dE/dV
-    restraint_multipliers(a, lam);
-    // control point 0 (the origin) and this wave's own control points
+    restraint_multipliers<WS * kWave>(a, slot, lam);
+    // control point 0 (the origin) and this wave's own control points (the
+    // staging scratch lies in the points' reverse rows, unused until then)
     double base[3];
     {
         double t[1][3];
@@ -745,7 +898,7 @@ __device__ __forceinline__ void face_group_cps(const FaceArgs &a, const double *
     }
     {
         double t[kOwn][3];
-        gather_cps<NCP, WS>(a, r, fc, w, t);
+        stage_cps<NCP, WS, WS * kWave>(a, r, slot, fc, w, (double *)(sP + 13 * 3 * kWave), t);
 #pragma unroll
         for (int i = 0; i < kOwn; i++) {
             const int c = w + i * WS;

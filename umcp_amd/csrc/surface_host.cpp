@@ -645,6 +645,7 @@ int build_plan(const Topology &T, Plan &P, std::string &err)
         C.gl_vert.clear();
         C.gl_inc_off.assign(1, 0);
         C.gl_inc.clear();
+        C.loc.assign(C.cp.size(), 0);
         std::vector<int> local(T.nv, -1);
         for (int g0 = 0; g0 < n; g0 += 64) {
             const int g1 = std::min(n, g0 + 64);
@@ -659,6 +660,7 @@ int build_plan(const Topology &T, Plan &P, std::string &err)
                         inc.emplace_back();
                     }
                     inc[local[v]].push_back((uint16_t)(q * 64 + (x - g0)));
+                    C.loc[(size_t)q * n + x] = (uint16_t)local[v];
                 }
             for (size_t l = 0; l < verts.size(); l++) {
                 C.gl_vert.push_back(verts[l]);

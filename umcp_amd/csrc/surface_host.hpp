@@ -55,6 +55,7 @@ struct FaceClass {
     std::vector<int> gl_vert;          // local vertices
     std::vector<int> gl_inc_off;       // local vertices + 1
     std::vector<uint16_t> gl_inc;      // ncp * nf
+    std::vector<uint16_t> loc;         // [ncp][nf] index of (slot, face)'s vertex in its group's list
     int nf() const { return (int)face_id.size(); }
 };
 

@@ -144,12 +144,15 @@ FaceArgs face_args(umcp_surface *s, size_t c, const double *d_r)
     a.gl_start = s->d_gl_start;
     a.linc_start = s->d_linc_start;
     a.linc = s->d_linc;
+    a.lvert = s->d_lvert;
+    a.loc = D.loc;
     a.ph = phys(s);
     a.step_ctr = nullptr;
     a.partial = s->d_partial;
     a.rs = restraints_on(s) ? s->d_rs : nullptr;
     a.rpart = s->d_rpart;
     a.ngroups = s->npartial;
+    a.rs_fold = a.rs && a.ngroups <= kFoldMaxGroups ? 1 : 0;
     a.KA_g = s->rsp.KA_global;
     a.KV = s->rsp.KV;
     a.A0 = s->rsp.A0 > 0.0 ? s->rsp.A0 : s->A0_setg0;
@@ -167,7 +170,7 @@ size_t plan_bytes(const umcp_surface *s)
     int metric_total = 0, ngroups = 0;
     for (const FaceClass &C : s->P.classes) {
         const size_t nf = C.nf();
-        b += sz(C.cp.size(), 4) + sz(C.pos.size(), 4) + sz(C.img.size(), 1) + sz(nf, 1);
+        b += sz(C.cp.size(), 4) + sz(C.pos.size(), 4) + sz(C.loc.size(), 2) + sz(C.img.size(), 1) + sz(nf, 1);
         b += 3 * sz(nf * C.npt, 8) + 3 * sz(nf, 8);
         metric_total += metric_blocks(C.nf());
         ngroups += face_groups(C.nf());
@@ -178,8 +181,8 @@ size_t plan_bytes(const umcp_surface *s)
     for (const FaceClass &C : s->P.classes) nloc += C.gl_vert.size();
     b += sz(4 * (size_t)std::max(ngroups, metric_total), 8) + sz(8, 8) + sz(s->P.vstart.size(), 4);
     // group records: per-group bases, incidence ranges and lists, vertex CSR
-    b += sz(ngroups + 1, 4) + sz(nloc + 1, 4) + sz(s->P.nnz, 2) + sz(nv + 1, 4) + sz(nloc, 4);
-    b += sz(8, 8) + sz(2 * (size_t)ngroups, 8) + sz(1, 4);   // rs, rpart, step
+    b += sz(ngroups + 1, 4) + sz(nloc + 1, 4) + sz(s->P.nnz, 2) + sz(nv + 1, 4) + 2 * sz(nloc, 4);
+    b += sz(8, 8) + sz(2 * (size_t)ngroups, 8) + 2 * sz(1, 4);   // rs, rpart, step, fin
     b += 2 * sz(s->P.classes.size(), sizeof(FaceArgs)) + sz(ngroups, sizeof(FaceItem));
     return b;
 }
@@ -196,6 +199,7 @@ int enqueue_volume(umcp_surface *s, const double *d_r, hipStream_t st)
     for (size_t c = 0; c < fa.size(); c++) {
         fa[c] = face_args(s, c, d_r);
         fa[c].rs = s->d_rs;
+        fa[c].rs_fold = 0;   // no face kernel follows: the finalize writes d_rs
         fa[c].do_area = 0;
         fa[c].do_volume = 1;
         fa[c].KA_g = 0.0;
@@ -203,7 +207,7 @@ int enqueue_volume(umcp_surface *s, const double *d_r, hipStream_t st)
     }
     hipError_t e = hipMemcpyAsync(s->d_fa_aux, fa.data(), fa.size() * sizeof(FaceArgs), hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(s, e, "upload face table");
-    BatchFaceArgs fb{s->d_fa_aux, s->d_items, s->npartial, d_r, nullptr};
+    BatchFaceArgs fb{s->d_fa_aux, s->d_items, s->npartial, d_r, nullptr, s->d_fin, 1};
     e = launch_restraint_prepass(fb, st);
     if (e != hipSuccess) return hip_fail(s, e, "restraint kernel launch");
     return UMCP_OK;
@@ -264,6 +268,8 @@ int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs 
     fb.nitems = s->npartial;
     fb.r = d_r;
     fb.step_ctr = step_ctr;
+    fb.fin = s->d_fin;
+    fb.nfin = rson && s->npartial > kFoldMaxGroups ? 1 : 0;   // else folded into the face kernel
     hipEvent_t ev[2] = {nullptr, nullptr};
     const bool timed = s->prof_on && (s->prof_tick++ % s->prof_every) == 0;
     if (rson) {
@@ -481,6 +487,7 @@ int umcp_plan(umcp_surface *s)
         const size_t nf = C.nf();
         if (kind_of(C.ncp, C.npt) < 0) return fail(s, UMCP_ERR_BAD_MESH, "no kernel for this face valence");
         if ((rc = dev_alloc(s, &D.cp, C.cp.size())) || (rc = dev_alloc(s, &D.pos, C.pos.size())) || (rc = dev_alloc(s, &D.img, C.img.size())) ||
+            (rc = dev_alloc(s, &D.loc, C.loc.size())) ||
             (rc = dev_alloc(s, &D.has_img, nf)) || (rc = dev_alloc(s, &D.g0, nf * C.npt)) ||
             (rc = dev_alloc(s, &D.rurv0, nf * C.npt)) || (rc = dev_alloc(s, &D.ig0, nf * C.npt)) || (rc = dev_alloc(s, &D.Lo, nf)) ||
             (rc = dev_alloc(s, &D.Li, nf)) || (rc = dev_alloc(s, &D.c0, nf)) ||
@@ -489,6 +496,7 @@ int umcp_plan(umcp_surface *s)
         if ((rc = upload_tables_once(s, C))) return rc;
         if ((rc = dev_upload(s, D.cp, C.cp.data(), C.cp.size())) ||
             (rc = dev_upload(s, D.pos, C.pos.data(), C.pos.size())) ||
+            (rc = dev_upload(s, D.loc, C.loc.data(), C.loc.size())) ||
             (rc = dev_upload(s, D.img, C.img.data(), C.img.size())) ||
             (rc = dev_upload(s, D.has_img, C.has_img.data(), nf)) ||
             0)
@@ -534,7 +542,8 @@ int umcp_plan(umcp_surface *s)
         s->nrecords = (int)lvert.size();
         if ((rc = dev_alloc(s, &s->d_gl_start, gl_start.size())) || (rc = dev_alloc(s, &s->d_linc_start, linc_start.size())) ||
             (rc = dev_alloc(s, &s->d_linc, linc.size())) || (rc = dev_alloc(s, &s->d_lvstart, lvstart.size())) ||
-            (rc = dev_alloc(s, &s->d_lvrec, lvrec.size())) ||
+            (rc = dev_alloc(s, &s->d_lvrec, lvrec.size())) || (rc = dev_alloc(s, &s->d_lvert, lvert.size())) ||
+            (rc = dev_upload(s, s->d_lvert, lvert.data(), lvert.size())) ||
             (rc = dev_upload(s, s->d_gl_start, gl_start.data(), gl_start.size())) ||
             (rc = dev_upload(s, s->d_linc_start, linc_start.data(), linc_start.size())) ||
             (rc = dev_upload(s, s->d_linc, linc.data(), linc.size())) ||
@@ -547,7 +556,7 @@ int umcp_plan(umcp_surface *s)
     for (const FaceClass &C : s->P.classes)
         for (uint8_t h : C.has_img) s->has_images = s->has_images || h;
     if ((rc = dev_alloc(s, &s->d_rs, 8)) || (rc = dev_alloc(s, &s->d_rpart, 2 * (size_t)s->npartial)) ||
-        (rc = dev_alloc(s, &s->d_step, 1)))
+        (rc = dev_alloc(s, &s->d_step, 1)) || (rc = dev_alloc(s, &s->d_fin, 1)))
         return rc;
     // face work tables
     std::vector<FaceItem> items;
@@ -564,7 +573,7 @@ int umcp_plan(umcp_surface *s)
     s->fa_dirty = true;
     const int zero_step = 0;
     const double zeros8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if ((rc = dev_upload(s, s->d_step, &zero_step, 1)) ||
+    if ((rc = dev_upload(s, s->d_step, &zero_step, 1)) || (rc = dev_upload(s, s->d_fin, &zero_step, 1)) ||
         (rc = dev_upload(s, s->d_rs, zeros8, 8)))
         return rc;
     if ((rc = sync(s))) return rc;

@@ -51,7 +51,8 @@ struct umcp_ensemble {
     FaceItem *d_items = nullptr;
     GatherArgs *d_ga = nullptr;
     GatherItem *d_gitems = nullptr;
-    int nitems = 0, ngitems = 0, nrestrained = 0;
+    int *d_fin = nullptr;         // index in d_items of each restrained member's slot-0 item
+    int nitems = 0, ngitems = 0, nrestrained = 0, nfin = 0;
     std::vector<void *> tables;   // allocations of the current build
     // captured step sequence
     hipGraph_t graph = nullptr;
@@ -101,7 +102,8 @@ void drop_tables(umcp_ensemble *e)
     e->d_items = nullptr;
     e->d_ga = nullptr;
     e->d_gitems = nullptr;
-    e->nitems = e->ngitems = e->nrestrained = 0;
+    e->d_fin = nullptr;
+    e->nitems = e->ngitems = e->nrestrained = e->nfin = 0;
 }
 
 template <typename T>
@@ -204,10 +206,17 @@ int build(umcp_ensemble *e)
         }
     }
     e->feat_start[3] = (int)items.size();
+    // the restraint finalize's items: the slot-0 item of each restrained
+    // member that does not fold the finalize into its face blocks
+    std::vector<int> fin;
+    for (size_t i = 0; i < items.size(); i++)
+        if (items[i].slot == 0 && fa[items[i].fa].rs && !fa[items[i].fa].rs_fold) fin.push_back((int)i);
     int rc;
     if ((rc = upload_table(e, &e->d_fa, fa)) || (rc = upload_table(e, &e->d_items, items)) ||
-        (rc = upload_table(e, &e->d_ga, ga)) || (rc = upload_table(e, &e->d_gitems, gitems)))
+        (rc = upload_table(e, &e->d_ga, ga)) || (rc = upload_table(e, &e->d_gitems, gitems)) ||
+        (rc = upload_table(e, &e->d_fin, fin)))
         return rc;
+    e->nfin = (int)fin.size();
     e->nitems = (int)items.size();
     e->ngitems = (int)gitems.size();
     e->nrestrained = nrestrained;
@@ -221,14 +230,14 @@ hipError_t enqueue_step(umcp_ensemble *e, hipStream_t st)
     hipError_t err = hipSuccess;
     if (e->nrestrained > 0) {
         // over every item: groups of members without restraints return at once
-        BatchFaceArgs rb{e->d_fa, e->d_items, e->nitems, nullptr, nullptr};
+        BatchFaceArgs rb{e->d_fa, e->d_items, e->nitems, nullptr, nullptr, e->d_fin, e->nfin};
         err = launch_restraint_prepass(rb, st);
         if (err != hipSuccess) return err;
     }
     for (int f = 0; f < 3; f++) {
         const int n = e->feat_start[f + 1] - e->feat_start[f];
         if (n == 0) continue;
-        BatchFaceArgs fb{e->d_fa, e->d_items + e->feat_start[f], n, nullptr, nullptr};
+        BatchFaceArgs fb{e->d_fa, e->d_items + e->feat_start[f], n, nullptr, nullptr, nullptr, 0};
         err = launch_faces_batch(fb, e->face_grid, e->point_split, e->flags, e->mixed, f, st);
         if (err != hipSuccess) return err;
     }

@@ -68,7 +68,7 @@ __device__ __forceinline__ void face_group_mx(const FaceArgs &a, const double *_
     // 1. the restraint multipliers, then staging: lane = face, the waves
     // split the control points
     double lam[2];   // dE/dA_tot, dE/dV
-    restraint_multipliers(a, lam);
+    restraint_multipliers<WS * kWave>(a, slot, lam);
     {
         const int f = group * kWave + lane;
         const int fc = f < nf ? f : nf - 1;

@@ -542,17 +542,17 @@ __global__ __launch_bounds__(kGatherBlock) void vertex_gather_batch(BatchGatherA
 // each surface's partials in fixed order and writes the multipliers the face
 // kernel reads.
 // ---------------------------------------------------------------------------
-// One 64-face group, lane = face: the waves gather the group's control
-// points once (wave w takes c = w, w + WS, ...) into LDS, then each wave
-// reads all of them into registers and takes the Gauss points p = w, w + WS,
-// ...; the last wave also forms the corner limit points and the volume term.
+// One 64-face group, lane = face: the block stages the group's distinct
+// vertices in LDS, each wave reads all of a face's control points from there
+// into registers and takes the Gauss points p = w, w + WS, ...; the last wave
+// also forms the corner limit points and the volume term.
 // (Single-surface and batched evaluations run this same kernel, so
 // contraction choices cannot differ between them.)
 // FP64 throughout, for the mixed path too: the multipliers depend on
 // A_tot - A0 and V - V0, small differences of the sums.
 template <int NCP, int NPT, int KIND, int WS>
-__device__ __forceinline__ void restraint_group(const FaceArgs &a, const double *__restrict__ r, int group,
-                                                double *lds, double acc[4])
+__device__ __forceinline__ void restraint_group(const FaceArgs &a, const double *__restrict__ r, int group, int slot,
+                                                double *sV, double acc[4])
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -560,23 +560,8 @@ __device__ __forceinline__ void restraint_group(const FaceArgs &a, const double 
     const bool live = f < a.nf;
     const int fc = live ? f : a.nf - 1;
     const double al[3] = {r[3 * a.nv], r[3 * a.nv + 1], r[3 * a.nv + 2]};
-    {
-        double t[(NCP + WS - 1) / WS][3];
-        gather_cps<NCP, WS>(a, r, fc, WS == 1 ? 0 : w, t);
-#pragma unroll
-        for (int i = 0; i < (NCP + WS - 1) / WS; i++) {
-            const int c = w + i * WS;
-            if (c < NCP)
-#pragma unroll
-                for (int k = 0; k < 3; k++) lds[(c * 3 + k) * kWave + lane] = t[i][k];
-        }
-    }
-    __syncthreads();
     double x[NCP][3];
-#pragma unroll
-    for (int c = 0; c < NCP; c++)
-#pragma unroll
-        for (int k = 0; k < 3; k++) x[c][k] = lds[(c * 3 + k) * kWave + lane];
+    stage_cps<NCP, 1, WS * kWave>(a, r, slot, fc, 0, sV, x);
     double A = 0.0, V = 0.0;
     if (a.do_area) {
         const double *tab = kind_table<KIND, double>();
@@ -621,41 +606,34 @@ __device__ __forceinline__ void restraint_group(const FaceArgs &a, const double 
     }
 }
 
-constexpr int kPrepassWaves = 2;   // waves per 64-face group
-
-template <int WS>
-__global__ __launch_bounds__(kWave *WS) void restraint_prepass_kernel(BatchFaceArgs B)
+constexpr int kPrepassWaves = 4;   // waves per 64-face group (Gauss points split)
+__global__ __launch_bounds__(kWave *kPrepassWaves) void restraint_prepass_kernel(BatchFaceArgs B)
 {
     const int it = xcd_logical_block(blockIdx.x, gridDim.x);
     if (it >= B.nitems) return;
     const FaceItem item = B.items[it];
     const FaceArgs &a = B.fa[item.fa];
     if (!a.rs) return;   // a surface without restraints (uniform over the block)
-    __shared__ double lds[13 * 3 * kWave];
+    __shared__ double sV[kStageMax * 3];
     const double *r = B.r ? B.r : a.r;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     switch (item.kind) {
-    case 0: restraint_group<12, 15, 0, WS>(a, r, item.group, lds, acc); break;
-    case 1: restraint_group<11, 16, 1, WS>(a, r, item.group, lds, acc); break;
-    default: restraint_group<13, 16, 2, WS>(a, r, item.group, lds, acc); break;
+    case 0: restraint_group<12, 15, 0, kPrepassWaves>(a, r, item.group, item.slot, sV, acc); break;
+    case 1: restraint_group<11, 16, 1, kPrepassWaves>(a, r, item.group, item.slot, sV, acc); break;
+    default: restraint_group<13, 16, 2, kPrepassWaves>(a, r, item.group, item.slot, sV, acc); break;
     }
-    block_sum4<kWave * WS>(acc);
+    block_sum4<kWave * kPrepassWaves>(acc);
     if (threadIdx.x == 0) {
         a.rpart[2 * (size_t)item.slot] = acc[0];
         a.rpart[2 * (size_t)item.slot + 1] = acc[1];
     }
 }
 
-// The restraint scalars of every surface with restraints: the block of the
-// surface's slot-0 item forms them from the pre-pass partials; every other
-// block returns at once.  (A separate launch: a pre-pass block cannot see the
-// other blocks' partials without a device-wide cache write-back, which cost
-// far more than this launch.)
-__global__ __launch_bounds__(kWave) void restraint_finalize_kernel(BatchFaceArgs B)
+// The restraint scalars of every surface with restraints: block b forms them
+// for the surface of item fin[b] (its slot-0 item) from the pre-pass partials.
+__global__ __launch_bounds__(kFinalizeThreads) void restraint_finalize_kernel(BatchFaceArgs B)
 {
-    const FaceItem item = B.items[blockIdx.x];
-    if (item.slot != 0) return;
-    const FaceArgs &a = B.fa[item.fa];
+    const FaceArgs &a = B.fa[B.items[B.fin[blockIdx.x]].fa];
     if (!a.rs) return;
     double o[5];
     restraint_scalars(a, o);
@@ -868,10 +846,10 @@ hipError_t launch_gather_batch(const BatchGatherArgs &a, int grid, bool mixed, h
 hipError_t launch_restraint_prepass(const BatchFaceArgs &a, hipStream_t st)
 {
     if (a.nitems <= 0) return hipSuccess;
-    hipLaunchKernelGGL((restraint_prepass_kernel<kPrepassWaves>), dim3(a.nitems), dim3(kWave * kPrepassWaves), 0, st, a);
+    hipLaunchKernelGGL(restraint_prepass_kernel, dim3(a.nitems), dim3(kWave * kPrepassWaves), 0, st, a);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(restraint_finalize_kernel, dim3(a.nitems), dim3(kWave), 0, st, a);
+    if (e != hipSuccess || a.nfin <= 0) return e;
+    hipLaunchKernelGGL(restraint_finalize_kernel, dim3(a.nfin), dim3(kFinalizeThreads), 0, st, a);
     return hipGetLastError();
 }
 

@@ -47,6 +47,11 @@ struct FaceArgs {
     const int *gl_start;
     const int *linc_start;
     const uint16_t *linc;
+    // record L's vertex, and the index among its group's records of the
+    // vertex of (face, slot): [ncp][nf] (the face kernels stage a group's
+    // control points through these)
+    const int *lvert;
+    const uint16_t *loc;
     PhysParams ph;
     int *step_ctr;          // Brownian step counter, advanced once per evaluation, or null
     double *partial;        // the surface's block partials [ngroups][4]
@@ -56,6 +61,9 @@ struct FaceArgs {
     double *rs;
     double *rpart;
     int ngroups;
+    // 1: the face kernel's blocks form the restraint scalars from rpart
+    // themselves (no finalize launch; the slot-0 block writes rs)
+    int rs_fold;
     double KA_g, KV, A0, V0;
     int do_area, do_volume;
 };
@@ -74,6 +82,10 @@ struct BatchFaceArgs {
     int nitems;
     const double *r;        // non-null: positions for every item (a surface's own launch)
     int *step_ctr;          // non-null: the step counter the slot-0 item advances
+    // the restraint finalize: one block per entry, each the index in items of
+    // a surface's slot-0 item (surfaces with restraints only)
+    const int *fin;
+    int nfin;
 };
 
 // Counter-based normal deviates: Philox4x64-10 keyed by (seed, 'UMCP') at
@@ -301,8 +313,13 @@ hipError_t launch_vertex_gather(const GatherArgs &a, hipStream_t st);
 int gather_blocks(int nv);
 
 // restraint pre-pass over the same items as the face kernel (per-group area
-// and volume), then the finalize: per surface, the partials summed in fixed
-// order into the restraint scalars
+// and volume), then the finalize for the surfaces a.fin names: per surface,
+// the partials summed in fixed order into the restraint scalars.  Surfaces of
+// at most kFoldMaxGroups groups fold the finalize into the face kernel
+// instead (each face block sums the partials itself: one load per thread,
+// cheaper than a launch for small surfaces; on the 1280-group vesicle the
+// fold measured 6 us slower than the 4.7 us finalize launch).
+constexpr int kFoldMaxGroups = 256;
 hipError_t launch_restraint_prepass(const BatchFaceArgs &a, hipStream_t st);
 
 // face kernel over a work-item table (one surface or many); grid = 0: one

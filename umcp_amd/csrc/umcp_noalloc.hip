@@ -120,6 +120,8 @@ extern "C" int umcp_energy_grad_dev_noalloc(umcp_surface *s, const double *d_r, 
     fb.nitems = s->npartial;
     fb.r = d_r;
     fb.step_ctr = nullptr;
+    fb.fin = s->d_fin;
+    fb.nfin = s->npartial > kFoldMaxGroups ? 1 : 0;   // else folded into the face kernel
     if (rson) {
         e = launch_restraint_prepass(fb, st);
         if (e != hipSuccess) return fail(s, UMCP_ERR_DEVICE, std::string("restraint kernel launch: ") + hipGetErrorString(e));

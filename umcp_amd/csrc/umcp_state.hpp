@@ -15,6 +15,7 @@ namespace umcp {
 
 struct ClassDev {
     int *cp = nullptr, *pos = nullptr;
+    uint16_t *loc = nullptr;
     uint8_t *img = nullptr, *has_img = nullptr;
     double *g0 = nullptr, *rurv0 = nullptr, *ig0 = nullptr, *Lo = nullptr, *Li = nullptr, *c0 = nullptr;
 };
@@ -57,6 +58,7 @@ struct umcp_surface {
     uint16_t *d_linc = nullptr;
     int *d_lvstart = nullptr;
     int *d_lvrec = nullptr;
+    int *d_lvert = nullptr;               // [records] vertex of each record
     int npartial = 0;
     // launch shape of the face kernel: waves sharing 64 faces' Gauss points,
     // and whether the faces' control points are staged in LDS
@@ -82,6 +84,7 @@ struct umcp_surface {
     double A0_setg0 = 0.0, V_setg0 = 0.0;
     double *d_rs = nullptr;               // [8] dE/dA, dE/dV, E, A, V
     double *d_rpart = nullptr;            // [npartial][2] per-group area and volume
+    int *d_fin = nullptr;                 // [1] = 0: the restraint finalize's item (the slot-0 item)
     umcp::FaceArgs *d_fa_aux = nullptr;   // a second face table (volume-only passes)
     // Brownian step counter of the device-noise path
     int *d_step = nullptr;
