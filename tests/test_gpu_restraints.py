@@ -7,7 +7,10 @@ trajectories with restraints, one host noise array, <= 1e-9 A after 10 steps;
 the device noise equals the CPU restatement of Philox4x64-10 + Box-Muller to
 1e-13 (libm-level differences only); device-noise steps equal explicit-noise
 steps fed the same deviates bitwise; FP32-mixed with restraints on the
-40962-vertex vesicle within 1e-6 (energy) and 1e-5 (gradient RMS).
+40962-vertex vesicle within 1e-6 (energy) and 1e-5 (gradient RMS); runs of
+several Brownian steps bitwise equal to one-step calls, and on the
+40962-vertex vesicle (restraint finalize as a launch) within 1e-9 A of the
+oracle.
 """
 import sys
 
@@ -195,3 +198,52 @@ def test_mixed_with_restraints_40962():
     say("   energy rel", rel_e, "grad rms rel", rel_g)
     assert rel_e <= 1e-6
     assert rel_g <= 1e-5
+
+
+@pytest.mark.skipif(torch is None, reason="torch provides the device buffers")
+@pytest.mark.parametrize("mesh", ["vesicle_642", "vesicle_40962"])
+def test_step_runs_equal_single_steps(mesh):
+    """K Brownian steps in one call and K calls of one step give the same
+    bits (642 vertices: restraint finalize folded into the face blocks;
+    40962: a finalize launch)."""
+    w = workloads.vesicle_642() if mesh == "vesicle_642" else workloads.vesicle_40962()
+    o0 = O.OracleSurface(w.mesh_path)
+    o0.set_params(**w.params)
+    rs = away_targets(o0, w.r, 0.3, 0.02)
+    c0 = workloads.face_c0_uniform(81920, w.extra["c0_seed"]) if mesh == "vesicle_40962" else None
+    a, _ = make_pair(w, c0=c0, restraints=rs)
+    b, _ = make_pair(w, c0=c0, restraints=rs)
+    mob, amp = S.bd_coefficients(1e-12, 1e-9, 298.0)
+    K = 5
+    say("[steps] %s: %d steps in one call against one-step calls" % (mesh, K))
+    xi = torch.tensor(np.random.default_rng(9).standard_normal((K, 3 * a.nv)), device="cuda", dtype=torch.float64)
+    d_ra = torch.tensor(w.r, device="cuda", dtype=torch.float64)
+    d_rb = d_ra.clone()
+    e_a = torch.zeros(K, device="cuda", dtype=torch.float64)
+    e_b = torch.zeros(K, device="cuda", dtype=torch.float64)
+    a.bd_steps_dev(d_ra, xi, K, mob, amp, e_a)
+    for k in range(K):
+        b.bd_steps_dev(d_rb, xi[k:k + 1], 1, mob, amp, e_b[k:k + 1])
+    torch.cuda.synchronize()
+    say("   max |dr|", float((d_ra - d_rb).abs().max()))
+    assert torch.equal(d_ra, d_rb)
+    assert torch.equal(e_a, e_b)
+
+
+def test_bd_with_restraints_40962():
+    """Three Brownian steps of the 40962-vertex vesicle (per-face c0, both
+    restraints, targets away from the start) against the oracle."""
+    w = workloads.vesicle_40962()
+    c0 = workloads.face_c0_uniform(81920, w.extra["c0_seed"])
+    o0 = O.OracleSurface(w.mesh_path)
+    o0.set_params(**w.params)
+    g, o = make_pair(w, c0=c0, restraints=away_targets(o0, w.r, 0.3, 0.02))
+    mob, amp = S.bd_coefficients(1e-12, 1e-9, 298.0)
+    xi = np.random.default_rng(6).standard_normal((3, 3 * g.nv))
+    say("[restraints] bd 40962 3 steps")
+    r_g, e_g = g.bd_steps(w.r, xi, mob, amp, want_energies=True)
+    r_o, e_o = o.bd_steps(w.r, xi, mob, amp, want_energies=True)
+    err = np.abs(r_g - r_o).max()
+    say("   max |dr|", err, "energies", e_g, e_o)
+    assert err <= 1e-9
+    assert np.abs(e_g / e_o - 1).max() <= 1e-11

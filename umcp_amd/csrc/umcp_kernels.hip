@@ -199,15 +199,9 @@ __device__ __forceinline__ void gather_reduce(const GatherArgs &a)
 // before the first add (clamped to the range; the adds skip what lies beyond
 // it), in the same order.
 template <typename GB>
-__device__ __forceinline__ void gather_vertices(const GatherArgs &a, int vblock)
+__device__ __forceinline__ void vertex_sum(const GatherArgs &a, int v, int q, bool mine, double g[3])
 {
-    // every operation as written (no FMA contraction): the single-surface and
-    // batched kernels that inline this must round identically
     #pragma clang fp contract(off)
-    if (a.out_mode == 2) return;
-    const int q = threadIdx.x & (kGatherLanes - 1);
-    const int v = vblock * kGatherVerts + (int)threadIdx.x / kGatherLanes;
-    const bool mine = v < a.nv;
     double g0 = 0.0, g1 = 0.0, g2 = 0.0;
     if (mine) {
         const int s0 = a.vstart[v], s1 = a.vstart[v + 1];
@@ -240,6 +234,43 @@ __device__ __forceinline__ void gather_vertices(const GatherArgs &a, int vblock)
         g1 += __shfl_xor(g1, off, kWave);
         g2 += __shfl_xor(g2, off, kWave);
     }
+    g[0] = g0;
+    g[1] = g1;
+    g[2] = g2;
+}
+
+// The Brownian update of one vertex (out_mode 1): r + (-(D dt/kT) g + sqrt(2
+// D dt) xi), xi from the noise array or the device stream of (seed, step, v)
+__device__ __forceinline__ void bd_position(const GatherArgs &a, int v, const double r[3], const double g[3],
+                                            double out[3])
+{
+    #pragma clang fp contract(off)
+    double z[3];
+    if (a.noise == kNoisePhilox) {
+        normal3(a.seed, philox_step(a), v, z);
+    } else {
+        const double *xi = a.xi + 3 * (size_t)v;
+        z[0] = xi[0];
+        z[1] = xi[1];
+        z[2] = xi[2];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) out[k] = r[k] + (-a.mobility_dt * g[k] + a.noise_amp * z[k]);
+}
+
+template <typename GB>
+__device__ __forceinline__ void gather_vertices(const GatherArgs &a, int vblock)
+{
+    // every operation as written (no FMA contraction): the single-surface and
+    // batched kernels that inline this must round identically
+    #pragma clang fp contract(off)
+    if (a.out_mode == 2) return;
+    const int q = threadIdx.x & (kGatherLanes - 1);
+    const int v = vblock * kGatherVerts + (int)threadIdx.x / kGatherLanes;
+    const bool mine = v < a.nv;
+    double gsum[3];
+    vertex_sum<GB>(a, v, q, mine, gsum);
+    const double g0 = gsum[0], g1 = gsum[1], g2 = gsum[2];
     if (q != 0) return;
     if (!mine) return;
     if (a.out_mode == 0) {
@@ -276,19 +307,13 @@ __device__ __forceinline__ void gather_vertices(const GatherArgs &a, int vblock)
             a.pp_out[3 * (size_t)v + c] = next;
         }
     } else {
-        double z[3];
-        if (a.noise == kNoisePhilox) {
-            normal3(a.seed, philox_step(a), v, z);
-        } else {
-            const double *xi = a.xi + 3 * (size_t)v;
-            z[0] = xi[0];
-            z[1] = xi[1];
-            z[2] = xi[2];
-        }
         double *r = a.r + 3 * (size_t)v;
-        r[0] += -a.mobility_dt * g0 + a.noise_amp * z[0];
-        r[1] += -a.mobility_dt * g1 + a.noise_amp * z[1];
-        r[2] += -a.mobility_dt * g2 + a.noise_amp * z[2];
+        const double ro[3] = {r[0], r[1], r[2]};
+        double rn[3];
+        bd_position(a, v, ro, gsum, rn);
+        r[0] = rn[0];
+        r[1] = rn[1];
+        r[2] = rn[2];
     }
 }
 
@@ -550,18 +575,13 @@ __global__ __launch_bounds__(kGatherBlock) void vertex_gather_batch(BatchGatherA
 // contraction choices cannot differ between them.)
 // FP64 throughout, for the mixed path too: the multipliers depend on
 // A_tot - A0 and V - V0, small differences of the sums.
+// The area and volume terms of one face (lane) from its control points x:
+// wave w of WS takes Gauss points w, w + WS, ..., the last wave the volume.
 template <int NCP, int NPT, int KIND, int WS>
-__device__ __forceinline__ void restraint_group(const FaceArgs &a, const double *__restrict__ r, int group, int slot,
-                                                double *sV, double acc[4])
+__device__ __forceinline__ void restraint_points(const FaceArgs &a, const double al[3], const double x[NCP][3],
+                                                 bool live, double acc[4])
 {
-    const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int f = group * kWave + lane;
-    const bool live = f < a.nf;
-    const int fc = live ? f : a.nf - 1;
-    const double al[3] = {r[3 * a.nv], r[3 * a.nv + 1], r[3 * a.nv + 2]};
-    double x[NCP][3];
-    stage_cps<NCP, 1, WS * kWave>(a, r, slot, fc, 0, sV, x);
     double A = 0.0, V = 0.0;
     if (a.do_area) {
         const double *tab = kind_table<KIND, double>();
@@ -604,6 +624,20 @@ __device__ __forceinline__ void restraint_group(const FaceArgs &a, const double 
         acc[0] += A;
         acc[1] += V;
     }
+}
+
+template <int NCP, int NPT, int KIND, int WS>
+__device__ __forceinline__ void restraint_group(const FaceArgs &a, const double *__restrict__ r, int group, int slot,
+                                                double *sV, double acc[4])
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int f = group * kWave + lane;
+    const bool live = f < a.nf;
+    const int fc = live ? f : a.nf - 1;
+    const double al[3] = {r[3 * a.nv], r[3 * a.nv + 1], r[3 * a.nv + 2]};
+    double x[NCP][3];
+    stage_cps<NCP, 1, WS * kWave>(a, r, slot, fc, 0, sV, x);
+    restraint_points<NCP, NPT, KIND, WS>(a, al, x, live, acc);
 }
 
 constexpr int kPrepassWaves = 4;   // waves per 64-face group (Gauss points split)
