@@ -155,11 +155,19 @@ int umcp_bd_steps(umcp_surface *s, double *r, const double *xi, int nsteps,
 
 /* Device-resident variants: all pointers are device pointers (HBM); stream
  * is a hipStream_t (NULL = the surface's own stream).  They enqueue work and
- * return without synchronising. */
+ * return without synchronising, so they cannot return UMCP_ERR_NAN_ENERGY
+ * themselves: every evaluation whose energy is not finite (the reference's
+ * "nan en." exit, src/interpLib.C:3546-3550) sets a flag on the surface, and
+ * umcp_nan_check reports it. */
 int umcp_energy_grad_dev(umcp_surface *s, const double *d_r, double *d_E, double *d_g,
                          void *stream);
 int umcp_bd_steps_dev(umcp_surface *s, double *d_r, const double *d_xi, int nsteps,
                       const umcp_bd_params *bd, double *d_energies, void *stream);
+/* Waits for the device, then returns UMCP_ERR_NAN_ENERGY if any evaluation
+ * of s since the last report (by this call or by a host entry point that
+ * returned UMCP_ERR_NAN_ENERGY) had a non-finite energy, and clears the
+ * report; UMCP_OK otherwise.  No counterpart in the reference, which exits. */
+int umcp_nan_check(umcp_surface *s);
 
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under stream capture, serving loops that forbid allocation):
@@ -390,6 +398,10 @@ int umcp_ensemble_get_launch(const umcp_ensemble *e, int *point_split, int *flag
  * umcp_bd_steps_rng_dev); graph_steps > 0 replays a captured graph of that
  * many steps; stream NULL = the ensemble's own stream; asynchronous */
 int umcp_ensemble_bd_steps(umcp_ensemble *e, int nsteps, int graph_steps, void *stream);
+/* umcp_nan_check over the members: waits for the device; if some member had
+ * a non-finite energy, sets *member (may be NULL) to the first such member's
+ * index, clears every member's report and returns UMCP_ERR_NAN_ENERGY */
+int umcp_ensemble_nan_check(umcp_ensemble *e, int *member);
 /* work of the current tables (after the first umcp_ensemble_bd_steps): face
  * items (64-face groups), gather items (256-vertex blocks), members with
  * restraints */

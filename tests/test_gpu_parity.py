@@ -250,6 +250,60 @@ def test_bd_nan_energy_is_an_error(ves642):
         g.bd_steps(bad, xi, mob, amp, want_energies=True)
     r2, e2 = g.bd_steps(ves642.r, xi, mob, amp, want_energies=True)
     assert np.array_equal(r2, r1) and np.array_equal(e2, e1)
+    say("[parity] bd without energies, nan position")
+    with pytest.raises(S.UmcpError, match=r"\(-2\).*nan energy"):
+        g.bd_steps(bad, xi, mob, amp, want_energies=False)
+    r3 = g.bd_steps(ves642.r, xi, mob, amp, want_energies=False)
+    assert np.array_equal(r3, r1)
+
+
+def test_device_paths_flag_nan_energy(ves642):
+    """The device entry points return before the work runs; a non-finite
+    energy on any of them is reported by nan_check (then cleared), for a
+    surface and for the members of an ensemble."""
+    if torch is None:
+        pytest.skip("torch is needed for device buffers")
+    g, o = make_pair(ves642)
+    mob, amp = S.bd_coefficients(1e-12, 1e-9, 298.0)
+    bad = ves642.r.copy()
+    bad[3 * 41] = np.nan
+    d_bad = torch.tensor(bad, device="cuda", dtype=torch.float64)
+    d_ok = torch.tensor(ves642.r, device="cuda", dtype=torch.float64)
+    d_E = torch.zeros(1, device="cuda", dtype=torch.float64)
+    d_g = torch.zeros_like(d_ok)
+    xi = torch.zeros((2, 3 * g.nv), device="cuda", dtype=torch.float64)
+    g.nan_check()
+    say("[parity] device energy_grad with a nan position")
+    g.energy_grad_dev(d_bad, d_E, d_g)
+    with pytest.raises(S.UmcpError, match=r"\(-2\).*nan energy"):
+        g.nan_check()
+    g.nan_check()                        # cleared
+    g.energy_grad_dev(d_ok, d_E, d_g)
+    g.nan_check()                        # finite
+    for path in ("bd_steps_dev", "bd_steps_rng_dev", "noalloc"):
+        say("[parity] device path", path)
+        d_r = d_bad.clone()
+        if path == "bd_steps_dev":
+            g.bd_steps_dev(d_r, xi, 2, mob, amp, None)
+        elif path == "bd_steps_rng_dev":
+            g.set_step(0)
+            g.bd_steps_rng_dev(d_r, 2, mob, amp, 5, None)
+        else:
+            g.energy_grad_dev_noalloc(d_r, d_E, d_g)
+        with pytest.raises(S.UmcpError, match=r"\(-2\)"):
+            g.nan_check()
+    say("[parity] ensemble member with a nan position")
+    ens = S.Ensemble()
+    h = make_pair(ves642)[0]
+    h.set_step(0)
+    g.set_step(0)
+    d_a, d_b = d_ok.clone(), d_bad.clone()
+    ens.add(g, d_a, mob, amp, 1, None)
+    ens.add(h, d_b, mob, amp, 2, None)
+    ens.bd_steps(2, 0)
+    assert ens.nan_check() == 1
+    assert ens.nan_check() is None
+    ens.close()
 
 
 def test_device_entry_points(ves642):

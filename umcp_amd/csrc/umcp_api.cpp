@@ -182,7 +182,7 @@ size_t plan_bytes(const umcp_surface *s)
     b += sz(4 * (size_t)std::max(ngroups, metric_total), 8) + sz(8, 8) + sz(s->P.vstart.size(), 4);
     // group records: per-group bases, incidence ranges and lists, vertex CSR
     b += sz(ngroups + 1, 4) + sz(nloc + 1, 4) + sz(s->P.nnz, 2) + sz(nv + 1, 4) + 2 * sz(nloc, 4);
-    b += sz(8, 8) + sz(2 * (size_t)ngroups, 8) + 2 * sz(1, 4);   // rs, rpart, step, fin
+    b += sz(8, 8) + sz(2 * (size_t)ngroups, 8) + 3 * sz(1, 4);   // rs, rpart, step, fin, nan
     b += 2 * sz(s->P.classes.size(), sizeof(FaceArgs)) + sz(ngroups, sizeof(FaceItem));
     return b;
 }
@@ -295,6 +295,7 @@ int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs 
     ga.nv = s->T.nv;
     ga.vstart = s->d_lvstart;
     ga.vrec = s->d_lvrec;
+    ga.nan_flag = s->d_nan;
     ga.gbuf = s->d_gbuf;
     ga.gbuf32 = mixed ? s->d_gbuf32 : nullptr;
     ga.partial = s->d_partial;
@@ -309,6 +310,26 @@ int enqueue_eval(umcp_surface *s, const double *d_r, hipStream_t st, GatherArgs 
     if (e != hipSuccess) return hip_fail(s, e, "gather kernel launch");
     if (timed) (void)hipEventRecord(evg[1], st);
     return UMCP_OK;
+}
+
+// UMCP_ERR_NAN_ENERGY if an evaluation since the last report had a
+// non-finite energy (the flag is then cleared).  device_sync: wait for the
+// whole device (device entry points may run on the caller's streams), else
+// for the surface's stream.
+int nan_status(umcp_surface *s, bool device_sync)
+{
+    hipError_t e = device_sync ? hipDeviceSynchronize() : hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return hip_fail(s, e, "synchronize");
+    int flag = 0;
+    e = hipMemcpyAsync(&flag, s->d_nan, sizeof flag, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return hip_fail(s, e, "download nan flag");
+    if (!flag) return UMCP_OK;
+    const int zero = 0;
+    e = hipMemcpyAsync(s->d_nan, &zero, sizeof zero, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return hip_fail(s, e, "clear nan flag");
+    return fail(s, UMCP_ERR_NAN_ENERGY, "nan energy");
 }
 
 int check_ready(umcp_surface *s, bool need_g0 = true)
@@ -556,7 +577,7 @@ int umcp_plan(umcp_surface *s)
     for (const FaceClass &C : s->P.classes)
         for (uint8_t h : C.has_img) s->has_images = s->has_images || h;
     if ((rc = dev_alloc(s, &s->d_rs, 8)) || (rc = dev_alloc(s, &s->d_rpart, 2 * (size_t)s->npartial)) ||
-        (rc = dev_alloc(s, &s->d_step, 1)) || (rc = dev_alloc(s, &s->d_fin, 1)))
+        (rc = dev_alloc(s, &s->d_step, 1)) || (rc = dev_alloc(s, &s->d_fin, 1)) || (rc = dev_alloc(s, &s->d_nan, 1)))
         return rc;
     // face work tables
     std::vector<FaceItem> items;
@@ -574,6 +595,7 @@ int umcp_plan(umcp_surface *s)
     const int zero_step = 0;
     const double zeros8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if ((rc = dev_upload(s, s->d_step, &zero_step, 1)) || (rc = dev_upload(s, s->d_fin, &zero_step, 1)) ||
+        (rc = dev_upload(s, s->d_nan, &zero_step, 1)) ||
         (rc = dev_upload(s, s->d_rs, zeros8, 8)))
         return rc;
     if ((rc = sync(s))) return rc;
@@ -685,7 +707,7 @@ static int eval_host(umcp_surface *s, const double *r, double *E, double *g, boo
         if (e != hipSuccess) return hip_fail(s, e, "download g");
     }
     if ((rc = sync(s))) return rc;
-    if (std::isnan(Eh)) return fail(s, UMCP_ERR_NAN_ENERGY, "nan energy");
+    if ((rc = nan_status(s, false))) return rc;
     if (E) *E = Eh;
     if (g) {
         if (accumulate)
@@ -1277,13 +1299,18 @@ int umcp_bd_steps(umcp_surface *s, double *r, const double *xi, int nsteps, cons
             if (e != hipSuccess) return hip_fail(s, e, "download energies");
         }
         if ((rc = sync(s))) return rc;  // the staging buffer is reused by the next chunk
-        if (energies)
-            for (int k = 0; k < nk; k++)
-                if (std::isnan(energies[k0 + k])) return fail(s, UMCP_ERR_NAN_ENERGY, "nan energy");
+        if ((rc = nan_status(s, false))) return rc;
     }
     hipError_t e = hipMemcpyAsync(r, s->d_r, n3 * sizeof(double), hipMemcpyDeviceToHost, s->stream);
     if (e != hipSuccess) return hip_fail(s, e, "download r");
     return sync(s);
+}
+
+int umcp_nan_check(umcp_surface *s)
+{
+    int rc = check_ready(s, false);
+    if (rc) return rc;
+    return nan_status(s, true);
 }
 
 int umcp_set_launch(umcp_surface *s, int point_split, int flags)

@@ -169,6 +169,7 @@ int build(umcp_ensemble *e)
         g.nv = s->T.nv;
         g.vstart = s->d_lvstart;
         g.vrec = s->d_lvrec;
+        g.nan_flag = s->d_nan;
         g.gbuf = s->d_gbuf;
         g.gbuf32 = e->mixed ? s->d_gbuf32 : nullptr;
         g.partial = s->d_partial;
@@ -385,6 +386,29 @@ int umcp_ensemble_bd_steps(umcp_ensemble *e, int nsteps, int graph_steps, void *
         if (err != hipSuccess) return ehip(e, err, "batched launch");
     }
     return UMCP_OK;
+}
+
+int umcp_ensemble_nan_check(umcp_ensemble *e, int *member)
+{
+    if (!e) return UMCP_ERR_ARG;
+    if (member) *member = -1;
+    hipError_t err = hipDeviceSynchronize();
+    if (err != hipSuccess) return ehip(e, err, "hipDeviceSynchronize");
+    int first = -1;
+    for (size_t i = 0; i < e->members.size(); i++) {
+        umcp_surface *s = e->members[i].s;
+        int flag = 0;
+        err = hipMemcpy(&flag, s->d_nan, sizeof flag, hipMemcpyDeviceToHost);
+        if (err != hipSuccess) return ehip(e, err, "download nan flag");
+        if (!flag) continue;
+        if (first < 0) first = (int)i;
+        const int zero = 0;
+        err = hipMemcpy(s->d_nan, &zero, sizeof zero, hipMemcpyHostToDevice);
+        if (err != hipSuccess) return ehip(e, err, "clear nan flag");
+    }
+    if (first < 0) return UMCP_OK;
+    if (member) *member = first;
+    return efail(e, UMCP_ERR_NAN_ENERGY, "nan energy in member " + std::to_string(first));
 }
 
 int umcp_ensemble_work(const umcp_ensemble *e, int *face_items, int *gather_items, int *restrained)

@@ -170,6 +170,9 @@ __device__ __forceinline__ void gather_reduce(const GatherArgs &a)
     block_sum4<kGatherBlock>(acc);
     if (threadIdx.x == 0) {
         if (a.rs) acc[0] += a.rs[2];
+        // the reference stops on a NaN energy (src/interpLib.C:3546-3550):
+        // flag it for the host, whatever path the evaluation takes
+        if (a.nan_flag && !isfinite(acc[0])) a.nan_flag[0] = 1;
         if (a.E_out) a.E_out[0] = acc[0];
         if (a.energies) {
             // Brownian steps with device noise index their energies by the

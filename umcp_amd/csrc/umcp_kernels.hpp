@@ -112,6 +112,8 @@ struct GatherArgs {
     double *sums;           // may be null: the four reduced partial sums
     // restraint scalars of the surface (rs[2] = restraint energy) or null
     const double *rs;
+    // set to 1 when the evaluation's energy is not finite (may be null)
+    int *nan_flag;
     // noise source of out_mode 1
     int noise;              // kNoiseArray (xi) or kNoisePhilox
     unsigned long long seed;

@@ -135,6 +135,7 @@ extern "C" int umcp_energy_grad_dev_noalloc(umcp_surface *s, const double *d_r, 
     ga.nv = s->T.nv;
     ga.vstart = s->d_lvstart;
     ga.vrec = s->d_lvrec;
+    ga.nan_flag = s->d_nan;
     ga.gbuf = mixed ? nullptr : (const double *)(base + o_gbuf);
     ga.gbuf32 = mixed ? (const float *)(base + o_gbuf) : nullptr;
     ga.partial = partial;

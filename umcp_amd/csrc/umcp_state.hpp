@@ -85,6 +85,7 @@ struct umcp_surface {
     double *d_rs = nullptr;               // [8] dE/dA, dE/dV, E, A, V
     double *d_rpart = nullptr;            // [npartial][2] per-group area and volume
     int *d_fin = nullptr;                 // [1] = 0: the restraint finalize's item (the slot-0 item)
+    int *d_nan = nullptr;                 // [1] set by any evaluation whose energy is not finite
     umcp::FaceArgs *d_fa_aux = nullptr;   // a second face table (volume-only passes)
     // Brownian step counter of the device-noise path
     int *d_step = nullptr;

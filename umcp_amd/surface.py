@@ -42,6 +42,7 @@ EXPORTS = [
     "umcp_noise", "umcp_ensemble_create", "umcp_ensemble_destroy", "umcp_ensemble_last_error",
     "umcp_ensemble_add", "umcp_ensemble_remove", "umcp_ensemble_size", "umcp_ensemble_refresh",
     "umcp_ensemble_set_launch", "umcp_ensemble_get_launch", "umcp_ensemble_bd_steps", "umcp_ensemble_work",
+    "umcp_nan_check", "umcp_ensemble_nan_check",
     "umcp_clone", "umcp_device_bytes", "umcp_server_create", "umcp_server_submit", "umcp_server_wait",
     "umcp_server_drain", "umcp_server_stats", "umcp_server_last_error", "umcp_server_destroy",
     "umcp_effm_build", "umcp_effm_get", "umcp_effm_steps", "umcp_effm_steps_dev",
@@ -173,6 +174,8 @@ def load_library(path: str = LIB_PATH):
         "umcp_ensemble_get_launch": ([vp, ip, ip, ip, ip], C.c_int),
         "umcp_ensemble_bd_steps": ([vp, C.c_int, C.c_int, vp], C.c_int),
         "umcp_ensemble_work": ([vp, ip, ip, ip], C.c_int),
+        "umcp_nan_check": ([vp], C.c_int),
+        "umcp_ensemble_nan_check": ([vp, ip], C.c_int),
         "umcp_clone": ([vp, C.POINTER(vp)], C.c_int),
         "umcp_device_bytes": ([vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], C.c_int),
         "umcp_server_create": ([C.POINTER(ServerConfig), C.POINTER(vp)], C.c_int),
@@ -317,6 +320,11 @@ class Surface:
                                           C.c_void_p(d_g.data_ptr()) if d_g is not None else None,
                                           C.c_void_p(stream) if stream else None)
         self._check(rc, "energy_grad_dev")
+
+    def nan_check(self):
+        """Raises UmcpError (UMCP_ERR_NAN_ENERGY) if an evaluation since the last
+        report had a non-finite energy (device entry points flag it)."""
+        self._check(self._L.umcp_nan_check(self._h), "nan_check")
 
     def energy_grad_dev_noalloc(self, d_r, d_E, d_g, stream=None):
         """energy_grad_dev with the library's static scratch array as work space
@@ -690,6 +698,16 @@ class Ensemble:
         v = [C.c_int() for _ in range(3)]
         self._check(self._L.umcp_ensemble_work(self._h, *[C.byref(x) for x in v]), "ensemble_work")
         return tuple(x.value for x in v)
+
+    def nan_check(self):
+        """None if no member had a non-finite energy since the last check, else
+        the first such member's index (every member's report is cleared)."""
+        m = C.c_int(-1)
+        rc = self._L.umcp_ensemble_nan_check(self._h, C.byref(m))
+        if rc == -2:
+            return m.value
+        self._check(rc, "ensemble_nan_check")
+        return None
 
 
 class Server:
