@@ -93,6 +93,26 @@ int umcp_get(const umcp_surface *s, double *r);
 /* parameters; see umcp_params */
 int umcp_set_params(umcp_surface *s, const umcp_params *p);
 
+/* surface::readLipidComposition(FILE *) (src/interp.h:536,
+ * src/lipid_composition.C:47-333): the reference's lipid library extended by
+ * the file's "lipid library NAME APL c0" lines and the composition of its
+ * "lipid inner NAME parts" / "lipid outer NAME parts" lines; the next
+ * umcp_setg0 packs each triangle's leaflets with the reference's rand()
+ * draws (src/interpLib.C:3127-3196, 3286-3358) from a glibc-compatible
+ * stream seeded with seed (the reference's srand(random_seed), hd.C:118).
+ * The lipid-mode energy then sums over the types (src/interpLib.C:3818-3838).
+ * Without a composition: one type of umcp_params' lipid_c0 and lipid_apl.
+ * UMCP_ERR_ARG for an unreadable file, a malformed directive or a lipid not
+ * in the library (the reference exits or reads out of bounds). */
+int umcp_read_lipid_composition(umcp_surface *s, const char *path, unsigned int seed);
+/* the same composition from arrays: ntypes (1..64) types with spontaneous
+ * curvature c0 (1/A), area per lipid apl (A^2) and input parts per leaflet
+ * (inner_parts, outer_parts >= 0, each summing above 0) */
+int umcp_set_lipids(umcp_surface *s, int ntypes, const double *c0, const double *apl,
+                    const double *inner_parts, const double *outer_parts, unsigned int seed);
+/* number of lipid types in force (1 without a composition) */
+int umcp_lipid_type_count(const umcp_surface *s);
+
 /* host half of surface::generatePlan (src/interp.h:337): face ownership,
  * control-point lists, periodic image codes and the shared Gauss-point
  * tables; needs no GPU (introspection works after it) */
@@ -495,6 +515,10 @@ int umcp_class_info(const umcp_surface *s, int cls, int *valence, int *ncp, int 
 int umcp_class_table(const umcp_surface *s, int cls, double *table, double *weights);
 int umcp_face_points(const umcp_surface *s, int face, int *ncp, int *cp);
 int umcp_face_lipids(const umcp_surface *s, int face, double *Lo, double *Li);
+/* each lipid type's area in the leaflets of face's triangle after setg0
+ * (theTriangles[t].composition.outerLeaflet / innerLeaflet): Lo, Li hold
+ * umcp_lipid_type_count values */
+int umcp_face_lipid_types(const umcp_surface *s, int face, double *Lo, double *Li);
 
 #ifdef __cplusplus
 }

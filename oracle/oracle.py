@@ -63,6 +63,10 @@ def lib():
         L.orc_face_info.argtypes = [vp, C.c_int, ip, ip, ip, ip]
         L.orc_face_table.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp, dp]
         L.orc_tri_lipids.argtypes = [vp, C.c_int, dp, dp]
+        L.orc_tri_lipid_types.argtypes = [vp, C.c_int, dp, dp]
+        L.orc_set_lipids.argtypes = [vp, C.c_int, dp, dp, dp, dp, C.c_uint]
+        L.orc_grand_seed.argtypes = [vp, C.c_uint]
+        L.orc_grand_next.argtypes = [vp]
         L.orc_ntri.argtypes = [vp]
         L.orc_tri_order_face.argtypes = [vp, C.c_int]
         L.orc_neighbors.argtypes = [vp, C.c_int, ip, ip]
@@ -411,6 +415,21 @@ class OracleSurface:
         lib().orc_tri_lipids(self._h, t, C.byref(lo), C.byref(li))
         return lo.value, li.value
 
+    def set_lipids(self, c0, apl, inner, outer, seed: int = 1):
+        """A lipid composition for the next setg0 (types' c0, area per lipid,
+        input parts per leaflet; packing with rand() after srand(seed))."""
+        arr = [np.ascontiguousarray(x, dtype=np.float64) for x in (c0, apl, inner, outer)]
+        self._ntypes = len(arr[0])
+        rc = lib().orc_set_lipids(self._h, self._ntypes, *[_p(x) for x in arr], int(seed))
+        if rc != 0:
+            raise ValueError("bad lipid composition")
+
+    def tri_lipid_types(self, t: int):
+        n = getattr(self, "_ntypes", 1)
+        lo, li = np.zeros(n), np.zeros(n)
+        lib().orc_tri_lipid_types(self._h, t, _p(lo), _p(li))
+        return lo, li
+
     def ntri(self) -> int:
         return lib().orc_ntri(self._h)
 
@@ -448,3 +467,11 @@ def irregular_table(val: int, u: float, v: float) -> np.ndarray:
     if rc:
         raise ValueError("bad valence")
     return out.reshape(6, val + 6)
+
+
+def glibc_rand(seed: int, n: int) -> np.ndarray:
+    """The oracle's restatement of glibc rand(): n outputs after srand(seed)."""
+    st = (C.c_int32 * 33)()
+    lib().orc_grand_seed(st, int(seed))
+    return np.array([lib().orc_grand_next(st) for _ in range(n)], dtype=np.int64)
+

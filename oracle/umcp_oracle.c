@@ -808,7 +808,15 @@ struct omesh {
     int nt;
     int *tri;         /* 3 nt, ascending ids */
     int *tri_face;    /* face index of triangle t (theTriangles[t].f) */
-    double *Lo, *Li;  /* lipid areas per triangle (single lipid type) */
+    /* lipid composition (src/lipid_composition.C:47-333): nlt types with
+     * spontaneous curvature lt_c0, area per lipid lt_apl and input parts per
+     * leaflet; Lo/Li the area of each type in each triangle's outer/inner
+     * leaflet, [nt][nlt]; with no composition set, one type of
+     * (prm.lipid_c0, prm.lipid_apl) */
+    int nlt;
+    double *lt_c0, *lt_apl, *lt_in, *lt_out;
+    unsigned int lt_seed;
+    double *Lo, *Li;
     /* faces */
     int planned;
     int nfr, nfi;
@@ -1034,6 +1042,9 @@ int orc_load(const char *path, omesh **out)
     set_edge_rev(m);
     m->tri_face = malloc(sizeof(int) * (size_t)m->nt);
     for (int t = 0; t < m->nt; t++) m->tri_face[t] = -1;
+    m->nlt = 0;
+    m->lt_c0 = m->lt_apl = m->lt_in = m->lt_out = NULL;
+    m->lt_seed = 1;
     m->Lo = calloc((size_t)m->nt, sizeof(double));
     m->Li = calloc((size_t)m->nt, sizeof(double));
     m->prm.kc = 14.0; m->prm.kg = 0.0; m->prm.KA = 0.0; m->prm.k_reg = 0.0;
@@ -1047,6 +1058,7 @@ void orc_free(omesh *m)
     if (!m) return;
     free(m->r); free(m->val); free(m->nb); free(m->rev); free(m->tri); free(m->tri_face);
     free(m->Lo); free(m->Li); free(m->face_owner); free(m->face_edge); free(m->F);
+    free(m->lt_c0); free(m->lt_apl); free(m->lt_in); free(m->lt_out);
     free(m->face_c0);
     free(m);
 }
@@ -1296,10 +1308,98 @@ int orc_face_table(const omesh *m, int f, int p, double *w6, double *weight, dou
     return ncp;
 }
 
+/* total lipid area of each leaflet of triangle t (over the types) */
 void orc_tri_lipids(const omesh *m, int t, double *Lo, double *Li)
 {
-    *Lo = m->Lo[t];
-    *Li = m->Li[t];
+    const int n = m->nlt > 0 ? m->nlt : 1;
+    double so = 0.0, si = 0.0;
+    for (int x = 0; x < n; x++) {
+        so += m->Lo[(size_t)t * n + x];
+        si += m->Li[(size_t)t * n + x];
+    }
+    *Lo = so;
+    *Li = si;
+}
+
+/* the area of each lipid type in triangle t's leaflets */
+void orc_tri_lipid_types(const omesh *m, int t, double *Lo, double *Li)
+{
+    const int n = m->nlt > 0 ? m->nlt : 1;
+    memcpy(Lo, m->Lo + (size_t)t * n, sizeof(double) * (size_t)n);
+    memcpy(Li, m->Li + (size_t)t * n, sizeof(double) * (size_t)n);
+}
+
+/* glibc's rand() (the TYPE_3 additive feedback generator of random_r:
+ * r[i] = r[i-3] + r[i-31] mod 2^32, output r >> 1; srand fills r[0..30]
+ * with 16807 r mod (2^31 - 1) from the seed and discards 310 outputs),
+ * restated for the reference's rand()-driven lipid packing */
+void orc_grand_seed(orc_grand *g, unsigned int seed)
+{
+    int32_t *r = g->r;
+    if (seed == 0) seed = 1;
+    int32_t word = (int32_t)seed;   /* as glibc: seeds of 2^31 and above start negative */
+    r[0] = word;
+    for (int i = 1; i < 31; i++) {
+        const int64_t hi = word / 127773, lo = word % 127773;
+        word = (int32_t)(16807 * lo - 2836 * hi);
+        if (word < 0) word += 2147483647;
+        r[i] = word;
+    }
+    g->f = 3;
+    g->b = 0;
+    for (int i = 0; i < 310; i++) (void)orc_grand_next(g);
+}
+
+int orc_grand_next(orc_grand *g)
+{
+    const uint32_t val = (uint32_t)g->r[g->f] + (uint32_t)g->r[g->b];
+    g->r[g->f] = (int32_t)val;
+    g->f = (g->f + 1) % 31;
+    g->b = (g->b + 1) % 31;
+    return (int)(val >> 1);
+}
+
+/* a lipid composition (in place of the single type of the parameters) for
+ * the next setg0, whose packing draws from rand() after srand(seed) */
+int orc_set_lipids(omesh *m, int n, const double *c0, const double *apl, const double *in_parts,
+                   const double *out_parts, unsigned int seed)
+{
+    if (n < 1) return -1;
+    double si = 0.0, so = 0.0;
+    for (int x = 0; x < n; x++) {
+        if (!(apl[x] > 0.0) || in_parts[x] < 0.0 || out_parts[x] < 0.0) return -1;
+        si += in_parts[x];
+        so += out_parts[x];
+    }
+    if (!(si > 0.0) || !(so > 0.0)) return -1;
+    free(m->lt_c0); free(m->lt_apl); free(m->lt_in); free(m->lt_out);
+    m->lt_c0 = malloc(sizeof(double) * (size_t)n);
+    m->lt_apl = malloc(sizeof(double) * (size_t)n);
+    m->lt_in = malloc(sizeof(double) * (size_t)n);
+    m->lt_out = malloc(sizeof(double) * (size_t)n);
+    memcpy(m->lt_c0, c0, sizeof(double) * (size_t)n);
+    memcpy(m->lt_apl, apl, sizeof(double) * (size_t)n);
+    memcpy(m->lt_in, in_parts, sizeof(double) * (size_t)n);
+    memcpy(m->lt_out, out_parts, sizeof(double) * (size_t)n);
+    m->nlt = n;
+    m->lt_seed = seed;
+    free(m->Lo); free(m->Li);
+    m->Lo = calloc((size_t)m->nt * (size_t)n, sizeof(double));
+    m->Li = calloc((size_t)m->nt * (size_t)n, sizeof(double));
+    return 0;
+}
+
+/* the lipid types in force: the composition, or the parameters' single type */
+static int lipid_types(const omesh *m, const double **c0, const double **apl)
+{
+    if (m->nlt > 0) {
+        *c0 = m->lt_c0;
+        *apl = m->lt_apl;
+        return m->nlt;
+    }
+    *c0 = &m->prm.lipid_c0;
+    *apl = &m->prm.lipid_apl;
+    return 1;
 }
 
 /* forward contraction of one Gauss point: X[0..5] = R, Ru, Rv, Ruu, Ruv, Rvv */
@@ -1330,10 +1430,69 @@ static void normalize3(double *v)
 
 /* setg0: reference metric and single-lipid packing (src/interpLib.C:
  * 2969-3388); leaflets: add APL until the packed area exceeds A - 65/2 */
+/* Lipid packing of triangle t (several types; src/interpLib.C:3127-3196 for
+ * regular faces, 3286-3358 for irregular ones): leaflet fractions from the
+ * input parts, cumulated; draws of rand()/RAND_MAX pick a type by the
+ * cumulative fractions and add its area until a leaflet holds more than
+ * A - 65/2.  Regular faces draw one (inner, outer) pair per round, irregular
+ * faces one pair per type within a round. */
+static void pack_lipids(omesh *m, orc_grand *gr, int t, double A, int irregular)
+{
+    const int n = m->nlt;
+    double *in = m->Li + (size_t)t * n, *out = m->Lo + (size_t)t * n;
+    memset(in, 0, sizeof(double) * (size_t)n);
+    memset(out, 0, sizeof(double) * (size_t)n);
+    double fr_i[ORC_MAX_LIPID_TYPES], fr_o[ORC_MAX_LIPID_TYPES];
+    double sum_i = 0.0, sum_o = 0.0;
+    for (int x = 0; x < n; x++) {
+        fr_i[x] = m->lt_in[x];
+        fr_o[x] = m->lt_out[x];
+        sum_i += fr_i[x];
+        sum_o += fr_o[x];
+    }
+    for (int x = 0; x < n; x++) {
+        fr_i[x] /= sum_i;
+        fr_o[x] /= sum_o;
+    }
+    for (int x = 1; x < n; x++) fr_i[x] += fr_i[x - 1];
+    for (int x = 1; x < n; x++) fr_o[x] += fr_o[x - 1];
+    const double area_per_lipid_av = 65.0;
+    double a_i = 0.0, a_o = 0.0;
+    for (;;) {
+        double rn_i = 0.0, rn_o = 0.0;
+        if (!irregular) {
+            rn_i = orc_grand_next(gr) / 2147483647.0;
+            rn_o = orc_grand_next(gr) / 2147483647.0;
+        }
+        int idone = 0, odone = 0;
+        if (a_i > 0 && a_i > A - 0.5 * area_per_lipid_av) idone = 1;
+        if (a_o > 0 && a_o > A - 0.5 * area_per_lipid_av) odone = 1;
+        if (idone && odone) break;
+        for (int x = 0; x < n; x++) {
+            if (irregular) {
+                rn_i = orc_grand_next(gr) / 2147483647.0;
+                rn_o = orc_grand_next(gr) / 2147483647.0;
+            }
+            if (rn_i < fr_i[x] && !idone) {
+                in[x] += m->lt_apl[x];
+                idone = 1;
+                a_i += m->lt_apl[x];
+            }
+            if (rn_o < fr_o[x] && !odone) {
+                out[x] += m->lt_apl[x];
+                odone = 1;
+                a_o += m->lt_apl[x];
+            }
+        }
+    }
+}
+
 void orc_setg0(omesh *m, const double *r)
 {
     int nf = m->nfr + m->nfi;
     double apl = m->prm.lipid_apl;
+    orc_grand gr;
+    orc_grand_seed(&gr, m->lt_seed);
     for (int f = 0; f < nf; f++) {
         int npt;
         double dudv;
@@ -1360,16 +1519,20 @@ void orc_setg0(omesh *m, const double *r)
             else A += g * F[p].weight;
         }
         int t = F[0].tri;
-        double a_i = 0.0, a_o = 0.0, Li = 0.0, Lo = 0.0;
-        for (;;) {
-            int idone = (a_i > 0 && a_i > A - 0.5 * 65.0);
-            int odone = (a_o > 0 && a_o > A - 0.5 * 65.0);
-            if (idone && odone) break;
-            if (!idone) { Li += apl; a_i += apl; }
-            if (!odone) { Lo += apl; a_o += apl; }
+        if (m->nlt == 0) {
+            double a_i = 0.0, a_o = 0.0, Li = 0.0, Lo = 0.0;
+            for (;;) {
+                int idone = (a_i > 0 && a_i > A - 0.5 * 65.0);
+                int odone = (a_o > 0 && a_o > A - 0.5 * 65.0);
+                if (idone && odone) break;
+                if (!idone) { Li += apl; a_i += apl; }
+                if (!odone) { Lo += apl; a_o += apl; }
+            }
+            m->Li[t] = Li;
+            m->Lo[t] = Lo;
+        } else {
+            pack_lipids(m, &gr, t, A, f >= m->nfr);
         }
-        m->Li[t] = Li;
-        m->Lo[t] = Lo;
     }
     double A0 = 0.0;
     for (int f = 0; f < nf; f++) {
@@ -1422,11 +1585,16 @@ double orc_face_energy(omesh *m, int f, const double *r)
         double val = RuRv / sqrt(RuRu * RvRv);
         e += P->k_reg * (val - F[p].RuRv0) * (val - F[p].RuRv0);
         if (P->mode == 0) {
+            /* every lipid type of both leaflets (src/interpLib.C:3818-3838) */
             int t = F[p].tri;
-            double dc_o = (c1 + c2 - P->lipid_c0), dc_i = (-c1 - c2 - P->lipid_c0);
+            const double *lc0, *lapl;
+            const int n = lipid_types(m, &lc0, &lapl);
             double lec = 0.0;
-            lec += 0.5 * P->kc * dc_o * dc_o * m->Lo[t] * 0.5 * w;
-            lec += 0.5 * P->kc * dc_i * dc_i * m->Li[t] * 0.5 * w;
+            for (int x = 0; x < n; x++) {
+                double dc_o = (c1 + c2 - lc0[x]), dc_i = (-c1 - c2 - lc0[x]);
+                lec += 0.5 * P->kc * dc_o * dc_o * m->Lo[(size_t)t * n + x] * 0.5 * w;
+                lec += 0.5 * P->kc * dc_i * dc_i * m->Li[(size_t)t * n + x] * 0.5 * w;
+            }
             e += lec;
         }
         e += dudv * dAf * en;
@@ -1589,12 +1757,17 @@ static void point_grad(omesh *m, int f, const oform *F, double dudv, const doubl
     /* dE/dc1, dE/dc2, dE/dg */
     double de_c1 = 0.0, de_c2 = 0.0, de_g = 0.0;
     if (P->mode == 0) {
+        /* every lipid type (src/grad.C:405-423) */
         int t = F->tri;
-        double dc_o = (e1 + e2 - P->lipid_c0), dc_i = (-e1 - e2 - P->lipid_c0);
-        de_c1 += P->kc * dc_o * 0.5 * m->Lo[t] * w;
-        de_c1 -= P->kc * dc_i * 0.5 * m->Li[t] * w;
-        de_c2 += P->kc * dc_o * 0.5 * m->Lo[t] * w;
-        de_c2 -= P->kc * dc_i * 0.5 * m->Li[t] * w;
+        const double *lc0, *lapl;
+        const int n = lipid_types(m, &lc0, &lapl);
+        for (int x = 0; x < n; x++) {
+            double dc_o = (e1 + e2 - lc0[x]), dc_i = (-e1 - e2 - lc0[x]);
+            de_c1 += P->kc * dc_o * 0.5 * m->Lo[(size_t)t * n + x] * w;
+            de_c1 -= P->kc * dc_i * 0.5 * m->Li[(size_t)t * n + x] * w;
+            de_c2 += P->kc * dc_o * 0.5 * m->Lo[(size_t)t * n + x] * w;
+            de_c2 -= P->kc * dc_i * 0.5 * m->Li[(size_t)t * n + x] * w;
+        }
     } else {
         double c0 = m->face_c0[f];
         de_c1 += (P->kc * (e1 + e2 - c0)) * dudv * dAf;

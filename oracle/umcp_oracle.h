@@ -18,6 +18,8 @@
 extern "C" {
 #endif
 
+#include <stdint.h>
+
 typedef struct omesh omesh;
 
 /* energy modes: 0 = reference default (FIXED_A + LOCAL_LIPID_ENERGY,
@@ -91,6 +93,18 @@ int    orc_face_info(const omesh *m, int f, int *owner, int *ncp, int *cp, int *
 int    orc_face_table(const omesh *m, int f, int p, double *w6, double *weight,
                       double *pbc, double *g0);
 void   orc_tri_lipids(const omesh *m, int t, double *Lo, double *Li);
+/* lipid compositions (src/lipid_composition.C:47-333, src/interpLib.C:
+ * 3127-3196, 3286-3358): n types with spontaneous curvature c0, area per
+ * lipid apl and input parts per leaflet, packed at the next setg0 with
+ * rand() after srand(seed); -1 for a bad composition */
+#define ORC_MAX_LIPID_TYPES 64
+int    orc_set_lipids(omesh *m, int n, const double *c0, const double *apl, const double *in_parts,
+                      const double *out_parts, unsigned int seed);
+void   orc_tri_lipid_types(const omesh *m, int t, double *Lo, double *Li);
+/* glibc's rand(), restated (srand(seed), then rand()) */
+typedef struct { int32_t r[31]; int f, b; } orc_grand;
+void   orc_grand_seed(orc_grand *g, unsigned int seed);
+int    orc_grand_next(orc_grand *g);
 int    orc_ntri(const omesh *m);
 int    orc_tri_order_face(const omesh *m, int t);
 void   orc_neighbors(const omesh *m, int v, int *val, int *nb);

@@ -506,7 +506,7 @@ struct PointGrad {
 };
 
 template <typename R, int FEAT, bool REG>
-__device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R ig0, R rurv0, R Lo, R Li,
+__device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R ig0, R rurv0, R lw, R ls, R lv,
                                                R c0f, R dudv, double lamA, const PhysParams &ph, PointGrad<R> &out)
 {
     constexpr bool kAnyMode = FEAT == kFeatGeneral;
@@ -535,12 +535,13 @@ __device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R ig0
     R e, dH, hg = R(0);
     const bool lipid = kAnyMode ? (ph.mode == 0) : (FEAT == kFeatLipid);
     if (lipid) {
-        // lipid-area weighted leaflets, no dudv (src/interpLib.C:3832-3834)
-        const R lc0 = (R)ph.lipid_c0;
-        const R ho = H - lc0, hi = H + lc0;
+        // lipid-area weighted leaflets, no dudv, every type (src/interpLib.C:
+        // 3818-3838): sum_j 1/4 kc w W_j (H - s_j)^2 = 1/4 kc w (lw (H - ls)^2
+        // + lv), the spread lv fixed per face (no cancellation near H = s_j)
         const R q = R(0.25) * kc * w;
-        e = q * (ho * ho * Lo + hi * hi * Li);
-        dH = R(2) * q * (ho * Lo + hi * Li);
+        const R hs = H - ls;
+        e = q * (lw * hs * hs + lv);
+        dH = R(2) * q * lw * hs;
     } else {
         const R hd = H - c0f;
         const R eh = R(0.5) * kc * hd * hd;
@@ -610,11 +611,11 @@ __device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R ig0
 
 // the same with the five rows expanded: D[d][k] = dE/dX[d][k]
 template <typename R, int FEAT, bool REG>
-__device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R ig0, R rurv0, R Lo, R Li,
+__device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R ig0, R rurv0, R lw, R ls, R lv,
                                                R c0f, R dudv, double lamA, const PhysParams &ph, R D[5][3])
 {
     PointGrad<R> pg;
-    const R e = point_energy_grad<R, FEAT, REG>(X, w, g0, ig0, rurv0, Lo, Li, c0f, dudv, lamA, ph, pg);
+    const R e = point_energy_grad<R, FEAT, REG>(X, w, g0, ig0, rurv0, lw, ls, lv, c0f, dudv, lamA, ph, pg);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         D[0][k] = pg.du[k];
@@ -743,7 +744,7 @@ __device__ __forceinline__ void face_group(const FaceArgs &a, const double *__re
 #pragma unroll
         for (int k = 0; k < 3; k++) s_corr[k][lane] = corr[k];
     }
-    const R Lo = (R)a.Lo[fc], Li = (R)a.Li[fc], c0f = (R)a.c0[fc];
+    const R lw = (R)a.lw[fc], ls = (R)a.ls[fc], lv = (R)a.lv[fc], c0f = (R)a.c0[fc];
     const double lamA = lam[0];
     double e = 0.0;
     for (int p = w; p < NPT; p += WS) {
@@ -770,7 +771,7 @@ __device__ __forceinline__ void face_group(const FaceArgs &a, const double *__re
         PointGrad<R> pg;
         const R rr0 = FEAT == kFeatGeneral ? (R)a.rurv0[p * nf + fc] : R(0);
         e += (double)point_energy_grad<R, FEAT, KIND == 0>(X, wts[p], (R)a.g0[p * nf + fc], (R)a.ig0[p * nf + fc], rr0,
-                                                Lo, Li, c0f, (R)a.dudv, lamA, a.ph, pg);
+                                                lw, ls, lv, c0f, (R)a.dudv, lamA, a.ph, pg);
         if constexpr (sizeof(R) == 4) {
             // FP32: the second-derivative rows factored, sum_d TB[c][d]
             // D[d][k] = TB0 du_k + TB1 dv_k + (TB2 dL + TB3 dM + TB4 dN) n_k
@@ -909,7 +910,7 @@ __device__ __forceinline__ void face_group_cps(const FaceArgs &a, const double *
     }
     __syncthreads();
 #define PV(c, k) sP[(((c) - 1) * 3 + (k)) * kWave + lane]
-    const R Lo = (R)a.Lo[fc], Li = (R)a.Li[fc], c0f = (R)a.c0[fc];
+    const R lw = (R)a.lw[fc], ls = (R)a.ls[fc], lv = (R)a.lv[fc], c0f = (R)a.c0[fc];
     const double lamA = lam[0];
     R G[kOwn][3];
 #pragma unroll
@@ -939,8 +940,8 @@ __device__ __forceinline__ void face_group_cps(const FaceArgs &a, const double *
             PointGrad<R> pg;
             const R rr0 = FEAT == kFeatGeneral ? (R)a.rurv0[p * nf + fc] : R(0);
             e += (double)point_energy_grad<R, FEAT, KIND == 0>(X, wts[p], (R)a.g0[p * nf + fc],
-                                                               (R)a.ig0[p * nf + fc], rr0, Lo, Li, c0f, (R)a.dudv,
-                                                               lamA, a.ph, pg);
+                                                               (R)a.ig0[p * nf + fc], rr0, lw, ls, lv, c0f,
+                                                               (R)a.dudv, lamA, a.ph, pg);
             R *d = sD + w * kDRows * kWave + lane;
 #pragma unroll
             for (int k = 0; k < 3; k++) {

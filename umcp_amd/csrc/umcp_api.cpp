@@ -134,8 +134,9 @@ FaceArgs face_args(umcp_surface *s, size_t c, const double *d_r)
     a.g0 = D.g0;
     a.rurv0 = D.rurv0;
     a.ig0 = D.ig0;
-    a.Lo = D.Lo;
-    a.Li = D.Li;
+    a.lw = D.lw;
+    a.ls = D.ls;
+    a.lv = D.lv;
     a.c0 = D.c0;
     a.dudv = C.dudv;
     std::memcpy(a.pbc, s->T.pbc, sizeof a.pbc);
@@ -510,8 +511,8 @@ int umcp_plan(umcp_surface *s)
         if ((rc = dev_alloc(s, &D.cp, C.cp.size())) || (rc = dev_alloc(s, &D.pos, C.pos.size())) || (rc = dev_alloc(s, &D.img, C.img.size())) ||
             (rc = dev_alloc(s, &D.loc, C.loc.size())) ||
             (rc = dev_alloc(s, &D.has_img, nf)) || (rc = dev_alloc(s, &D.g0, nf * C.npt)) ||
-            (rc = dev_alloc(s, &D.rurv0, nf * C.npt)) || (rc = dev_alloc(s, &D.ig0, nf * C.npt)) || (rc = dev_alloc(s, &D.Lo, nf)) ||
-            (rc = dev_alloc(s, &D.Li, nf)) || (rc = dev_alloc(s, &D.c0, nf)) ||
+            (rc = dev_alloc(s, &D.rurv0, nf * C.npt)) || (rc = dev_alloc(s, &D.ig0, nf * C.npt)) || (rc = dev_alloc(s, &D.lw, nf)) ||
+            (rc = dev_alloc(s, &D.ls, nf)) || (rc = dev_alloc(s, &D.lv, nf)) || (rc = dev_alloc(s, &D.c0, nf)) ||
             0)
             return rc;
         if ((rc = upload_tables_once(s, C))) return rc;
@@ -523,8 +524,9 @@ int umcp_plan(umcp_surface *s)
             0)
             return rc;
         std::vector<double> zero(nf * C.npt, 0.0);
-        if ((rc = dev_upload(s, D.c0, zero.data(), nf)) || (rc = dev_upload(s, D.Lo, zero.data(), nf)) ||
-            (rc = dev_upload(s, D.Li, zero.data(), nf)) || (rc = dev_upload(s, D.g0, zero.data(), nf * C.npt)) ||
+        if ((rc = dev_upload(s, D.c0, zero.data(), nf)) || (rc = dev_upload(s, D.lw, zero.data(), nf)) ||
+            (rc = dev_upload(s, D.ls, zero.data(), nf)) || (rc = dev_upload(s, D.lv, zero.data(), nf)) ||
+            (rc = dev_upload(s, D.g0, zero.data(), nf * C.npt)) ||
             (rc = dev_upload(s, D.rurv0, zero.data(), nf * C.npt)) ||
             (rc = dev_upload(s, D.ig0, zero.data(), nf * C.npt)))
             return rc;
@@ -649,34 +651,79 @@ int umcp_setg0(umcp_surface *s, const double *r)
     }
     if ((rc = sync(s))) return rc;
     s->V_setg0 = h_rs[4];
-    // single-lipid packing per triangle from the reference area of each face
-    // (src/interpLib.C:3113-3196): add APL per leaflet until it exceeds A - 65/2
-    const double apl = s->prm.lipid_apl;
+    // lipid packing per triangle from the reference area of each face, in the
+    // reference's face order (regular faces, then irregular): one type,
+    // add its area per lipid per leaflet until it exceeds A - 65/2
+    // (src/interpLib.C:3113-3196); a composition: the reference's rand()
+    // draws (src/interpLib.C:3127-3196, 3286-3358)
+    const int nfaces = s->P.nfaces;
+    const bool comp = s->lipids.ntypes() > 0;
+    const int nt = comp ? s->lipids.ntypes() : 1;
+    const std::vector<double> one_c0(1, s->prm.lipid_c0);
+    const std::vector<double> &tc0 = comp ? s->lipids.c0 : one_c0;
+    std::vector<double> Aface(nfaces, 0.0);
+    std::vector<char> irr(nfaces, 0);
     double A0 = 0.0;
-    std::vector<std::vector<double>> Ls(s->P.classes.size());
     for (size_t c = 0; c < s->P.classes.size(); c++) {
         const FaceClass &C = s->P.classes[c];
         const int nf = C.nf();
         const std::vector<double> &g0 = g0s[c];
-        std::vector<double> &L_c = Ls[c];
-        L_c.resize(nf);
         for (int f = 0; f < nf; f++) {
             double A = 0.0;
             for (int p = 0; p < C.npt; p++) A += g0[(size_t)p * nf + f] * C.dudv * C.weight[p];
+            Aface[C.face_id[f]] = A;
+            irr[C.face_id[f]] = C.valence != 6;
+        }
+        for (int p = 0; p < C.npt; p++)
+            for (int f = 0; f < nf; f++) A0 += C.dudv * g0[(size_t)p * nf + f] * C.weight[p];
+    }
+    s->face_lip_o.assign((size_t)nfaces * nt, 0.0);
+    s->face_lip_i.assign((size_t)nfaces * nt, 0.0);
+    if (comp) {
+        GlibcRand gr(s->lipids.seed);
+        for (int f = 0; f < nfaces; f++)
+            pack_lipids(s->lipids, gr, Aface[f], irr[f], &s->face_lip_i[(size_t)f * nt], &s->face_lip_o[(size_t)f * nt]);
+    } else {
+        const double apl = s->prm.lipid_apl;
+        for (int f = 0; f < nfaces; f++) {
             double packed = 0.0, L = 0.0;
-            while (!(packed > 0 && packed > A - 0.5 * 65.0)) {
+            while (!(packed > 0 && packed > Aface[f] - 0.5 * 65.0)) {
                 L += apl;
                 packed += apl;
             }
-            L_c[f] = L;
-            s->face_Lo[C.face_id[f]] = L;
-            s->face_Li[C.face_id[f]] = L;
+            s->face_lip_o[f] = s->face_lip_i[f] = L;
         }
-        // both leaflets hold the same packing
-        if ((rc = dev_upload(s, s->cls[c].Lo, L_c.data(), nf)) || (rc = dev_upload(s, s->cls[c].Li, L_c.data(), nf)))
+    }
+    // per face: each leaflet's total, and the lipid energy's (lw, ls, lv)
+    for (size_t c = 0; c < s->P.classes.size(); c++) {
+        const FaceClass &C = s->P.classes[c];
+        const int nf = C.nf();
+        std::vector<double> lw(nf), ls(nf), lv(nf);
+        for (int f = 0; f < nf; f++) {
+            const int gf = C.face_id[f];
+            const double *Lo = &s->face_lip_o[(size_t)gf * nt], *Li = &s->face_lip_i[(size_t)gf * nt];
+            double so = 0.0, si = 0.0, W = 0.0, Ws = 0.0;
+            for (int x = 0; x < nt; x++) {
+                so += Lo[x];
+                si += Li[x];
+                W += Lo[x] + Li[x];
+                Ws += Lo[x] * tc0[x] - Li[x] * tc0[x];
+            }
+            const double sb = W > 0 ? Ws / W : 0.0;
+            double V = 0.0;
+            for (int x = 0; x < nt; x++) {
+                const double d_o = tc0[x] - sb, d_i = -tc0[x] - sb;
+                V += Lo[x] * d_o * d_o + Li[x] * d_i * d_i;
+            }
+            s->face_Lo[gf] = so;
+            s->face_Li[gf] = si;
+            lw[f] = W;
+            ls[f] = sb;
+            lv[f] = V;
+        }
+        if ((rc = dev_upload(s, s->cls[c].lw, lw.data(), nf)) || (rc = dev_upload(s, s->cls[c].ls, ls.data(), nf)) ||
+            (rc = dev_upload(s, s->cls[c].lv, lv.data(), nf)))
             return rc;
-        for (int p = 0; p < C.npt; p++)
-            for (int f = 0; f < nf; f++) A0 += C.dudv * g0[(size_t)p * nf + f] * C.weight[p];
     }
     s->A0_setg0 = A0;
     if ((rc = sync(s))) return rc;
@@ -1412,6 +1459,58 @@ int umcp_face_lipids(const umcp_surface *s, int face, double *Lo, double *Li)
     if (!s || !s->g0_set || face < 0 || face >= s->P.nfaces) return UMCP_ERR_ARG;
     if (Lo) *Lo = s->face_Lo[face];
     if (Li) *Li = s->face_Li[face];
+    return UMCP_OK;
+}
+
+int umcp_lipid_type_count(const umcp_surface *s)
+{
+    if (!s) return UMCP_ERR_ARG;
+    return s->lipids.ntypes() > 0 ? s->lipids.ntypes() : 1;
+}
+
+int umcp_face_lipid_types(const umcp_surface *s, int face, double *Lo, double *Li)
+{
+    if (!s || !s->g0_set || face < 0 || face >= s->P.nfaces || !Lo || !Li) return UMCP_ERR_ARG;
+    const size_t nt = s->face_lip_o.size() / (size_t)s->P.nfaces;
+    for (size_t x = 0; x < nt; x++) {
+        Lo[x] = s->face_lip_o[(size_t)face * nt + x];
+        Li[x] = s->face_lip_i[(size_t)face * nt + x];
+    }
+    return UMCP_OK;
+}
+
+int umcp_set_lipids(umcp_surface *s, int ntypes, const double *c0, const double *apl, const double *inner_parts,
+                    const double *outer_parts, unsigned int seed)
+{
+    if (!s) return UMCP_ERR_ARG;
+    if (ntypes < 1 || ntypes > kMaxLipidTypes || !c0 || !apl || !inner_parts || !outer_parts)
+        return fail(s, UMCP_ERR_ARG, "a composition has 1 to 64 lipid types");
+    LipidComposition L;
+    double si = 0.0, so = 0.0;
+    for (int x = 0; x < ntypes; x++) {
+        if (!(apl[x] > 0.0) || !(inner_parts[x] >= 0.0) || !(outer_parts[x] >= 0.0) || !std::isfinite(c0[x]))
+            return fail(s, UMCP_ERR_ARG, "lipid types need an area per lipid above 0 and parts of at least 0");
+        si += inner_parts[x];
+        so += outer_parts[x];
+        L.names.push_back("type" + std::to_string(x));
+        L.c0.push_back(c0[x]);
+        L.apl.push_back(apl[x]);
+        L.inner.push_back(inner_parts[x]);
+        L.outer.push_back(outer_parts[x]);
+    }
+    if (!(si > 0.0) || !(so > 0.0)) return fail(s, UMCP_ERR_ARG, "each leaflet needs parts above 0");
+    L.seed = seed;
+    s->lipids = L;
+    return UMCP_OK;
+}
+
+int umcp_read_lipid_composition(umcp_surface *s, const char *path, unsigned int seed)
+{
+    if (!s || !path) return s ? fail(s, UMCP_ERR_ARG, "null path") : UMCP_ERR_ARG;
+    LipidComposition L;
+    std::string err;
+    if (read_lipid_composition(path, seed, L, err)) return fail(s, UMCP_ERR_ARG, err);
+    s->lipids = L;
     return UMCP_OK;
 }
 

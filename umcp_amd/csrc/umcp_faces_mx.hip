@@ -141,14 +141,15 @@ __device__ __forceinline__ void face_group_mx(const FaceArgs &a, const double *_
     const float dudv = (float)a.dudv;
     // per-face inputs of the lane's four faces (16 t + j), loaded up front
     constexpr bool kLip = FEAT != kFeatHelfrich, kC0 = FEAT != kFeatLipid;
-    float fLo[4], fLi[4], fc0[4];
+    float flw[4], fls[4], flv[4], fc0[4];
     int fcs[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
         const int f = group * kWave + 16 * t + j;
         fcs[t] = f < nf ? f : nf - 1;
-        fLo[t] = kLip ? (float)a.Lo[fcs[t]] : 0.f;
-        fLi[t] = kLip ? (float)a.Li[fcs[t]] : 0.f;
+        flw[t] = kLip ? (float)a.lw[fcs[t]] : 0.f;
+        fls[t] = kLip ? (float)a.ls[fcs[t]] : 0.f;
+        flv[t] = kLip ? (float)a.lv[fcs[t]] : 0.f;
         fc0[t] = kC0 ? (float)a.c0[fcs[t]] : 0.f;
     }
     double e = 0.0;
@@ -213,8 +214,8 @@ __device__ __forceinline__ void face_group_mx(const FaceArgs &a, const double *_
                     X[3][k] = X0[k][3] * al[k];
                     X[4][k] = X1[k][0] * al[k];
                 }
-                const float ee = point_energy_grad<float, FEAT, KIND == 0>(X, wp, pg0[t], pig0[t], prr0[t], fLo[t], fLi[t],
-                                                                fc0[t], dudv, lamA, a.ph, D);
+                const float ee = point_energy_grad<float, FEAT, KIND == 0>(X, wp, pg0[t], pig0[t], prr0[t], flw[t], fls[t],
+                                                                flv[t], fc0[t], dudv, lamA, a.ph, D);
                 if (group * kWave + fl < nf) e += (double)ee;
             } else {
 #pragma unroll

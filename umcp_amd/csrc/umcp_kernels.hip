@@ -713,7 +713,7 @@ __device__ double face_energy_one(const FaceArgs &a, const double *__restrict__ 
             }
         PointGrad<double> pg;
         e += point_energy_grad<double, kFeatGeneral, KIND == 0>(X, wts[p], a.g0[p * nf + fc], a.ig0[p * nf + fc],
-                                                     a.rurv0[p * nf + fc], a.Lo[fc], a.Li[fc], a.c0[fc], a.dudv, 0.0,
+                                                     a.rurv0[p * nf + fc], a.lw[fc], a.ls[fc], a.lv[fc], a.c0[fc], a.dudv, 0.0,
                                                      a.ph, pg);
         // surface::area's terms (src/interpLib.C:4480-4560)
         const double E11 = X[0][0] * X[0][0] + X[0][1] * X[0][1] + X[0][2] * X[0][2];

@@ -34,7 +34,10 @@ struct FaceArgs {
     const double *g0;       // [npt][nf]
     const double *rurv0;    // [npt][nf]
     const double *ig0;      // [npt][nf]   1 / g0
-    const double *Lo, *Li;  // [nf] lipid areas of the face's triangle
+    // [nf] lipid mode (every type of both leaflets of the face's triangle,
+    // j = (type, leaflet), W_j the type's area there, s_j = c0 outer, -c0
+    // inner): lw = sum W_j, ls = sum W_j s_j / lw, lv = sum W_j (s_j - ls)^2
+    const double *lw, *ls, *lv;
     const double *c0;       // [nf]
     double dudv;
     double pbc[3][3];

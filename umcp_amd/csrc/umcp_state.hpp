@@ -9,6 +9,7 @@
 
 #include "../../include/umcp.h"
 #include "surface_host.hpp"
+#include "umcp_lipids.hpp"
 #include "umcp_kernels.hpp"
 
 namespace umcp {
@@ -17,7 +18,8 @@ struct ClassDev {
     int *cp = nullptr, *pos = nullptr;
     uint16_t *loc = nullptr;
     uint8_t *img = nullptr, *has_img = nullptr;
-    double *g0 = nullptr, *rurv0 = nullptr, *ig0 = nullptr, *Lo = nullptr, *Li = nullptr, *c0 = nullptr;
+    double *g0 = nullptr, *rurv0 = nullptr, *ig0 = nullptr, *c0 = nullptr;
+    double *lw = nullptr, *ls = nullptr, *lv = nullptr;   // FaceArgs::lw, ls, lv
 };
 
 }  // namespace umcp
@@ -38,7 +40,11 @@ struct umcp_surface {
     umcp_params prm{};
     bool prepared = false, planned = false, g0_set = false;
     std::vector<double> face_c0;   // global face order
-    std::vector<double> face_Lo, face_Li;
+    std::vector<double> face_Lo, face_Li;             // each leaflet's lipid area per face
+    // lipid composition (none: one type of prm.lipid_c0 / lipid_apl) and the
+    // area of each type per face after setg0, [nfaces][ntypes]
+    umcp::LipidComposition lipids;
+    std::vector<double> face_lip_o, face_lip_i;
     std::vector<umcp::ClassDev> cls;
     std::vector<void *> allocations;
     size_t device_bytes = 0;              // sum of the allocation sizes

@@ -42,7 +42,8 @@ EXPORTS = [
     "umcp_noise", "umcp_ensemble_create", "umcp_ensemble_destroy", "umcp_ensemble_last_error",
     "umcp_ensemble_add", "umcp_ensemble_remove", "umcp_ensemble_size", "umcp_ensemble_refresh",
     "umcp_ensemble_set_launch", "umcp_ensemble_get_launch", "umcp_ensemble_bd_steps", "umcp_ensemble_work",
-    "umcp_nan_check", "umcp_ensemble_nan_check",
+    "umcp_nan_check", "umcp_ensemble_nan_check", "umcp_read_lipid_composition", "umcp_set_lipids",
+    "umcp_lipid_type_count", "umcp_face_lipid_types",
     "umcp_clone", "umcp_device_bytes", "umcp_server_create", "umcp_server_submit", "umcp_server_wait",
     "umcp_server_drain", "umcp_server_stats", "umcp_server_last_error", "umcp_server_destroy",
     "umcp_effm_build", "umcp_effm_get", "umcp_effm_steps", "umcp_effm_steps_dev",
@@ -175,6 +176,10 @@ def load_library(path: str = LIB_PATH):
         "umcp_ensemble_bd_steps": ([vp, C.c_int, C.c_int, vp], C.c_int),
         "umcp_ensemble_work": ([vp, ip, ip, ip], C.c_int),
         "umcp_nan_check": ([vp], C.c_int),
+        "umcp_read_lipid_composition": ([vp, C.c_char_p, C.c_uint], C.c_int),
+        "umcp_set_lipids": ([vp, C.c_int, dp, dp, dp, dp, C.c_uint], C.c_int),
+        "umcp_lipid_type_count": ([vp], C.c_int),
+        "umcp_face_lipid_types": ([vp, C.c_int, dp, dp], C.c_int),
         "umcp_ensemble_nan_check": ([vp, ip], C.c_int),
         "umcp_clone": ([vp, C.POINTER(vp)], C.c_int),
         "umcp_device_bytes": ([vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], C.c_int),
@@ -320,6 +325,24 @@ class Surface:
                                           C.c_void_p(d_g.data_ptr()) if d_g is not None else None,
                                           C.c_void_p(stream) if stream else None)
         self._check(rc, "energy_grad_dev")
+
+    def readLipidComposition(self, path: str, seed: int = 1):
+        """surface::readLipidComposition: the composition of a file's "lipid"
+        directives, packed at the next setg0 with rand() after srand(seed)."""
+        self._check(self._L.umcp_read_lipid_composition(self._h, path.encode(), int(seed)), "readLipidComposition")
+
+    def set_lipids(self, c0, apl, inner, outer, seed: int = 1):
+        """A lipid composition from arrays (types' c0 1/A, area per lipid A^2,
+        input parts of the inner and outer leaflet)."""
+        arr = [np.ascontiguousarray(x, dtype=np.float64) for x in (c0, apl, inner, outer)]
+        self._check(self._L.umcp_set_lipids(self._h, len(arr[0]), *[_dp(x) for x in arr], int(seed)), "set_lipids")
+
+    def face_lipid_types(self, f: int):
+        """Each lipid type's area in the outer and inner leaflet of face f."""
+        n = self._L.umcp_lipid_type_count(self._h)
+        lo, li = np.zeros(n), np.zeros(n)
+        self._check(self._L.umcp_face_lipid_types(self._h, int(f), _dp(lo), _dp(li)), "face_lipid_types")
+        return lo, li
 
     def nan_check(self):
         """Raises UmcpError (UMCP_ERR_NAN_ENERGY) if an evaluation since the last
