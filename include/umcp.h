@@ -243,7 +243,7 @@ int umcp_volume(umcp_surface *s, const double *r, double *V);
  * uses. */
 #define UMCP_EFFM_BD 0
 #define UMCP_EFFM_LD 1
-#define UMCP_EFFM_MAX_NV 65536
+#define UMCP_EFFM_MAX_NV 65536   /* umcp_effm_build (dense inverse) only */
 typedef struct {
     int mode;              /* UMCP_EFFM_BD or UMCP_EFFM_LD */
     double dt;             /* time step, s */
@@ -254,6 +254,22 @@ int umcp_effm_build(umcp_surface *s, long *nnz, double *point_mass);
 /* the (truncated) inverse effective mass as a dense nv x nv array; *nnz of
  * umcp_effm_build counts the entries the truncation keeps */
 int umcp_effm_get(umcp_surface *s, double *Minv);
+/* The scalable form of getSparseEffectiveMass (src/modes.C:1719-1960): the
+ * mass matrix M stays sparse on the device (ELL, about 37 entries per row)
+ * and every product Mi v of the integrators is a solve of M x = v by a
+ * Chebyshev iteration preconditioned with diag(M)^-1, of a fixed length set
+ * here from the spectral interval of diag(M)^-1 M (Gershgorin above, Lanczos
+ * below) and checked on a pseudo-random right-hand side: the relative
+ * residual is below tol (0: 1e-12; accepted 1e-14 .. 1e-3).  The reference's
+ * truncation drops entries below 1e-20 of the diagonal, so the two forms'
+ * products agree to the solve's tolerance.  No vertex limit; memory O(nv).
+ * *nnz: the entries of M; *iterations: the solve's length.  Replaces an
+ * earlier umcp_effm_build (and the reverse); umcp_effm_get then fails
+ * (UMCP_ERR_STATE), umcp_effm_apply gives products. */
+int umcp_effm_build_sparse(umcp_surface *s, double tol, long *nnz, double *point_mass, int *iterations);
+/* out = Mi v for the built form (host arrays [nv][3]): the product with the
+ * truncated inverse (umcp_effm_build) or the solve (umcp_effm_build_sparse) */
+int umcp_effm_apply(umcp_surface *s, const double *v, double *out);
 /* host arrays: r (3nv+3) and pp (3nv) in/out, xi [nsteps][3nv], energies
  * (energy at the start of each step) may be NULL */
 int umcp_effm_steps(umcp_surface *s, double *r, double *pp, const double *xi, int nsteps,

@@ -78,6 +78,17 @@ int build_plan(const Topology &T, Plan &P, std::string &err);
 int effm_points_per_face();
 int effm_mass_csr(const Topology &T, const Plan &P, double mass_per_pt, std::vector<int> &rowptr,
                   std::vector<int> &col, std::vector<double> &val, std::string &err);
+// the Chebyshev solve of M x = b (umcp_effm_build_sparse): M in ELL layout
+// (slot-major [width][n], rows padded with zeros on the diagonal) and D^-1;
+// the spectral interval [lo, hi] of D^-1 M (0 on success); the iterations
+// that bring the residual below tol on it; the per-iteration coefficients
+void effm_ell(const std::vector<int> &rowptr, const std::vector<int> &col, const std::vector<double> &val,
+              int &width, std::vector<int> &ell_col, std::vector<double> &ell_val, std::vector<double> &dinv);
+int effm_jacobi_interval(const std::vector<int> &rowptr, const std::vector<int> &col, const std::vector<double> &val,
+                         double &lo, double &hi);
+int effm_cheb_iterations(double lo, double hi, double tol);
+void effm_cheb_coeffs(double lo, double hi, int iters, std::vector<double> &c1, std::vector<double> &c2,
+                      double &inv_theta);
 
 // 12 quartic box-spline basis functions and their first/second derivatives at
 // (u, v), in the control-point slot order of the face plan: out[d * 12 + y].

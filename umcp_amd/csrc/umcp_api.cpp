@@ -1032,32 +1032,93 @@ int umcp_bd_steps_rng_dev(umcp_surface *s, double *d_r, int nsteps, const umcp_b
     return UMCP_OK;
 }
 
+namespace {
+
+// point mass (src/hd.C:449-455): two leaflets of POPC (MSCALE 5) over the
+// reference area, spread over the lattice points of all faces
+double effm_mass_per_point(const umcp_surface *s)
+{
+    const double kAmuPerKg = 1.0 / 1.661e-27;
+    const double mass_per_lipid = 5 * 1000 * 760.09 / 6.022e23 / 1000;
+    const double area_per_lipid = 65.35;
+    return kAmuPerKg * 2 * s->A0_setg0 / s->P.nfaces / umcp::effm_points_per_face() * mass_per_lipid /
+           area_per_lipid;
+}
+
+// drop an earlier build (either form)
+void effm_release(umcp_surface *s)
+{
+    (void)hipStreamSynchronize(s->stream);
+    for (void *p : s->effm_allocs) {
+        auto it = std::find(s->allocations.begin(), s->allocations.end(), p);
+        if (it == s->allocations.end()) continue;   // carved from the arena
+        (void)hipFree(p);
+        s->allocations.erase(it);
+    }
+    s->effm_allocs.clear();
+    s->effm_ready = false;
+    s->effm_solve = false;
+    s->d_effm = nullptr;
+    s->d_effm_rowptr = nullptr;
+    s->d_effm_col = nullptr;
+    s->d_effm_val = nullptr;
+    s->d_ell_col = nullptr;
+    s->d_ell_val = s->d_dinv = s->d_cheb_r = s->d_cheb_d0 = s->d_cheb_d1 = nullptr;
+    s->d_pp_next = s->d_mp = nullptr;
+}
+
+// x = Mi b of the scalable form: the fixed-length Chebyshev solve of M x = b
+// (b and x device arrays [nv][3]; x must not alias b)
+hipError_t effm_solve(umcp_surface *s, const double *b, double *x, hipStream_t st)
+{
+    using namespace umcp;
+    ChebArgs a{};
+    a.n = s->T.nv;
+    a.width = s->effm_ell_width;
+    a.col = s->d_ell_col;
+    a.val = s->d_ell_val;
+    a.dinv = s->d_dinv;
+    a.x = x;
+    a.r = s->d_cheb_r;
+    hipError_t e = launch_cheb_init(a, b, s->d_cheb_d0, s->cheb_inv_theta, st);
+    double *d[2] = {s->d_cheb_d0, s->d_cheb_d1};
+    for (size_t k = 0; k < s->cheb_c1.size() && e == hipSuccess; k++)
+        e = launch_cheb_step(a, d[k & 1], d[(k + 1) & 1], s->cheb_c1[k], s->cheb_c2[k], st);
+    return e;
+}
+
+// out = Mi v on the device for either form (v, out [nv][3], distinct)
+hipError_t effm_apply_dev(umcp_surface *s, const double *v, double *out, hipStream_t st)
+{
+    using namespace umcp;
+    if (s->effm_solve) return effm_solve(s, v, out, st);
+    EffmArgs ea{};
+    ea.nv = s->T.nv;
+    ea.Mi = s->d_effm;
+    ea.rowptr = s->d_effm_rowptr;
+    ea.col = s->d_effm_col;
+    ea.val = s->d_effm_val;
+    ea.mp = out;
+    return launch_effm_product(ea, v, st);
+}
+
+}  // namespace
+
 int umcp_effm_build(umcp_surface *s, long *nnz, double *point_mass)
 {
     int rc = check_ready(s);
     if (rc) return rc;
     const int n = s->T.nv;
     if (n > UMCP_EFFM_MAX_NV)
-        return fail(s, UMCP_ERR_ARG, "effective-mass integrators hold a dense nv x nv inverse: at most "
-                                     "UMCP_EFFM_MAX_NV vertices");
-    // point mass (src/hd.C:449-455): two leaflets of POPC (MSCALE 5) over the
-    // reference area, spread over the lattice points of all faces
-    const double kAmuPerKg = 1.0 / 1.661e-27;
-    const double mass_per_lipid = 5 * 1000 * 760.09 / 6.022e23 / 1000;
-    const double area_per_lipid = 65.35;
-    const double m = kAmuPerKg * 2 * s->A0_setg0 / s->P.nfaces / effm_points_per_face() * mass_per_lipid /
-                     area_per_lipid;
+        return fail(s, UMCP_ERR_ARG, "the dense effective-mass inverse holds nv x nv doubles: at most "
+                                     "UMCP_EFFM_MAX_NV vertices (umcp_effm_build_sparse has no limit)");
+    const double m = effm_mass_per_point(s);
     std::vector<int> rowptr, col;
     std::vector<double> val;
     std::string err;
     if (effm_mass_csr(s->T, s->P, m, rowptr, col, val, err)) return fail(s, UMCP_ERR_BAD_MESH, err);
     // replace an earlier build
-    for (void *p : s->effm_allocs) {
-        (void)hipFree(p);
-        s->allocations.erase(std::find(s->allocations.begin(), s->allocations.end(), p));
-    }
-    s->effm_allocs.clear();
-    s->effm_ready = false;
+    effm_release(s);
     const size_t before = s->allocations.size();
     if ((rc = dev_alloc(s, &s->d_effm, (size_t)n * n)) || (rc = dev_alloc(s, &s->d_pp_next, 3 * (size_t)n)) ||
         (rc = dev_alloc(s, &s->d_mp, 3 * (size_t)n)))
@@ -1163,10 +1224,124 @@ int umcp_effm_build(umcp_surface *s, long *nnz, double *point_mass)
     return UMCP_OK;
 }
 
+int umcp_effm_build_sparse(umcp_surface *s, double tol, long *nnz, double *point_mass, int *iterations)
+{
+    int rc = check_ready(s);
+    if (rc) return rc;
+    if (tol == 0.0) tol = 1e-12;
+    if (!(tol >= 1e-14 && tol <= 1e-3)) return fail(s, UMCP_ERR_ARG, "solve tolerance outside [1e-14, 1e-3]");
+    const int n = s->T.nv;
+    const double m = effm_mass_per_point(s);
+    std::vector<int> rowptr, col;
+    std::vector<double> val;
+    std::string err;
+    if (effm_mass_csr(s->T, s->P, m, rowptr, col, val, err)) return fail(s, UMCP_ERR_BAD_MESH, err);
+    double lo = 0.0, hi = 0.0;
+    if (effm_jacobi_interval(rowptr, col, val, lo, hi))
+        return fail(s, UMCP_ERR_BAD_MESH, "effective mass matrix is not positive definite");
+    int width = 0;
+    std::vector<int> ell_col;
+    std::vector<double> ell_val, dinv;
+    effm_ell(rowptr, col, val, width, ell_col, ell_val, dinv);
+    effm_release(s);
+    const size_t before = s->allocations.size();
+    const size_t n3 = 3 * (size_t)n;
+    if ((rc = dev_alloc(s, &s->d_pp_next, n3)) || (rc = dev_alloc(s, &s->d_mp, n3)) ||
+        (rc = dev_alloc(s, &s->d_ell_col, ell_col.size())) || (rc = dev_alloc(s, &s->d_ell_val, ell_val.size())) ||
+        (rc = dev_alloc(s, &s->d_dinv, (size_t)n)) || (rc = dev_alloc(s, &s->d_cheb_r, n3)) ||
+        (rc = dev_alloc(s, &s->d_cheb_d0, n3)) || (rc = dev_alloc(s, &s->d_cheb_d1, n3))) {
+        s->effm_allocs.assign(s->allocations.begin() + (long)before, s->allocations.end());
+        effm_release(s);
+        return rc;
+    }
+    s->effm_allocs.assign(s->allocations.begin() + (long)before, s->allocations.end());
+    hipError_t e = hipMemcpyAsync(s->d_ell_col, ell_col.data(), sizeof(int) * ell_col.size(), hipMemcpyHostToDevice,
+                                  s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s->d_ell_val, ell_val.data(), sizeof(double) * ell_val.size(), hipMemcpyHostToDevice,
+                           s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s->d_dinv, dinv.data(), sizeof(double) * dinv.size(), hipMemcpyHostToDevice, s->stream);
+    s->effm_ell_width = width;
+    s->effm_solve = true;
+    s->cheb_lo = lo;
+    s->cheb_hi = hi;
+    // the iteration count of the Chebyshev bound, checked on a pseudo-random
+    // right-hand side: the residual of the device solve, formed on the host
+    // from the CSR, must fall below tol (else 1.5 times the iterations, up to
+    // four tries)
+    std::vector<double> b(n3), x(n3);
+    unsigned long long z = 0x2545F4914F6CDD1Dull;
+    double bn = 0.0;
+    for (double &q : b) {
+        z ^= z << 13, z ^= z >> 7, z ^= z << 17;
+        q = (double)(z >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+        bn += q * q;
+    }
+    bn = std::sqrt(bn);
+    int iters = effm_cheb_iterations(lo, hi, tol);
+    double res = 0.0;
+    for (int attempt = 0; attempt < 4 && e == hipSuccess; attempt++) {
+        effm_cheb_coeffs(lo, hi, iters, s->cheb_c1, s->cheb_c2, s->cheb_inv_theta);
+        e = hipMemcpyAsync(s->d_pp_next, b.data(), sizeof(double) * n3, hipMemcpyHostToDevice, s->stream);
+        if (e == hipSuccess) e = effm_solve(s, s->d_pp_next, s->d_mp, s->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(x.data(), s->d_mp, sizeof(double) * n3, hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) break;
+        double rn = 0.0;
+        for (int i = 0; i < n; i++)
+            for (int c = 0; c < 3; c++) {
+                double q = b[3 * (size_t)i + c];
+                for (int k = rowptr[i]; k < rowptr[i + 1]; k++) q -= val[k] * x[3 * (size_t)col[k] + c];
+                rn += q * q;
+            }
+        res = std::sqrt(rn) / bn;
+        if (res <= tol) break;
+        iters = (iters * 3 + 1) / 2;
+    }
+    if (e != hipSuccess) {
+        effm_release(s);
+        return hip_fail(s, e, "effective mass solve set-up");
+    }
+    if (!(res <= tol)) {
+        effm_release(s);
+        return fail(s, UMCP_ERR_BAD_MESH, "Chebyshev solve of the effective mass did not reach the tolerance "
+                                          "(residual " + std::to_string(res) + ")");
+    }
+    s->effm_nnz = (long)val.size();
+    s->effm_point_mass = m;
+    s->effm_ready = true;
+    if (nnz) *nnz = s->effm_nnz;
+    if (point_mass) *point_mass = m;
+    if (iterations) *iterations = (int)s->cheb_c1.size();
+    return UMCP_OK;
+}
+
+int umcp_effm_apply(umcp_surface *s, const double *v, double *out)
+{
+    if (!s || !v || !out) return s ? fail(s, UMCP_ERR_ARG, "null argument") : UMCP_ERR_ARG;
+    if (!s->effm_ready) return fail(s, UMCP_ERR_STATE, "effective mass not built (call umcp_effm_build)");
+    if (hipSetDevice(s->device) != hipSuccess) return fail(s, UMCP_ERR_DEVICE, "hipSetDevice failed");
+    const size_t n3 = 3 * (size_t)s->T.nv;
+    double *d_v = nullptr, *d_o = nullptr;
+    hipError_t e = hipMalloc(&d_v, sizeof(double) * n3);
+    if (e == hipSuccess) e = hipMalloc(&d_o, sizeof(double) * n3);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_v, v, sizeof(double) * n3, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = effm_apply_dev(s, d_v, d_o, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_o, sizeof(double) * n3, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(d_v);
+    (void)hipFree(d_o);
+    if (e != hipSuccess) return hip_fail(s, e, "effective mass product");
+    return UMCP_OK;
+}
+
 int umcp_effm_get(umcp_surface *s, double *Minv)
 {
     if (!s || !Minv) return UMCP_ERR_ARG;
     if (!s->effm_ready) return fail(s, UMCP_ERR_STATE, "effective mass not built (call umcp_effm_build)");
+    if (s->effm_solve)
+        return fail(s, UMCP_ERR_STATE, "the sparse form keeps no inverse (products: umcp_effm_apply)");
     const size_t n = (size_t)s->T.nv;
     if (s->d_effm) {
         hipError_t e = hipMemcpy(Minv, s->d_effm, sizeof(double) * n * n, hipMemcpyDeviceToHost);
@@ -1215,8 +1390,8 @@ int umcp_effm_steps_dev(umcp_surface *s, double *d_r, double *d_pp, const double
     hipError_t e = hipSuccess;
     if (lang && nsteps > 0) {
         // the first step's friction product from the caller's momenta; later
-        // steps reuse the product the position kernel forms
-        e = launch_effm_product(ea, d_pp, st);
+        // steps reuse the product the position kernel (or the solve) forms
+        e = effm_apply_dev(s, d_pp, s->d_mp, st);
         if (e != hipSuccess) return hip_fail(s, e, "effective-mass product launch");
     }
     for (int k = 0; k < nsteps; k++) {
@@ -1242,7 +1417,14 @@ int umcp_effm_steps_dev(umcp_surface *s, double *d_r, double *d_pp, const double
         ga.effm_kT = p->kT;
         ga.effm_gamma = p->gamma;
         if ((rc = enqueue_eval(s, d_r, st, ga, s->d_step))) return rc;
-        e = launch_effm_position(ea, st);
+        if (s->effm_solve) {
+            // mp = Mi pp_next by the solve (the gather above has read the
+            // previous mp), then the positions from it
+            e = effm_solve(s, s->d_pp_next, s->d_mp, st);
+            if (e == hipSuccess) e = launch_effm_position_solved(ea, st);
+        } else {
+            e = launch_effm_position(ea, st);
+        }
         if (e != hipSuccess) return hip_fail(s, e, "effective-mass step launch");
     }
     return UMCP_OK;

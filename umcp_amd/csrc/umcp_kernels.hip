@@ -541,6 +541,92 @@ hipError_t launch_effm_product(const EffmArgs &a, const double *pp, hipStream_t 
     return hipGetLastError();
 }
 
+// Chebyshev solve (the scalable effective mass).  Contraction off, as in the
+// products above; every sum runs in the rows' column order.
+__global__ __launch_bounds__(kChebBlock) void cheb_init_kernel(ChebArgs a, const double *b, double *d0,
+                                                               double inv_theta)
+{
+    #pragma clang fp contract(off)
+    const int i = blockIdx.x * kChebBlock + (int)threadIdx.x;
+    if (i >= a.n) return;
+    const double di = a.dinv[i];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double bc = b[3 * (size_t)i + c];
+        a.r[3 * (size_t)i + c] = bc;
+        a.x[3 * (size_t)i + c] = 0.0;
+        d0[3 * (size_t)i + c] = di * bc * inv_theta;
+    }
+}
+
+__global__ __launch_bounds__(kChebBlock) void cheb_step_kernel(ChebArgs a, const double *__restrict__ d_in,
+                                                               double *__restrict__ d_out, double c1, double c2)
+{
+    #pragma clang fp contract(off)
+    const int i = blockIdx.x * kChebBlock + (int)threadIdx.x;
+    if (i >= a.n) return;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    const size_t n = (size_t)a.n;
+#pragma unroll 8
+    for (int k = 0; k < a.width; k++) {
+        const int j = a.col[k * n + i];
+        const double m = a.val[k * n + i];
+        s0 += m * d_in[3 * (size_t)j];
+        s1 += m * d_in[3 * (size_t)j + 1];
+        s2 += m * d_in[3 * (size_t)j + 2];
+    }
+    const double di = a.dinv[i];
+    const double s[3] = {s0, s1, s2};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const size_t q = 3 * (size_t)i + c;
+        const double dk = d_in[q];
+        a.x[q] += dk;
+        const double rr = a.r[q] - s[c];
+        a.r[q] = rr;
+        d_out[q] = c1 * dk + c2 * (di * rr);
+    }
+}
+
+__global__ __launch_bounds__(kChebBlock) void effm_position_solved_kernel(EffmArgs a)
+{
+    #pragma clang fp contract(off)
+    const int v = blockIdx.x * kChebBlock + (int)threadIdx.x;
+    if (v >= a.nv) return;
+    const double al[3] = {a.r[3 * a.nv], a.r[3 * a.nv + 1], a.r[3 * a.nv + 2]};
+    const bool lang = a.mode == kEffmLangevin;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        a.r[3 * (size_t)v + c] += a.mp[3 * (size_t)v + c] * (1.0 / (al[c] * al[c])) * a.akma * a.dt;
+        a.pp_keep[3 * (size_t)v + c] = lang ? a.pp_next[3 * (size_t)v + c] : 0.0;
+    }
+}
+
+hipError_t launch_cheb_init(const ChebArgs &a, const double *b, double *d0, double inv_theta, hipStream_t st)
+{
+    const int nb = (a.n + kChebBlock - 1) / kChebBlock;
+    if (nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(cheb_init_kernel, dim3(nb), dim3(kChebBlock), 0, st, a, b, d0, inv_theta);
+    return hipGetLastError();
+}
+
+hipError_t launch_cheb_step(const ChebArgs &a, const double *d_in, double *d_out, double c1, double c2,
+                            hipStream_t st)
+{
+    const int nb = (a.n + kChebBlock - 1) / kChebBlock;
+    if (nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(cheb_step_kernel, dim3(nb), dim3(kChebBlock), 0, st, a, d_in, d_out, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t launch_effm_position_solved(const EffmArgs &a, hipStream_t st)
+{
+    const int nb = (a.nv + kChebBlock - 1) / kChebBlock;
+    if (nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(effm_position_solved_kernel, dim3(nb), dim3(kChebBlock), 0, st, a);
+    return hipGetLastError();
+}
+
 template <typename GB>
 __global__ __launch_bounds__(kGatherBlock) void vertex_gather(GatherArgs a)
 {

@@ -156,6 +156,26 @@ struct EffmArgs {
 hipError_t launch_effm_position(const EffmArgs &a, hipStream_t st);
 // mp = Mi pp (the first step's friction product)
 hipError_t launch_effm_product(const EffmArgs &a, const double *pp, hipStream_t st);
+// The scalable form: Mi pp by a Chebyshev solve of M q = pp (D^-1
+// preconditioned, a fixed number of iterations), M in ELL layout, one thread
+// per vertex, the three Cartesian components at once.  Init: r = b, x = 0,
+// d_0 = D^-1 b / theta; step k: x += d_k, r -= M d_k,
+// d_{k+1} = c1_k d_k + c2_k D^-1 r.  The step reads d_k of the neighbours and
+// writes d_{k+1} to the other buffer.
+constexpr int kChebBlock = 256;
+struct ChebArgs {
+    int n, width;
+    const int *col;         // [width][n], rows padded with their own index
+    const double *val;      // [width][n], padding zero
+    const double *dinv;     // 1 / M_ii
+    double *x, *r;          // [n][3]
+};
+hipError_t launch_cheb_init(const ChebArgs &a, const double *b, double *d0, double inv_theta, hipStream_t st);
+hipError_t launch_cheb_step(const ChebArgs &a, const double *d_in, double *d_out, double c1, double c2,
+                            hipStream_t st);
+// r += (Mi pp_next) / alpha^2 Delta with Mi pp_next already solved into mp,
+// and pp_keep (thread per vertex)
+hipError_t launch_effm_position_solved(const EffmArgs &a, hipStream_t st);
 // set-up of the dense inverse: M from CSR into a zeroed nv x nv array; after
 // the in-place potrf + potri (upper triangle, column-major), the full
 // symmetric inverse; then the reference's truncation with per-row counts of

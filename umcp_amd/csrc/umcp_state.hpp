@@ -112,6 +112,16 @@ struct umcp_surface {
     double *d_effm_val = nullptr;
     double *d_pp_next = nullptr, *d_mp = nullptr;
     long effm_nnz = 0;
+    // the scalable form (umcp_effm_build_sparse): M itself in ELL layout and
+    // D^-1; products Mi v are Chebyshev solves (residual buffer, two
+    // direction buffers, the iteration's coefficients)
+    bool effm_solve = false;
+    int effm_ell_width = 0;
+    int *d_ell_col = nullptr;
+    double *d_ell_val = nullptr, *d_dinv = nullptr;
+    double *d_cheb_r = nullptr, *d_cheb_d0 = nullptr, *d_cheb_d1 = nullptr;
+    std::vector<double> cheb_c1, cheb_c2;
+    double cheb_inv_theta = 0.0, cheb_lo = 0.0, cheb_hi = 0.0;
     std::vector<void *> effm_allocs;
     // per-face energies (umcp_face_energies, umcp_energy_vertex): faces in
     // reference order as (class, index), each vertex's faces as

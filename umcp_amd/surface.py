@@ -47,6 +47,7 @@ EXPORTS = [
     "umcp_clone", "umcp_device_bytes", "umcp_server_create", "umcp_server_submit", "umcp_server_wait",
     "umcp_server_drain", "umcp_server_stats", "umcp_server_last_error", "umcp_server_destroy",
     "umcp_effm_build", "umcp_effm_get", "umcp_effm_steps", "umcp_effm_steps_dev",
+    "umcp_effm_build_sparse", "umcp_effm_apply",
     "umcp_save_mesh", "umcp_write_restart", "umcp_read_restart", "umcp_write_limit_surface",
     "umcp_point_eval", "umcp_point_grad", "umcp_point_geometry", "umcp_write_limit_psf",
     "umcp_face_energies", "umcp_energy_vertex", "umcp_area_vertex", "umcp_next_face", "umcp_point_coeffs",
@@ -161,6 +162,8 @@ def load_library(path: str = LIB_PATH):
         "umcp_read_restart": ([C.c_char_p, C.c_int, dp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
         "umcp_effm_build": ([vp, C.POINTER(C.c_long), dp], C.c_int),
         "umcp_effm_get": ([vp, dp], C.c_int),
+        "umcp_effm_build_sparse": ([vp, C.c_double, C.POINTER(C.c_long), dp, C.POINTER(C.c_int)], C.c_int),
+        "umcp_effm_apply": ([vp, dp, dp], C.c_int),
         "umcp_effm_steps": ([vp, dp, dp, dp, C.c_int, C.POINTER(EffmParams), dp], C.c_int),
         "umcp_effm_steps_dev": ([vp, vp, vp, vp, C.c_int, C.POINTER(EffmParams), C.c_ulonglong, vp, vp],
                                 C.c_int),
@@ -519,6 +522,24 @@ class Surface:
         nnz, m = C.c_long(), C.c_double()
         self._check(self._L.umcp_effm_build(self._h, C.byref(nnz), C.byref(m)), "effm_build")
         return nnz.value, m.value
+
+    def effm_build_sparse(self, tol: float = 0.0):
+        """The scalable effective mass: M kept sparse, products by a Chebyshev
+        solve to relative residual tol (0: 1e-12).  Returns (nnz of M, point
+        mass, iterations per solve)."""
+        nnz, m, it = C.c_long(), C.c_double(), C.c_int()
+        self._check(self._L.umcp_effm_build_sparse(self._h, float(tol), C.byref(nnz), C.byref(m), C.byref(it)),
+                    "effm_build_sparse")
+        return nnz.value, m.value, it.value
+
+    def effm_apply(self, v: np.ndarray) -> np.ndarray:
+        """Mi v for v of shape (nv, 3) or (3 nv,) with the built form."""
+        x = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        if x.size != 3 * self.nv:
+            raise ValueError("effm_apply: need 3 nv values")
+        out = np.empty_like(x)
+        self._check(self._L.umcp_effm_apply(self._h, _dp(x), _dp(out)), "effm_apply")
+        return out.reshape(np.shape(v))
 
     def effm_get(self) -> np.ndarray:
         Mi = np.zeros((self.nv, self.nv))
