@@ -505,6 +505,7 @@ def run_c2_leg(args, dev, torch):
         d_e = torch.zeros(K, device=dev, dtype=torch.float64)
         s.set_step(0)
         s.effm_steps_dev(d_r, d_pp, 3, S.EFFM_LD, dt, kT, gamma, seed=12, stream=sptr)   # warm-up
+        torch.cuda.synchronize()   # the reset below runs on torch's stream
         d_r.copy_(torch.tensor(w.r, device=dev, dtype=torch.float64))
         d_pp.zero_()
         s.set_step(0)
@@ -522,6 +523,11 @@ def run_c2_leg(args, dev, torch):
                       "r_end": d_r.cpu().numpy()}
         if prec == "fp64" and not args.no_cpu_baseline:
             Mi = s.effm_get()
+        if prec == "fp64":
+            # the scalable form on the same steps (M sparse, Mi pp by the
+            # Chebyshev solve)
+            legs["fp64_sparse_mass"] = _effm_sparse_run(s, w, dt, K, kT, gamma, dev, torch, stream,
+                                                        r_ref=legs["fp64"]["r_end"], seed=12)
         s.close()
         del d_r, d_pp, d_e
     out["dt_trials"] = trials
@@ -532,6 +538,7 @@ def run_c2_leg(args, dev, torch):
                                                    legs["fp64"]["energy_first_last"][1] - 1)
     legs["fp64"]["displacement_A"] = float(np.abs(r64 - w.r).max())
     out["fp64"], out["mixed"] = legs["fp64"], legs["mixed"]
+    out["fp64_sparse_mass"] = legs["fp64_sparse_mass"]
     out["mixed_speedup_vs_fp64"] = legs["fp64"]["ms_per_step"] / legs["mixed"]["ms_per_step"]
     if not args.no_cpu_baseline:
         from oracle import oracle as O
@@ -625,6 +632,44 @@ def run_points_leg(args, dev, torch):
     return out
 
 
+def _effm_sparse_run(s, w, dt, K, kT, gamma, dev, torch, stream, r_ref=None, seed=7):
+    """K device-noise Langevin steps (after 3 warm-up steps, from the start
+    positions and step 0) with the scalable effective mass
+    (umcp_effm_build_sparse) on surface s; set-up time, ms per step, the
+    solve's length and, given the dense form's end positions for the same
+    steps, the largest difference to them."""
+    import numpy as np
+    from umcp_amd import surface as S
+    sptr = stream.cuda_stream
+    t0 = time.perf_counter()
+    nnz, m, iters = s.effm_build_sparse()
+    setup = time.perf_counter() - t0
+    d_r = torch.tensor(w.r, device=dev, dtype=torch.float64)
+    d_pp = torch.zeros(3 * s.nv, device=dev, dtype=torch.float64)
+    s.set_step(0)
+    s.effm_steps_dev(d_r, d_pp, 3, S.EFFM_LD, dt, kT, gamma, seed=seed, stream=sptr)   # warm-up
+    torch.cuda.synchronize()   # the reset below runs on torch's stream
+    d_r.copy_(torch.tensor(w.r, device=dev, dtype=torch.float64))
+    d_pp.zero_()
+    s.set_step(0)
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    s.effm_steps_dev(d_r, d_pp, K, S.EFFM_LD, dt, kT, gamma, seed=seed, stream=sptr)
+    end.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / K
+    out = {"nv": s.nv, "mass_nnz": nnz, "solve_iterations": iters, "setup_s": setup, "dt_s": dt,
+           "langevin_steps": K, "langevin_ms_per_step": ms, "finite": bool(torch.isfinite(d_r).all()),
+           # M's entries (8 + 4 bytes) and per row x, r read and written, d
+           # read and written, 1/M_pp: 152 bytes
+           "bytes_per_solve_iteration": nnz * 12 + s.nv * 152}
+    if r_ref is not None:
+        out["max_dr_vs_dense_A"] = float(np.abs(d_r.cpu().numpy() - r_ref).max())
+        out["displacement_A"] = float(np.abs(r_ref - w.r).max())
+    return out
+
+
 def run_effm_leg(args, dev, torch):
     """Momentum-space integrators of src/hd.C with the effective-mass matrix
     on the 642-vertex vesicle (the size the reference's O(nv^3) set-up
@@ -679,6 +724,10 @@ def run_effm_leg(args, dev, torch):
     sl.set_step(0)
     dt_l, K_l = 1e-12, max(20, K // 5)
     sl.effm_steps_dev(d_r, d_pp, 3, S.EFFM_LD, dt_l, kT, gamma, seed=7, stream=sptr)
+    torch.cuda.synchronize()   # the reset below runs on torch's stream
+    d_r.copy_(torch.tensor(wl.r, device=dev, dtype=torch.float64))
+    d_pp.zero_()
+    sl.set_step(0)
     torch.cuda.synchronize()
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     start.record(stream)
@@ -692,8 +741,22 @@ def run_effm_leg(args, dev, torch):
                             "langevin_ms_per_step": ms_l, "inverse_bytes_per_product": bytes_l,
                             "product_GB_per_s_upper_bound": bytes_l / (ms_l * 1e-3) / 1e9,
                             "finite": bool(torch.isfinite(d_r).all())}
+    r_dense = d_r.cpu().numpy()
+    # the scalable form on the same surface and steps: M sparse, Mi pp by the
+    # Chebyshev solve
+    out["vesicle_40962"]["sparse_mass"] = _effm_sparse_run(sl, wl, dt_l, K_l, kT, gamma, dev, torch, stream,
+                                                           r_ref=r_dense)
     sl.close()
     del d_r, d_pp
+    # beyond the dense form's limit: 163842 vertices (the dense inverse would
+    # be 215 GB)
+    wx = workloads.vesicle(7, 1600.0, 0.5, seed=12, name="effm_ves163842")
+    sx = S.Surface(wx.mesh_path)
+    sx.set_params(precision=S.PREC_FP64, **wx.params)
+    sx.generatePlan()
+    sx.setg0(wx.r0)
+    out["vesicle_163842"] = _effm_sparse_run(sx, wx, dt_l, max(10, K_l // 2), kT, gamma, dev, torch, stream)
+    sx.close()
     if not args.no_cpu_baseline:
         from oracle import oracle as O
         o = O.OracleSurface(w.mesh_path)

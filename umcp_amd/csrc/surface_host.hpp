@@ -78,12 +78,17 @@ int build_plan(const Topology &T, Plan &P, std::string &err);
 int effm_points_per_face();
 int effm_mass_csr(const Topology &T, const Plan &P, double mass_per_pt, std::vector<int> &rowptr,
                   std::vector<int> &col, std::vector<double> &val, std::string &err);
-// the Chebyshev solve of M x = b (umcp_effm_build_sparse): M in ELL layout
-// (slot-major [width][n], rows padded with zeros on the diagonal) and D^-1;
+// the Chebyshev solve of M x = b (umcp_effm_build_sparse): a reverse
+// Cuthill-McKee order of the vertices; M in that order in ELL layout
+// (slot-major [width][n], columns ascending, rows padded with zeros on the
+// diagonal) and D^-1;
 // the spectral interval [lo, hi] of D^-1 M (0 on success); the iterations
 // that bring the residual below tol on it; the per-iteration coefficients
+// (rows and columns in the order perm: new index p is vertex perm[p])
+std::vector<int> effm_rcm_order(const std::vector<int> &rowptr, const std::vector<int> &col);
 void effm_ell(const std::vector<int> &rowptr, const std::vector<int> &col, const std::vector<double> &val,
-              int &width, std::vector<int> &ell_col, std::vector<double> &ell_val, std::vector<double> &dinv);
+              const std::vector<int> &perm, int &width, std::vector<int> &ell_col, std::vector<double> &ell_val,
+              std::vector<double> &dinv);
 int effm_jacobi_interval(const std::vector<int> &rowptr, const std::vector<int> &col, const std::vector<double> &val,
                          double &lo, double &hi);
 int effm_cheb_iterations(double lo, double hi, double tol);

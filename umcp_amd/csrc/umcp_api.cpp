@@ -1062,13 +1062,14 @@ void effm_release(umcp_surface *s)
     s->d_effm_rowptr = nullptr;
     s->d_effm_col = nullptr;
     s->d_effm_val = nullptr;
-    s->d_ell_col = nullptr;
-    s->d_ell_val = s->d_dinv = s->d_cheb_r = s->d_cheb_d0 = s->d_cheb_d1 = nullptr;
+    s->d_ell_col = s->d_ell_perm = nullptr;
+    s->d_ell_val = s->d_dinv = s->d_cheb_x = s->d_cheb_r = s->d_cheb_d0 = s->d_cheb_d1 = nullptr;
     s->d_pp_next = s->d_mp = nullptr;
 }
 
-// x = Mi b of the scalable form: the fixed-length Chebyshev solve of M x = b
-// (b and x device arrays [nv][3]; x must not alias b)
+// x = Mi b of the scalable form: the Chebyshev solve of M x = b, K
+// iterations (b and x device arrays [nv][3] in vertex order): K - 1 full
+// steps, then x_K = x_{K-1} + d_{K-1} written back in vertex order
 hipError_t effm_solve(umcp_surface *s, const double *b, double *x, hipStream_t st)
 {
     using namespace umcp;
@@ -1078,12 +1079,15 @@ hipError_t effm_solve(umcp_surface *s, const double *b, double *x, hipStream_t s
     a.col = s->d_ell_col;
     a.val = s->d_ell_val;
     a.dinv = s->d_dinv;
-    a.x = x;
+    a.perm = s->d_ell_perm;
+    a.x = s->d_cheb_x;
     a.r = s->d_cheb_r;
     hipError_t e = launch_cheb_init(a, b, s->d_cheb_d0, s->cheb_inv_theta, st);
     double *d[2] = {s->d_cheb_d0, s->d_cheb_d1};
-    for (size_t k = 0; k < s->cheb_c1.size() && e == hipSuccess; k++)
+    const size_t K = s->cheb_c1.size();
+    for (size_t k = 0; k + 1 < K && e == hipSuccess; k++)
         e = launch_cheb_step(a, d[k & 1], d[(k + 1) & 1], s->cheb_c1[k], s->cheb_c2[k], st);
+    if (e == hipSuccess) e = launch_cheb_final(a, d[(K - 1) & 1], x, st);
     return e;
 }
 
@@ -1242,13 +1246,15 @@ int umcp_effm_build_sparse(umcp_surface *s, double tol, long *nnz, double *point
     int width = 0;
     std::vector<int> ell_col;
     std::vector<double> ell_val, dinv;
-    effm_ell(rowptr, col, val, width, ell_col, ell_val, dinv);
+    const std::vector<int> perm = effm_rcm_order(rowptr, col);
+    effm_ell(rowptr, col, val, perm, width, ell_col, ell_val, dinv);
     effm_release(s);
     const size_t before = s->allocations.size();
     const size_t n3 = 3 * (size_t)n;
     if ((rc = dev_alloc(s, &s->d_pp_next, n3)) || (rc = dev_alloc(s, &s->d_mp, n3)) ||
         (rc = dev_alloc(s, &s->d_ell_col, ell_col.size())) || (rc = dev_alloc(s, &s->d_ell_val, ell_val.size())) ||
-        (rc = dev_alloc(s, &s->d_dinv, (size_t)n)) || (rc = dev_alloc(s, &s->d_cheb_r, n3)) ||
+        (rc = dev_alloc(s, &s->d_dinv, (size_t)n)) || (rc = dev_alloc(s, &s->d_ell_perm, (size_t)n)) ||
+        (rc = dev_alloc(s, &s->d_cheb_x, n3)) || (rc = dev_alloc(s, &s->d_cheb_r, n3)) ||
         (rc = dev_alloc(s, &s->d_cheb_d0, n3)) || (rc = dev_alloc(s, &s->d_cheb_d1, n3))) {
         s->effm_allocs.assign(s->allocations.begin() + (long)before, s->allocations.end());
         effm_release(s);
@@ -1262,6 +1268,8 @@ int umcp_effm_build_sparse(umcp_surface *s, double tol, long *nnz, double *point
                            s->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(s->d_dinv, dinv.data(), sizeof(double) * dinv.size(), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s->d_ell_perm, perm.data(), sizeof(int) * perm.size(), hipMemcpyHostToDevice, s->stream);
     s->effm_ell_width = width;
     s->effm_solve = true;
     s->cheb_lo = lo;

@@ -542,20 +542,22 @@ hipError_t launch_effm_product(const EffmArgs &a, const double *pp, hipStream_t 
 }
 
 // Chebyshev solve (the scalable effective mass).  Contraction off, as in the
-// products above; every sum runs in the rows' column order.
+// products above; a row's four lanes take its entries k = q, q + 4, ... in
+// column order and are summed by a fixed butterfly.
 __global__ __launch_bounds__(kChebBlock) void cheb_init_kernel(ChebArgs a, const double *b, double *d0,
                                                                double inv_theta)
 {
     #pragma clang fp contract(off)
-    const int i = blockIdx.x * kChebBlock + (int)threadIdx.x;
-    if (i >= a.n) return;
-    const double di = a.dinv[i];
+    const int p = blockIdx.x * kChebBlock + (int)threadIdx.x;
+    if (p >= a.n) return;
+    const double di = a.dinv[p];
+    const size_t v = (size_t)a.perm[p];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        const double bc = b[3 * (size_t)i + c];
-        a.r[3 * (size_t)i + c] = bc;
-        a.x[3 * (size_t)i + c] = 0.0;
-        d0[3 * (size_t)i + c] = di * bc * inv_theta;
+        const double bc = b[3 * v + c];
+        a.r[3 * (size_t)p + c] = bc;
+        a.x[3 * (size_t)p + c] = 0.0;
+        d0[3 * (size_t)p + c] = di * bc * inv_theta;
     }
 }
 
@@ -563,29 +565,51 @@ __global__ __launch_bounds__(kChebBlock) void cheb_step_kernel(ChebArgs a, const
                                                                double *__restrict__ d_out, double c1, double c2)
 {
     #pragma clang fp contract(off)
-    const int i = blockIdx.x * kChebBlock + (int)threadIdx.x;
-    if (i >= a.n) return;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int q = lane >> 4;   // which of the row's four lanes
+    const int p = blockIdx.x * kChebRows + ((int)threadIdx.x >> 6) * 16 + (lane & 15);
+    const bool live = p < a.n;
     const size_t n = (size_t)a.n;
-#pragma unroll 8
-    for (int k = 0; k < a.width; k++) {
-        const int j = a.col[k * n + i];
-        const double m = a.val[k * n + i];
-        s0 += m * d_in[3 * (size_t)j];
-        s1 += m * d_in[3 * (size_t)j + 1];
-        s2 += m * d_in[3 * (size_t)j + 2];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    if (live) {
+#pragma unroll 4
+        for (int k = q; k < a.width; k += kChebLanes) {
+            const int j = a.col[k * n + p];
+            const double m = a.val[k * n + p];
+            s0 += m * d_in[3 * (size_t)j];
+            s1 += m * d_in[3 * (size_t)j + 1];
+            s2 += m * d_in[3 * (size_t)j + 2];
+        }
     }
-    const double di = a.dinv[i];
+    // every lane of the wave takes part in the butterfly
+    s0 += __shfl_xor(s0, 16, kWave);
+    s1 += __shfl_xor(s1, 16, kWave);
+    s2 += __shfl_xor(s2, 16, kWave);
+    s0 += __shfl_xor(s0, 32, kWave);
+    s1 += __shfl_xor(s1, 32, kWave);
+    s2 += __shfl_xor(s2, 32, kWave);
+    if (!live || q != 0) return;
+    const double di = a.dinv[p];
     const double s[3] = {s0, s1, s2};
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        const size_t q = 3 * (size_t)i + c;
-        const double dk = d_in[q];
-        a.x[q] += dk;
-        const double rr = a.r[q] - s[c];
-        a.r[q] = rr;
-        d_out[q] = c1 * dk + c2 * (di * rr);
+        const size_t o = 3 * (size_t)p + c;
+        const double dk = d_in[o];
+        a.x[o] += dk;
+        const double rr = a.r[o] - s[c];
+        a.r[o] = rr;
+        d_out[o] = c1 * dk + c2 * (di * rr);
     }
+}
+
+__global__ __launch_bounds__(kChebBlock) void cheb_final_kernel(ChebArgs a, const double *d, double *out)
+{
+    #pragma clang fp contract(off)
+    const int p = blockIdx.x * kChebBlock + (int)threadIdx.x;
+    if (p >= a.n) return;
+    const size_t v = (size_t)a.perm[p];
+#pragma unroll
+    for (int c = 0; c < 3; c++) out[3 * v + c] = a.x[3 * (size_t)p + c] + d[3 * (size_t)p + c];
 }
 
 __global__ __launch_bounds__(kChebBlock) void effm_position_solved_kernel(EffmArgs a)
@@ -613,9 +637,17 @@ hipError_t launch_cheb_init(const ChebArgs &a, const double *b, double *d0, doub
 hipError_t launch_cheb_step(const ChebArgs &a, const double *d_in, double *d_out, double c1, double c2,
                             hipStream_t st)
 {
-    const int nb = (a.n + kChebBlock - 1) / kChebBlock;
+    const int nb = (a.n + kChebRows - 1) / kChebRows;
     if (nb == 0) return hipSuccess;
     hipLaunchKernelGGL(cheb_step_kernel, dim3(nb), dim3(kChebBlock), 0, st, a, d_in, d_out, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t launch_cheb_final(const ChebArgs &a, const double *d, double *out, hipStream_t st)
+{
+    const int nb = (a.n + kChebBlock - 1) / kChebBlock;
+    if (nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(cheb_final_kernel, dim3(nb), dim3(kChebBlock), 0, st, a, d, out);
     return hipGetLastError();
 }
 

@@ -157,22 +157,26 @@ hipError_t launch_effm_position(const EffmArgs &a, hipStream_t st);
 // mp = Mi pp (the first step's friction product)
 hipError_t launch_effm_product(const EffmArgs &a, const double *pp, hipStream_t st);
 // The scalable form: Mi pp by a Chebyshev solve of M q = pp (D^-1
-// preconditioned, a fixed number of iterations), M in ELL layout, one thread
-// per vertex, the three Cartesian components at once.  Init: r = b, x = 0,
-// d_0 = D^-1 b / theta; step k: x += d_k, r -= M d_k,
-// d_{k+1} = c1_k d_k + c2_k D^-1 r.  The step reads d_k of the neighbours and
-// writes d_{k+1} to the other buffer.
-constexpr int kChebBlock = 256;
+// preconditioned, a fixed number of iterations), the three Cartesian
+// components at once, in a reverse Cuthill-McKee order of the vertices
+// (perm[p] is the vertex of row p).  Init: r = b (permuted), x = 0,
+// d_0 = D^-1 r / theta.  Step k: x += d_k, r -= M d_k,
+// d_{k+1} = c1_k d_k + c2_k D^-1 r (reads d_k of the neighbours, writes d_{k+1}
+// to the other buffer; four lanes per row split the row's entries).
+// Final: out = x + d in vertex order.
+constexpr int kChebBlock = 256, kChebLanes = 4, kChebRows = kChebBlock / kChebLanes;
 struct ChebArgs {
     int n, width;
-    const int *col;         // [width][n], rows padded with their own index
+    const int *col;         // [width][n] in row order, rows padded with their own index
     const double *val;      // [width][n], padding zero
-    const double *dinv;     // 1 / M_ii
-    double *x, *r;          // [n][3]
+    const double *dinv;     // 1 / M_pp
+    const int *perm;        // vertex of row p
+    double *x, *r;          // [n][3] in row order
 };
 hipError_t launch_cheb_init(const ChebArgs &a, const double *b, double *d0, double inv_theta, hipStream_t st);
 hipError_t launch_cheb_step(const ChebArgs &a, const double *d_in, double *d_out, double c1, double c2,
                             hipStream_t st);
+hipError_t launch_cheb_final(const ChebArgs &a, const double *d, double *out, hipStream_t st);
 // r += (Mi pp_next) / alpha^2 Delta with Mi pp_next already solved into mp,
 // and pp_keep (thread per vertex)
 hipError_t launch_effm_position_solved(const EffmArgs &a, hipStream_t st);

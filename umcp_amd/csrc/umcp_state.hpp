@@ -117,9 +117,9 @@ struct umcp_surface {
     // direction buffers, the iteration's coefficients)
     bool effm_solve = false;
     int effm_ell_width = 0;
-    int *d_ell_col = nullptr;
+    int *d_ell_col = nullptr, *d_ell_perm = nullptr;
     double *d_ell_val = nullptr, *d_dinv = nullptr;
-    double *d_cheb_r = nullptr, *d_cheb_d0 = nullptr, *d_cheb_d1 = nullptr;
+    double *d_cheb_x = nullptr, *d_cheb_r = nullptr, *d_cheb_d0 = nullptr, *d_cheb_d1 = nullptr;
     std::vector<double> cheb_c1, cheb_c2;
     double cheb_inv_theta = 0.0, cheb_lo = 0.0, cheb_hi = 0.0;
     std::vector<void *> effm_allocs;
