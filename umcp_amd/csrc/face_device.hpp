@@ -627,6 +627,47 @@ __device__ __forceinline__ R point_energy_grad(const R X[5][3], R w, R g0, R ig0
     return e;
 }
 
+// One point's reverse into the control points' gradient G:
+// G[c][k] += sum_d TB[c][d] D[d][k], D = dE/d(Ru, Rv, Ruu, Ruv, Rvv).
+template <typename R, int NCP>
+__device__ __forceinline__ void reverse_point(R G[NCP][3], const R *TB, const PointGrad<R> &pg)
+{
+    if constexpr (sizeof(R) == 4) {
+        // FP32: the second-derivative rows factored, sum_d TB[c][d]
+        // D[d][k] = TB0 du_k + TB1 dv_k + (TB2 dL + TB3 dM + TB4 dN) n_k
+        // (12 multiply-adds per control point instead of 15; measured 5 %
+        // faster on the vesicle, 14 % on the sheet)
+#pragma unroll
+        for (int c = 0; c < NCP; c++) {
+            const R sn = TB[c * 5 + 2] * pg.dl + TB[c * 5 + 3] * pg.dm + TB[c * 5 + 4] * pg.dn;
+#pragma unroll
+            for (int k = 0; k < 3; k++) G[c][k] += TB[c * 5] * pg.du[k] + TB[c * 5 + 1] * pg.dv[k] + sn * pg.n[k];
+        }
+    } else {
+        // FP64: the five rows expanded (the factored form removes the
+        // register spills but measured no faster here)
+        R D[5][3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            D[0][k] = pg.du[k];
+            D[1][k] = pg.dv[k];
+            D[2][k] = pg.dl * pg.n[k];
+            D[3][k] = pg.dm * pg.n[k];
+            D[4][k] = pg.dn * pg.n[k];
+        }
+#pragma unroll
+        for (int c = 0; c < NCP; c++) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                R s = G[c][k];
+#pragma unroll
+                for (int d = 0; d < 5; d++) s += TB[c * 5 + d] * D[d][k];
+                G[c][k] = s;
+            }
+        }
+    }
+}
+
 // Fused energy + gradient of 64 faces by one workgroup of WS waves.
 // Lane l of every wave owns face group*64 + l; wave w evaluates the Gauss
 // points p = w, w + WS, ... so the table row of each point is wave-uniform
@@ -749,7 +790,6 @@ __device__ __forceinline__ void face_group(const FaceArgs &a, const double *__re
     double e = 0.0;
     for (int p = w; p < NPT; p += WS) {
         const R *T = tab + p * 5 * NCP;
-        const R *TB = kind_table_b<KIND, R>() + p * 5 * NCP;
         R X[5][3];
 #pragma unroll
         for (int d = 0; d < 5; d++)
@@ -772,41 +812,7 @@ __device__ __forceinline__ void face_group(const FaceArgs &a, const double *__re
         const R rr0 = FEAT == kFeatGeneral ? (R)a.rurv0[p * nf + fc] : R(0);
         e += (double)point_energy_grad<R, FEAT, KIND == 0>(X, wts[p], (R)a.g0[p * nf + fc], (R)a.ig0[p * nf + fc], rr0,
                                                 lw, ls, lv, c0f, (R)a.dudv, lamA, a.ph, pg);
-        if constexpr (sizeof(R) == 4) {
-            // FP32: the second-derivative rows factored, sum_d TB[c][d]
-            // D[d][k] = TB0 du_k + TB1 dv_k + (TB2 dL + TB3 dM + TB4 dN) n_k
-            // (12 multiply-adds per control point instead of 15; measured 5 %
-            // faster on the vesicle, 14 % on the sheet)
-#pragma unroll
-            for (int c = 0; c < NCP; c++) {
-                const R sn = TB[c * 5 + 2] * pg.dl + TB[c * 5 + 3] * pg.dm + TB[c * 5 + 4] * pg.dn;
-#pragma unroll
-                for (int k = 0; k < 3; k++)
-                    G[c][k] += TB[c * 5] * pg.du[k] + TB[c * 5 + 1] * pg.dv[k] + sn * pg.n[k];
-            }
-        } else {
-            // FP64: the five rows expanded (the factored form removes the
-            // register spills but measured no faster here)
-            R D[5][3];
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                D[0][k] = pg.du[k];
-                D[1][k] = pg.dv[k];
-                D[2][k] = pg.dl * pg.n[k];
-                D[3][k] = pg.dm * pg.n[k];
-                D[4][k] = pg.dn * pg.n[k];
-            }
-#pragma unroll
-            for (int c = 0; c < NCP; c++) {
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    R s = G[c][k];
-#pragma unroll
-                    for (int d = 0; d < 5; d++) s += TB[c * 5 + d] * D[d][k];
-                    G[c][k] = s;
-                }
-            }
-        }
+        reverse_point<R, NCP>(G, kind_table_b<KIND, R>() + p * 5 * NCP, pg);
     }
     if (live) {
         acc[0] += e;
