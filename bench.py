@@ -69,9 +69,11 @@ def kernel_name(precision: str, split: int, flags: int, feat: int = FEAT_HELFRIC
     """The face kernel's symbol as rocprofv3 prints it (one workgroup per item)."""
     if precision == "mixed" and flags & 4:   # matrix-core path
         return "faces_batch_mfma<%d, %d, %d, false>" % (split, 2 if flags & 2 else 1, feat)
+    r = "float" if precision == "mixed" else "double"
+    if flags & 8:   # control points split over the waves
+        return "faces_batch_kernel<%s, %d, true, 1, %d, false, true>" % (r, split, feat)
     occ = (4 if precision == "mixed" else 2) if flags & 2 else 1
-    return "faces_batch_kernel<%s, %d, true, %d, %d, false>" % (
-        "float" if precision == "mixed" else "double", split, occ, feat)
+    return "faces_batch_kernel<%s, %d, true, %d, %d, false, false>" % (r, split, occ, feat)
 
 
 def measured_traffic(name: str):
