@@ -309,71 +309,88 @@ __device__ __forceinline__ void restraint_scalars(const FaceArgs &a, double out[
     if (threadIdx.x == 0) restraint_form(a, red, out);
 }
 
-// The restraint scalars formed by a face block (see restraint_multipliers).
+// The restraint scalars formed by a face block, in two phases so the
+// partials' loads share the block's first memory round trip: load() issues
+// every load (call it before the group's staging), finish() sums them in
+// the finalize's order and forms the scalars (after the staging).  Finalize
+// thread t (of NF) sums groups t, t + NF, ... in turn; face thread i stands
+// in for finalize threads i, i + NT, ...  Up to kFoldMaxGroups groups.
 template <int NT>
-__device__ __forceinline__ void restraint_fold(const FaceArgs &a, int slot, double lam[2])
-{
-    #pragma clang fp contract(off)
-    constexpr int NF = kFinalizeThreads, kU = 8, kR = (NF + NT - 1) / NT;
-    const double2 *part = reinterpret_cast<const double2 *>(a.rpart);
-    // finalize thread t = threadIdx.x + j NT (t < NF) sums groups t, t + NF, ...
-    double sA[kR], sV[kR];
+struct RestraintFold {
+    static constexpr int NF = kFinalizeThreads, kR = (NF + NT - 1) / NT;
+    static constexpr int kG = (kFoldMaxGroups + NF - 1) / NF;
+    double2 v[kR][kG];
+
+    __device__ __forceinline__ void load(const FaceArgs &a)
+    {
+        if (!a.rs || !a.rs_fold) return;   // uniform over the block
+        const double2 *part = reinterpret_cast<const double2 *>(a.rpart);
 #pragma unroll
-    for (int j = 0; j < kR; j++) {
-        const int t = (int)threadIdx.x + j * NT;
-        sA[j] = sV[j] = 0.0;
-        if (t >= NF) continue;
-        for (int i0 = 0; i0 < a.ngroups; i0 += kU * NF) {
-            double2 v[kU];
+        for (int j = 0; j < kR; j++)
 #pragma unroll
-            for (int u = 0; u < kU; u++) v[u] = part[min(i0 + u * NF + t, a.ngroups - 1)];
+            for (int u = 0; u < kG; u++) {
+                const int t = (int)threadIdx.x + j * NT, i = t + u * NF;
+                v[j][u] = part[min(i, a.ngroups - 1)];
+            }
+    }
+
+    // lam = {dE/dA_tot, dE/dV}: from the finalize's rs, or (a.rs_fold) formed
+    // here, and then the slot-0 block also writes rs for the gather and the
+    // host.  Every thread of the block calls this.
+    __device__ __forceinline__ void finish(const FaceArgs &a, int slot, double lam[2])
+    {
+        #pragma clang fp contract(off)
+        lam[0] = lam[1] = 0.0;
+        if (!a.rs) return;
+        if (!a.rs_fold) {
+            lam[0] = a.rs[0];
+            lam[1] = a.rs[1];
+            return;
+        }
+        double sA[kR], sV[kR];
 #pragma unroll
-            for (int u = 0; u < kU; u++)
-                if (i0 + u * NF + t < a.ngroups) {
-                    sA[j] += v[u].x;
-                    sV[j] += v[u].y;
+        for (int j = 0; j < kR; j++) {
+            const int t = (int)threadIdx.x + j * NT;
+            sA[j] = sV[j] = 0.0;
+#pragma unroll
+            for (int u = 0; u < kG; u++)
+                if (t < NF && t + u * NF < a.ngroups) {
+                    sA[j] += v[j][u].x;
+                    sV[j] += v[j][u].y;
                 }
         }
-    }
-    __shared__ double red[NF / kWave][2];
+        __shared__ double red[NF / kWave][2];
 #pragma unroll
-    for (int j = 0; j < kR; j++) {
+        for (int j = 0; j < kR; j++) {
 #pragma unroll
-        for (int off = 1; off < kWave; off <<= 1) {
-            sA[j] += __shfl_xor(sA[j], off, kWave);
-            sV[j] += __shfl_xor(sV[j], off, kWave);
+            for (int off = 1; off < kWave; off <<= 1) {
+                sA[j] += __shfl_xor(sA[j], off, kWave);
+                sV[j] += __shfl_xor(sV[j], off, kWave);
+            }
+            if ((threadIdx.x & (kWave - 1)) == 0 && (int)threadIdx.x + j * NT < NF) {
+                red[(threadIdx.x + j * NT) / kWave][0] = sA[j];
+                red[(threadIdx.x + j * NT) / kWave][1] = sV[j];
+            }
         }
-        if ((threadIdx.x & (kWave - 1)) == 0 && (int)threadIdx.x + j * NT < NF) {
-            red[(threadIdx.x + j * NT) / kWave][0] = sA[j];
-            red[(threadIdx.x + j * NT) / kWave][1] = sV[j];
-        }
-    }
-    __syncthreads();
-    double o[5];
-    restraint_form(a, red, o);
-    lam[0] = o[0];
-    lam[1] = o[1];
-    if (slot == 0 && threadIdx.x == 0)
+        __syncthreads();
+        double o[5];
+        restraint_form(a, red, o);
+        lam[0] = o[0];
+        lam[1] = o[1];
+        if (slot == 0 && threadIdx.x == 0)
 #pragma unroll
-        for (int i = 0; i < 5; i++) a.rs[i] = o[i];
-}
+            for (int i = 0; i < 5; i++) a.rs[i] = o[i];
+    }
+};
 
-// The multipliers a face block uses: written by the restraint finalize, or
-// (a.rs_fold) formed by the block itself from the pre-pass partials, the
-// finalize's sums in the finalize's order (its block size NF), so both give
-// the same bits; the slot-0 block then also writes rs for the gather and the
-// host.  Every thread of the block calls this.
+// The multipliers in one call (the fold's loads not overlapped with
+// anything).  Every thread of the block calls this.
 template <int NT>
 __device__ __forceinline__ void restraint_multipliers(const FaceArgs &a, int slot, double lam[2])
 {
-    lam[0] = lam[1] = 0.0;
-    if (!a.rs) return;
-    if (!a.rs_fold) {
-        lam[0] = a.rs[0];
-        lam[1] = a.rs[1];
-        return;
-    }
-    restraint_fold<NT>(a, slot, lam);
+    RestraintFold<NT> rf;
+    rf.load(a);
+    rf.finish(a, slot, lam);
 }
 
 // The gradient records of one 64-face group: T[(c * 3 + k) * 64 + lane] holds
@@ -701,9 +718,10 @@ __device__ __forceinline__ void face_group(const FaceArgs &a, const double *__re
     const int fc = live ? f : nf - 1;
     const double al64[3] = {r[3 * a.nv], r[3 * a.nv + 1], r[3 * a.nv + 2]};
     const R al[3] = {(R)al64[0], (R)al64[1], (R)al64[2]};
-    // the restraint multipliers first, while few registers are live
-    double lam[2];   // dE/dA_tot, dE/dV
-    restraint_multipliers<WS * kWave>(a, slot, lam);
+    // the restraint partials' loads first (summed after the staging, so
+    // their round trip overlaps the staging's), while few registers are live
+    RestraintFold<WS * kWave> rf;
+    rf.load(a);
     double base[3] = {0.0, 0.0, 0.0};
     R P[PLDS ? 1 : NCP][3];
     if (PLDS) {
@@ -739,6 +757,8 @@ __device__ __forceinline__ void face_group(const FaceArgs &a, const double *__re
             for (int k = 0; k < 3; k++) P[c][k] = (R)(t[c][k] - base[k]);
     }
 #define PV(c, k) (PLDS ? lds[((c) * 3 + (k)) * kWave + lane] : P[PLDS ? 0 : (c)][k])
+    double lam[2];   // dE/dA_tot, dE/dV
+    rf.finish(a, slot, lam);
 
     R G[NCP][3];
 #pragma unroll

@@ -345,9 +345,10 @@ int gather_blocks(int nv);
 // and volume), then the finalize for the surfaces a.fin names: per surface,
 // the partials summed in fixed order into the restraint scalars.  Surfaces of
 // at most kFoldMaxGroups groups fold the finalize into the face kernel
-// instead (each face block sums the partials itself: one load per thread,
-// cheaper than a launch for small surfaces; on the 1280-group vesicle the
-// fold measured 6 us slower than the 4.7 us finalize launch).
+// instead: each face block loads the partials beside its group's staging
+// loads and sums them itself, in the finalize's order.  On the 1281-group
+// vesicle the fold costs the face kernel 3.6 us against the finalize
+// launch's 2.6 (4.2 us kernel), so large surfaces keep the launch.
 constexpr int kFoldMaxGroups = 256;
 hipError_t launch_restraint_prepass(const BatchFaceArgs &a, hipStream_t st);
 
