@@ -567,7 +567,10 @@ __global__ __launch_bounds__(kChebBlock) void cheb_step_kernel(ChebArgs a, const
     #pragma clang fp contract(off)
     const int lane = threadIdx.x & (kWave - 1);
     const int q = lane >> 4;   // which of the row's four lanes
-    const int p = blockIdx.x * kChebRows + ((int)threadIdx.x >> 6) * 16 + (lane & 15);
+    // each XCD takes a contiguous run of rows, so the neighbours' d it reads
+    // (rows nearby in the bandwidth-reducing order) sit in its own L2
+    const int b = xcd_logical_block(blockIdx.x, gridDim.x);
+    const int p = b * kChebRows + ((int)threadIdx.x >> 6) * 16 + (lane & 15);
     const bool live = p < a.n;
     const size_t n = (size_t)a.n;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
