@@ -598,6 +598,27 @@ def run_points_leg(args, dev, torch):
     tg = (time.perf_counter() - t0) / reps
     out["eval_s"] = te
     out["eval_points_per_s"] = n / te
+    # the device-resident evaluation: inputs and outputs in HBM, HIP events
+    # on the stream it runs on (resolve, partition by kind, point kernels)
+    stream = torch.cuda.Stream(device=dev)
+    d_r = torch.tensor(r, device=dev, dtype=torch.float64)
+    d_f = torch.tensor(faces, device=dev, dtype=torch.int32)
+    d_uv = torch.tensor(uv, device=dev, dtype=torch.float64)
+    d_R = torch.empty((n, 3), device=dev, dtype=torch.float64)
+    d_N = torch.empty_like(d_R)
+    s.point_eval_dev(d_r, d_f, d_uv, d_R, d_N, stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    s.point_status()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(10):
+        s.point_eval_dev(d_r, d_f, d_uv, d_R, d_N, stream=stream.cuda_stream)
+    end.record(stream)
+    torch.cuda.synchronize()
+    out["eval_dev_s"] = start.elapsed_time(end) / 10 * 1e-3
+    out["eval_dev_points_per_s"] = n / out["eval_dev_s"]
+    out["eval_dev_equals_host"] = bool(np.array_equal(d_R.cpu().numpy(), R) and np.array_equal(d_N.cpu().numpy(), N))
+    del d_r, d_f, d_uv, d_R, d_N
     out["grad_s"] = tg
     out["grad_points_per_s"] = n / tg
     # the geometry query (ru, rv, g, c)

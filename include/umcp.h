@@ -325,6 +325,20 @@ int umcp_point_eval(umcp_surface *s, const double *r, int n, const int *faces, c
 int umcp_point_grad(umcp_surface *s, const double *r, int n, const int *faces, const double *uv,
                     const double *de_dr, const double *de_dnrm, const double *frac_mult, int on_surface, double kT,
                     double *g, double *g_puv);
+/* umcp_point_eval with every array on the device (d_r 3nv+3, d_faces n,
+ * d_uv 2n, d_R and d_nrm 3n) and no synchronisation: the points are resolved
+ * on the device by the same rules (evaluateRNRM, src/interpLib.C:7936-8196;
+ * the irregular sub-patch level of src/irr_kernel.C:380-541, found with the
+ * device's log10, which may differ from libm's in the last bit: a point
+ * within rounding of a level border may take the adjacent sub-patch, where
+ * the surface agrees to rounding), then evaluated by the same kernels, so
+ * the results equal umcp_point_eval's.  A rejected point gets NaN outputs;
+ * umcp_point_status (after the caller's synchronisation) then returns
+ * UMCP_ERR_ARG naming the first one (*bad_point, else -1).  stream may be
+ * NULL (the surface's stream). */
+int umcp_point_eval_dev(umcp_surface *s, const double *d_r, int n, const int *d_faces, const double *d_uv,
+                        double *d_R, double *d_nrm, void *stream);
+int umcp_point_status(umcp_surface *s, int *bad_point);
 /* Local geometry at the points, replacing surface::ru, surface::rv,
  * surface::g and surface::c (src/uv_map.C:1347-1700, 884-1106, 2361-2990):
  * geom[15 i ..] = Ru[3], Rv[3] (alpha-scaled tangents along the face's u and

@@ -1,0 +1,125 @@
+"""GPU: the device-resident point evaluation (umcp_point_eval_dev) against
+the host-array call (umcp_point_eval), which the points tests pin against the
+oracle (evaluateRNRM, src/interpLib.C:7936-8196).  Both resolve the points by
+the same rules and evaluate them with the same kernels, so the results are
+bitwise equal, also for sub-patch levels 1-40 next to an extraordinary
+vertex (random points do not fall within rounding of a level border, where
+the device's log10 could pick the adjacent level), the vertex itself, a
+periodic sheet and a million points on the 40962-vertex vesicle.  Rejected
+points get NaN outputs, leave the others unchanged and are named by
+umcp_point_status."""
+import sys
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import oracle as O  # noqa: E402
+from umcp_amd import surface as S  # noqa: E402
+from umcp_amd import workloads  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make(w):
+    g = S.Surface(w.mesh_path)
+    g.set_params(precision=S.PREC_FP64, **w.params)
+    g.generatePlan()
+    return g
+
+
+def random_points(nfaces, nreg, n, seed, deep=0, vertex=0):
+    rng = np.random.default_rng(seed)
+    faces = rng.integers(0, nfaces, n)
+    u, v = rng.random(n), rng.random(n)
+    flip = u + v > 1
+    u[flip], v[flip] = 1 - u[flip], 1 - v[flip]
+    uv = np.stack([u, v], 1)
+    irr = np.flatnonzero(faces >= nreg)
+    for k, i in enumerate(irr[:deep]):
+        s = 10.0 ** -(1 + k % 12)
+        uv[i] *= s / max(uv[i].sum(), 1e-300)
+    for i in irr[deep:deep + vertex]:
+        uv[i] = 0.0
+    return faces.astype(np.int32), uv
+
+
+def dev_eval(g, r, faces, uv):
+    d_r = torch.tensor(r, device="cuda", dtype=torch.float64)
+    d_f = torch.tensor(faces, device="cuda", dtype=torch.int32)
+    d_uv = torch.tensor(uv, device="cuda", dtype=torch.float64)
+    d_R = torch.zeros((len(faces), 3), device="cuda", dtype=torch.float64)
+    d_N = torch.zeros_like(d_R)
+    g.point_eval_dev(d_r, d_f, d_uv, d_R, d_N)
+    torch.cuda.synchronize()
+    return d_R.cpu().numpy(), d_N.cpu().numpy()
+
+
+def same(a, b):
+    return np.array_equal(a, b, equal_nan=True)
+
+
+@pytest.mark.parametrize("name", ["vesicle_642", "defect_sheet"])
+def test_device_equals_host(name):
+    w = {"vesicle_642": workloads.vesicle_642, "defect_sheet": lambda: workloads.defect_sheet()}[name]()
+    g = make(w)
+    o = O.OracleSurface(w.mesh_path)
+    r = w.r.copy()
+    r[-3:] = [1.02, 0.97, 1.01]     # alpha != 1
+    faces, uv = random_points(o.nfaces, o.nreg, 20000, 3, deep=600, vertex=4)
+    say("[points dev]", name, "points", len(faces), "irregular", int((faces >= o.nreg).sum()))
+    R_h, N_h = g.point_eval(r, faces, uv)
+    R_d, N_d = dev_eval(g, r, faces, uv)
+    g.point_status()
+    assert same(R_d, R_h) and same(N_d, N_h)
+    # repeatable
+    R_2, N_2 = dev_eval(g, r, faces, uv)
+    assert same(R_2, R_d) and same(N_2, N_d)
+
+
+def test_rejections_name_the_first_point():
+    w = workloads.vesicle_642()
+    g = make(w)
+    o = O.OracleSurface(w.mesh_path)
+    faces, uv = random_points(o.nfaces, o.nreg, 5000, 8)
+    R_h, N_h = g.point_eval(w.r, faces, uv)
+    irr = int(np.flatnonzero(faces >= o.nreg)[0])
+    bad_face = faces.copy()
+    bad_face[4000] = o.nfaces + 3              # face out of range
+    bad_uv = uv.copy()
+    bad_uv[irr] = [0.9, 0.9]                    # outside an irregular face
+    bad_uv[4500, 0] = np.nan                    # non-finite
+    R_d, N_d = dev_eval(g, w.r, bad_face, bad_uv)
+    with pytest.raises(S.UmcpError) as ex:
+        g.point_status()
+    first = min(4000, irr, 4500)
+    say("[points dev] rejection:", ex.value)
+    assert ("point %d:" % first) in str(ex.value)
+    bad = np.zeros(len(faces), bool)
+    bad[[4000, irr, 4500]] = True
+    assert np.isnan(R_d[bad]).all() and np.isnan(N_d[bad]).all()
+    assert same(R_d[~bad], R_h[~bad]) and same(N_d[~bad], N_h[~bad])
+    # the host call rejects the same batch, naming the same point
+    with pytest.raises(S.UmcpError) as ex2:
+        g.point_eval(w.r, bad_face, bad_uv)
+    assert ("point %d:" % first) in str(ex2.value)
+    # a clean batch clears the status
+    dev_eval(g, w.r, faces, uv)
+    g.point_status()
+
+
+def test_million_points_full_size():
+    w = workloads.vesicle_40962()
+    g = make(w)
+    faces, uv = random_points(g.nfaces, g.nreg, 1_000_000, 11, deep=60)
+    R_h, N_h = g.point_eval(w.r, faces, uv)
+    R_d, N_d = dev_eval(g, w.r, faces, uv)
+    g.point_status()
+    say("[points dev] 1M points equal:", same(R_d, R_h), same(N_d, N_h))
+    assert same(R_d, R_h) and same(N_d, N_h)

@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libumcp_core.so")
-SOURCES = ["umcp_kernels.hip", "umcp_faces64.hip", "umcp_faces32.hip", "umcp_faces_mx.hip", "umcp_noalloc.hip", "surface_host.cpp", "effm_host.cpp", "umcp_lipids.cpp", "umcp_api.cpp", "umcp_ensemble.cpp",
+SOURCES = ["umcp_kernels.hip", "umcp_faces64.hip", "umcp_faces32.hip", "umcp_faces_mx.hip", "umcp_noalloc.hip", "umcp_sort.hip", "surface_host.cpp", "effm_host.cpp", "umcp_lipids.cpp", "umcp_api.cpp", "umcp_ensemble.cpp",
            "umcp_server.cpp", "umcp_io.cpp", "umcp_points.cpp"]
 ARCH = os.environ.get("UMCP_OFFLOAD_ARCH", "gfx950")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")

@@ -1626,13 +1626,10 @@ __device__ void point_weights_out(const PointArgs &a, int p, const PointRec &pr,
     }
 }
 
-// one thread per point of one kind (idx: the batch's positions of that kind)
+// one point of kind KIND (j: its position in the batch)
 template <int KIND>
-__global__ __launch_bounds__(kPointBlock) void point_kernel(PointArgs a, const int *__restrict__ idx, int count)
+__device__ __forceinline__ void point_one(const PointArgs &a, int j)
 {
-    const int t = blockIdx.x * kPointBlock + threadIdx.x;
-    if (t >= count) return;
-    const int j = idx[t];
     const int p = (a.grad == 1 || a.grad == 4) ? a.order[j] : j;
     const PointRec pr = a.rec[p];
     const PointClass C = a.cls[pr.cls];
@@ -1873,6 +1870,177 @@ hipError_t launch_particle_density(const PointArgs &a, const DensityItem *items,
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(particle_density_kernel, dim3((count + kPointBlock - 1) / kPointBlock), dim3(kPointBlock), 0,
                        st, a, items, count);
+    return hipGetLastError();
+}
+
+// one thread per point of one kind (idx: the batch's positions of that kind)
+template <int KIND>
+__global__ __launch_bounds__(kPointBlock) void point_kernel(PointArgs a, const int *__restrict__ idx, int count)
+{
+    const int t = blockIdx.x * kPointBlock + threadIdx.x;
+    if (t >= count) return;
+    point_one<KIND>(a, idx[t]);
+}
+
+// the same with the kinds' counts on the device: kind KIND's indices start
+// after the earlier kinds' in idx
+template <int KIND>
+__global__ __launch_bounds__(kPointBlock) void point_kernel_dev(PointArgs a, const int *__restrict__ idx,
+                                                                const int *__restrict__ counts)
+{
+    const int t = blockIdx.x * kPointBlock + threadIdx.x;
+    if (t >= counts[KIND]) return;
+    const int start = KIND == 0 ? 0 : (KIND == 1 ? counts[0] : counts[0] + counts[1]);
+    point_one<KIND>(a, idx[start + t]);
+}
+
+hipError_t launch_points_dev(const PointArgs &a, const int *idx, const int *dcounts, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const dim3 grid((n + kPointBlock - 1) / kPointBlock);
+    hipLaunchKernelGGL(point_kernel_dev<kPointRegular>, grid, dim3(kPointBlock), 0, st, a, idx, dcounts);
+    hipLaunchKernelGGL(point_kernel_dev<kPointIrregular>, grid, dim3(kPointBlock), 0, st, a, idx, dcounts);
+    hipLaunchKernelGGL(point_kernel_dev<kPointVertex>, grid, dim3(kPointBlock), 0, st, a, idx, dcounts);
+    return hipGetLastError();
+}
+
+// irr_kernel::domain / get_map / get_map_transform (src/irr_kernel.C:
+// 380-541), as the host path restates them; the level uses the device's
+// log10, which can differ from libm's in the last bit, so a point whose
+// -log2(u + v) lies within rounding of an integer may take the adjacent
+// level (the sub-patches agree on their shared border)
+__device__ int resolve_irregular_dev(double fu, double fv, bool vertex_ok, PointRec &rec)
+{
+    double f_domain = -log10(fu + fv) / log10(2.0);
+    if (!(f_domain < 1e300) || !(f_domain > -1e300)) {
+        if (!vertex_ok) return 3;
+        if (!(fu < 1e-10 && fv < 1e-10)) return 4;
+        rec.kind = kPointVertex;
+        rec.map = 0;
+        rec.u = rec.v = 0.0;
+        rec.su = rec.sv = 1.0;
+        return 0;
+    }
+    if (fu + fv >= 1.0 - 1e-9) f_domain = 1;
+    const double dom = ceil(f_domain);
+    if (!(dom - 1.0 < (double)kPointDomains)) return 5;
+    const int domain = (int)dom;
+    const double pow2 = ldexp(1.0, domain - 1);
+    double su = pow2, sv = pow2;
+    fu *= pow2;
+    fv *= pow2;
+    int sub;
+    if (fu > 0.5) {
+        sub = 0;
+        fv = 2 * fv;
+        fu = 2.0 * fu - 1.0;
+        su *= 2.0;
+        sv *= 2.0;
+    } else if (fv > 0.5) {
+        sub = 1;
+        fu = 2 * fu;
+        fv = 2.0 * fv - 1.0;
+        su *= 2;
+        sv *= 2;
+    } else {
+        sub = 2;
+        fv = 1 - 2 * fv;
+        fu = 1 - 2 * fu;
+        su *= -2;
+        sv *= -2;
+    }
+    constexpr double kThresh = 1e-4;   // src/interpLib.C:24
+    if (fu + fv > 1.0 + kThresh || fu < -kThresh || fv < -kThresh) return 4;
+    rec.kind = kPointIrregular;
+    rec.map = (domain - 1) * 3 + sub;
+    rec.u = fu;
+    rec.v = fv;
+    rec.su = su;
+    rec.sv = sv;
+    return 0;
+}
+
+__global__ __launch_bounds__(kPointBlock) void point_resolve_kernel(PointResolveArgs a)
+{
+    #pragma clang fp contract(off)
+    const int i = blockIdx.x * kPointBlock + threadIdx.x;
+    if (i >= a.n) return;
+    int why = 0;
+    PointRec q{};
+    const int f = a.faces[i];
+    const double u = a.uv[2 * (size_t)i], v = a.uv[2 * (size_t)i + 1];
+    if (f < 0 || f >= a.nfaces) {
+        why = 1;
+    } else {
+        const int2 cf = a.face[f];
+        q.cls = cf.x;
+        q.fc = cf.y;
+        q.fu = u;
+        q.fv = v;
+        if (!isfinite(u) || !isfinite(v)) {
+            why = 2;
+        } else if (a.cls[cf.x].valence == 6) {
+            q.kind = kPointRegular;
+            q.map = 0;
+            q.u = u;
+            q.v = v;
+            q.su = q.sv = 1.0;
+        } else {
+            why = resolve_irregular_dev(u, v, a.vertex_ok != 0, q);
+        }
+    }
+    a.iota[i] = i;
+    if (why) {
+        a.kind[i] = 3;
+        atomicMin(a.status, ((unsigned long long)i << 8) | (unsigned long long)why);
+        for (int k = 0; k < 3; k++) {
+            a.R[3 * (size_t)i + k] = __builtin_nan("");
+            a.nrm[3 * (size_t)i + k] = __builtin_nan("");
+        }
+        return;
+    }
+    a.rec[i] = q;
+    a.kind[i] = (unsigned char)q.kind;
+}
+
+// the points per kind from the kind-sorted keys: the first position of each
+// kind, by binary search (one thread)
+__global__ void point_counts_kernel(const unsigned char *__restrict__ sorted, int n, int *counts)
+{
+    if (threadIdx.x != 0) return;
+    int first[4];
+    for (int k = 0; k < 4; k++) {
+        int lo = 0, hi = n;   // first index with key >= k
+        while (lo < hi) {
+            const int mid = lo + (hi - lo) / 2;
+            if (sorted[mid] < k)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        first[k] = lo;
+    }
+    for (int k = 0; k < 3; k++) counts[k] = first[k + 1] - first[k];
+}
+
+hipError_t launch_point_counts(const unsigned char *sorted, int n, int *counts, hipStream_t st)
+{
+    hipLaunchKernelGGL(point_counts_kernel, dim3(1), dim3(64), 0, st, sorted, n, counts);
+    return hipGetLastError();
+}
+
+__global__ void point_resolve_init_kernel(int *counts, unsigned long long *status)
+{
+    if (threadIdx.x < 3) counts[threadIdx.x] = 0;
+    if (threadIdx.x == 0) status[0] = ~0ull;
+}
+
+hipError_t launch_point_resolve(const PointResolveArgs &a, hipStream_t st)
+{
+    hipLaunchKernelGGL(point_resolve_init_kernel, dim3(1), dim3(64), 0, st, a.counts, a.status);
+    if (a.n > 0)
+        hipLaunchKernelGGL(point_resolve_kernel, dim3((a.n + kPointBlock - 1) / kPointBlock), dim3(kPointBlock), 0,
+                           st, a);
     return hipGetLastError();
 }
 

@@ -268,6 +268,39 @@ hipError_t launch_particle_density(const PointArgs &a, const DensityItem *items,
 // CSR position (gbuf zeroed first), and its alpha-gradient part into apart;
 // then per vertex the sum of its CSR range in order into gout [nv][3], and
 // the alpha gradient into gout[nv] by a two-stage tree over apart
+// Device-resident evaluation (umcp_point_eval_dev): each point resolved on
+// the device by the host path's rules (face -> class and index, the
+// irregular sub-patch level and local coordinates, the same rejections);
+// kind[i] is the point's kind (3: rejected, whose outputs become NaN), iota
+// the point indices for the stable partition by kind; status holds the first rejected point as (index << 8 | reason),
+// reasons: 1 face out of range, 2 non-finite (u, v), 3 the extraordinary
+// vertex without a map, 4 outside the face, 5 level beyond MAX_DOMAIN.
+struct PointResolveArgs {
+    int n, nfaces, vertex_ok;
+    const int *faces;
+    const double *uv;
+    const int2 *face;        // per global face: (class, index in the class)
+    const PointClass *cls;
+    PointRec *rec;
+    unsigned char *kind;
+    int *iota;
+    int *counts;             // [3], zeroed here (the partition's counts fill them)
+    unsigned long long *status;
+    double *R, *nrm;         // rejected points: NaN
+};
+// counts zeroed and status set to all ones first (a fill kernel)
+hipError_t launch_point_resolve(const PointResolveArgs &a, hipStream_t st);
+// counts[k] = the points of kind k, from the kind-sorted keys
+hipError_t launch_point_counts(const unsigned char *sorted, int n, int *counts, hipStream_t st);
+// the point kernels over the kind-partitioned indices with the per-kind
+// counts on the device (grids sized for n points, threads past a kind's
+// count return)
+hipError_t launch_points_dev(const PointArgs &a, const int *idx, const int *dcounts, int n, hipStream_t st);
+// stable partition by kind (radix sort of the 2-bit keys with the indices as
+// values); temp == nullptr sets *temp_bytes to the work space needed
+hipError_t point_partition_by_kind(const unsigned char *keys_in, unsigned char *keys_out, const int *vals_in,
+                                   int *vals_out, int n, void *temp, size_t *temp_bytes, hipStream_t st);
+
 struct PointReduceArgs {
     const double *r;
     int nv, n, nseg, nnz;

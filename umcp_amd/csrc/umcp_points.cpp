@@ -136,6 +136,15 @@ int point_setup(umcp_surface *s)
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) return phip(s, e, "upload point tables");
     s->d_pt_cls = (PointClass *)(d + tab_off);
+    // the face table of device-resident calls
+    std::vector<int2> face((size_t)P.nfaces);
+    for (int f = 0; f < P.nfaces; f++) face[(size_t)f] = make_int2(s->pt_face_cls[(size_t)f], s->pt_face_fc[(size_t)f]);
+    void *df = nullptr;
+    if ((rc = palloc(s, &df, sizeof(int2) * face.size()))) return rc;
+    e = hipMemcpyAsync(df, face.data(), sizeof(int2) * face.size(), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return phip(s, e, "upload point face table");
+    s->d_pt_face = (int2 *)df;
     s->pt_ready = true;
     return UMCP_OK;
 }
@@ -541,6 +550,99 @@ int umcp_point_eval(umcp_surface *s, const double *r, int n, const int *faces, c
     par_copy(nt, R, hst, bytes);
     par_copy(nt, nrm, hst + (b.off_N - b.off_R), bytes);
     return UMCP_OK;
+}
+
+int umcp_point_eval_dev(umcp_surface *s, const double *d_r, int n, const int *d_faces, const double *d_uv,
+                        double *d_R, double *d_nrm, void *stream)
+{
+    int rc = point_check(s, d_r, n, d_faces, d_uv);
+    if (rc) return rc;
+    if (n > 0 && (!d_R || !d_nrm)) return pfail(s, UMCP_ERR_ARG, "null output arrays");
+    hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+    // work space: records, keys and indices (twice: sort in and out), the
+    // per-kind counts, the sort's temporary storage; the status word apart
+    size_t temp = 0;
+    hipError_t e = point_partition_by_kind(nullptr, nullptr, nullptr, nullptr, std::max(n, 1), nullptr, &temp, st);
+    if (e != hipSuccess) return phip(s, e, "point partition size");
+    size_t o = 0;
+    const size_t o_rec = o;   o = align_up(o + sizeof(PointRec) * (size_t)n);
+    const size_t o_k0 = o;    o = align_up(o + (size_t)n);
+    const size_t o_k1 = o;    o = align_up(o + (size_t)n);
+    const size_t o_v0 = o;    o = align_up(o + sizeof(int) * (size_t)n);
+    const size_t o_v1 = o;    o = align_up(o + sizeof(int) * (size_t)n);
+    const size_t o_cnt = o;   o = align_up(o + sizeof(int) * 3);
+    const size_t o_tmp = o;   o = align_up(o + temp);
+    if (o > s->ptd_cap) {
+        if (s->d_ptd) {
+            (void)hipStreamSynchronize(st);
+            pfree(s, s->d_ptd, s->ptd_cap);
+            s->d_ptd = nullptr;
+            s->ptd_cap = 0;
+        }
+        void *p = nullptr;
+        if ((rc = palloc(s, &p, o))) return rc;
+        s->d_ptd = (char *)p;
+        s->ptd_cap = o;
+    }
+    if (!s->d_ptd_status) {
+        void *p = nullptr;
+        if ((rc = palloc(s, &p, sizeof(unsigned long long)))) return rc;
+        s->d_ptd_status = (unsigned long long *)p;
+    }
+    char *w = s->d_ptd;
+    PointResolveArgs ra{};
+    ra.n = n;
+    ra.nfaces = s->P.nfaces;
+    ra.vertex_ok = 1;
+    ra.faces = d_faces;
+    ra.uv = d_uv;
+    ra.face = s->d_pt_face;
+    ra.cls = s->d_pt_cls;
+    ra.rec = (PointRec *)(w + o_rec);
+    ra.kind = (unsigned char *)(w + o_k0);
+    ra.iota = (int *)(w + o_v0);
+    ra.counts = (int *)(w + o_cnt);
+    ra.status = s->d_ptd_status;
+    ra.R = d_R;
+    ra.nrm = d_nrm;
+    e = launch_point_resolve(ra, st);
+    if (e != hipSuccess) return phip(s, e, "point resolve launch");
+    if (n == 0) return UMCP_OK;
+    e = point_partition_by_kind(ra.kind, (unsigned char *)(w + o_k1), ra.iota, (int *)(w + o_v1), n, w + o_tmp, &temp,
+                                st);
+    if (e == hipSuccess) e = launch_point_counts((const unsigned char *)(w + o_k1), n, ra.counts, st);
+    if (e != hipSuccess) return phip(s, e, "point partition");
+    PointArgs a{};
+    a.r = d_r;
+    a.nv = s->T.nv;
+    a.n = n;
+    a.grad = 0;
+    a.kc = s->prm.kc;
+    std::memcpy(a.pbc, s->T.pbc, sizeof a.pbc);
+    a.rec = ra.rec;
+    a.cls = s->d_pt_cls;
+    a.R = d_R;
+    a.nrm = d_nrm;
+    e = launch_points_dev(a, (const int *)(w + o_v1), ra.counts, n, st);
+    if (e != hipSuccess) return phip(s, e, "point kernel launch");
+    return UMCP_OK;
+}
+
+int umcp_point_status(umcp_surface *s, int *bad_point)
+{
+    if (!s) return UMCP_ERR_ARG;
+    if (bad_point) *bad_point = -1;
+    if (!s->d_ptd_status) return UMCP_OK;
+    unsigned long long st = 0;
+    hipError_t e = hipMemcpy(&st, s->d_ptd_status, sizeof st, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return phip(s, e, "point status");
+    if (st == ~0ull) return UMCP_OK;
+    static const char *why[] = {"", "face index out of range", "non-finite (u, v)",
+                                "the extraordinary vertex of an irregular face has no sub-patch (the reference has no map there)",
+                                "(u, v) outside the face", "requested sub-patch level beyond MAX_DOMAIN (100)"};
+    const int i = (int)(st >> 8), c = (int)(st & 0xff);
+    if (bad_point) *bad_point = i;
+    return pfail(s, UMCP_ERR_ARG, "point " + std::to_string(i) + ": " + (c >= 1 && c <= 5 ? why[c] : "rejected"));
 }
 
 int umcp_point_grad(umcp_surface *s, const double *r, int n, const int *faces, const double *uv,

@@ -139,6 +139,12 @@ struct umcp_surface {
     bool pt_ready = false;
     std::vector<int> pt_face_cls, pt_face_fc;
     umcp::PointClass *d_pt_cls = nullptr;
+    int2 *d_pt_face = nullptr;            // per global face: (class, index in the class)
+    // device-resident evaluation's work space (grown on demand) and the
+    // first rejected point of the last such call
+    char *d_ptd = nullptr;
+    size_t ptd_cap = 0;
+    unsigned long long *d_ptd_status = nullptr;
     double *d_pt_gbuf = nullptr;          // [nnz][3] (face, slot) sums, then the alpha partials
     char *d_pt_in = nullptr, *d_pt_out = nullptr;
     size_t pt_in_cap = 0, pt_out_cap = 0;

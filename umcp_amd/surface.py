@@ -47,7 +47,7 @@ EXPORTS = [
     "umcp_clone", "umcp_device_bytes", "umcp_server_create", "umcp_server_submit", "umcp_server_wait",
     "umcp_server_drain", "umcp_server_stats", "umcp_server_last_error", "umcp_server_destroy",
     "umcp_effm_build", "umcp_effm_get", "umcp_effm_steps", "umcp_effm_steps_dev",
-    "umcp_effm_build_sparse", "umcp_effm_apply",
+    "umcp_effm_build_sparse", "umcp_effm_apply", "umcp_point_eval_dev", "umcp_point_status",
     "umcp_save_mesh", "umcp_write_restart", "umcp_read_restart", "umcp_write_limit_surface",
     "umcp_point_eval", "umcp_point_grad", "umcp_point_geometry", "umcp_write_limit_psf",
     "umcp_face_energies", "umcp_energy_vertex", "umcp_area_vertex", "umcp_next_face", "umcp_point_coeffs",
@@ -164,6 +164,8 @@ def load_library(path: str = LIB_PATH):
         "umcp_effm_get": ([vp, dp], C.c_int),
         "umcp_effm_build_sparse": ([vp, C.c_double, C.POINTER(C.c_long), dp, C.POINTER(C.c_int)], C.c_int),
         "umcp_effm_apply": ([vp, dp, dp], C.c_int),
+        "umcp_point_eval_dev": ([vp, vp, C.c_int, vp, vp, vp, vp, vp], C.c_int),
+        "umcp_point_status": ([vp, C.POINTER(C.c_int)], C.c_int),
         "umcp_effm_steps": ([vp, dp, dp, dp, C.c_int, C.POINTER(EffmParams), dp], C.c_int),
         "umcp_effm_steps_dev": ([vp, vp, vp, vp, C.c_int, C.POINTER(EffmParams), C.c_ulonglong, vp, vp],
                                 C.c_int),
@@ -403,6 +405,22 @@ class Surface:
         R, N = np.zeros((n, 3)), np.zeros((n, 3))
         self._check(self._L.umcp_point_eval(self._h, _dp(r), n, _ip(f), _dp(uv), _dp(R), _dp(N)), "point_eval")
         return R, N
+
+    def point_eval_dev(self, d_r, d_faces, d_uv, d_R, d_nrm, stream=None):
+        """umcp_point_eval on device tensors (d_faces int32 [n], d_uv [n, 2],
+        d_R and d_nrm [n, 3] float64); asynchronous: point_status after a
+        synchronisation reports a rejected point."""
+        n = d_faces.numel()
+        rc = self._L.umcp_point_eval_dev(self._h, C.c_void_p(d_r.data_ptr()), n, C.c_void_p(d_faces.data_ptr()),
+                                         C.c_void_p(d_uv.data_ptr()), C.c_void_p(d_R.data_ptr()),
+                                         C.c_void_p(d_nrm.data_ptr()), C.c_void_p(stream) if stream else None)
+        self._check(rc, "point_eval_dev")
+
+    def point_status(self):
+        """Raises UmcpError naming the first rejected point of the last
+        point_eval_dev (call after synchronising)."""
+        bad = C.c_int(-1)
+        self._check(self._L.umcp_point_status(self._h, C.byref(bad)), "point_status")
 
     def point_grad(self, r: np.ndarray, faces, uv, de_dr, de_dnrm, frac_mult=None, on_surface: bool = False,
                    kT: float = 0.0, g: np.ndarray | None = None, g_puv: np.ndarray | None = None):
