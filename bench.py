@@ -621,6 +621,25 @@ def run_points_leg(args, dev, torch):
     del d_r, d_f, d_uv, d_R, d_N
     out["grad_s"] = tg
     out["grad_points_per_s"] = n / tg
+    # the device-resident gradient (inputs and the accumulating outputs in HBM)
+    stream = torch.cuda.Stream(device=dev)
+    t = lambda x, dt=torch.float64: torch.tensor(x, device=dev, dtype=dt)  # noqa: E731
+    d_r, d_f, d_uv, d_a, d_b = t(r), t(faces, torch.int32), t(uv), t(a), t(b)
+    d_g = torch.zeros(len(r), device=dev, dtype=torch.float64)
+    d_gp = torch.zeros((n, 2), device=dev, dtype=torch.float64)
+    s.point_grad_dev(d_r, d_f, d_uv, d_a, d_b, d_g, d_gp, on_surface=True, kT=0.6, stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    s.point_status()
+    out["grad_dev_equals_host"] = bool(np.array_equal(d_g.cpu().numpy(), g))
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(10):
+        s.point_grad_dev(d_r, d_f, d_uv, d_a, d_b, d_g, d_gp, on_surface=True, kT=0.6, stream=stream.cuda_stream)
+    end.record(stream)
+    torch.cuda.synchronize()
+    out["grad_dev_s"] = start.elapsed_time(end) / 10 * 1e-3
+    out["grad_dev_points_per_s"] = n / out["grad_dev_s"]
+    del d_r, d_f, d_uv, d_a, d_b, d_g, d_gp
     # the geometry query (ru, rv, g, c)
     s.point_geometry(r, faces[:1000], uv[:1000])
     t0 = time.perf_counter()

@@ -338,6 +338,15 @@ int umcp_point_grad(umcp_surface *s, const double *r, int n, const int *faces, c
  * NULL (the surface's stream). */
 int umcp_point_eval_dev(umcp_surface *s, const double *d_r, int n, const int *d_faces, const double *d_uv,
                         double *d_R, double *d_nrm, void *stream);
+/* umcp_point_grad with every array on the device (d_de_dr, d_de_dnrm 3n,
+ * d_frac_mult n or NULL; d_g 3nv+3 and d_g_puv 2n accumulate) and no
+ * synchronisation: resolve as umcp_point_eval_dev, a stable sort by face,
+ * the faces' segments, then the same kernels and fixed-order sums, so the
+ * results equal umcp_point_grad's.  A rejected point adds nothing and is
+ * named by umcp_point_status. */
+int umcp_point_grad_dev(umcp_surface *s, const double *d_r, int n, const int *d_faces, const double *d_uv,
+                        const double *d_de_dr, const double *d_de_dnrm, const double *d_frac_mult, int on_surface,
+                        double kT, double *d_g, double *d_g_puv, void *stream);
 int umcp_point_status(umcp_surface *s, int *bad_point);
 /* Local geometry at the points, replacing surface::ru, surface::rv,
  * surface::g and surface::c (src/uv_map.C:1347-1700, 884-1106, 2361-2990):

@@ -1685,8 +1685,9 @@ __device__ __forceinline__ void point_one(const PointArgs &a, int j)
 __global__ __launch_bounds__(kPointBlock) void point_segment_kernel(PointReduceArgs a)
 {
     const int t = blockIdx.x * kPointBlock + threadIdx.x;
-    if (t >= a.nseg * kPointSlots) return;
-    const int x = t / a.nseg, sidx = t - x * a.nseg;
+    const int nseg = a.dnseg ? *a.dnseg : a.nseg;
+    if (t >= nseg * kPointSlots) return;
+    const int x = t / nseg, sidx = t - x * nseg;
     const PointSeg sg = a.seg[sidx];
     const PointClass C = a.cls[sg.cls];
     double *ap = a.apart + (size_t)t * 3;
@@ -1754,9 +1755,8 @@ constexpr int kPointAlphaBlock = 256;
 
 // block b sums its contiguous chunk of src ([len][3]) into dst[b]; the
 // partition and the tree depend on len only
-__global__ __launch_bounds__(kPointAlphaBlock) void point_alpha_kernel(int len, int chunk,
-                                                                       const double *__restrict__ src,
-                                                                       double *__restrict__ dst)
+__device__ __forceinline__ void point_alpha_block(int len, int chunk, const double *__restrict__ src,
+                                                  double *__restrict__ dst)
 {
     __shared__ double sh[3][kPointAlphaBlock];
     const int t = threadIdx.x;
@@ -1780,6 +1780,31 @@ __global__ __launch_bounds__(kPointAlphaBlock) void point_alpha_kernel(int len, 
         __syncthreads();
     }
     if (t < 3) dst[3 * blockIdx.x + t] = sh[t][0];
+}
+
+__global__ __launch_bounds__(kPointAlphaBlock) void point_alpha_kernel(int len, int chunk,
+                                                                       const double *__restrict__ src,
+                                                                       double *__restrict__ dst)
+{
+    point_alpha_block(len, chunk, src, dst);
+}
+
+// the same two passes with the segment count on the device (pass 0: block b
+// of the first partition, blocks past it return; pass 1: one block over the
+// first pass's sums)
+__global__ __launch_bounds__(kPointAlphaBlock) void point_alpha_dev_kernel(const int *__restrict__ dnseg, int pass,
+                                                                           const double *__restrict__ src,
+                                                                           double *__restrict__ dst)
+{
+    const int ns = *dnseg * kPointSlots;
+    const int chunk = (ns + kPointAlphaParts - 1) / kPointAlphaParts;
+    const int nb = chunk > 0 ? (ns + chunk - 1) / chunk : 0;
+    if (pass == 0) {
+        if ((int)blockIdx.x >= nb) return;
+        point_alpha_block(ns, chunk, src, dst);
+    } else {
+        point_alpha_block(nb, nb, src, dst);
+    }
 }
 
 // grad's particle-area term of one face carrying particles (src/grad.C:93-103,
@@ -1992,15 +2017,88 @@ __global__ __launch_bounds__(kPointBlock) void point_resolve_kernel(PointResolve
     a.iota[i] = i;
     if (why) {
         a.kind[i] = 3;
+        if (a.fkey) a.fkey[i] = a.nfaces;
         atomicMin(a.status, ((unsigned long long)i << 8) | (unsigned long long)why);
-        for (int k = 0; k < 3; k++) {
-            a.R[3 * (size_t)i + k] = __builtin_nan("");
-            a.nrm[3 * (size_t)i + k] = __builtin_nan("");
-        }
+        if (a.R)
+            for (int k = 0; k < 3; k++) {
+                a.R[3 * (size_t)i + k] = __builtin_nan("");
+                a.nrm[3 * (size_t)i + k] = __builtin_nan("");
+            }
         return;
     }
     a.rec[i] = q;
     a.kind[i] = (unsigned char)q.kind;
+    if (a.fkey) a.fkey[i] = f;
+}
+
+__global__ __launch_bounds__(kPointBlock) void point_seg_flags_kernel(const int *__restrict__ skey, int n, int nfaces,
+                                                                      int *__restrict__ flags)
+{
+    const int j = blockIdx.x * kPointBlock + threadIdx.x;
+    if (j >= n) return;
+    const int k = skey[j];
+    flags[j] = (k < nfaces && (j == 0 || skey[j - 1] != k)) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kPointBlock) void point_seg_fill_kernel(const int *__restrict__ skey,
+                                                                     const int *__restrict__ incl, int n, int nfaces,
+                                                                     const int2 *__restrict__ face,
+                                                                     const PointClass *__restrict__ cls,
+                                                                     PointSeg *__restrict__ seg,
+                                                                     unsigned char *__restrict__ pkind)
+{
+    const int j = blockIdx.x * kPointBlock + threadIdx.x;
+    if (j >= n) return;
+    const int k = skey[j];
+    if (k >= nfaces) {
+        pkind[j] = 3;
+        return;
+    }
+    const int2 cf = face[k];
+    const int sid = incl[j] - 1;
+    pkind[j] = cls[cf.x].valence == 6 ? (unsigned char)kPointRegular : (unsigned char)kPointIrregular;
+    if (j == 0 || skey[j - 1] != k) {
+        seg[sid].cls = cf.x;
+        seg[sid].fc = cf.y;
+        seg[sid].start = j;
+    }
+    if (j == n - 1 || skey[j + 1] != k) seg[sid].end = j + 1;
+}
+
+hipError_t launch_point_seg_flags(const int *skey, int n, int nfaces, int *flags, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(point_seg_flags_kernel, dim3((n + kPointBlock - 1) / kPointBlock), dim3(kPointBlock), 0, st,
+                       skey, n, nfaces, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_point_seg_fill(const int *skey, const int *incl, int n, int nfaces, const int2 *face,
+                                 const PointClass *cls, PointSeg *seg, unsigned char *pkind, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(point_seg_fill_kernel, dim3((n + kPointBlock - 1) / kPointBlock), dim3(kPointBlock), 0, st,
+                       skey, incl, n, nfaces, face, cls, seg, pkind);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kPointBlock) void point_accumulate_kernel(const double *__restrict__ src,
+                                                                       double *__restrict__ dst, int n, int stride,
+                                                                       const unsigned char *__restrict__ skip)
+{
+    const int i = blockIdx.x * kPointBlock + threadIdx.x;
+    if (i >= n) return;
+    if (skip && skip[i] == 3) return;
+    for (int k = 0; k < stride; k++) dst[(size_t)i * stride + k] += src[(size_t)i * stride + k];
+}
+
+hipError_t launch_point_accumulate(const double *src, double *dst, int n, int stride, const unsigned char *skip,
+                                   hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(point_accumulate_kernel, dim3((n + kPointBlock - 1) / kPointBlock), dim3(kPointBlock), 0, st,
+                       src, dst, n, stride, skip);
+    return hipGetLastError();
 }
 
 // the points per kind from the kind-sorted keys: the first position of each
@@ -2060,6 +2158,25 @@ hipError_t launch_points(const PointArgs &a, const int *idx, const int counts[3]
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+hipError_t launch_point_reduce_dev(const PointReduceArgs &a, int nseg_max, hipStream_t st)
+{
+    hipError_t e = hipMemsetAsync(a.gbuf, 0, sizeof(double) * 3 * (size_t)a.nnz, st);
+    if (e != hipSuccess) return e;
+    const int ns = nseg_max * kPointSlots;
+    if (ns > 0) {
+        hipLaunchKernelGGL(point_segment_kernel, dim3((ns + kPointBlock - 1) / kPointBlock), dim3(kPointBlock), 0, st,
+                           a);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(point_vertex_kernel, dim3((a.nv + kPointBlock - 1) / kPointBlock), dim3(kPointBlock), 0, st,
+                       a);
+    hipLaunchKernelGGL(point_alpha_dev_kernel, dim3(kPointAlphaParts), dim3(kPointAlphaBlock), 0, st, a.dnseg, 0,
+                       a.apart, a.partial);
+    hipLaunchKernelGGL(point_alpha_dev_kernel, dim3(1), dim3(kPointAlphaBlock), 0, st, a.dnseg, 1, a.partial,
+                       a.gout + 3 * (size_t)a.nv);
+    return hipGetLastError();
 }
 
 hipError_t launch_point_reduce(const PointReduceArgs &a, hipStream_t st)

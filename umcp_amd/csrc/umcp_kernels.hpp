@@ -286,7 +286,8 @@ struct PointResolveArgs {
     int *iota;
     int *counts;             // [3], zeroed here (the partition's counts fill them)
     unsigned long long *status;
-    double *R, *nrm;         // rejected points: NaN
+    double *R, *nrm;         // evaluation: rejected points get NaN (null for a gradient)
+    int *fkey;               // gradient: the face sort's keys (nfaces for a rejected point), or null
 };
 // counts zeroed and status set to all ones first (a fill kernel)
 hipError_t launch_point_resolve(const PointResolveArgs &a, hipStream_t st);
@@ -296,6 +297,22 @@ hipError_t launch_point_counts(const unsigned char *sorted, int n, int *counts, 
 // counts on the device (grids sized for n points, threads past a kind's
 // count return)
 hipError_t launch_points_dev(const PointArgs &a, const int *idx, const int *dcounts, int n, hipStream_t st);
+// A gradient batch on the device: after the stable sort by face (skey the
+// sorted face keys), the segments (one per face with points: incl the
+// inclusive scan of the head flags, so a position's segment is incl - 1 and
+// the count is incl[n - 1]) and each sorted position's kind (3: rejected)
+hipError_t launch_point_seg_flags(const int *skey, int n, int nfaces, int *flags, hipStream_t st);
+hipError_t launch_point_seg_fill(const int *skey, const int *incl, int n, int nfaces, const int2 *face,
+                                 const PointClass *cls, PointSeg *seg, unsigned char *pkind, hipStream_t st);
+// dst[i] += src[i] (i < n), skipping i with skip[i] == 3 when skip is given
+// (pairs: stride 2 entries per i)
+hipError_t launch_point_accumulate(const double *src, double *dst, int n, int stride, const unsigned char *skip,
+                                   hipStream_t st);
+// radix sort of int keys (end_bit bits) carrying int values, stable; the
+// inclusive sum of int flags
+hipError_t point_sort_by_face(const int *keys_in, int *keys_out, const int *vals_in, int *vals_out, int n, int end_bit,
+                              void *temp, size_t *temp_bytes, hipStream_t st);
+hipError_t point_inclusive_sum(const int *in, int *out, int n, void *temp, size_t *temp_bytes, hipStream_t st);
 // stable partition by kind (radix sort of the 2-bit keys with the indices as
 // values); temp == nullptr sets *temp_bytes to the work space needed
 hipError_t point_partition_by_kind(const unsigned char *keys_in, unsigned char *keys_out, const int *vals_in,
@@ -313,9 +330,13 @@ struct PointReduceArgs {
     double *apart;           // [kPointSlots][nseg][3]
     double *partial;         // [kPointAlphaParts][3]
     double *gout;            // [nv + 1][3]
+    const int *dnseg;        // device-resident batches: the segment count on the device
 };
 constexpr int kPointAlphaParts = 256;
 hipError_t launch_point_reduce(const PointReduceArgs &a, hipStream_t st);
+// the reduction with the segment count on the device (a.dnseg; grids for
+// nseg_max segments); the alpha partition and tree as launch_point_reduce's
+hipError_t launch_point_reduce_dev(const PointReduceArgs &a, int nseg_max, hipStream_t st);
 
 // energies of listed faces (surface::faceEnergy, src/interpLib.C:3405-3424):
 // item i = (class, index in the class) of the face whose energy lands in E[i],

@@ -628,6 +628,135 @@ int umcp_point_eval_dev(umcp_surface *s, const double *d_r, int n, const int *d_
     return UMCP_OK;
 }
 
+int umcp_point_grad_dev(umcp_surface *s, const double *d_r, int n, const int *d_faces, const double *d_uv,
+                        const double *d_de_dr, const double *d_de_dnrm, const double *d_frac_mult, int on_surface,
+                        double kT, double *d_g, double *d_g_puv, void *stream)
+{
+    int rc = point_check(s, d_r, n, d_faces, d_uv);
+    if (rc) return rc;
+    if (n > 0 && (!d_de_dr || !d_de_dnrm || !d_g || !d_g_puv)) return pfail(s, UMCP_ERR_ARG, "null gradient arrays");
+    hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+    const Plan &P = s->P;
+    const int nv = s->T.nv, nfaces = P.nfaces;
+    const int nseg_max = std::max(1, std::min(n, nfaces));
+    int fbits = 1;
+    while ((1 << fbits) <= nfaces) fbits++;   // keys up to nfaces (rejected points)
+    const int nn = std::max(n, 1);
+    size_t t_face = 0, t_kind = 0, t_scan = 0;
+    hipError_t e = point_sort_by_face(nullptr, nullptr, nullptr, nullptr, nn, fbits, nullptr, &t_face, st);
+    if (e == hipSuccess) e = point_partition_by_kind(nullptr, nullptr, nullptr, nullptr, nn, nullptr, &t_kind, st);
+    if (e == hipSuccess) e = point_inclusive_sum(nullptr, nullptr, nn, nullptr, &t_scan, st);
+    if (e != hipSuccess) return phip(s, e, "point sort sizes");
+    size_t temp = std::max(t_face, std::max(t_kind, t_scan));
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o = align_up(o + bytes);
+        return at;
+    };
+    const size_t o_rec = take(sizeof(PointRec) * (size_t)n), o_kind = take((size_t)n),
+                 o_iota = take(sizeof(int) * (size_t)n), o_order = take(sizeof(int) * (size_t)n),
+                 o_fkey = take(sizeof(int) * (size_t)n), o_skey = take(sizeof(int) * (size_t)n),
+                 o_flag = take(sizeof(int) * (size_t)n), o_incl = take(sizeof(int) * (size_t)n),
+                 o_pk0 = take((size_t)n), o_pk1 = take((size_t)n), o_idx = take(sizeof(int) * (size_t)n),
+                 o_cnt = take(sizeof(int) * 3), o_seg = take(sizeof(PointSeg) * (size_t)nseg_max),
+                 o_con = take(sizeof(double) * 3 * kPointSlots * (size_t)n), o_gp = take(sizeof(double) * 2 * (size_t)n),
+                 o_ap = take(sizeof(double) * 3 * kPointSlots * (size_t)nseg_max),
+                 o_gout = take(sizeof(double) * (3 * (size_t)nv + 3)), o_tmp = take(temp);
+    if (o > s->ptd_cap) {
+        if (s->d_ptd) {
+            (void)hipStreamSynchronize(st);
+            pfree(s, s->d_ptd, s->ptd_cap);
+            s->d_ptd = nullptr;
+            s->ptd_cap = 0;
+        }
+        void *p = nullptr;
+        if ((rc = palloc(s, &p, o))) return rc;
+        s->d_ptd = (char *)p;
+        s->ptd_cap = o;
+    }
+    if (!s->d_ptd_status) {
+        void *p = nullptr;
+        if ((rc = palloc(s, &p, sizeof(unsigned long long)))) return rc;
+        s->d_ptd_status = (unsigned long long *)p;
+    }
+    char *w = s->d_ptd;
+    PointResolveArgs ra{};
+    ra.n = n;
+    ra.nfaces = nfaces;
+    ra.vertex_ok = 0;   // the gradient has no rule at the extraordinary vertex
+    ra.faces = d_faces;
+    ra.uv = d_uv;
+    ra.face = s->d_pt_face;
+    ra.cls = s->d_pt_cls;
+    ra.rec = (PointRec *)(w + o_rec);
+    ra.kind = (unsigned char *)(w + o_kind);
+    ra.iota = (int *)(w + o_iota);
+    ra.counts = (int *)(w + o_cnt);
+    ra.status = s->d_ptd_status;
+    ra.fkey = (int *)(w + o_fkey);
+    e = launch_point_resolve(ra, st);
+    if (e != hipSuccess) return phip(s, e, "point resolve launch");
+    if (n == 0) return UMCP_OK;
+    // positions in face order (stable: a face's points keep the caller's
+    // order), the faces' segments, the positions grouped by kind
+    int *order = (int *)(w + o_order), *skey = (int *)(w + o_skey), *flags = (int *)(w + o_flag),
+        *incl = (int *)(w + o_incl), *idx = (int *)(w + o_idx);
+    unsigned char *pk0 = (unsigned char *)(w + o_pk0), *pk1 = (unsigned char *)(w + o_pk1);
+    PointSeg *seg = (PointSeg *)(w + o_seg);
+    size_t tb = temp;
+    e = point_sort_by_face(ra.fkey, skey, ra.iota, order, n, fbits, w + o_tmp, &tb, st);
+    if (e == hipSuccess) e = launch_point_seg_flags(skey, n, nfaces, flags, st);
+    tb = temp;
+    if (e == hipSuccess) e = point_inclusive_sum(flags, incl, n, w + o_tmp, &tb, st);
+    if (e == hipSuccess) e = launch_point_seg_fill(skey, incl, n, nfaces, s->d_pt_face, s->d_pt_cls, seg, pk0, st);
+    tb = temp;
+    if (e == hipSuccess) e = point_partition_by_kind(pk0, pk1, ra.iota, idx, n, w + o_tmp, &tb, st);
+    if (e == hipSuccess) e = launch_point_counts(pk1, n, ra.counts, st);
+    if (e != hipSuccess) return phip(s, e, "point face sort");
+    PointArgs a{};
+    a.r = d_r;
+    a.nv = nv;
+    a.n = n;
+    a.grad = 1;
+    a.on_surface = on_surface ? 1 : 0;
+    a.kT = kT;
+    a.kc = s->prm.kc;
+    std::memcpy(a.pbc, s->T.pbc, sizeof a.pbc);
+    a.rec = ra.rec;
+    a.cls = s->d_pt_cls;
+    a.order = order;
+    a.de_dr = d_de_dr;
+    a.de_dnrm = d_de_dnrm;
+    a.frac = d_frac_mult;
+    a.contrib = (double *)(w + o_con);
+    a.gpuv = (double *)(w + o_gp);
+    e = launch_points_dev(a, idx, ra.counts, n, st);
+    if (e != hipSuccess) return phip(s, e, "point kernel launch");
+    PointReduceArgs rd{};
+    rd.r = d_r;
+    rd.nv = nv;
+    rd.n = n;
+    rd.nseg = 0;
+    rd.nnz = P.nnz;
+    std::memcpy(rd.pbc, s->T.pbc, sizeof rd.pbc);
+    rd.seg = seg;
+    rd.cls = s->d_pt_cls;
+    rd.contrib = a.contrib;
+    rd.vstart = s->d_vstart;
+    rd.gbuf = s->d_pt_gbuf;
+    rd.apart = (double *)(w + o_ap);
+    rd.partial = s->d_pt_gbuf + 3 * (size_t)P.nnz;
+    rd.gout = (double *)(w + o_gout);
+    rd.dnseg = incl + (n - 1);
+    e = launch_point_reduce_dev(rd, nseg_max, st);
+    // the reference's contract: the caller's arrays accumulate (+=)
+    if (e == hipSuccess) e = launch_point_accumulate(rd.gout, d_g, 3 * nv + 3, 1, nullptr, st);
+    if (e == hipSuccess) e = launch_point_accumulate(a.gpuv, d_g_puv, n, 2, ra.kind, st);
+    if (e != hipSuccess) return phip(s, e, "point reduce launch");
+    return UMCP_OK;
+}
+
 int umcp_point_status(umcp_surface *s, int *bad_point)
 {
     if (!s) return UMCP_ERR_ARG;

@@ -47,7 +47,7 @@ EXPORTS = [
     "umcp_clone", "umcp_device_bytes", "umcp_server_create", "umcp_server_submit", "umcp_server_wait",
     "umcp_server_drain", "umcp_server_stats", "umcp_server_last_error", "umcp_server_destroy",
     "umcp_effm_build", "umcp_effm_get", "umcp_effm_steps", "umcp_effm_steps_dev",
-    "umcp_effm_build_sparse", "umcp_effm_apply", "umcp_point_eval_dev", "umcp_point_status",
+    "umcp_effm_build_sparse", "umcp_effm_apply", "umcp_point_eval_dev", "umcp_point_status", "umcp_point_grad_dev",
     "umcp_save_mesh", "umcp_write_restart", "umcp_read_restart", "umcp_write_limit_surface",
     "umcp_point_eval", "umcp_point_grad", "umcp_point_geometry", "umcp_write_limit_psf",
     "umcp_face_energies", "umcp_energy_vertex", "umcp_area_vertex", "umcp_next_face", "umcp_point_coeffs",
@@ -166,6 +166,7 @@ def load_library(path: str = LIB_PATH):
         "umcp_effm_apply": ([vp, dp, dp], C.c_int),
         "umcp_point_eval_dev": ([vp, vp, C.c_int, vp, vp, vp, vp, vp], C.c_int),
         "umcp_point_status": ([vp, C.POINTER(C.c_int)], C.c_int),
+        "umcp_point_grad_dev": ([vp, vp, C.c_int, vp, vp, vp, vp, vp, C.c_int, C.c_double, vp, vp, vp], C.c_int),
         "umcp_effm_steps": ([vp, dp, dp, dp, C.c_int, C.POINTER(EffmParams), dp], C.c_int),
         "umcp_effm_steps_dev": ([vp, vp, vp, vp, C.c_int, C.POINTER(EffmParams), C.c_ulonglong, vp, vp],
                                 C.c_int),
@@ -415,6 +416,17 @@ class Surface:
                                          C.c_void_p(d_uv.data_ptr()), C.c_void_p(d_R.data_ptr()),
                                          C.c_void_p(d_nrm.data_ptr()), C.c_void_p(stream) if stream else None)
         self._check(rc, "point_eval_dev")
+
+    def point_grad_dev(self, d_r, d_faces, d_uv, d_de_dr, d_de_dnrm, d_g, d_g_puv, d_frac=None,
+                       on_surface: bool = False, kT: float = 0.0, stream=None):
+        """umcp_point_grad on device tensors; d_g (3nv+3) and d_g_puv [n, 2]
+        accumulate; asynchronous (point_status after a synchronisation)."""
+        n = d_faces.numel()
+        ptr = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
+        rc = self._L.umcp_point_grad_dev(self._h, ptr(d_r), n, ptr(d_faces), ptr(d_uv), ptr(d_de_dr),
+                                         ptr(d_de_dnrm), ptr(d_frac), 1 if on_surface else 0, float(kT),
+                                         ptr(d_g), ptr(d_g_puv), C.c_void_p(stream) if stream else None)
+        self._check(rc, "point_grad_dev")
 
     def point_status(self):
         """Raises UmcpError naming the first rejected point of the last
