@@ -335,7 +335,10 @@ int umcp_point_grad(umcp_surface *s, const double *r, int n, const int *faces, c
  * the results equal umcp_point_eval's.  A rejected point gets NaN outputs;
  * umcp_point_status (after the caller's synchronisation) then returns
  * UMCP_ERR_ARG naming the first one (*bad_point, else -1).  stream may be
- * NULL (the surface's stream). */
+ * NULL (the surface's stream).  The device calls of one surface share its
+ * work space (grown on demand, the only allocation): order them on one
+ * stream; calls that overlap race.  The sorts inside use memset nodes, so
+ * these calls are not meant for graph capture. */
 int umcp_point_eval_dev(umcp_surface *s, const double *d_r, int n, const int *d_faces, const double *d_uv,
                         double *d_R, double *d_nrm, void *stream);
 /* umcp_point_grad with every array on the device (d_de_dr, d_de_dnrm 3n,
