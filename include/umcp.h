@@ -350,6 +350,11 @@ int umcp_point_eval_dev(umcp_surface *s, const double *d_r, int n, const int *d_
 int umcp_point_grad_dev(umcp_surface *s, const double *d_r, int n, const int *d_faces, const double *d_uv,
                         const double *d_de_dr, const double *d_de_dnrm, const double *d_frac_mult, int on_surface,
                         double kT, double *d_g, double *d_g_puv, void *stream);
+/* umcp_point_geometry with every array on the device (d_geom [n][15]), the
+ * same way: rejected points (the extraordinary vertex included, as in
+ * umcp_point_geometry) get NaN records */
+int umcp_point_geometry_dev(umcp_surface *s, const double *d_r, int n, const int *d_faces, const double *d_uv,
+                            double *d_geom, void *stream);
 int umcp_point_status(umcp_surface *s, int *bad_point);
 /* Local geometry at the points, replacing surface::ru, surface::rv,
  * surface::g and surface::c (src/uv_map.C:1347-1700, 884-1106, 2361-2990):

@@ -196,3 +196,35 @@ def test_gradient_million_points_full_size():
     g.point_status()
     say("[points dev] 1M gradient equal:", np.array_equal(g_d, g_h), np.array_equal(gp_d, gp_h))
     assert np.array_equal(g_d, g_h) and np.array_equal(gp_d, gp_h)
+
+
+@pytest.mark.parametrize("name", ["vesicle_642", "defect_sheet"])
+def test_geometry_device_equals_host(name):
+    w = {"vesicle_642": workloads.vesicle_642, "defect_sheet": lambda: workloads.defect_sheet()}[name]()
+    g = make(w)
+    o = O.OracleSurface(w.mesh_path)
+    r = w.r.copy()
+    r[-3:] = [1.02, 0.97, 1.01]
+    faces, uv = random_points(o.nfaces, o.nreg, 20000, 17, deep=600)
+    G_h = g.point_geometry(r, faces, uv)
+    d_G = torch.zeros((len(faces), 15), device="cuda", dtype=torch.float64)
+    g.point_geometry_dev(torch.tensor(r, device="cuda"), torch.tensor(faces, device="cuda", dtype=torch.int32),
+                         torch.tensor(uv, device="cuda"), d_G)
+    torch.cuda.synchronize()
+    g.point_status()
+    assert same(d_G.cpu().numpy(), G_h)
+    # the extraordinary vertex has no geometry: NaN record, named
+    irr = int(np.flatnonzero(faces >= o.nreg)[0])
+    uv2 = uv.copy()
+    uv2[irr] = 0.0
+    g.point_geometry_dev(torch.tensor(r, device="cuda"), torch.tensor(faces, device="cuda", dtype=torch.int32),
+                         torch.tensor(uv2, device="cuda"), d_G)
+    torch.cuda.synchronize()
+    with pytest.raises(S.UmcpError) as ex:
+        g.point_status()
+    assert ("point %d:" % irr) in str(ex.value)
+    G2 = d_G.cpu().numpy()
+    assert np.isnan(G2[irr]).all()
+    keep = np.ones(len(faces), bool)
+    keep[irr] = False
+    assert same(G2[keep], G_h[keep])

@@ -2024,6 +2024,8 @@ __global__ __launch_bounds__(kPointBlock) void point_resolve_kernel(PointResolve
                 a.R[3 * (size_t)i + k] = __builtin_nan("");
                 a.nrm[3 * (size_t)i + k] = __builtin_nan("");
             }
+        if (a.geom)
+            for (int k = 0; k < kPointGeom; k++) a.geom[(size_t)kPointGeom * i + k] = __builtin_nan("");
         return;
     }
     a.rec[i] = q;

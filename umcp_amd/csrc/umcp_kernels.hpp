@@ -286,7 +286,8 @@ struct PointResolveArgs {
     int *iota;
     int *counts;             // [3], zeroed here (the partition's counts fill them)
     unsigned long long *status;
-    double *R, *nrm;         // evaluation: rejected points get NaN (null for a gradient)
+    double *R, *nrm;         // evaluation: rejected points get NaN (null otherwise)
+    double *geom;            // geometry: rejected points get NaN records (null otherwise)
     int *fkey;               // gradient: the face sort's keys (nfaces for a rejected point), or null
 };
 // counts zeroed and status set to all ones first (a fill kernel)

@@ -552,12 +552,13 @@ int umcp_point_eval(umcp_surface *s, const double *r, int n, const int *faces, c
     return UMCP_OK;
 }
 
-int umcp_point_eval_dev(umcp_surface *s, const double *d_r, int n, const int *d_faces, const double *d_uv,
-                        double *d_R, double *d_nrm, void *stream)
+// the device-resident evaluation (mode 0: R, nrm) or geometry (mode 2: geom)
+static int point_dev_run(umcp_surface *s, const double *d_r, int n, const int *d_faces, const double *d_uv, int mode,
+                         double *d_R, double *d_nrm, double *d_geom, void *stream)
 {
     int rc = point_check(s, d_r, n, d_faces, d_uv);
     if (rc) return rc;
-    if (n > 0 && (!d_R || !d_nrm)) return pfail(s, UMCP_ERR_ARG, "null output arrays");
+    if (n > 0 && (mode == 0 ? (!d_R || !d_nrm) : !d_geom)) return pfail(s, UMCP_ERR_ARG, "null output arrays");
     hipStream_t st = stream ? (hipStream_t)stream : s->stream;
     // work space: records, keys and indices (twice: sort in and out), the
     // per-kind counts, the sort's temporary storage; the status word apart
@@ -593,7 +594,7 @@ int umcp_point_eval_dev(umcp_surface *s, const double *d_r, int n, const int *d_
     PointResolveArgs ra{};
     ra.n = n;
     ra.nfaces = s->P.nfaces;
-    ra.vertex_ok = 1;
+    ra.vertex_ok = mode == 0 ? 1 : 0;   // only the evaluation has a rule at the vertex
     ra.faces = d_faces;
     ra.uv = d_uv;
     ra.face = s->d_pt_face;
@@ -603,8 +604,9 @@ int umcp_point_eval_dev(umcp_surface *s, const double *d_r, int n, const int *d_
     ra.iota = (int *)(w + o_v0);
     ra.counts = (int *)(w + o_cnt);
     ra.status = s->d_ptd_status;
-    ra.R = d_R;
-    ra.nrm = d_nrm;
+    ra.R = mode == 0 ? d_R : nullptr;
+    ra.nrm = mode == 0 ? d_nrm : nullptr;
+    ra.geom = mode == 2 ? d_geom : nullptr;
     e = launch_point_resolve(ra, st);
     if (e != hipSuccess) return phip(s, e, "point resolve launch");
     if (n == 0) return UMCP_OK;
@@ -616,16 +618,29 @@ int umcp_point_eval_dev(umcp_surface *s, const double *d_r, int n, const int *d_
     a.r = d_r;
     a.nv = s->T.nv;
     a.n = n;
-    a.grad = 0;
+    a.grad = mode;
     a.kc = s->prm.kc;
     std::memcpy(a.pbc, s->T.pbc, sizeof a.pbc);
     a.rec = ra.rec;
     a.cls = s->d_pt_cls;
-    a.R = d_R;
-    a.nrm = d_nrm;
+    a.R = mode == 0 ? d_R : nullptr;
+    a.nrm = mode == 0 ? d_nrm : nullptr;
+    a.geom = mode == 2 ? d_geom : nullptr;
     e = launch_points_dev(a, (const int *)(w + o_v1), ra.counts, n, st);
     if (e != hipSuccess) return phip(s, e, "point kernel launch");
     return UMCP_OK;
+}
+
+int umcp_point_eval_dev(umcp_surface *s, const double *d_r, int n, const int *d_faces, const double *d_uv,
+                        double *d_R, double *d_nrm, void *stream)
+{
+    return point_dev_run(s, d_r, n, d_faces, d_uv, 0, d_R, d_nrm, nullptr, stream);
+}
+
+int umcp_point_geometry_dev(umcp_surface *s, const double *d_r, int n, const int *d_faces, const double *d_uv,
+                            double *d_geom, void *stream)
+{
+    return point_dev_run(s, d_r, n, d_faces, d_uv, 2, nullptr, nullptr, d_geom, stream);
 }
 
 int umcp_point_grad_dev(umcp_surface *s, const double *d_r, int n, const int *d_faces, const double *d_uv,

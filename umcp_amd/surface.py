@@ -47,7 +47,7 @@ EXPORTS = [
     "umcp_clone", "umcp_device_bytes", "umcp_server_create", "umcp_server_submit", "umcp_server_wait",
     "umcp_server_drain", "umcp_server_stats", "umcp_server_last_error", "umcp_server_destroy",
     "umcp_effm_build", "umcp_effm_get", "umcp_effm_steps", "umcp_effm_steps_dev",
-    "umcp_effm_build_sparse", "umcp_effm_apply", "umcp_point_eval_dev", "umcp_point_status", "umcp_point_grad_dev",
+    "umcp_effm_build_sparse", "umcp_effm_apply", "umcp_point_eval_dev", "umcp_point_status", "umcp_point_grad_dev", "umcp_point_geometry_dev",
     "umcp_save_mesh", "umcp_write_restart", "umcp_read_restart", "umcp_write_limit_surface",
     "umcp_point_eval", "umcp_point_grad", "umcp_point_geometry", "umcp_write_limit_psf",
     "umcp_face_energies", "umcp_energy_vertex", "umcp_area_vertex", "umcp_next_face", "umcp_point_coeffs",
@@ -166,6 +166,7 @@ def load_library(path: str = LIB_PATH):
         "umcp_effm_apply": ([vp, dp, dp], C.c_int),
         "umcp_point_eval_dev": ([vp, vp, C.c_int, vp, vp, vp, vp, vp], C.c_int),
         "umcp_point_status": ([vp, C.POINTER(C.c_int)], C.c_int),
+        "umcp_point_geometry_dev": ([vp, vp, C.c_int, vp, vp, vp, vp], C.c_int),
         "umcp_point_grad_dev": ([vp, vp, C.c_int, vp, vp, vp, vp, vp, C.c_int, C.c_double, vp, vp, vp], C.c_int),
         "umcp_effm_steps": ([vp, dp, dp, dp, C.c_int, C.POINTER(EffmParams), dp], C.c_int),
         "umcp_effm_steps_dev": ([vp, vp, vp, vp, C.c_int, C.POINTER(EffmParams), C.c_ulonglong, vp, vp],
@@ -427,6 +428,14 @@ class Surface:
                                          ptr(d_de_dnrm), ptr(d_frac), 1 if on_surface else 0, float(kT),
                                          ptr(d_g), ptr(d_g_puv), C.c_void_p(stream) if stream else None)
         self._check(rc, "point_grad_dev")
+
+    def point_geometry_dev(self, d_r, d_faces, d_uv, d_geom, stream=None):
+        """umcp_point_geometry on device tensors (d_geom [n, 15] float64)."""
+        n = d_faces.numel()
+        rc = self._L.umcp_point_geometry_dev(self._h, C.c_void_p(d_r.data_ptr()), n, C.c_void_p(d_faces.data_ptr()),
+                                             C.c_void_p(d_uv.data_ptr()), C.c_void_p(d_geom.data_ptr()),
+                                             C.c_void_p(stream) if stream else None)
+        self._check(rc, "point_geometry_dev")
 
     def point_status(self):
         """Raises UmcpError naming the first rejected point of the last
