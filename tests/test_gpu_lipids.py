@@ -103,3 +103,28 @@ def test_composition_file_equals_arrays(tmp_path):
     Eb, gb = b.energy_grad(w.r)
     assert Ea == Eb and np.array_equal(ga, gb)
     check(a, o, w.r, 1e-12, 1e-12, "file composition")
+
+
+def test_bad_compositions_are_refused(tmp_path):
+    w = workloads.vesicle_642()
+    g = S.Surface(w.mesh_path)
+    g.set_params(**w.params)
+    g.generatePlan()
+    path = tmp_path / "bad.inp"
+    path.write_text("lipid library BAD 0.0 -0.02\nlipid inner BAD 1\nlipid outer POPC 1\n")
+    with pytest.raises(S.UmcpError):
+        g.readLipidComposition(str(path), SEED)
+    path.write_text("lipid inner NOSUCH 1\n")
+    with pytest.raises(S.UmcpError):
+        g.readLipidComposition(str(path), SEED)
+    with pytest.raises(S.UmcpError):
+        g.set_lipids([0.0], [-1.0], [1.0], [1.0], SEED)
+    # an area per lipid so small that a face would hold more than 1e7 lipids
+    g.set_lipids([0.0, 0.01], [68.3, 1e-9], [1.0, 1.0], [1.0, 1.0], SEED)
+    with pytest.raises(S.UmcpError):
+        g.setg0(w.r0)
+    # a sane composition afterwards works
+    g.set_lipids(MIX["c0"], MIX["apl"], MIX["inner"], MIX["outer"], SEED)
+    g.setg0(w.r0)
+    E, _ = g.energy_grad(w.r)
+    assert np.isfinite(E)

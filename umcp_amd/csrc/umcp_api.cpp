@@ -677,6 +677,17 @@ int umcp_setg0(umcp_surface *s, const double *r)
         for (int p = 0; p < C.npt; p++)
             for (int f = 0; f < nf; f++) A0 += C.dudv * g0[(size_t)p * nf + f] * C.weight[p];
     }
+    // the packing adds one lipid per round until the face's area is covered:
+    // refuse compositions that would put more than 1e7 lipids on a face
+    {
+        double amax = 0.0, amin_apl = comp ? s->lipids.apl[0] : s->prm.lipid_apl;
+        for (int f = 0; f < nfaces; f++) amax = std::max(amax, Aface[f]);
+        if (comp)
+            for (double a : s->lipids.apl) amin_apl = std::min(amin_apl, a);
+        if (!(amin_apl > 0.0) || !(amax / amin_apl <= 1e7))
+            return fail(s, UMCP_ERR_ARG, "the lipid packing would place more than 1e7 lipids on a face "
+                                         "(area per lipid too small for the faces)");
+    }
     s->face_lip_o.assign((size_t)nfaces * nt, 0.0);
     s->face_lip_i.assign((size_t)nfaces * nt, 0.0);
     if (comp) {

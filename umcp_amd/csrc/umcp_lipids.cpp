@@ -5,6 +5,7 @@
 // rand()-driven packing of regular and irregular faces).
 #include "umcp_lipids.hpp"
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <strings.h>
@@ -86,6 +87,10 @@ int read_lipid_composition(const char *path, unsigned int seed, LipidComposition
         const int nr = sscanf(l.c_str(), "%255s %255s %255s %lf %lf %255s", junk, cmd, name, &apl, &c0, atom);
         if (nr < 5) {
             err = "expected 'lipid library lipidName APL c0': " + l;
+            return -1;
+        }
+        if (!(apl > 0.0) || !std::isfinite(apl) || !std::isfinite(c0)) {
+            err = "a library lipid needs a finite area per lipid above 0 and a finite c0: " + l;
             return -1;
         }
         const int at = find_name(lib, name);
