@@ -575,7 +575,8 @@ static int point_dev_run(umcp_surface *s, const double *d_r, int n, const int *d
     const size_t o_tmp = o;   o = align_up(o + temp);
     if (o > s->ptd_cap) {
         if (s->d_ptd) {
-            (void)hipStreamSynchronize(st);
+            // earlier calls may still use it, on any stream
+            (void)hipDeviceSynchronize();
             pfree(s, s->d_ptd, s->ptd_cap);
             s->d_ptd = nullptr;
             s->ptd_cap = 0;
@@ -680,7 +681,8 @@ int umcp_point_grad_dev(umcp_surface *s, const double *d_r, int n, const int *d_
                  o_gout = take(sizeof(double) * (3 * (size_t)nv + 3)), o_tmp = take(temp);
     if (o > s->ptd_cap) {
         if (s->d_ptd) {
-            (void)hipStreamSynchronize(st);
+            // earlier calls may still use it, on any stream
+            (void)hipDeviceSynchronize();
             pfree(s, s->d_ptd, s->ptd_cap);
             s->d_ptd = nullptr;
             s->ptd_cap = 0;
