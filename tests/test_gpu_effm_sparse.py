@@ -137,3 +137,24 @@ def test_beyond_dense_limit_163842():
     torch.cuda.synchronize()
     moved = (d_r[:-3].cpu().numpy() - w.r[:-3])
     assert np.isfinite(moved).all() and 0 < np.abs(moved).max() < 10.0
+
+
+def test_sparse_device_noise_equals_host_array():
+    sys.path.insert(0, __file__.rsplit("/", 1)[0])
+    import noise_ref
+    w = workloads.vesicle_642()
+    g, _ = make(w, oracle=False)
+    g.effm_build_sparse()
+    kT = units.kT(298.0)
+    steps, seed = 6, 99
+    g.set_step(0)
+    d_r = torch.tensor(w.r, device="cuda", dtype=torch.float64)
+    d_pp = torch.zeros(3 * g.nv, device="cuda", dtype=torch.float64)
+    g.effm_steps_dev(d_r, d_pp, steps, S.EFFM_LD, DT, kT, GAMMA, seed=seed)
+    torch.cuda.synchronize()
+    xi = np.stack([noise_ref.normals(seed, k, g.nv) for k in range(steps)])
+    r_h, pp_h = g.effm_steps(w.r, np.zeros(3 * g.nv), xi, S.EFFM_LD, DT, kT, GAMMA)
+    dr = np.abs(d_r.cpu().numpy() - r_h).max()
+    say("[effm sparse] device noise vs host array max |dr|", dr)
+    assert dr <= 1e-9 * np.abs(r_h - w.r).max()
+    assert np.array_equal(d_pp.cpu().numpy(), pp_h) or np.abs(d_pp.cpu().numpy() - pp_h).max() <= 1e-9 * np.abs(pp_h).max()
