@@ -59,6 +59,9 @@ def make_member(sp, precision=S.PREC_FP64, launch=None):
     g.generatePlan()
     if launch:
         g.set_launch(*launch)
+    if sp.get("lipids"):
+        L = sp["lipids"]
+        g.set_lipids(L["c0"], L["apl"], L["inner"], L["outer"], L["seed"])
     g.setg0(w.r0)
     if sp["restraints"]:
         g.set_restraints(**sp["restraints"])
@@ -213,3 +216,18 @@ def test_mfma_ensemble_equals_alone(mix, split):
     got, _ = run_ensemble(sps, STEPS, graph_steps=4, face_grid=64, precision=S.PREC_MIXED, chunks=2,
                           launch=launch)
     check_same("mfma", got, alone)
+
+
+def test_ensemble_with_lipid_composition_equals_alone():
+    """A member with a three-type lipid composition (lipid form, per-face
+    per-type leaflet areas) batched beside a Helfrich member: both bitwise
+    equal to their runs alone."""
+    w = workloads.vesicle_642()
+    comp = dict(c0=[-1.0 / 87.3, -1.0 / 29.4, -1.0 / 27.2], apl=[68.3, 63.4, 31.0], inner=[3.0, 2.0, 1.0],
+                outer=[5.0, 0.0, 2.0], seed=11)
+    sps = [dict(w=w, params=dict(w.params), restraints=None, seed=2001, lipids=comp)]
+    wh = workloads.vesicle(3, 100.0, 0.3, seed=21, kc_kT=25.0, name="ens_ves1")
+    sps.append(dict(w=wh, params={**wh.params, "mode": 1}, restraints=None, seed=2002))
+    alone = [run_alone(sp, STEPS) for sp in sps]
+    got, _ = run_ensemble(sps, STEPS)
+    check_same("lipids", got, alone)
