@@ -16,8 +16,11 @@
  *     subdivision matrix power A^(domain-1) taken by repeated multiplication
  *     (the reference uses LAPACK dgeev/dgetri eigen-powers, equal in exact
  *     arithmetic; LAPACK is absent here);
- *   - setg0 with the single-lipid packing rule (src/interpLib.C:2969-3388,
- *     src/lipid_composition.C:18-34, 285-302);
+ *   - setg0 with the lipid packing of src/interpLib.C:2969-3388 for one
+ *     type (src/lipid_composition.C:18-34, 285-302) or a composition of
+ *     several (src/lipid_composition.C:47-333, drawn with a restatement of
+ *     glibc's TYPE_3 rand(), which the reference calls), and the per-type
+ *     energy (src/interpLib.C:3576-3610, 3805-3840);
  *   - fenergy / ifenergy summed in triangle order (src/interpLib.C:3426-3881,
  *     4180-4352; src/parallel.C:319-326);
  *   - grad / igrad through the reference's own computational graph
@@ -25,7 +28,20 @@
  *     src/grad.C:65-942, 6445-7457);
  *   - the per-vertex overdamped Euler-Maruyama Brownian step
  *     r <- r - (D dt / kT) grad E + sqrt(2 D dt) xi (docs/timescales.tex:59-67,
- *     src/bd.C:1035-1036).
+ *     src/bd.C:1035-1036);
+ *   - the global area and volume restraints (src/interpLib.C:4322-4326,
+ *     src/minimize.C:90-114, src/volume.C:81-430), the hd.C effective-mass
+ *     matrix, inverse, truncation and Brownian/Langevin steps
+ *     (src/hd.C:418-481, 1543-1720; src/modes.C:1719-1960), evaluation at
+ *     arbitrary (face, u, v) points and the particle terms (src/grad.C:
+ *     3487-5690, src/interpLib.C:8330-).
+ *
+ * Each part is pinned by the tests against the reference's fixtures and
+ * against closed forms, invariants and finite differences.  Two results are
+ * pinned only to the mathematics, not bit for bit, because the libraries the
+ * reference calls are absent: the irregular tables' matrix power (LAPACK's
+ * eigen-route equals repeated multiplication in exact arithmetic) and QAGS
+ * (GSL's output itself; the restatement is checked against exact moments).
  */
 #include "umcp_oracle.h"
 
