@@ -164,3 +164,29 @@ def test_mixed_bd_positions_stay_fp64():
     # the gradient term moves r by mob*|g| ~ 1e-2 A per step; FP32 error in it
     # is ~1e-7 relative, so the trajectories agree far below the noise step
     assert err < 1e-6 * step
+
+
+def test_mixed_scattered_geometry(tmp_path):
+    """The mixed kernels' direct-gather path (a group beyond the 192 staged
+    vertices), reached with an icosphere's topology and its positions dealt
+    out at random, in every launch shape.  The faces fold through one
+    another, so the smooth-membrane budget does not apply (measured 5.7e-5
+    energy, 1.5e-4 gradient RMS in the VALU shapes); the bars here, 1e-3
+    and 1e-2, are far from the errors of order one a wrong index gives."""
+    from umcp_amd import meshgen
+    mesh = meshgen.icosphere(3, 120.0)
+    mesh.r = mesh.r[np.random.default_rng(17).permutation(mesh.nv)]
+    path = str(tmp_path / "scattered_geometry_mixed.mesh")
+    meshgen.write_mesh(mesh, path)
+    r0 = meshgen.flat_coords(mesh)
+    r = r0.copy()
+    r[:-3] += np.random.default_rng(23).normal(0.0, 0.3, 3 * mesh.nv)
+    w = workloads.Workload(path, path, mesh.nv, r0, r, dict(kc=6.0, kg=0.0, KA=0.5, k_reg=1.5, mode=1), T=298.0)
+    g, o = pair(w, c0=workloads.face_c0_uniform(1280, 8, -0.02, 0.02))
+    g.set_restraints(KA_global=0.1)
+    o.set_restraints(KA_global=0.1)
+    for split, flags in SHAPES:
+        g.set_launch(split, flags)
+        rel_e, rms = errors(g, o, w.r, "scattered geometry split %d flags %d" % (split, flags))
+        assert rel_e <= 1e-3
+        assert rms <= 1e-2
