@@ -431,3 +431,30 @@ def test_scattered_geometry(tmp_path):
     for split, flags in SHAPES:
         g.set_launch(split, flags)
         check_eval(g, o, w.r, "scattered geometry split %d flags %d" % (split, flags), g_tol=1e-11)
+
+
+@pytest.mark.parametrize("case", ["vesicle", "sheet"])
+def test_box_scale_not_one(case):
+    """The box scale alpha (the last three entries of r) away from 1: the
+    face kernel scales every contracted derivative by it, divides it out of
+    the scattered gradient and forms dE/dalpha from the unscaled positions.
+    The vesicle with per-face c0 and both restraints (whose volume seed is
+    scaled the same way), the periodic defect sheet, every launch shape,
+    against the oracle at the usual bars."""
+    if case == "vesicle":
+        w = workloads.vesicle_642()
+        g, o = make_pair(w, params=dict(kc=6.0, kg=-2.0, KA=0.5, k_reg=1.5, mode=1))
+        c0 = workloads.face_c0_uniform(g.nfaces, 12, -0.02, 0.02)
+        g.set_face_c0(c0)
+        o.set_face_c0(c0)
+        rst = {"KA_global": 0.1, "KV": 0.004}
+        g.set_restraints(**rst)
+        o.set_restraints(**rst)
+    else:
+        w = workloads.defect_sheet()
+        g, o = make_pair(w)
+    r = w.r.copy()
+    r[-3:] = [1.02, 0.97, 1.01]
+    for split, flags in SHAPES:
+        g.set_launch(split, flags)
+        check_eval(g, o, r, "alpha %s %s split %d flags %d" % (list(r[-3:]), case, split, flags))
